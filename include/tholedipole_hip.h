@@ -1,0 +1,154 @@
+/*
+ * tholedipole_hip.h -- C ABI of libtholedipole_hip.so, the MI355X (gfx950) implementation of
+ * the TholeDipoleForce force + induction path.
+ *
+ * The ABI replaces, below openmmapi/include/TholeDipoleKernels.h:46-122
+ * (class CalcTholeDipoleForceKernel), the reference platform's kernel
+ * (platforms/reference/src/ReferenceTholeDipoleKernels.{h,cpp}).  A host-side kernel class
+ * (C++, Python, ...) implements the ten CalcTholeDipoleForceKernel virtuals by calling these
+ * functions; each entry point below names the virtual it serves.
+ *
+ * Conventions
+ *   - plain C types only; no HIP, torch or OpenMM types cross the boundary;
+ *   - every function returns an int status (TDP_OK == 0) and never throws; the message of the
+ *     last failure is available from tdp_last_error() (or tdp_last_create_error() when
+ *     tdp_create itself failed);
+ *   - positions, forces and dipoles are arrays of 3*n doubles (x,y,z per particle, the memory
+ *     layout of std::vector<OpenMM::Vec3>); units nm, e, e*nm, energies e^2/nm (the reference
+ *     uses _electric = 1, ReferenceTholeDipoleForce.cpp:20);
+ *   - forces are ACCUMULATED into the caller's array (+=), as the reference does
+ *     (ReferenceTholeDipoleForce.cpp:359-362);
+ *   - a context is not re-entrant: one thread at a time, as an OpenMM kernel.
+ */
+#ifndef THOLEDIPOLE_HIP_H
+#define THOLEDIPOLE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TDP_OK 0
+#define TDP_ERR_INVALID_ARGUMENT 1   /* bad parameters (axis atoms, covalent lists, sizes) */
+#define TDP_ERR_HIP 2                /* a HIP runtime call failed */
+#define TDP_ERR_NOT_PME 3            /* getPMEParameters on a NoCutoff context */
+#define TDP_ERR_BOX_TOO_SMALL 4      /* box < 2*cutoff (ReferenceTholeDipoleKernels.cpp:170-173) */
+#define TDP_ERR_PARTICLE_COUNT 5     /* copyParameters with a different particle count */
+#define TDP_ERR_NO_DEVICE 6          /* no usable gfx950 device */
+
+/* nonbonded_method (TholeDipoleForce.h:58-70) */
+#define TDP_NO_CUTOFF 0
+#define TDP_PME 1
+/* polarization_type (TholeDipoleForce.h:72-92) */
+#define TDP_DIRECT 0
+#define TDP_MUTUAL 1
+#define TDP_EXTRAPOLATED 2
+/* precision */
+#define TDP_PRECISION_DOUBLE 0       /* every pair term in FP64 */
+#define TDP_PRECISION_MIXED 1        /* T*mu matvecs in FP32 with FP64 accumulation; FP64 elsewhere */
+/* solver for Mutual polarization */
+#define TDP_SOLVER_JACOBI 0          /* the reference's fixed-point iteration and stopping rule */
+#define TDP_SOLVER_PCG 1             /* Jacobi-preconditioned conjugate gradients, same stopping rule */
+/* dipole kinds for tdp_get_dipoles */
+#define TDP_DIPOLES_INDUCED 0
+#define TDP_DIPOLES_LAB_PERMANENT 1
+#define TDP_DIPOLES_TOTAL 2
+
+typedef struct tdp_context tdp_context;
+
+/* Everything CalcTholeDipoleForceKernel::initialize reads from TholeDipoleForce and System
+ * (ReferenceTholeDipoleKernels.cpp:83-158).  Arrays are borrowed for the duration of the call
+ * and deep-copied. */
+typedef struct tdp_force_desc {
+    int32_t num_particles;
+    const double* charges;            /* [n] */
+    const double* dipoles;            /* [3n] molecular-frame dipoles */
+    const double* polarizabilities;   /* [n] nm^3 */
+    const double* thole;              /* [n] */
+    const int32_t* axis_types;        /* [n] MultipoleAxisTypes */
+    const int32_t* atom_z;            /* [n] */
+    const int32_t* atom_x;            /* [n] */
+    const int32_t* atom_y;            /* [n] */
+    const int32_t* covalent_offsets;  /* [4n+1]: list t of particle i is covalent_atoms[off[4i+t] .. off[4i+t+1]) */
+    const int32_t* covalent_atoms;
+    int32_t nonbonded_method;
+    int32_t polarization_type;
+    double cutoff;
+    double ewald_error_tolerance;
+    double pme_alpha;                 /* setPMEParameters; 0 selects from the tolerance */
+    int32_t pme_grid[3];
+    int32_t mutual_max_iterations;
+    double mutual_target_epsilon;     /* RMS dipole update, e*nm */
+    const double* extrapolation_coefficients;
+    int32_t num_extrapolation_coefficients;
+    double default_box[9];            /* System default periodic box vectors (row-major a,b,c) */
+} tdp_force_desc;
+
+typedef struct tdp_options {
+    int32_t precision;                /* TDP_PRECISION_* */
+    int32_t solver;                   /* TDP_SOLVER_* */
+    int32_t device;                   /* HIP device ordinal */
+} tdp_options;
+
+typedef struct tdp_solver_stats {
+    int32_t iterations;               /* matvec iterations of the last solve */
+    double epsilon;                   /* final RMS dipole update, e*nm */
+    int32_t converged;
+    double matvec_ms;                 /* GPU time of the T*mu kernels since timing was enabled (HIP events) */
+    int32_t matvec_launches;          /* T*mu launches timed since timing was enabled */
+} tdp_solver_stats;
+
+/* Replaces CalcTholeDipoleForceKernel::initialize (TholeDipoleKernels.h:59,
+ * ReferenceTholeDipoleKernels.cpp:83-158): validates, uploads parameters and builds the
+ * exclusion tiles.  Errors as TholeDipoleForceImpl::initialize (TholeDipoleForceImpl.cpp:52-108). */
+int tdp_create(const tdp_force_desc* desc, const tdp_options* opts, tdp_context** out);
+void tdp_destroy(tdp_context* ctx);
+const char* tdp_last_error(const tdp_context* ctx);
+const char* tdp_last_create_error(void);
+
+/* Replaces copyParametersToContext (TholeDipoleKernels.h:75, ReferenceTholeDipoleKernels.cpp:317-339):
+ * per-particle parameters only; covalent maps and solver settings are kept. */
+int tdp_copy_parameters(tdp_context* ctx, const tdp_force_desc* desc);
+
+/* Replaces execute (TholeDipoleKernels.h:68, ReferenceTholeDipoleKernels.cpp:201-215).
+ * positions: host [3n]; box: host [9] (ignored for NoCutoff); forces: host [3n], accumulated
+ * (may be NULL when include_forces == 0); energy: host scalar (may be NULL). */
+int tdp_execute(tdp_context* ctx, const double* positions, const double* box, int32_t include_forces,
+                int32_t include_energy, double* forces, double* energy);
+
+/* Same evaluation with device-resident buffers on the caller's HIP stream (a hipStream_t
+ * passed as void*, NULL = the null stream): d_positions [3n] doubles, d_forces [3n] doubles
+ * accumulated on the device, d_energy one double written on the device (may be NULL). */
+int tdp_execute_device(tdp_context* ctx, const double* d_positions, const double* box, double* d_forces,
+                       double* d_energy, void* stream);
+
+/* Replaces getInducedDipoles / getLabFramePermanentDipoles / getTotalDipoles
+ * (TholeDipoleKernels.h:91-105, ReferenceTholeDipoleKernels.cpp:217-282): recomputes from the
+ * given positions, like the reference.  out: host [3n]. */
+int tdp_get_dipoles(tdp_context* ctx, const double* positions, const double* box, int32_t kind, double* out);
+
+/* Replaces getElectrostaticPotential (TholeDipoleKernels.h:113-114, ReferenceTholeDipoleKernels.cpp:284-307). */
+int tdp_get_electrostatic_potential(tdp_context* ctx, const double* positions, const double* box,
+                                    const double* grid, int32_t num_grid, double* out);
+
+/* Replaces getSystemMultipoleMoments (TholeDipoleKernels.h:121, ReferenceTholeDipoleKernels.cpp:309-315):
+ * masses [n]; out [13] (charge, dipole xyz in Debye, traceless quadrupole 3x3 in Debye*Angstrom). */
+int tdp_get_system_multipole_moments(tdp_context* ctx, const double* positions, const double* box,
+                                     const double* masses, double* out);
+
+/* Replaces getPMEParameters (TholeDipoleKernels.h:84, ReferenceTholeDipoleKernels.cpp:341-348). */
+int tdp_get_pme_parameters(const tdp_context* ctx, double* alpha, int32_t* nx, int32_t* ny, int32_t* nz);
+
+/* Solver diagnostics of the last evaluation (the reference records these silently,
+ * ReferenceTholeDipoleForce.cpp:849-850). */
+int tdp_get_solver_stats(const tdp_context* ctx, tdp_solver_stats* stats);
+
+/* Enables (and resets) GPU timing of every T*mu launch with HIP events recorded on the
+ * context's stream around the launch; the totals are reported by tdp_get_solver_stats. */
+int tdp_set_timing(tdp_context* ctx, int32_t enable);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,263 @@
+"""HIP path vs the CPU oracle, through the C ABI.
+
+Tolerances (from the path's parity bar): energy relative error <= 1e-6, force relative RMS
+error <= 1e-5, induced dipoles max error <= 1e-6 D (2.08e-8 e*nm).  The FP64 path is expected
+to agree far tighter; the assertions use the bar.  Solves compared here run to a tight target
+(1e-12 e*nm) unless a test checks the reference's own stopping rule, so that the comparison is
+not dominated by solver truncation.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import ref_systems
+from tholedipoleplugin_amd import systems
+from tholedipoleplugin_amd.context import Context
+from tholedipoleplugin_amd.force import OpenMMException, TholeDipoleForce
+
+pytestmark = pytest.mark.gpu
+
+T = TholeDipoleForce
+DEBYE = 0.020819434
+E_TOL = 1e-6
+F_TOL = 1e-5
+MU_TOL = 1e-6 * DEBYE
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def rel_rms(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
+
+
+def gpu_eval(rs, precision="double", solver="jacobi"):
+    ctx = Context(rs.system, precision=precision, solver=solver)
+    if rs.box is not None:
+        ctx.setPeriodicBoxVectors(*rs.box)
+    ctx.setPositions(rs.positions)
+    st = ctx.getState(getForces=True, getEnergy=True)
+    k = ctx._kernel_for(rs.force)
+    return ctx, st, k
+
+
+def tighten(rs, eps=1e-12, maxit=1000):
+    rs.force.setMutualInducedTargetEpsilon(eps)
+    rs.force.setMutualInducedMaxIterations(maxit)
+    return rs
+
+
+CASES = [("ammonia", T.Direct), ("ammonia", T.Mutual), ("ammonia", T.Extrapolated), ("water4", T.Direct),
+         ("water4", T.Mutual), ("water4", T.Extrapolated), ("triclinic", T.Mutual), ("zbisect", T.Mutual),
+         ("zbisect", T.Direct), ("zonly", T.Mutual), ("plasma", T.Mutual)]
+
+
+@pytest.mark.parametrize("name,pol", CASES)
+def test_reference_systems_fp64(name, pol):
+    _say("case %s pol=%d" % (name, pol))
+    rs = tighten(ref_systems.ALL[name](pol))
+    ctx, st, k = gpu_eval(rs)
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box)
+    e, f = st.getPotentialEnergy(), st.getForces()
+    assert abs(e - ref["energy"]) <= E_TOL * max(abs(ref["energy"]), 1e-12)
+    assert rel_rms(f, ref["forces"]) <= F_TOL
+    ind = rs.force.getInducedDipoles(ctx)
+    assert np.max(np.abs(ind - ref["induced"])) <= MU_TOL
+    lab = rs.force.getLabFramePermanentDipoles(ctx)
+    assert np.max(np.abs(lab - ref["labdip"])) <= 1e-12
+    tot = rs.force.getTotalDipoles(ctx)
+    assert np.max(np.abs(tot - (ref["labdip"] + ref["induced"]))) <= MU_TOL
+    stats = k.solverStats()
+    if pol == T.Mutual:
+        assert stats["converged"] == 1
+
+
+@pytest.mark.parametrize("name", ["ammonia", "water4", "triclinic"])
+def test_reference_stopping_rule(name):
+    """With the system's own target and the Jacobi solver, the iteration count and the final
+    epsilon follow convergeInducedDipolesByDIIS (ReferenceTholeDipoleForce.cpp:829-850)."""
+    _say("case stopping %s" % name)
+    rs = ref_systems.ALL[name](T.Mutual)
+    ctx, st, k = gpu_eval(rs)
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box)
+    stats = k.solverStats()
+    assert stats["iterations"] == ref["iterations"]
+    assert abs(stats["epsilon"] - ref["epsilon"]) <= 1e-6 * ref["epsilon"]
+    assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+
+
+@pytest.mark.parametrize("solver,precision", [("pcg", "double"), ("pcg", "mixed"), ("jacobi", "mixed")])
+@pytest.mark.parametrize("name", ["ammonia", "water4", "triclinic", "zbisect"])
+def test_solvers_and_precision(name, solver, precision):
+    _say("case %s %s %s" % (name, solver, precision))
+    rs = tighten(ref_systems.ALL[name](T.Mutual), eps=1e-11)
+    ctx, st, k = gpu_eval(rs, precision=precision, solver=solver)
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box, target_eps=1e-13, max_iter=5000)
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= E_TOL * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+    assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
+
+
+def test_water216_fp64_jacobi_matches_oracle():
+    """216 waters, 648 sites, reference PME semantics, Mutual to 1e-6 D with the reference iteration."""
+    _say("case water216 jacobi fp64")
+    s = systems.config0()
+    rs = ref_systems.RefSystem("water216", None, s.force, s.positions, s.box)
+    from tholedipoleplugin_amd.context import System
+    sysm = System()
+    for m in s.masses:
+        sysm.addParticle(m)
+    sysm.setDefaultPeriodicBoxVectors(*s.box)
+    sysm.addForce(s.force)
+    rs.system = sysm
+    ctx, st, k = gpu_eval(rs)
+    ref = oracle.force_energy(s.force, s.positions, s.box, threads=8)
+    stats = k.solverStats()
+    assert stats["iterations"] == ref["iterations"]
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= E_TOL * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+    assert np.max(np.abs(s.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
+    # mixed precision + PCG against the same oracle
+    ctx2, st2, k2 = gpu_eval(rs, precision="mixed", solver="pcg")
+    assert rel_rms(st2.getForces(), ref["forces"]) <= F_TOL
+    assert abs(st2.getPotentialEnergy() - ref["energy"]) <= 1e-5 * abs(ref["energy"])
+
+
+def test_potential_and_moments():
+    _say("case potential/moments")
+    for name in ("ammonia", "water4"):
+        rs = tighten(ref_systems.ALL[name](T.Mutual))
+        ctx, st, k = gpu_eval(rs)
+        rng = np.random.default_rng(5)
+        grid = rng.uniform(-0.8, 0.8, (100, 3))
+        pot = rs.force.getElectrostaticPotential(grid, ctx)
+        ref = oracle.potential(rs.force, rs.positions, grid, rs.box)
+        assert np.max(np.abs(pot - ref)) <= 1e-9 * max(1.0, np.max(np.abs(ref)))
+        mom = rs.force.getSystemMultipoleMoments(ctx)
+        masses = [rs.system.getParticleMass(i) for i in range(rs.system.getNumParticles())]
+        refm = oracle.multipole_moments(rs.force, rs.positions, masses, rs.box)
+        assert np.max(np.abs(mom - refm)) <= 1e-9 * max(1.0, np.max(np.abs(refm)))
+
+
+def test_update_parameters_in_context():
+    """testTholeDipoleAmmoniaMutualPolarization live assertions (TestReferenceTholeDipoleForce.cpp:285-313)."""
+    _say("case updateParametersInContext")
+    rs = ref_systems.ammonia(T.Mutual)
+    ctx, st1, _ = gpu_eval(rs)
+    f = rs.force
+    for i in range(f.getNumParticles()):
+        q, d, a, th, ax, z, x, y = f.getParticleParameters(i)
+        d[0] *= 0.7
+        f.setParticleParameters(i, 1.1 * q, d, 1.5 * a, 1.3 * th, ax, z, x, y)
+    ctx2 = Context(rs.system)
+    ctx2.setPositions(rs.positions)
+    s1 = ctx.getState(getForces=True, getEnergy=True)
+    s2 = ctx2.getState(getForces=True, getEnergy=True)
+    assert abs(s1.getPotentialEnergy() - s2.getPotentialEnergy()) > 1e-4 * abs(s2.getPotentialEnergy())
+    f.updateParametersInContext(ctx)
+    s1 = ctx.getState(getForces=True, getEnergy=True)
+    assert abs(s1.getPotentialEnergy() - s2.getPotentialEnergy()) <= 1e-10 * abs(s2.getPotentialEnergy())
+    assert rel_rms(s1.getForces(), s2.getForces()) <= 1e-10
+
+
+def test_plasma_energy_independent_of_cutoff():
+    """testNeutralizingPlasmaCorrection (TestReferenceTholeDipoleForce.cpp:761-801)."""
+    _say("case plasma")
+    rs = ref_systems.plasma()
+    ctx, st, _ = gpu_eval(rs)
+    e1 = st.getPotentialEnergy()
+    rs.force.setCutoffDistance(0.7)
+    ctx.reinitialize(True)
+    e2 = ctx.getState(getEnergy=True).getPotentialEnergy()
+    assert abs(e1 - e2) <= 1e-4 * abs(e1)
+    rs.force.setParticleParameters(0, 2.0, [0, 0, 0], 0.001, 0.39, T.NoAxisType, 0, 0, 0)
+    rs.force.updateParametersInContext(ctx)
+    e3 = ctx.getState(getEnergy=True).getPotentialEnergy()
+    rs.force.setCutoffDistance(1.0)
+    ctx.reinitialize(True)
+    e4 = ctx.getState(getEnergy=True).getPotentialEnergy()
+    assert abs(e3 - e4) <= 1e-4 * abs(e3)
+    assert abs(e3 - 2.0 * e1) <= 1e-9 * abs(e3)  # q0 doubled: charge-charge energy doubles
+
+
+def test_error_paths():
+    _say("case errors")
+    rs = ref_systems.water4()
+    ctx, _, k = gpu_eval(rs)
+    # box shrunk below twice the cutoff (ReferenceTholeDipoleKernels.cpp:170-173)
+    ctx.setPeriodicBoxVectors([1.3, 0, 0], [0, 1.3, 0], [0, 0, 1.3])
+    with pytest.raises(OpenMMException, match="less than twice the nonbonded cutoff"):
+        ctx.getState(getEnergy=True)
+    # particle count change (ReferenceTholeDipoleKernels.cpp:318-319)
+    rs.force.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    with pytest.raises(OpenMMException, match="number of particles has changed"):
+        k.copyParametersToContext(ctx, rs.force)
+    # PME parameters on a NoCutoff context (ReferenceTholeDipoleKernels.cpp:341-343)
+    am = ref_systems.ammonia()
+    actx, _, ak = gpu_eval(am)
+    with pytest.raises(OpenMMException, match="not using PME"):
+        am.force.getPMEParametersInContext(actx)
+    # explicit PME parameters are reported as given; alpha = 0 selects from the tolerance
+    w = ref_systems.water4()
+    wctx, _, _ = gpu_eval(w)
+    a, nx, ny, nz = w.force.getPMEParametersInContext(wctx)
+    assert (a, nx, ny, nz) == (5.4459052, 20, 20, 20)
+    w2 = ref_systems.water4()
+    w2.force.setPMEParameters(0.0, 0, 0, 0)
+    w2ctx, _, _ = gpu_eval(w2)
+    a, nx, ny, nz = w2.force.getPMEParametersInContext(w2ctx)
+    tol, rc, L = 1e-4, 0.7, 1.8643
+    assert abs(a - np.sqrt(-np.log(2 * tol)) / rc) < 1e-12
+    assert nx == int(np.ceil(2 * a * L / (3 * tol ** 0.2)))
+
+
+def test_device_entry_point_matches_host():
+    """tdp_execute_device with torch-allocated device buffers equals tdp_execute."""
+    _say("case device entry")
+    torch = pytest.importorskip("torch")
+    rs = tighten(ref_systems.water4())
+    ctx, st, k = gpu_eval(rs)
+    dev = torch.device("cuda:0")
+    pos = torch.tensor(rs.positions.reshape(-1), dtype=torch.float64, device=dev)
+    frc = torch.ones(pos.numel(), dtype=torch.float64, device=dev)
+    en = torch.zeros(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    k.handle.execute_device(pos.data_ptr(), rs.box, frc.data_ptr(), en.data_ptr(), stream.cuda_stream)
+    torch.cuda.synchronize()
+    f = frc.cpu().numpy().reshape(-1, 3) - 1.0
+    assert np.max(np.abs(f - st.getForces())) <= 1e-12 * np.max(np.abs(st.getForces()))
+    assert abs(en.item() - st.getPotentialEnergy()) <= 1e-12 * abs(st.getPotentialEnergy())
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_water12288_rows_full_size(precision):
+    """36,864 sites (12,288 waters, 4.95 nm box, reference PME semantics), PCG to 1e-6 D.
+    Checks at full size: the solve converged; sampled rows of the fixed point
+    mu_i = alpha_i (E0_i + (T mu)_i) and of the forces, evaluated by the oracle over all
+    36,863 partners with the GPU's dipoles; mixed vs FP64 force agreement."""
+    _say("case water12288 %s" % precision)
+    s = systems.config1()
+    from tholedipoleplugin_amd.context import System
+    sysm = System()
+    for m in s.masses:
+        sysm.addParticle(m)
+    sysm.setDefaultPeriodicBoxVectors(*s.box)
+    sysm.addForce(s.force)
+    ctx = Context(sysm, precision=precision, solver="pcg")
+    ctx.setPeriodicBoxVectors(*s.box)
+    ctx.setPositions(s.positions)
+    st = ctx.getState(getForces=True, getEnergy=True)
+    k = ctx._kernel_for(s.force)
+    stats = k.solverStats()
+    _say("  stats %s" % stats)
+    assert stats["converged"] == 1
+    mu = s.force.getInducedDipoles(ctx)
+    rows = np.random.default_rng(7).choice(s.n, 48, replace=False)
+    ref = oracle.rows(s.force, s.positions, mu, rows, s.box, threads=8)
+    alpha = s.force.packed_arrays()["polarity"][rows][:, None]
+    resid = mu[rows] - alpha * (ref["e0"] + ref["eind"])
+    assert np.max(np.abs(resid)) <= 5 * MU_TOL
+    assert rel_rms(st.getForces()[rows], ref["force"]) <= F_TOL

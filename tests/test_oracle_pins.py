@@ -1,0 +1,468 @@
+"""Pins of the CPU oracle (CPU only).
+
+The reference holds no live numeric golden values for this path (its expected values are
+placeholders with the comparisons commented out, TestReferenceTholeDipoleForce.cpp:241-254 etc.)
+and the reference core cannot be compiled here (it needs OpenMM headers that are absent).  The
+oracle is therefore pinned by checks that a plausible restatement mistake fails:
+
+* hand-derived scale tables for the reference tests' own topologies (position-indexed,
+  one-sided lookups);
+* the Thole damping functions through the gradient identity that links the rank-3 and rank-5
+  factors, a scaling invariance that fixes the damping length a_i*a_j, and exact limits;
+* the permanent field and T*mu against a dense textbook Cartesian assembly, and the Jacobi
+  solution against a dense linear solve;
+* the pair energy against a textbook Cartesian multipole energy evaluated along the QI frame's
+  X axis, and the pair force/torques against finite differences of that energy in the
+  frame the reference's index pattern implies (x' = Y, y' = Z, z' = X), mapped back with the
+  reference's cyclic component order;
+* closed forms for the potential, the multipole moments and the minimum image;
+* net-force and net-torque identities of the torque-to-force mapping.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+import ref_systems
+from tholedipoleplugin_amd.force import TholeDipoleForce
+
+T = TholeDipoleForce
+
+
+# ----------------------------------------------------------------------------- scale maps (a2)
+def _scales(rs, pairs):
+    return {p: oracle.pair_scales(rs.force, *p) for p in pairs}
+
+
+def test_scale_table_water():
+    """Water pattern O:12=[H1,H2]; H1:12=[O],13=[H2]; H2:12=[O],13=[H1].  Only map[min] is read:
+    (O,H) take m from O's 1-2 list (positions 0 and 1 -> 0) and i from O's empty 1-3 list (1);
+    (H1,H2) take m from H1's 1-2 list (H2 absent -> 1) and i from position 0 of H1's 1-3 list (0)."""
+    rs = ref_systems.water4()
+    s = _scales(rs, [(0, 1), (0, 2), (1, 2), (1, 0), (2, 1), (0, 3), (2, 4), (3, 4), (4, 5)])
+    assert s[(0, 1)] == (0.0, 1.0) and s[(0, 2)] == (0.0, 1.0)
+    assert s[(1, 2)] == (1.0, 0.0) and s[(2, 1)] == (1.0, 0.0) and s[(1, 0)] == (0.0, 1.0)
+    assert s[(0, 3)] == (1.0, 1.0) and s[(2, 4)] == (1.0, 1.0)
+    assert s[(3, 4)] == (0.0, 1.0) and s[(4, 5)] == (1.0, 0.0)
+
+
+def test_scale_table_ammonia():
+    """N:12=[H1,H2,H3]; H1:13=[H2,H3]; H2:13=[H1,H3]: (N,Hk) m=0; (H1,H2) i = position 0 -> 0;
+    (H1,H3) i = position 1 -> 1; (H2,H3) i = position 1 of H2's list -> 1."""
+    rs = ref_systems.ammonia()
+    s = _scales(rs, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (0, 4), (5, 6), (5, 7), (6, 7)])
+    assert s[(0, 1)] == s[(0, 2)] == s[(0, 3)] == (0.0, 1.0)
+    assert s[(1, 2)] == (1.0, 0.0)
+    assert s[(1, 3)] == (1.0, 1.0) and s[(2, 3)] == (1.0, 1.0)
+    assert s[(0, 4)] == (1.0, 1.0)
+    assert s[(5, 6)] == (1.0, 0.0) and s[(5, 7)] == (1.0, 1.0) and s[(6, 7)] == (1.0, 1.0)
+
+
+def test_scale_table_zbisect():
+    """Particle 0 has an empty 1-2 list and 1-3 = [1,2,3]: (0,1) i=0, (0,2)/(0,3) i=1, all m=1."""
+    rs = ref_systems.zbisect()
+    s = _scales(rs, [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3), (4, 5), (4, 6), (5, 6)])
+    assert s[(0, 1)] == (1.0, 0.0) and s[(0, 2)] == (1.0, 1.0) and s[(0, 3)] == (1.0, 1.0)
+    assert s[(1, 2)] == (1.0, 0.0) and s[(1, 3)] == (1.0, 1.0) and s[(2, 3)] == (1.0, 1.0)
+    assert s[(4, 5)] == (1.0, 0.0) and s[(4, 6)] == (1.0, 1.0) and s[(5, 6)] == (1.0, 0.0)
+
+
+def test_scale_positions_duplicates_and_limits():
+    """Positions 0..4 of a 1-2 list give m = 0,0,0,0.5,1; a duplicated atom takes its last
+    position; a sixth entry is rejected (the reference indexes a 5-element table out of bounds)."""
+    f = T()
+    for _ in range(8):
+        f.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    f.setCovalentMap(0, T.Covalent12, [1, 2, 3, 4, 5])
+    f.setCovalentMap(0, T.Covalent13, [6, 7])
+    f.setCovalentMap(1, T.Covalent12, [7, 3, 7])
+    assert [oracle.pair_scales(f, 0, j)[0] for j in (1, 2, 3, 4, 5)] == [0.0, 0.0, 0.0, 0.5, 1.0]
+    assert oracle.pair_scales(f, 0, 6) == (1.0, 0.0) and oracle.pair_scales(f, 0, 7) == (1.0, 1.0)
+    assert oracle.pair_scales(f, 1, 7)[0] == 0.0   # last occurrence: position 2 -> 0.0
+    assert oracle.pair_scales(f, 1, 3)[0] == 0.0   # position 1
+    f.setCovalentMap(0, T.Covalent12, [1, 2, 3, 4, 5, 6])
+    with pytest.raises(RuntimeError):
+        oracle.pair_scales(f, 0, 1)
+
+
+# ----------------------------------------------------------------------------- Thole damping (a5)
+def _lam(ai, aj, r):
+    rr3, rr5 = oracle.damped_rr(ai, aj, r)
+    return rr3 * r ** 3, rr5 * r ** 5 / 3.0
+
+
+@pytest.mark.parametrize("ai,aj", [(0.39, 0.39), (0.3, 0.45), (0.45, 0.33)])
+def test_damping_gradient_identity(ai, aj):
+    """A damped dipole tensor is the Hessian of a damped potential only if
+    lambda5 = lambda3 - (r/3) d(lambda3)/dr; checked by central differences."""
+    for r in (0.02, 0.05, 0.09, 0.15, 0.3):
+        h = 1e-6 * r
+        l3p, _ = _lam(ai, aj, r + h)
+        l3m, _ = _lam(ai, aj, r - h)
+        l3, l5 = _lam(ai, aj, r)
+        d = (l3p - l3m) / (2 * h)
+        assert abs(l5 - (l3 - r / 3.0 * d)) < 1e-7
+
+
+def test_damping_length_is_product_of_thole_parameters():
+    """With a_i = a_j = a the exponent is -a (r/a^2)^3 = -r^3/a^5: (r, a) -> (c^(5/3) r, c a)
+    leaves lambda3 unchanged.  A damping length (alpha_i alpha_j)^(1/6) would not."""
+    for a, r in ((0.39, 0.07), (0.3, 0.12)):
+        c = 1.3
+        assert abs(_lam(a, a, r)[0] - _lam(c * a, c * a, c ** (5.0 / 3.0) * r)[0]) < 1e-12
+
+
+def test_damping_limits():
+    r = 0.1
+    rr3, rr5 = oracle.damped_rr(0.0, 0.39, r)         # damp = 0: undamped
+    assert abs(rr3 - r ** -3) < 1e-12 * r ** -3
+    assert abs(rr5 - 3 * r ** -5) < 1e-12 * r ** -5
+    # gamma*(r/damp)^3 > 50 -> no damping at all (exactly the undamped values)
+    # (a = 0.39: 0.39 (1.0/0.1521)^3 = 111 > 50)
+    rr3, rr5 = oracle.damped_rr(0.39, 0.39, 1.0)
+    assert rr3 == 1.0 and rr5 == 3.0
+    rr3, rr5 = oracle.damped_rr(0.39, 0.39, 0.6)    # 0.39 (0.6/0.1521)^3 = 23.9 < 50: damped
+    assert rr3 < (1.0 / 0.6) ** 3
+    # small r: lambda3 ~ gamma (r/damp)^3, lambda5 ~ gamma^2 (r/damp)^6 / 2
+    a = 0.39
+    for r in (0.015, 0.025):
+        l3, l5 = _lam(a, a, r)
+        x = a * (r / (a * a)) ** 3
+        assert abs(l3 / x - 1) < x
+        assert abs(l5 / (x * x / 2) - 1) < x
+
+
+# ----------------------------------------------------------------------------- dense textbook helpers
+def _textbook_lambdas(ai, aj, r):
+    damp = np.outer(ai, aj)
+    g = np.minimum(ai[:, None], aj[None, :])
+    x = g * (r / np.where(damp != 0, damp, 1.0)) ** 3
+    e = np.where((damp != 0) & (x < 50), np.exp(-np.minimum(x, 50)), 0.0)
+    return 1 - e, 1 - (1 + x) * e
+
+
+def _dense(force, pos, box):
+    a = force.packed_arrays()
+    n = force.getNumParticles()
+    R = pos[:, None, :] - pos[None, :, :]          # R_ij = r_i - r_j
+    if box is not None:
+        L = np.diag(np.asarray(box).reshape(3, 3))
+        D = -R                                     # reference delta r_j - r_i, minimum image on the diagonal
+        D = D - L * np.floor(D / L + 0.5)
+        R = -D
+    r = np.sqrt((R ** 2).sum(-1))
+    np.fill_diagonal(r, 1.0)
+    l3, l5 = _textbook_lambdas(a["thole"], a["thole"], r)
+    M = np.ones((n, n)); I = np.ones((n, n))
+    for i in range(n):
+        for j in range(i + 1, n):
+            M[i, j], I[i, j] = oracle.pair_scales(force, i, j)
+            M[j, i], I[j, i] = M[i, j], I[i, j]
+    np.fill_diagonal(M, 0); np.fill_diagonal(I, 0)
+    return a, R, r, l3, l5, M, I
+
+
+def _field_tensor(R, r, l3, l5, w):
+    """Thole-damped dipole field tensor T_ij = w (3 l5 R R^T / r^5 - l3 I / r^3)."""
+    n = r.shape[0]
+    Tt = (3 * l5 / r ** 5)[:, :, None, None] * R[:, :, :, None] * R[:, :, None, :]
+    Tt -= (l3 / r ** 3)[:, :, None, None] * np.eye(3)[None, None]
+    Tt *= w[:, :, None, None]
+    return Tt.transpose(0, 2, 1, 3).reshape(3 * n, 3 * n)
+
+
+def _random_cluster(n_mol, seed, box=None):
+    from tholedipoleplugin_amd import systems
+    s = systems.water_cluster(n_mol, seed)
+    # give the sites permanent dipoles too (NoAxisType: used as lab-frame dipoles)
+    rng = np.random.default_rng(seed + 1)
+    for i in range(s.force.getNumParticles()):
+        q, d, al, th, ax, z, x, y = s.force.getParticleParameters(i)
+        s.force.setParticleParameters(i, q, list(rng.normal(0, 0.01, 3)), al, th, ax, z, x, y)
+    return s
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_fixed_field_and_matvec_vs_dense_textbook(periodic):
+    s = _random_cluster(14, 11)
+    pos = s.positions
+    box = None
+    if periodic:
+        s.force.setNonbondedMethod(T.PME)
+        box = np.diag([1.2, 1.3, 1.1])
+    a, R, r, l3, l5, M, I = _dense(s.force, pos, box)
+    n = pos.shape[0]
+    d = a["dipole"].reshape(n, 3)
+    q = a["charge"]
+    # charge field q_j l3 R/r^3 plus dipole field T d, scaled by m
+    E0 = ((M * q[None, :] * l3 / r ** 3)[:, :, None] * R).sum(1)
+    E0 += (_field_tensor(R, r, l3, l5, M) @ d.reshape(-1)).reshape(n, 3)
+    Kmat = _field_tensor(R, r, l3, l5, I)
+    assert np.allclose(Kmat, Kmat.T, atol=1e-9 * np.abs(Kmat).max())
+    alpha = np.repeat(a["polarity"], 3)
+    mu_exact = np.linalg.solve(np.diag(1 / alpha) - Kmat, E0.reshape(-1)).reshape(n, 3)
+    ref = oracle.induced_dipoles(s.force, pos, box, target_eps=1e-14, max_iter=5000)
+    assert np.allclose(ref["fixed_field"], E0, rtol=0, atol=1e-10 * np.abs(E0).max())
+    assert np.max(np.abs(ref["induced"] - mu_exact)) < 1e-12
+    # rows of T*mu against the dense product
+    rows = oracle.rows(s.force, pos, mu_exact, np.arange(n), box)
+    assert np.allclose(rows["eind"], (Kmat @ mu_exact.reshape(-1)).reshape(n, 3), atol=1e-10)
+    assert np.allclose(rows["e0"], E0, atol=1e-10 * np.abs(E0).max())
+
+
+def test_direct_and_extrapolated_vs_dense_recurrence():
+    s = _random_cluster(8, 3)
+    a, R, r, l3, l5, M, I = _dense(s.force, s.positions, None)
+    n = s.n
+    Kmat = _field_tensor(R, r, l3, l5, I)
+    alpha = np.repeat(a["polarity"], 3)
+    E0 = oracle.induced_dipoles(s.force, s.positions, polarization=T.Direct)["fixed_field"].reshape(-1)
+    direct = oracle.induced_dipoles(s.force, s.positions, polarization=T.Direct)
+    assert np.allclose(direct["induced"].reshape(-1), alpha * E0, rtol=1e-14, atol=0)
+    coef = [-0.154, 0.017, 0.658, 0.474]
+    hist = [alpha * E0]
+    for k in range(3):
+        hist.append(alpha * (E0 + Kmat @ hist[-1]))
+    mu = sum(c * h for c, h in zip(coef, hist))
+    ext = oracle.induced_dipoles(s.force, s.positions, polarization=T.Extrapolated)
+    assert np.max(np.abs(ext["induced"].reshape(-1) - mu)) < 1e-14
+    eps = np.sqrt(np.sum((alpha * (E0 + Kmat @ mu) - mu) ** 2) / n)
+    assert abs(ext["epsilon"] - eps) < 1e-10 * eps and ext["iterations"] == 4
+    # one Jacobi iteration: epsilon = RMS of the first update (ReferenceTholeDipoleForce.cpp:838-846)
+    one = oracle.induced_dipoles(s.force, s.positions, polarization=T.Mutual, max_iter=1, target_eps=0.0)
+    mu1 = alpha * (E0 + Kmat @ (alpha * E0))
+    assert np.max(np.abs(one["induced"].reshape(-1) - mu1)) < 1e-14
+    assert abs(one["epsilon"] - np.sqrt(np.sum((mu1 - alpha * E0) ** 2) / n)) < 1e-12
+    assert one["iterations"] == 1 and one["converged"] == 0
+
+
+# ----------------------------------------------------------------------------- pair electrostatics (a9)
+def _frame_x(pi, pk, delta):
+    z = delta / np.linalg.norm(delta)
+    e = np.array([1.0, 0, 0]) if (pi[1] != pk[1] or pi[2] != pk[2]) else np.array([0, 1.0, 0])
+    x = e - z * (z @ e)
+    return x / np.linalg.norm(x)
+
+
+def _thole(ai, ak, r):
+    dmp = ai * ak
+    if abs(dmp) <= 1e-5:
+        return 1.0, 1.0
+    x = min(ai, ak) * (r / dmp) ** 3
+    e = math.exp(-x)
+    return 1 - e, 1 - (1 + x) * e
+
+
+def _energy_textbook(D, qi, qk, di, dk, ui, uk, ai, ak, m, s, w_pind, w_uu):
+    """Cartesian charge+dipole pair energy for separation D = r_K - r_I:
+    m [qq/r + (qK n.dI - qI n.dK)/r^2 + (dI.dK - 3 n.dI n.dK)/r^3]
+    + w_pind s [l3 (qK n.uI - qI n.uK)/r^2 + (l3 (dI.uK + dK.uI) - 3 l5 (n.dI n.uK + n.dK n.uI))/r^3]
+    + w_uu s (l3 uI.uK - 3 l5 n.uI n.uK)/r^3, with Thole l3 = 1-e^-x, l5 = 1-(1+x)e^-x."""
+    r = np.linalg.norm(D)
+    n = D / r
+    l3, l5 = _thole(ai, ak, r)
+    E = m * (qi * qk / r + (qk * (n @ di) - qi * (n @ dk)) / r ** 2 + (di @ dk - 3 * (n @ di) * (n @ dk)) / r ** 3)
+    E += w_pind * s * (l3 * (qk * (n @ ui) - qi * (n @ uk)) / r ** 2
+                       + (l3 * (di @ uk + dk @ ui) - 3 * l5 * ((n @ di) * (n @ uk) + (n @ dk) * (n @ ui))) / r ** 3)
+    E += w_uu * s * (l3 * (ui @ uk) - 3 * l5 * (n @ ui) * (n @ uk)) / r ** 3
+    return E
+
+
+def _cyc(v):
+    return np.array([v[1], v[2], v[0]])
+
+
+def _grad(fn, x, h=1e-6):
+    g = np.zeros(3)
+    for k in range(3):
+        e = np.zeros(3); e[k] = h
+        g[k] = (fn(x + e) - fn(x - e)) / (2 * h)
+    return g
+
+
+PAIR_CASES = [
+    # pos_k, m, iScale, polarization
+    ((0.23, -0.11, 0.17), 1.0, 1.0, T.Mutual),
+    ((0.31, 0.05, -0.09), 0.5, 1.0, T.Direct),
+    ((-0.08, 0.27, 0.12), 1.0, 0.0, T.Extrapolated),
+    ((0.12, 0.0, 0.0), 0.0, 1.0, T.Mutual),           # same y and z: the e_y branch of the frame
+    ((0.07, 0.05, 0.02), 1.0, 1.0, T.Mutual),         # strongly damped
+]
+
+
+@pytest.mark.parametrize("pk,m,s,pol", PAIR_CASES)
+def test_pair_energy_force_torque_textbook(pk, m, s, pol):
+    rng = np.random.default_rng(int(1000 * abs(pk[0])))
+    pi = np.array([0.0, 0.0, 0.0]) if pk[1] == 0.0 else rng.normal(0, 0.05, 3)
+    pk = np.array(pk) + (pi if pk[1] != 0.0 else 0.0)
+    qi, qk = 0.41, -0.73
+    di, dk, ui, uk = (rng.normal(0, 0.02, 3) for _ in range(4))
+    ai, ak = 0.39, 0.33
+    e, fk, ti, tk = oracle.pair_electrostatic(pi, pk, qi, qk, di, dk, ui, uk, ai, ak, m, s, pol)
+    D = pk - pi
+    r = np.linalg.norm(D)
+    X = _frame_x(pi, pk, D)
+    # energy: the textbook energy with the bond axis replaced by the frame's X axis
+    e_txt = _energy_textbook(r * X, qi, qk, di, dk, ui, uk, ai, ak, m, s, 0.5, 0.0)
+    assert abs(e - e_txt) < 1e-10 * max(1.0, abs(e_txt))
+    # force on K and torques: derivatives of the interaction energy (perm-induced at full weight,
+    # induced-induced at weight 2 for Mutual) in the fictitious geometry D' = r X, mapped back
+    # with the cyclic component order of the reference's force rotation.
+    w_uu = 2.0 if pol == T.Mutual else 0.0
+    Ef = lambda Dp: _energy_textbook(Dp, qi, qk, di, dk, ui, uk, ai, ak, m, s, 1.0, w_uu)
+    f_fict = -_grad(Ef, r * X, h=1e-7 * r)
+    assert np.allclose(fk, _cyc(f_fict), rtol=0, atol=2e-6 * max(1.0, np.abs(f_fict).max()))
+    Ed_i = lambda d: _energy_textbook(r * X, qi, qk, d, dk, ui, uk, ai, ak, m, s, 1.0, w_uu)
+    Ed_k = lambda d: _energy_textbook(r * X, qi, qk, di, d, ui, uk, ai, ak, m, s, 1.0, w_uu)
+    tau_i = -np.cross(di, _grad(Ed_i, di, h=1e-8))
+    tau_k = -np.cross(dk, _grad(Ed_k, dk, h=1e-8))
+    scale = max(1e-6, np.abs(tau_i).max(), np.abs(tau_k).max())
+    assert np.allclose(ti, _cyc(tau_i), atol=1e-6 * scale)
+    assert np.allclose(tk, _cyc(tau_k), atol=1e-6 * scale)
+
+
+def test_charge_pair_closed_form():
+    """Two charges: E = m qI qK / r exactly (rotation invariant); the force on K is
+    m qI qK / r^2 times cyc(X), perpendicular to the bond for a generic orientation."""
+    pi = np.zeros(3)
+    pk = np.array([1.0, 1.0, 1.0]) / math.sqrt(3) * 0.25
+    e, fk, ti, tk = oracle.pair_electrostatic(pi, pk, 0.8, -0.6, np.zeros(3), np.zeros(3), np.zeros(3),
+                                              np.zeros(3), 0.39, 0.39, 1.0, 1.0, T.Direct)
+    r = 0.25
+    assert abs(e - 0.8 * -0.6 / r) < 1e-14
+    X = np.array([2.0, -1.0, -1.0]) / math.sqrt(6.0)
+    assert np.allclose(fk, 0.8 * -0.6 / r ** 2 * _cyc(X), atol=1e-12)
+    assert abs(fk @ pk) < 1e-12   # perpendicular to the bond (finding: forces are not -grad E)
+    assert np.all(ti == 0) and np.all(tk == 0)
+    # along z the frame is right: force along the bond
+    e, fk, _, _ = oracle.pair_electrostatic(pi, np.array([0, 0, r]), 0.8, -0.6, np.zeros(3), np.zeros(3), np.zeros(3),
+                                            np.zeros(3), 0.39, 0.39, 1.0, 1.0, T.Direct)
+    assert np.allclose(fk, [0, 0, 0.8 * -0.6 / r ** 2], atol=1e-13)
+
+
+def test_qi_frame_is_orthonormal():
+    rng = np.random.default_rng(2)
+    for _ in range(20):
+        pi, pk = rng.normal(0, 1, 3), rng.normal(0, 1, 3)
+        R = oracle.qi_frame(pi, pk)
+        assert np.allclose(R @ R.T, np.eye(3), atol=1e-13)
+        assert np.allclose(R[2], (pk - pi) / np.linalg.norm(pk - pi), atol=1e-14)
+        assert abs(np.linalg.det(R) - 1) < 1e-13
+        assert np.allclose(R[0], _frame_x(pi, pk, pk - pi), atol=1e-13)
+
+
+# ----------------------------------------------------------------------------- whole evaluations
+def test_energy_is_sum_of_pair_energies_and_rows():
+    rs = ref_systems.water4(T.Mutual)
+    rs.force.setMutualInducedTargetEpsilon(1e-13)
+    full = oracle.force_energy(rs.force, rs.positions, rs.box)
+    rows = oracle.rows(rs.force, rs.positions, full["induced"], np.arange(12), rs.box)
+    assert abs(rows["energy"].sum() - full["energy"]) < 1e-12 * abs(full["energy"])
+    # no axis atoms are involved in the torque mapping of forces on rows? -> compare torques
+    assert np.allclose(rows["torque"], full["torques"], atol=1e-12)
+
+
+def test_min_image_closed_form():
+    """Two unit charges 0.2 nm apart through the periodic boundary: E = 1/0.2."""
+    f = T()
+    f.setNonbondedMethod(T.PME)
+    f.setPolarizationType(T.Direct)
+    for _ in range(2):
+        f.addParticle(1.0, [0, 0, 0], 0.0, 0.0)
+    L = 2.0
+    pos = np.array([[0.1, 0.5, 0.5], [L - 0.1, 0.5, 0.5]])
+    r = oracle.force_energy(f, pos, np.diag([L, L, L]))
+    assert abs(r["energy"] - 1.0 / 0.2) < 1e-12
+    # off-diagonal box terms are ignored (ReferenceTholeDipoleForce.cpp:1414-1423)
+    tri = np.array([[L, 0, 0], [0.3, L, 0], [-0.2, 0.1, L]])
+    assert abs(oracle.force_energy(f, pos, tri)["energy"] - 1.0 / 0.2) < 1e-12
+
+
+def test_potential_closed_form():
+    """One particle (no other field, so no induced dipole): phi(g) = q/r + d.(g - x)/r^3."""
+    f = T()
+    d = np.array([0.01, -0.02, 0.005])
+    f.addParticle(0.3, d, 0.001, 0.39)
+    x = np.array([0.1, 0.2, -0.1])
+    g = np.array([[0.5, 0.1, 0.2], [-0.3, 0.4, 0.0]])
+    phi = oracle.potential(f, x[None, :], g)
+    for k in range(2):
+        v = g[k] - x
+        r = np.linalg.norm(v)
+        assert abs(phi[k] - (0.3 / r + d @ v / r ** 3)) < 1e-13
+
+
+def test_multipole_moments_closed_form():
+    """+q/-q at +-a/2 on x around the centre of mass: net charge 0, dipole q a (e nm) in Debye
+    (x 48.0321), traceless quadrupole xx = q a^2/2 * 2/3 ... checked against its definition."""
+    f = T()
+    q, a = 0.5, 0.2
+    f.addParticle(q, [0, 0, 0], 0.0, 0.0)
+    f.addParticle(-q, [0, 0, 0], 0.0, 0.0)
+    pos = np.array([[a / 2, 0, 0], [-a / 2, 0, 0]])
+    m = oracle.multipole_moments(f, pos, [1.0, 1.0])
+    assert abs(m[0]) < 1e-15
+    assert abs(m[1] - q * a * 48.0321) < 1e-12 and abs(m[2]) < 1e-15 and abs(m[3]) < 1e-15
+    xx = q * (a / 2) ** 2 + (-q) * (a / 2) ** 2   # = 0 for a symmetric pair
+    assert abs(m[4] - 0.5 * (xx - xx / 3) * 100 * 3 * 4.80321) < 1e-12
+
+
+# ----------------------------------------------------------------------------- frames (a3) and torques (a10)
+def test_lab_frame_identity_and_norms():
+    f = T()
+    d0 = [0.01, 0.02, 0.03]
+    f.addParticle(0.0, d0, 0.001, 0.39, T.ZThenX, 1, 2, -1)
+    f.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    f.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    pos = np.array([[0, 0, 0], [0, 0, 0.1], [0.1, 0.0, 0.03]])
+    lab = oracle.lab_dipoles(f, pos)
+    assert np.allclose(lab[0], d0, atol=1e-15)   # Z along z, X orthogonalised onto x: frame = lab axes
+    rs = ref_systems.ammonia()
+    lab = oracle.lab_dipoles(rs.force, rs.positions)
+    mol = rs.force.packed_arrays()["dipole"].reshape(-1, 3)
+    assert np.allclose(np.linalg.norm(lab, axis=1), np.linalg.norm(mol, axis=1), rtol=1e-13)
+
+
+def test_chirality_flip():
+    """ZThenX with a Y atom: mirroring the Y atom through the Z-X plane flips the y component."""
+    f = T()
+    f.addParticle(0.0, [0.0, 0.02, 0.0], 0.001, 0.39, T.ZThenX, 1, 2, 3)
+    for _ in range(3):
+        f.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    pos = np.array([[0, 0, 0], [0, 0, 0.1], [0.1, 0, 0], [0.03, 0.1, 0.02]])
+    a = oracle.lab_dipoles(f, pos)[0]
+    pos[3, 1] *= -1
+    b = oracle.lab_dipoles(f, pos)[0]
+    assert np.allclose(a, -b, atol=1e-15) and abs(abs(a[1]) - 0.02) < 1e-15
+
+
+@pytest.mark.parametrize("axis", [T.ZThenX, T.Bisector, T.ZBisect, T.ThreeFold, T.ZOnly])
+def test_torque_mapping_conserves_force(axis):
+    f = T()
+    f.addParticle(0.0, [0, 0, 0], 0.001, 0.39, axis, 1, 2, 3)
+    for _ in range(3):
+        f.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    pos = np.array([[0, 0, 0], [0.02, 0.01, 0.1], [0.1, -0.02, 0.01], [-0.03, 0.09, 0.02]])
+    tq = np.zeros((4, 3))
+    tq[0] = [0.3, -0.2, 0.5]
+    F = oracle.map_torques(f, pos, tq)
+    assert np.allclose(F.sum(0), 0, atol=1e-13)
+    if axis in (T.ZThenX, T.Bisector):
+        # the mapped forces reproduce the applied torque
+        assert np.allclose(np.cross(pos - pos[0], F).sum(0), tq[0], atol=1e-12)
+
+
+def test_reference_zonly_finite_difference_check_does_not_hold():
+    """The reference's own live ZOnly assertion (TestReferenceTholeDipoleForce.cpp:743-758)
+    compares dE along the force with |F|*delta; with the reference's frame pattern the forces are
+    not -grad E, so that assertion does not hold for the reference arithmetic either."""
+    rs = ref_systems.zonly()
+    base = oracle.force_energy(rs.force, rs.positions)
+    F = base["forces"]
+    norm = np.sqrt((F ** 2).sum())
+    delta = 1e-3
+    step = 0.5 * delta / norm
+    e2 = oracle.force_energy(rs.force, rs.positions - F * step)["energy"]
+    e3 = oracle.force_energy(rs.force, rs.positions + F * step)["energy"]
+    assert abs(e2 - (e3 + norm * delta)) > 1e-3 * max(abs(e2), 1.0)

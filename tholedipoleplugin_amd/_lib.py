@@ -1,0 +1,226 @@
+"""ctypes binding of libtholedipole_hip.so (C ABI declared in include/tholedipole_hip.h).
+
+There is no fallback: if the library is missing or cannot be loaded, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libtholedipole_hip.so")
+
+TDP_OK = 0
+TDP_ERR_INVALID_ARGUMENT = 1
+TDP_ERR_HIP = 2
+TDP_ERR_NOT_PME = 3
+TDP_ERR_BOX_TOO_SMALL = 4
+TDP_ERR_PARTICLE_COUNT = 5
+TDP_ERR_NO_DEVICE = 6
+
+PRECISION = {"double": 0, "mixed": 1}
+SOLVER = {"jacobi": 0, "pcg": 1}
+DIPOLES_INDUCED, DIPOLES_LAB_PERMANENT, DIPOLES_TOTAL = 0, 1, 2
+
+# Every symbol include/tholedipole_hip.h declares.
+EXPORTED_SYMBOLS = [
+    "tdp_create", "tdp_destroy", "tdp_last_error", "tdp_last_create_error", "tdp_copy_parameters",
+    "tdp_execute", "tdp_execute_device", "tdp_get_dipoles", "tdp_get_electrostatic_potential",
+    "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
+]
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+class ForceDesc(C.Structure):
+    _fields_ = [("num_particles", C.c_int32), ("charges", _dp), ("dipoles", _dp), ("polarizabilities", _dp),
+                ("thole", _dp), ("axis_types", _ip), ("atom_z", _ip), ("atom_x", _ip), ("atom_y", _ip),
+                ("covalent_offsets", _ip), ("covalent_atoms", _ip), ("nonbonded_method", C.c_int32),
+                ("polarization_type", C.c_int32), ("cutoff", C.c_double), ("ewald_error_tolerance", C.c_double),
+                ("pme_alpha", C.c_double), ("pme_grid", C.c_int32 * 3), ("mutual_max_iterations", C.c_int32),
+                ("mutual_target_epsilon", C.c_double), ("extrapolation_coefficients", _dp),
+                ("num_extrapolation_coefficients", C.c_int32), ("default_box", C.c_double * 9)]
+
+
+class Options(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("device", C.c_int32)]
+
+
+class SolverStats(C.Structure):
+    _fields_ = [("iterations", C.c_int32), ("epsilon", C.c_double), ("converged", C.c_int32),
+                ("matvec_ms", C.c_double), ("matvec_launches", C.c_int32)]
+
+
+_lib = None
+
+
+class TdpError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__(message)
+        self.code = code
+
+
+def load():
+    """Loads the HIP library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libtholedipole_hip.so is not built (run __graft_entry__.build() or make -C "
+                           "tholedipoleplugin_amd/csrc); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    ctx_p = C.c_void_p
+    L.tdp_create.argtypes = [C.POINTER(ForceDesc), C.POINTER(Options), C.POINTER(ctx_p)]
+    L.tdp_destroy.argtypes = [ctx_p]
+    L.tdp_destroy.restype = None
+    L.tdp_last_error.argtypes = [ctx_p]
+    L.tdp_last_error.restype = C.c_char_p
+    L.tdp_last_create_error.argtypes = []
+    L.tdp_last_create_error.restype = C.c_char_p
+    L.tdp_copy_parameters.argtypes = [ctx_p, C.POINTER(ForceDesc)]
+    L.tdp_execute.argtypes = [ctx_p, _dp, _dp, C.c_int32, C.c_int32, _dp, _dp]
+    L.tdp_execute_device.argtypes = [ctx_p, C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tdp_get_dipoles.argtypes = [ctx_p, _dp, _dp, C.c_int32, _dp]
+    L.tdp_get_electrostatic_potential.argtypes = [ctx_p, _dp, _dp, _dp, C.c_int32, _dp]
+    L.tdp_get_system_multipole_moments.argtypes = [ctx_p, _dp, _dp, _dp, _dp]
+    L.tdp_get_pme_parameters.argtypes = [ctx_p, _dp, _ip, _ip, _ip]
+    L.tdp_get_solver_stats.argtypes = [ctx_p, C.POINTER(SolverStats)]
+    L.tdp_set_timing.argtypes = [ctx_p, C.c_int32]
+    for name in EXPORTED_SYMBOLS:
+        if name not in ("tdp_destroy", "tdp_last_error", "tdp_last_create_error"):
+            getattr(L, name).restype = C.c_int
+    _lib = L
+    return L
+
+
+def _ptr(a, t=_dp):
+    return a.ctypes.data_as(t)
+
+
+class PackedForce:
+    """tdp_force_desc over numpy arrays owned by this object."""
+
+    def __init__(self, force, default_box):
+        a = force.packed_arrays()
+        self.arrays = a
+        self.extrap = np.asarray(force.getExtrapolationCoefficients(), np.float64)
+        d = ForceDesc()
+        d.num_particles = force.getNumParticles()
+        d.charges = _ptr(a["charge"]); d.dipoles = _ptr(a["dipole"]); d.polarizabilities = _ptr(a["polarity"])
+        d.thole = _ptr(a["thole"])
+        d.axis_types = _ptr(a["axis_type"], _ip); d.atom_z = _ptr(a["atom_z"], _ip)
+        d.atom_x = _ptr(a["atom_x"], _ip); d.atom_y = _ptr(a["atom_y"], _ip)
+        d.covalent_offsets = _ptr(a["cov_offsets"], _ip); d.covalent_atoms = _ptr(a["cov_atoms"], _ip)
+        d.nonbonded_method = force.getNonbondedMethod()
+        d.polarization_type = force.getPolarizationType()
+        d.cutoff = force.getCutoffDistance()
+        d.ewald_error_tolerance = force.getEwaldErrorTolerance()
+        alpha, nx, ny, nz = force.getPMEParameters()
+        d.pme_alpha = alpha
+        d.pme_grid[0], d.pme_grid[1], d.pme_grid[2] = nx, ny, nz
+        d.mutual_max_iterations = force.getMutualInducedMaxIterations()
+        d.mutual_target_epsilon = force.getMutualInducedTargetEpsilon()
+        d.extrapolation_coefficients = _ptr(self.extrap) if self.extrap.size else None
+        d.num_extrapolation_coefficients = int(self.extrap.size)
+        b = np.asarray(default_box, np.float64).reshape(9)
+        for k in range(9):
+            d.default_box[k] = b[k]
+        self.desc = d
+
+
+class HipContextHandle:
+    """Owns one tdp_context; methods raise TdpError on a nonzero status."""
+
+    def __init__(self, force, default_box, precision="double", solver="jacobi", device=0):
+        self.L = load()
+        self.n = force.getNumParticles()
+        pf = PackedForce(force, default_box)
+        opts = Options(PRECISION[precision], SOLVER[solver], int(device))
+        h = C.c_void_p()
+        rc = self.L.tdp_create(C.byref(pf.desc), C.byref(opts), C.byref(h))
+        if rc != TDP_OK:
+            raise TdpError(rc, self.L.tdp_last_create_error().decode())
+        self.h = h
+        self.precision = precision
+        self.solver = solver
+
+    def _check(self, rc):
+        if rc != TDP_OK:
+            raise TdpError(rc, self.L.tdp_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.tdp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _box(box):
+        if box is None:
+            return None
+        return np.ascontiguousarray(np.asarray(box, np.float64).reshape(9))
+
+    def copy_parameters(self, force, default_box):
+        pf = PackedForce(force, default_box)
+        self._check(self.L.tdp_copy_parameters(self.h, C.byref(pf.desc)))
+
+    def execute(self, positions, box, forces=None):
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        f = np.zeros(pos.size) if forces is None else forces
+        e = C.c_double(0.0)
+        b = self._box(box)
+        self._check(self.L.tdp_execute(self.h, _ptr(pos), None if b is None else _ptr(b), 1, 1, _ptr(f),
+                                       C.byref(e)))
+        return e.value, f.reshape(-1, 3)
+
+    def execute_device(self, d_positions_ptr, box, d_forces_ptr, d_energy_ptr, stream_ptr):
+        b = self._box(box)
+        self._check(self.L.tdp_execute_device(self.h, C.c_void_p(d_positions_ptr), None if b is None else _ptr(b),
+                                              C.c_void_p(d_forces_ptr), C.c_void_p(d_energy_ptr),
+                                              C.c_void_p(stream_ptr)))
+
+    def dipoles(self, positions, box, kind):
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        out = np.zeros(pos.size)
+        b = self._box(box)
+        self._check(self.L.tdp_get_dipoles(self.h, _ptr(pos), None if b is None else _ptr(b), kind, _ptr(out)))
+        return out.reshape(-1, 3)
+
+    def potential(self, positions, box, grid):
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        g = np.ascontiguousarray(grid, np.float64).reshape(-1)
+        out = np.zeros(g.size // 3)
+        b = self._box(box)
+        self._check(self.L.tdp_get_electrostatic_potential(self.h, _ptr(pos), None if b is None else _ptr(b),
+                                                           _ptr(g), int(out.size), _ptr(out)))
+        return out
+
+    def multipole_moments(self, positions, box, masses):
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        m = np.ascontiguousarray(masses, np.float64)
+        out = np.zeros(13)
+        b = self._box(box)
+        self._check(self.L.tdp_get_system_multipole_moments(self.h, _ptr(pos), None if b is None else _ptr(b),
+                                                            _ptr(m), _ptr(out)))
+        return out
+
+    def pme_parameters(self):
+        a = C.c_double(); nx = C.c_int32(); ny = C.c_int32(); nz = C.c_int32()
+        self._check(self.L.tdp_get_pme_parameters(self.h, C.byref(a), C.byref(nx), C.byref(ny), C.byref(nz)))
+        return a.value, nx.value, ny.value, nz.value
+
+    def solver_stats(self):
+        st = SolverStats()
+        self._check(self.L.tdp_get_solver_stats(self.h, C.byref(st)))
+        return dict(iterations=st.iterations, epsilon=st.epsilon, converged=st.converged,
+                    matvec_ms=st.matvec_ms, matvec_launches=st.matvec_launches)
+
+    def set_timing(self, enable=True):
+        self._check(self.L.tdp_set_timing(self.h, 1 if enable else 0))

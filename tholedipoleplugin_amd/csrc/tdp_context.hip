@@ -1,0 +1,830 @@
+// tdp_context.hip -- the C ABI of libtholedipole_hip.so (include/tholedipole_hip.h).
+//
+// A tdp_context is the device-side state of one CalcTholeDipoleForceKernel
+// (platforms/reference/src/ReferenceTholeDipoleKernels.{h,cpp}): parameters uploaded once at
+// create time, the exclusion tiles derived from the covalent maps, and the work buffers of one
+// evaluation.  Every evaluation restarts the induced dipoles from alpha*E0, as the reference
+// does (a new ReferenceTholeDipoleForce per call, ReferenceTholeDipoleKernels.cpp:201-215).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/tholedipole_hip.h"
+#include "tdp_kernels.h"
+
+using namespace tdp;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace
+
+struct tdp_context {
+    // ---- parameters (host copies)
+    int n = 0, npad = 0, nblk = 0, nsplit = 1, tiles_per_split = 1;
+    int method = TDP_NO_CUTOFF, polarization = TDP_MUTUAL, precision = TDP_PRECISION_DOUBLE;
+    int solver = TDP_SOLVER_JACOBI, device = 0;
+    double cutoff = 1.0, ewald_tol = 1e-4, pme_alpha = 0.0;
+    int grid[3] = {0, 0, 0};
+    int max_iter = 60;
+    double target_eps = 1e-5;
+    std::vector<double> extrap;
+    double default_box[9] = {0};
+    std::vector<double> charge, dmol, polarity, thole;
+    std::vector<int> axis_type, atom_z, atom_x, atom_y;
+    // ---- derived host tables
+    std::vector<int> exc_start, exc_J;
+    std::vector<unsigned long long> exc_mask;
+    std::vector<int> goff, gidx;
+    // ---- device state
+    hipStream_t stream = nullptr;
+    std::vector<void*> allocations;
+    double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
+            *q = nullptr, *part = nullptr, *part_f = nullptr, *part_t = nullptr, *torque = nullptr,
+            *contrib = nullptr, *hist = nullptr;
+    float4 *posa32 = nullptr, *vec32 = nullptr;
+    double *d_pos3 = nullptr, *d_frc = nullptr, *d_alpha = nullptr, *d_inva = nullptr, *d_thole = nullptr,
+           *d_charge = nullptr, *d_dmol = nullptr, *d_red = nullptr, *d_scalars = nullptr, *d_part_e = nullptr,
+           *d_coef = nullptr, *d_energy = nullptr, *d_mass = nullptr, *d_out = nullptr;
+    float* d_inva32 = nullptr;
+    int4* d_axis = nullptr;
+    int *d_exc_start = nullptr, *d_exc_J = nullptr, *d_goff = nullptr, *d_gidx = nullptr;
+    unsigned long long* d_exc_mask = nullptr;
+    int hist_order = 0;
+    size_t out_bytes = 0;
+    // ---- results of the last solve
+    int iterations = 0;
+    double epsilon = 1.0e50;
+    int converged = 0;
+    // ---- timing
+    bool timing = false;
+    double matvec_ms = 0.0;
+    int matvec_launches = 0;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+    std::string err;
+};
+
+namespace {
+
+int fail(tdp_context* c, int code, const std::string& msg) {
+    if (c) c->err = msg; else g_create_error = msg;
+    return code;
+}
+
+#define TDP_HIP(ctx, call)                                                                              \
+    do {                                                                                                \
+        hipError_t e_ = (call);                                                                         \
+        if (e_ != hipSuccess)                                                                           \
+            return fail(ctx, TDP_ERR_HIP, std::string("HIP error in ") + #call + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+template <typename T>
+int dalloc(tdp_context* c, T** ptr, size_t count) {
+    void* p = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    e = hipMemset(p, 0, bytes);
+    if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipMemset failed: ") + hipGetErrorString(e));
+    c->allocations.push_back(p);
+    *ptr = reinterpret_cast<T*>(p);
+    return TDP_OK;
+}
+
+const double kMScale[5] = {0.0, 0.0, 0.0, 0.5, 1.0};  // ReferenceTholeDipoleForce.cpp:37-41
+const double kIScale[5] = {0.0, 1.0, 1.0, 1.0, 1.0};  // ReferenceTholeDipoleForce.cpp:43-47
+
+// Validation that TholeDipoleForceImpl::initialize performs (TholeDipoleForceImpl.cpp:52-108),
+// plus the inputs the reference math would dereference out of bounds.
+int validate(tdp_context* c, const tdp_force_desc* d) {
+    int n = d->num_particles;
+    if (n < 0) return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: negative particle count");
+    if (d->nonbonded_method != TDP_NO_CUTOFF && d->nonbonded_method != TDP_PME)
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: Illegal value for nonbonded method");
+    if (d->polarization_type < 0 || d->polarization_type > 2)
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, "Polarization type not recognized.");
+    if (d->nonbonded_method == TDP_PME) {
+        const double* b = d->default_box;
+        if (d->cutoff > 0.5 * b[0] || d->cutoff > 0.5 * b[4] || d->cutoff > 0.5 * b[8])
+            return fail(c, TDP_ERR_INVALID_ARGUMENT,
+                        "TholeDipoleForce: The cutoff distance cannot be greater than half the periodic box size.");
+    }
+    for (int i = 0; i < n; i++) {
+        int at = d->axis_types[i], z = d->atom_z[i], x = d->atom_x[i], y = d->atom_y[i];
+        if (at < 0 || at > 5) {
+            char buf[320];
+            snprintf(buf, sizeof buf,
+                     "TholeDipoleForce: axis type=%d not currently handled - only axisTypes[ 0, 1, 2, 3, 4, 5] "
+                     "(ZThenX, Bisector, Z-Bisect, ThreeFold, ZOnly, NoAxisType) currently handled.", at);
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, buf);
+        }
+        if (at != 5 && (z < 0 || z >= n))
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: invalid z axis particle: " + std::to_string(z));
+        if (at != 5 && at != 4 && (x < 0 || x >= n))
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: invalid x axis particle: " + std::to_string(x));
+        if ((at == 2 || at == 3) && (y < 0 || y >= n))
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: invalid y axis particle: " + std::to_string(y));
+        // The reference also dereferences these (applyRotationMatrix :556-568 for any axis type
+        // with Z>=0, mapTorqueToForce :1104-1108 reads X even for ZOnly); out-of-range values
+        // would read outside its particle array, so they are rejected here.
+        if (z >= n || x >= n || y >= n)
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: axis particle index out of range for particle " +
+                                                         std::to_string(i));
+        if (z >= 0 && z != i && at != 4 && x < 0)
+            return fail(c, TDP_ERR_INVALID_ARGUMENT,
+                        "TholeDipoleForce: particle " + std::to_string(i) + " has a z axis particle but no x axis particle");
+        if (at == 4 && x < 0)
+            return fail(c, TDP_ERR_INVALID_ARGUMENT,
+                        "TholeDipoleForce: ZOnly particle " + std::to_string(i) + " needs a valid x axis particle index");
+        if (y > -1 && at == 0 && (z < 0 || x < 0))
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "TholeDipoleForce: invalid axis particles for particle " + std::to_string(i));
+    }
+    for (int i = 0; i < n; i++)
+        for (int t = 0; t < 2; t++) {
+            int len = d->covalent_offsets[4 * i + t + 1] - d->covalent_offsets[4 * i + t];
+            if (len > 5)
+                return fail(c, TDP_ERR_INVALID_ARGUMENT,
+                            "TholeDipoleForce: covalent list " + std::to_string(t) + " of particle " + std::to_string(i) +
+                                " has " + std::to_string(len) + " entries; the scale tables hold 5");
+        }
+    return TDP_OK;
+}
+
+void copy_particles(tdp_context* c, const tdp_force_desc* d) {
+    int n = d->num_particles;
+    c->charge.assign(d->charges, d->charges + n);
+    c->dmol.assign(d->dipoles, d->dipoles + 3 * n);
+    c->polarity.assign(d->polarizabilities, d->polarizabilities + n);
+    c->thole.assign(d->thole, d->thole + n);
+    c->axis_type.assign(d->axis_types, d->axis_types + n);
+    c->atom_z.assign(d->atom_z, d->atom_z + n);
+    c->atom_x.assign(d->atom_x, d->atom_x + n);
+    c->atom_y.assign(d->atom_y, d->atom_y + n);
+}
+
+// setupScaleMaps/getScaleFactor (ReferenceTholeDipoleForce.cpp:609-644): pair (i<j) takes its
+// scales from particle i's own lists only: mScale from list 0 at the position of j, iScale from
+// list 1, the last occurrence of a duplicated atom winning.  Pairs whose scales are both 1 are
+// plain pairs; the rest, and the self pairs, become bits in the exclusion tiles.
+void build_exclusion_tiles(tdp_context* c, const tdp_force_desc* d) {
+    const int n = c->n, nblk = c->nblk;
+    std::unordered_map<long long, std::pair<double, double>> pairs;
+    auto key = [n](int i, int j) { return (long long)i * n + j; };
+    for (int i = 0; i < n; i++) {
+        for (int t = 0; t < 2; t++) {
+            int b = d->covalent_offsets[4 * i + t], e = d->covalent_offsets[4 * i + t + 1];
+            for (int p = b; p < e; p++) {
+                int a = d->covalent_atoms[p];
+                if (a <= i || a >= n) continue;  // never looked up by the reference
+                auto it = pairs.find(key(i, a));
+                if (it == pairs.end()) it = pairs.emplace(key(i, a), std::make_pair(1.0, 1.0)).first;
+                if (t == 0) it->second.first = kMScale[p - b];
+                else it->second.second = kIScale[p - b];
+            }
+        }
+    }
+    // tiles: (I,J) -> row masks
+    std::vector<std::unordered_map<int, int>> tile_of(nblk);
+    std::vector<std::vector<int>> tiles_J(nblk);
+    std::vector<unsigned long long> masks;  // [tile][3][64]
+    std::vector<std::pair<int, int>> tile_ij;
+    auto tile = [&](int I, int J) {
+        auto it = tile_of[I].find(J);
+        if (it != tile_of[I].end()) return it->second;
+        int id = (int)tile_ij.size();
+        tile_of[I][J] = id;
+        tile_ij.push_back({I, J});
+        masks.resize(masks.size() + 3 * 64, 0ull);
+        return id;
+    };
+    auto setbits = [&](int i, int j, double m, double is) {
+        int t = tile(i / 64, j / 64);
+        int row = i % 64, col = j % 64;
+        unsigned long long bit = 1ull << col;
+        if (m == 0.0) masks[((size_t)t * 3 + 0) * 64 + row] |= bit;
+        if (m == 0.5) masks[((size_t)t * 3 + 1) * 64 + row] |= bit;
+        if (is == 0.0) masks[((size_t)t * 3 + 2) * 64 + row] |= bit;
+    };
+    for (int i = 0; i < n; i++) setbits(i, i, 0.0, 0.0);
+    for (auto& kv : pairs) {
+        double m = kv.second.first, is = kv.second.second;
+        if (m == 1.0 && is == 1.0) continue;
+        int i = (int)(kv.first / n), j = (int)(kv.first % n);
+        setbits(i, j, m, is);
+        setbits(j, i, m, is);
+    }
+    // order by (I, J)
+    std::vector<int> order(tile_ij.size());
+    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
+    std::sort(order.begin(), order.end(), [&](int a, int b) { return tile_ij[a] < tile_ij[b]; });
+    c->exc_start.assign(nblk + 1, 0);
+    c->exc_J.resize(order.size());
+    c->exc_mask.resize(masks.size());
+    for (size_t k = 0; k < order.size(); k++) {
+        int t = order[k];
+        c->exc_start[tile_ij[t].first + 1]++;
+        c->exc_J[k] = tile_ij[t].second;
+        std::memcpy(&c->exc_mask[k * 3 * 64], &masks[(size_t)t * 3 * 64], 3 * 64 * sizeof(unsigned long long));
+    }
+    for (int I = 0; I < nblk; I++) c->exc_start[I + 1] += c->exc_start[I];
+}
+
+// Gather lists for mapTorqueToForce (ReferenceTholeDipoleForce.cpp:1092-1115): each particle's
+// contributions (to itself and its axis atoms) are summed in the reference's order: particle
+// index ascending, and within one particle U, V, W, then I.
+void build_torque_gather(tdp_context* c) {
+    const int n = c->n;
+    struct Entry { int atom, ii, role, slot; };
+    std::vector<Entry> es;
+    for (int ii = 0; ii < n; ii++) {
+        int at = c->axis_type[ii];
+        if (at == 5) continue;
+        es.push_back({c->atom_z[ii], ii, 0, 4 * ii + 1});
+        if (at != 4) es.push_back({c->atom_x[ii], ii, 1, 4 * ii + 2});
+        if ((at == 2 || at == 3) && c->atom_y[ii] > -1) es.push_back({c->atom_y[ii], ii, 2, 4 * ii + 3});
+        es.push_back({ii, ii, 3, 4 * ii + 0});
+    }
+    std::stable_sort(es.begin(), es.end(), [](const Entry& a, const Entry& b) {
+        if (a.atom != b.atom) return a.atom < b.atom;
+        if (a.ii != b.ii) return a.ii < b.ii;
+        return a.role < b.role;
+    });
+    c->goff.assign(n + 1, 0);
+    c->gidx.resize(es.size());
+    for (size_t k = 0; k < es.size(); k++) {
+        c->goff[es[k].atom + 1]++;
+        c->gidx[k] = es[k].slot;
+    }
+    for (int a = 0; a < n; a++) c->goff[a + 1] += c->goff[a];
+}
+
+int upload_particles(tdp_context* c) {
+    const int n = c->n, npad = c->npad;
+    std::vector<double> alpha(npad, 0.0), inva(npad, 0.0), thole(npad, 0.0), charge(npad, 0.0);
+    std::vector<float> inva32(npad, 0.f);
+    std::vector<int4> axis(std::max(n, 1));
+    for (int i = 0; i < n; i++) {
+        alpha[i] = c->polarity[i];
+        thole[i] = c->thole[i];
+        charge[i] = c->charge[i];
+        inva[i] = c->thole[i] != 0.0 ? 1.0 / c->thole[i] : 0.0;
+        inva32[i] = (float)inva[i];
+        axis[i] = make_int4(c->axis_type[i], c->atom_z[i], c->atom_x[i], c->atom_y[i]);
+    }
+    TDP_HIP(c, hipMemcpy(c->d_alpha, alpha.data(), npad * sizeof(double), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_inva, inva.data(), npad * sizeof(double), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_inva32, inva32.data(), npad * sizeof(float), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_thole, thole.data(), npad * sizeof(double), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_charge, charge.data(), npad * sizeof(double), hipMemcpyHostToDevice));
+    if (n > 0) {
+        TDP_HIP(c, hipMemcpy(c->d_dmol, c->dmol.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
+        TDP_HIP(c, hipMemcpy(c->d_axis, axis.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+    }
+    return TDP_OK;
+}
+
+PairLaunch pair_launch(const tdp_context* c, const double* box9) {
+    PairLaunch L;
+    L.n = c->n;
+    L.npad = c->npad;
+    L.nblk = c->nblk;
+    L.nsplit = c->nsplit;
+    L.tiles_per_split = c->tiles_per_split;
+    L.pbc = c->method == TDP_PME;
+    L.box[0] = box9 ? box9[0] : 0.0;
+    L.box[1] = box9 ? box9[4] : 0.0;
+    L.box[2] = box9 ? box9[8] : 0.0;
+    L.exc_start = c->d_exc_start;
+    L.exc_J = c->d_exc_J;
+    L.exc_mask = c->d_exc_mask;
+    return L;
+}
+
+hipEvent_t take_event(tdp_context* c) {
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    hipEventCreate(&e);
+    return e;
+}
+
+void harvest_timing(tdp_context* c) {
+    for (auto& pr : c->pending) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
+            c->matvec_ms += ms;
+            c->matvec_launches++;
+        }
+        c->ev_pool.push_back(pr.first);
+        c->ev_pool.push_back(pr.second);
+    }
+    c->pending.clear();
+}
+
+// T * v for the solver: v is mu (double4) or its FP32 copy in vec32.
+void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const double* scalars) {
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        e0 = take_event(c);
+        e1 = take_event(c);
+        hipEventRecord(e0, c->stream);
+    }
+    if (c->precision == TDP_PRECISION_MIXED)
+        launch_induced_field32(c->stream, L, c->posa32, c->vec32, c->d_inva32, scalars, c->part);
+    else
+        launch_induced_field64(c->stream, L, c->posa, v64, c->d_inva, scalars, c->part);
+    if (c->timing) {
+        hipEventRecord(e1, c->stream);
+        c->pending.push_back({e0, e1});
+    }
+}
+
+int check_box(tdp_context* c, const double* box) {
+    if (c->method != TDP_PME) return TDP_OK;
+    if (!box) return fail(c, TDP_ERR_INVALID_ARGUMENT, "a periodic box is required for PME");
+    double minAllowed = 1.999999 * c->cutoff;  // ReferenceTholeDipoleKernels.cpp:170-173
+    if (box[0] < minAllowed || box[4] < minAllowed || box[8] < minAllowed)
+        return fail(c, TDP_ERR_BOX_TOO_SMALL, "The periodic box size has decreased to less than twice the nonbonded cutoff.");
+    return TDP_OK;
+}
+
+// setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764) from
+// positions already in d_pos3.
+int setup_and_solve(tdp_context* c, const double* box) {
+    if (c->n == 0) {
+        c->iterations = 0;
+        c->converged = 1;
+        c->epsilon = c->polarization == TDP_DIRECT ? 1.0e50 : 0.0;
+        return TDP_OK;
+    }
+    hipStream_t s = c->stream;
+    const int n = c->n, npad = c->npad;
+    PairLaunch L = pair_launch(c, box);
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    launch_load_positions(s, n, npad, c->d_pos3, c->d_thole, c->posa, mixed ? c->posa32 : nullptr);
+    launch_rotate(s, n, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
+    launch_fixed_field(s, L, c->posa, c->qd, c->d_inva, c->part);
+    launch_field_init(s, n, npad, c->nsplit, c->part, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
+    const int nred = red_blocks(npad);
+
+    if (c->polarization == TDP_DIRECT) {
+        c->iterations = 0;
+        c->converged = 1;
+        c->epsilon = 1.0e50;  // the reference leaves its initial value (:24, :749-753)
+        return TDP_OK;
+    }
+    if (c->polarization == TDP_EXTRAPOLATED) {
+        // convergeInducedDipolesByExtrapolation (:853-909)
+        static const double opt4[4] = {-0.154, 0.017, 0.658, 0.474};
+        std::vector<double> coef = c->extrap;
+        if (coef.empty()) coef.assign(opt4, opt4 + 4);
+        int order = (int)coef.size();
+        if (order > c->hist_order) return fail(c, TDP_ERR_INVALID_ARGUMENT, "extrapolation history not allocated");
+        TDP_HIP(c, hipMemcpyAsync(c->d_coef, coef.data(), order * sizeof(double), hipMemcpyHostToDevice, s));
+        TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
+        for (int o = 1; o < order; o++) {
+            matvec(c, L, c->mu, nullptr);
+            launch_extrap_step(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
+                               c->hist + (size_t)o * npad);
+        }
+        launch_extrap_combine(s, n, npad, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
+        matvec(c, L, c->mu, nullptr);
+        launch_extrap_error(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
+        double sc[SC_COUNT];
+        TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
+        TDP_HIP(c, hipStreamSynchronize(s));
+        c->epsilon = sc[SC_EPS];
+        c->converged = 1;
+        c->iterations = order;
+        if (c->timing) harvest_timing(c);
+        return TDP_OK;
+    }
+    // Mutual
+    double sc0[SC_COUNT] = {0};
+    sc0[SC_MAXIT] = c->max_iter;
+    sc0[SC_TARGET] = c->target_eps;
+    sc0[SC_EPS] = 1.0;
+    const int chunk = n > 20000 ? 4 : 16;  // iterations enqueued between host checks
+    if (c->solver == TDP_SOLVER_JACOBI) {
+        // Loop condition of convergeInducedDipolesByDIIS (:829-832) with epsilon starting at 1.0.
+        sc0[SC_DONE] = (c->max_iter <= 0 || !(1.0 > c->target_eps)) ? 1.0 : 0.0;
+        TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc0, sizeof sc0, hipMemcpyHostToDevice, s));
+        double sc[SC_COUNT];
+        for (;;) {
+            for (int k = 0; k < chunk; k++) {
+                matvec(c, L, c->mu, c->d_scalars);
+                launch_jacobi_update(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu,
+                                     mixed ? c->vec32 : nullptr, c->d_scalars, c->d_red);
+            }
+            TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
+            TDP_HIP(c, hipStreamSynchronize(s));
+            if (sc[SC_DONE] != 0.0) break;
+        }
+        c->iterations = (int)sc[SC_ITER];
+        c->epsilon = sc[SC_EPS];
+        c->converged = c->epsilon < c->target_eps ? 1 : 0;
+        if (c->iterations == 0) c->epsilon = 1.0;
+    } else {
+        TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc0, sizeof sc0, hipMemcpyHostToDevice, s));
+        // x = mu; r = E0 - x/alpha + T x; z = alpha r; p = z
+        matvec(c, L, c->mu, nullptr);
+        launch_pcg_init(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu, c->r, c->p,
+                        mixed ? c->vec32 : nullptr, c->d_red, c->d_scalars);
+        double sc[SC_COUNT];
+        TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
+        TDP_HIP(c, hipStreamSynchronize(s));
+        while (sc[SC_DONE] == 0.0) {
+            for (int k = 0; k < chunk; k++) {
+                matvec(c, L, c->p, c->d_scalars);
+                launch_pcg_iteration_tail(s, n, npad, c->nsplit, c->part, c->d_alpha, c->mu, c->r, c->z, c->p,
+                                          mixed ? c->vec32 : nullptr, c->q, c->d_red, c->d_scalars);
+            }
+            TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
+            TDP_HIP(c, hipStreamSynchronize(s));
+        }
+        c->iterations = (int)sc[SC_ITER];
+        c->epsilon = sc[SC_EPS];
+        c->converged = c->epsilon < c->target_eps ? 1 : 0;
+    }
+    if (c->timing) harvest_timing(c);
+    return TDP_OK;
+}
+
+// calculateElectrostatic + mapTorqueToForce (ReferenceTholeDipoleForce.cpp:118-124) into d_frc
+// and d_energy.
+void forces_and_energy(tdp_context* c, const double* box) {
+    hipStream_t s = c->stream;
+    if (c->n == 0) {
+        hipMemsetAsync(c->d_energy, 0, sizeof(double), s);
+        return;
+    }
+    const int n = c->n, npad = c->npad;
+    PairLaunch L = pair_launch(c, box);
+    launch_electrostatic(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->part_f, c->part_t,
+                         c->d_part_e);
+    launch_reduce_forces(s, n, npad, c->nsplit, c->part_f, c->part_t, c->d_frc, c->torque);
+    launch_reduce_energy(s, c->nsplit * c->nblk, c->d_part_e, c->d_energy);
+    if (!c->gidx.empty()) {
+        launch_map_torque(s, n, c->posa, c->d_axis, c->torque, c->contrib);
+        launch_gather_torque(s, n, c->d_goff, c->d_gidx, c->contrib, c->d_frc);
+    }
+}
+
+int ensure_out(tdp_context* c, size_t bytes) {
+    if (bytes <= c->out_bytes) return TDP_OK;
+    if (c->d_out) {
+        hipFree(c->d_out);
+        auto it = std::find(c->allocations.begin(), c->allocations.end(), (void*)c->d_out);
+        if (it != c->allocations.end()) c->allocations.erase(it);
+        c->d_out = nullptr;
+    }
+    int rc = dalloc(c, &c->d_out, bytes / sizeof(double) + 1);
+    if (rc) return rc;
+    c->out_bytes = bytes;
+    return TDP_OK;
+}
+
+int upload_positions(tdp_context* c, const double* positions) {
+    if (c->n > 0)
+        TDP_HIP(c, hipMemcpyAsync(c->d_pos3, positions, 3 * c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return TDP_OK;
+}
+
+}  // namespace
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* tdp_last_create_error(void) { return g_create_error.c_str(); }
+
+const char* tdp_last_error(const tdp_context* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** out) {
+    if (!d || !out) return fail(nullptr, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc = validate(nullptr, d);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, TDP_ERR_NO_DEVICE, "no HIP device is available");
+    tdp_context* c = new tdp_context();
+    c->device = opts ? opts->device : 0;
+    c->precision = opts ? opts->precision : TDP_PRECISION_DOUBLE;
+    c->solver = opts ? opts->solver : TDP_SOLVER_JACOBI;
+    if (c->device < 0 || c->device >= ndev) {
+        delete c;
+        return fail(nullptr, TDP_ERR_NO_DEVICE, "invalid device ordinal");
+    }
+    auto bail = [&](int code) {
+        g_create_error = c->err;
+        tdp_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) {
+        c->err = "hipSetDevice failed";
+        return bail(TDP_ERR_HIP);
+    }
+    c->n = d->num_particles;
+    c->npad = std::max(64, (c->n + 63) / 64 * 64);
+    c->nblk = (c->n + 63) / 64;
+    // Split the j range so that the pair kernels launch >= ~2048 workgroups (8 per CU).
+    int nb = std::max(c->nblk, 1);
+    int want = std::max(1, std::min(nb, (2048 + nb - 1) / nb));
+    c->tiles_per_split = (nb + want - 1) / want;
+    c->nsplit = (nb + c->tiles_per_split - 1) / c->tiles_per_split;
+    c->method = d->nonbonded_method;
+    c->polarization = d->polarization_type;
+    c->cutoff = d->cutoff;
+    c->ewald_tol = d->ewald_error_tolerance;
+    c->pme_alpha = d->pme_alpha;
+    for (int k = 0; k < 3; k++) c->grid[k] = d->pme_grid[k];
+    std::memcpy(c->default_box, d->default_box, sizeof c->default_box);
+    // Solver settings are read only for the active polarization type (ReferenceTholeDipoleKernels.cpp:127-134).
+    if (c->polarization == TDP_MUTUAL) {
+        c->max_iter = d->mutual_max_iterations;
+        c->target_eps = d->mutual_target_epsilon;
+    } else if (c->polarization == TDP_EXTRAPOLATED) {
+        if (d->num_extrapolation_coefficients > 0)
+            c->extrap.assign(d->extrapolation_coefficients,
+                             d->extrapolation_coefficients + d->num_extrapolation_coefficients);
+    }
+    copy_particles(c, d);
+    build_exclusion_tiles(c, d);
+    build_torque_gather(c);
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(TDP_ERR_HIP);
+    }
+    const int npad = c->npad, ns = c->nsplit;
+    const size_t ntiles = c->exc_J.size();
+    if ((rc = dalloc(c, &c->posa, npad)) || (rc = dalloc(c, &c->qd, npad)) || (rc = dalloc(c, &c->mu, npad)) ||
+        (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
+        (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||
+        (rc = dalloc(c, &c->part, (size_t)ns * npad)) || (rc = dalloc(c, &c->part_f, (size_t)ns * npad)) ||
+        (rc = dalloc(c, &c->part_t, (size_t)ns * npad)) || (rc = dalloc(c, &c->torque, npad)) ||
+        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || (rc = dalloc(c, &c->posa32, npad)) ||
+        (rc = dalloc(c, &c->vec32, npad)) || (rc = dalloc(c, &c->d_pos3, 3 * (size_t)npad)) ||
+        (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_alpha, npad)) ||
+        (rc = dalloc(c, &c->d_inva, npad)) || (rc = dalloc(c, &c->d_inva32, npad)) ||
+        (rc = dalloc(c, &c->d_thole, npad)) || (rc = dalloc(c, &c->d_charge, npad)) ||
+        (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)red_blocks(npad))) ||
+        (rc = dalloc(c, &c->d_scalars, SC_COUNT)) || (rc = dalloc(c, &c->d_part_e, (size_t)ns * c->nblk + 1)) ||
+        (rc = dalloc(c, &c->d_coef, 64)) || (rc = dalloc(c, &c->d_energy, 1)) || (rc = dalloc(c, &c->d_axis, npad)) ||
+        (rc = dalloc(c, &c->d_exc_start, c->nblk + 1)) || (rc = dalloc(c, &c->d_exc_J, ntiles)) ||
+        (rc = dalloc(c, &c->d_exc_mask, ntiles * 3 * 64)) || (rc = dalloc(c, &c->d_goff, c->n + 1)) ||
+        (rc = dalloc(c, &c->d_gidx, c->gidx.size())) || (rc = dalloc(c, &c->d_mass, npad)))
+        return bail(rc);
+    if (c->polarization == TDP_EXTRAPOLATED) {
+        c->hist_order = std::max(4, (int)c->extrap.size());
+        if ((rc = dalloc(c, &c->hist, (size_t)c->hist_order * npad))) return bail(rc);
+    }
+    if (hipMemcpy(c->d_exc_start, c->exc_start.data(), c->exc_start.size() * sizeof(int), hipMemcpyHostToDevice) !=
+            hipSuccess ||
+        (ntiles && hipMemcpy(c->d_exc_J, c->exc_J.data(), ntiles * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) ||
+        (ntiles && hipMemcpy(c->d_exc_mask, c->exc_mask.data(), c->exc_mask.size() * sizeof(unsigned long long),
+                             hipMemcpyHostToDevice) != hipSuccess) ||
+        hipMemcpy(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        (!c->gidx.empty() &&
+         hipMemcpy(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+        c->err = "hipMemcpy of the exclusion tables failed";
+        return bail(TDP_ERR_HIP);
+    }
+    if ((rc = upload_particles(c))) return bail(rc);
+    *out = c;
+    return TDP_OK;
+}
+
+void tdp_destroy(tdp_context* c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (auto& pr : c->pending) {
+        hipEventDestroy(pr.first);
+        hipEventDestroy(pr.second);
+    }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
+    for (void* p : c->allocations) hipFree(p);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+int tdp_copy_parameters(tdp_context* c, const tdp_force_desc* d) {
+    if (!c || !d) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (d->num_particles != c->n)
+        return fail(c, TDP_ERR_PARTICLE_COUNT, "updateParametersInContext: The number of particles has changed");
+    // Only the per-particle parameters are copied (ReferenceTholeDipoleKernels.cpp:317-339).
+    tdp_force_desc v = *d;
+    int rc = validate(c, &v);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    copy_particles(c, d);
+    build_torque_gather(c);
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->d_gidx) {
+        auto it = std::find(c->allocations.begin(), c->allocations.end(), (void*)c->d_gidx);
+        if (it != c->allocations.end()) c->allocations.erase(it);
+        hipFree(c->d_gidx);
+        c->d_gidx = nullptr;
+    }
+    if ((rc = dalloc(c, &c->d_gidx, c->gidx.size()))) return rc;
+    TDP_HIP(c, hipMemcpy(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!c->gidx.empty())
+        TDP_HIP(c, hipMemcpy(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice));
+    return upload_particles(c);
+}
+
+int tdp_execute(tdp_context* c, const double* positions, const double* box, int32_t include_forces,
+                int32_t include_energy, double* forces, double* energy) {
+    if (!c || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    if ((rc = upload_positions(c, positions))) return rc;
+    if ((rc = setup_and_solve(c, box))) return rc;
+    forces_and_energy(c, box);
+    const int n = c->n;
+    std::vector<double> f(3 * (size_t)n);
+    double e = 0.0;
+    if (n > 0) TDP_HIP(c, hipMemcpyAsync(f.data(), c->d_frc, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(&e, c->d_energy, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    if (forces && include_forces)
+        for (size_t k = 0; k < f.size(); k++) forces[k] += f[k];
+    if (energy) *energy = e;
+    (void)include_energy;  // both are always computed, as in the reference (ReferenceTholeDipoleKernels.cpp:201-215)
+    return TDP_OK;
+}
+
+int tdp_execute_device(tdp_context* c, const double* d_positions, const double* box, double* d_forces,
+                       double* d_energy, void* stream) {
+    if (!c) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null context");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    hipStream_t user = reinterpret_cast<hipStream_t>(stream);
+    // order the context's stream after the caller's work, run, then order the caller's stream after ours
+    hipEvent_t ev = take_event(c);
+    TDP_HIP(c, hipEventRecord(ev, user));
+    TDP_HIP(c, hipStreamWaitEvent(c->stream, ev, 0));
+    if (c->n > 0)
+        TDP_HIP(c, hipMemcpyAsync(c->d_pos3, d_positions, 3 * c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if ((rc = setup_and_solve(c, box))) {
+        c->ev_pool.push_back(ev);
+        return rc;
+    }
+    forces_and_energy(c, box);
+    if (d_forces && c->n > 0) launch_accumulate(c->stream, 3 * c->n, c->d_frc, d_forces);
+    if (d_energy) TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    TDP_HIP(c, hipEventRecord(ev, c->stream));
+    TDP_HIP(c, hipStreamWaitEvent(user, ev, 0));
+    c->ev_pool.push_back(ev);
+    return TDP_OK;
+}
+
+int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, int32_t kind, double* out) {
+    if (!c || !out || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (kind < 0 || kind > 2) return fail(c, TDP_ERR_INVALID_ARGUMENT, "unknown dipole kind");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    if ((rc = upload_positions(c, positions))) return rc;
+    if ((rc = setup_and_solve(c, box))) return rc;
+    const int n = c->n;
+    std::vector<double4> mu(n), qd(n);
+    if (n > 0) {
+        TDP_HIP(c, hipMemcpyAsync(mu.data(), c->mu, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        TDP_HIP(c, hipMemcpyAsync(qd.data(), c->qd, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    }
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    for (int i = 0; i < n; i++) {
+        double a[3] = {mu[i].x, mu[i].y, mu[i].z}, b[3] = {qd[i].x, qd[i].y, qd[i].z};
+        for (int k = 0; k < 3; k++)
+            out[3 * i + k] = kind == TDP_DIPOLES_INDUCED ? a[k] : (kind == TDP_DIPOLES_LAB_PERMANENT ? b[k] : b[k] + a[k]);
+    }
+    return TDP_OK;
+}
+
+int tdp_get_electrostatic_potential(tdp_context* c, const double* positions, const double* box, const double* grid,
+                                    int32_t num_grid, double* out) {
+    if (!c || (!positions && c->n > 0) || (num_grid > 0 && (!grid || !out)) || num_grid < 0)
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    if ((rc = upload_positions(c, positions))) return rc;
+    if ((rc = setup_and_solve(c, box))) return rc;
+    if (num_grid == 0) return TDP_OK;
+    size_t gb = (size_t)num_grid * 3 * sizeof(double);
+    if ((rc = ensure_out(c, gb + num_grid * sizeof(double)))) return rc;
+    double* d_grid = c->d_out;
+    double* d_pot = c->d_out + 3 * (size_t)num_grid;
+    TDP_HIP(c, hipMemcpyAsync(d_grid, grid, gb, hipMemcpyHostToDevice, c->stream));
+    double b3[3] = {box ? box[0] : 0.0, box ? box[4] : 0.0, box ? box[8] : 0.0};
+    launch_potential(c->stream, c->n, num_grid, d_grid, c->posa, c->qd, c->mu, c->method == TDP_PME, b3, d_pot);
+    TDP_HIP(c, hipMemcpyAsync(out, d_pot, num_grid * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    return TDP_OK;
+}
+
+int tdp_get_system_multipole_moments(tdp_context* c, const double* positions, const double* box, const double* masses,
+                                     double* out) {
+    if (!c || !out || !masses || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    if ((rc = upload_positions(c, positions))) return rc;
+    if ((rc = setup_and_solve(c, box))) return rc;
+    if (c->n > 0) TDP_HIP(c, hipMemcpyAsync(c->d_mass, masses, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = ensure_out(c, 16 * sizeof(double)))) return rc;
+    launch_moments(c->stream, c->n, c->posa, c->qd, c->mu, c->d_mass, c->d_out);
+    double raw[10];
+    TDP_HIP(c, hipMemcpyAsync(raw, c->d_out, sizeof raw, hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    // unit conversion of calculateTholeDipoleSystemMultipoleMoments (:1285-1312)
+    double xx = raw[4], xy = raw[5], xz = raw[6], yy = raw[7], yz = raw[8], zz = raw[9];
+    double qave = (xx + yy + zz) / 3.0;
+    out[4] = 0.5 * (xx - qave);
+    out[5] = 0.5 * xy;
+    out[6] = 0.5 * xz;
+    out[8] = 0.5 * (yy - qave);
+    out[9] = 0.5 * yz;
+    out[12] = 0.5 * (zz - qave);
+    out[7] = out[5];
+    out[10] = out[6];
+    out[11] = out[9];
+    double debye = 4.80321;
+    out[0] = raw[0];
+    out[1] = raw[1] * 10.0 * debye;
+    out[2] = raw[2] * 10.0 * debye;
+    out[3] = raw[3] * 10.0 * debye;
+    debye *= 3.0;
+    for (int k = 4; k < 13; k++) out[k] *= 100.0 * debye;
+    return TDP_OK;
+}
+
+int tdp_get_pme_parameters(const tdp_context* cc, double* alpha, int32_t* nx, int32_t* ny, int32_t* nz) {
+    tdp_context* c = const_cast<tdp_context*>(cc);
+    if (!c || !alpha || !nx || !ny || !nz) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->method != TDP_PME)
+        return fail(c, TDP_ERR_NOT_PME, "getPMEParametersInContext: This Context is not using PME");
+    double a = c->pme_alpha;
+    int g[3] = {c->grid[0], c->grid[1], c->grid[2]};
+    if (g[0] == 0 || a == 0.0) {
+        // The reference asks OpenMM's NonbondedForceImpl::calcPMEParameters (ReferenceTholeDipoleKernels.cpp:143-152)
+        // for a fresh NonbondedForce with this tolerance and cutoff; OpenMM's published closed form:
+        // alpha = sqrt(-ln(2 tol))/cutoff, n_d = max(6, ceil(2 alpha L_d / (3 tol^(1/5)))).
+        double tol = c->ewald_tol;
+        a = std::sqrt(-std::log(2.0 * tol)) / c->cutoff;
+        const double L[3] = {c->default_box[0], c->default_box[4], c->default_box[8]};
+        for (int k = 0; k < 3; k++) g[k] = std::max(6, (int)std::ceil(2 * a * L[k] / (3 * std::pow(tol, 0.2))));
+    }
+    *alpha = a;
+    *nx = g[0];
+    *ny = g[1];
+    *nz = g[2];
+    return TDP_OK;
+}
+
+int tdp_get_solver_stats(const tdp_context* cc, tdp_solver_stats* st) {
+    tdp_context* c = const_cast<tdp_context*>(cc);
+    if (!c || !st) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->timing) {
+        hipStreamSynchronize(c->stream);
+        harvest_timing(c);
+    }
+    st->iterations = c->iterations;
+    st->epsilon = c->epsilon;
+    st->converged = c->converged;
+    st->matvec_ms = c->matvec_ms;
+    st->matvec_launches = c->matvec_launches;
+    return TDP_OK;
+}
+
+int tdp_set_timing(tdp_context* c, int32_t enable) {
+    if (!c) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null context");
+    hipSetDevice(c->device);
+    hipStreamSynchronize(c->stream);
+    harvest_timing(c);
+    c->timing = enable != 0;
+    c->matvec_ms = 0.0;
+    c->matvec_launches = 0;
+    return TDP_OK;
+}
+
+}  // extern "C"

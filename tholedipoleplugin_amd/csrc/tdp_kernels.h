@@ -1,0 +1,72 @@
+// tdp_kernels.h -- launch wrappers of the gfx950 kernels (tdp_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tdp {
+
+// Solver scalars kept in device memory so an iteration needs no host round trip.
+enum SolverScalars {
+    SC_DONE = 0,    // != 0: the solve has stopped; iteration kernels return immediately
+    SC_ITER = 1,    // iterations done
+    SC_MAXIT = 2,   // maximum iterations
+    SC_TARGET = 3,  // target RMS dipole update (e*nm)
+    SC_EPS = 4,     // current RMS dipole update
+    SC_RZ = 5,      // PCG r.z
+    SC_GAMMA = 6,   // PCG step length
+    SC_BETA = 7,    // PCG direction update
+    SC_COUNT = 8
+};
+
+// Geometry of one all-pairs launch: nblk tiles of 64 particles, the j range of each i tile
+// split over nsplit workgroups of tiles_per_split tiles.
+struct PairLaunch {
+    int n, npad, nblk, nsplit, tiles_per_split;
+    bool pbc;
+    double box[3];
+    const int* exc_start;                 // [nblk+1]
+    const int* exc_J;                     // [ntiles]
+    const unsigned long long* exc_mask;   // [ntiles][3][64]
+};
+
+void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const double* thole, double4* posa,
+                           float4* posa32);
+void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
+                   const int4* axis, double4* qd);
+void launch_fixed_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
+                        double4* part);
+void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
+                            const double* inva, const double* scalars, double4* part);
+void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
+                            const double* scalars, double4* part);
+void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
+                          const double4* mu, const double* inva, double4* part_f, double4* part_t, double* part_e);
+void launch_field_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+                       double4* E0, double4* mu, float4* mu32);
+int red_blocks(int npad);
+void launch_jacobi_update(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                          const double* alpha, double4* mu, float4* mu32, const double* scalars, double* red);
+void launch_pcg_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                     const double* alpha, const double4* x, double4* r, double4* p, float4* p32, double* red,
+                     double* scalars);
+void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+                               double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
+                               double* scalars);
+void launch_extrap_step(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                        const double* alpha, double4* mu, float4* mu32, double4* hist_o);
+void launch_extrap_combine(hipStream_t s, int n, int npad, int order, const double4* hist, const double* coef,
+                           double4* mu, float4* mu32);
+void launch_extrap_error(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                         const double* alpha, const double4* mu, double* red, double* scalars);
+void launch_reduce_forces(hipStream_t s, int n, int npad, int nsplit, const double4* part_f, const double4* part_t,
+                          double* frc, double4* torque);
+void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double* out);
+void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,
+                       double4* contrib);
+void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx, const double4* contrib, double* frc);
+void launch_accumulate(hipStream_t s, int m, const double* src, double* dst);
+void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const double4* posa, const double4* qd,
+                      const double4* mu, bool pbc, const double* box3, double* out);
+void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd, const double4* mu,
+                    const double* mass, double* out);
+
+}  // namespace tdp

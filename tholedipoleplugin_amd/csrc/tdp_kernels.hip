@@ -1,0 +1,1033 @@
+// tdp_kernels.hip -- gfx950 kernels of the TholeDipoleForce force + induction path.
+//
+// Pair kernels use a "full" all-pairs formulation: a workgroup is one 64-lane wavefront that
+// owns a tile of 64 particles i (one per lane) and sweeps a contiguous range of j tiles; each
+// j tile is staged once into LDS with coalesced loads and then read back as wave-wide
+// broadcasts, so each lane accumulates its own particle's sums without atomics.  The j range
+// of one i tile is split over gridDim.y workgroups; the splits write separate partial rows that
+// the consumer kernel adds in a fixed order, so every result is deterministic.
+//
+// Exclusions/scalings come from per-tile bit masks built on the host (tdp_context.hip):
+// for a tile pair (I,J) that holds at least one scaled pair, three 64-bit masks per lane mark
+// mScale == 0, mScale == 0.5 and iScale == 0 (self pairs set all of them).  Tiles without masks
+// take a branch-free fast path.
+#include "tdp_kernels.h"
+#include "tdp_device.hpp"
+
+using namespace tdp;
+
+namespace {
+
+struct BoxT64 { double bx, by, bz, ibx, iby, ibz; };
+struct BoxT32 { float bx, by, bz, ibx, iby, ibz; };
+
+// Scalars shared between the solver kernels (device memory, see SolverScalars in tdp_kernels.h).
+__device__ __forceinline__ bool solver_done(const double* sc) { return sc[SC_DONE] != 0.0; }
+
+// ------------------------------------------------------------------ frame rotation (a3)
+// checkChiralCenterAtParticle + applyRotationMatrixToParticle
+// (ReferenceTholeDipoleForce.cpp:432-452, :477-546, called for Z>=0 && Z!=i at :556).
+__global__ void k_rotate_dipoles(int n, const double4* __restrict__ posa, const double* __restrict__ dmol,
+                                 const double* __restrict__ charge, const int4* __restrict__ axis,
+                                 double4* __restrict__ qd) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int4 ax = axis[i];  // (type, Z, X, Y)
+    double d0 = dmol[3 * i], d1 = dmol[3 * i + 1], d2 = dmol[3 * i + 2];
+    double4 pi = posa[i];
+    if (ax.w > -1 && ax.x == 0) {  // checkChiral: ZThenX with a Y atom (:437)
+        double4 pz = posa[ax.y], px = posa[ax.z], py = posa[ax.w];
+        double adx = pi.x - py.x, ady = pi.y - py.y, adz = pi.z - py.z;
+        double bdx = pz.x - py.x, bdy = pz.y - py.y, bdz = pz.z - py.z;
+        double cdx = px.x - py.x, cdy = px.y - py.y, cdz = px.z - py.z;
+        double cx = bdy * cdz - bdz * cdy, cy = bdz * cdx - bdx * cdz, cz = bdx * cdy - bdy * cdx;
+        double vol = cx * adx + cy * ady + cz * adz;
+        if (vol < 0.0) d1 = -d1;
+    }
+    double lx = d0, ly = d1, lz = d2;
+    if (ax.y >= 0 && ax.y != i) {
+        double4 pz = posa[ax.y];
+        double zx = pz.x - pi.x, zy = pz.y - pi.y, zz = pz.z - pi.z;
+        double nz = sqrt(zx * zx + zy * zy + zz * zz);
+        if (nz > 0.0) { double s = 1.0 / nz; zx *= s; zy *= s; zz *= s; }
+        double xx, xy, xz;
+        if (ax.x == 4) {  // ZOnly
+            if (fabs(zx) < 0.866) { xx = 1.0; xy = 0.0; xz = 0.0; } else { xx = 0.0; xy = 1.0; xz = 0.0; }
+        } else {
+            double4 px = posa[ax.z];
+            xx = px.x - pi.x; xy = px.y - pi.y; xz = px.z - pi.z;
+            if (ax.x == 1 || ax.x == 2 || ax.x == 3) {
+                double nx = sqrt(xx * xx + xy * xy + xz * xz);
+                if (nx > 0.0) { double s = 1.0 / nx; xx *= s; xy *= s; xz *= s; }
+            }
+            if (ax.x == 1) {  // Bisector
+                zx += xx; zy += xy; zz += xz;
+                double n2 = sqrt(zx * zx + zy * zy + zz * zz);
+                if (n2 > 0.0) { double s = 1.0 / n2; zx *= s; zy *= s; zz *= s; }
+            } else if (ax.x == 2) {  // ZBisect
+                double4 py = posa[ax.w];
+                double yx = py.x - pi.x, yy = py.y - pi.y, yz = py.z - pi.z;
+                double ny = sqrt(yx * yx + yy * yy + yz * yz);
+                if (ny > 0.0) { double s = 1.0 / ny; yx *= s; yy *= s; yz *= s; }
+                xx += yx; xy += yy; xz += yz;
+                double n2 = sqrt(xx * xx + xy * xy + xz * xz);
+                if (n2 > 0.0) { double s = 1.0 / n2; xx *= s; xy *= s; xz *= s; }
+            } else if (ax.x == 3) {  // ThreeFold
+                double4 py = posa[ax.w];
+                double yx = py.x - pi.x, yy = py.y - pi.y, yz = py.z - pi.z;
+                double ny = sqrt(yx * yx + yy * yy + yz * yz);
+                if (ny > 0.0) { double s = 1.0 / ny; yx *= s; yy *= s; yz *= s; }
+                zx += xx + yx; zy += xy + yy; zz += xz + yz;
+                double n2 = sqrt(zx * zx + zy * zy + zz * zz);
+                if (n2 > 0.0) { double s = 1.0 / n2; zx *= s; zy *= s; zz *= s; }
+            }
+        }
+        double dt = zx * xx + zy * xy + zz * xz;
+        xx -= zx * dt; xy -= zy * dt; xz -= zz * dt;
+        double nx = sqrt(xx * xx + xy * xy + xz * xz);
+        if (nx > 0.0) { double s = 1.0 / nx; xx *= s; xy *= s; xz *= s; }
+        double yx = zy * xz - zz * xy, yy = zz * xx - zx * xz, yz = zx * xy - zy * xx;
+        lx = d0 * xx + d1 * yx + d2 * zx;
+        ly = d0 * xy + d1 * yy + d2 * zy;
+        lz = d0 * xz + d1 * yz + d2 * zz;
+    }
+    qd[i] = make_double4(lx, ly, lz, charge[i]);
+}
+
+// ------------------------------------------------------------------ positions
+__global__ void k_load_positions(int n, int npad, const double* __restrict__ pos3, const double* __restrict__ thole,
+                                 double4* __restrict__ posa, float4* __restrict__ posa32) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (i < n) p = make_double4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], thole[i]);
+    posa[i] = p;
+    if (posa32) posa32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+}
+
+// ------------------------------------------------------------------ permanent field (a6)
+// calculateFixedDipoleField(+PairIxn) (ReferenceTholeDipoleForce.cpp:678-733): field at i from
+// the charge and permanent dipole of j, Delta = r_j - r_i:
+//   E_i -= m_ij * [Delta (rr3 q_j - rr5 d_j.Delta) + rr3 d_j].
+template <bool PBC>
+__global__ __launch_bounds__(64) void k_fixed_field(int n, int npad, int tiles_per_split,
+                                                    const double4* __restrict__ posa, const double4* __restrict__ qd,
+                                                    const double* __restrict__ inva, const int* __restrict__ exc_start,
+                                                    const int* __restrict__ exc_J,
+                                                    const unsigned long long* __restrict__ exc_mask, BoxT64 box,
+                                                    double4* __restrict__ part) {
+    __shared__ double4 s_p[kTile];
+    __shared__ double4 s_q[kTile];
+    __shared__ double s_ia[kTile];
+    const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
+    const int i = I * kTile + lane;
+    const int nblk = (n + kTile - 1) / kTile;
+    const int J0 = split * tiles_per_split;
+    const int J1 = min(nblk, J0 + tiles_per_split);
+    const double4 pi = posa[i];
+    const double iai = inva[i];
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    int e = exc_start[I];
+    const int eEnd = exc_start[I + 1];
+    while (e < eEnd && exc_J[e] < J0) e++;
+    for (int J = J0; J < J1; J++) {
+        const int jb = J * kTile;
+        __syncthreads();
+        s_p[lane] = posa[jb + lane];
+        s_q[lane] = qd[jb + lane];
+        s_ia[lane] = inva[jb + lane];
+        __syncthreads();
+        const int jn = min(kTile, n - jb);
+        const bool masked = (e < eEnd && exc_J[e] == J);
+        unsigned long long m0 = 0ull, mh = 0ull;
+        if (masked) {
+            m0 = exc_mask[((size_t)e * 3 + 0) * kTile + lane];
+            mh = exc_mask[((size_t)e * 3 + 1) * kTile + lane];
+            e++;
+        }
+        for (int jj = 0; jj < jn; jj++) {
+            double scale = 1.0;
+            if (masked) {
+                if ((m0 >> jj) & 1ull) continue;
+                if ((mh >> jj) & 1ull) scale = 0.5;
+            }
+            const double4 pj = s_p[jj];
+            const double4 qj = s_q[jj];
+            double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double rr3, rr5;
+            damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+            double dD = qj.x * dx + qj.y * dy + qj.z * dz;
+            double f = rr3 * qj.w - rr5 * dD;
+            ex -= (dx * f + qj.x * rr3) * scale;
+            ey -= (dy * f + qj.y * rr3) * scale;
+            ez -= (dz * f + qj.z * rr3) * scale;
+        }
+    }
+    part[(size_t)split * npad + i] = make_double4(ex, ey, ez, 0.0);
+}
+
+// ------------------------------------------------------------------ T * mu (a7)
+// calculateInducedDipoleFields(+PairIxn) (ReferenceTholeDipoleForce.cpp:766-818):
+//   E_i += iScale * [-rr3 mu_j + rr5 (mu_j.Delta) Delta], pairs with iScale == 0 skipped.
+// T = double: every term in FP64.  T = float: pair terms in FP32, each tile's 64 terms summed in
+// FP32 and added to FP64 accumulators.
+template <typename T, bool PBC>
+__global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles_per_split,
+                                                      const typename Vec4<T>::type* __restrict__ posa,
+                                                      const typename Vec4<T>::type* __restrict__ mu,
+                                                      const T* __restrict__ inva, const int* __restrict__ exc_start,
+                                                      const int* __restrict__ exc_J,
+                                                      const unsigned long long* __restrict__ exc_mask, BoxT64 box64,
+                                                      const double* __restrict__ scalars, double4* __restrict__ part) {
+    using V4 = typename Vec4<T>::type;
+    if (scalars && solver_done(scalars)) return;
+    __shared__ V4 s_p[kTile];
+    __shared__ V4 s_u[kTile];
+    __shared__ T s_ia[kTile];
+    const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
+    const int i = I * kTile + lane;
+    const int nblk = (n + kTile - 1) / kTile;
+    const int J0 = split * tiles_per_split;
+    const int J1 = min(nblk, J0 + tiles_per_split);
+    const V4 pi = posa[i];
+    const T iai = inva[i];
+    const T bx = (T)box64.bx, by = (T)box64.by, bz = (T)box64.bz;
+    const T ibx = (T)box64.ibx, iby = (T)box64.iby, ibz = (T)box64.ibz;
+    double ex = 0.0, ey = 0.0, ez = 0.0;
+    int e = exc_start[I];
+    const int eEnd = exc_start[I + 1];
+    while (e < eEnd && exc_J[e] < J0) e++;
+    for (int J = J0; J < J1; J++) {
+        const int jb = J * kTile;
+        __syncthreads();
+        s_p[lane] = posa[jb + lane];
+        s_u[lane] = mu[jb + lane];
+        s_ia[lane] = inva[jb + lane];
+        __syncthreads();
+        const int jn = min(kTile, n - jb);
+        const bool masked = (e < eEnd && exc_J[e] == J);
+        unsigned long long mi = 0ull;
+        if (masked) {
+            mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
+            e++;
+        }
+        T tx = T(0), ty = T(0), tz = T(0);
+        for (int jj = 0; jj < jn; jj++) {
+            if (masked && ((mi >> jj) & 1ull)) continue;
+            const V4 pj = s_p[jj];
+            const V4 uj = s_u[jj];
+            T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+            if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
+            T r2 = dx * dx + dy * dy + dz * dz;
+            T rr3, rr5;
+            damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+            T ud = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+            tx += ud * dx - rr3 * uj.x;
+            ty += ud * dy - rr3 * uj.y;
+            tz += ud * dz - rr3 * uj.z;
+        }
+        ex += (double)tx;
+        ey += (double)ty;
+        ez += (double)tz;
+    }
+    part[(size_t)split * npad + i] = make_double4(ex, ey, ez, 0.0);
+}
+
+// ------------------------------------------------------------------ energy, forces, torques (a9)
+// calculateElectrostatic (ReferenceTholeDipoleForce.cpp:368-393).  Every lane evaluates all its
+// pairs in the canonical orientation (lower index = I); tiles J > I make the lane I, tiles J < I
+// make it K, the diagonal tile decides per lane.  The pair energy is counted by the I side.
+__device__ __forceinline__ void load_site(ElecSite& s, double4 p, double4 q, double4 u, double ia) {
+    s.x = p.x; s.y = p.y; s.z = p.z; s.a = p.w; s.ia = ia;
+    s.dx = q.x; s.dy = q.y; s.dz = q.z; s.q = q.w;
+    s.ux = u.x; s.uy = u.y; s.uz = u.z;
+}
+
+template <bool PBC, bool MUTUAL>
+__global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles_per_split,
+                                                      const double4* __restrict__ posa, const double4* __restrict__ qd,
+                                                      const double4* __restrict__ mu, const double* __restrict__ inva,
+                                                      const int* __restrict__ exc_start, const int* __restrict__ exc_J,
+                                                      const unsigned long long* __restrict__ exc_mask, BoxT64 box,
+                                                      double4* __restrict__ part_f, double4* __restrict__ part_t,
+                                                      double* __restrict__ part_e) {
+    __shared__ double4 s_p[kTile];
+    __shared__ double4 s_q[kTile];
+    __shared__ double4 s_u[kTile];
+    __shared__ double s_ia[kTile];
+    const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
+    const int i = I * kTile + lane;
+    const int nblk = (n + kTile - 1) / kTile;
+    const int J0 = split * tiles_per_split;
+    const int J1 = min(nblk, J0 + tiles_per_split);
+    ElecSite me;
+    load_site(me, posa[i], qd[i], mu[i], inva[i]);
+    double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, en = 0;
+    int e = exc_start[I];
+    const int eEnd = exc_start[I + 1];
+    while (e < eEnd && exc_J[e] < J0) e++;
+    for (int J = J0; J < J1; J++) {
+        const int jb = J * kTile;
+        __syncthreads();
+        s_p[lane] = posa[jb + lane];
+        s_q[lane] = qd[jb + lane];
+        s_u[lane] = mu[jb + lane];
+        s_ia[lane] = inva[jb + lane];
+        __syncthreads();
+        const int jn = min(kTile, n - jb);
+        const bool masked = (e < eEnd && exc_J[e] == J);
+        unsigned long long m0 = 0ull, mh = 0ull, mi = 0ull;
+        if (masked) {
+            m0 = exc_mask[((size_t)e * 3 + 0) * kTile + lane];
+            mh = exc_mask[((size_t)e * 3 + 1) * kTile + lane];
+            mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
+            e++;
+        }
+        for (int jj = 0; jj < jn; jj++) {
+            double mS = 1.0, iS = 1.0;
+            if (masked) {
+                bool z0 = (m0 >> jj) & 1ull, h = (mh >> jj) & 1ull, zi = (mi >> jj) & 1ull;
+                if (z0 && zi) continue;  // all coefficients zero (also the self pair)
+                mS = z0 ? 0.0 : (h ? 0.5 : 1.0);
+                iS = zi ? 0.0 : 1.0;
+            }
+            ElecSite other;
+            load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
+            const int j = jb + jj;
+            const bool lo = i < j;  // uniform except on the diagonal tile
+            const ElecSite& A = lo ? me : other;
+            const ElecSite& B = lo ? other : me;
+            double dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
+            if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+            ElecOut o;
+            elec_pair<MUTUAL>(A, B, dx, dy, dz, mS, iS, o);
+            if (lo) {
+                fx -= o.fx; fy -= o.fy; fz -= o.fz;
+                tx += o.tix; ty += o.tiy; tz += o.tiz;
+                en += o.e;
+            } else {
+                fx += o.fx; fy += o.fy; fz += o.fz;
+                tx += o.tkx; ty += o.tky; tz += o.tkz;
+            }
+        }
+    }
+    part_f[(size_t)split * npad + i] = make_double4(fx, fy, fz, 0.0);
+    part_t[(size_t)split * npad + i] = make_double4(tx, ty, tz, 0.0);
+    if (i >= n) en = 0.0;
+    en = wave_sum(en);
+    if (lane == 0) part_e[(size_t)split * gridDim.x + I] = en;
+}
+
+// ------------------------------------------------------------------ split reductions
+__device__ __forceinline__ double4 sum_parts(const double4* __restrict__ part, int nsplit, int npad, int i) {
+    double4 s = part[i];
+    for (int k = 1; k < nsplit; k++) {
+        double4 v = part[(size_t)k * npad + i];
+        s.x += v.x; s.y += v.y; s.z += v.z;
+    }
+    return s;
+}
+
+// E0 = sum of split partials; mu = alpha*E0 (calculateInducedDipoles, ReferenceTholeDipoleForce.cpp:743-746).
+__global__ void k_field_init(int n, int npad, int nsplit, const double4* __restrict__ part,
+                             const double* __restrict__ alpha, double4* __restrict__ E0, double4* __restrict__ mu,
+                             float4* __restrict__ mu32) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    double4 f = make_double4(0, 0, 0, 0), m = make_double4(0, 0, 0, 0);
+    if (i < n) {
+        f = sum_parts(part, nsplit, npad, i);
+        f.w = 0.0;
+        double a = alpha[i];
+        m = make_double4(f.x * a, f.y * a, f.z * a, 0.0);
+    }
+    E0[i] = f;
+    mu[i] = m;
+    if (mu32) mu32[i] = make_float4((float)m.x, (float)m.y, (float)m.z, 0.f);
+}
+
+// Block partial sums for the solver norms: 256 threads, fixed-order tree in LDS.
+__device__ __forceinline__ void block_sum2(double a, double b, double* __restrict__ out, int stride) {
+    __shared__ double sa[256];
+    __shared__ double sb[256];
+    int t = threadIdx.x;
+    sa[t] = a;
+    sb[t] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (t < s) {
+            sa[t] += sa[t + s];
+            sb[t] += sb[t + s];
+        }
+        __syncthreads();
+    }
+    if (t == 0) {
+        out[blockIdx.x] = sa[0];
+        out[stride + blockIdx.x] = sb[0];
+    }
+}
+
+// One Jacobi step of convergeInducedDipolesByDIIS (ReferenceTholeDipoleForce.cpp:835-846):
+// mu_new = alpha*(E0 + T mu); partial sums of |mu_new - mu|^2.
+__global__ __launch_bounds__(256) void k_jacobi_update(int n, int npad, int nsplit, const double4* __restrict__ part,
+                                                       const double4* __restrict__ E0, const double* __restrict__ alpha,
+                                                       double4* __restrict__ mu, float4* __restrict__ mu32,
+                                                       const double* __restrict__ scalars, double* __restrict__ red,
+                                                       int red_stride) {
+    if (solver_done(scalars)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < n) {
+        double4 f = sum_parts(part, nsplit, npad, i);
+        double4 e0 = E0[i];
+        double a = alpha[i];
+        double nx = (e0.x + f.x) * a, ny = (e0.y + f.y) * a, nz = (e0.z + f.z) * a;
+        double4 o = mu[i];
+        double dx = nx - o.x, dy = ny - o.y, dz = nz - o.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+        mu[i] = make_double4(nx, ny, nz, 0.0);
+        if (mu32) mu32[i] = make_float4((float)nx, (float)ny, (float)nz, 0.f);
+    }
+    block_sum2(d2, 0.0, red, red_stride);
+}
+
+// Jacobi bookkeeping (ReferenceTholeDipoleForce.cpp:829-850): eps = sqrt(sum/n), iteration count,
+// stop when iterations == max or !(eps > target).
+__global__ void k_jacobi_finalize(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
+    if (solver_done(scalars)) return;
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < nred; k++) s += red[k];
+    double eps = sqrt(s / n);
+    double it = scalars[SC_ITER] + 1.0;
+    scalars[SC_EPS] = eps;
+    scalars[SC_ITER] = it;
+    if (it >= scalars[SC_MAXIT] || !(eps > scalars[SC_TARGET])) scalars[SC_DONE] = 1.0;
+}
+
+// ---- Jacobi-preconditioned CG on (alpha^-1 - T) mu = E0, stopped on the same RMS(alpha r) the
+// Jacobi iteration uses (alpha r is exactly the Jacobi update).
+// init: x = mu (= alpha E0), r = E0 - x/alpha + T x, z = alpha r, p = z.
+__global__ __launch_bounds__(256) void k_pcg_init(int n, int npad, int nsplit, const double4* __restrict__ part,
+                                                  const double4* __restrict__ E0, const double* __restrict__ alpha,
+                                                  const double4* __restrict__ x, double4* __restrict__ r,
+                                                  double4* __restrict__ p, float4* __restrict__ p32,
+                                                  double* __restrict__ red, int red_stride) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, zz = 0.0;
+    if (i < npad) {
+        double4 rv = make_double4(0, 0, 0, 0), zv = make_double4(0, 0, 0, 0);
+        if (i < n) {
+            double4 f = sum_parts(part, nsplit, npad, i);
+            double4 e0 = E0[i], xv = x[i];
+            double a = alpha[i];
+            double ia = 1.0 / a;
+            rv = make_double4(e0.x - xv.x * ia + f.x, e0.y - xv.y * ia + f.y, e0.z - xv.z * ia + f.z, 0.0);
+            zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
+            rz = rv.x * zv.x + rv.y * zv.y + rv.z * zv.z;
+            zz = zv.x * zv.x + zv.y * zv.y + zv.z * zv.z;
+        }
+        r[i] = rv;
+        p[i] = zv;
+        if (p32) p32[i] = make_float4((float)zv.x, (float)zv.y, (float)zv.z, 0.f);
+    }
+    block_sum2(rz, zz, red, red_stride);
+}
+
+__global__ void k_pcg_init_finalize(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
+    if (threadIdx.x != 0) return;
+    double rz = 0.0, zz = 0.0;
+    for (int k = 0; k < nred; k++) {
+        rz += red[k];
+        zz += red[nred + k];
+    }
+    double eps = sqrt(zz / n);
+    scalars[SC_RZ] = rz;
+    scalars[SC_EPS] = eps;
+    scalars[SC_ITER] = 0.0;
+    scalars[SC_DONE] = (scalars[SC_MAXIT] <= 0.0 || !(eps > scalars[SC_TARGET])) ? 1.0 : 0.0;
+}
+
+// q = p/alpha - T p; partial p.q
+__global__ __launch_bounds__(256) void k_pcg_q(int n, int npad, int nsplit, const double4* __restrict__ part,
+                                               const double* __restrict__ alpha, const double4* __restrict__ p,
+                                               double4* __restrict__ q, const double* __restrict__ scalars,
+                                               double* __restrict__ red, int red_stride) {
+    if (solver_done(scalars)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double pq = 0.0;
+    if (i < n) {
+        double4 f = sum_parts(part, nsplit, npad, i);
+        double4 pv = p[i];
+        double ia = 1.0 / alpha[i];
+        double4 qv = make_double4(pv.x * ia - f.x, pv.y * ia - f.y, pv.z * ia - f.z, 0.0);
+        q[i] = qv;
+        pq = pv.x * qv.x + pv.y * qv.y + pv.z * qv.z;
+    }
+    block_sum2(pq, 0.0, red, red_stride);
+}
+
+__global__ void k_pcg_gamma(int nred, const double* __restrict__ red, double* __restrict__ scalars) {
+    if (solver_done(scalars)) return;
+    if (threadIdx.x != 0) return;
+    double pq = 0.0;
+    for (int k = 0; k < nred; k++) pq += red[k];
+    scalars[SC_GAMMA] = scalars[SC_RZ] / pq;
+}
+
+// x += gamma p; r -= gamma q; z = alpha r; partial r.z and z.z
+__global__ __launch_bounds__(256) void k_pcg_update(int n, const double* __restrict__ alpha, const double4* __restrict__ p,
+                                                    const double4* __restrict__ q, double4* __restrict__ x,
+                                                    double4* __restrict__ r, double4* __restrict__ z,
+                                                    const double* __restrict__ scalars, double* __restrict__ red,
+                                                    int red_stride) {
+    if (solver_done(scalars)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double rz = 0.0, zz = 0.0;
+    if (i < n) {
+        double g = scalars[SC_GAMMA];
+        double4 pv = p[i], qv = q[i], xv = x[i], rv = r[i];
+        double a = alpha[i];
+        xv.x += g * pv.x; xv.y += g * pv.y; xv.z += g * pv.z;
+        rv.x -= g * qv.x; rv.y -= g * qv.y; rv.z -= g * qv.z;
+        double4 zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
+        x[i] = xv;
+        r[i] = rv;
+        z[i] = zv;
+        rz = rv.x * zv.x + rv.y * zv.y + rv.z * zv.z;
+        zz = zv.x * zv.x + zv.y * zv.y + zv.z * zv.z;
+    }
+    block_sum2(rz, zz, red, red_stride);
+}
+
+__global__ void k_pcg_beta(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
+    if (solver_done(scalars)) return;
+    if (threadIdx.x != 0) return;
+    double rz = 0.0, zz = 0.0;
+    for (int k = 0; k < nred; k++) {
+        rz += red[k];
+        zz += red[nred + k];
+    }
+    double eps = sqrt(zz / n);
+    double it = scalars[SC_ITER] + 1.0;
+    scalars[SC_BETA] = rz / scalars[SC_RZ];
+    scalars[SC_RZ] = rz;
+    scalars[SC_EPS] = eps;
+    scalars[SC_ITER] = it;
+    // Once DONE is set the remaining kernels of this iteration (the p update) return early;
+    // p is not needed after the last iteration.
+    if (it >= scalars[SC_MAXIT] || !(eps > scalars[SC_TARGET])) scalars[SC_DONE] = 1.0;
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(256) void k_pcg_direction(int n, int npad, const double4* __restrict__ z,
+                                                       double4* __restrict__ p, float4* __restrict__ p32,
+                                                       const double* __restrict__ scalars) {
+    if (solver_done(scalars)) return;
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double b = scalars[SC_BETA];
+    double4 zv = z[i], pv = p[i];
+    double4 np = make_double4(zv.x + b * pv.x, zv.y + b * pv.y, zv.z + b * pv.z, 0.0);
+    p[i] = np;
+    if (p32) p32[i] = make_float4((float)np.x, (float)np.y, (float)np.z, 0.f);
+}
+
+// Extrapolated polarization (convergeInducedDipolesByExtrapolation, ReferenceTholeDipoleForce.cpp:876-887):
+// mu = alpha*(E0 + T mu); hist[o] = mu.
+__global__ void k_extrap_step(int n, int npad, int nsplit, const double4* __restrict__ part,
+                              const double4* __restrict__ E0, const double* __restrict__ alpha,
+                              double4* __restrict__ mu, float4* __restrict__ mu32, double4* __restrict__ hist_o) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 f = sum_parts(part, nsplit, npad, i);
+    double4 e0 = E0[i];
+    double a = alpha[i];
+    double4 m = make_double4((e0.x + f.x) * a, (e0.y + f.y) * a, (e0.z + f.z) * a, 0.0);
+    mu[i] = m;
+    if (mu32) mu32[i] = make_float4((float)m.x, (float)m.y, (float)m.z, 0.f);
+    hist_o[i] = m;
+}
+
+// mu = sum_o c_o hist[o]  (:890-895)
+__global__ void k_extrap_combine(int n, int npad, int order, const double4* __restrict__ hist,
+                                 const double* __restrict__ coef, double4* __restrict__ mu, float4* __restrict__ mu32) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = 0, y = 0, z = 0;
+    for (int o = 0; o < order; o++) {
+        double4 h = hist[(size_t)o * npad + i];
+        double c = coef[o];
+        x += h.x * c; y += h.y * c; z += h.z * c;
+    }
+    mu[i] = make_double4(x, y, z, 0.0);
+    if (mu32) mu32[i] = make_float4((float)x, (float)y, (float)z, 0.f);
+}
+
+// final error of the extrapolated dipoles (:897-906): partial sums of |alpha(E0+T mu) - mu|^2
+__global__ __launch_bounds__(256) void k_extrap_error(int n, int npad, int nsplit, const double4* __restrict__ part,
+                                                      const double4* __restrict__ E0, const double* __restrict__ alpha,
+                                                      const double4* __restrict__ mu, double* __restrict__ red,
+                                                      int red_stride) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double d2 = 0.0;
+    if (i < n) {
+        double4 f = sum_parts(part, nsplit, npad, i);
+        double4 e0 = E0[i], m = mu[i];
+        double a = alpha[i];
+        double dx = (e0.x + f.x) * a - m.x, dy = (e0.y + f.y) * a - m.y, dz = (e0.z + f.z) * a - m.z;
+        d2 = dx * dx + dy * dy + dz * dz;
+    }
+    block_sum2(d2, 0.0, red, red_stride);
+}
+
+__global__ void k_sum_eps(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
+    if (threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int k = 0; k < nred; k++) s += red[k];
+    scalars[SC_EPS] = sqrt(s / n);
+}
+
+// Forces/torques: sum split partials.  frc is [3n] doubles (accumulated later into the caller's array).
+__global__ void k_reduce_forces(int n, int npad, int nsplit, const double4* __restrict__ part_f,
+                                const double4* __restrict__ part_t, double* __restrict__ frc,
+                                double4* __restrict__ torque) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double4 f = sum_parts(part_f, nsplit, npad, i);
+    double4 t = sum_parts(part_t, nsplit, npad, i);
+    frc[3 * i] = f.x;
+    frc[3 * i + 1] = f.y;
+    frc[3 * i + 2] = f.z;
+    torque[i] = make_double4(t.x, t.y, t.z, 0.0);
+}
+
+__global__ void k_reduce_energy(int count, const double* __restrict__ part_e, double* __restrict__ out) {
+    __shared__ double s[256];
+    int t = threadIdx.x;
+    double a = 0.0;
+    for (int k = t; k < count; k += 256) a += part_e[k];
+    s[t] = a;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w) s[t] += s[t + w];
+        __syncthreads();
+    }
+    if (t == 0) *out = s[0];
+}
+
+// ------------------------------------------------------------------ torque -> force (a10)
+// mapTorqueToForceForParticle (ReferenceTholeDipoleForce.cpp:915-1090), one thread per particle
+// with an axis; writes the contributions to I, U(=Z), V(=X), W(=Y) into contrib[4*i + role].
+__device__ __forceinline__ double nrm3(double& x, double& y, double& z) {
+    double n = sqrt(x * x + y * y + z * z);
+    if (n > 0.0) { double s = 1.0 / n; x *= s; y *= s; z *= s; }
+    return n;
+}
+__device__ __forceinline__ void crs(double ax, double ay, double az, double bx, double by, double bz, double& cx,
+                                    double& cy, double& cz) {
+    cx = ay * bz - az * by;
+    cy = az * bx - ax * bz;
+    cz = ax * by - ay * bx;
+}
+
+__global__ void k_map_torque(int n, const double4* __restrict__ posa, const int4* __restrict__ axis,
+                             const double4* __restrict__ torque, double4* __restrict__ contrib) {
+    int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= n) return;
+    int4 ax = axis[I];
+    double4 cI = make_double4(0, 0, 0, 0), cU = cI, cV = cI, cW = cI;
+    const int type = ax.x;
+    if (type != 5) {
+        const double4 pI = posa[I], pU = posa[ax.y], pV = posa[ax.z];
+        const int W = ax.w > -1 ? ax.w : -1;
+        double ux = pU.x - pI.x, uy = pU.y - pI.y, uz = pU.z - pI.z;
+        double vx = pV.x - pI.x, vy = pV.y - pI.y, vz = pV.z - pI.z;
+        double normU = nrm3(ux, uy, uz);
+        double normV = nrm3(vx, vy, vz);
+        double wx, wy, wz;
+        if (W >= 0 && (type == 2 || type == 3)) {
+            double4 pW = posa[W];
+            wx = pW.x - pI.x; wy = pW.y - pI.y; wz = pW.z - pI.z;
+        } else {
+            crs(ux, uy, uz, vx, vy, vz, wx, wy, wz);
+        }
+        double normW = nrm3(wx, wy, wz);
+        double uvx, uvy, uvz, uwx, uwy, uwz, vwx, vwy, vwz;
+        crs(vx, vy, vz, ux, uy, uz, uvx, uvy, uvz);
+        crs(wx, wy, wz, ux, uy, uz, uwx, uwy, uwz);
+        crs(wx, wy, wz, vx, vy, vz, vwx, vwy, vwz);
+        nrm3(uvx, uvy, uvz);
+        nrm3(uwx, uwy, uwz);
+        nrm3(vwx, vwy, vwz);
+        double cosUV = ux * vx + uy * vy + uz * vz, sinUV = sqrt(1.0 - cosUV * cosUV);
+        double cosUW = ux * wx + uy * wy + uz * wz, sinUW = sqrt(1.0 - cosUW * cosUW);
+        double cosVW = vx * wx + vy * wy + vz * wz, sinVW = sqrt(1.0 - cosVW * cosVW);
+        double4 t = torque[I];
+        double dphi0 = -(ux * t.x + uy * t.y + uz * t.z);
+        double dphi1 = -(vx * t.x + vy * t.y + vz * t.z);
+        double dphi2 = -(wx * t.x + wy * t.y + wz * t.z);
+        if (type == 0 || type == 1) {
+            double f1 = dphi1 / (normU * sinUV);
+            double f2 = dphi2 / normU;
+            double f3 = -dphi0 / (normV * sinUV);
+            double f4 = 0.0;
+            if (type == 1) {
+                f2 *= 0.5;
+                f4 = 0.5 * dphi2 / normV;
+            }
+            double FUx = uvx * f1 + f2 * uwx, FUy = uvy * f1 + f2 * uwy, FUz = uvz * f1 + f2 * uwz;
+            double FVx = uvx * f3 + f4 * vwx, FVy = uvy * f3 + f4 * vwy, FVz = uvz * f3 + f4 * vwz;
+            cU = make_double4(-FUx, -FUy, -FUz, 0);
+            cV = make_double4(-FVx, -FVy, -FVz, 0);
+            cI = make_double4(FUx + FVx, FUy + FVy, FUz + FVz, 0);
+        } else if (type == 2) {
+            double rx = vx + wx, ry = vy + wy, rz = vz + wz;
+            double sx, sy, sz;
+            crs(ux, uy, uz, rx, ry, rz, sx, sy, sz);
+            nrm3(rx, ry, rz);
+            nrm3(sx, sy, sz);
+            double urx, ury, urz, usx, usy, usz;
+            crs(rx, ry, rz, ux, uy, uz, urx, ury, urz);
+            crs(sx, sy, sz, ux, uy, uz, usx, usy, usz);
+            nrm3(urx, ury, urz);
+            nrm3(usx, usy, usz);
+            double cosUR = ux * rx + uy * ry + uz * rz, sinUR = sqrt(1.0 - cosUR * cosUR);
+            double cosVS = vx * sx + vy * sy + vz * sz, sinVS = sqrt(1.0 - cosVS * cosVS);
+            double cosWS = wx * sx + wy * sy + wz * sz, sinWS = sqrt(1.0 - cosWS * cosWS);
+            double t1x = vx - sx * cosVS, t1y = vy - sy * cosVS, t1z = vz - sz * cosVS;
+            double t2x = wx - sx * cosWS, t2y = wy - sy * cosWS, t2z = wz - sz * cosWS;
+            nrm3(t1x, t1y, t1z);
+            nrm3(t2x, t2y, t2z);
+            double ut1cos = ux * t1x + uy * t1y + uz * t1z, ut1sin = sqrt(1.0 - ut1cos * ut1cos);
+            double ut2cos = ux * t2x + uy * t2y + uz * t2z, ut2sin = sqrt(1.0 - ut2cos * ut2cos);
+            double dphiR = -(rx * t.x + ry * t.y + rz * t.z);
+            double dphiS = -(sx * t.x + sy * t.y + sz * t.z);
+            double f1 = dphiR / (normU * sinUR);
+            double f2 = dphiS / normU;
+            double f3 = dphi0 / (normV * (ut1sin + ut2sin));
+            double f4 = dphi0 / (normW * (ut1sin + ut2sin));
+            double FUx = urx * f1 + usx * f2, FUy = ury * f1 + usy * f2, FUz = urz * f1 + usz * f2;
+            double FVx = (sx * sinVS - t1x * cosVS) * f3, FVy = (sy * sinVS - t1y * cosVS) * f3,
+                   FVz = (sz * sinVS - t1z * cosVS) * f3;
+            double FWx = (sx * sinWS - t2x * cosWS) * f4, FWy = (sy * sinWS - t2y * cosWS) * f4,
+                   FWz = (sz * sinWS - t2z * cosWS) * f4;
+            cU = make_double4(-FUx, -FUy, -FUz, 0);
+            cV = make_double4(-FVx, -FVy, -FVz, 0);
+            cW = make_double4(-FWx, -FWy, -FWz, 0);
+            cI = make_double4(FUx + FVx + FWx, FUy + FVy + FWy, FUz + FVz + FWz, 0);
+        } else if (type == 3) {
+            double du[3], dv[3], dw[3];
+            const double UW[3] = {uwx, uwy, uwz}, UV[3] = {uvx, uvy, uvz}, VW[3] = {vwx, vwy, vwz};
+            for (int k = 0; k < 3; k++) {
+                du[k] = (UW[k] * dphi2 / (normU * sinUW) + UV[k] * dphi1 / (normU * sinUV) -
+                         UW[k] * dphi0 / (normU * sinUW) - UV[k] * dphi0 / (normU * sinUV)) / 3.0;
+                dv[k] = (VW[k] * dphi2 / (normV * sinVW) - UV[k] * dphi0 / (normV * sinUV) -
+                         VW[k] * dphi1 / (normV * sinVW) + UV[k] * dphi1 / (normV * sinUV)) / 3.0;
+                dw[k] = (-UW[k] * dphi0 / (normW * sinUW) - VW[k] * dphi1 / (normW * sinVW) +
+                         UW[k] * dphi2 / (normW * sinUW) + VW[k] * dphi2 / (normW * sinVW)) / 3.0;
+            }
+            cU = make_double4(-du[0], -du[1], -du[2], 0);
+            cV = make_double4(-dv[0], -dv[1], -dv[2], 0);
+            if (W >= 0) cW = make_double4(-dw[0], -dw[1], -dw[2], 0);
+            cI = make_double4(du[0] + dv[0] + dw[0], du[1] + dv[1] + dw[1], du[2] + dv[2] + dw[2], 0);
+        } else if (type == 4) {
+            double dux = uvx * dphi1 / (normU * sinUV) + uwx * dphi2 / normU;
+            double duy = uvy * dphi1 / (normU * sinUV) + uwy * dphi2 / normU;
+            double duz = uvz * dphi1 / (normU * sinUV) + uwz * dphi2 / normU;
+            cU = make_double4(-dux, -duy, -duz, 0);
+            cI = make_double4(dux, duy, duz, 0);
+        }
+    }
+    contrib[4 * I + 0] = cI;
+    contrib[4 * I + 1] = cU;
+    contrib[4 * I + 2] = cV;
+    contrib[4 * I + 3] = cW;
+}
+
+// Gather of the torque contributions: atom a receives the entries listed in gather[goff[a]..goff[a+1])
+// (indices into contrib, host-sorted so the sum order is fixed), added to frc.
+__global__ void k_gather_torque(int n, const int* __restrict__ goff, const int* __restrict__ gidx,
+                                const double4* __restrict__ contrib, double* __restrict__ frc) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= n) return;
+    double x = 0, y = 0, z = 0;
+    for (int k = goff[a]; k < goff[a + 1]; k++) {
+        double4 c = contrib[gidx[k]];
+        x += c.x; y += c.y; z += c.z;
+    }
+    frc[3 * a] += x;
+    frc[3 * a + 1] += y;
+    frc[3 * a + 2] += z;
+}
+
+// out[i] += frc[i]  (forces are accumulated into the caller's array)
+__global__ void k_accumulate(int m, const double* __restrict__ src, double* __restrict__ dst) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < m) dst[k] += src[k];
+}
+
+// ------------------------------------------------------------------ potential (next row)
+// calculateElectrostaticPotential (ReferenceTholeDipoleForce.cpp:1315-1372):
+// phi(g) = sum_i q_i/r - (d_i + mu_i).Delta/r^3, Delta = x_i - g (min-imaged).
+template <bool PBC>
+__global__ __launch_bounds__(64) void k_potential(int n, int ngrid, const double* __restrict__ grid,
+                                                  const double4* __restrict__ posa, const double4* __restrict__ qd,
+                                                  const double4* __restrict__ mu, BoxT64 box, double* __restrict__ out) {
+    __shared__ double4 s_p[kTile];
+    __shared__ double4 s_q[kTile];
+    __shared__ double4 s_u[kTile];
+    int lane = threadIdx.x;
+    int g = blockIdx.x * kTile + lane;
+    double gx = 0, gy = 0, gz = 0;
+    if (g < ngrid) { gx = grid[3 * g]; gy = grid[3 * g + 1]; gz = grid[3 * g + 2]; }
+    double phi = 0.0;
+    int nblk = (n + kTile - 1) / kTile;
+    for (int J = 0; J < nblk; J++) {
+        int jb = J * kTile;
+        __syncthreads();
+        s_p[lane] = posa[jb + lane];
+        s_q[lane] = qd[jb + lane];
+        s_u[lane] = mu[jb + lane];
+        __syncthreads();
+        int jn = min(kTile, n - jb);
+        for (int jj = 0; jj < jn; jj++) {
+            double4 p = s_p[jj], q = s_q[jj], u = s_u[jj];
+            double dx = p.x - gx, dy = p.y - gy, dz = p.z - gz;
+            if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+            double r2 = dx * dx + dy * dy + dz * dz;
+            double rr1 = 1.0 / sqrt(r2);
+            double rr3 = rr1 * rr1 * rr1;
+            double v = q.w * rr1;
+            double sc = (q.x + u.x) * dx + (q.y + u.y) * dy + (q.z + u.z) * dz;
+            phi += v - sc * rr3;
+        }
+    }
+    if (g < ngrid) out[g] = phi;
+}
+
+// ------------------------------------------------------------------ multipole moments (next row)
+// calculateTholeDipoleSystemMultipoleMoments (ReferenceTholeDipoleForce.cpp:1225-1313): raw sums
+// in one 256-thread block (fixed-order reduction), unit conversion on the host.
+__global__ __launch_bounds__(256) void k_moments(int n, const double4* __restrict__ posa, const double4* __restrict__ qd,
+                                                 const double4* __restrict__ mu, const double* __restrict__ mass,
+                                                 double* __restrict__ out /* 4 + 10 */) {
+    __shared__ double s[256][10];
+    __shared__ double com[4];
+    int t = threadIdx.x;
+    double acc[10];
+    for (int k = 0; k < 4; k++) acc[k] = 0.0;
+    for (int i = t; i < n; i += 256) {
+        double m = mass[i];
+        double4 p = posa[i];
+        acc[0] += m; acc[1] += p.x * m; acc[2] += p.y * m; acc[3] += p.z * m;
+    }
+    for (int k = 0; k < 4; k++) s[t][k] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 4; k++) s[t][k] += s[t + w][k];
+        __syncthreads();
+    }
+    if (t == 0) {
+        double tm = s[0][0];
+        double cx = s[0][1], cy = s[0][2], cz = s[0][3];
+        if (tm > 0.0) { double inv = 1.0 / tm; cx *= inv; cy *= inv; cz *= inv; }
+        com[0] = cx; com[1] = cy; com[2] = cz; com[3] = tm;
+    }
+    __syncthreads();
+    for (int k = 0; k < 10; k++) acc[k] = 0.0;
+    for (int i = t; i < n; i += 256) {
+        double4 p = posa[i], q = qd[i], u = mu[i];
+        double x = p.x - com[0], y = p.y - com[1], z = p.z - com[2];
+        double c = q.w;
+        double nx = q.x + u.x, ny = q.y + u.y, nz = q.z + u.z;
+        acc[0] += c;
+        acc[1] += x * c + nx; acc[2] += y * c + ny; acc[3] += z * c + nz;
+        acc[4] += x * x * c + 2.0 * x * nx;
+        acc[5] += x * y * c + x * ny + y * nx;
+        acc[6] += x * z * c + x * nz + z * nx;
+        acc[7] += y * y * c + 2.0 * y * ny;
+        acc[8] += y * z * c + y * nz + z * ny;
+        acc[9] += z * z * c + 2.0 * z * nz;
+    }
+    for (int k = 0; k < 10; k++) s[t][k] = acc[k];
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (t < w)
+            for (int k = 0; k < 10; k++) s[t][k] += s[t + w][k];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 10; k++) out[k] = s[0][k];
+}
+
+}  // namespace
+
+// ====================================================================== launch wrappers
+namespace tdp {
+
+static BoxT64 mkbox(const double* b3) {
+    BoxT64 b;
+    b.bx = b3[0]; b.by = b3[1]; b.bz = b3[2];
+    b.ibx = b3[0] > 0 ? 1.0 / b3[0] : 0.0;
+    b.iby = b3[1] > 0 ? 1.0 / b3[1] : 0.0;
+    b.ibz = b3[2] > 0 ? 1.0 / b3[2] : 0.0;
+    return b;
+}
+
+void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const double* thole, double4* posa,
+                           float4* posa32) {
+    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, thole, posa, posa32);
+}
+
+void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
+                   const int4* axis, double4* qd) {
+    hipLaunchKernelGGL(k_rotate_dipoles, dim3((n + 255) / 256), dim3(256), 0, s, n, posa, dmol, charge, axis, qd);
+}
+
+void launch_fixed_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
+                        double4* part) {
+    dim3 grid(L.nblk, L.nsplit);
+    BoxT64 b = mkbox(L.box);
+    if (L.pbc)
+        hipLaunchKernelGGL(k_fixed_field<true>, grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd, inva,
+                           L.exc_start, L.exc_J, L.exc_mask, b, part);
+    else
+        hipLaunchKernelGGL(k_fixed_field<false>, grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd,
+                           inva, L.exc_start, L.exc_J, L.exc_mask, b, part);
+}
+
+void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
+                            const double* inva, const double* scalars, double4* part) {
+    dim3 grid(L.nblk, L.nsplit);
+    BoxT64 b = mkbox(L.box);
+    if (L.pbc)
+        hipLaunchKernelGGL((k_induced_field<double, true>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+    else
+        hipLaunchKernelGGL((k_induced_field<double, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+}
+
+void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
+                            const double* scalars, double4* part) {
+    dim3 grid(L.nblk, L.nsplit);
+    BoxT64 b = mkbox(L.box);
+    if (L.pbc)
+        hipLaunchKernelGGL((k_induced_field<float, true>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+    else
+        hipLaunchKernelGGL((k_induced_field<float, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+}
+
+void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
+                          const double4* mu, const double* inva, double4* part_f, double4* part_t, double* part_e) {
+    dim3 grid(L.nblk, L.nsplit);
+    BoxT64 b = mkbox(L.box);
+#define TDP_ELEC(P, M)                                                                                             \
+    hipLaunchKernelGGL((k_electrostatic<P, M>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd, \
+                       mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, part_f, part_t, part_e)
+    if (L.pbc) {
+        if (mutual) TDP_ELEC(true, true); else TDP_ELEC(true, false);
+    } else {
+        if (mutual) TDP_ELEC(false, true); else TDP_ELEC(false, false);
+    }
+#undef TDP_ELEC
+}
+
+void launch_field_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+                       double4* E0, double4* mu, float4* mu32) {
+    hipLaunchKernelGGL(k_field_init, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, nsplit, part, alpha, E0, mu,
+                       mu32);
+}
+
+int red_blocks(int npad) { return (npad + 255) / 256; }
+
+void launch_jacobi_update(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                          const double* alpha, double4* mu, float4* mu32, const double* scalars, double* red) {
+    int nb = red_blocks(npad);
+    hipLaunchKernelGGL(k_jacobi_update, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, mu, mu32, scalars,
+                       red, nb);
+    hipLaunchKernelGGL(k_jacobi_finalize, dim3(1), dim3(64), 0, s, n, nb, red, (double*)scalars);
+}
+
+void launch_pcg_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                     const double* alpha, const double4* x, double4* r, double4* p, float4* p32, double* red,
+                     double* scalars) {
+    int nb = red_blocks(npad);
+    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, x, r, p, p32, red, nb);
+    hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
+}
+
+void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+                               double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
+                               double* scalars) {
+    int nb = red_blocks(npad);
+    hipLaunchKernelGGL(k_pcg_q, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, alpha, p, q, scalars, red, nb);
+    hipLaunchKernelGGL(k_pcg_gamma, dim3(1), dim3(64), 0, s, nb, red, scalars);
+    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, s, n, alpha, p, q, x, r, z, scalars, red, nb);
+    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
+    hipLaunchKernelGGL(k_pcg_direction, dim3(nb), dim3(256), 0, s, n, npad, z, p, p32, scalars);
+}
+
+void launch_extrap_step(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                        const double* alpha, double4* mu, float4* mu32, double4* hist_o) {
+    hipLaunchKernelGGL(k_extrap_step, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, mu,
+                       mu32, hist_o);
+}
+
+void launch_extrap_combine(hipStream_t s, int n, int npad, int order, const double4* hist, const double* coef,
+                           double4* mu, float4* mu32) {
+    hipLaunchKernelGGL(k_extrap_combine, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, order, hist, coef, mu, mu32);
+}
+
+void launch_extrap_error(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+                         const double* alpha, const double4* mu, double* red, double* scalars) {
+    int nb = red_blocks(npad);
+    hipLaunchKernelGGL(k_extrap_error, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, mu, red, nb);
+    hipLaunchKernelGGL(k_sum_eps, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
+}
+
+void launch_reduce_forces(hipStream_t s, int n, int npad, int nsplit, const double4* part_f, const double4* part_t,
+                          double* frc, double4* torque) {
+    hipLaunchKernelGGL(k_reduce_forces, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, nsplit, part_f, part_t, frc,
+                       torque);
+}
+
+void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double* out) {
+    hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, s, count, part_e, out);
+}
+
+void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,
+                       double4* contrib) {
+    hipLaunchKernelGGL(k_map_torque, dim3((n + 255) / 256), dim3(256), 0, s, n, posa, axis, torque, contrib);
+}
+
+void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx, const double4* contrib, double* frc) {
+    hipLaunchKernelGGL(k_gather_torque, dim3((n + 255) / 256), dim3(256), 0, s, n, goff, gidx, contrib, frc);
+}
+
+void launch_accumulate(hipStream_t s, int m, const double* src, double* dst) {
+    hipLaunchKernelGGL(k_accumulate, dim3((m + 255) / 256), dim3(256), 0, s, m, src, dst);
+}
+
+void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const double4* posa, const double4* qd,
+                      const double4* mu, bool pbc, const double* box3, double* out) {
+    BoxT64 b = mkbox(box3);
+    dim3 g((ngrid + kTile - 1) / kTile);
+    if (pbc)
+        hipLaunchKernelGGL(k_potential<true>, g, dim3(kTile), 0, s, n, ngrid, grid, posa, qd, mu, b, out);
+    else
+        hipLaunchKernelGGL(k_potential<false>, g, dim3(kTile), 0, s, n, ngrid, grid, posa, qd, mu, b, out);
+}
+
+void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd, const double4* mu,
+                    const double* mass, double* out) {
+    hipLaunchKernelGGL(k_moments, dim3(1), dim3(256), 0, s, n, posa, qd, mu, mass, out);
+}
+
+}  // namespace tdp

@@ -55,6 +55,8 @@ def lib():
         L.tdpo_lab_dipoles.argtypes = [C.POINTER(_Sys), _dp, _dp]
         L.tdpo_map_torques.argtypes = [C.POINTER(_Sys), _dp, _dp, _dp]
         L.tdpo_set_threads.argtypes = [C.c_int]
+        L.tdpo_pass_slice.restype = C.c_double
+        L.tdpo_pass_slice.argtypes = [C.POINTER(_Sys), _dp, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.tdpo_rows.argtypes = [C.POINTER(_Sys), _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
@@ -211,3 +213,25 @@ def rows(force, positions, mu, row_idx, box=None, threads=8):
     _check(L.tdpo_rows(C.byref(os_.s), _p(pos), _p(m), _p(r, _ip), k, _p(e0), _p(ei), _p(f), _p(t), _p(en)))
     L.tdpo_set_threads(1)
     return dict(e0=e0.reshape(k, 3), eind=ei.reshape(k, 3), force=f.reshape(k, 3), torque=t.reshape(k, 3), energy=en)
+
+
+def time_pass_slices(force, positions, mu, box, which, slices, threads=1):
+    """Times reference pair-pass slices (rows [i0, i1), partners j > i); returns (seconds, pairs)."""
+    import time
+    L = lib()
+    L.tdpo_set_threads(int(threads))
+    os_ = OracleSystem(force, box)
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    m = np.ascontiguousarray(mu, np.float64).reshape(-1)
+    out = np.zeros(pos.size)
+    secs = 0.0
+    pairs = 0.0
+    for i0, i1 in slices:
+        t = time.perf_counter()
+        p = L.tdpo_pass_slice(C.byref(os_.s), _p(pos), _p(m), int(which), int(i0), int(i1), _p(out))
+        secs += time.perf_counter() - t
+        if p < 0:
+            raise RuntimeError("oracle rejected the input")
+        pairs += p
+    L.tdpo_set_threads(1)
+    return secs, pairs

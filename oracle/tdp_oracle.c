@@ -1000,3 +1000,60 @@ int tdpo_rows(const tdpo_system* s, const double* pos, const double* mu, const i
     free(p);
     return 0;
 }
+
+/* One slice of a reference pair pass: the outer-loop rows i in [i0, i1) with partners
+ * j in (i, n), exactly the loop bodies of calculateFixedDipoleField (which == 0),
+ * calculateInducedDipoleFields (which == 1) or calculateElectrostatic (which == 2)
+ * (ReferenceTholeDipoleForce.cpp:715-733, :782-818, :368-393).  Used to time the CPU baseline
+ * on a bounded sample of a large system; returns the number of pairs visited.  out accumulates
+ * the field or force (3n); mu is the induced-dipole input (3n). */
+double tdpo_pass_slice(const tdpo_system* s, const double* pos, const double* mu, int which, int i0, int i1,
+                       double* out) {
+    int n = s->n;
+    if (check_cov_lists(s)) return -1.0;
+    pdata* p = (pdata*)malloc(sizeof(pdata) * (size_t)(n > 0 ? n : 1));
+    if (load_and_rotate(s, pos, p)) { free(p); return -1.0; }
+    const double* box = s->method == TDPO_PME ? s->box : NULL;
+    double* tq = (double*)calloc(3 * (size_t)(n > 0 ? n : 1), sizeof(double));
+    double pairs = 0.0, energy = 0.0;
+    int nt = g_threads;
+    double* priv = (double*)calloc((size_t)nt * 3 * n, sizeof(double));
+    double* privt = (double*)calloc((size_t)nt * 3 * n, sizeof(double));
+#pragma omp parallel num_threads(nt) reduction(+ : pairs, energy)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        double* acc = priv + (size_t)t * 3 * n;
+        double* acct = privt + (size_t)t * 3 * n;
+#pragma omp for schedule(dynamic, 4)
+        for (int i = i0; i < i1; i++) {
+            for (int j = i + 1; j < n; j++) {
+                if (which == 0) {
+                    double m = lookup_scale(s, i, j, 0);
+                    fixed_field_pair(s, &p[i], &p[j], m, &acc[3 * i], &acc[3 * j]);
+                } else if (which == 1) {
+                    induced_pair(s, p, i, j, mu, &acc[3 * i], &acc[3 * j]);
+                } else {
+                    double m = lookup_scale(s, i, j, 0), is = lookup_scale(s, i, j, 1);
+                    double f[3], ti[3], tj[3];
+                    energy += pair_elec(p[i].pos, p[j].pos, p[i].q, p[j].q, p[i].d, p[j].d, &mu[3 * i], &mu[3 * j],
+                                        p[i].thole, p[j].thole, m, is, s->polarization, box, f, ti, tj);
+                    for (int k = 0; k < 3; k++) {
+                        acct[3 * i + k] += ti[k];
+                        acct[3 * j + k] += tj[k];
+                        acc[3 * i + k] -= f[k];
+                        acc[3 * j + k] += f[k];
+                    }
+                }
+                pairs += 1.0;
+            }
+        }
+    }
+    for (int t = 0; t < nt; t++)
+        for (int k = 0; k < 3 * n; k++) out[k] += priv[(size_t)t * 3 * n + k];
+    free(priv); free(privt); free(tq); free(p);
+    (void)energy;
+    return pairs;
+}

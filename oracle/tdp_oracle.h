@@ -86,6 +86,11 @@ int tdpo_pair_scales(const tdpo_system* s, int i, int j, double* m_scale, double
 int tdpo_rows(const tdpo_system* s, const double* pos, const double* mu, const int* rows, int nrows,
               double* e0_out, double* eind_out, double* force_out, double* torque_out, double* energy_out);
 
+/* Rows [i0, i1) of one reference pair pass (which: 0 fixed field, 1 T*mu, 2 electrostatics),
+ * partners j > i, for timing a bounded sample; returns the number of pairs visited. */
+double tdpo_pass_slice(const tdpo_system* s, const double* pos, const double* mu, int which, int i0, int i1,
+                       double* out);
+
 /* Single-pair building blocks, exported for the pinning tests. */
 void tdpo_damped_rr(double thole_i, double thole_j, double r, double* rr3, double* rr5);
 void tdpo_qi_frame(const double* pi, const double* pj, const double* delta, double r, double* rot9);

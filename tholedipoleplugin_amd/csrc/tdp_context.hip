@@ -24,12 +24,16 @@ namespace {
 
 thread_local std::string g_create_error;
 
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
+}  // namespace
+
+// A pair whose scales differ from 1, in the caller's indices (i < j).
+struct ExcPair {
+    int i, j;
+    double m, is;
 };
 
-}  // namespace
+// Evaluations between recomputations of the internal spatial order.
+constexpr int kReorderInterval = 100;
 
 struct tdp_context {
     // ---- parameters (host copies)
@@ -45,6 +49,11 @@ struct tdp_context {
     std::vector<double> charge, dmol, polarity, thole;
     std::vector<int> axis_type, atom_z, atom_x, atom_y;
     // ---- derived host tables
+    std::vector<ExcPair> exc_pairs;
+    std::vector<int> perm, inv;  // internal slot -> caller index, caller index -> internal slot
+    bool ordered = false;
+    int evals_since_order = 0;
+    size_t cap_exc_J = 0, cap_exc_mask = 0, cap_gidx = 0;
     std::vector<int> exc_start, exc_J;
     std::vector<unsigned long long> exc_mask;
     std::vector<int> goff, gidx;
@@ -60,7 +69,7 @@ struct tdp_context {
            *d_coef = nullptr, *d_energy = nullptr, *d_mass = nullptr, *d_out = nullptr;
     float* d_inva32 = nullptr;
     int4* d_axis = nullptr;
-    int *d_exc_start = nullptr, *d_exc_J = nullptr, *d_goff = nullptr, *d_gidx = nullptr;
+    int *d_exc_start = nullptr, *d_exc_J = nullptr, *d_goff = nullptr, *d_gidx = nullptr, *d_perm = nullptr;
     unsigned long long* d_exc_mask = nullptr;
     int hist_order = 0;
     size_t out_bytes = 0;
@@ -178,9 +187,9 @@ void copy_particles(tdp_context* c, const tdp_force_desc* d) {
 // setupScaleMaps/getScaleFactor (ReferenceTholeDipoleForce.cpp:609-644): pair (i<j) takes its
 // scales from particle i's own lists only: mScale from list 0 at the position of j, iScale from
 // list 1, the last occurrence of a duplicated atom winning.  Pairs whose scales are both 1 are
-// plain pairs; the rest, and the self pairs, become bits in the exclusion tiles.
-void build_exclusion_tiles(tdp_context* c, const tdp_force_desc* d) {
-    const int n = c->n, nblk = c->nblk;
+// plain pairs; the rest are kept (in the caller's indices) for the exclusion tiles.
+void build_exclusion_pairs(tdp_context* c, const tdp_force_desc* d) {
+    const int n = c->n;
     std::unordered_map<long long, std::pair<double, double>> pairs;
     auto key = [n](int i, int j) { return (long long)i * n + j; };
     for (int i = 0; i < n; i++) {
@@ -196,9 +205,20 @@ void build_exclusion_tiles(tdp_context* c, const tdp_force_desc* d) {
             }
         }
     }
-    // tiles: (I,J) -> row masks
+    c->exc_pairs.clear();
+    for (auto& kv : pairs) {
+        if (kv.second.first == 1.0 && kv.second.second == 1.0) continue;
+        c->exc_pairs.push_back({(int)(kv.first / n), (int)(kv.first % n), kv.second.first, kv.second.second});
+    }
+    std::sort(c->exc_pairs.begin(), c->exc_pairs.end(),
+              [](const ExcPair& a, const ExcPair& b) { return a.i != b.i ? a.i < b.i : a.j < b.j; });
+}
+
+// Exclusion tiles in the internal order: for a tile pair (I,J) holding a scaled pair, three
+// 64-bit masks per row mark mScale == 0, mScale == 0.5 and iScale == 0; self pairs set all three.
+void build_tiles(tdp_context* c) {
+    const int n = c->n, nblk = c->nblk;
     std::vector<std::unordered_map<int, int>> tile_of(nblk);
-    std::vector<std::vector<int>> tiles_J(nblk);
     std::vector<unsigned long long> masks;  // [tile][3][64]
     std::vector<std::pair<int, int>> tile_ij;
     auto tile = [&](int I, int J) {
@@ -218,15 +238,12 @@ void build_exclusion_tiles(tdp_context* c, const tdp_force_desc* d) {
         if (m == 0.5) masks[((size_t)t * 3 + 1) * 64 + row] |= bit;
         if (is == 0.0) masks[((size_t)t * 3 + 2) * 64 + row] |= bit;
     };
-    for (int i = 0; i < n; i++) setbits(i, i, 0.0, 0.0);
-    for (auto& kv : pairs) {
-        double m = kv.second.first, is = kv.second.second;
-        if (m == 1.0 && is == 1.0) continue;
-        int i = (int)(kv.first / n), j = (int)(kv.first % n);
-        setbits(i, j, m, is);
-        setbits(j, i, m, is);
+    for (int k = 0; k < n; k++) setbits(k, k, 0.0, 0.0);
+    for (const ExcPair& e : c->exc_pairs) {
+        int a = c->inv[e.i], b = c->inv[e.j];
+        setbits(a, b, e.m, e.is);
+        setbits(b, a, e.m, e.is);
     }
-    // order by (I, J)
     std::vector<int> order(tile_ij.size());
     for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
     std::sort(order.begin(), order.end(), [&](int a, int b) { return tile_ij[a] < tile_ij[b]; });
@@ -244,7 +261,8 @@ void build_exclusion_tiles(tdp_context* c, const tdp_force_desc* d) {
 
 // Gather lists for mapTorqueToForce (ReferenceTholeDipoleForce.cpp:1092-1115): each particle's
 // contributions (to itself and its axis atoms) are summed in the reference's order: particle
-// index ascending, and within one particle U, V, W, then I.
+// index ascending, and within one particle U, V, W, then I.  contrib slots and receiving atoms
+// are internal indices.
 void build_torque_gather(tdp_context* c) {
     const int n = c->n;
     struct Entry { int atom, ii, role, slot; };
@@ -252,10 +270,11 @@ void build_torque_gather(tdp_context* c) {
     for (int ii = 0; ii < n; ii++) {
         int at = c->axis_type[ii];
         if (at == 5) continue;
-        es.push_back({c->atom_z[ii], ii, 0, 4 * ii + 1});
-        if (at != 4) es.push_back({c->atom_x[ii], ii, 1, 4 * ii + 2});
-        if ((at == 2 || at == 3) && c->atom_y[ii] > -1) es.push_back({c->atom_y[ii], ii, 2, 4 * ii + 3});
-        es.push_back({ii, ii, 3, 4 * ii + 0});
+        int k = c->inv[ii];
+        es.push_back({c->inv[c->atom_z[ii]], ii, 0, 4 * k + 1});
+        if (at != 4) es.push_back({c->inv[c->atom_x[ii]], ii, 1, 4 * k + 2});
+        if ((at == 2 || at == 3) && c->atom_y[ii] > -1) es.push_back({c->inv[c->atom_y[ii]], ii, 2, 4 * k + 3});
+        es.push_back({k, ii, 3, 4 * k + 0});
     }
     std::stable_sort(es.begin(), es.end(), [](const Entry& a, const Entry& b) {
         if (a.atom != b.atom) return a.atom < b.atom;
@@ -271,28 +290,132 @@ void build_torque_gather(tdp_context* c) {
     for (int a = 0; a < n; a++) c->goff[a + 1] += c->goff[a];
 }
 
+// Replaces a device buffer with one of at least count elements.
+template <typename T>
+int dresize(tdp_context* c, T** ptr, size_t& cap, size_t count) {
+    if (*ptr && count <= cap) return TDP_OK;
+    if (*ptr) {
+        auto it = std::find(c->allocations.begin(), c->allocations.end(), (void*)*ptr);
+        if (it != c->allocations.end()) c->allocations.erase(it);
+        TDP_HIP(c, hipFree(*ptr));
+        *ptr = nullptr;
+    }
+    int rc = dalloc(c, ptr, std::max<size_t>(count, 1));
+    if (rc) return rc;
+    cap = std::max<size_t>(count, 1);
+    return TDP_OK;
+}
+
+// Per-particle parameters in the internal order, with axis atoms renumbered.
 int upload_particles(tdp_context* c) {
     const int n = c->n, npad = c->npad;
-    std::vector<double> alpha(npad, 0.0), inva(npad, 0.0), thole(npad, 0.0), charge(npad, 0.0);
+    std::vector<double> alpha(npad, 0.0), inva(npad, 0.0), thole(npad, 0.0), charge(npad, 0.0), dmol(3 * npad, 0.0);
     std::vector<float> inva32(npad, 0.f);
-    std::vector<int4> axis(std::max(n, 1));
-    for (int i = 0; i < n; i++) {
-        alpha[i] = c->polarity[i];
-        thole[i] = c->thole[i];
-        charge[i] = c->charge[i];
-        inva[i] = c->thole[i] != 0.0 ? 1.0 / c->thole[i] : 0.0;
-        inva32[i] = (float)inva[i];
-        axis[i] = make_int4(c->axis_type[i], c->atom_z[i], c->atom_x[i], c->atom_y[i]);
+    std::vector<int4> axis(npad, make_int4(5, -1, -1, -1));
+    std::vector<int> perm(npad, 0x7fffffff);
+    auto ren = [c](int a) { return a >= 0 ? c->inv[a] : a; };
+    for (int k = 0; k < n; k++) {
+        int i = c->perm[k];
+        perm[k] = i;
+        alpha[k] = c->polarity[i];
+        thole[k] = c->thole[i];
+        charge[k] = c->charge[i];
+        inva[k] = c->thole[i] != 0.0 ? 1.0 / c->thole[i] : 0.0;
+        inva32[k] = (float)inva[k];
+        for (int d = 0; d < 3; d++) dmol[3 * k + d] = c->dmol[3 * i + d];
+        axis[k] = make_int4(c->axis_type[i], ren(c->atom_z[i]), ren(c->atom_x[i]), ren(c->atom_y[i]));
     }
     TDP_HIP(c, hipMemcpy(c->d_alpha, alpha.data(), npad * sizeof(double), hipMemcpyHostToDevice));
     TDP_HIP(c, hipMemcpy(c->d_inva, inva.data(), npad * sizeof(double), hipMemcpyHostToDevice));
     TDP_HIP(c, hipMemcpy(c->d_inva32, inva32.data(), npad * sizeof(float), hipMemcpyHostToDevice));
     TDP_HIP(c, hipMemcpy(c->d_thole, thole.data(), npad * sizeof(double), hipMemcpyHostToDevice));
     TDP_HIP(c, hipMemcpy(c->d_charge, charge.data(), npad * sizeof(double), hipMemcpyHostToDevice));
-    if (n > 0) {
-        TDP_HIP(c, hipMemcpy(c->d_dmol, c->dmol.data(), 3 * n * sizeof(double), hipMemcpyHostToDevice));
-        TDP_HIP(c, hipMemcpy(c->d_axis, axis.data(), n * sizeof(int4), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_dmol, dmol.data(), 3 * npad * sizeof(double), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_axis, axis.data(), npad * sizeof(int4), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpy(c->d_perm, perm.data(), npad * sizeof(int), hipMemcpyHostToDevice));
+    return TDP_OK;
+}
+
+// Exclusion tiles and torque-gather lists of the current order.
+int upload_tables(tdp_context* c) {
+    build_tiles(c);
+    build_torque_gather(c);
+    const size_t ntiles = c->exc_J.size();
+    int rc;
+    if ((rc = dresize(c, &c->d_exc_J, c->cap_exc_J, ntiles)) ||
+        (rc = dresize(c, &c->d_exc_mask, c->cap_exc_mask, ntiles * 3 * 64)) ||
+        (rc = dresize(c, &c->d_gidx, c->cap_gidx, c->gidx.size())))
+        return rc;
+    TDP_HIP(c, hipMemcpy(c->d_exc_start, c->exc_start.data(), c->exc_start.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (ntiles) {
+        TDP_HIP(c, hipMemcpy(c->d_exc_J, c->exc_J.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
+        TDP_HIP(c, hipMemcpy(c->d_exc_mask, c->exc_mask.data(), c->exc_mask.size() * sizeof(unsigned long long),
+                             hipMemcpyHostToDevice));
     }
+    TDP_HIP(c, hipMemcpy(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice));
+    if (!c->gidx.empty())
+        TDP_HIP(c, hipMemcpy(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice));
+    return TDP_OK;
+}
+
+// Internal spatial order: particles sorted by the Morton (Z-order) code of their position on a
+// 1024^3 grid over the box (PME: wrapped into the box) or the bounding box (NoCutoff), ties by
+// index.  Tiles of 64 consecutive particles are then compact, so a j tile is either near or far
+// from a whole i tile: the Thole exponential and the masked exclusion path run for few tiles.
+// The order only changes which thread evaluates which pair; results are returned in the caller's order.
+void compute_order(tdp_context* c, const double* pos3, const double* box) {
+    const int n = c->n;
+    double lo[3], span[3];
+    const bool pbc = c->method == TDP_PME && box;
+    for (int d = 0; d < 3; d++) {
+        if (pbc) {
+            lo[d] = 0.0;
+            span[d] = box[4 * d];
+        } else {
+            double a = 1e300, b = -1e300;
+            for (int i = 0; i < n; i++) {
+                a = std::min(a, pos3[3 * i + d]);
+                b = std::max(b, pos3[3 * i + d]);
+            }
+            lo[d] = a;
+            span[d] = b - a;
+        }
+        if (!(span[d] > 0.0)) span[d] = 1.0;
+    }
+    auto spread = [](unsigned v) {
+        unsigned long long x = v & 0x3ff;
+        x = (x | (x << 16)) & 0x30000ffull;
+        x = (x | (x << 8)) & 0x300f00full;
+        x = (x | (x << 4)) & 0x30c30c3ull;
+        x = (x | (x << 2)) & 0x9249249ull;
+        return x;
+    };
+    std::vector<std::pair<unsigned long long, int>> key(n);
+    for (int i = 0; i < n; i++) {
+        unsigned q[3];
+        for (int d = 0; d < 3; d++) {
+            double u = (pos3[3 * i + d] - lo[d]) / span[d];
+            if (pbc) u -= std::floor(u);
+            if (!(u >= 0.0)) u = 0.0;   // also catches NaN
+            q[d] = (unsigned)std::min(1023.0, u * 1024.0);
+        }
+        key[i] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), i};
+    }
+    std::sort(key.begin(), key.end());
+    for (int k = 0; k < n; k++) {
+        c->perm[k] = key[k].second;
+        c->inv[key[k].second] = k;
+    }
+}
+
+int reorder(tdp_context* c, const double* host_pos3, const double* box) {
+    compute_order(c, host_pos3, box);
+    int rc = upload_particles(c);
+    if (rc) return rc;
+    rc = upload_tables(c);
+    if (rc) return rc;
+    c->ordered = true;
+    c->evals_since_order = 0;
     return TDP_OK;
 }
 
@@ -365,8 +488,9 @@ int check_box(tdp_context* c, const double* box) {
 }
 
 // setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764) from
-// positions already in d_pos3.
-int setup_and_solve(tdp_context* c, const double* box) {
+// positions already in d_pos3 (caller's order).  host_pos3 (may be NULL) are the same positions
+// on the host, used when the internal order is recomputed.
+int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) {
     if (c->n == 0) {
         c->iterations = 0;
         c->converged = 1;
@@ -375,9 +499,22 @@ int setup_and_solve(tdp_context* c, const double* box) {
     }
     hipStream_t s = c->stream;
     const int n = c->n, npad = c->npad;
+    if (!c->ordered || c->evals_since_order >= kReorderInterval) {
+        std::vector<double> tmp;
+        const double* hp = host_pos3;
+        if (!hp) {
+            tmp.resize(3 * (size_t)n);
+            TDP_HIP(c, hipMemcpyAsync(tmp.data(), c->d_pos3, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+            hp = tmp.data();
+        }
+        TDP_HIP(c, hipStreamSynchronize(s));  // no kernel may still read the tables being replaced
+        int rc = reorder(c, hp, box);
+        if (rc) return rc;
+    }
+    c->evals_since_order++;
     PairLaunch L = pair_launch(c, box);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
-    launch_load_positions(s, n, npad, c->d_pos3, c->d_thole, c->posa, mixed ? c->posa32 : nullptr);
+    launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa, mixed ? c->posa32 : nullptr);
     launch_rotate(s, n, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     launch_fixed_field(s, L, c->posa, c->qd, c->d_inva, c->part);
     launch_field_init(s, n, npad, c->nsplit, c->part, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
@@ -476,8 +613,8 @@ void forces_and_energy(tdp_context* c, const double* box) {
     }
     const int n = c->n, npad = c->npad;
     PairLaunch L = pair_launch(c, box);
-    launch_electrostatic(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->part_f, c->part_t,
-                         c->d_part_e);
+    launch_electrostatic(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->d_perm, c->part_f,
+                         c->part_t, c->d_part_e);
     launch_reduce_forces(s, n, npad, c->nsplit, c->part_f, c->part_t, c->d_frc, c->torque);
     launch_reduce_energy(s, c->nsplit * c->nblk, c->d_part_e, c->d_energy);
     if (!c->gidx.empty()) {
@@ -565,14 +702,16 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
                              d->extrapolation_coefficients + d->num_extrapolation_coefficients);
     }
     copy_particles(c, d);
-    build_exclusion_tiles(c, d);
-    build_torque_gather(c);
+    build_exclusion_pairs(c, d);
+    // identity order until the first evaluation computes the spatial one
+    c->perm.resize(c->n);
+    c->inv.resize(c->n);
+    for (int i = 0; i < c->n; i++) c->perm[i] = c->inv[i] = i;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(TDP_ERR_HIP);
     }
     const int npad = c->npad, ns = c->nsplit;
-    const size_t ntiles = c->exc_J.size();
     if ((rc = dalloc(c, &c->posa, npad)) || (rc = dalloc(c, &c->qd, npad)) || (rc = dalloc(c, &c->mu, npad)) ||
         (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
         (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||
@@ -586,26 +725,14 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)red_blocks(npad))) ||
         (rc = dalloc(c, &c->d_scalars, SC_COUNT)) || (rc = dalloc(c, &c->d_part_e, (size_t)ns * c->nblk + 1)) ||
         (rc = dalloc(c, &c->d_coef, 64)) || (rc = dalloc(c, &c->d_energy, 1)) || (rc = dalloc(c, &c->d_axis, npad)) ||
-        (rc = dalloc(c, &c->d_exc_start, c->nblk + 1)) || (rc = dalloc(c, &c->d_exc_J, ntiles)) ||
-        (rc = dalloc(c, &c->d_exc_mask, ntiles * 3 * 64)) || (rc = dalloc(c, &c->d_goff, c->n + 1)) ||
-        (rc = dalloc(c, &c->d_gidx, c->gidx.size())) || (rc = dalloc(c, &c->d_mass, npad)))
+        (rc = dalloc(c, &c->d_exc_start, c->nblk + 1)) || (rc = dalloc(c, &c->d_goff, c->n + 1)) ||
+        (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
         c->hist_order = std::max(4, (int)c->extrap.size());
         if ((rc = dalloc(c, &c->hist, (size_t)c->hist_order * npad))) return bail(rc);
     }
-    if (hipMemcpy(c->d_exc_start, c->exc_start.data(), c->exc_start.size() * sizeof(int), hipMemcpyHostToDevice) !=
-            hipSuccess ||
-        (ntiles && hipMemcpy(c->d_exc_J, c->exc_J.data(), ntiles * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) ||
-        (ntiles && hipMemcpy(c->d_exc_mask, c->exc_mask.data(), c->exc_mask.size() * sizeof(unsigned long long),
-                             hipMemcpyHostToDevice) != hipSuccess) ||
-        hipMemcpy(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-        (!c->gidx.empty() &&
-         hipMemcpy(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
-        c->err = "hipMemcpy of the exclusion tables failed";
-        return bail(TDP_ERR_HIP);
-    }
-    if ((rc = upload_particles(c))) return bail(rc);
+    if ((rc = upload_tables(c)) || (rc = upload_particles(c))) return bail(rc);
     *out = c;
     return TDP_OK;
 }
@@ -629,23 +756,12 @@ int tdp_copy_parameters(tdp_context* c, const tdp_force_desc* d) {
     if (d->num_particles != c->n)
         return fail(c, TDP_ERR_PARTICLE_COUNT, "updateParametersInContext: The number of particles has changed");
     // Only the per-particle parameters are copied (ReferenceTholeDipoleKernels.cpp:317-339).
-    tdp_force_desc v = *d;
-    int rc = validate(c, &v);
+    int rc = validate(c, d);
     if (rc) return rc;
-    hipSetDevice(c->device);
-    copy_particles(c, d);
-    build_torque_gather(c);
+    TDP_HIP(c, hipSetDevice(c->device));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
-    if (c->d_gidx) {
-        auto it = std::find(c->allocations.begin(), c->allocations.end(), (void*)c->d_gidx);
-        if (it != c->allocations.end()) c->allocations.erase(it);
-        hipFree(c->d_gidx);
-        c->d_gidx = nullptr;
-    }
-    if ((rc = dalloc(c, &c->d_gidx, c->gidx.size()))) return rc;
-    TDP_HIP(c, hipMemcpy(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice));
-    if (!c->gidx.empty())
-        TDP_HIP(c, hipMemcpy(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice));
+    copy_particles(c, d);
+    if ((rc = upload_tables(c))) return rc;  // axis atoms may have changed: torque gather lists
     return upload_particles(c);
 }
 
@@ -656,7 +772,7 @@ int tdp_execute(tdp_context* c, const double* positions, const double* box, int3
     if (rc) return rc;
     hipSetDevice(c->device);
     if ((rc = upload_positions(c, positions))) return rc;
-    if ((rc = setup_and_solve(c, box))) return rc;
+    if ((rc = setup_and_solve(c, box, positions))) return rc;
     forces_and_energy(c, box);
     const int n = c->n;
     std::vector<double> f(3 * (size_t)n);
@@ -665,7 +781,8 @@ int tdp_execute(tdp_context* c, const double* positions, const double* box, int3
     TDP_HIP(c, hipMemcpyAsync(&e, c->d_energy, sizeof(double), hipMemcpyDeviceToHost, c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
     if (forces && include_forces)
-        for (size_t k = 0; k < f.size(); k++) forces[k] += f[k];
+        for (int k = 0; k < n; k++)
+            for (int d = 0; d < 3; d++) forces[3 * c->perm[k] + d] += f[3 * k + d];
     if (energy) *energy = e;
     (void)include_energy;  // both are always computed, as in the reference (ReferenceTholeDipoleKernels.cpp:201-215)
     return TDP_OK;
@@ -684,12 +801,12 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
     TDP_HIP(c, hipStreamWaitEvent(c->stream, ev, 0));
     if (c->n > 0)
         TDP_HIP(c, hipMemcpyAsync(c->d_pos3, d_positions, 3 * c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = setup_and_solve(c, box))) {
+    if ((rc = setup_and_solve(c, box, nullptr))) {
         c->ev_pool.push_back(ev);
         return rc;
     }
     forces_and_energy(c, box);
-    if (d_forces && c->n > 0) launch_accumulate(c->stream, 3 * c->n, c->d_frc, d_forces);
+    if (d_forces && c->n > 0) launch_scatter_accumulate(c->stream, c->n, c->d_perm, c->d_frc, d_forces);
     if (d_energy) TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     TDP_HIP(c, hipEventRecord(ev, c->stream));
     TDP_HIP(c, hipStreamWaitEvent(user, ev, 0));
@@ -704,7 +821,7 @@ int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, 
     if (rc) return rc;
     hipSetDevice(c->device);
     if ((rc = upload_positions(c, positions))) return rc;
-    if ((rc = setup_and_solve(c, box))) return rc;
+    if ((rc = setup_and_solve(c, box, positions))) return rc;
     const int n = c->n;
     std::vector<double4> mu(n), qd(n);
     if (n > 0) {
@@ -714,8 +831,9 @@ int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, 
     TDP_HIP(c, hipStreamSynchronize(c->stream));
     for (int i = 0; i < n; i++) {
         double a[3] = {mu[i].x, mu[i].y, mu[i].z}, b[3] = {qd[i].x, qd[i].y, qd[i].z};
+        int o = c->perm[i];
         for (int k = 0; k < 3; k++)
-            out[3 * i + k] = kind == TDP_DIPOLES_INDUCED ? a[k] : (kind == TDP_DIPOLES_LAB_PERMANENT ? b[k] : b[k] + a[k]);
+            out[3 * o + k] = kind == TDP_DIPOLES_INDUCED ? a[k] : (kind == TDP_DIPOLES_LAB_PERMANENT ? b[k] : b[k] + a[k]);
     }
     return TDP_OK;
 }
@@ -728,7 +846,7 @@ int tdp_get_electrostatic_potential(tdp_context* c, const double* positions, con
     if (rc) return rc;
     hipSetDevice(c->device);
     if ((rc = upload_positions(c, positions))) return rc;
-    if ((rc = setup_and_solve(c, box))) return rc;
+    if ((rc = setup_and_solve(c, box, positions))) return rc;
     if (num_grid == 0) return TDP_OK;
     size_t gb = (size_t)num_grid * 3 * sizeof(double);
     if ((rc = ensure_out(c, gb + num_grid * sizeof(double)))) return rc;
@@ -749,8 +867,11 @@ int tdp_get_system_multipole_moments(tdp_context* c, const double* positions, co
     if (rc) return rc;
     hipSetDevice(c->device);
     if ((rc = upload_positions(c, positions))) return rc;
-    if ((rc = setup_and_solve(c, box))) return rc;
-    if (c->n > 0) TDP_HIP(c, hipMemcpyAsync(c->d_mass, masses, c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if ((rc = setup_and_solve(c, box, positions))) return rc;
+    std::vector<double> mperm(c->n);
+    for (int k = 0; k < c->n; k++) mperm[k] = masses[c->perm[k]];
+    if (c->n > 0)
+        TDP_HIP(c, hipMemcpyAsync(c->d_mass, mperm.data(), c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
     if ((rc = ensure_out(c, 16 * sizeof(double)))) return rc;
     launch_moments(c->stream, c->n, c->posa, c->qd, c->mu, c->d_mass, c->d_out);
     double raw[10];

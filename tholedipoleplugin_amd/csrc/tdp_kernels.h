@@ -28,8 +28,8 @@ struct PairLaunch {
     const unsigned long long* exc_mask;   // [ntiles][3][64]
 };
 
-void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const double* thole, double4* posa,
-                           float4* posa32);
+void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
+                           double4* posa, float4* posa32);
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
                    const int4* axis, double4* qd);
 void launch_fixed_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
@@ -39,7 +39,8 @@ void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* p
 void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
                             const double* scalars, double4* part);
 void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
-                          const double4* mu, const double* inva, double4* part_f, double4* part_t, double* part_e);
+                          const double4* mu, const double* inva, const int* orig, double4* part_f, double4* part_t,
+                          double* part_e);
 void launch_field_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
                        double4* E0, double4* mu, float4* mu32);
 int red_blocks(int npad);
@@ -63,7 +64,7 @@ void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,
                        double4* contrib);
 void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx, const double4* contrib, double* frc);
-void launch_accumulate(hipStream_t s, int m, const double* src, double* dst);
+void launch_scatter_accumulate(hipStream_t s, int n, const int* perm, const double* src, double* dst);
 void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const double4* posa, const double4* qd,
                       const double4* mu, bool pbc, const double* box3, double* out);
 void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd, const double4* mu,

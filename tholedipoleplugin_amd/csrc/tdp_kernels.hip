@@ -95,12 +95,18 @@ __global__ void k_rotate_dipoles(int n, const double4* __restrict__ posa, const 
 }
 
 // ------------------------------------------------------------------ positions
-__global__ void k_load_positions(int n, int npad, const double* __restrict__ pos3, const double* __restrict__ thole,
-                                 double4* __restrict__ posa, float4* __restrict__ posa32) {
+// Gathers the caller's positions (original order) into the internal spatial order:
+// slot i holds particle perm[i].
+__global__ void k_load_positions(int n, int npad, const double* __restrict__ pos3, const int* __restrict__ perm,
+                                 const double* __restrict__ thole, double4* __restrict__ posa,
+                                 float4* __restrict__ posa32) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
     double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (i < n) p = make_double4(pos3[3 * i], pos3[3 * i + 1], pos3[3 * i + 2], thole[i]);
+    if (i < n) {
+        int o = perm[i];
+        p = make_double4(pos3[3 * o], pos3[3 * o + 1], pos3[3 * o + 2], thole[i]);
+    }
     posa[i] = p;
     if (posa32) posa32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
 }
@@ -237,8 +243,8 @@ __global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles
 
 // ------------------------------------------------------------------ energy, forces, torques (a9)
 // calculateElectrostatic (ReferenceTholeDipoleForce.cpp:368-393).  Every lane evaluates all its
-// pairs in the canonical orientation (lower index = I); tiles J > I make the lane I, tiles J < I
-// make it K, the diagonal tile decides per lane.  The pair energy is counted by the I side.
+// pairs in the reference's orientation: the particle with the lower ORIGINAL index is I (orig[]
+// maps internal slots back to the caller's indices).  The pair energy is counted by the I side.
 __device__ __forceinline__ void load_site(ElecSite& s, double4 p, double4 q, double4 u, double ia) {
     s.x = p.x; s.y = p.y; s.z = p.z; s.a = p.w; s.ia = ia;
     s.dx = q.x; s.dy = q.y; s.dz = q.z; s.q = q.w;
@@ -249,7 +255,8 @@ template <bool PBC, bool MUTUAL>
 __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles_per_split,
                                                       const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                       const double4* __restrict__ mu, const double* __restrict__ inva,
-                                                      const int* __restrict__ exc_start, const int* __restrict__ exc_J,
+                                                      const int* __restrict__ orig, const int* __restrict__ exc_start,
+                                                      const int* __restrict__ exc_J,
                                                       const unsigned long long* __restrict__ exc_mask, BoxT64 box,
                                                       double4* __restrict__ part_f, double4* __restrict__ part_t,
                                                       double* __restrict__ part_e) {
@@ -257,6 +264,7 @@ __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles
     __shared__ double4 s_q[kTile];
     __shared__ double4 s_u[kTile];
     __shared__ double s_ia[kTile];
+    __shared__ int s_o[kTile];
     const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
     const int i = I * kTile + lane;
     const int nblk = (n + kTile - 1) / kTile;
@@ -264,6 +272,7 @@ __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles
     const int J1 = min(nblk, J0 + tiles_per_split);
     ElecSite me;
     load_site(me, posa[i], qd[i], mu[i], inva[i]);
+    const int my_orig = orig[i];
     double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, en = 0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
@@ -275,6 +284,7 @@ __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles
         s_q[lane] = qd[jb + lane];
         s_u[lane] = mu[jb + lane];
         s_ia[lane] = inva[jb + lane];
+        s_o[lane] = orig[jb + lane];
         __syncthreads();
         const int jn = min(kTile, n - jb);
         const bool masked = (e < eEnd && exc_J[e] == J);
@@ -295,8 +305,7 @@ __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles
             }
             ElecSite other;
             load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
-            const int j = jb + jj;
-            const bool lo = i < j;  // uniform except on the diagonal tile
+            const bool lo = my_orig < s_o[jj];
             const ElecSite& A = lo ? me : other;
             const ElecSite& B = lo ? other : me;
             double dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
@@ -763,10 +772,16 @@ __global__ void k_gather_torque(int n, const int* __restrict__ goff, const int* 
     frc[3 * a + 2] += z;
 }
 
-// out[i] += frc[i]  (forces are accumulated into the caller's array)
-__global__ void k_accumulate(int m, const double* __restrict__ src, double* __restrict__ dst) {
+// dst[perm[i]] += src[i] per component: forces in the internal order are accumulated into the
+// caller's array in its own order.
+__global__ void k_scatter_accumulate(int n, const int* __restrict__ perm, const double* __restrict__ src,
+                                     double* __restrict__ dst) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < m) dst[k] += src[k];
+    if (k >= n) return;
+    int o = perm[k];
+    dst[3 * o] += src[3 * k];
+    dst[3 * o + 1] += src[3 * k + 1];
+    dst[3 * o + 2] += src[3 * k + 2];
 }
 
 // ------------------------------------------------------------------ potential (next row)
@@ -878,9 +893,10 @@ static BoxT64 mkbox(const double* b3) {
     return b;
 }
 
-void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const double* thole, double4* posa,
-                           float4* posa32) {
-    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, thole, posa, posa32);
+void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
+                           double4* posa, float4* posa32) {
+    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, perm, thole, posa,
+                       posa32);
 }
 
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
@@ -925,12 +941,13 @@ void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* po
 }
 
 void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
-                          const double4* mu, const double* inva, double4* part_f, double4* part_t, double* part_e) {
+                          const double4* mu, const double* inva, const int* orig, double4* part_f, double4* part_t,
+                          double* part_e) {
     dim3 grid(L.nblk, L.nsplit);
     BoxT64 b = mkbox(L.box);
 #define TDP_ELEC(P, M)                                                                                             \
     hipLaunchKernelGGL((k_electrostatic<P, M>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd, \
-                       mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, part_f, part_t, part_e)
+                       mu, inva, orig, L.exc_start, L.exc_J, L.exc_mask, b, part_f, part_t, part_e)
     if (L.pbc) {
         if (mutual) TDP_ELEC(true, true); else TDP_ELEC(true, false);
     } else {
@@ -1011,8 +1028,8 @@ void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx
     hipLaunchKernelGGL(k_gather_torque, dim3((n + 255) / 256), dim3(256), 0, s, n, goff, gidx, contrib, frc);
 }
 
-void launch_accumulate(hipStream_t s, int m, const double* src, double* dst) {
-    hipLaunchKernelGGL(k_accumulate, dim3((m + 255) / 256), dim3(256), 0, s, m, src, dst);
+void launch_scatter_accumulate(hipStream_t s, int n, const int* perm, const double* src, double* dst) {
+    hipLaunchKernelGGL(k_scatter_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, src, dst);
 }
 
 void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const double4* posa, const double4* qd,

@@ -63,7 +63,7 @@ struct tdp_context {
     double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
             *q = nullptr, *part = nullptr, *part_f = nullptr, *part_t = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
-    float4 *posa32 = nullptr, *vec32 = nullptr;
+    float4 *posa32 = nullptr, *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr;
     double *d_pos3 = nullptr, *d_frc = nullptr, *d_alpha = nullptr, *d_inva = nullptr, *d_thole = nullptr,
            *d_charge = nullptr, *d_dmol = nullptr, *d_red = nullptr, *d_scalars = nullptr, *d_part_e = nullptr,
            *d_coef = nullptr, *d_energy = nullptr, *d_mass = nullptr, *d_out = nullptr;
@@ -433,6 +433,8 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.exc_start = c->d_exc_start;
     L.exc_J = c->d_exc_J;
     L.exc_mask = c->d_exc_mask;
+    L.tb_c = c->tb_c;
+    L.tb_h = c->tb_h;
     return L;
 }
 
@@ -515,6 +517,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
     PairLaunch L = pair_launch(c, box);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa, mixed ? c->posa32 : nullptr);
+    launch_tile_bounds(s, n, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h);
     launch_rotate(s, n, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     launch_fixed_field(s, L, c->posa, c->qd, c->d_inva, c->part);
     launch_field_init(s, n, npad, c->nsplit, c->part, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
@@ -680,9 +683,10 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
     c->n = d->num_particles;
     c->npad = std::max(64, (c->n + 63) / 64 * 64);
     c->nblk = (c->n + 63) / 64;
-    // Split the j range so that the pair kernels launch >= ~2048 workgroups (8 per CU).
+    // Split the j range so that the pair kernels launch >= ~8192 one-wave workgroups
+    // (8 waves per SIMD on 256 CUs) when the system is large enough.
     int nb = std::max(c->nblk, 1);
-    int want = std::max(1, std::min(nb, (2048 + nb - 1) / nb));
+    int want = std::max(1, std::min(nb, (8192 + nb - 1) / nb));
     c->tiles_per_split = (nb + want - 1) / want;
     c->nsplit = (nb + c->tiles_per_split - 1) / c->tiles_per_split;
     c->method = d->nonbonded_method;
@@ -726,7 +730,8 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         (rc = dalloc(c, &c->d_scalars, SC_COUNT)) || (rc = dalloc(c, &c->d_part_e, (size_t)ns * c->nblk + 1)) ||
         (rc = dalloc(c, &c->d_coef, 64)) || (rc = dalloc(c, &c->d_energy, 1)) || (rc = dalloc(c, &c->d_axis, npad)) ||
         (rc = dalloc(c, &c->d_exc_start, c->nblk + 1)) || (rc = dalloc(c, &c->d_goff, c->n + 1)) ||
-        (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)))
+        (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
+        (rc = dalloc(c, &c->tb_c, c->nblk + 1)) || (rc = dalloc(c, &c->tb_h, c->nblk + 1)))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
         c->hist_order = std::max(4, (int)c->extrap.size());

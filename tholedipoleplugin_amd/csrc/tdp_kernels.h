@@ -26,7 +26,12 @@ struct PairLaunch {
     const int* exc_start;                 // [nblk+1]
     const int* exc_J;                     // [ntiles]
     const unsigned long long* exc_mask;   // [ntiles][3][64]
+    const float4* tb_c;                   // [nblk] tile bounding-box centre, largest Thole parameter
+    const float4* tb_h;                   // [nblk] tile bounding-box half extent, smallest Thole parameter
 };
+
+void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
+                        float4* tb_c, float4* tb_h);
 
 void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
                            double4* posa, float4* posa32);

@@ -111,6 +111,65 @@ __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos
     if (posa32) posa32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
 }
 
+// ------------------------------------------------------------------ tile classification
+// Per tile of 64 particles (internal order): centre and half extent of its bounding box
+// (PBC: offsets from the tile's first particle wrapped to the minimum image, so a tile that
+// straddles the box edge stays compact), and the largest/smallest Thole parameter.
+__global__ void k_tile_bounds(int n, int nblk, const double4* __restrict__ posa, double bx, double by, double bz,
+                              int pbc, float4* __restrict__ tb_c, float4* __restrict__ tb_h) {
+    int T = blockIdx.x * blockDim.x + threadIdx.x;
+    if (T >= nblk) return;
+    int b = T * kTile, e = min(n, b + kTile);
+    double4 p0 = posa[b];
+    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    double amax = p0.w, amin = p0.w;
+    for (int i = b; i < e; i++) {
+        double4 p = posa[i];
+        double d[3] = {p.x - p0.x, p.y - p0.y, p.z - p0.z};
+        if (pbc) {
+            const double L[3] = {bx, by, bz};
+            for (int k = 0; k < 3; k++)
+                if (L[k] > 0) d[k] -= L[k] * floor(d[k] / L[k] + 0.5);
+        }
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], d[k]);
+            hi[k] = fmax(hi[k], d[k]);
+        }
+        amax = fmax(amax, p.w);
+        amin = fmin(amin, p.w);
+    }
+    tb_c[T] = make_float4((float)(p0.x + 0.5 * (lo[0] + hi[0])), (float)(p0.y + 0.5 * (lo[1] + hi[1])),
+                          (float)(p0.z + 0.5 * (lo[2] + hi[2])), (float)amax);
+    // half extents rounded up so the box still contains every particle after the float conversion
+    tb_h[T] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0001f + 1e-6f, (float)(0.5 * (hi[1] - lo[1])) * 1.0001f + 1e-6f,
+                          (float)(0.5 * (hi[2] - lo[2])) * 1.0001f + 1e-6f, (float)amin);
+}
+
+// True when every pair of tiles I and J lies beyond the Thole range, i.e. the reference's
+// exponent s = -gamma (r/(a_i a_j))^3 is below -50 for all of them and no damping applies
+// (getAndScaleInverseRs, ReferenceTholeDipoleForce.cpp:660-675; the same bound gives a u^3 >= 50
+// in calculateElectrostaticPairIxn, where every Thole factor is then exactly 1 in FP64).  Uses a
+// lower bound of the minimum-image distance between the two bounding boxes, with a 2% margin.
+__device__ __forceinline__ bool tiles_beyond_damping(float4 cI, float4 hI, float4 cJ, float4 hJ, bool pbc, float bx,
+                                                     float by, float bz) {
+    float dx = cJ.x - cI.x, dy = cJ.y - cI.y, dz = cJ.z - cI.z;
+    if (pbc) {
+        dx -= bx * rintf(dx / bx);
+        dy -= by * rintf(dy / by);
+        dz -= bz * rintf(dz / bz);
+    }
+    dx = fmaxf(0.f, fabsf(dx) - hI.x - hJ.x);
+    dy = fmaxf(0.f, fabsf(dy) - hI.y - hJ.y);
+    dz = fmaxf(0.f, fabsf(dz) - hI.z - hJ.z);
+    float rmin2 = dx * dx + dy * dy + dz * dz;
+    float dmax = cI.w * cJ.w;           // largest a_i a_j
+    float gmin = fminf(hI.w, hJ.w);      // smallest gamma
+    if (dmax == 0.f) return true;        // no pair is damped
+    if (!(gmin > 0.f)) return false;
+    float rd = dmax * cbrtf(50.f / gmin) * 1.02f;
+    return rmin2 > rd * rd;
+}
+
 // ------------------------------------------------------------------ permanent field (a6)
 // calculateFixedDipoleField(+PairIxn) (ReferenceTholeDipoleForce.cpp:678-733): field at i from
 // the charge and permanent dipole of j, Delta = r_j - r_i:
@@ -120,8 +179,9 @@ __global__ __launch_bounds__(64) void k_fixed_field(int n, int npad, int tiles_p
                                                     const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                     const double* __restrict__ inva, const int* __restrict__ exc_start,
                                                     const int* __restrict__ exc_J,
-                                                    const unsigned long long* __restrict__ exc_mask, BoxT64 box,
-                                                    double4* __restrict__ part) {
+                                                    const unsigned long long* __restrict__ exc_mask,
+                                                    const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                    BoxT64 box, double4* __restrict__ part) {
     __shared__ double4 s_p[kTile];
     __shared__ double4 s_q[kTile];
     __shared__ double s_ia[kTile];
@@ -132,6 +192,7 @@ __global__ __launch_bounds__(64) void k_fixed_field(int n, int npad, int tiles_p
     const int J1 = min(nblk, J0 + tiles_per_split);
     const double4 pi = posa[i];
     const double iai = inva[i];
+    const float4 cI = tb_c[I], hI = tb_h[I];
     double ex = 0.0, ey = 0.0, ez = 0.0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
@@ -145,6 +206,26 @@ __global__ __launch_bounds__(64) void k_fixed_field(int n, int npad, int tiles_p
         __syncthreads();
         const int jn = min(kTile, n - jb);
         const bool masked = (e < eEnd && exc_J[e] == J);
+        if (!masked && jn == kTile &&
+            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
+#pragma unroll 4
+            for (int jj = 0; jj < kTile; jj++) {
+                const double4 pj = s_p[jj];
+                const double4 qj = s_q[jj];
+                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double rinv = rsqrt(r2);
+                double rinv2 = rinv * rinv;
+                double rr3 = rinv * rinv2;
+                double rr5 = 3.0 * rr3 * rinv2;
+                double f = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
+                ex -= dx * f + qj.x * rr3;
+                ey -= dy * f + qj.y * rr3;
+                ez -= dz * f + qj.z * rr3;
+            }
+            continue;
+        }
         unsigned long long m0 = 0ull, mh = 0ull;
         if (masked) {
             m0 = exc_mask[((size_t)e * 3 + 0) * kTile + lane];
@@ -178,15 +259,19 @@ __global__ __launch_bounds__(64) void k_fixed_field(int n, int npad, int tiles_p
 // calculateInducedDipoleFields(+PairIxn) (ReferenceTholeDipoleForce.cpp:766-818):
 //   E_i += iScale * [-rr3 mu_j + rr5 (mu_j.Delta) Delta], pairs with iScale == 0 skipped.
 // T = double: every term in FP64.  T = float: pair terms in FP32, each tile's 64 terms summed in
-// FP32 and added to FP64 accumulators.
+// FP32 and added to FP64 accumulators.  Three paths per j tile: tiles holding scaled pairs (bit
+// masks), tiles within the Thole range (per-pair damping), and tiles beyond it (no damping,
+// unrolled, branch-free).
 template <typename T, bool PBC>
 __global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles_per_split,
                                                       const typename Vec4<T>::type* __restrict__ posa,
                                                       const typename Vec4<T>::type* __restrict__ mu,
                                                       const T* __restrict__ inva, const int* __restrict__ exc_start,
                                                       const int* __restrict__ exc_J,
-                                                      const unsigned long long* __restrict__ exc_mask, BoxT64 box64,
-                                                      const double* __restrict__ scalars, double4* __restrict__ part) {
+                                                      const unsigned long long* __restrict__ exc_mask,
+                                                      const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                      BoxT64 box64, const double* __restrict__ scalars,
+                                                      double4* __restrict__ part) {
     using V4 = typename Vec4<T>::type;
     if (scalars && solver_done(scalars)) return;
     __shared__ V4 s_p[kTile];
@@ -201,6 +286,7 @@ __global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles
     const T iai = inva[i];
     const T bx = (T)box64.bx, by = (T)box64.by, bz = (T)box64.bz;
     const T ibx = (T)box64.ibx, iby = (T)box64.iby, ibz = (T)box64.ibz;
+    const float4 cI = tb_c[I], hI = tb_h[I];
     double ex = 0.0, ey = 0.0, ez = 0.0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
@@ -214,25 +300,44 @@ __global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles
         __syncthreads();
         const int jn = min(kTile, n - jb);
         const bool masked = (e < eEnd && exc_J[e] == J);
-        unsigned long long mi = 0ull;
-        if (masked) {
-            mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
-            e++;
-        }
         T tx = T(0), ty = T(0), tz = T(0);
-        for (int jj = 0; jj < jn; jj++) {
-            if (masked && ((mi >> jj) & 1ull)) continue;
-            const V4 pj = s_p[jj];
-            const V4 uj = s_u[jj];
-            T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
-            T r2 = dx * dx + dy * dy + dz * dz;
-            T rr3, rr5;
-            damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
-            T ud = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
-            tx += ud * dx - rr3 * uj.x;
-            ty += ud * dy - rr3 * uj.y;
-            tz += ud * dz - rr3 * uj.z;
+        if (!masked && jn == kTile &&
+            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box64.bx, (float)box64.by, (float)box64.bz)) {
+#pragma unroll 8
+            for (int jj = 0; jj < kTile; jj++) {
+                const V4 pj = s_p[jj];
+                const V4 uj = s_u[jj];
+                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
+                T r2 = dx * dx + dy * dy + dz * dz;
+                T rinv = rsqrt_t(r2);
+                T rinv2 = rinv * rinv;
+                T rr3 = rinv * rinv2;
+                T ud = T(3) * rr3 * rinv2 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                tx += ud * dx - rr3 * uj.x;
+                ty += ud * dy - rr3 * uj.y;
+                tz += ud * dz - rr3 * uj.z;
+            }
+        } else {
+            unsigned long long mi = 0ull;
+            if (masked) {
+                mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
+                e++;
+            }
+            for (int jj = 0; jj < jn; jj++) {
+                if ((mi >> jj) & 1ull) continue;
+                const V4 pj = s_p[jj];
+                const V4 uj = s_u[jj];
+                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
+                T r2 = dx * dx + dy * dy + dz * dz;
+                T rr3, rr5;
+                damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+                T ud = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                tx += ud * dx - rr3 * uj.x;
+                ty += ud * dy - rr3 * uj.y;
+                tz += ud * dz - rr3 * uj.z;
+            }
         }
         ex += (double)tx;
         ey += (double)ty;
@@ -899,6 +1004,12 @@ void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, c
                        posa32);
 }
 
+void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
+                        float4* tb_c, float4* tb_h) {
+    hipLaunchKernelGGL(k_tile_bounds, dim3((nblk + 63) / 64), dim3(64), 0, s, n, nblk, posa, box3[0], box3[1], box3[2],
+                       pbc ? 1 : 0, tb_c, tb_h);
+}
+
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
                    const int4* axis, double4* qd) {
     hipLaunchKernelGGL(k_rotate_dipoles, dim3((n + 255) / 256), dim3(256), 0, s, n, posa, dmol, charge, axis, qd);
@@ -910,10 +1021,10 @@ void launch_fixed_field(hipStream_t s, const PairLaunch& L, const double4* posa,
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_fixed_field<true>, grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, b, part);
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, part);
     else
         hipLaunchKernelGGL(k_fixed_field<false>, grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd,
-                           inva, L.exc_start, L.exc_J, L.exc_mask, b, part);
+                           inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, part);
 }
 
 void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
@@ -922,10 +1033,10 @@ void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* p
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL((k_induced_field<double, true>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
     else
         hipLaunchKernelGGL((k_induced_field<double, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
 }
 
 void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
@@ -934,10 +1045,10 @@ void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* po
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL((k_induced_field<float, true>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
     else
         hipLaunchKernelGGL((k_induced_field<float, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, b, scalars, part);
+                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
 }
 
 void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,

@@ -229,12 +229,15 @@ def test_device_entry_point_matches_host():
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_water12288_rows_full_size(precision):
-    """36,864 sites (12,288 waters, 4.95 nm box, reference PME semantics), PCG to 1e-6 D.
-    Checks at full size: the solve converged; sampled rows of the fixed point
-    mu_i = alpha_i (E0_i + (T mu)_i) and of the forces, evaluated by the oracle over all
-    36,863 partners with the GPU's dipoles; mixed vs FP64 force agreement."""
+    """36,864 sites (12,288 waters, 4.95 nm box, reference PME semantics), PCG solved to an RMS
+    dipole update of 1e-9 e*nm (well below the 1e-6 D bar, so that solver truncation does not
+    hide the operator's error).  Checks at full size: the solve converged; on sampled rows the
+    fixed point mu_i = alpha_i (E0_i + (T mu)_i), evaluated by the oracle in FP64 over all 36,863
+    partners with the GPU's dipoles, holds to 1e-6 D; the GPU forces on those rows match the
+    oracle's rows to the force tolerance."""
     _say("case water12288 %s" % precision)
     s = systems.config1()
+    s.force.setMutualInducedTargetEpsilon(1e-9)
     from tholedipoleplugin_amd.context import System
     sysm = System()
     for m in s.masses:
@@ -254,7 +257,8 @@ def test_water12288_rows_full_size(precision):
     ref = oracle.rows(s.force, s.positions, mu, rows, s.box, threads=8)
     alpha = s.force.packed_arrays()["polarity"][rows][:, None]
     resid = mu[rows] - alpha * (ref["e0"] + ref["eind"])
-    assert np.max(np.abs(resid)) <= 5 * MU_TOL
+    _say("  max |mu - alpha (E0 + T mu)| on sampled rows: %.3e e*nm" % np.max(np.abs(resid)))
+    assert np.max(np.abs(resid)) <= MU_TOL
     assert rel_rms(st.getForces()[rows], ref["force"]) <= F_TOL
 
 

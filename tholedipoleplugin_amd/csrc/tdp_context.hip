@@ -64,6 +64,13 @@ struct tdp_context {
             *q = nullptr, *part = nullptr, *part_f = nullptr, *part_t = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
     float4 *posa32 = nullptr, *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr;
+    // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
+    int nwork = 0;
+    size_t ntri = 0;
+    int4* d_work = nullptr;
+    int* d_row_start = nullptr;
+    void* d_pbuf = nullptr;   // (nwork + ntri) * 64 partial vectors (double4-sized slots)
+    double4* Emv = nullptr;   // T*mu of the last matvec, reduced
     double *d_pos3 = nullptr, *d_frc = nullptr, *d_alpha = nullptr, *d_inva = nullptr, *d_thole = nullptr,
            *d_charge = nullptr, *d_dmol = nullptr, *d_red = nullptr, *d_scalars = nullptr, *d_part_e = nullptr,
            *d_coef = nullptr, *d_energy = nullptr, *d_mass = nullptr, *d_out = nullptr;
@@ -106,7 +113,8 @@ int dalloc(tdp_context* c, T** ptr, size_t count) {
     size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-    e = hipMemset(p, 0, bytes);
+    // zero on the context's stream, so that later work on that stream is ordered after it
+    e = hipMemsetAsync(p, 0, bytes, c->stream);
     if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipMemset failed: ") + hipGetErrorString(e));
     c->allocations.push_back(p);
     *ptr = reinterpret_cast<T*>(p);
@@ -325,14 +333,15 @@ int upload_particles(tdp_context* c) {
         for (int d = 0; d < 3; d++) dmol[3 * k + d] = c->dmol[3 * i + d];
         axis[k] = make_int4(c->axis_type[i], ren(c->atom_z[i]), ren(c->atom_x[i]), ren(c->atom_y[i]));
     }
-    TDP_HIP(c, hipMemcpy(c->d_alpha, alpha.data(), npad * sizeof(double), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_inva, inva.data(), npad * sizeof(double), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_inva32, inva32.data(), npad * sizeof(float), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_thole, thole.data(), npad * sizeof(double), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_charge, charge.data(), npad * sizeof(double), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_dmol, dmol.data(), 3 * npad * sizeof(double), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_axis, axis.data(), npad * sizeof(int4), hipMemcpyHostToDevice));
-    TDP_HIP(c, hipMemcpy(c->d_perm, perm.data(), npad * sizeof(int), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpyAsync(c->d_alpha, alpha.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_inva, inva.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_inva32, inva32.data(), npad * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_thole, thole.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_charge, charge.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_dmol, dmol.data(), 3 * npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_axis, axis.data(), npad * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_perm, perm.data(), npad * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));  // the host arrays go out of scope
     return TDP_OK;
 }
 
@@ -346,15 +355,16 @@ int upload_tables(tdp_context* c) {
         (rc = dresize(c, &c->d_exc_mask, c->cap_exc_mask, ntiles * 3 * 64)) ||
         (rc = dresize(c, &c->d_gidx, c->cap_gidx, c->gidx.size())))
         return rc;
-    TDP_HIP(c, hipMemcpy(c->d_exc_start, c->exc_start.data(), c->exc_start.size() * sizeof(int), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpyAsync(c->d_exc_start, c->exc_start.data(), c->exc_start.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     if (ntiles) {
-        TDP_HIP(c, hipMemcpy(c->d_exc_J, c->exc_J.data(), ntiles * sizeof(int), hipMemcpyHostToDevice));
-        TDP_HIP(c, hipMemcpy(c->d_exc_mask, c->exc_mask.data(), c->exc_mask.size() * sizeof(unsigned long long),
-                             hipMemcpyHostToDevice));
+        TDP_HIP(c, hipMemcpyAsync(c->d_exc_J, c->exc_J.data(), ntiles * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        TDP_HIP(c, hipMemcpyAsync(c->d_exc_mask, c->exc_mask.data(), c->exc_mask.size() * sizeof(unsigned long long),
+                                  hipMemcpyHostToDevice, c->stream));
     }
-    TDP_HIP(c, hipMemcpy(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice));
+    TDP_HIP(c, hipMemcpyAsync(c->d_goff, c->goff.data(), c->goff.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     if (!c->gidx.empty())
-        TDP_HIP(c, hipMemcpy(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice));
+        TDP_HIP(c, hipMemcpyAsync(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
     return TDP_OK;
 }
 
@@ -435,6 +445,9 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.exc_mask = c->d_exc_mask;
     L.tb_c = c->tb_c;
     L.tb_h = c->tb_h;
+    L.nwork = c->nwork;
+    L.work = c->d_work;
+    L.row_start = c->d_row_start;
     return L;
 }
 
@@ -470,10 +483,14 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         e1 = take_event(c);
         hipEventRecord(e0, c->stream);
     }
-    if (c->precision == TDP_PRECISION_MIXED)
-        launch_induced_field32(c->stream, L, c->posa32, c->vec32, c->d_inva32, scalars, c->part);
-    else
-        launch_induced_field64(c->stream, L, c->posa, v64, c->d_inva, scalars, c->part);
+    if (c->precision == TDP_PRECISION_MIXED) {
+        double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
+        launch_sym_matvec32(c->stream, L, c->posa32, c->vec32, c->d_inva32, scalars, pi,
+                            reinterpret_cast<float4*>(pi + (size_t)c->nwork * 64), c->Emv);
+    } else {
+        double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
+        launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pi + (size_t)c->nwork * 64, c->Emv);
+    }
     if (c->timing) {
         hipEventRecord(e1, c->stream);
         c->pending.push_back({e0, e1});
@@ -540,12 +557,12 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
         for (int o = 1; o < order; o++) {
             matvec(c, L, c->mu, nullptr);
-            launch_extrap_step(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
+            launch_extrap_step(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
                                c->hist + (size_t)o * npad);
         }
         launch_extrap_combine(s, n, npad, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
         matvec(c, L, c->mu, nullptr);
-        launch_extrap_error(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
+        launch_extrap_error(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
         double sc[SC_COUNT];
         TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
         TDP_HIP(c, hipStreamSynchronize(s));
@@ -569,7 +586,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         for (;;) {
             for (int k = 0; k < chunk; k++) {
                 matvec(c, L, c->mu, c->d_scalars);
-                launch_jacobi_update(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu,
+                launch_jacobi_update(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu,
                                      mixed ? c->vec32 : nullptr, c->d_scalars, c->d_red);
             }
             TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
@@ -584,7 +601,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc0, sizeof sc0, hipMemcpyHostToDevice, s));
         // x = mu; r = E0 - x/alpha + T x; z = alpha r; p = z
         matvec(c, L, c->mu, nullptr);
-        launch_pcg_init(s, n, npad, c->nsplit, c->part, c->E0, c->d_alpha, c->mu, c->r, c->p,
+        launch_pcg_init(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p,
                         mixed ? c->vec32 : nullptr, c->d_red, c->d_scalars);
         double sc[SC_COUNT];
         TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
@@ -592,7 +609,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         while (sc[SC_DONE] == 0.0) {
             for (int k = 0; k < chunk; k++) {
                 matvec(c, L, c->p, c->d_scalars);
-                launch_pcg_iteration_tail(s, n, npad, c->nsplit, c->part, c->d_alpha, c->mu, c->r, c->z, c->p,
+                launch_pcg_iteration_tail(s, n, npad, 1, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p,
                                           mixed ? c->vec32 : nullptr, c->q, c->d_red, c->d_scalars);
             }
             TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
@@ -711,11 +728,37 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
     c->perm.resize(c->n);
     c->inv.resize(c->n);
     for (int i = 0; i < c->n; i++) c->perm[i] = c->inv[i] = i;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(TDP_ERR_HIP);
     }
     const int npad = c->npad, ns = c->nsplit;
+    // Upper-triangle work items: tile row I split into chunks of C tiles (J >= I), C chosen for
+    // >= ~8192 one-wave work items on large systems.
+    {
+        const int nb = c->nblk;
+        c->ntri = (size_t)nb * (nb + 1) / 2;
+        int C = (int)std::max<size_t>(1, c->ntri / 8192);
+        std::vector<int4> work;
+        std::vector<int> row_start(nb + 1, 0);
+        for (int I = 0; I < nb; I++) {
+            row_start[I] = (int)work.size();
+            for (int J0 = I; J0 < nb; J0 += C) work.push_back(make_int4(I, J0, std::min(nb, J0 + C), 0));
+        }
+        row_start[nb] = (int)work.size();
+        c->nwork = (int)work.size();
+        if ((rc = dalloc(c, &c->d_work, work.size())) || (rc = dalloc(c, &c->d_row_start, nb + 1)) ||
+            (rc = dalloc(c, (double4**)&c->d_pbuf, ((size_t)c->nwork + c->ntri) * 64)) ||
+            (rc = dalloc(c, &c->Emv, npad)))
+            return bail(rc);
+        if (!work.empty() &&
+            (hipMemcpyAsync(c->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
+             hipMemcpyAsync(c->d_row_start, row_start.data(), row_start.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) !=
+                 hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)) {
+            c->err = "hipMemcpy of the work list failed";
+            return bail(TDP_ERR_HIP);
+        }
+    }
     if ((rc = dalloc(c, &c->posa, npad)) || (rc = dalloc(c, &c->qd, npad)) || (rc = dalloc(c, &c->mu, npad)) ||
         (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
         (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||

@@ -28,7 +28,17 @@ struct PairLaunch {
     const unsigned long long* exc_mask;   // [ntiles][3][64]
     const float4* tb_c;                   // [nblk] tile bounding-box centre, largest Thole parameter
     const float4* tb_h;                   // [nblk] tile bounding-box half extent, smallest Thole parameter
+    // upper-triangle sweep: work item w = tiles (I, J0..J1) as int4 (I, J0, J1, 0); items of tile row I
+    // are row_start[I] .. row_start[I+1]
+    int nwork;
+    const int4* work;
+    const int* row_start;
 };
+
+void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
+                         const double* scalars, double4* Pi, double4* Pj, double4* out);
+void launch_sym_matvec32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
+                         const double* scalars, double4* Pi, float4* Pj, double4* out);
 
 void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h);

@@ -346,6 +346,168 @@ __global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles
     part[(size_t)split * npad + i] = make_double4(ex, ey, ez, 0.0);
 }
 
+// ------------------------------------------------------------------ symmetric T * mu (a7)
+// Same arithmetic as k_induced_field, but every pair is evaluated once, as the reference's
+// i < j loop does.  Work item w covers the tiles (I, J) for J in [J0, J1), J >= I: lane l owns
+// particle i = 64 I + l (sums in registers), and at step k evaluates the pair with slot
+// (l + k) mod 64 of the staged j tile.  The j-side sums travel with their slot: after every step
+// they rotate one lane down the wavefront (DPP wave_rol:1), so lane l always holds the sum of
+// slot (l + k) mod 64.  On the diagonal tile steps k = 1..32 cover each pair once (lanes >= 32 sit
+// out k = 32).  Each tile's j sums go to Pj[tile], each item's i sums to Pi[w]; k_sym_reduce adds
+// them in a fixed order, so the result is deterministic.
+__device__ __forceinline__ int tri_tile(int I, int J, int nblk) { return I * nblk - (I * (I - 1)) / 2 + (J - I); }
+
+// value of lane (l + 1) mod 64 (all lanes must be active)
+__device__ __forceinline__ float rot1(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double rot1(double v) {
+    long long u = __double_as_longlong(v);
+    int lo = __builtin_amdgcn_mov_dpp((int)(u & 0xffffffffll), 0x134, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), 0x134, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+template <typename T, bool PBC>
+__global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* __restrict__ work,
+                                                   const typename Vec4<T>::type* __restrict__ posa,
+                                                   const typename Vec4<T>::type* __restrict__ mu,
+                                                   const T* __restrict__ inva, const int* __restrict__ exc_start,
+                                                   const int* __restrict__ exc_J,
+                                                   const unsigned long long* __restrict__ exc_mask,
+                                                   const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                   BoxT64 box64, const double* __restrict__ scalars,
+                                                   double4* __restrict__ Pi,
+                                                   typename Vec4<T>::type* __restrict__ Pj) {
+    using V4 = typename Vec4<T>::type;
+    if (scalars && solver_done(scalars)) return;
+    __shared__ V4 s_p[kTile];
+    __shared__ V4 s_u[kTile];
+    __shared__ T s_ia[kTile];
+    const int lane = threadIdx.x, w = blockIdx.x;
+    const int4 wk = work[w];
+    const int I = wk.x, J0 = wk.y, J1 = wk.z;
+    const int i = I * kTile + lane;
+    const bool ivalid = i < n;
+    const V4 pi = posa[i];
+    const V4 ui = mu[i];
+    const T iai = inva[i];
+    const T bx = (T)box64.bx, by = (T)box64.by, bz = (T)box64.bz;
+    const T ibx = (T)box64.ibx, iby = (T)box64.iby, ibz = (T)box64.ibz;
+    const float4 cI = tb_c[I], hI = tb_h[I];
+    const bool ifull = I * kTile + kTile <= n;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int e = exc_start[I];
+    const int eEnd = exc_start[I + 1];
+    while (e < eEnd && exc_J[e] < J0) e++;
+    for (int J = J0; J < J1; J++) {
+        const int jb = J * kTile;
+        __syncthreads();
+        s_p[lane] = posa[jb + lane];
+        s_u[lane] = mu[jb + lane];
+        s_ia[lane] = inva[jb + lane];
+        __syncthreads();
+        const bool diag = J == I;
+        const bool masked = (e < eEnd && exc_J[e] == J);
+        unsigned long long mi = 0ull;
+        if (masked) {
+            mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
+            e++;
+        }
+        T tx = T(0), ty = T(0), tz = T(0);   // i side
+        T jx = T(0), jy = T(0), jz = T(0);   // j side of slot (lane + k) mod 64
+        int kend;
+        if (!masked && !diag && ifull && jb + kTile <= n &&
+            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box64.bx, (float)box64.by, (float)box64.bz)) {
+#pragma unroll 4
+            for (int k = 0; k < kTile; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const V4 pj = s_p[jj];
+                const V4 uj = s_u[jj];
+                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
+                T r2 = dx * dx + dy * dy + dz * dz;
+                T rinv = rsqrt_t(r2);
+                T rinv2 = rinv * rinv;
+                T rr3 = rinv * rinv2;
+                T rr5 = T(3) * rr3 * rinv2;
+                T udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                T udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
+                tx += udj * dx - rr3 * uj.x;
+                ty += udj * dy - rr3 * uj.y;
+                tz += udj * dz - rr3 * uj.z;
+                jx += udi * dx - rr3 * ui.x;
+                jy += udi * dy - rr3 * ui.y;
+                jz += udi * dz - rr3 * ui.z;
+                jx = rot1(jx);
+                jy = rot1(jy);
+                jz = rot1(jz);
+            }
+            kend = kTile;
+        } else {
+            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            for (int k = k0; k <= k1; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const bool ok = ivalid && (jb + jj < n) && !((mi >> jj) & 1ull) &&
+                                !(diag && k == kTile / 2 && lane >= kTile / 2);
+                const V4 pj = s_p[jj];
+                const V4 uj = s_u[jj];
+                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
+                T r2 = ok ? dx * dx + dy * dy + dz * dz : T(1);
+                T rr3, rr5;
+                damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+                if (!ok) {
+                    rr3 = T(0);
+                    rr5 = T(0);
+                }
+                T udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                T udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
+                tx += udj * dx - rr3 * uj.x;
+                ty += udj * dy - rr3 * uj.y;
+                tz += udj * dz - rr3 * uj.z;
+                jx += udi * dx - rr3 * ui.x;
+                jy += udi * dy - rr3 * ui.y;
+                jz += udi * dz - rr3 * ui.z;
+                jx = rot1(jx);
+                jy = rot1(jy);
+                jz = rot1(jz);
+            }
+            kend = k1 + 1;
+        }
+        ax += (double)tx;
+        ay += (double)ty;
+        az += (double)tz;
+        // after the last rotation this lane holds the sums of slot (lane + kend) mod 64
+        V4 o;
+        o.x = jx; o.y = jy; o.z = jz; o.w = T(0);
+        Pj[(size_t)tri_tile(I, J, nblk) * kTile + ((lane + kend) & (kTile - 1))] = o;
+    }
+    Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
+}
+
+// Fixed-order sum of the symmetric partials of particle x: the i partials of the work items of
+// tile row B = x/64, then the j partials of tiles (I', B) for I' = 0..B.
+template <typename P4>
+__global__ __launch_bounds__(256) void k_sym_reduce(int n, int nblk, const int* __restrict__ row_start,
+                                                    const double4* __restrict__ Pi, const P4* __restrict__ Pj,
+                                                    const double* __restrict__ scalars, double4* __restrict__ out) {
+    if (scalars && solver_done(scalars)) return;
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= n) return;
+    int B = x / kTile, l = x % kTile;
+    double sx = 0, sy = 0, sz = 0;
+    for (int w = row_start[B]; w < row_start[B + 1]; w++) {
+        double4 v = Pi[(size_t)w * kTile + l];
+        sx += v.x; sy += v.y; sz += v.z;
+    }
+    for (int Ip = 0; Ip <= B; Ip++) {
+        P4 v = Pj[(size_t)tri_tile(Ip, B, nblk) * kTile + l];
+        sx += v.x; sy += v.y; sz += v.z;
+    }
+    out[x] = make_double4(sx, sy, sz, 0.0);
+}
+
 // ------------------------------------------------------------------ energy, forces, torques (a9)
 // calculateElectrostatic (ReferenceTholeDipoleForce.cpp:368-393).  Every lane evaluates all its
 // pairs in the reference's orientation: the particle with the lower ORIGINAL index is I (orig[]
@@ -1049,6 +1211,33 @@ void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* po
     else
         hipLaunchKernelGGL((k_induced_field<float, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
                            posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
+}
+
+void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
+                         const double* scalars, double4* Pi, double4* Pj, double4* out) {
+    // Pi and Pj are the two parts of one partial buffer
+    BoxT64 b = mkbox(L.box);
+    if (L.pbc)
+        hipLaunchKernelGGL((k_sym_matvec<double, true>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
+                           inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
+    else
+        hipLaunchKernelGGL((k_sym_matvec<double, false>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
+                           mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
+    hipLaunchKernelGGL(k_sym_reduce<double4>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.nblk, L.row_start, Pi,
+                       Pj, scalars, out);
+}
+
+void launch_sym_matvec32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
+                         const double* scalars, double4* Pi, float4* Pj, double4* out) {
+    BoxT64 b = mkbox(L.box);
+    if (L.pbc)
+        hipLaunchKernelGGL((k_sym_matvec<float, true>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
+                           inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
+    else
+        hipLaunchKernelGGL((k_sym_matvec<float, false>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
+                           mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
+    hipLaunchKernelGGL(k_sym_reduce<float4>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.nblk, L.row_start, Pi,
+                       Pj, scalars, out);
 }
 
 void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,

@@ -262,6 +262,80 @@ def test_water12288_rows_full_size(precision):
     assert rel_rms(st.getForces()[rows], ref["force"]) <= F_TOL
 
 
+def half_box_system():
+    """128 polarizable charges in a 4 nm periodic box: cluster A (64 sites on a 4x4x4 grid, 0.2 nm
+    spacing, in the box octant nearest the origin, so that A and B fill one 64-site tile each
+    and the pair of tiles lies beyond the Thole range) and cluster B, each site of B displaced
+    from its partner in A by
+    (L/2 - delta_k, 0.3, 0.2) nm with delta_k from +-1e-7 down to +-1e-11 nm and 0.  Those pairs
+    sit closer to half a box than FP32 coordinates can resolve, and the 0.3/0.2 nm transverse
+    offset makes the two images differ; the minimum image of each must follow the FP64 rule
+    (d - L floor(d/L + 0.5), ReferenceTholeDipoleForce.cpp:1414-1423)."""
+    from tholedipoleplugin_amd.context import System
+    L = 4.0
+    g = np.arange(4) * 0.2
+    a = np.array([[x, y, z] for x in g for y in g for z in g]) + 0.3
+    deltas = np.array([1e-7, -1e-7, 1e-8, -1e-8, 1e-9, -1e-9, 1e-10, -1e-10, 1e-11, -1e-11, 0.0])
+    d = np.resize(deltas, 64)
+    b = a + np.stack([L / 2 - d, np.full(64, 0.3), np.full(64, 0.2)], axis=1)
+    pos = np.concatenate([a, b])
+    rng = np.random.default_rng(11)
+    q = rng.choice([-0.5, 0.5], 128)
+    system = System()
+    f = TholeDipoleForce()
+    f.setNonbondedMethod(T.PME)
+    f.setPolarizationType(T.Mutual)
+    for i in range(128):
+        system.addParticle(1.0)
+        f.addParticle(float(q[i]), [0.0, 0.0, 0.0], 1.0e-3, 0.39, T.NoAxisType, -1, -1, -1)
+    box = np.diag([L, L, L])
+    system.setDefaultPeriodicBoxVectors(*box)
+    system.addForce(f)
+    return ref_systems.RefSystem("halfbox", system, f, pos, box)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_half_box_images(precision):
+    _say("case half box %s" % precision)
+    rs = tighten(half_box_system(), eps=1e-12)
+    ctx, st, k = gpu_eval(rs, precision=precision, solver="pcg")
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box, target_eps=1e-13, max_iter=5000)
+    assert k.solverStats()["converged"] == 1
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= E_TOL * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+    assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
+
+
+def test_water12288_mixed_matches_double():
+    """36,864 sites: the mixed-precision solve agrees with the FP64 one at every site (induced
+    dipoles within 1e-6 D, forces within the force tolerance, energy within 1e-6), both solved
+    to an RMS dipole update of 1e-10 e*nm so that the difference is the operator's."""
+    _say("case water12288 mixed vs double")
+    s = systems.config1()
+    s.force.setMutualInducedTargetEpsilon(1e-10)
+    from tholedipoleplugin_amd.context import System
+    out = {}
+    for precision in ("double", "mixed"):
+        sysm = System()
+        for m in s.masses:
+            sysm.addParticle(m)
+        sysm.setDefaultPeriodicBoxVectors(*s.box)
+        sysm.addForce(s.force)
+        ctx = Context(sysm, precision=precision, solver="pcg")
+        ctx.setPeriodicBoxVectors(*s.box)
+        ctx.setPositions(s.positions)
+        st = ctx.getState(getForces=True, getEnergy=True)
+        assert ctx._kernel_for(s.force).solverStats()["converged"] == 1
+        out[precision] = (st.getPotentialEnergy(), st.getForces(), s.force.getInducedDipoles(ctx))
+    (e64, f64, mu64), (em, fm, mum) = out["double"], out["mixed"]
+    dmu = np.max(np.abs(mum - mu64))
+    _say("  max |mu_mixed - mu_double| = %.3e D, force rel RMS %.3e, energy rel %.3e"
+         % (dmu / DEBYE, rel_rms(fm, f64), abs(em - e64) / abs(e64)))
+    assert dmu <= MU_TOL
+    assert rel_rms(fm, f64) <= F_TOL
+    assert abs(em - e64) <= E_TOL * abs(e64)
+
+
 def test_cpp_host_kernel():
     """The C++ host-side kernel mirror (tholedipoleplugin_amd/host) run through its own driver."""
     import os

@@ -37,7 +37,7 @@ constexpr int kReorderInterval = 100;
 
 struct tdp_context {
     // ---- parameters (host copies)
-    int n = 0, npad = 0, nblk = 0, nsplit = 1, tiles_per_split = 1;
+    int n = 0, npad = 0, nblk = 0;
     int method = TDP_NO_CUTOFF, polarization = TDP_MUTUAL, precision = TDP_PRECISION_DOUBLE;
     int solver = TDP_SOLVER_JACOBI, device = 0;
     double cutoff = 1.0, ewald_tol = 1e-4, pme_alpha = 0.0;
@@ -61,7 +61,7 @@ struct tdp_context {
     hipStream_t stream = nullptr;
     std::vector<void*> allocations;
     double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
-            *q = nullptr, *part = nullptr, *part_f = nullptr, *part_t = nullptr, *torque = nullptr,
+            *q = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
     float4 *posa32 = nullptr, *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
@@ -69,7 +69,8 @@ struct tdp_context {
     size_t ntri = 0;
     int4* d_work = nullptr;
     int* d_row_start = nullptr;
-    void* d_pbuf = nullptr;   // (nwork + ntri) * 64 partial vectors (double4-sized slots)
+    void* d_pbuf = nullptr;   // item partials (nwork * 64 double4), then the tile partial slab (3 * ntri * 64 doubles)
+    double4* d_pbuf2 = nullptr;  // second buffer of the same shape (torque partials)
     double4* Emv = nullptr;   // T*mu of the last matvec, reduced
     double *d_pos3 = nullptr, *d_frc = nullptr, *d_alpha = nullptr, *d_inva = nullptr, *d_thole = nullptr,
            *d_charge = nullptr, *d_dmol = nullptr, *d_red = nullptr, *d_scalars = nullptr, *d_part_e = nullptr,
@@ -434,8 +435,6 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.n = c->n;
     L.npad = c->npad;
     L.nblk = c->nblk;
-    L.nsplit = c->nsplit;
-    L.tiles_per_split = c->tiles_per_split;
     L.pbc = c->method == TDP_PME;
     L.box[0] = box9 ? box9[0] : 0.0;
     L.box[1] = box9 ? box9[4] : 0.0;
@@ -448,6 +447,7 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.nwork = c->nwork;
     L.work = c->d_work;
     L.row_start = c->d_row_start;
+    L.slab = c->ntri * 64;
     return L;
 }
 
@@ -475,7 +475,7 @@ void harvest_timing(tdp_context* c) {
     c->pending.clear();
 }
 
-// T * v for the solver: v is mu (double4) or its FP32 copy in vec32.
+// T * v for the solver: v64 is mu or p; in mixed precision vec32 holds its FP32 copy.
 void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const double* scalars) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
@@ -485,11 +485,12 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
     }
     if (c->precision == TDP_PRECISION_MIXED) {
         double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
-        launch_sym_matvec32(c->stream, L, c->posa32, c->vec32, c->d_inva32, scalars, pi,
-                            reinterpret_cast<float4*>(pi + (size_t)c->nwork * 64), c->Emv);
+        launch_sym_matvec_mixed(c->stream, L, c->posa, c->posa32, v64, c->vec32, c->d_inva, scalars, pi,
+                                reinterpret_cast<double*>(pi + (size_t)c->nwork * 64), c->Emv);
     } else {
         double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
-        launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pi + (size_t)c->nwork * 64, c->Emv);
+        launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi,
+                            reinterpret_cast<double*>(pi + (size_t)c->nwork * 64), c->Emv);
     }
     if (c->timing) {
         hipEventRecord(e1, c->stream);
@@ -533,11 +534,16 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
     c->evals_since_order++;
     PairLaunch L = pair_launch(c, box);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
-    launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa, mixed ? c->posa32 : nullptr);
+    launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, L.pbc, L.box, c->posa,
+                          mixed ? c->posa32 : nullptr);
     launch_tile_bounds(s, n, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h);
     launch_rotate(s, n, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
-    launch_fixed_field(s, L, c->posa, c->qd, c->d_inva, c->part);
-    launch_field_init(s, n, npad, c->nsplit, c->part, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
+    {
+        double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
+        launch_sym_field(s, L, c->posa, c->qd, c->d_inva, pi, reinterpret_cast<double*>(pi + (size_t)c->nwork * 64),
+                         c->Emv);
+    }
+    launch_field_init(s, n, npad, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
     const int nred = red_blocks(npad);
 
     if (c->polarization == TDP_DIRECT) {
@@ -557,12 +563,12 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
         for (int o = 1; o < order; o++) {
             matvec(c, L, c->mu, nullptr);
-            launch_extrap_step(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
+            launch_extrap_step(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
                                c->hist + (size_t)o * npad);
         }
         launch_extrap_combine(s, n, npad, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
         matvec(c, L, c->mu, nullptr);
-        launch_extrap_error(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
+        launch_extrap_error(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
         double sc[SC_COUNT];
         TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
         TDP_HIP(c, hipStreamSynchronize(s));
@@ -586,7 +592,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         for (;;) {
             for (int k = 0; k < chunk; k++) {
                 matvec(c, L, c->mu, c->d_scalars);
-                launch_jacobi_update(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu,
+                launch_jacobi_update(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu,
                                      mixed ? c->vec32 : nullptr, c->d_scalars, c->d_red);
             }
             TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
@@ -601,7 +607,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc0, sizeof sc0, hipMemcpyHostToDevice, s));
         // x = mu; r = E0 - x/alpha + T x; z = alpha r; p = z
         matvec(c, L, c->mu, nullptr);
-        launch_pcg_init(s, n, npad, 1, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p,
+        launch_pcg_init(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p,
                         mixed ? c->vec32 : nullptr, c->d_red, c->d_scalars);
         double sc[SC_COUNT];
         TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
@@ -609,7 +615,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         while (sc[SC_DONE] == 0.0) {
             for (int k = 0; k < chunk; k++) {
                 matvec(c, L, c->p, c->d_scalars);
-                launch_pcg_iteration_tail(s, n, npad, 1, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p,
+                launch_pcg_iteration_tail(s, n, npad, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p,
                                           mixed ? c->vec32 : nullptr, c->q, c->d_red, c->d_scalars);
             }
             TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
@@ -633,10 +639,13 @@ void forces_and_energy(tdp_context* c, const double* box) {
     }
     const int n = c->n, npad = c->npad;
     PairLaunch L = pair_launch(c, box);
-    launch_electrostatic(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->d_perm, c->part_f,
-                         c->part_t, c->d_part_e);
-    launch_reduce_forces(s, n, npad, c->nsplit, c->part_f, c->part_t, c->d_frc, c->torque);
-    launch_reduce_energy(s, c->nsplit * c->nblk, c->d_part_e, c->d_energy);
+    double4* pf = reinterpret_cast<double4*>(c->d_pbuf);
+    double4* pt = c->d_pbuf2;
+    const size_t off = (size_t)c->nwork * 64;
+    launch_sym_elec(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->d_perm, pf,
+                    reinterpret_cast<double*>(pf + off), pt, reinterpret_cast<double*>(pt + off), c->d_part_e, c->d_frc,
+                    c->torque);
+    launch_reduce_energy(s, c->nwork, c->d_part_e, c->d_energy);
     if (!c->gidx.empty()) {
         launch_map_torque(s, n, c->posa, c->d_axis, c->torque, c->contrib);
         launch_gather_torque(s, n, c->d_goff, c->d_gidx, c->contrib, c->d_frc);
@@ -700,12 +709,6 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
     c->n = d->num_particles;
     c->npad = std::max(64, (c->n + 63) / 64 * 64);
     c->nblk = (c->n + 63) / 64;
-    // Split the j range so that the pair kernels launch >= ~8192 one-wave workgroups
-    // (8 waves per SIMD on 256 CUs) when the system is large enough.
-    int nb = std::max(c->nblk, 1);
-    int want = std::max(1, std::min(nb, (8192 + nb - 1) / nb));
-    c->tiles_per_split = (nb + want - 1) / want;
-    c->nsplit = (nb + c->tiles_per_split - 1) / c->tiles_per_split;
     c->method = d->nonbonded_method;
     c->polarization = d->polarization_type;
     c->cutoff = d->cutoff;
@@ -732,7 +735,7 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         c->err = "hipStreamCreate failed";
         return bail(TDP_ERR_HIP);
     }
-    const int npad = c->npad, ns = c->nsplit;
+    const int npad = c->npad;
     // Upper-triangle work items: tile row I split into chunks of C tiles (J >= I), C chosen for
     // >= ~8192 one-wave work items on large systems.
     {
@@ -749,7 +752,8 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         c->nwork = (int)work.size();
         if ((rc = dalloc(c, &c->d_work, work.size())) || (rc = dalloc(c, &c->d_row_start, nb + 1)) ||
             (rc = dalloc(c, (double4**)&c->d_pbuf, ((size_t)c->nwork + c->ntri) * 64)) ||
-            (rc = dalloc(c, &c->Emv, npad)))
+            (rc = dalloc(c, &c->d_pbuf2, ((size_t)c->nwork + c->ntri) * 64)) ||
+            (rc = dalloc(c, &c->d_part_e, (size_t)c->nwork + 1)) || (rc = dalloc(c, &c->Emv, npad)))
             return bail(rc);
         if (!work.empty() &&
             (hipMemcpyAsync(c->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
@@ -762,15 +766,14 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
     if ((rc = dalloc(c, &c->posa, npad)) || (rc = dalloc(c, &c->qd, npad)) || (rc = dalloc(c, &c->mu, npad)) ||
         (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
         (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||
-        (rc = dalloc(c, &c->part, (size_t)ns * npad)) || (rc = dalloc(c, &c->part_f, (size_t)ns * npad)) ||
-        (rc = dalloc(c, &c->part_t, (size_t)ns * npad)) || (rc = dalloc(c, &c->torque, npad)) ||
+        (rc = dalloc(c, &c->torque, npad)) ||
         (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || (rc = dalloc(c, &c->posa32, npad)) ||
         (rc = dalloc(c, &c->vec32, npad)) || (rc = dalloc(c, &c->d_pos3, 3 * (size_t)npad)) ||
         (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_alpha, npad)) ||
         (rc = dalloc(c, &c->d_inva, npad)) || (rc = dalloc(c, &c->d_inva32, npad)) ||
         (rc = dalloc(c, &c->d_thole, npad)) || (rc = dalloc(c, &c->d_charge, npad)) ||
         (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)red_blocks(npad))) ||
-        (rc = dalloc(c, &c->d_scalars, SC_COUNT)) || (rc = dalloc(c, &c->d_part_e, (size_t)ns * c->nblk + 1)) ||
+        (rc = dalloc(c, &c->d_scalars, SC_COUNT)) ||
         (rc = dalloc(c, &c->d_coef, 64)) || (rc = dalloc(c, &c->d_energy, 1)) || (rc = dalloc(c, &c->d_axis, npad)) ||
         (rc = dalloc(c, &c->d_exc_start, c->nblk + 1)) || (rc = dalloc(c, &c->d_goff, c->n + 1)) ||
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||

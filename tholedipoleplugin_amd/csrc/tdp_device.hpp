@@ -78,8 +78,9 @@ struct ElecOut {
 // first rotated component is used as the bond-axis component and forces are rotated back with
 // the reference's index pattern (:150-159).  rInvVec[4] (read past a 4-element vector in the
 // reference, :196 vs :264-333) is taken as 1/r^4.  Thole factors: when |a_I a_K| <= 1e-5 or
-// a*u^3 >= 50 every factor is exactly 1 in FP64, so the exponential is skipped there.
-template <bool MUTUAL>
+// a*u^3 >= 50 every factor is exactly 1 in FP64, so the exponential is skipped there; DAMP = false
+// is the specialisation for pairs known to lie beyond that range (all factors 1).
+template <bool MUTUAL, bool DAMP = true>
 __device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, double dx, double dy, double dz,
                                           double mS, double iS, ElecOut& o) {
     double r2 = dx * dx + dy * dy + dz * dz;
@@ -112,7 +113,7 @@ __device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, 
     double dmp = I.a * K.a;
     double am = I.a < K.a ? I.a : K.a;
     double expau3 = 0.0, au3 = 0.0;
-    if (fabs(dmp) > 1.0e-5) {
+    if (DAMP && fabs(dmp) > 1.0e-5) {
         double u = r * (I.ia * K.ia);
         au3 = am * u * u * u;
         if (au3 < 50.0) expau3 = exp(-au3);

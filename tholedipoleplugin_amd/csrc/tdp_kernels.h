@@ -17,10 +17,9 @@ enum SolverScalars {
     SC_COUNT = 8
 };
 
-// Geometry of one all-pairs launch: nblk tiles of 64 particles, the j range of each i tile
-// split over nsplit workgroups of tiles_per_split tiles.
+// Geometry of one all-pairs launch: nblk tiles of 64 particles.
 struct PairLaunch {
-    int n, npad, nblk, nsplit, tiles_per_split;
+    int n, npad, nblk;
     bool pbc;
     double box[3];
     const int* exc_start;                 // [nblk+1]
@@ -33,48 +32,49 @@ struct PairLaunch {
     int nwork;
     const int4* work;
     const int* row_start;
+    size_t slab;   // 64 * number of upper-triangle tiles: stride between components of the tile partials
 };
 
+// Pairwise passes over the upper-triangle work items.  Pi: nwork*64 item partials; Pj: the
+// column-major tile partial slab, 3 * L.slab doubles.
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
-                         const double* scalars, double4* Pi, double4* Pj, double4* out);
-void launch_sym_matvec32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
-                         const double* scalars, double4* Pi, float4* Pj, double4* out);
+                         const double* scalars, double4* Pi, double* Pj, double4* out);
+void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const float4* posa32,
+                             const double4* mu, const float4* mu32, const double* inva, const double* scalars,
+                             double4* Pi, double* Pj, double4* out);
 
 void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h);
 
+// posa: FP64 positions (internal order) and Thole parameter.  posa32 (mixed precision only): with
+// PBC the bit patterns of the quantized box fractions (see k_load_positions), else FP32 offsets
+// from the first particle's position; w = Thole parameter.
 void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
-                           double4* posa, float4* posa32);
+                           bool pbc, const double* box3, double4* posa, float4* posa32);
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
                    const int4* axis, double4* qd);
-void launch_fixed_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
-                        double4* part);
-void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
-                            const double* inva, const double* scalars, double4* part);
-void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
-                            const double* scalars, double4* part);
-void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
-                          const double4* mu, const double* inva, const int* orig, double4* part_f, double4* part_t,
-                          double* part_e);
-void launch_field_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
+                      double4* Pi, double* Pj, double4* out);
+void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
+                     const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
+                     double* PjT, double* part_e, double* frc, double4* torque);
+void launch_field_init(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
                        double4* E0, double4* mu, float4* mu32);
 int red_blocks(int npad);
-void launch_jacobi_update(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_jacobi_update(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                           const double* alpha, double4* mu, float4* mu32, const double* scalars, double* red);
-void launch_pcg_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_pcg_init(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                      const double* alpha, const double4* x, double4* r, double4* p, float4* p32, double* red,
                      double* scalars);
-void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
                                double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
                                double* scalars);
-void launch_extrap_step(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_extrap_step(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                         const double* alpha, double4* mu, float4* mu32, double4* hist_o);
 void launch_extrap_combine(hipStream_t s, int n, int npad, int order, const double4* hist, const double* coef,
                            double4* mu, float4* mu32);
-void launch_extrap_error(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_extrap_error(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                          const double* alpha, const double4* mu, double* red, double* scalars);
-void launch_reduce_forces(hipStream_t s, int n, int npad, int nsplit, const double4* part_f, const double4* part_t,
-                          double* frc, double4* torque);
 void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double* out);
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,
                        double4* contrib);

@@ -98,7 +98,7 @@ __global__ void k_rotate_dipoles(int n, const double4* __restrict__ posa, const 
 // Gathers the caller's positions (original order) into the internal spatial order:
 // slot i holds particle perm[i].
 __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos3, const int* __restrict__ perm,
-                                 const double* __restrict__ thole, double4* __restrict__ posa,
+                                 const double* __restrict__ thole, int pbc, BoxT64 box, double4* __restrict__ posa,
                                  float4* __restrict__ posa32) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
@@ -108,7 +108,22 @@ __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos
         p = make_double4(pos3[3 * o], pos3[3 * o + 1], pos3[3 * o + 2], thole[i]);
     }
     posa[i] = p;
-    if (posa32) posa32[i] = make_float4((float)p.x, (float)p.y, (float)p.z, (float)p.w);
+    if (!posa32) return;
+    if (pbc) {
+        // fraction of the box edge in [0, 1), in units of 2^-32, kept modulo 2^32: the wrapped
+        // difference of two such values is the minimum-image separation in the same units
+        auto quant = [](double x, double ib) {
+            double f = x * ib;
+            f -= floor(f);
+            return (unsigned)(unsigned long long)(long long)rint(f * 0x1p32);
+        };
+        posa32[i] = make_float4(__uint_as_float(quant(p.x, box.ibx)), __uint_as_float(quant(p.y, box.iby)),
+                                __uint_as_float(quant(p.z, box.ibz)), (float)p.w);
+    } else {
+        const int o0 = perm[0];
+        posa32[i] = make_float4((float)(p.x - pos3[3 * o0]), (float)(p.y - pos3[3 * o0 + 1]),
+                                (float)(p.z - pos3[3 * o0 + 2]), (float)p.w);
+    }
 }
 
 // ------------------------------------------------------------------ tile classification
@@ -170,192 +185,32 @@ __device__ __forceinline__ bool tiles_beyond_damping(float4 cI, float4 hI, float
     return rmin2 > rd * rd;
 }
 
-// ------------------------------------------------------------------ permanent field (a6)
-// calculateFixedDipoleField(+PairIxn) (ReferenceTholeDipoleForce.cpp:678-733): field at i from
-// the charge and permanent dipole of j, Delta = r_j - r_i:
-//   E_i -= m_ij * [Delta (rr3 q_j - rr5 d_j.Delta) + rr3 d_j].
-template <bool PBC>
-__global__ __launch_bounds__(64) void k_fixed_field(int n, int npad, int tiles_per_split,
-                                                    const double4* __restrict__ posa, const double4* __restrict__ qd,
-                                                    const double* __restrict__ inva, const int* __restrict__ exc_start,
-                                                    const int* __restrict__ exc_J,
-                                                    const unsigned long long* __restrict__ exc_mask,
-                                                    const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
-                                                    BoxT64 box, double4* __restrict__ part) {
-    __shared__ double4 s_p[kTile];
-    __shared__ double4 s_q[kTile];
-    __shared__ double s_ia[kTile];
-    const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
-    const int i = I * kTile + lane;
-    const int nblk = (n + kTile - 1) / kTile;
-    const int J0 = split * tiles_per_split;
-    const int J1 = min(nblk, J0 + tiles_per_split);
-    const double4 pi = posa[i];
-    const double iai = inva[i];
-    const float4 cI = tb_c[I], hI = tb_h[I];
-    double ex = 0.0, ey = 0.0, ez = 0.0;
-    int e = exc_start[I];
-    const int eEnd = exc_start[I + 1];
-    while (e < eEnd && exc_J[e] < J0) e++;
-    for (int J = J0; J < J1; J++) {
-        const int jb = J * kTile;
-        __syncthreads();
-        s_p[lane] = posa[jb + lane];
-        s_q[lane] = qd[jb + lane];
-        s_ia[lane] = inva[jb + lane];
-        __syncthreads();
-        const int jn = min(kTile, n - jb);
-        const bool masked = (e < eEnd && exc_J[e] == J);
-        if (!masked && jn == kTile &&
-            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
-#pragma unroll 4
-            for (int jj = 0; jj < kTile; jj++) {
-                const double4 pj = s_p[jj];
-                const double4 qj = s_q[jj];
-                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-                double r2 = dx * dx + dy * dy + dz * dz;
-                double rinv = rsqrt(r2);
-                double rinv2 = rinv * rinv;
-                double rr3 = rinv * rinv2;
-                double rr5 = 3.0 * rr3 * rinv2;
-                double f = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
-                ex -= dx * f + qj.x * rr3;
-                ey -= dy * f + qj.y * rr3;
-                ez -= dz * f + qj.z * rr3;
-            }
-            continue;
-        }
-        unsigned long long m0 = 0ull, mh = 0ull;
-        if (masked) {
-            m0 = exc_mask[((size_t)e * 3 + 0) * kTile + lane];
-            mh = exc_mask[((size_t)e * 3 + 1) * kTile + lane];
-            e++;
-        }
-        for (int jj = 0; jj < jn; jj++) {
-            double scale = 1.0;
-            if (masked) {
-                if ((m0 >> jj) & 1ull) continue;
-                if ((mh >> jj) & 1ull) scale = 0.5;
-            }
-            const double4 pj = s_p[jj];
-            const double4 qj = s_q[jj];
-            double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-            if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-            double r2 = dx * dx + dy * dy + dz * dz;
-            double rr3, rr5;
-            damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
-            double dD = qj.x * dx + qj.y * dy + qj.z * dz;
-            double f = rr3 * qj.w - rr5 * dD;
-            ex -= (dx * f + qj.x * rr3) * scale;
-            ey -= (dy * f + qj.y * rr3) * scale;
-            ez -= (dz * f + qj.z * rr3) * scale;
-        }
-    }
-    part[(size_t)split * npad + i] = make_double4(ex, ey, ez, 0.0);
+__device__ __forceinline__ void load_site(ElecSite& s, double4 p, double4 q, double4 u, double ia) {
+    s.x = p.x; s.y = p.y; s.z = p.z; s.a = p.w; s.ia = ia;
+    s.dx = q.x; s.dy = q.y; s.dz = q.z; s.q = q.w;
+    s.ux = u.x; s.uy = u.y; s.uz = u.z;
 }
 
-// ------------------------------------------------------------------ T * mu (a7)
-// calculateInducedDipoleFields(+PairIxn) (ReferenceTholeDipoleForce.cpp:766-818):
-//   E_i += iScale * [-rr3 mu_j + rr5 (mu_j.Delta) Delta], pairs with iScale == 0 skipped.
-// T = double: every term in FP64.  T = float: pair terms in FP32, each tile's 64 terms summed in
-// FP32 and added to FP64 accumulators.  Three paths per j tile: tiles holding scaled pairs (bit
-// masks), tiles within the Thole range (per-pair damping), and tiles beyond it (no damping,
-// unrolled, branch-free).
-template <typename T, bool PBC>
-__global__ __launch_bounds__(64) void k_induced_field(int n, int npad, int tiles_per_split,
-                                                      const typename Vec4<T>::type* __restrict__ posa,
-                                                      const typename Vec4<T>::type* __restrict__ mu,
-                                                      const T* __restrict__ inva, const int* __restrict__ exc_start,
-                                                      const int* __restrict__ exc_J,
-                                                      const unsigned long long* __restrict__ exc_mask,
-                                                      const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
-                                                      BoxT64 box64, const double* __restrict__ scalars,
-                                                      double4* __restrict__ part) {
-    using V4 = typename Vec4<T>::type;
-    if (scalars && solver_done(scalars)) return;
-    __shared__ V4 s_p[kTile];
-    __shared__ V4 s_u[kTile];
-    __shared__ T s_ia[kTile];
-    const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
-    const int i = I * kTile + lane;
-    const int nblk = (n + kTile - 1) / kTile;
-    const int J0 = split * tiles_per_split;
-    const int J1 = min(nblk, J0 + tiles_per_split);
-    const V4 pi = posa[i];
-    const T iai = inva[i];
-    const T bx = (T)box64.bx, by = (T)box64.by, bz = (T)box64.bz;
-    const T ibx = (T)box64.ibx, iby = (T)box64.iby, ibz = (T)box64.ibz;
-    const float4 cI = tb_c[I], hI = tb_h[I];
-    double ex = 0.0, ey = 0.0, ez = 0.0;
-    int e = exc_start[I];
-    const int eEnd = exc_start[I + 1];
-    while (e < eEnd && exc_J[e] < J0) e++;
-    for (int J = J0; J < J1; J++) {
-        const int jb = J * kTile;
-        __syncthreads();
-        s_p[lane] = posa[jb + lane];
-        s_u[lane] = mu[jb + lane];
-        s_ia[lane] = inva[jb + lane];
-        __syncthreads();
-        const int jn = min(kTile, n - jb);
-        const bool masked = (e < eEnd && exc_J[e] == J);
-        T tx = T(0), ty = T(0), tz = T(0);
-        if (!masked && jn == kTile &&
-            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box64.bx, (float)box64.by, (float)box64.bz)) {
-#pragma unroll 8
-            for (int jj = 0; jj < kTile; jj++) {
-                const V4 pj = s_p[jj];
-                const V4 uj = s_u[jj];
-                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
-                T r2 = dx * dx + dy * dy + dz * dz;
-                T rinv = rsqrt_t(r2);
-                T rinv2 = rinv * rinv;
-                T rr3 = rinv * rinv2;
-                T ud = T(3) * rr3 * rinv2 * (uj.x * dx + uj.y * dy + uj.z * dz);
-                tx += ud * dx - rr3 * uj.x;
-                ty += ud * dy - rr3 * uj.y;
-                tz += ud * dz - rr3 * uj.z;
-            }
-        } else {
-            unsigned long long mi = 0ull;
-            if (masked) {
-                mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
-                e++;
-            }
-            for (int jj = 0; jj < jn; jj++) {
-                if ((mi >> jj) & 1ull) continue;
-                const V4 pj = s_p[jj];
-                const V4 uj = s_u[jj];
-                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
-                T r2 = dx * dx + dy * dy + dz * dz;
-                T rr3, rr5;
-                damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
-                T ud = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
-                tx += ud * dx - rr3 * uj.x;
-                ty += ud * dy - rr3 * uj.y;
-                tz += ud * dz - rr3 * uj.z;
-            }
-        }
-        ex += (double)tx;
-        ey += (double)ty;
-        ez += (double)tz;
-    }
-    part[(size_t)split * npad + i] = make_double4(ex, ey, ez, 0.0);
-}
+// ------------------------------------------------------------------ symmetric pair sweep
+// calculateInducedDipoleFields + calculateInducedDipolePairIxn (ReferenceTholeDipoleForce.cpp:766-819),
+// calculateFixedDipoleField (:715-733) and calculateElectrostatic (:368-393) visit every pair once
+// (i < j).  The kernels below do the same over 64 x 64 tiles of the internal order.  Work item w
+// covers the tiles (I, J) for J in [J0, J1), J >= I: lane l owns particle i = 64 I + l (its sums
+// stay in registers), and at step k evaluates the pair with slot (l + k) mod 64 of the staged j
+// tile.  The j-side sums travel with their slot: after every step they rotate one lane down the
+// wavefront (DPP wave_rol:1), so lane l always holds the sums of slot (l + k) mod 64.  On the
+// diagonal tile steps k = 1..32 cover each pair once (lanes >= 32 sit out k = 32).  Each tile's
+// j sums go to the column-major partial slab Pj (component c of slot s at Pj[c S + s]), each
+// item's i sums to Pi[w]; k_col_reduce adds them in a fixed order, so results are deterministic.
 
-// ------------------------------------------------------------------ symmetric T * mu (a7)
-// Same arithmetic as k_induced_field, but every pair is evaluated once, as the reference's
-// i < j loop does.  Work item w covers the tiles (I, J) for J in [J0, J1), J >= I: lane l owns
-// particle i = 64 I + l (sums in registers), and at step k evaluates the pair with slot
-// (l + k) mod 64 of the staged j tile.  The j-side sums travel with their slot: after every step
-// they rotate one lane down the wavefront (DPP wave_rol:1), so lane l always holds the sum of
-// slot (l + k) mod 64.  On the diagonal tile steps k = 1..32 cover each pair once (lanes >= 32 sit
-// out k = 32).  Each tile's j sums go to Pj[tile], each item's i sums to Pi[w]; k_sym_reduce adds
-// them in a fixed order, so the result is deterministic.
-__device__ __forceinline__ int tri_tile(int I, int J, int nblk) { return I * nblk - (I * (I - 1)) / 2 + (J - I); }
+// Tile (I, J), I <= J, in column-major upper-triangle order: the tiles of column J are contiguous.
+__device__ __forceinline__ size_t col_tile(int I, int J) { return (size_t)J * (J + 1) / 2 + I; }
+
+__device__ __forceinline__ void store3(double* __restrict__ P, size_t S, size_t slot, double x, double y, double z) {
+    P[slot] = x;
+    P[S + slot] = y;
+    P[2 * S + slot] = z;
+}
 
 // value of lane (l + 1) mod 64 (all lanes must be active)
 __device__ __forceinline__ float rot1(float v) {
@@ -368,32 +223,62 @@ __device__ __forceinline__ double rot1(double v) {
     return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
 }
 
-template <typename T, bool PBC>
+// damped_rr in FP64 for a full wavefront: the exponential is evaluated only when some lane of
+// the wavefront has a damped pair (s > -50), which in tiles near the Thole range is a minority
+// of the steps.  Same results as damped_rr<double>.  All lanes must be active.
+__device__ __forceinline__ void damped_rr_wave(double r2, double ai, double aj, double iai, double iaj, double& rr3,
+                                               double& rr5) {
+    double rinv = rsqrt(r2);
+    double r = r2 * rinv;
+    double rinv2 = rinv * rinv;
+    rr3 = rinv * rinv2;
+    rr5 = 3.0 * rr3 * rinv2;
+    double g = ai < aj ? ai : aj;
+    double ratio = r * (iai * iaj);
+    double s = -g * (ratio * ratio * ratio);
+    const bool damped = (ai * aj != 0.0) && (s > -50.0);
+    if (__builtin_amdgcn_ballot_w64(damped)) {
+        double e = exp(damped ? s : -50.0);
+        if (damped) {
+            rr3 *= 1.0 - e;
+            rr5 *= 1.0 - (1.0 - s) * e;
+        }
+    }
+}
+
+// Separation of two quantized periodic coordinates (fractions of the box edge in units of 2^-32,
+// see k_load_positions): the wrapped 32-bit difference is the minimum image.  Returns false when
+// the separation lies within kQuantSlack units of half a box, where the quantized and the FP64
+// image choice could differ; such a pair is imaged again in FP64.
+constexpr unsigned kQuantSlack = 4u;
+__device__ __forceinline__ bool quant_delta(unsigned qi, unsigned qj, float scale, float& d) {
+    int dq = (int)(qj - qi);
+    d = (float)dq * scale;
+    return ((unsigned)dq + 0x80000000u + kQuantSlack) >= 2u * kQuantSlack;
+}
+
+// ------------------------------------------------------------------ T * mu, FP64 (a7)
+template <bool PBC>
 __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* __restrict__ work,
-                                                   const typename Vec4<T>::type* __restrict__ posa,
-                                                   const typename Vec4<T>::type* __restrict__ mu,
-                                                   const T* __restrict__ inva, const int* __restrict__ exc_start,
+                                                   const double4* __restrict__ posa, const double4* __restrict__ mu,
+                                                   const double* __restrict__ inva, const int* __restrict__ exc_start,
                                                    const int* __restrict__ exc_J,
                                                    const unsigned long long* __restrict__ exc_mask,
                                                    const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
-                                                   BoxT64 box64, const double* __restrict__ scalars,
-                                                   double4* __restrict__ Pi,
-                                                   typename Vec4<T>::type* __restrict__ Pj) {
-    using V4 = typename Vec4<T>::type;
+                                                   BoxT64 box, const double* __restrict__ scalars,
+                                                   double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     if (scalars && solver_done(scalars)) return;
-    __shared__ V4 s_p[kTile];
-    __shared__ V4 s_u[kTile];
-    __shared__ T s_ia[kTile];
+    __shared__ double4 s_p[kTile];
+    __shared__ double4 s_u[kTile];
+    __shared__ double s_ia[kTile];
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
     const int i = I * kTile + lane;
     const bool ivalid = i < n;
-    const V4 pi = posa[i];
-    const V4 ui = mu[i];
-    const T iai = inva[i];
-    const T bx = (T)box64.bx, by = (T)box64.by, bz = (T)box64.bz;
-    const T ibx = (T)box64.ibx, iby = (T)box64.iby, ibz = (T)box64.ibz;
+    const double4 pi = posa[i];
+    const double4 ui = mu[i];
+    const double iai = inva[i];
     const float4 cI = tb_c[I], hI = tb_h[I];
     const bool ifull = I * kTile + kTile <= n;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -414,28 +299,27 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
             mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
             e++;
         }
-        T tx = T(0), ty = T(0), tz = T(0);   // i side
-        T jx = T(0), jy = T(0), jz = T(0);   // j side of slot (lane + k) mod 64
+        double jx = 0.0, jy = 0.0, jz = 0.0;   // j side of slot (lane + k) mod 64
         int kend;
         if (!masked && !diag && ifull && jb + kTile <= n &&
-            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box64.bx, (float)box64.by, (float)box64.bz)) {
+            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
 #pragma unroll 4
             for (int k = 0; k < kTile; k++) {
                 const int jj = (lane + k) & (kTile - 1);
-                const V4 pj = s_p[jj];
-                const V4 uj = s_u[jj];
-                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
-                T r2 = dx * dx + dy * dy + dz * dz;
-                T rinv = rsqrt_t(r2);
-                T rinv2 = rinv * rinv;
-                T rr3 = rinv * rinv2;
-                T rr5 = T(3) * rr3 * rinv2;
-                T udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
-                T udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
-                tx += udj * dx - rr3 * uj.x;
-                ty += udj * dy - rr3 * uj.y;
-                tz += udj * dz - rr3 * uj.z;
+                const double4 pj = s_p[jj];
+                const double4 uj = s_u[jj];
+                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double rinv = rsqrt(r2);
+                double rinv2 = rinv * rinv;
+                double rr3 = rinv * rinv2;
+                double rr5 = 3.0 * rr3 * rinv2;
+                double udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                double udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
+                ax += udj * dx - rr3 * uj.x;
+                ay += udj * dy - rr3 * uj.y;
+                az += udj * dz - rr3 * uj.z;
                 jx += udi * dx - rr3 * ui.x;
                 jy += udi * dy - rr3 * ui.y;
                 jz += udi * dz - rr3 * ui.z;
@@ -450,22 +334,22 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
                 const int jj = (lane + k) & (kTile - 1);
                 const bool ok = ivalid && (jb + jj < n) && !((mi >> jj) & 1ull) &&
                                 !(diag && k == kTile / 2 && lane >= kTile / 2);
-                const V4 pj = s_p[jj];
-                const V4 uj = s_u[jj];
-                T dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                if (PBC) min_image(dx, dy, dz, bx, by, bz, ibx, iby, ibz);
-                T r2 = ok ? dx * dx + dy * dy + dz * dz : T(1);
-                T rr3, rr5;
-                damped_rr(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+                const double4 pj = s_p[jj];
+                const double4 uj = s_u[jj];
+                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double r2 = ok ? dx * dx + dy * dy + dz * dz : 1.0;
+                double rr3, rr5;
+                damped_rr_wave(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
                 if (!ok) {
-                    rr3 = T(0);
-                    rr5 = T(0);
+                    rr3 = 0.0;
+                    rr5 = 0.0;
                 }
-                T udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
-                T udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
-                tx += udj * dx - rr3 * uj.x;
-                ty += udj * dy - rr3 * uj.y;
-                tz += udj * dz - rr3 * uj.z;
+                double udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                double udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
+                ax += udj * dx - rr3 * uj.x;
+                ay += udj * dy - rr3 * uj.y;
+                az += udj * dz - rr3 * uj.z;
                 jx += udi * dx - rr3 * ui.x;
                 jy += udi * dy - rr3 * ui.y;
                 jz += udi * dz - rr3 * ui.z;
@@ -475,72 +359,333 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
             }
             kend = k1 + 1;
         }
-        ax += (double)tx;
-        ay += (double)ty;
-        az += (double)tz;
         // after the last rotation this lane holds the sums of slot (lane + kend) mod 64
-        V4 o;
-        o.x = jx; o.y = jy; o.z = jz; o.w = T(0);
-        Pj[(size_t)tri_tile(I, J, nblk) * kTile + ((lane + kend) & (kTile - 1))] = o;
+        store3(Pj, S, col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
     }
     Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
 
-// Fixed-order sum of the symmetric partials of particle x: the i partials of the work items of
-// tile row B = x/64, then the j partials of tiles (I', B) for I' = 0..B.
-template <typename P4>
-__global__ __launch_bounds__(256) void k_sym_reduce(int n, int nblk, const int* __restrict__ row_start,
-                                                    const double4* __restrict__ Pi, const P4* __restrict__ Pj,
-                                                    const double* __restrict__ scalars, double4* __restrict__ out) {
+// ------------------------------------------------------------------ T * mu, mixed precision (a7)
+// Tile pairs inside the Thole range, and the diagonal and exclusion tiles, are evaluated in FP64
+// as in k_sym_matvec.  Tile pairs beyond it, most of the pairs of a large system, carry only the
+// undamped 1/r^3 and 3/r^5 terms and are evaluated in FP32 with FP32 dipoles.  Their separations
+// come from quantized periodic coordinates (PBC; the 32-bit difference is the minimum image, and
+// pairs within a few units of half a box are imaged again from the FP64 positions so that the
+// image always agrees with the FP64 rule) or from FP32 offsets to a common origin (no PBC).  The
+// near pairs carry the largest terms, so keeping them in FP64 holds the operator's error to that
+// of the FP32 far field; all partial sums are FP64.
+template <bool PBC>
+__global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const int4* __restrict__ work,
+                                                         const double4* __restrict__ posa,
+                                                         const float4* __restrict__ posa32,
+                                                         const double4* __restrict__ mu,
+                                                         const float4* __restrict__ mu32,
+                                                         const double* __restrict__ inva,
+                                                         const int* __restrict__ exc_start,
+                                                         const int* __restrict__ exc_J,
+                                                         const unsigned long long* __restrict__ exc_mask,
+                                                         const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                         BoxT64 box, const double* __restrict__ scalars,
+                                                         double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     if (scalars && solver_done(scalars)) return;
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= n) return;
-    int B = x / kTile, l = x % kTile;
-    double sx = 0, sy = 0, sz = 0;
-    for (int w = row_start[B]; w < row_start[B + 1]; w++) {
-        double4 v = Pi[(size_t)w * kTile + l];
-        sx += v.x; sy += v.y; sz += v.z;
+    __shared__ double4 s_p[kTile];
+    __shared__ double4 s_u[kTile];
+    __shared__ double s_ia[kTile];
+    __shared__ float4 s_p32[kTile];
+    __shared__ float4 s_u32[kTile];
+    const int lane = threadIdx.x, w = blockIdx.x;
+    const int4 wk = work[w];
+    const int I = wk.x, J0 = wk.y, J1 = wk.z;
+    const int i = I * kTile + lane;
+    const bool ivalid = i < n;
+    const double4 pi = posa[i];
+    const double4 ui = mu[i];
+    const double iai = inva[i];
+    const float4 pf = posa32[i];
+    const float4 uf = mu32[i];
+    const float sx = (float)(box.bx * 0x1p-32), sy = (float)(box.by * 0x1p-32), sz = (float)(box.bz * 0x1p-32);
+    const float4 cI = tb_c[I], hI = tb_h[I];
+    const bool ifull = I * kTile + kTile <= n;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int e = exc_start[I];
+    const int eEnd = exc_start[I + 1];
+    while (e < eEnd && exc_J[e] < J0) e++;
+    for (int J = J0; J < J1; J++) {
+        const int jb = J * kTile;
+        const bool diag = J == I;
+        const bool masked = (e < eEnd && exc_J[e] == J);
+        unsigned long long mi = 0ull;
+        if (masked) {
+            mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
+            e++;
+        }
+        const bool far = !masked && !diag && ifull && jb + kTile <= n &&
+                         tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by,
+                                              (float)box.bz);
+        double jx, jy, jz;
+        int kend;
+        __syncthreads();
+        if (far) {
+            s_p32[lane] = posa32[jb + lane];
+            s_u32[lane] = mu32[jb + lane];
+            __syncthreads();
+            float tx = 0.f, ty = 0.f, tz = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll 4
+            for (int k = 0; k < kTile; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const float4 pj = s_p32[jj];
+                const float4 uj = s_u32[jj];
+                float dx, dy, dz;
+                if (PBC) {
+                    bool clear = quant_delta(__float_as_uint(pf.x), __float_as_uint(pj.x), sx, dx);
+                    clear &= quant_delta(__float_as_uint(pf.y), __float_as_uint(pj.y), sy, dy);
+                    clear &= quant_delta(__float_as_uint(pf.z), __float_as_uint(pj.z), sz, dz);
+                    if (!clear) {
+                        const double4 pjd = posa[jb + jj];
+                        double ex = pjd.x - pi.x, ey = pjd.y - pi.y, ez = pjd.z - pi.z;
+                        min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                        dx = (float)ex;
+                        dy = (float)ey;
+                        dz = (float)ez;
+                    }
+                } else {
+                    dx = pj.x - pf.x;
+                    dy = pj.y - pf.y;
+                    dz = pj.z - pf.z;
+                }
+                float r2 = dx * dx + dy * dy + dz * dz;
+                float rinv = rsqrt_t(r2);
+                float rinv2 = rinv * rinv;
+                float rr3 = rinv * rinv2;
+                float rr5 = 3.f * rr3 * rinv2;
+                float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                float udi = rr5 * (uf.x * dx + uf.y * dy + uf.z * dz);
+                tx += udj * dx - rr3 * uj.x;
+                ty += udj * dy - rr3 * uj.y;
+                tz += udj * dz - rr3 * uj.z;
+                fx += udi * dx - rr3 * uf.x;
+                fy += udi * dy - rr3 * uf.y;
+                fz += udi * dz - rr3 * uf.z;
+                fx = rot1(fx);
+                fy = rot1(fy);
+                fz = rot1(fz);
+            }
+            ax += (double)tx;
+            ay += (double)ty;
+            az += (double)tz;
+            jx = (double)fx;
+            jy = (double)fy;
+            jz = (double)fz;
+            kend = kTile;
+        } else {
+            s_p[lane] = posa[jb + lane];
+            s_u[lane] = mu[jb + lane];
+            s_ia[lane] = inva[jb + lane];
+            __syncthreads();
+            jx = jy = jz = 0.0;
+            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            for (int k = k0; k <= k1; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const bool ok = ivalid && (jb + jj < n) && !((mi >> jj) & 1ull) &&
+                                !(diag && k == kTile / 2 && lane >= kTile / 2);
+                const double4 pj = s_p[jj];
+                const double4 uj = s_u[jj];
+                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double r2 = ok ? dx * dx + dy * dy + dz * dz : 1.0;
+                double rr3, rr5;
+                damped_rr_wave(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+                if (!ok) {
+                    rr3 = 0.0;
+                    rr5 = 0.0;
+                }
+                double udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                double udi = rr5 * (ui.x * dx + ui.y * dy + ui.z * dz);
+                ax += udj * dx - rr3 * uj.x;
+                ay += udj * dy - rr3 * uj.y;
+                az += udj * dz - rr3 * uj.z;
+                jx += udi * dx - rr3 * ui.x;
+                jy += udi * dy - rr3 * ui.y;
+                jz += udi * dz - rr3 * ui.z;
+                jx = rot1(jx);
+                jy = rot1(jy);
+                jz = rot1(jz);
+            }
+            kend = k1 + 1;
+        }
+        store3(Pj, S, col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
     }
-    for (int Ip = 0; Ip <= B; Ip++) {
-        P4 v = Pj[(size_t)tri_tile(Ip, B, nblk) * kTile + l];
-        sx += v.x; sy += v.y; sz += v.z;
-    }
-    out[x] = make_double4(sx, sy, sz, 0.0);
+    Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
 
-// ------------------------------------------------------------------ energy, forces, torques (a9)
-// calculateElectrostatic (ReferenceTholeDipoleForce.cpp:368-393).  Every lane evaluates all its
-// pairs in the reference's orientation: the particle with the lower ORIGINAL index is I (orig[]
-// maps internal slots back to the caller's indices).  The pair energy is counted by the I side.
-__device__ __forceinline__ void load_site(ElecSite& s, double4 p, double4 q, double4 u, double ia) {
-    s.x = p.x; s.y = p.y; s.z = p.z; s.a = p.w; s.ia = ia;
-    s.dx = q.x; s.dy = q.y; s.dz = q.z; s.q = q.w;
-    s.ux = u.x; s.uy = u.y; s.uz = u.z;
+
+// ------------------------------------------------------------------ symmetric permanent field (a6)
+// calculateFixedDipoleField (ReferenceTholeDipoleForce.cpp:715-733) with each pair once, in the
+// sweep of k_sym_matvec: Delta = r_j - r_i,
+//   E_i -= m [Delta (rr3 q_j - rr5 d_j.Delta) + rr3 d_j],  E_j += m [Delta (rr3 q_i + rr5 d_i.Delta) - rr3 d_i].
+template <bool PBC>
+__global__ __launch_bounds__(64) void k_sym_field(int n, int nblk, const int4* __restrict__ work,
+                                                  const double4* __restrict__ posa, const double4* __restrict__ qd,
+                                                  const double* __restrict__ inva, const int* __restrict__ exc_start,
+                                                  const int* __restrict__ exc_J,
+                                                  const unsigned long long* __restrict__ exc_mask,
+                                                  const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                  BoxT64 box, double4* __restrict__ Pi, double* __restrict__ Pj,
+                                                  size_t S) {
+    __shared__ double4 s_p[kTile];
+    __shared__ double4 s_q[kTile];
+    __shared__ double s_ia[kTile];
+    const int lane = threadIdx.x, w = blockIdx.x;
+    const int4 wk = work[w];
+    const int I = wk.x, J0 = wk.y, J1 = wk.z;
+    const int i = I * kTile + lane;
+    const bool ivalid = i < n;
+    const double4 pi = posa[i];
+    const double4 qi = qd[i];
+    const double iai = inva[i];
+    const float4 cI = tb_c[I], hI = tb_h[I];
+    const bool ifull = I * kTile + kTile <= n;
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    int e = exc_start[I];
+    const int eEnd = exc_start[I + 1];
+    while (e < eEnd && exc_J[e] < J0) e++;
+    for (int J = J0; J < J1; J++) {
+        const int jb = J * kTile;
+        __syncthreads();
+        s_p[lane] = posa[jb + lane];
+        s_q[lane] = qd[jb + lane];
+        s_ia[lane] = inva[jb + lane];
+        __syncthreads();
+        const bool diag = J == I;
+        const bool masked = (e < eEnd && exc_J[e] == J);
+        unsigned long long m0 = 0ull, mh = 0ull;
+        if (masked) {
+            m0 = exc_mask[((size_t)e * 3 + 0) * kTile + lane];
+            mh = exc_mask[((size_t)e * 3 + 1) * kTile + lane];
+            e++;
+        }
+        double jx = 0.0, jy = 0.0, jz = 0.0;
+        int kend;
+        if (!masked && !diag && ifull && jb + kTile <= n &&
+            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
+#pragma unroll 2
+            for (int k = 0; k < kTile; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const double4 pj = s_p[jj];
+                const double4 qj = s_q[jj];
+                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double r2 = dx * dx + dy * dy + dz * dz;
+                double rinv = rsqrt(r2);
+                double rinv2 = rinv * rinv;
+                double rr3 = rinv * rinv2;
+                double rr5 = 3.0 * rr3 * rinv2;
+                double fj = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
+                double fi = rr3 * qi.w + rr5 * (qi.x * dx + qi.y * dy + qi.z * dz);
+                ax -= dx * fj + qj.x * rr3;
+                ay -= dy * fj + qj.y * rr3;
+                az -= dz * fj + qj.z * rr3;
+                jx += dx * fi - qi.x * rr3;
+                jy += dy * fi - qi.y * rr3;
+                jz += dz * fi - qi.z * rr3;
+                jx = rot1(jx);
+                jy = rot1(jy);
+                jz = rot1(jz);
+            }
+            kend = kTile;
+        } else {
+            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            for (int k = k0; k <= k1; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const bool ok = ivalid && (jb + jj < n) && !((m0 >> jj) & 1ull) &&
+                                !(diag && k == kTile / 2 && lane >= kTile / 2);
+                const double scale = ((mh >> jj) & 1ull) ? 0.5 : 1.0;
+                const double4 pj = s_p[jj];
+                const double4 qj = s_q[jj];
+                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double r2 = ok ? dx * dx + dy * dy + dz * dz : 1.0;
+                double rr3, rr5;
+                damped_rr_wave(r2, pi.w, pj.w, iai, s_ia[jj], rr3, rr5);
+                const double m = ok ? scale : 0.0;
+                rr3 = ok ? rr3 : 0.0;
+                rr5 = ok ? rr5 : 0.0;
+                double fj = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
+                double fi = rr3 * qi.w + rr5 * (qi.x * dx + qi.y * dy + qi.z * dz);
+                ax -= (dx * fj + qj.x * rr3) * m;
+                ay -= (dy * fj + qj.y * rr3) * m;
+                az -= (dz * fj + qj.z * rr3) * m;
+                jx += (dx * fi - qi.x * rr3) * m;
+                jy += (dy * fi - qi.y * rr3) * m;
+                jz += (dz * fi - qi.z * rr3) * m;
+                jx = rot1(jx);
+                jy = rot1(jy);
+                jz = rot1(jz);
+            }
+            kend = k1 + 1;
+        }
+        store3(Pj, S, col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
+    }
+    Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
+}
+
+// ------------------------------------------------------------------ symmetric energy/force/torque (a9)
+// calculateElectrostatic (ReferenceTholeDipoleForce.cpp:368-393) with each pair once.  The pair is
+// evaluated in the reference's orientation (I = lower caller index, orig[]); the force on I is -f,
+// on K +f; torques tI, tK; the energy is added by the lane.
+struct ElecAcc {
+    double fx, fy, fz, tx, ty, tz;
+};
+
+template <bool MUTUAL, bool DAMP>
+__device__ __forceinline__ void elec_sym_step(const ElecSite& me, int my_orig, const ElecSite& other, int other_orig,
+                                              bool ok, double mS, double iS, bool pbc, const BoxT64& box, ElecAcc& a,
+                                              ElecAcc& b, double& en) {
+    const bool lo = my_orig < other_orig;
+    const ElecSite& A = lo ? me : other;
+    const ElecSite& B = lo ? other : me;
+    double dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
+    if (pbc) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+    if (!ok) { dx = 1.0; dy = 0.0; dz = 0.0; mS = 0.0; iS = 0.0; }
+    ElecOut o;
+    elec_pair<MUTUAL, DAMP>(A, B, dx, dy, dz, mS, iS, o);
+    if (!ok) { o.e = o.fx = o.fy = o.fz = o.tix = o.tiy = o.tiz = o.tkx = o.tky = o.tkz = 0.0; }
+    const double s = lo ? -1.0 : 1.0;   // force on this lane's particle = s f
+    a.fx += s * o.fx; a.fy += s * o.fy; a.fz += s * o.fz;
+    b.fx -= s * o.fx; b.fy -= s * o.fy; b.fz -= s * o.fz;
+    a.tx += lo ? o.tix : o.tkx; a.ty += lo ? o.tiy : o.tky; a.tz += lo ? o.tiz : o.tkz;
+    b.tx += lo ? o.tkx : o.tix; b.ty += lo ? o.tky : o.tiy; b.tz += lo ? o.tkz : o.tiz;
+    en += o.e;
 }
 
 template <bool PBC, bool MUTUAL>
-__global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles_per_split,
-                                                      const double4* __restrict__ posa, const double4* __restrict__ qd,
-                                                      const double4* __restrict__ mu, const double* __restrict__ inva,
-                                                      const int* __restrict__ orig, const int* __restrict__ exc_start,
-                                                      const int* __restrict__ exc_J,
-                                                      const unsigned long long* __restrict__ exc_mask, BoxT64 box,
-                                                      double4* __restrict__ part_f, double4* __restrict__ part_t,
-                                                      double* __restrict__ part_e) {
+__global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __restrict__ work,
+                                                 const double4* __restrict__ posa, const double4* __restrict__ qd,
+                                                 const double4* __restrict__ mu, const double* __restrict__ inva,
+                                                 const int* __restrict__ orig, const int* __restrict__ exc_start,
+                                                 const int* __restrict__ exc_J,
+                                                 const unsigned long long* __restrict__ exc_mask,
+                                                 const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                 BoxT64 box, double4* __restrict__ PiF, double* __restrict__ PjF,
+                                                 double4* __restrict__ PiT, double* __restrict__ PjT, size_t S,
+                                                 double* __restrict__ part_e) {
     __shared__ double4 s_p[kTile];
     __shared__ double4 s_q[kTile];
     __shared__ double4 s_u[kTile];
     __shared__ double s_ia[kTile];
     __shared__ int s_o[kTile];
-    const int lane = threadIdx.x, I = blockIdx.x, split = blockIdx.y;
+    const int lane = threadIdx.x, w = blockIdx.x;
+    const int4 wk = work[w];
+    const int I = wk.x, J0 = wk.y, J1 = wk.z;
     const int i = I * kTile + lane;
-    const int nblk = (n + kTile - 1) / kTile;
-    const int J0 = split * tiles_per_split;
-    const int J1 = min(nblk, J0 + tiles_per_split);
+    const bool ivalid = i < n;
     ElecSite me;
     load_site(me, posa[i], qd[i], mu[i], inva[i]);
     const int my_orig = orig[i];
-    double fx = 0, fy = 0, fz = 0, tx = 0, ty = 0, tz = 0, en = 0;
+    const float4 cI = tb_c[I], hI = tb_h[I];
+    const bool ifull = I * kTile + kTile <= n;
+    ElecAcc acc = {0, 0, 0, 0, 0, 0};
+    double en = 0.0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
@@ -553,7 +698,7 @@ __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles
         s_ia[lane] = inva[jb + lane];
         s_o[lane] = orig[jb + lane];
         __syncthreads();
-        const int jn = min(kTile, n - jb);
+        const bool diag = J == I;
         const bool masked = (e < eEnd && exc_J[e] == J);
         unsigned long long m0 = 0ull, mh = 0ull, mi = 0ull;
         if (masked) {
@@ -562,59 +707,108 @@ __global__ __launch_bounds__(64) void k_electrostatic(int n, int npad, int tiles
             mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
             e++;
         }
-        for (int jj = 0; jj < jn; jj++) {
-            double mS = 1.0, iS = 1.0;
-            if (masked) {
-                bool z0 = (m0 >> jj) & 1ull, h = (mh >> jj) & 1ull, zi = (mi >> jj) & 1ull;
-                if (z0 && zi) continue;  // all coefficients zero (also the self pair)
-                mS = z0 ? 0.0 : (h ? 0.5 : 1.0);
-                iS = zi ? 0.0 : 1.0;
+        ElecAcc jacc = {0, 0, 0, 0, 0, 0};
+        int kend;
+        if (!masked && !diag && ifull && jb + kTile <= n &&
+            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
+            for (int k = 0; k < kTile; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                ElecSite other;
+                load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
+                elec_sym_step<MUTUAL, false>(me, my_orig, other, s_o[jj], true, 1.0, 1.0, PBC, box, acc, jacc, en);
+                jacc.fx = rot1(jacc.fx); jacc.fy = rot1(jacc.fy); jacc.fz = rot1(jacc.fz);
+                jacc.tx = rot1(jacc.tx); jacc.ty = rot1(jacc.ty); jacc.tz = rot1(jacc.tz);
             }
-            ElecSite other;
-            load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
-            const bool lo = my_orig < s_o[jj];
-            const ElecSite& A = lo ? me : other;
-            const ElecSite& B = lo ? other : me;
-            double dx = B.x - A.x, dy = B.y - A.y, dz = B.z - A.z;
-            if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-            ElecOut o;
-            elec_pair<MUTUAL>(A, B, dx, dy, dz, mS, iS, o);
-            if (lo) {
-                fx -= o.fx; fy -= o.fy; fz -= o.fz;
-                tx += o.tix; ty += o.tiy; tz += o.tiz;
-                en += o.e;
-            } else {
-                fx += o.fx; fy += o.fy; fz += o.fz;
-                tx += o.tkx; ty += o.tky; tz += o.tkz;
+            kend = kTile;
+        } else {
+            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            for (int k = k0; k <= k1; k++) {
+                const int jj = (lane + k) & (kTile - 1);
+                const bool z0 = (m0 >> jj) & 1ull, h = (mh >> jj) & 1ull, zi = (mi >> jj) & 1ull;
+                // a pair with both scales 0 contributes nothing (this also covers the self pair)
+                const bool ok = ivalid && (jb + jj < n) && !(z0 && zi) && !(diag && k == kTile / 2 && lane >= kTile / 2);
+                const double mS = z0 ? 0.0 : (h ? 0.5 : 1.0);
+                const double iS = zi ? 0.0 : 1.0;
+                ElecSite other;
+                load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
+                elec_sym_step<MUTUAL, true>(me, my_orig, other, s_o[jj], ok, mS, iS, PBC, box, acc, jacc, en);
+                jacc.fx = rot1(jacc.fx); jacc.fy = rot1(jacc.fy); jacc.fz = rot1(jacc.fz);
+                jacc.tx = rot1(jacc.tx); jacc.ty = rot1(jacc.ty); jacc.tz = rot1(jacc.tz);
             }
+            kend = k1 + 1;
+        }
+        const size_t slot = col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1));
+        store3(PjF, S, slot, jacc.fx, jacc.fy, jacc.fz);
+        store3(PjT, S, slot, jacc.tx, jacc.ty, jacc.tz);
+    }
+    PiF[(size_t)w * kTile + lane] = make_double4(acc.fx, acc.fy, acc.fz, 0.0);
+    PiT[(size_t)w * kTile + lane] = make_double4(acc.tx, acc.ty, acc.tz, 0.0);
+    en = wave_sum(en);
+    if (lane == 0) part_e[w] = en;
+}
+
+// ------------------------------------------------------------------ fixed-order reduction
+// Sum of the symmetric partials of tile column B (one workgroup of kRedWaves wavefronts per
+// column): wave v adds the i partials of work items row_start[B] + v, + v + kRedWaves, ... and
+// the j partials of tiles (I', B) for I' = v, v + kRedWaves, ...; the waves' sums are then added
+// in wave order.  The order is fixed, so the result is deterministic.  out4 (double4 per
+// particle) or out3 (three doubles per particle) receives the sums.
+constexpr int kRedWaves = 8;
+__global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(int n, const int* __restrict__ row_start,
+                                                               const double4* __restrict__ Pi,
+                                                               const double* __restrict__ Pj, size_t S,
+                                                               const double* __restrict__ scalars,
+                                                               double4* __restrict__ out4, double* __restrict__ out3) {
+    if (scalars && solver_done(scalars)) return;
+    __shared__ double red[3][kRedWaves][kTile];
+    const int B = blockIdx.x, l = threadIdx.x & (kTile - 1), v = threadIdx.x / kTile;
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    for (int w = row_start[B] + v; w < row_start[B + 1]; w += kRedWaves) {
+        const double4 p = Pi[(size_t)w * kTile + l];
+        sx += p.x;
+        sy += p.y;
+        sz += p.z;
+    }
+    const size_t col = col_tile(0, B);
+#pragma unroll 4
+    for (int Ip = v; Ip <= B; Ip += kRedWaves) {
+        const size_t k = (col + Ip) * kTile + l;
+        sx += Pj[k];
+        sy += Pj[S + k];
+        sz += Pj[2 * S + k];
+    }
+    red[0][v][l] = sx;
+    red[1][v][l] = sy;
+    red[2][v][l] = sz;
+    __syncthreads();
+    const int x = B * kTile + l;
+    if (v == 0 && x < n) {
+        for (int u = 1; u < kRedWaves; u++) {
+            sx += red[0][u][l];
+            sy += red[1][u][l];
+            sz += red[2][u][l];
+        }
+        if (out3) {
+            out3[3 * x] = sx;
+            out3[3 * x + 1] = sy;
+            out3[3 * x + 2] = sz;
+        } else {
+            out4[x] = make_double4(sx, sy, sz, 0.0);
         }
     }
-    part_f[(size_t)split * npad + i] = make_double4(fx, fy, fz, 0.0);
-    part_t[(size_t)split * npad + i] = make_double4(tx, ty, tz, 0.0);
-    if (i >= n) en = 0.0;
-    en = wave_sum(en);
-    if (lane == 0) part_e[(size_t)split * gridDim.x + I] = en;
 }
 
-// ------------------------------------------------------------------ split reductions
-__device__ __forceinline__ double4 sum_parts(const double4* __restrict__ part, int nsplit, int npad, int i) {
-    double4 s = part[i];
-    for (int k = 1; k < nsplit; k++) {
-        double4 v = part[(size_t)k * npad + i];
-        s.x += v.x; s.y += v.y; s.z += v.z;
-    }
-    return s;
-}
+// ------------------------------------------------------------------ solver vector kernels
 
 // E0 = sum of split partials; mu = alpha*E0 (calculateInducedDipoles, ReferenceTholeDipoleForce.cpp:743-746).
-__global__ void k_field_init(int n, int npad, int nsplit, const double4* __restrict__ part,
+__global__ void k_field_init(int n, int npad, const double4* __restrict__ part,
                              const double* __restrict__ alpha, double4* __restrict__ E0, double4* __restrict__ mu,
                              float4* __restrict__ mu32) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
     double4 f = make_double4(0, 0, 0, 0), m = make_double4(0, 0, 0, 0);
     if (i < n) {
-        f = sum_parts(part, nsplit, npad, i);
+        f = part[i];
         f.w = 0.0;
         double a = alpha[i];
         m = make_double4(f.x * a, f.y * a, f.z * a, 0.0);
@@ -647,7 +841,7 @@ __device__ __forceinline__ void block_sum2(double a, double b, double* __restric
 
 // One Jacobi step of convergeInducedDipolesByDIIS (ReferenceTholeDipoleForce.cpp:835-846):
 // mu_new = alpha*(E0 + T mu); partial sums of |mu_new - mu|^2.
-__global__ __launch_bounds__(256) void k_jacobi_update(int n, int npad, int nsplit, const double4* __restrict__ part,
+__global__ __launch_bounds__(256) void k_jacobi_update(int n, int npad, const double4* __restrict__ part,
                                                        const double4* __restrict__ E0, const double* __restrict__ alpha,
                                                        double4* __restrict__ mu, float4* __restrict__ mu32,
                                                        const double* __restrict__ scalars, double* __restrict__ red,
@@ -656,7 +850,7 @@ __global__ __launch_bounds__(256) void k_jacobi_update(int n, int npad, int nspl
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
     if (i < n) {
-        double4 f = sum_parts(part, nsplit, npad, i);
+        double4 f = part[i];
         double4 e0 = E0[i];
         double a = alpha[i];
         double nx = (e0.x + f.x) * a, ny = (e0.y + f.y) * a, nz = (e0.z + f.z) * a;
@@ -686,7 +880,7 @@ __global__ void k_jacobi_finalize(int n, int nred, const double* __restrict__ re
 // ---- Jacobi-preconditioned CG on (alpha^-1 - T) mu = E0, stopped on the same RMS(alpha r) the
 // Jacobi iteration uses (alpha r is exactly the Jacobi update).
 // init: x = mu (= alpha E0), r = E0 - x/alpha + T x, z = alpha r, p = z.
-__global__ __launch_bounds__(256) void k_pcg_init(int n, int npad, int nsplit, const double4* __restrict__ part,
+__global__ __launch_bounds__(256) void k_pcg_init(int n, int npad, const double4* __restrict__ part,
                                                   const double4* __restrict__ E0, const double* __restrict__ alpha,
                                                   const double4* __restrict__ x, double4* __restrict__ r,
                                                   double4* __restrict__ p, float4* __restrict__ p32,
@@ -696,7 +890,7 @@ __global__ __launch_bounds__(256) void k_pcg_init(int n, int npad, int nsplit, c
     if (i < npad) {
         double4 rv = make_double4(0, 0, 0, 0), zv = make_double4(0, 0, 0, 0);
         if (i < n) {
-            double4 f = sum_parts(part, nsplit, npad, i);
+            double4 f = part[i];
             double4 e0 = E0[i], xv = x[i];
             double a = alpha[i];
             double ia = 1.0 / a;
@@ -727,7 +921,7 @@ __global__ void k_pcg_init_finalize(int n, int nred, const double* __restrict__ 
 }
 
 // q = p/alpha - T p; partial p.q
-__global__ __launch_bounds__(256) void k_pcg_q(int n, int npad, int nsplit, const double4* __restrict__ part,
+__global__ __launch_bounds__(256) void k_pcg_q(int n, int npad, const double4* __restrict__ part,
                                                const double* __restrict__ alpha, const double4* __restrict__ p,
                                                double4* __restrict__ q, const double* __restrict__ scalars,
                                                double* __restrict__ red, int red_stride) {
@@ -735,7 +929,7 @@ __global__ __launch_bounds__(256) void k_pcg_q(int n, int npad, int nsplit, cons
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double pq = 0.0;
     if (i < n) {
-        double4 f = sum_parts(part, nsplit, npad, i);
+        double4 f = part[i];
         double4 pv = p[i];
         double ia = 1.0 / alpha[i];
         double4 qv = make_double4(pv.x * ia - f.x, pv.y * ia - f.y, pv.z * ia - f.z, 0.0);
@@ -813,12 +1007,12 @@ __global__ __launch_bounds__(256) void k_pcg_direction(int n, int npad, const do
 
 // Extrapolated polarization (convergeInducedDipolesByExtrapolation, ReferenceTholeDipoleForce.cpp:876-887):
 // mu = alpha*(E0 + T mu); hist[o] = mu.
-__global__ void k_extrap_step(int n, int npad, int nsplit, const double4* __restrict__ part,
+__global__ void k_extrap_step(int n, int npad, const double4* __restrict__ part,
                               const double4* __restrict__ E0, const double* __restrict__ alpha,
                               double4* __restrict__ mu, float4* __restrict__ mu32, double4* __restrict__ hist_o) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double4 f = sum_parts(part, nsplit, npad, i);
+    double4 f = part[i];
     double4 e0 = E0[i];
     double a = alpha[i];
     double4 m = make_double4((e0.x + f.x) * a, (e0.y + f.y) * a, (e0.z + f.z) * a, 0.0);
@@ -843,14 +1037,14 @@ __global__ void k_extrap_combine(int n, int npad, int order, const double4* __re
 }
 
 // final error of the extrapolated dipoles (:897-906): partial sums of |alpha(E0+T mu) - mu|^2
-__global__ __launch_bounds__(256) void k_extrap_error(int n, int npad, int nsplit, const double4* __restrict__ part,
+__global__ __launch_bounds__(256) void k_extrap_error(int n, int npad, const double4* __restrict__ part,
                                                       const double4* __restrict__ E0, const double* __restrict__ alpha,
                                                       const double4* __restrict__ mu, double* __restrict__ red,
                                                       int red_stride) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     double d2 = 0.0;
     if (i < n) {
-        double4 f = sum_parts(part, nsplit, npad, i);
+        double4 f = part[i];
         double4 e0 = E0[i], m = mu[i];
         double a = alpha[i];
         double dx = (e0.x + f.x) * a - m.x, dy = (e0.y + f.y) * a - m.y, dz = (e0.z + f.z) * a - m.z;
@@ -867,19 +1061,6 @@ __global__ void k_sum_eps(int n, int nred, const double* __restrict__ red, doubl
 }
 
 // Forces/torques: sum split partials.  frc is [3n] doubles (accumulated later into the caller's array).
-__global__ void k_reduce_forces(int n, int npad, int nsplit, const double4* __restrict__ part_f,
-                                const double4* __restrict__ part_t, double* __restrict__ frc,
-                                double4* __restrict__ torque) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double4 f = sum_parts(part_f, nsplit, npad, i);
-    double4 t = sum_parts(part_t, nsplit, npad, i);
-    frc[3 * i] = f.x;
-    frc[3 * i + 1] = f.y;
-    frc[3 * i + 2] = f.z;
-    torque[i] = make_double4(t.x, t.y, t.z, 0.0);
-}
-
 __global__ void k_reduce_energy(int count, const double* __restrict__ part_e, double* __restrict__ out) {
     __shared__ double s[256];
     int t = threadIdx.x;
@@ -1161,9 +1342,10 @@ static BoxT64 mkbox(const double* b3) {
 }
 
 void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
-                           double4* posa, float4* posa32) {
-    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, perm, thole, posa,
-                       posa32);
+                           bool pbc, const double* box3, double4* posa, float4* posa32) {
+    BoxT64 b = mkbox(box3);
+    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, perm, thole,
+                       pbc ? 1 : 0, b, posa, posa32);
 }
 
 void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
@@ -1177,123 +1359,106 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
     hipLaunchKernelGGL(k_rotate_dipoles, dim3((n + 255) / 256), dim3(256), 0, s, n, posa, dmol, charge, axis, qd);
 }
 
-void launch_fixed_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
-                        double4* part) {
-    dim3 grid(L.nblk, L.nsplit);
-    BoxT64 b = mkbox(L.box);
-    if (L.pbc)
-        hipLaunchKernelGGL(k_fixed_field<true>, grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, part);
-    else
-        hipLaunchKernelGGL(k_fixed_field<false>, grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd,
-                           inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, part);
-}
-
-void launch_induced_field64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
-                            const double* inva, const double* scalars, double4* part) {
-    dim3 grid(L.nblk, L.nsplit);
-    BoxT64 b = mkbox(L.box);
-    if (L.pbc)
-        hipLaunchKernelGGL((k_induced_field<double, true>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
-    else
-        hipLaunchKernelGGL((k_induced_field<double, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
-}
-
-void launch_induced_field32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
-                            const double* scalars, double4* part) {
-    dim3 grid(L.nblk, L.nsplit);
-    BoxT64 b = mkbox(L.box);
-    if (L.pbc)
-        hipLaunchKernelGGL((k_induced_field<float, true>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
-    else
-        hipLaunchKernelGGL((k_induced_field<float, false>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split,
-                           posa, mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, part);
+static void col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj, const double* scalars,
+                       double4* out4, double* out3) {
+    hipLaunchKernelGGL(k_col_reduce, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.n, L.row_start, Pi, Pj,
+                       L.slab, scalars, out4, out3);
 }
 
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
-                         const double* scalars, double4* Pi, double4* Pj, double4* out) {
-    // Pi and Pj are the two parts of one partial buffer
+                         const double* scalars, double4* Pi, double* Pj, double4* out) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL((k_sym_matvec<double, true>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
-                           inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
+        hipLaunchKernelGGL(k_sym_matvec<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu, inva,
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
     else
-        hipLaunchKernelGGL((k_sym_matvec<double, false>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
-                           mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
-    hipLaunchKernelGGL(k_sym_reduce<double4>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.nblk, L.row_start, Pi,
-                       Pj, scalars, out);
+        hipLaunchKernelGGL(k_sym_matvec<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu, inva,
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
+    col_reduce(s, L, Pi, Pj, scalars, out, nullptr);
 }
 
-void launch_sym_matvec32(hipStream_t s, const PairLaunch& L, const float4* posa, const float4* mu, const float* inva,
-                         const double* scalars, double4* Pi, float4* Pj, double4* out) {
+void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const float4* posa32,
+                             const double4* mu, const float4* mu32, const double* inva, const double* scalars,
+                             double4* Pi, double* Pj, double4* out) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL((k_sym_matvec<float, true>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
-                           inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
+        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, posa32,
+                           mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
+                           L.slab);
     else
-        hipLaunchKernelGGL((k_sym_matvec<float, false>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
-                           mu, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj);
-    hipLaunchKernelGGL(k_sym_reduce<float4>, dim3((L.n + 255) / 256), dim3(256), 0, s, L.n, L.nblk, L.row_start, Pi,
-                       Pj, scalars, out);
+        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
+                           posa32, mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
+                           L.slab);
+    col_reduce(s, L, Pi, Pj, scalars, out, nullptr);
 }
 
-void launch_electrostatic(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
-                          const double4* mu, const double* inva, const int* orig, double4* part_f, double4* part_t,
-                          double* part_e) {
-    dim3 grid(L.nblk, L.nsplit);
+void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
+                      double4* Pi, double* Pj, double4* out) {
     BoxT64 b = mkbox(L.box);
-#define TDP_ELEC(P, M)                                                                                             \
-    hipLaunchKernelGGL((k_electrostatic<P, M>), grid, dim3(kTile), 0, s, L.n, L.npad, L.tiles_per_split, posa, qd, \
-                       mu, inva, orig, L.exc_start, L.exc_J, L.exc_mask, b, part_f, part_t, part_e)
+    if (L.pbc)
+        hipLaunchKernelGGL(k_sym_field<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, inva,
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
+    else
+        hipLaunchKernelGGL(k_sym_field<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, inva,
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
+    col_reduce(s, L, Pi, Pj, nullptr, out, nullptr);
+}
+
+void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
+                     const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
+                     double* PjT, double* part_e, double* frc, double4* torque) {
+    BoxT64 b = mkbox(L.box);
+#define TDP_SELEC(P, M)                                                                                              \
+    hipLaunchKernelGGL((k_sym_elec<P, M>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, mu, inva, \
+                       orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, PiF, PjF, PiT, PjT, L.slab, part_e)
     if (L.pbc) {
-        if (mutual) TDP_ELEC(true, true); else TDP_ELEC(true, false);
+        if (mutual) TDP_SELEC(true, true); else TDP_SELEC(true, false);
     } else {
-        if (mutual) TDP_ELEC(false, true); else TDP_ELEC(false, false);
+        if (mutual) TDP_SELEC(false, true); else TDP_SELEC(false, false);
     }
-#undef TDP_ELEC
+#undef TDP_SELEC
+    col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc);
+    col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
 }
 
-void launch_field_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+void launch_field_init(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
                        double4* E0, double4* mu, float4* mu32) {
-    hipLaunchKernelGGL(k_field_init, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, nsplit, part, alpha, E0, mu,
+    hipLaunchKernelGGL(k_field_init, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, part, alpha, E0, mu,
                        mu32);
 }
 
 int red_blocks(int npad) { return (npad + 255) / 256; }
 
-void launch_jacobi_update(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_jacobi_update(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                           const double* alpha, double4* mu, float4* mu32, const double* scalars, double* red) {
     int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_jacobi_update, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, mu, mu32, scalars,
+    hipLaunchKernelGGL(k_jacobi_update, dim3(nb), dim3(256), 0, s, n, npad, part, E0, alpha, mu, mu32, scalars,
                        red, nb);
     hipLaunchKernelGGL(k_jacobi_finalize, dim3(1), dim3(64), 0, s, n, nb, red, (double*)scalars);
 }
 
-void launch_pcg_init(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_pcg_init(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                      const double* alpha, const double4* x, double4* r, double4* p, float4* p32, double* red,
                      double* scalars) {
     int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, x, r, p, p32, red, nb);
+    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s, n, npad, part, E0, alpha, x, r, p, p32, red, nb);
     hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
 }
 
-void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double* alpha,
+void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
                                double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
                                double* scalars) {
     int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_pcg_q, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, alpha, p, q, scalars, red, nb);
+    hipLaunchKernelGGL(k_pcg_q, dim3(nb), dim3(256), 0, s, n, npad, part, alpha, p, q, scalars, red, nb);
     hipLaunchKernelGGL(k_pcg_gamma, dim3(1), dim3(64), 0, s, nb, red, scalars);
     hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, s, n, alpha, p, q, x, r, z, scalars, red, nb);
     hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
     hipLaunchKernelGGL(k_pcg_direction, dim3(nb), dim3(256), 0, s, n, npad, z, p, p32, scalars);
 }
 
-void launch_extrap_step(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_extrap_step(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                         const double* alpha, double4* mu, float4* mu32, double4* hist_o) {
-    hipLaunchKernelGGL(k_extrap_step, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, mu,
+    hipLaunchKernelGGL(k_extrap_step, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, part, E0, alpha, mu,
                        mu32, hist_o);
 }
 
@@ -1302,17 +1467,11 @@ void launch_extrap_combine(hipStream_t s, int n, int npad, int order, const doub
     hipLaunchKernelGGL(k_extrap_combine, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, order, hist, coef, mu, mu32);
 }
 
-void launch_extrap_error(hipStream_t s, int n, int npad, int nsplit, const double4* part, const double4* E0,
+void launch_extrap_error(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
                          const double* alpha, const double4* mu, double* red, double* scalars) {
     int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_extrap_error, dim3(nb), dim3(256), 0, s, n, npad, nsplit, part, E0, alpha, mu, red, nb);
+    hipLaunchKernelGGL(k_extrap_error, dim3(nb), dim3(256), 0, s, n, npad, part, E0, alpha, mu, red, nb);
     hipLaunchKernelGGL(k_sum_eps, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
-}
-
-void launch_reduce_forces(hipStream_t s, int n, int npad, int nsplit, const double4* part_f, const double4* part_t,
-                          double* frc, double4* torque) {
-    hipLaunchKernelGGL(k_reduce_forces, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, nsplit, part_f, part_t, frc,
-                       torque);
 }
 
 void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double* out) {

@@ -9,8 +9,8 @@ in HBM (tdp_execute_device).
 The reported value is the mixed-precision path (T*mu matvecs with tile pairs beyond the Thole
 range in FP32 and the rest in FP64, all partial sums in FP64, PCG with FP64 norms); the FP64
 path with the same solver is timed beside it.  The roofline entry covers the dominant kernel, the
-T*mu pass (k_sym_matvec_mixed and its fixed-order reduction k_sym_reduce), timed with HIP events
-on the context's stream over a timed region of its own.  Its ceiling is the compute roofline at
+T*mu pass (k_sym_matvec_mixed, the pair terms), timed with HIP events on the context's stream over
+a timed region of its own; its fixed-order reduction is a separate, HBM-bound kernel.  Its ceiling is the compute roofline at
 MI355X's published dense FP32 rate: the pair arithmetic runs on the vector ALUs.  The CPU baseline is the oracle (a C restatement of the reference loops)
 timed on a bounded sample of the same workload's pair passes.
 """
@@ -33,6 +33,10 @@ FLOPS_PER_PAIR_MATVEC = 72.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
+# HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1): 116.5 MB read + 256.8 MB
+# written (the tile partial slab).  Counters cannot be read inside this process.
+MATVEC_HBM_BYTES = 373.3e6
 
 
 def parse():
@@ -43,6 +47,9 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
+    p.add_argument("--copy-probe-mib", type=int, default=0,
+                   help="after the run, stream this many MiB through a plain device copy (y = x * 1 in float32) "
+                        "three times: the reference kernel for scaling HBM byte counters")
     return p.parse_args()
 
 
@@ -184,8 +191,8 @@ def main():
         "mixed_energy_rel_err_vs_fp64": e_err,
         "mixed_mu_max_err_vs_fp64_debye": mu_err,
         "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_sym_matvec_mixed<true> + k_sym_reduce<double4> (one T*mu pass)",
+                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": MATVEC_HBM_BYTES,
+                     "kernel": "k_sym_matvec_mixed<true> (pair terms of one T*mu pass)",
                      "ceiling": "compute roofline at the published dense FP32 rate; the pair terms run on the "
                                 "vector ALUs (no MFMA instruction)",
                      "flops_per_pair": FLOPS_PER_PAIR_MATVEC, "pairs_per_launch": n * (n - 1) // 2,
@@ -193,6 +200,15 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+    if args.copy_probe_mib > 0:
+        x = torch.ones(args.copy_probe_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print("# copy probe: %d MiB read + %d MiB written per launch, 3 launches" % (args.copy_probe_mib,
+                                                                                    args.copy_probe_mib),
+              file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

@@ -475,7 +475,8 @@ void harvest_timing(tdp_context* c) {
     c->pending.clear();
 }
 
-// T * v for the solver: v64 is mu or p; in mixed precision vec32 holds its FP32 copy.
+// T * v for the solver: v64 is mu or p; in mixed precision vec32 holds its FP32 copy.  With
+// timing on, events bracket the pair kernel (not the reduction that follows it).
 void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const double* scalars) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
@@ -483,19 +484,17 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         e1 = take_event(c);
         hipEventRecord(e0, c->stream);
     }
-    if (c->precision == TDP_PRECISION_MIXED) {
-        double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
-        launch_sym_matvec_mixed(c->stream, L, c->posa, c->posa32, v64, c->vec32, c->d_inva, scalars, pi,
-                                reinterpret_cast<double*>(pi + (size_t)c->nwork * 64), c->Emv);
-    } else {
-        double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
-        launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi,
-                            reinterpret_cast<double*>(pi + (size_t)c->nwork * 64), c->Emv);
-    }
+    double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
+    double* pj = reinterpret_cast<double*>(pi + (size_t)c->nwork * 64);
+    if (c->precision == TDP_PRECISION_MIXED)
+        launch_sym_matvec_mixed(c->stream, L, c->posa, c->posa32, v64, c->vec32, c->d_inva, scalars, pi, pj);
+    else
+        launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pj);
     if (c->timing) {
         hipEventRecord(e1, c->stream);
         c->pending.push_back({e0, e1});
     }
+    launch_col_reduce(c->stream, L, pi, pj, scalars, c->Emv, nullptr);
 }
 
 int check_box(tdp_context* c, const double* box) {

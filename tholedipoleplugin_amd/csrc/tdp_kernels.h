@@ -36,12 +36,15 @@ struct PairLaunch {
 };
 
 // Pairwise passes over the upper-triangle work items.  Pi: nwork*64 item partials; Pj: the
-// column-major tile partial slab, 3 * L.slab doubles.
+// column-major tile partial slab, 3 * L.slab doubles.  The T*mu passes leave their partials for
+// launch_col_reduce, which writes T*mu to out4 (double4 per particle) or out3 (3 doubles).
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
-                         const double* scalars, double4* Pi, double* Pj, double4* out);
+                         const double* scalars, double4* Pi, double* Pj);
+void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj, const double* scalars,
+                       double4* out4, double* out3);
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const float4* posa32,
                              const double4* mu, const float4* mu32, const double* inva, const double* scalars,
-                             double4* Pi, double* Pj, double4* out);
+                             double4* Pi, double* Pj);
 
 void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h);

@@ -1359,14 +1359,14 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
     hipLaunchKernelGGL(k_rotate_dipoles, dim3((n + 255) / 256), dim3(256), 0, s, n, posa, dmol, charge, axis, qd);
 }
 
-static void col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj, const double* scalars,
-                       double4* out4, double* out3) {
+void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj,
+                       const double* scalars, double4* out4, double* out3) {
     hipLaunchKernelGGL(k_col_reduce, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.n, L.row_start, Pi, Pj,
                        L.slab, scalars, out4, out3);
 }
 
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
-                         const double* scalars, double4* Pi, double* Pj, double4* out) {
+                         const double* scalars, double4* Pi, double* Pj) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu, inva,
@@ -1374,12 +1374,11 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
     else
         hipLaunchKernelGGL(k_sym_matvec<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu, inva,
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
-    col_reduce(s, L, Pi, Pj, scalars, out, nullptr);
 }
 
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const float4* posa32,
                              const double4* mu, const float4* mu32, const double* inva, const double* scalars,
-                             double4* Pi, double* Pj, double4* out) {
+                             double4* Pi, double* Pj) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, posa32,
@@ -1389,7 +1388,6 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
         hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
                            posa32, mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
                            L.slab);
-    col_reduce(s, L, Pi, Pj, scalars, out, nullptr);
 }
 
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
@@ -1401,7 +1399,7 @@ void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, c
     else
         hipLaunchKernelGGL(k_sym_field<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, inva,
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
-    col_reduce(s, L, Pi, Pj, nullptr, out, nullptr);
+    launch_col_reduce(s, L, Pi, Pj, nullptr, out, nullptr);
 }
 
 void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
@@ -1417,8 +1415,8 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const doub
         if (mutual) TDP_SELEC(false, true); else TDP_SELEC(false, false);
     }
 #undef TDP_SELEC
-    col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc);
-    col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
+    launch_col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc);
+    launch_col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
 }
 
 void launch_field_init(hipStream_t s, int n, int npad, const double4* part, const double* alpha,

@@ -369,53 +369,60 @@ int upload_tables(tdp_context* c) {
     return TDP_OK;
 }
 
-// Internal spatial order: particles sorted by the Morton (Z-order) code of their position on a
-// 1024^3 grid over the box (PME: wrapped into the box) or the bounding box (NoCutoff), ties by
-// index.  Tiles of 64 consecutive particles are then compact, so a j tile is either near or far
-// from a whole i tile: the Thole exponential and the masked exclusion path run for few tiles.
+// Internal spatial order: recursive bisection.  The particles (PME: wrapped into the box) are
+// split along the longest extent of their bounding box into a part holding floor(t/2) whole tiles
+// of 64 and the rest, t = ceil(count/64), until each part is one tile; a tile is therefore a compact,
+// nearly cubic group, and a j tile is either near or far from a whole i tile, so the Thole
+// exponential and the masked exclusion path run for few tiles.  Splits order particles by
+// (coordinate, index), so the order is deterministic; inside a tile particles are in index order.
 // The order only changes which thread evaluates which pair; results are returned in the caller's order.
 void compute_order(tdp_context* c, const double* pos3, const double* box) {
     const int n = c->n;
-    double lo[3], span[3];
     const bool pbc = c->method == TDP_PME && box;
-    for (int d = 0; d < 3; d++) {
-        if (pbc) {
-            lo[d] = 0.0;
-            span[d] = box[4 * d];
-        } else {
-            double a = 1e300, b = -1e300;
-            for (int i = 0; i < n; i++) {
-                a = std::min(a, pos3[3 * i + d]);
-                b = std::max(b, pos3[3 * i + d]);
-            }
-            lo[d] = a;
-            span[d] = b - a;
-        }
-        if (!(span[d] > 0.0)) span[d] = 1.0;
-    }
-    auto spread = [](unsigned v) {
-        unsigned long long x = v & 0x3ff;
-        x = (x | (x << 16)) & 0x30000ffull;
-        x = (x | (x << 8)) & 0x300f00full;
-        x = (x | (x << 4)) & 0x30c30c3ull;
-        x = (x | (x << 2)) & 0x9249249ull;
-        return x;
-    };
-    std::vector<std::pair<unsigned long long, int>> key(n);
-    for (int i = 0; i < n; i++) {
-        unsigned q[3];
+    std::vector<double> w(3 * (size_t)n);
+    for (int i = 0; i < n; i++)
         for (int d = 0; d < 3; d++) {
-            double u = (pos3[3 * i + d] - lo[d]) / span[d];
-            if (pbc) u -= std::floor(u);
-            if (!(u >= 0.0)) u = 0.0;   // also catches NaN
-            q[d] = (unsigned)std::min(1023.0, u * 1024.0);
+            double x = pos3[3 * i + d];
+            if (pbc && box[4 * d] > 0.0) {
+                double L = box[4 * d];
+                x -= L * std::floor(x / L);
+            }
+            w[3 * (size_t)i + d] = std::isfinite(x) ? x : 0.0;
         }
-        key[i] = {spread(q[0]) | (spread(q[1]) << 1) | (spread(q[2]) << 2), i};
+    std::vector<int> ids(n);
+    for (int i = 0; i < n; i++) ids[i] = i;
+    struct Range { int b, e; };
+    std::vector<Range> stack{{0, n}};
+    while (!stack.empty()) {
+        Range r = stack.back();
+        stack.pop_back();
+        const int m = r.e - r.b;
+        if (m <= 64) {
+            std::sort(ids.begin() + r.b, ids.begin() + r.e);
+            continue;
+        }
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int k = r.b; k < r.e; k++)
+            for (int d = 0; d < 3; d++) {
+                double x = w[3 * (size_t)ids[k] + d];
+                lo[d] = std::min(lo[d], x);
+                hi[d] = std::max(hi[d], x);
+            }
+        int ax = 0;
+        for (int d = 1; d < 3; d++)
+            if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+        const int tiles = (m + 63) / 64;
+        const int left = (tiles / 2) * 64;
+        std::nth_element(ids.begin() + r.b, ids.begin() + r.b + left, ids.begin() + r.e, [&](int a, int b2) {
+            double xa = w[3 * (size_t)a + ax], xb = w[3 * (size_t)b2 + ax];
+            return xa < xb || (xa == xb && a < b2);
+        });
+        stack.push_back({r.b + left, r.e});
+        stack.push_back({r.b, r.b + left});
     }
-    std::sort(key.begin(), key.end());
     for (int k = 0; k < n; k++) {
-        c->perm[k] = key[k].second;
-        c->inv[key[k].second] = k;
+        c->perm[k] = ids[k];
+        c->inv[ids[k]] = k;
     }
 }
 
@@ -735,12 +742,14 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         return bail(TDP_ERR_HIP);
     }
     const int npad = c->npad;
-    // Upper-triangle work items: tile row I split into chunks of C tiles (J >= I), C chosen for
-    // >= ~8192 one-wave work items on large systems.
+    // Upper-triangle work items: tile row I split into chunks of C tiles (J >= I).  Short items
+    // (C = 3 on large systems) keep the many one-wave workgroups evenly spread over the CUs to the
+    // end of the launch; measured at 36,864 sites, C = 3 beats C = 20 by 25% on the T*mu pass, and
+    // C = 1 loses more in the reduction than it gains.
     {
         const int nb = c->nblk;
         c->ntri = (size_t)nb * (nb + 1) / 2;
-        int C = (int)std::max<size_t>(1, c->ntri / 8192);
+        const int C = (int)std::max<size_t>(1, std::min<size_t>(3, c->ntri / 16384));
         std::vector<int4> work;
         std::vector<int> row_start(nb + 1, 0);
         for (int I = 0; I < nb; I++) {
@@ -766,7 +775,7 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
         (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||
         (rc = dalloc(c, &c->torque, npad)) ||
-        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || (rc = dalloc(c, &c->posa32, npad)) ||
+        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || (rc = dalloc(c, &c->posa32, 2 * (size_t)npad)) ||
         (rc = dalloc(c, &c->vec32, npad)) || (rc = dalloc(c, &c->d_pos3, 3 * (size_t)npad)) ||
         (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_alpha, npad)) ||
         (rc = dalloc(c, &c->d_inva, npad)) || (rc = dalloc(c, &c->d_inva32, npad)) ||

@@ -110,15 +110,19 @@ __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos
     posa[i] = p;
     if (!posa32) return;
     if (pbc) {
-        // fraction of the box edge in [0, 1), in units of 2^-32, kept modulo 2^32: the wrapped
-        // difference of two such values is the minimum-image separation in the same units
+        // fraction of the box edge in [0, 1) as a 64-bit fixed-point number (units of 2^-64), kept
+        // modulo 2^64: the wrapped difference of two such values is the minimum-image separation.
+        // High words go to posa32[i], low words to posa32[npad + i].
         auto quant = [](double x, double ib) {
             double f = x * ib;
             f -= floor(f);
-            return (unsigned)(unsigned long long)(long long)rint(f * 0x1p32);
+            return (unsigned long long)(f * 0x1p64);
         };
-        posa32[i] = make_float4(__uint_as_float(quant(p.x, box.ibx)), __uint_as_float(quant(p.y, box.iby)),
-                                __uint_as_float(quant(p.z, box.ibz)), (float)p.w);
+        const unsigned long long qx = quant(p.x, box.ibx), qy = quant(p.y, box.iby), qz = quant(p.z, box.ibz);
+        posa32[i] = make_float4(__uint_as_float((unsigned)(qx >> 32)), __uint_as_float((unsigned)(qy >> 32)),
+                                __uint_as_float((unsigned)(qz >> 32)), (float)p.w);
+        posa32[npad + i] = make_float4(__uint_as_float((unsigned)qx), __uint_as_float((unsigned)qy),
+                                       __uint_as_float((unsigned)qz), 0.0f);
     } else {
         const int o0 = perm[0];
         posa32[i] = make_float4((float)(p.x - pos3[3 * o0]), (float)(p.y - pos3[3 * o0 + 1]),
@@ -214,7 +218,7 @@ __device__ __forceinline__ void store3(double* __restrict__ P, size_t S, size_t 
 
 // value of lane (l + 1) mod 64 (all lanes must be active)
 __device__ __forceinline__ float rot1(float v) {
-    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x134, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), 0x134, 0xF, 0xF, true));
 }
 __device__ __forceinline__ double rot1(double v) {
     long long u = __double_as_longlong(v);
@@ -246,15 +250,15 @@ __device__ __forceinline__ void damped_rr_wave(double r2, double ai, double aj, 
     }
 }
 
-// Separation of two quantized periodic coordinates (fractions of the box edge in units of 2^-32,
-// see k_load_positions): the wrapped 32-bit difference is the minimum image.  Returns false when
-// the separation lies within kQuantSlack units of half a box, where the quantized and the FP64
-// image choice could differ; such a pair is imaged again in FP64.
-constexpr unsigned kQuantSlack = 4u;
-__device__ __forceinline__ bool quant_delta(unsigned qi, unsigned qj, float scale, float& d) {
-    int dq = (int)(qj - qi);
-    d = (float)dq * scale;
-    return ((unsigned)dq + 0x80000000u + kQuantSlack) >= 2u * kQuantSlack;
+// Separation of two periodic coordinates held as 64-bit fixed-point box fractions (high and low
+// words as float bit patterns, see k_load_positions): the wrapped 64-bit difference is the minimum
+// image, decided to 2^-64 of a box, finer than the FP64 rule resolves; its high word, as a float,
+// is the separation in units of 2^-32 box edges.
+__device__ __forceinline__ float fixed_delta(float hi_i, float lo_i, float hi_j, float lo_j) {
+    unsigned borrow, unused;
+    (void)__builtin_subc(__float_as_uint(lo_j), __float_as_uint(lo_i), 0u, &borrow);
+    const unsigned h = __builtin_subc(__float_as_uint(hi_j), __float_as_uint(hi_i), borrow, &unused);
+    return (float)(int)h;
 }
 
 // ------------------------------------------------------------------ T * mu, FP64 (a7)
@@ -369,14 +373,13 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
 // Tile pairs inside the Thole range, and the diagonal and exclusion tiles, are evaluated in FP64
 // as in k_sym_matvec.  Tile pairs beyond it, most of the pairs of a large system, carry only the
 // undamped 1/r^3 and 3/r^5 terms and are evaluated in FP32 with FP32 dipoles.  Their separations
-// come from quantized periodic coordinates (PBC; the 32-bit difference is the minimum image, and
-// pairs within a few units of half a box are imaged again from the FP64 positions so that the
-// image always agrees with the FP64 rule) or from FP32 offsets to a common origin (no PBC).  The
-// near pairs carry the largest terms, so keeping them in FP64 holds the operator's error to that
-// of the FP32 far field; all partial sums are FP64.
+// come from 64-bit fixed-point box fractions (PBC; the wrapped difference is the minimum image, so
+// the image agrees with the FP64 rule even for pairs a hair from half a box) or from FP32 offsets
+// to a common origin (no PBC).  The near pairs carry the largest terms, so keeping them in FP64
+// holds the operator's error to that of the FP32 far field; all partial sums are FP64.
 template <bool PBC>
 __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const int4* __restrict__ work,
-                                                         const double4* __restrict__ posa,
+                                                         int npad, const double4* __restrict__ posa,
                                                          const float4* __restrict__ posa32,
                                                          const double4* __restrict__ mu,
                                                          const float4* __restrict__ mu32,
@@ -388,11 +391,18 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
                                                          BoxT64 box, const double* __restrict__ scalars,
                                                          double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     if (scalars && solver_done(scalars)) return;
-    __shared__ double4 s_p[kTile];
-    __shared__ double4 s_u[kTile];
-    __shared__ double s_ia[kTile];
-    __shared__ float4 s_p32[kTile];
-    __shared__ float4 s_u32[kTile];
+    // One LDS area holds either the FP64 tile (near path) or the FP32 tile (far path).  The FP32
+    // tile is staged twice over, so that slot (lane + k) mod 64 is entry lane + k: the step's
+    // address is the lane's base plus a constant offset.
+    constexpr int kBytes64 = kTile * (2 * sizeof(double4) + sizeof(double));
+    constexpr int kBytes32 = 2 * kTile * 3 * sizeof(float4);
+    __shared__ __align__(16) unsigned char s_raw[kBytes64 > kBytes32 ? kBytes64 : kBytes32];
+    double4* s_p = reinterpret_cast<double4*>(s_raw);
+    double4* s_u = s_p + kTile;
+    double* s_ia = reinterpret_cast<double*>(s_u + kTile);
+    float4* s_p32 = reinterpret_cast<float4*>(s_raw);
+    float4* s_l32 = s_p32 + 2 * kTile;
+    float4* s_u32 = s_l32 + 2 * kTile;
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
@@ -402,7 +412,9 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
     const double4 ui = mu[i];
     const double iai = inva[i];
     const float4 pf = posa32[i];
+    const float4 lf = PBC ? posa32[npad + i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 uf = mu32[i];
+    const float3 uf3 = make_float3(3.f * uf.x, 3.f * uf.y, 3.f * uf.z);
     const float sx = (float)(box.bx * 0x1p-32), sy = (float)(box.by * 0x1p-32), sz = (float)(box.bz * 0x1p-32);
     const float4 cI = tb_c[I], hI = tb_h[I];
     const bool ifull = I * kTile + kTile <= n;
@@ -426,28 +438,32 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
         int kend;
         __syncthreads();
         if (far) {
-            s_p32[lane] = posa32[jb + lane];
-            s_u32[lane] = mu32[jb + lane];
+            {
+                const float4 p = posa32[jb + lane], u = mu32[jb + lane];
+                s_p32[lane] = p;
+                s_p32[lane + kTile] = p;
+                s_u32[lane] = u;
+                s_u32[lane + kTile] = u;
+                if (PBC) {
+                    const float4 l = posa32[npad + jb + lane];
+                    s_l32[lane] = l;
+                    s_l32[lane + kTile] = l;
+                }
+            }
             __syncthreads();
-            float tx = 0.f, ty = 0.f, tz = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+            // i side as two sums per component (sum udj*d and sum rr3*u_j), j side lagging one step
+            float tx = 0.f, ty = 0.f, tz = 0.f, sx3 = 0.f, sy3 = 0.f, sz3 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
 #pragma unroll 4
             for (int k = 0; k < kTile; k++) {
-                const int jj = (lane + k) & (kTile - 1);
+                const int jj = lane + k;
                 const float4 pj = s_p32[jj];
                 const float4 uj = s_u32[jj];
                 float dx, dy, dz;
                 if (PBC) {
-                    bool clear = quant_delta(__float_as_uint(pf.x), __float_as_uint(pj.x), sx, dx);
-                    clear &= quant_delta(__float_as_uint(pf.y), __float_as_uint(pj.y), sy, dy);
-                    clear &= quant_delta(__float_as_uint(pf.z), __float_as_uint(pj.z), sz, dz);
-                    if (!clear) {
-                        const double4 pjd = posa[jb + jj];
-                        double ex = pjd.x - pi.x, ey = pjd.y - pi.y, ez = pjd.z - pi.z;
-                        min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-                        dx = (float)ex;
-                        dy = (float)ey;
-                        dz = (float)ez;
-                    }
+                    const float4 pl = s_l32[jj];
+                    dx = fixed_delta(pf.x, lf.x, pj.x, pl.x) * sx;
+                    dy = fixed_delta(pf.y, lf.y, pj.y, pl.y) * sy;
+                    dz = fixed_delta(pf.z, lf.z, pj.z, pl.z) * sz;
                 } else {
                     dx = pj.x - pf.x;
                     dy = pj.y - pf.y;
@@ -457,26 +473,28 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
                 float rinv = rsqrt_t(r2);
                 float rinv2 = rinv * rinv;
                 float rr3 = rinv * rinv2;
-                float rr5 = 3.f * rr3 * rinv2;
+                float rr5 = rr3 * rinv2;   // 1/r^5: the factor 3 sits in uf3 and in the flush of tx..tz
                 float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
-                float udi = rr5 * (uf.x * dx + uf.y * dy + uf.z * dz);
-                tx += udj * dx - rr3 * uj.x;
-                ty += udj * dy - rr3 * uj.y;
-                tz += udj * dz - rr3 * uj.z;
-                fx += udi * dx - rr3 * uf.x;
-                fy += udi * dy - rr3 * uf.y;
-                fz += udi * dz - rr3 * uf.z;
-                fx = rot1(fx);
-                fy = rot1(fy);
-                fz = rot1(fz);
+                float udi = rr5 * (uf3.x * dx + uf3.y * dy + uf3.z * dz);
+                tx += udj * dx;
+                ty += udj * dy;
+                tz += udj * dz;
+                sx3 += rr3 * uj.x;
+                sy3 += rr3 * uj.y;
+                sz3 += rr3 * uj.z;
+                // j side, lagging one step: before step k, lane l holds slot (l + k - 1); the
+                // rotation and the add form one DPP instruction
+                fx = rot1(fx) + (udi * dx - rr3 * uf.x);
+                fy = rot1(fy) + (udi * dy - rr3 * uf.y);
+                fz = rot1(fz) + (udi * dz - rr3 * uf.z);
             }
-            ax += (double)tx;
-            ay += (double)ty;
-            az += (double)tz;
+            ax += 3.0 * (double)tx - (double)sx3;
+            ay += 3.0 * (double)ty - (double)sy3;
+            az += 3.0 * (double)tz - (double)sz3;
             jx = (double)fx;
             jy = (double)fy;
             jz = (double)fz;
-            kend = kTile;
+            kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
         } else {
             s_p[lane] = posa[jb + lane];
             s_u[lane] = mu[jb + lane];
@@ -1381,12 +1399,13 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
                              double4* Pi, double* Pj) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, posa32,
+        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, L.npad, posa,
+                           posa32,
                            mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
                            L.slab);
     else
-        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa,
-                           posa32, mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
+        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, L.npad,
+                           posa, posa32, mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
                            L.slab);
 }
 

@@ -63,7 +63,7 @@ struct tdp_context {
     double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
             *q = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
-    float4 *posa32 = nullptr, *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr;
+    float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
     int nwork = 0;
     size_t ntri = 0;
@@ -75,7 +75,6 @@ struct tdp_context {
     double *d_pos3 = nullptr, *d_frc = nullptr, *d_alpha = nullptr, *d_inva = nullptr, *d_thole = nullptr,
            *d_charge = nullptr, *d_dmol = nullptr, *d_red = nullptr, *d_scalars = nullptr, *d_part_e = nullptr,
            *d_coef = nullptr, *d_energy = nullptr, *d_mass = nullptr, *d_out = nullptr;
-    float* d_inva32 = nullptr;
     int4* d_axis = nullptr;
     int *d_exc_start = nullptr, *d_exc_J = nullptr, *d_goff = nullptr, *d_gidx = nullptr, *d_perm = nullptr;
     unsigned long long* d_exc_mask = nullptr;
@@ -319,7 +318,6 @@ int dresize(tdp_context* c, T** ptr, size_t& cap, size_t count) {
 int upload_particles(tdp_context* c) {
     const int n = c->n, npad = c->npad;
     std::vector<double> alpha(npad, 0.0), inva(npad, 0.0), thole(npad, 0.0), charge(npad, 0.0), dmol(3 * npad, 0.0);
-    std::vector<float> inva32(npad, 0.f);
     std::vector<int4> axis(npad, make_int4(5, -1, -1, -1));
     std::vector<int> perm(npad, 0x7fffffff);
     auto ren = [c](int a) { return a >= 0 ? c->inv[a] : a; };
@@ -330,13 +328,11 @@ int upload_particles(tdp_context* c) {
         thole[k] = c->thole[i];
         charge[k] = c->charge[i];
         inva[k] = c->thole[i] != 0.0 ? 1.0 / c->thole[i] : 0.0;
-        inva32[k] = (float)inva[k];
         for (int d = 0; d < 3; d++) dmol[3 * k + d] = c->dmol[3 * i + d];
         axis[k] = make_int4(c->axis_type[i], ren(c->atom_z[i]), ren(c->atom_x[i]), ren(c->atom_y[i]));
     }
     TDP_HIP(c, hipMemcpyAsync(c->d_alpha, alpha.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipMemcpyAsync(c->d_inva, inva.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    TDP_HIP(c, hipMemcpyAsync(c->d_inva32, inva32.data(), npad * sizeof(float), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipMemcpyAsync(c->d_thole, thole.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipMemcpyAsync(c->d_charge, charge.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipMemcpyAsync(c->d_dmol, dmol.data(), 3 * npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -494,7 +490,7 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
     double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
     double* pj = reinterpret_cast<double*>(pi + (size_t)c->nwork * 64);
     if (c->precision == TDP_PRECISION_MIXED)
-        launch_sym_matvec_mixed(c->stream, L, c->posa, c->posa32, v64, c->vec32, c->d_inva, scalars, pi, pj);
+        launch_sym_matvec_mixed(c->stream, L, c->posa, v64, c->vec32, c->d_inva, scalars, pi, pj);
     else
         launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pj);
     if (c->timing) {
@@ -540,8 +536,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
     c->evals_since_order++;
     PairLaunch L = pair_launch(c, box);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
-    launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, L.pbc, L.box, c->posa,
-                          mixed ? c->posa32 : nullptr);
+    launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
     launch_tile_bounds(s, n, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h);
     launch_rotate(s, n, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     {
@@ -775,10 +770,10 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
         (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
         (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||
         (rc = dalloc(c, &c->torque, npad)) ||
-        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || (rc = dalloc(c, &c->posa32, 2 * (size_t)npad)) ||
+        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || 
         (rc = dalloc(c, &c->vec32, npad)) || (rc = dalloc(c, &c->d_pos3, 3 * (size_t)npad)) ||
         (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_alpha, npad)) ||
-        (rc = dalloc(c, &c->d_inva, npad)) || (rc = dalloc(c, &c->d_inva32, npad)) ||
+        (rc = dalloc(c, &c->d_inva, npad)) ||
         (rc = dalloc(c, &c->d_thole, npad)) || (rc = dalloc(c, &c->d_charge, npad)) ||
         (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)red_blocks(npad))) ||
         (rc = dalloc(c, &c->d_scalars, SC_COUNT)) ||

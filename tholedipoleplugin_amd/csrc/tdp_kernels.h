@@ -42,18 +42,15 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
                          const double* scalars, double4* Pi, double* Pj);
 void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj, const double* scalars,
                        double4* out4, double* out3);
-void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const float4* posa32,
-                             const double4* mu, const float4* mu32, const double* inva, const double* scalars,
-                             double4* Pi, double* Pj);
+void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
+                             const float4* mu32, const double* inva, const double* scalars, double4* Pi, double* Pj);
 
 void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h);
 
-// posa: FP64 positions (internal order) and Thole parameter.  posa32 (mixed precision only): with
-// PBC the bit patterns of the quantized box fractions (see k_load_positions), else FP32 offsets
-// from the first particle's position; w = Thole parameter.
+// posa: FP64 positions (internal order) and Thole parameter.
 void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
-                           bool pbc, const double* box3, double4* posa, float4* posa32);
+                           double4* posa);
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
                    const int4* axis, double4* qd);
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,

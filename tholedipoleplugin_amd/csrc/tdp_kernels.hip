@@ -98,8 +98,7 @@ __global__ void k_rotate_dipoles(int n, const double4* __restrict__ posa, const 
 // Gathers the caller's positions (original order) into the internal spatial order:
 // slot i holds particle perm[i].
 __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos3, const int* __restrict__ perm,
-                                 const double* __restrict__ thole, int pbc, BoxT64 box, double4* __restrict__ posa,
-                                 float4* __restrict__ posa32) {
+                                 const double* __restrict__ thole, double4* __restrict__ posa) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
     double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -108,26 +107,6 @@ __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos
         p = make_double4(pos3[3 * o], pos3[3 * o + 1], pos3[3 * o + 2], thole[i]);
     }
     posa[i] = p;
-    if (!posa32) return;
-    if (pbc) {
-        // fraction of the box edge in [0, 1) as a 64-bit fixed-point number (units of 2^-64), kept
-        // modulo 2^64: the wrapped difference of two such values is the minimum-image separation.
-        // High words go to posa32[i], low words to posa32[npad + i].
-        auto quant = [](double x, double ib) {
-            double f = x * ib;
-            f -= floor(f);
-            return (unsigned long long)(f * 0x1p64);
-        };
-        const unsigned long long qx = quant(p.x, box.ibx), qy = quant(p.y, box.iby), qz = quant(p.z, box.ibz);
-        posa32[i] = make_float4(__uint_as_float((unsigned)(qx >> 32)), __uint_as_float((unsigned)(qy >> 32)),
-                                __uint_as_float((unsigned)(qz >> 32)), (float)p.w);
-        posa32[npad + i] = make_float4(__uint_as_float((unsigned)qx), __uint_as_float((unsigned)qy),
-                                       __uint_as_float((unsigned)qz), 0.0f);
-    } else {
-        const int o0 = perm[0];
-        posa32[i] = make_float4((float)(p.x - pos3[3 * o0]), (float)(p.y - pos3[3 * o0 + 1]),
-                                (float)(p.z - pos3[3 * o0 + 2]), (float)p.w);
-    }
 }
 
 // ------------------------------------------------------------------ tile classification
@@ -250,17 +229,6 @@ __device__ __forceinline__ void damped_rr_wave(double r2, double ai, double aj, 
     }
 }
 
-// Separation of two periodic coordinates held as 64-bit fixed-point box fractions (high and low
-// words as float bit patterns, see k_load_positions): the wrapped 64-bit difference is the minimum
-// image, decided to 2^-64 of a box, finer than the FP64 rule resolves; its high word, as a float,
-// is the separation in units of 2^-32 box edges.
-__device__ __forceinline__ float fixed_delta(float hi_i, float lo_i, float hi_j, float lo_j) {
-    unsigned borrow, unused;
-    (void)__builtin_subc(__float_as_uint(lo_j), __float_as_uint(lo_i), 0u, &borrow);
-    const unsigned h = __builtin_subc(__float_as_uint(hi_j), __float_as_uint(hi_i), borrow, &unused);
-    return (float)(int)h;
-}
-
 // ------------------------------------------------------------------ T * mu, FP64 (a7)
 template <bool PBC>
 __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* __restrict__ work,
@@ -372,15 +340,14 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
 // ------------------------------------------------------------------ T * mu, mixed precision (a7)
 // Tile pairs inside the Thole range, and the diagonal and exclusion tiles, are evaluated in FP64
 // as in k_sym_matvec.  Tile pairs beyond it, most of the pairs of a large system, carry only the
-// undamped 1/r^3 and 3/r^5 terms and are evaluated in FP32 with FP32 dipoles.  Their separations
-// come from 64-bit fixed-point box fractions (PBC; the wrapped difference is the minimum image, so
-// the image agrees with the FP64 rule even for pairs a hair from half a box) or from FP32 offsets
-// to a common origin (no PBC).  The near pairs carry the largest terms, so keeping them in FP64
-// holds the operator's error to that of the FP32 far field; all partial sums are FP64.
+// undamped 1/r^3 and 3/r^5 terms: their separations and minimum images are formed in FP64 exactly
+// as in k_sym_matvec (so both precisions pick the same image, even for a pair exactly half a box
+// apart), and the tensor terms are evaluated in FP32 with FP32 dipoles.  The near pairs carry the
+// largest terms, so keeping them in FP64 holds the operator's error to that of the FP32 far field;
+// all partial sums are FP64.
 template <bool PBC>
 __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const int4* __restrict__ work,
-                                                         int npad, const double4* __restrict__ posa,
-                                                         const float4* __restrict__ posa32,
+                                                         const double4* __restrict__ posa,
                                                          const double4* __restrict__ mu,
                                                          const float4* __restrict__ mu32,
                                                          const double* __restrict__ inva,
@@ -391,18 +358,17 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
                                                          BoxT64 box, const double* __restrict__ scalars,
                                                          double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     if (scalars && solver_done(scalars)) return;
-    // One LDS area holds either the FP64 tile (near path) or the FP32 tile (far path).  The FP32
-    // tile is staged twice over, so that slot (lane + k) mod 64 is entry lane + k: the step's
-    // address is the lane's base plus a constant offset.
-    constexpr int kBytes64 = kTile * (2 * sizeof(double4) + sizeof(double));
-    constexpr int kBytes32 = 2 * kTile * 3 * sizeof(float4);
-    __shared__ __align__(16) unsigned char s_raw[kBytes64 > kBytes32 ? kBytes64 : kBytes32];
+    // One LDS area holds either the near tile (FP64 dipoles and 1/a) or the far tile (FP32
+    // dipoles).  The far tile is staged twice over, so that slot (lane + k) mod 64 is entry
+    // lane + k: the step's address is the lane's base plus a constant offset.
+    constexpr int kBytesNear = kTile * (2 * sizeof(double4) + sizeof(double));
+    constexpr int kBytesFar = 2 * kTile * (sizeof(double4) + sizeof(float4));
+    __shared__ __align__(16) unsigned char s_raw[kBytesNear > kBytesFar ? kBytesNear : kBytesFar];
     double4* s_p = reinterpret_cast<double4*>(s_raw);
     double4* s_u = s_p + kTile;
     double* s_ia = reinterpret_cast<double*>(s_u + kTile);
-    float4* s_p32 = reinterpret_cast<float4*>(s_raw);
-    float4* s_l32 = s_p32 + 2 * kTile;
-    float4* s_u32 = s_l32 + 2 * kTile;
+    double4* s_pf = reinterpret_cast<double4*>(s_raw);
+    float4* s_uf = reinterpret_cast<float4*>(s_pf + 2 * kTile);
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
@@ -411,11 +377,8 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
     const double4 pi = posa[i];
     const double4 ui = mu[i];
     const double iai = inva[i];
-    const float4 pf = posa32[i];
-    const float4 lf = PBC ? posa32[npad + i] : make_float4(0.f, 0.f, 0.f, 0.f);
     const float4 uf = mu32[i];
     const float3 uf3 = make_float3(3.f * uf.x, 3.f * uf.y, 3.f * uf.z);
-    const float sx = (float)(box.bx * 0x1p-32), sy = (float)(box.by * 0x1p-32), sz = (float)(box.bz * 0x1p-32);
     const float4 cI = tb_c[I], hI = tb_h[I];
     const bool ifull = I * kTile + kTile <= n;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -439,16 +402,12 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
         __syncthreads();
         if (far) {
             {
-                const float4 p = posa32[jb + lane], u = mu32[jb + lane];
-                s_p32[lane] = p;
-                s_p32[lane + kTile] = p;
-                s_u32[lane] = u;
-                s_u32[lane + kTile] = u;
-                if (PBC) {
-                    const float4 l = posa32[npad + jb + lane];
-                    s_l32[lane] = l;
-                    s_l32[lane + kTile] = l;
-                }
+                const double4 p = posa[jb + lane];
+                const float4 u = mu32[jb + lane];
+                s_pf[lane] = p;
+                s_pf[lane + kTile] = p;
+                s_uf[lane] = u;
+                s_uf[lane + kTile] = u;
             }
             __syncthreads();
             // i side as two sums per component (sum udj*d and sum rr3*u_j), j side lagging one step
@@ -456,19 +415,11 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
 #pragma unroll 4
             for (int k = 0; k < kTile; k++) {
                 const int jj = lane + k;
-                const float4 pj = s_p32[jj];
-                const float4 uj = s_u32[jj];
-                float dx, dy, dz;
-                if (PBC) {
-                    const float4 pl = s_l32[jj];
-                    dx = fixed_delta(pf.x, lf.x, pj.x, pl.x) * sx;
-                    dy = fixed_delta(pf.y, lf.y, pj.y, pl.y) * sy;
-                    dz = fixed_delta(pf.z, lf.z, pj.z, pl.z) * sz;
-                } else {
-                    dx = pj.x - pf.x;
-                    dy = pj.y - pf.y;
-                    dz = pj.z - pf.z;
-                }
+                const double4 pj = s_pf[jj];
+                const float4 uj = s_uf[jj];
+                double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+                if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
                 float r2 = dx * dx + dy * dy + dz * dz;
                 float rinv = rsqrt_t(r2);
                 float rinv2 = rinv * rinv;
@@ -1360,10 +1311,8 @@ static BoxT64 mkbox(const double* b3) {
 }
 
 void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
-                           bool pbc, const double* box3, double4* posa, float4* posa32) {
-    BoxT64 b = mkbox(box3);
-    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, perm, thole,
-                       pbc ? 1 : 0, b, posa, posa32);
+                           double4* posa) {
+    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, perm, thole, posa);
 }
 
 void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
@@ -1394,19 +1343,15 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
 }
 
-void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const float4* posa32,
-                             const double4* mu, const float4* mu32, const double* inva, const double* scalars,
-                             double4* Pi, double* Pj) {
+void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
+                             const float4* mu32, const double* inva, const double* scalars, double4* Pi, double* Pj) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, L.npad, posa,
-                           posa32,
-                           mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
-                           L.slab);
+        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
+                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
     else
-        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, L.npad,
-                           posa, posa32, mu, mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj,
-                           L.slab);
+        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
+                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
 }
 
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,

@@ -119,7 +119,8 @@ int tdp_execute(tdp_context* ctx, const double* positions, const double* box, in
 
 /* Same evaluation with device-resident buffers on the caller's HIP stream (a hipStream_t
  * passed as void*, NULL = the null stream): d_positions [3n] doubles, d_forces [3n] doubles
- * accumulated on the device, d_energy one double written on the device (may be NULL). */
+ * accumulated on the device, d_energy one double written on the device (may be NULL; a batch
+ * context writes one energy per system). */
 int tdp_execute_device(tdp_context* ctx, const double* d_positions, const double* box, double* d_forces,
                        double* d_energy, void* stream);
 
@@ -147,6 +148,24 @@ int tdp_get_solver_stats(const tdp_context* ctx, tdp_solver_stats* stats);
 /* Enables (and resets) GPU timing of every T*mu launch with HIP events recorded on the
  * context's stream around the launch; the totals are reported by tdp_get_solver_stats. */
 int tdp_set_timing(tdp_context* ctx, int32_t enable);
+
+/* ---- Batches of independent systems.  No counterpart in the reference, whose kernel holds one
+ * system: these entry points evaluate many NoCutoff systems (clusters) at once, each with the
+ * same semantics as its own tdp_create context (same pair terms, scale maps, solver and stopping
+ * rule per system), in one sequence of launches over all of them.
+ * descs[count]: one description per system, all NoCutoff and of one polarization type; each
+ * system keeps its own solver limits.  Particles are numbered consecutively across the systems
+ * (system g's particle i is particle p0(g) + i, p0 = running sum of num_particles). */
+int tdp_batch_create(const tdp_force_desc* descs, int32_t count, const tdp_options* opts, tdp_context** out);
+int32_t tdp_batch_num_systems(const tdp_context* ctx);
+/* positions: host [3N], N = total particles; forces: host [3N], accumulated (may be NULL);
+ * energies: host [count] (may be NULL). */
+int tdp_batch_execute(tdp_context* ctx, const double* positions, double* forces, double* energies);
+/* device-resident form: d_positions [3N], d_forces [3N] accumulated, d_energies [count]. */
+int tdp_batch_execute_device(tdp_context* ctx, const double* d_positions, double* d_forces, double* d_energies,
+                             void* stream);
+/* solver diagnostics of one system of the last evaluation */
+int tdp_batch_get_solver_stats(const tdp_context* ctx, int32_t system, tdp_solver_stats* stats);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,186 @@
+"""Batched evaluation of independent NoCutoff systems (tdp_batch_* entry points) through the C ABI.
+
+Each system of a batch must get exactly what its own single-system context gives (same pair terms,
+scale maps, solver and stopping rule): energies, forces and induced dipoles are compared with
+single contexts (FP64, to 1e-10 relative: only the summation order of the tile partials differs)
+and with the CPU oracle at the parity bar.  Tolerances of the bar: energy relative 1e-6, force
+relative RMS 1e-5, induced dipoles 1e-6 D.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import ref_systems
+from tholedipoleplugin_amd import _lib, systems
+from tholedipoleplugin_amd.force import TholeDipoleForce
+
+pytestmark = pytest.mark.gpu
+
+T = TholeDipoleForce
+DEBYE = 0.020819434
+MU_TOL = 1e-6 * DEBYE
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def rel_rms(a, b):
+    a = np.asarray(a).reshape(-1)
+    b = np.asarray(b).reshape(-1)
+    return float(np.sqrt(np.mean((a - b) ** 2)) / max(np.sqrt(np.mean(b ** 2)), 1e-300))
+
+
+def batch_members(eps=1e-11, polarization=T.Mutual):
+    """Reference test systems with axis frames and exclusions (ammonia, ZOnly) and water clusters
+    of 7, 20, 45 and 100 molecules (one to five tiles), all NoCutoff."""
+    members = []
+    for rs in (ref_systems.ammonia(polarization), ref_systems.zonly(polarization)):
+        members.append((rs.force, np.asarray(rs.positions, np.float64)))
+    for k, m in enumerate((7, 20, 45, 100)):
+        c = systems.water_cluster(m, seed=500 + k)
+        c.force.setPolarizationType(polarization)
+        members.append((c.force, np.asarray(c.positions, np.float64)))
+    for f, _ in members:
+        f.setMutualInducedTargetEpsilon(eps)
+        f.setMutualInducedMaxIterations(1000)
+    return members
+
+
+@pytest.mark.parametrize("solver", ["pcg", "jacobi"])
+def test_batch_matches_single_contexts(solver):
+    _say("case batch vs single contexts, %s" % solver)
+    members = batch_members()
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver=solver)
+    e_b, f_b = b.execute(pos)
+    mu_b = b.dipoles(pos, None, _lib.DIPOLES_INDUCED)
+    off = 0
+    for g, (f, p) in enumerate(members):
+        n = f.getNumParticles()
+        h = _lib.HipContextHandle(f, np.zeros(9), precision="double", solver=solver)
+        e1, f1 = h.execute(p, None)
+        mu1 = h.dipoles(p, None, _lib.DIPOLES_INDUCED)
+        st1 = h.solver_stats()
+        h.close()
+        stb = b.system_stats(g)
+        assert stb["converged"] == 1 and st1["converged"] == 1
+        assert abs(stb["iterations"] - st1["iterations"]) <= 1
+        assert abs(e_b[g] - e1) <= 1e-10 * abs(e1)
+        assert rel_rms(f_b[off:off + n], f1) <= 1e-10
+        assert np.max(np.abs(mu_b[off:off + n] - mu1)) <= 1e-12
+        off += n
+    b.close()
+
+
+def test_batch_matches_oracle():
+    _say("case batch vs oracle")
+    members = batch_members()
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    e_b, f_b = b.execute(pos)
+    mu_b = b.dipoles(pos, None, _lib.DIPOLES_INDUCED)
+    b.close()
+    off = 0
+    for g, (f, p) in enumerate(members):
+        n = f.getNumParticles()
+        ref = oracle.force_energy(f, p, None, target_eps=1e-13, max_iter=5000)
+        assert abs(e_b[g] - ref["energy"]) <= 1e-6 * abs(ref["energy"])
+        assert rel_rms(f_b[off:off + n], ref["forces"]) <= 1e-5
+        assert np.max(np.abs(mu_b[off:off + n] - ref["induced"])) <= MU_TOL
+        off += n
+
+
+def test_batch_reference_stopping_rule():
+    """With each system's own target and the Jacobi solver, every system stops after the iteration
+    count and at the epsilon of convergeInducedDipolesByDIIS (ReferenceTholeDipoleForce.cpp:829-850)."""
+    _say("case batch stopping rule")
+    members = batch_members()
+    targets = [1e-5, 1e-6, 1e-7, 3e-6, 1e-8, 2e-7]
+    for (f, _), t in zip(members, targets):
+        f.setMutualInducedTargetEpsilon(t)
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="jacobi")
+    b.execute(pos)
+    for g, (f, p) in enumerate(members):
+        ref = oracle.force_energy(f, p, None)
+        st = b.system_stats(g)
+        assert st["iterations"] == ref["iterations"], g
+        assert abs(st["epsilon"] - ref["epsilon"]) <= 1e-6 * ref["epsilon"], g
+    b.close()
+
+
+@pytest.mark.parametrize("pol", [T.Direct, T.Extrapolated])
+def test_batch_other_polarization_types(pol):
+    _say("case batch polarization %d" % pol)
+    members = batch_members(polarization=pol)
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    e_b, f_b = b.execute(pos)
+    b.close()
+    off = 0
+    for g, (f, p) in enumerate(members):
+        n = f.getNumParticles()
+        ref = oracle.force_energy(f, p, None)
+        assert abs(e_b[g] - ref["energy"]) <= 1e-6 * abs(ref["energy"])
+        assert rel_rms(f_b[off:off + n], ref["forces"]) <= 1e-5
+        off += n
+
+
+def test_batch_mixed_precision():
+    _say("case batch mixed")
+    members = batch_members(eps=1e-10)
+    pos = np.concatenate([p for _, p in members])
+    bd = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    bm = _lib.HipBatchHandle([f for f, _ in members], precision="mixed", solver="pcg")
+    e_d, f_d = bd.execute(pos)
+    e_m, f_m = bm.execute(pos)
+    mu_d = bd.dipoles(pos, None, _lib.DIPOLES_INDUCED)
+    mu_m = bm.dipoles(pos, None, _lib.DIPOLES_INDUCED)
+    bd.close()
+    bm.close()
+    assert np.max(np.abs(mu_m - mu_d)) <= MU_TOL
+    assert rel_rms(f_m, f_d) <= 1e-5
+    assert np.all(np.abs(e_m - e_d) <= 1e-6 * np.abs(e_d))
+
+
+def test_batch_of_config3_clusters_size_range():
+    """Twelve clusters spanning the 20-400 molecule range of the cluster workload, solved to
+    1e-7 D, against single FP64 contexts."""
+    _say("case batch config3 sizes")
+    sizes = [20, 400, 33, 250, 57, 140, 21, 380, 75, 99, 160, 310]
+    members = []
+    for k, m in enumerate(sizes):
+        c = systems.water_cluster(m, seed=900 + k)
+        c.force.setMutualInducedTargetEpsilon(1e-7 * DEBYE)
+        c.force.setMutualInducedMaxIterations(1000)
+        members.append((c.force, np.asarray(c.positions, np.float64)))
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="mixed", solver="pcg")
+    e_b, f_b = b.execute(pos)
+    mu_b = b.dipoles(pos, None, _lib.DIPOLES_INDUCED)
+    b.close()
+    off = 0
+    for g, (f, p) in enumerate(members):
+        n = f.getNumParticles()
+        f.setMutualInducedTargetEpsilon(1e-11)
+        h = _lib.HipContextHandle(f, np.zeros(9), precision="double", solver="pcg")
+        e1, f1 = h.execute(p, None)
+        mu1 = h.dipoles(p, None, _lib.DIPOLES_INDUCED)
+        h.close()
+        assert np.max(np.abs(mu_b[off:off + n] - mu1)) <= MU_TOL, g
+        assert rel_rms(f_b[off:off + n], f1) <= 1e-5, g
+        assert abs(e_b[g] - e1) <= 1e-6 * abs(e1), g
+        off += n
+
+
+def test_batch_rejects_bad_input():
+    members = batch_members()
+    rs = ref_systems.water4()
+    with pytest.raises(_lib.TdpError, match="NoCutoff"):
+        _lib.HipBatchHandle([members[0][0], rs.force])
+    f = members[2][0]
+    f2 = systems.water_cluster(5, seed=1).force
+    f2.setPolarizationType(T.Direct)
+    with pytest.raises(_lib.TdpError, match="polarization"):
+        _lib.HipBatchHandle([f, f2])

@@ -106,7 +106,7 @@ def test_serialization_round_trip():
 def test_library_exports_every_declared_symbol():
     """libtholedipole_hip.so loads (no device needed) and exports every function the header declares."""
     header = open(os.path.join(ROOT, "include", "tholedipole_hip.h")).read()
-    declared = set(re.findall(r"^(?:int|void|const char\*)\s+(tdp_\w+)\s*\(", header, re.M))
+    declared = set(re.findall(r"^(?:int|int32_t|void|const char\*)\s+(tdp_\w+)\s*\(", header, re.M))
     assert declared == set(_lib.EXPORTED_SYMBOLS)
     L = ctypes.CDLL(_lib.LIB_PATH)
     for name in declared:

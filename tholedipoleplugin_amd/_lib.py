@@ -28,6 +28,8 @@ EXPORTED_SYMBOLS = [
     "tdp_create", "tdp_destroy", "tdp_last_error", "tdp_last_create_error", "tdp_copy_parameters",
     "tdp_execute", "tdp_execute_device", "tdp_get_dipoles", "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
+    "tdp_batch_create", "tdp_batch_num_systems", "tdp_batch_execute", "tdp_batch_execute_device",
+    "tdp_batch_get_solver_stats",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -88,6 +90,11 @@ def load():
     L.tdp_get_pme_parameters.argtypes = [ctx_p, _dp, _ip, _ip, _ip]
     L.tdp_get_solver_stats.argtypes = [ctx_p, C.POINTER(SolverStats)]
     L.tdp_set_timing.argtypes = [ctx_p, C.c_int32]
+    L.tdp_batch_create.argtypes = [C.POINTER(ForceDesc), C.c_int32, C.POINTER(Options), C.POINTER(ctx_p)]
+    L.tdp_batch_num_systems.argtypes = [ctx_p]
+    L.tdp_batch_execute.argtypes = [ctx_p, _dp, _dp, _dp]
+    L.tdp_batch_execute_device.argtypes = [ctx_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tdp_batch_get_solver_stats.argtypes = [ctx_p, C.c_int32, C.POINTER(SolverStats)]
     for name in EXPORTED_SYMBOLS:
         if name not in ("tdp_destroy", "tdp_last_error", "tdp_last_create_error"):
             getattr(L, name).restype = C.c_int
@@ -224,3 +231,39 @@ class HipContextHandle:
 
     def set_timing(self, enable=True):
         self._check(self.L.tdp_set_timing(self.h, 1 if enable else 0))
+
+
+class HipBatchHandle(HipContextHandle):
+    """Owns one batch context over several NoCutoff systems (tdp_batch_create); particles are
+    numbered consecutively across the systems."""
+
+    def __init__(self, forces, precision="double", solver="pcg", device=0):
+        self.L = load()
+        self.packed = [PackedForce(f, np.zeros(9)) for f in forces]
+        self.sizes = [f.getNumParticles() for f in forces]
+        self.n = int(sum(self.sizes))
+        descs = (ForceDesc * len(forces))(*[pf.desc for pf in self.packed])
+        opts = Options(PRECISION[precision], SOLVER[solver], int(device))
+        h = C.c_void_p()
+        rc = self.L.tdp_batch_create(descs, len(forces), C.byref(opts), C.byref(h))
+        if rc != TDP_OK:
+            raise TdpError(rc, self.L.tdp_last_create_error().decode())
+        self.h = h
+        self.precision = precision
+        self.solver = solver
+
+    def execute(self, positions, box=None, forces=None):
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        f = np.zeros(pos.size) if forces is None else forces
+        e = np.zeros(len(self.sizes))
+        self._check(self.L.tdp_batch_execute(self.h, _ptr(pos), _ptr(f), _ptr(e)))
+        return e, f.reshape(-1, 3)
+
+    def execute_device(self, d_positions_ptr, d_forces_ptr, d_energies_ptr, stream_ptr):
+        self._check(self.L.tdp_batch_execute_device(self.h, C.c_void_p(d_positions_ptr), C.c_void_p(d_forces_ptr),
+                                                    C.c_void_p(d_energies_ptr), C.c_void_p(stream_ptr)))
+
+    def system_stats(self, system):
+        st = SolverStats()
+        self._check(self.L.tdp_batch_get_solver_stats(self.h, int(system), C.byref(st)))
+        return dict(iterations=st.iterations, epsilon=st.epsilon, converged=st.converged)

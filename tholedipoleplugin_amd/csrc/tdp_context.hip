@@ -37,13 +37,20 @@ constexpr int kReorderInterval = 100;
 
 struct tdp_context {
     // ---- parameters (host copies)
+    // n particles in the caller's order; nblk tiles of 64 slots (npad = 64 nblk) in the internal
+    // order.  A context holds nseg independent systems: system s owns the caller's particles
+    // [seg_p0[s], seg_p0[s] + seg_n[s]) and the tiles [seg_t0[s], seg_t1[s]); its last tile is
+    // padded.  A context made by tdp_create holds one system.
     int n = 0, npad = 0, nblk = 0;
+    int nseg = 1;
+    std::vector<int> seg_n, seg_p0, seg_t0, seg_t1, seg_w0, seg_w1, seg_maxit;
+    std::vector<double> seg_target;
+    std::vector<int> tcnt, tfirst, tseg;
+    std::vector<size_t> tcol;
     int method = TDP_NO_CUTOFF, polarization = TDP_MUTUAL, precision = TDP_PRECISION_DOUBLE;
     int solver = TDP_SOLVER_JACOBI, device = 0;
     double cutoff = 1.0, ewald_tol = 1e-4, pme_alpha = 0.0;
     int grid[3] = {0, 0, 0};
-    int max_iter = 60;
-    double target_eps = 1e-5;
     std::vector<double> extrap;
     double default_box[9] = {0};
     std::vector<double> charge, dmol, polarity, thole;
@@ -80,10 +87,13 @@ struct tdp_context {
     unsigned long long* d_exc_mask = nullptr;
     int hist_order = 0;
     size_t out_bytes = 0;
-    // ---- results of the last solve
-    int iterations = 0;
-    double epsilon = 1.0e50;
-    int converged = 0;
+    int *d_tcnt = nullptr, *d_tfirst = nullptr, *d_tseg = nullptr, *d_done = nullptr;
+    size_t* d_tcol = nullptr;
+    int4* d_seg = nullptr;
+    int2* d_segw = nullptr;
+    // ---- results of the last solve (per system)
+    std::vector<int> iterations, converged;
+    std::vector<double> epsilon;
     // ---- timing
     bool timing = false;
     double matvec_ms = 0.0;
@@ -246,7 +256,7 @@ void build_tiles(tdp_context* c) {
         if (m == 0.5) masks[((size_t)t * 3 + 1) * 64 + row] |= bit;
         if (is == 0.0) masks[((size_t)t * 3 + 2) * 64 + row] |= bit;
     };
-    for (int k = 0; k < n; k++) setbits(k, k, 0.0, 0.0);
+    for (int i = 0; i < n; i++) setbits(c->inv[i], c->inv[i], 0.0, 0.0);
     for (const ExcPair& e : c->exc_pairs) {
         int a = c->inv[e.i], b = c->inv[e.j];
         setbits(a, b, e.m, e.is);
@@ -289,13 +299,14 @@ void build_torque_gather(tdp_context* c) {
         if (a.ii != b.ii) return a.ii < b.ii;
         return a.role < b.role;
     });
-    c->goff.assign(n + 1, 0);
+    const int npad = c->npad;
+    c->goff.assign(npad + 1, 0);
     c->gidx.resize(es.size());
     for (size_t k = 0; k < es.size(); k++) {
         c->goff[es[k].atom + 1]++;
         c->gidx[k] = es[k].slot;
     }
-    for (int a = 0; a < n; a++) c->goff[a + 1] += c->goff[a];
+    for (int a = 0; a < npad; a++) c->goff[a + 1] += c->goff[a];
 }
 
 // Replaces a device buffer with one of at least count elements.
@@ -319,10 +330,12 @@ int upload_particles(tdp_context* c) {
     const int n = c->n, npad = c->npad;
     std::vector<double> alpha(npad, 0.0), inva(npad, 0.0), thole(npad, 0.0), charge(npad, 0.0), dmol(3 * npad, 0.0);
     std::vector<int4> axis(npad, make_int4(5, -1, -1, -1));
-    std::vector<int> perm(npad, 0x7fffffff);
+    std::vector<int> perm(npad, -1);
     auto ren = [c](int a) { return a >= 0 ? c->inv[a] : a; };
-    for (int k = 0; k < n; k++) {
+    (void)n;
+    for (int k = 0; k < npad; k++) {
         int i = c->perm[k];
+        if (i < 0) continue;   // padding slot: no charge, dipole or polarizability
         perm[k] = i;
         alpha[k] = c->polarity[i];
         thole[k] = c->thole[i];
@@ -385,40 +398,45 @@ void compute_order(tdp_context* c, const double* pos3, const double* box) {
             }
             w[3 * (size_t)i + d] = std::isfinite(x) ? x : 0.0;
         }
-    std::vector<int> ids(n);
-    for (int i = 0; i < n; i++) ids[i] = i;
+    std::fill(c->perm.begin(), c->perm.end(), -1);
     struct Range { int b, e; };
-    std::vector<Range> stack{{0, n}};
-    while (!stack.empty()) {
-        Range r = stack.back();
-        stack.pop_back();
-        const int m = r.e - r.b;
-        if (m <= 64) {
-            std::sort(ids.begin() + r.b, ids.begin() + r.e);
-            continue;
-        }
-        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-        for (int k = r.b; k < r.e; k++)
-            for (int d = 0; d < 3; d++) {
-                double x = w[3 * (size_t)ids[k] + d];
-                lo[d] = std::min(lo[d], x);
-                hi[d] = std::max(hi[d], x);
+    for (int sg = 0; sg < c->nseg; sg++) {
+        const int p0 = c->seg_p0[sg], ns = c->seg_n[sg];
+        std::vector<int> ids(ns);
+        for (int k = 0; k < ns; k++) ids[k] = p0 + k;
+        std::vector<Range> stack{{0, ns}};
+        while (!stack.empty()) {
+            Range r = stack.back();
+            stack.pop_back();
+            const int m = r.e - r.b;
+            if (m <= 64) {
+                std::sort(ids.begin() + r.b, ids.begin() + r.e);
+                continue;
             }
-        int ax = 0;
-        for (int d = 1; d < 3; d++)
-            if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
-        const int tiles = (m + 63) / 64;
-        const int left = (tiles / 2) * 64;
-        std::nth_element(ids.begin() + r.b, ids.begin() + r.b + left, ids.begin() + r.e, [&](int a, int b2) {
-            double xa = w[3 * (size_t)a + ax], xb = w[3 * (size_t)b2 + ax];
-            return xa < xb || (xa == xb && a < b2);
-        });
-        stack.push_back({r.b + left, r.e});
-        stack.push_back({r.b, r.b + left});
-    }
-    for (int k = 0; k < n; k++) {
-        c->perm[k] = ids[k];
-        c->inv[ids[k]] = k;
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int k = r.b; k < r.e; k++)
+                for (int d = 0; d < 3; d++) {
+                    double x = w[3 * (size_t)ids[k] + d];
+                    lo[d] = std::min(lo[d], x);
+                    hi[d] = std::max(hi[d], x);
+                }
+            int ax = 0;
+            for (int d = 1; d < 3; d++)
+                if (hi[d] - lo[d] > hi[ax] - lo[ax]) ax = d;
+            const int tiles = (m + 63) / 64;
+            const int left = (tiles / 2) * 64;
+            std::nth_element(ids.begin() + r.b, ids.begin() + r.b + left, ids.begin() + r.e, [&](int a, int b2) {
+                double xa = w[3 * (size_t)a + ax], xb = w[3 * (size_t)b2 + ax];
+                return xa < xb || (xa == xb && a < b2);
+            });
+            stack.push_back({r.b + left, r.e});
+            stack.push_back({r.b, r.b + left});
+        }
+        const int slot0 = 64 * c->seg_t0[sg];
+        for (int k = 0; k < ns; k++) {
+            c->perm[slot0 + k] = ids[k];
+            c->inv[ids[k]] = slot0 + k;
+        }
     }
 }
 
@@ -435,8 +453,11 @@ int reorder(tdp_context* c, const double* host_pos3, const double* box) {
 
 PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     PairLaunch L;
-    L.n = c->n;
     L.npad = c->npad;
+    L.tm.cnt = c->d_tcnt;
+    L.tm.first = c->d_tfirst;
+    L.tm.seg = c->d_tseg;
+    L.tm.col = c->d_tcol;
     L.nblk = c->nblk;
     L.pbc = c->method == TDP_PME;
     L.box[0] = box9 ? box9[0] : 0.0;
@@ -452,6 +473,18 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.row_start = c->d_row_start;
     L.slab = c->ntri * 64;
     return L;
+}
+
+SegLaunch seg_launch(const tdp_context* c) {
+    SegLaunch G;
+    G.nseg = c->nseg;
+    G.ntile = c->nblk;
+    G.npad = c->npad;
+    G.tseg = c->d_tseg;
+    G.seg = c->d_seg;
+    G.segw = c->d_segw;
+    G.done = c->d_done;
+    return G;
 }
 
 hipEvent_t take_event(tdp_context* c) {
@@ -509,14 +542,57 @@ int check_box(tdp_context* c, const double* box) {
     return TDP_OK;
 }
 
+// Per-system solver results from the device scalars.
+int read_solver_results(tdp_context* c, bool jacobi) {
+    std::vector<double> sc((size_t)SC_COUNT * c->nseg);
+    TDP_HIP(c, hipMemcpyAsync(sc.data(), c->d_scalars, sc.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    for (int g = 0; g < c->nseg; g++) {
+        const double* x = &sc[(size_t)SC_COUNT * g];
+        c->iterations[g] = (int)x[SC_ITER];
+        c->epsilon[g] = x[SC_EPS];
+        c->converged[g] = c->epsilon[g] < c->seg_target[g] ? 1 : 0;
+        if (jacobi && c->iterations[g] == 0) c->epsilon[g] = 1.0;
+    }
+    return TDP_OK;
+}
+
+// Starts a Mutual solve: per-system iteration limit and target, epsilon 1, and for the Jacobi
+// iteration the reference's loop condition (:829-832) as the initial stop flag.
+int start_solve(tdp_context* c, bool jacobi) {
+    std::vector<double> sc((size_t)SC_COUNT * c->nseg, 0.0);
+    int done = 0;
+    for (int g = 0; g < c->nseg; g++) {
+        double* x = &sc[(size_t)SC_COUNT * g];
+        x[SC_MAXIT] = c->seg_maxit[g];
+        x[SC_TARGET] = c->seg_target[g];
+        x[SC_EPS] = 1.0;
+        if (jacobi && (c->seg_maxit[g] <= 0 || !(1.0 > c->seg_target[g]))) {
+            x[SC_DONE] = 1.0;
+            done++;
+        }
+    }
+    if (!jacobi) done = 0;   // the CG start kernel sets every system's flag afresh
+    TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int zero[2] = {done, 0};
+    TDP_HIP(c, hipMemcpyAsync(c->d_done, zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    return TDP_OK;
+}
+
+int read_done(tdp_context* c, int& done) {
+    TDP_HIP(c, hipMemcpyAsync(&done, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    return TDP_OK;
+}
+
 // setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764) from
 // positions already in d_pos3 (caller's order).  host_pos3 (may be NULL) are the same positions
 // on the host, used when the internal order is recomputed.
 int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) {
     if (c->n == 0) {
-        c->iterations = 0;
-        c->converged = 1;
-        c->epsilon = c->polarization == TDP_DIRECT ? 1.0e50 : 0.0;
+        std::fill(c->iterations.begin(), c->iterations.end(), 0);
+        std::fill(c->converged.begin(), c->converged.end(), 1);
+        std::fill(c->epsilon.begin(), c->epsilon.end(), c->polarization == TDP_DIRECT ? 1.0e50 : 0.0);
         return TDP_OK;
     }
     hipStream_t s = c->stream;
@@ -535,22 +611,22 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
     }
     c->evals_since_order++;
     PairLaunch L = pair_launch(c, box);
+    SegLaunch G = seg_launch(c);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
-    launch_load_positions(s, n, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
-    launch_tile_bounds(s, n, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h);
-    launch_rotate(s, n, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
+    launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h);
+    launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     {
         double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
         launch_sym_field(s, L, c->posa, c->qd, c->d_inva, pi, reinterpret_cast<double*>(pi + (size_t)c->nwork * 64),
                          c->Emv);
     }
-    launch_field_init(s, n, npad, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
-    const int nred = red_blocks(npad);
+    launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
 
     if (c->polarization == TDP_DIRECT) {
-        c->iterations = 0;
-        c->converged = 1;
-        c->epsilon = 1.0e50;  // the reference leaves its initial value (:24, :749-753)
+        std::fill(c->iterations.begin(), c->iterations.end(), 0);
+        std::fill(c->converged.begin(), c->converged.end(), 1);
+        std::fill(c->epsilon.begin(), c->epsilon.end(), 1.0e50);  // the reference leaves its initial value (:24, :749-753)
         return TDP_OK;
     }
     if (c->polarization == TDP_EXTRAPOLATED) {
@@ -564,68 +640,52 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
         for (int o = 1; o < order; o++) {
             matvec(c, L, c->mu, nullptr);
-            launch_extrap_step(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
+            launch_extrap_step(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
                                c->hist + (size_t)o * npad);
         }
-        launch_extrap_combine(s, n, npad, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
+        launch_extrap_combine(s, G, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
         matvec(c, L, c->mu, nullptr);
-        launch_extrap_error(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
-        double sc[SC_COUNT];
-        TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
-        TDP_HIP(c, hipStreamSynchronize(s));
-        c->epsilon = sc[SC_EPS];
-        c->converged = 1;
-        c->iterations = order;
+        launch_extrap_error(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
+        int rc = read_solver_results(c, false);
+        if (rc) return rc;
+        std::fill(c->converged.begin(), c->converged.end(), 1);
+        std::fill(c->iterations.begin(), c->iterations.end(), order);
         if (c->timing) harvest_timing(c);
         return TDP_OK;
     }
-    // Mutual
-    double sc0[SC_COUNT] = {0};
-    sc0[SC_MAXIT] = c->max_iter;
-    sc0[SC_TARGET] = c->target_eps;
-    sc0[SC_EPS] = 1.0;
+    // Mutual: every system iterates until its own stop flag is set; a launch skips the tiles and
+    // work items of stopped systems.  The host checks the count of stopped systems every chunk.
+    const bool jacobi = c->solver == TDP_SOLVER_JACOBI;
     const int chunk = n > 20000 ? 4 : 16;  // iterations enqueued between host checks
-    if (c->solver == TDP_SOLVER_JACOBI) {
-        // Loop condition of convergeInducedDipolesByDIIS (:829-832) with epsilon starting at 1.0.
-        sc0[SC_DONE] = (c->max_iter <= 0 || !(1.0 > c->target_eps)) ? 1.0 : 0.0;
-        TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc0, sizeof sc0, hipMemcpyHostToDevice, s));
-        double sc[SC_COUNT];
-        for (;;) {
+    int rc = start_solve(c, jacobi);
+    if (rc) return rc;
+    int done = 0;
+    if (jacobi) {
+        if ((rc = read_done(c, done))) return rc;
+        while (done < c->nseg) {
             for (int k = 0; k < chunk; k++) {
                 matvec(c, L, c->mu, c->d_scalars);
-                launch_jacobi_update(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu,
-                                     mixed ? c->vec32 : nullptr, c->d_scalars, c->d_red);
+                launch_jacobi_update(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr, c->d_scalars,
+                                     c->d_red);
             }
-            TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
-            TDP_HIP(c, hipStreamSynchronize(s));
-            if (sc[SC_DONE] != 0.0) break;
+            if ((rc = read_done(c, done))) return rc;
         }
-        c->iterations = (int)sc[SC_ITER];
-        c->epsilon = sc[SC_EPS];
-        c->converged = c->epsilon < c->target_eps ? 1 : 0;
-        if (c->iterations == 0) c->epsilon = 1.0;
     } else {
-        TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc0, sizeof sc0, hipMemcpyHostToDevice, s));
         // x = mu; r = E0 - x/alpha + T x; z = alpha r; p = z
         matvec(c, L, c->mu, nullptr);
-        launch_pcg_init(s, n, npad, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p,
-                        mixed ? c->vec32 : nullptr, c->d_red, c->d_scalars);
-        double sc[SC_COUNT];
-        TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
-        TDP_HIP(c, hipStreamSynchronize(s));
-        while (sc[SC_DONE] == 0.0) {
+        launch_pcg_init(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p, mixed ? c->vec32 : nullptr, c->d_red,
+                        c->d_scalars);
+        if ((rc = read_done(c, done))) return rc;
+        while (done < c->nseg) {
             for (int k = 0; k < chunk; k++) {
                 matvec(c, L, c->p, c->d_scalars);
-                launch_pcg_iteration_tail(s, n, npad, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p,
-                                          mixed ? c->vec32 : nullptr, c->q, c->d_red, c->d_scalars);
+                launch_pcg_iteration_tail(s, G, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr,
+                                          c->q, c->d_red, c->d_scalars);
             }
-            TDP_HIP(c, hipMemcpyAsync(sc, c->d_scalars, sizeof sc, hipMemcpyDeviceToHost, s));
-            TDP_HIP(c, hipStreamSynchronize(s));
+            if ((rc = read_done(c, done))) return rc;
         }
-        c->iterations = (int)sc[SC_ITER];
-        c->epsilon = sc[SC_EPS];
-        c->converged = c->epsilon < c->target_eps ? 1 : 0;
     }
+    if ((rc = read_solver_results(c, jacobi))) return rc;
     if (c->timing) harvest_timing(c);
     return TDP_OK;
 }
@@ -638,18 +698,19 @@ void forces_and_energy(tdp_context* c, const double* box) {
         hipMemsetAsync(c->d_energy, 0, sizeof(double), s);
         return;
     }
-    const int n = c->n, npad = c->npad;
+    const int npad = c->npad;
     PairLaunch L = pair_launch(c, box);
+    SegLaunch G = seg_launch(c);
     double4* pf = reinterpret_cast<double4*>(c->d_pbuf);
     double4* pt = c->d_pbuf2;
     const size_t off = (size_t)c->nwork * 64;
     launch_sym_elec(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->d_perm, pf,
                     reinterpret_cast<double*>(pf + off), pt, reinterpret_cast<double*>(pt + off), c->d_part_e, c->d_frc,
                     c->torque);
-    launch_reduce_energy(s, c->nwork, c->d_part_e, c->d_energy);
+    launch_reduce_energy(s, G, c->d_part_e, c->d_energy);
     if (!c->gidx.empty()) {
-        launch_map_torque(s, n, c->posa, c->d_axis, c->torque, c->contrib);
-        launch_gather_torque(s, n, c->d_goff, c->d_gidx, c->contrib, c->d_frc);
+        launch_map_torque(s, npad, c->posa, c->d_axis, c->torque, c->contrib);
+        launch_gather_torque(s, npad, c->d_goff, c->d_gidx, c->contrib, c->d_frc);
     }
 }
 
@@ -667,9 +728,203 @@ int ensure_out(tdp_context* c, size_t bytes) {
     return TDP_OK;
 }
 
+// Forces (accumulated into the caller's order) and the energy of every system.
+int download_results(tdp_context* c, double* forces, double* energies) {
+    const int npad = c->npad;
+    std::vector<double> f;
+    std::vector<double> e(c->nseg, 0.0);
+    if (forces && c->n > 0) {
+        f.resize(3 * (size_t)npad);
+        TDP_HIP(c, hipMemcpyAsync(f.data(), c->d_frc, 3 * (size_t)npad * sizeof(double), hipMemcpyDeviceToHost,
+                                  c->stream));
+    }
+    TDP_HIP(c, hipMemcpyAsync(e.data(), c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    if (!f.empty())
+        for (int k = 0; k < npad; k++) {
+            const int o = c->perm[k];
+            if (o < 0) continue;
+            for (int d = 0; d < 3; d++) forces[3 * o + d] += f[3 * k + d];
+        }
+    if (energies)
+        for (int g = 0; g < c->nseg; g++) energies[g] = e[g];
+    return TDP_OK;
+}
+
 int upload_positions(tdp_context* c, const double* positions) {
     if (c->n > 0)
         TDP_HIP(c, hipMemcpyAsync(c->d_pos3, positions, 3 * c->n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    return TDP_OK;
+}
+
+// Tiles, work items and partial-slab layout of the systems (sizes in the caller's order).
+void build_layout(tdp_context* c, const std::vector<int>& sizes) {
+    c->nseg = (int)sizes.size();
+    c->seg_n = sizes;
+    c->seg_p0.assign(c->nseg, 0);
+    c->seg_t0.assign(c->nseg, 0);
+    c->seg_t1.assign(c->nseg, 0);
+    c->seg_w0.assign(c->nseg, 0);
+    c->seg_w1.assign(c->nseg, 0);
+    int p0 = 0, t0 = 0;
+    for (int g = 0; g < c->nseg; g++) {
+        c->seg_p0[g] = p0;
+        c->seg_t0[g] = t0;
+        t0 += (sizes[g] + 63) / 64;
+        c->seg_t1[g] = t0;
+        p0 += sizes[g];
+    }
+    c->n = p0;
+    c->nblk = t0;
+    c->npad = std::max(64, 64 * c->nblk);
+    c->tcnt.assign(c->nblk, 0);
+    c->tfirst.assign(c->nblk, 0);
+    c->tseg.assign(c->nblk, 0);
+    c->tcol.assign(c->nblk, 0);
+    size_t base = 0;
+    for (int g = 0; g < c->nseg; g++) {
+        const int a = c->seg_t0[g], b = c->seg_t1[g];
+        for (int T = a; T < b; T++) {
+            c->tcnt[T] = std::min(64, sizes[g] - 64 * (T - a));
+            c->tfirst[T] = a;
+            c->tseg[T] = g;
+            c->tcol[T] = base + (size_t)(T - a) * (T - a + 1) / 2;
+        }
+        base += (size_t)(b - a) * (b - a + 1) / 2;
+    }
+    c->ntri = base;
+    // identity order within each system until the first evaluation computes the spatial one
+    c->perm.assign(c->npad, -1);
+    c->inv.assign(c->n, 0);
+    for (int g = 0; g < c->nseg; g++)
+        for (int k = 0; k < sizes[g]; k++) {
+            c->perm[64 * c->seg_t0[g] + k] = c->seg_p0[g] + k;
+            c->inv[c->seg_p0[g] + k] = 64 * c->seg_t0[g] + k;
+        }
+}
+
+int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const std::vector<int>& maxit,
+                   const std::vector<double>& target, const tdp_options* opts, tdp_context** out) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return fail(nullptr, TDP_ERR_NO_DEVICE, "no HIP device is available");
+    tdp_context* c = new tdp_context();
+    c->device = opts ? opts->device : 0;
+    c->precision = opts ? opts->precision : TDP_PRECISION_DOUBLE;
+    c->solver = opts ? opts->solver : TDP_SOLVER_JACOBI;
+    if (c->device < 0 || c->device >= ndev) {
+        delete c;
+        return fail(nullptr, TDP_ERR_NO_DEVICE, "invalid device ordinal");
+    }
+    int rc = 0;
+    auto bail = [&](int code) {
+        g_create_error = c->err;
+        tdp_destroy(c);
+        return code;
+    };
+    if (hipSetDevice(c->device) != hipSuccess) {
+        c->err = "hipSetDevice failed";
+        return bail(TDP_ERR_HIP);
+    }
+    build_layout(c, sizes);
+    c->method = d->nonbonded_method;
+    c->polarization = d->polarization_type;
+    c->cutoff = d->cutoff;
+    c->ewald_tol = d->ewald_error_tolerance;
+    c->pme_alpha = d->pme_alpha;
+    for (int k = 0; k < 3; k++) c->grid[k] = d->pme_grid[k];
+    std::memcpy(c->default_box, d->default_box, sizeof c->default_box);
+    // Solver settings are read only for the active polarization type (ReferenceTholeDipoleKernels.cpp:127-134).
+    c->seg_maxit.assign(c->nseg, 60);
+    c->seg_target.assign(c->nseg, 1e-5);
+    if (c->polarization == TDP_MUTUAL) {
+        c->seg_maxit = maxit;
+        c->seg_target = target;
+    } else if (c->polarization == TDP_EXTRAPOLATED) {
+        if (d->num_extrapolation_coefficients > 0)
+            c->extrap.assign(d->extrapolation_coefficients,
+                             d->extrapolation_coefficients + d->num_extrapolation_coefficients);
+    }
+    c->iterations.assign(c->nseg, 0);
+    c->converged.assign(c->nseg, 0);
+    c->epsilon.assign(c->nseg, 1.0e50);
+    copy_particles(c, d);
+    build_exclusion_pairs(c, d);
+    if (hipStreamCreate(&c->stream) != hipSuccess) {
+        c->err = "hipStreamCreate failed";
+        return bail(TDP_ERR_HIP);
+    }
+    const int npad = c->npad, nb = c->nblk, ns = c->nseg;
+    // Work items: tile row I of a system split into chunks of C tiles (J >= I, same system).
+    // Short items (C = 3 on large systems) keep the many one-wave workgroups evenly spread over the
+    // CUs to the end of the launch; measured at 36,864 sites, C = 3 beats C = 20 by 25% on the
+    // T*mu pass, and C = 1 loses more in the reduction than it gains.
+    {
+        const int C = (int)std::max<size_t>(1, std::min<size_t>(3, c->ntri / 16384));
+        std::vector<int4> work;
+        std::vector<int> row_start(nb + 1, 0);
+        for (int g = 0; g < ns; g++) {
+            c->seg_w0[g] = (int)work.size();
+            const int t1 = c->seg_t1[g];
+            for (int I = c->seg_t0[g]; I < t1; I++) {
+                row_start[I] = (int)work.size();
+                for (int J0 = I; J0 < t1; J0 += C) work.push_back(make_int4(I, J0, std::min(t1, J0 + C), 0));
+            }
+            c->seg_w1[g] = (int)work.size();
+        }
+        row_start[nb] = (int)work.size();
+        c->nwork = (int)work.size();
+        std::vector<int4> seg(ns);
+        std::vector<int2> segw(ns);
+        for (int g = 0; g < ns; g++) {
+            seg[g] = make_int4(c->seg_t0[g], c->seg_t1[g], c->seg_n[g], 0);
+            segw[g] = make_int2(c->seg_w0[g], c->seg_w1[g]);
+        }
+        if ((rc = dalloc(c, &c->d_work, work.size())) || (rc = dalloc(c, &c->d_row_start, nb + 1)) ||
+            (rc = dalloc(c, (double4**)&c->d_pbuf, ((size_t)c->nwork + c->ntri) * 64)) ||
+            (rc = dalloc(c, &c->d_pbuf2, ((size_t)c->nwork + c->ntri) * 64)) ||
+            (rc = dalloc(c, &c->d_part_e, (size_t)c->nwork + 1)) || (rc = dalloc(c, &c->Emv, npad)) ||
+            (rc = dalloc(c, &c->d_tcnt, nb)) || (rc = dalloc(c, &c->d_tfirst, nb)) || (rc = dalloc(c, &c->d_tseg, nb)) ||
+            (rc = dalloc(c, &c->d_tcol, nb)) || (rc = dalloc(c, &c->d_seg, ns)) || (rc = dalloc(c, &c->d_segw, ns)) ||
+            (rc = dalloc(c, &c->d_done, 2)))
+            return bail(rc);
+        bool ok = true;
+        auto up = [&](void* dst, const void* src, size_t bytes) {
+            if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) ok = false;
+        };
+        up(c->d_work, work.data(), work.size() * sizeof(int4));
+        up(c->d_row_start, row_start.data(), row_start.size() * sizeof(int));
+        up(c->d_tcnt, c->tcnt.data(), nb * sizeof(int));
+        up(c->d_tfirst, c->tfirst.data(), nb * sizeof(int));
+        up(c->d_tseg, c->tseg.data(), nb * sizeof(int));
+        up(c->d_tcol, c->tcol.data(), nb * sizeof(size_t));
+        up(c->d_seg, seg.data(), ns * sizeof(int4));
+        up(c->d_segw, segw.data(), ns * sizeof(int2));
+        if (!ok || hipStreamSynchronize(c->stream) != hipSuccess) {
+            c->err = "hipMemcpy of the tile tables failed";
+            return bail(TDP_ERR_HIP);
+        }
+    }
+    if ((rc = dalloc(c, &c->posa, npad)) || (rc = dalloc(c, &c->qd, npad)) || (rc = dalloc(c, &c->mu, npad)) ||
+        (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
+        (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) || (rc = dalloc(c, &c->torque, npad)) ||
+        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || (rc = dalloc(c, &c->vec32, npad)) ||
+        (rc = dalloc(c, &c->d_pos3, 3 * (size_t)std::max(c->n, 1))) || (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) ||
+        (rc = dalloc(c, &c->d_alpha, npad)) || (rc = dalloc(c, &c->d_inva, npad)) ||
+        (rc = dalloc(c, &c->d_thole, npad)) || (rc = dalloc(c, &c->d_charge, npad)) ||
+        (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)nb + 2)) ||
+        (rc = dalloc(c, &c->d_scalars, (size_t)SC_COUNT * ns)) || (rc = dalloc(c, &c->d_coef, 64)) ||
+        (rc = dalloc(c, &c->d_energy, ns)) || (rc = dalloc(c, &c->d_axis, npad)) ||
+        (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
+        (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
+        (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)))
+        return bail(rc);
+    if (c->polarization == TDP_EXTRAPOLATED) {
+        c->hist_order = std::max(4, (int)c->extrap.size());
+        if ((rc = dalloc(c, &c->hist, (size_t)c->hist_order * npad))) return bail(rc);
+    }
+    if ((rc = upload_tables(c)) || (rc = upload_particles(c))) return bail(rc);
+    *out = c;
     return TDP_OK;
 }
 
@@ -687,108 +942,79 @@ int tdp_create(const tdp_force_desc* d, const tdp_options* opts, tdp_context** o
     *out = nullptr;
     int rc = validate(nullptr, d);
     if (rc) return rc;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
-        return fail(nullptr, TDP_ERR_NO_DEVICE, "no HIP device is available");
-    tdp_context* c = new tdp_context();
-    c->device = opts ? opts->device : 0;
-    c->precision = opts ? opts->precision : TDP_PRECISION_DOUBLE;
-    c->solver = opts ? opts->solver : TDP_SOLVER_JACOBI;
-    if (c->device < 0 || c->device >= ndev) {
-        delete c;
-        return fail(nullptr, TDP_ERR_NO_DEVICE, "invalid device ordinal");
+    std::vector<int> sizes{d->num_particles};
+    std::vector<int> maxit{d->mutual_max_iterations};
+    std::vector<double> target{d->mutual_target_epsilon};
+    return create_context(d, sizes, maxit, target, opts, out);
+}
+
+int tdp_batch_create(const tdp_force_desc* descs, int32_t count, const tdp_options* opts, tdp_context** out) {
+    if (!descs || !out || count <= 0) return fail(nullptr, TDP_ERR_INVALID_ARGUMENT, "null argument or empty batch");
+    *out = nullptr;
+    for (int g = 0; g < count; g++) {
+        const tdp_force_desc& d = descs[g];
+        if (d.nonbonded_method != TDP_NO_CUTOFF)
+            return fail(nullptr, TDP_ERR_INVALID_ARGUMENT, "batch: every system must use the NoCutoff method");
+        if (d.polarization_type != descs[0].polarization_type)
+            return fail(nullptr, TDP_ERR_INVALID_ARGUMENT, "batch: every system must use the same polarization type");
+        if (d.num_particles <= 0)
+            return fail(nullptr, TDP_ERR_INVALID_ARGUMENT, "batch: system " + std::to_string(g) + " has no particles");
+        int rc = validate(nullptr, &d);
+        if (rc) return rc;
     }
-    auto bail = [&](int code) {
-        g_create_error = c->err;
-        tdp_destroy(c);
-        return code;
-    };
-    if (hipSetDevice(c->device) != hipSuccess) {
-        c->err = "hipSetDevice failed";
-        return bail(TDP_ERR_HIP);
+    // one force description with the systems' particles concatenated and their indices offset
+    std::vector<int> sizes(count), maxit(count);
+    std::vector<double> target(count);
+    size_t ntot = 0, ncov = 0;
+    for (int g = 0; g < count; g++) {
+        sizes[g] = descs[g].num_particles;
+        maxit[g] = descs[g].mutual_max_iterations;
+        target[g] = descs[g].mutual_target_epsilon;
+        ntot += sizes[g];
+        ncov += descs[g].covalent_offsets[4 * sizes[g]];
     }
-    c->n = d->num_particles;
-    c->npad = std::max(64, (c->n + 63) / 64 * 64);
-    c->nblk = (c->n + 63) / 64;
-    c->method = d->nonbonded_method;
-    c->polarization = d->polarization_type;
-    c->cutoff = d->cutoff;
-    c->ewald_tol = d->ewald_error_tolerance;
-    c->pme_alpha = d->pme_alpha;
-    for (int k = 0; k < 3; k++) c->grid[k] = d->pme_grid[k];
-    std::memcpy(c->default_box, d->default_box, sizeof c->default_box);
-    // Solver settings are read only for the active polarization type (ReferenceTholeDipoleKernels.cpp:127-134).
-    if (c->polarization == TDP_MUTUAL) {
-        c->max_iter = d->mutual_max_iterations;
-        c->target_eps = d->mutual_target_epsilon;
-    } else if (c->polarization == TDP_EXTRAPOLATED) {
-        if (d->num_extrapolation_coefficients > 0)
-            c->extrap.assign(d->extrapolation_coefficients,
-                             d->extrapolation_coefficients + d->num_extrapolation_coefficients);
-    }
-    copy_particles(c, d);
-    build_exclusion_pairs(c, d);
-    // identity order until the first evaluation computes the spatial one
-    c->perm.resize(c->n);
-    c->inv.resize(c->n);
-    for (int i = 0; i < c->n; i++) c->perm[i] = c->inv[i] = i;
-    if (hipStreamCreate(&c->stream) != hipSuccess) {
-        c->err = "hipStreamCreate failed";
-        return bail(TDP_ERR_HIP);
-    }
-    const int npad = c->npad;
-    // Upper-triangle work items: tile row I split into chunks of C tiles (J >= I).  Short items
-    // (C = 3 on large systems) keep the many one-wave workgroups evenly spread over the CUs to the
-    // end of the launch; measured at 36,864 sites, C = 3 beats C = 20 by 25% on the T*mu pass, and
-    // C = 1 loses more in the reduction than it gains.
-    {
-        const int nb = c->nblk;
-        c->ntri = (size_t)nb * (nb + 1) / 2;
-        const int C = (int)std::max<size_t>(1, std::min<size_t>(3, c->ntri / 16384));
-        std::vector<int4> work;
-        std::vector<int> row_start(nb + 1, 0);
-        for (int I = 0; I < nb; I++) {
-            row_start[I] = (int)work.size();
-            for (int J0 = I; J0 < nb; J0 += C) work.push_back(make_int4(I, J0, std::min(nb, J0 + C), 0));
+    if (ntot > (size_t)INT32_MAX / 4)
+        return fail(nullptr, TDP_ERR_INVALID_ARGUMENT, "batch: too many particles");
+    std::vector<double> q, dip, pol, th;
+    std::vector<int32_t> at, az, ax, ay, off(1, 0), atoms;
+    q.reserve(ntot); dip.reserve(3 * ntot); pol.reserve(ntot); th.reserve(ntot);
+    at.reserve(ntot); az.reserve(ntot); ax.reserve(ntot); ay.reserve(ntot); off.reserve(4 * ntot + 1);
+    atoms.reserve(ncov);
+    int p0 = 0;
+    for (int g = 0; g < count; g++) {
+        const tdp_force_desc& d = descs[g];
+        const int n = d.num_particles;
+        auto shift = [p0](int32_t a) { return a >= 0 ? a + p0 : a; };
+        for (int i = 0; i < n; i++) {
+            q.push_back(d.charges[i]);
+            for (int k = 0; k < 3; k++) dip.push_back(d.dipoles[3 * i + k]);
+            pol.push_back(d.polarizabilities[i]);
+            th.push_back(d.thole[i]);
+            at.push_back(d.axis_types[i]);
+            az.push_back(shift(d.atom_z[i]));
+            ax.push_back(shift(d.atom_x[i]));
+            ay.push_back(shift(d.atom_y[i]));
+            for (int t = 0; t < 4; t++) {
+                for (int k = d.covalent_offsets[4 * i + t]; k < d.covalent_offsets[4 * i + t + 1]; k++)
+                    atoms.push_back(shift(d.covalent_atoms[k]));
+                off.push_back((int32_t)atoms.size());
+            }
         }
-        row_start[nb] = (int)work.size();
-        c->nwork = (int)work.size();
-        if ((rc = dalloc(c, &c->d_work, work.size())) || (rc = dalloc(c, &c->d_row_start, nb + 1)) ||
-            (rc = dalloc(c, (double4**)&c->d_pbuf, ((size_t)c->nwork + c->ntri) * 64)) ||
-            (rc = dalloc(c, &c->d_pbuf2, ((size_t)c->nwork + c->ntri) * 64)) ||
-            (rc = dalloc(c, &c->d_part_e, (size_t)c->nwork + 1)) || (rc = dalloc(c, &c->Emv, npad)))
-            return bail(rc);
-        if (!work.empty() &&
-            (hipMemcpyAsync(c->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream) != hipSuccess ||
-             hipMemcpyAsync(c->d_row_start, row_start.data(), row_start.size() * sizeof(int), hipMemcpyHostToDevice, c->stream) !=
-                 hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)) {
-            c->err = "hipMemcpy of the work list failed";
-            return bail(TDP_ERR_HIP);
-        }
+        p0 += n;
     }
-    if ((rc = dalloc(c, &c->posa, npad)) || (rc = dalloc(c, &c->qd, npad)) || (rc = dalloc(c, &c->mu, npad)) ||
-        (rc = dalloc(c, &c->E0, npad)) || (rc = dalloc(c, &c->r, npad)) || (rc = dalloc(c, &c->z, npad)) ||
-        (rc = dalloc(c, &c->p, npad)) || (rc = dalloc(c, &c->q, npad)) ||
-        (rc = dalloc(c, &c->torque, npad)) ||
-        (rc = dalloc(c, &c->contrib, 4 * (size_t)npad)) || 
-        (rc = dalloc(c, &c->vec32, npad)) || (rc = dalloc(c, &c->d_pos3, 3 * (size_t)npad)) ||
-        (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_alpha, npad)) ||
-        (rc = dalloc(c, &c->d_inva, npad)) ||
-        (rc = dalloc(c, &c->d_thole, npad)) || (rc = dalloc(c, &c->d_charge, npad)) ||
-        (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)red_blocks(npad))) ||
-        (rc = dalloc(c, &c->d_scalars, SC_COUNT)) ||
-        (rc = dalloc(c, &c->d_coef, 64)) || (rc = dalloc(c, &c->d_energy, 1)) || (rc = dalloc(c, &c->d_axis, npad)) ||
-        (rc = dalloc(c, &c->d_exc_start, c->nblk + 1)) || (rc = dalloc(c, &c->d_goff, c->n + 1)) ||
-        (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
-        (rc = dalloc(c, &c->tb_c, c->nblk + 1)) || (rc = dalloc(c, &c->tb_h, c->nblk + 1)))
-        return bail(rc);
-    if (c->polarization == TDP_EXTRAPOLATED) {
-        c->hist_order = std::max(4, (int)c->extrap.size());
-        if ((rc = dalloc(c, &c->hist, (size_t)c->hist_order * npad))) return bail(rc);
-    }
-    if ((rc = upload_tables(c)) || (rc = upload_particles(c))) return bail(rc);
-    *out = c;
-    return TDP_OK;
+    tdp_force_desc all = descs[0];
+    all.num_particles = (int32_t)ntot;
+    all.charges = q.data();
+    all.dipoles = dip.data();
+    all.polarizabilities = pol.data();
+    all.thole = th.data();
+    all.axis_types = at.data();
+    all.atom_z = az.data();
+    all.atom_x = ax.data();
+    all.atom_y = ay.data();
+    all.covalent_offsets = off.data();
+    all.covalent_atoms = atoms.data();
+    return create_context(&all, sizes, maxit, target, opts, out);
 }
 
 void tdp_destroy(tdp_context* c) {
@@ -807,6 +1033,7 @@ void tdp_destroy(tdp_context* c) {
 
 int tdp_copy_parameters(tdp_context* c, const tdp_force_desc* d) {
     if (!c || !d) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->nseg != 1) return fail(c, TDP_ERR_INVALID_ARGUMENT, "updateParametersInContext: not available for a batch");
     if (d->num_particles != c->n)
         return fail(c, TDP_ERR_PARTICLE_COUNT, "updateParametersInContext: The number of particles has changed");
     // Only the per-particle parameters are copied (ReferenceTholeDipoleKernels.cpp:317-339).
@@ -828,16 +1055,7 @@ int tdp_execute(tdp_context* c, const double* positions, const double* box, int3
     if ((rc = upload_positions(c, positions))) return rc;
     if ((rc = setup_and_solve(c, box, positions))) return rc;
     forces_and_energy(c, box);
-    const int n = c->n;
-    std::vector<double> f(3 * (size_t)n);
-    double e = 0.0;
-    if (n > 0) TDP_HIP(c, hipMemcpyAsync(f.data(), c->d_frc, 3 * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    TDP_HIP(c, hipMemcpyAsync(&e, c->d_energy, sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    TDP_HIP(c, hipStreamSynchronize(c->stream));
-    if (forces && include_forces)
-        for (int k = 0; k < n; k++)
-            for (int d = 0; d < 3; d++) forces[3 * c->perm[k] + d] += f[3 * k + d];
-    if (energy) *energy = e;
+    if ((rc = download_results(c, include_forces ? forces : nullptr, energy))) return rc;
     (void)include_energy;  // both are always computed, as in the reference (ReferenceTholeDipoleKernels.cpp:201-215)
     return TDP_OK;
 }
@@ -860,8 +1078,9 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
         return rc;
     }
     forces_and_energy(c, box);
-    if (d_forces && c->n > 0) launch_scatter_accumulate(c->stream, c->n, c->d_perm, c->d_frc, d_forces);
-    if (d_energy) TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    if (d_forces && c->n > 0) launch_scatter_accumulate(c->stream, c->npad, c->d_perm, c->d_frc, d_forces);
+    if (d_energy)
+        TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     TDP_HIP(c, hipEventRecord(ev, c->stream));
     TDP_HIP(c, hipStreamWaitEvent(user, ev, 0));
     c->ev_pool.push_back(ev);
@@ -876,16 +1095,17 @@ int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, 
     hipSetDevice(c->device);
     if ((rc = upload_positions(c, positions))) return rc;
     if ((rc = setup_and_solve(c, box, positions))) return rc;
-    const int n = c->n;
-    std::vector<double4> mu(n), qd(n);
-    if (n > 0) {
-        TDP_HIP(c, hipMemcpyAsync(mu.data(), c->mu, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
-        TDP_HIP(c, hipMemcpyAsync(qd.data(), c->qd, n * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+    const int npad = c->npad;
+    std::vector<double4> mu(npad), qd(npad);
+    if (c->n > 0) {
+        TDP_HIP(c, hipMemcpyAsync(mu.data(), c->mu, npad * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
+        TDP_HIP(c, hipMemcpyAsync(qd.data(), c->qd, npad * sizeof(double4), hipMemcpyDeviceToHost, c->stream));
     }
     TDP_HIP(c, hipStreamSynchronize(c->stream));
-    for (int i = 0; i < n; i++) {
+    for (int i = 0; i < (c->n > 0 ? npad : 0); i++) {
         double a[3] = {mu[i].x, mu[i].y, mu[i].z}, b[3] = {qd[i].x, qd[i].y, qd[i].z};
         int o = c->perm[i];
+        if (o < 0) continue;
         for (int k = 0; k < 3; k++)
             out[3 * o + k] = kind == TDP_DIPOLES_INDUCED ? a[k] : (kind == TDP_DIPOLES_LAB_PERMANENT ? b[k] : b[k] + a[k]);
     }
@@ -896,6 +1116,7 @@ int tdp_get_electrostatic_potential(tdp_context* c, const double* positions, con
                                     int32_t num_grid, double* out) {
     if (!c || (!positions && c->n > 0) || (num_grid > 0 && (!grid || !out)) || num_grid < 0)
         return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->nseg != 1) return fail(c, TDP_ERR_INVALID_ARGUMENT, "getElectrostaticPotential: not available for a batch");
     int rc = check_box(c, box);
     if (rc) return rc;
     hipSetDevice(c->device);
@@ -917,6 +1138,7 @@ int tdp_get_electrostatic_potential(tdp_context* c, const double* positions, con
 int tdp_get_system_multipole_moments(tdp_context* c, const double* positions, const double* box, const double* masses,
                                      double* out) {
     if (!c || !out || !masses || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->nseg != 1) return fail(c, TDP_ERR_INVALID_ARGUMENT, "getSystemMultipoleMoments: not available for a batch");
     int rc = check_box(c, box);
     if (rc) return rc;
     hipSetDevice(c->device);
@@ -983,12 +1205,36 @@ int tdp_get_solver_stats(const tdp_context* cc, tdp_solver_stats* st) {
         hipStreamSynchronize(c->stream);
         harvest_timing(c);
     }
-    st->iterations = c->iterations;
-    st->epsilon = c->epsilon;
-    st->converged = c->converged;
+    st->iterations = c->iterations[0];
+    st->epsilon = c->epsilon[0];
+    st->converged = c->converged[0];
     st->matvec_ms = c->matvec_ms;
     st->matvec_launches = c->matvec_launches;
     return TDP_OK;
+}
+
+int tdp_batch_get_solver_stats(const tdp_context* cc, int32_t system, tdp_solver_stats* st) {
+    tdp_context* c = const_cast<tdp_context*>(cc);
+    if (!c || !st) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (system < 0 || system >= c->nseg) return fail(c, TDP_ERR_INVALID_ARGUMENT, "system index out of range");
+    int rc = tdp_get_solver_stats(cc, st);
+    if (rc) return rc;
+    st->iterations = c->iterations[system];
+    st->epsilon = c->epsilon[system];
+    st->converged = c->converged[system];
+    return TDP_OK;
+}
+
+int32_t tdp_batch_num_systems(const tdp_context* c) { return c ? c->nseg : 0; }
+
+int tdp_batch_execute(tdp_context* c, const double* positions, double* forces, double* energies) {
+    if (!c || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    return tdp_execute(c, positions, nullptr, 1, 1, forces, energies);
+}
+
+int tdp_batch_execute_device(tdp_context* c, const double* d_positions, double* d_forces, double* d_energies,
+                             void* stream) {
+    return tdp_execute_device(c, d_positions, nullptr, d_forces, d_energies, stream);
 }
 
 int tdp_set_timing(tdp_context* c, int32_t enable) {

@@ -17,9 +17,29 @@ enum SolverScalars {
     SC_COUNT = 8
 };
 
+// Tiles of a context that may hold several independent systems (segments), each in a
+// contiguous range of whole 64-slot tiles; slots past a system's particles are padding.
+struct TileMap {
+    const int* cnt;      // [nblk] valid particles in the tile (64 except a system's last tile)
+    const int* first;    // [nblk] first tile of the tile's system
+    const int* seg;      // [nblk] system of the tile
+    const size_t* col;   // [nblk] tile-partial slab index of tile (first[J], J)
+};
+
+// Systems for the solver kernels: tiles [seg.x, seg.y) and seg.z particles of system s, its pair
+// work items [segw.x, segw.y); done[0] counts systems whose solve has stopped.
+struct SegLaunch {
+    int nseg, ntile, npad;
+    const int* tseg;
+    const int4* seg;
+    const int2* segw;
+    int* done;
+};
+
 // Geometry of one all-pairs launch: nblk tiles of 64 particles.
 struct PairLaunch {
-    int n, npad, nblk;
+    int npad, nblk;
+    TileMap tm;
     bool pbc;
     double box[3];
     const int* exc_start;                 // [nblk+1]
@@ -32,7 +52,7 @@ struct PairLaunch {
     int nwork;
     const int4* work;
     const int* row_start;
-    size_t slab;   // 64 * number of upper-triangle tiles: stride between components of the tile partials
+    size_t slab;   // 64 * number of tile pairs: stride between components of the tile partials
 };
 
 // Pairwise passes over the upper-triangle work items.  Pi: nwork*64 item partials; Pj: the
@@ -45,11 +65,11 @@ void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, co
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
                              const float4* mu32, const double* inva, const double* scalars, double4* Pi, double* Pj);
 
-void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
+void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h);
 
-// posa: FP64 positions (internal order) and Thole parameter.
-void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
+// posa: FP64 positions (internal order) and Thole parameter; slots with perm < 0 are padding.
+void launch_load_positions(hipStream_t s, int npad, const double* pos3, const int* perm, const double* thole,
                            double4* posa);
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
                    const int4* axis, double4* qd);
@@ -58,24 +78,23 @@ void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, c
 void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
                      double* PjT, double* part_e, double* frc, double4* torque);
-void launch_field_init(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
-                       double4* E0, double4* mu, float4* mu32);
-int red_blocks(int npad);
-void launch_jacobi_update(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
-                          const double* alpha, double4* mu, float4* mu32, const double* scalars, double* red);
-void launch_pcg_init(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
-                     const double* alpha, const double4* x, double4* r, double4* p, float4* p32, double* red,
-                     double* scalars);
-void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
+void launch_field_init(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha, double4* E0,
+                       double4* mu, float4* mu32);
+// red: 2 * ntile doubles; scalars: SC_COUNT per system
+void launch_jacobi_update(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
+                          const double* alpha, double4* mu, float4* mu32, double* scalars, double* red);
+void launch_pcg_init(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0, const double* alpha,
+                     const double4* x, double4* r, double4* p, float4* p32, double* red, double* scalars);
+void launch_pcg_iteration_tail(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha,
                                double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
                                double* scalars);
-void launch_extrap_step(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
+void launch_extrap_step(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
                         const double* alpha, double4* mu, float4* mu32, double4* hist_o);
-void launch_extrap_combine(hipStream_t s, int n, int npad, int order, const double4* hist, const double* coef,
+void launch_extrap_combine(hipStream_t s, const SegLaunch& G, int order, const double4* hist, const double* coef,
                            double4* mu, float4* mu32);
-void launch_extrap_error(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
+void launch_extrap_error(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
                          const double* alpha, const double4* mu, double* red, double* scalars);
-void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double* out);
+void launch_reduce_energy(hipStream_t s, const SegLaunch& G, const double* part_e, double* out);
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,
                        double4* contrib);
 void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx, const double4* contrib, double* frc);

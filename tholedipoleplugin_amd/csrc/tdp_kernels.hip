@@ -97,15 +97,13 @@ __global__ void k_rotate_dipoles(int n, const double4* __restrict__ posa, const 
 // ------------------------------------------------------------------ positions
 // Gathers the caller's positions (original order) into the internal spatial order:
 // slot i holds particle perm[i].
-__global__ void k_load_positions(int n, int npad, const double* __restrict__ pos3, const int* __restrict__ perm,
+__global__ void k_load_positions(int npad, const double* __restrict__ pos3, const int* __restrict__ perm,
                                  const double* __restrict__ thole, double4* __restrict__ posa) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
     double4 p = make_double4(0.0, 0.0, 0.0, 0.0);
-    if (i < n) {
-        int o = perm[i];
-        p = make_double4(pos3[3 * o], pos3[3 * o + 1], pos3[3 * o + 2], thole[i]);
-    }
+    const int o = perm[i];
+    if (o >= 0) p = make_double4(pos3[3 * o], pos3[3 * o + 1], pos3[3 * o + 2], thole[i]);
     posa[i] = p;
 }
 
@@ -113,11 +111,11 @@ __global__ void k_load_positions(int n, int npad, const double* __restrict__ pos
 // Per tile of 64 particles (internal order): centre and half extent of its bounding box
 // (PBC: offsets from the tile's first particle wrapped to the minimum image, so a tile that
 // straddles the box edge stays compact), and the largest/smallest Thole parameter.
-__global__ void k_tile_bounds(int n, int nblk, const double4* __restrict__ posa, double bx, double by, double bz,
-                              int pbc, float4* __restrict__ tb_c, float4* __restrict__ tb_h) {
+__global__ void k_tile_bounds(const int* __restrict__ tcnt, int nblk, const double4* __restrict__ posa, double bx,
+                              double by, double bz, int pbc, float4* __restrict__ tb_c, float4* __restrict__ tb_h) {
     int T = blockIdx.x * blockDim.x + threadIdx.x;
     if (T >= nblk) return;
-    int b = T * kTile, e = min(n, b + kTile);
+    int b = T * kTile, e = b + tcnt[T];
     double4 p0 = posa[b];
     double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
     double amax = p0.w, amin = p0.w;
@@ -186,8 +184,11 @@ __device__ __forceinline__ void load_site(ElecSite& s, double4 p, double4 q, dou
 // j sums go to the column-major partial slab Pj (component c of slot s at Pj[c S + s]), each
 // item's i sums to Pi[w]; k_col_reduce adds them in a fixed order, so results are deterministic.
 
-// Tile (I, J), I <= J, in column-major upper-triangle order: the tiles of column J are contiguous.
-__device__ __forceinline__ size_t col_tile(int I, int J) { return (size_t)J * (J + 1) / 2 + I; }
+// Tile (I, J), I <= J, of one system in column-major upper-triangle order: the tiles of column J
+// are contiguous, starting at col[J] with the system's first tile.
+__device__ __forceinline__ size_t slab_tile(const TileMap& tm, int I, int J) {
+    return tm.col[J] + (size_t)(I - tm.first[J]);
+}
 
 __device__ __forceinline__ void store3(double* __restrict__ P, size_t S, size_t slot, double x, double y, double z) {
     P[slot] = x;
@@ -231,7 +232,7 @@ __device__ __forceinline__ void damped_rr_wave(double r2, double ai, double aj, 
 
 // ------------------------------------------------------------------ T * mu, FP64 (a7)
 template <bool PBC>
-__global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const int4* __restrict__ work,
                                                    const double4* __restrict__ posa, const double4* __restrict__ mu,
                                                    const double* __restrict__ inva, const int* __restrict__ exc_start,
                                                    const int* __restrict__ exc_J,
@@ -239,26 +240,28 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
                                                    const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
                                                    BoxT64 box, const double* __restrict__ scalars,
                                                    double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
-    if (scalars && solver_done(scalars)) return;
     __shared__ double4 s_p[kTile];
     __shared__ double4 s_u[kTile];
     __shared__ double s_ia[kTile];
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
+    if (scalars && solver_done(scalars + SC_COUNT * tm.seg[I])) return;
     const int i = I * kTile + lane;
-    const bool ivalid = i < n;
+    const int cntI = tm.cnt[I];
+    const bool ivalid = lane < cntI;
     const double4 pi = posa[i];
     const double4 ui = mu[i];
     const double iai = inva[i];
     const float4 cI = tb_c[I], hI = tb_h[I];
-    const bool ifull = I * kTile + kTile <= n;
+    const bool ifull = cntI == kTile;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
         const int jb = J * kTile;
+        const int cntJ = tm.cnt[J];
         __syncthreads();
         s_p[lane] = posa[jb + lane];
         s_u[lane] = mu[jb + lane];
@@ -273,7 +276,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
         }
         double jx = 0.0, jy = 0.0, jz = 0.0;   // j side of slot (lane + k) mod 64
         int kend;
-        if (!masked && !diag && ifull && jb + kTile <= n &&
+        if (!masked && !diag && ifull && cntJ == kTile &&
             tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
 #pragma unroll 4
             for (int k = 0; k < kTile; k++) {
@@ -304,7 +307,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
             const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
-                const bool ok = ivalid && (jb + jj < n) && !((mi >> jj) & 1ull) &&
+                const bool ok = ivalid && (jj < cntJ) && !((mi >> jj) & 1ull) &&
                                 !(diag && k == kTile / 2 && lane >= kTile / 2);
                 const double4 pj = s_p[jj];
                 const double4 uj = s_u[jj];
@@ -332,7 +335,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
             kend = k1 + 1;
         }
         // after the last rotation this lane holds the sums of slot (lane + kend) mod 64
-        store3(Pj, S, col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
+        store3(Pj, S, slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
     }
     Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
@@ -346,7 +349,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(int n, int nblk, const int4* 
 // largest terms, so keeping them in FP64 holds the operator's error to that of the FP32 far field;
 // all partial sums are FP64.
 template <bool PBC>
-__global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) void k_sym_matvec_mixed(TileMap tm, int nblk, const int4* __restrict__ work,
                                                          const double4* __restrict__ posa,
                                                          const double4* __restrict__ mu,
                                                          const float4* __restrict__ mu32,
@@ -357,7 +360,6 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
                                                          const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
                                                          BoxT64 box, const double* __restrict__ scalars,
                                                          double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
-    if (scalars && solver_done(scalars)) return;
     // One LDS area holds either the near tile (FP64 dipoles and 1/a) or the far tile (FP32
     // dipoles).  The far tile is staged twice over, so that slot (lane + k) mod 64 is entry
     // lane + k: the step's address is the lane's base plus a constant offset.
@@ -372,21 +374,24 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
+    if (scalars && solver_done(scalars + SC_COUNT * tm.seg[I])) return;
     const int i = I * kTile + lane;
-    const bool ivalid = i < n;
+    const int cntI = tm.cnt[I];
+    const bool ivalid = lane < cntI;
     const double4 pi = posa[i];
     const double4 ui = mu[i];
     const double iai = inva[i];
     const float4 uf = mu32[i];
     const float3 uf3 = make_float3(3.f * uf.x, 3.f * uf.y, 3.f * uf.z);
     const float4 cI = tb_c[I], hI = tb_h[I];
-    const bool ifull = I * kTile + kTile <= n;
+    const bool ifull = cntI == kTile;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
         const int jb = J * kTile;
+        const int cntJ = tm.cnt[J];
         const bool diag = J == I;
         const bool masked = (e < eEnd && exc_J[e] == J);
         unsigned long long mi = 0ull;
@@ -394,7 +399,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
             mi = exc_mask[((size_t)e * 3 + 2) * kTile + lane];
             e++;
         }
-        const bool far = !masked && !diag && ifull && jb + kTile <= n &&
+        const bool far = !masked && !diag && ifull && cntJ == kTile &&
                          tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by,
                                               (float)box.bz);
         double jx, jy, jz;
@@ -455,7 +460,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
             const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
-                const bool ok = ivalid && (jb + jj < n) && !((mi >> jj) & 1ull) &&
+                const bool ok = ivalid && (jj < cntJ) && !((mi >> jj) & 1ull) &&
                                 !(diag && k == kTile / 2 && lane >= kTile / 2);
                 const double4 pj = s_p[jj];
                 const double4 uj = s_u[jj];
@@ -482,7 +487,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
             }
             kend = k1 + 1;
         }
-        store3(Pj, S, col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
+        store3(Pj, S, slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
     }
     Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
@@ -493,7 +498,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(int n, int nblk, const 
 // sweep of k_sym_matvec: Delta = r_j - r_i,
 //   E_i -= m [Delta (rr3 q_j - rr5 d_j.Delta) + rr3 d_j],  E_j += m [Delta (rr3 q_i + rr5 d_i.Delta) - rr3 d_i].
 template <bool PBC>
-__global__ __launch_bounds__(64) void k_sym_field(int n, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const int4* __restrict__ work,
                                                   const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                   const double* __restrict__ inva, const int* __restrict__ exc_start,
                                                   const int* __restrict__ exc_J,
@@ -508,18 +513,20 @@ __global__ __launch_bounds__(64) void k_sym_field(int n, int nblk, const int4* _
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
     const int i = I * kTile + lane;
-    const bool ivalid = i < n;
+    const int cntI = tm.cnt[I];
+    const bool ivalid = lane < cntI;
     const double4 pi = posa[i];
     const double4 qi = qd[i];
     const double iai = inva[i];
     const float4 cI = tb_c[I], hI = tb_h[I];
-    const bool ifull = I * kTile + kTile <= n;
+    const bool ifull = cntI == kTile;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int e = exc_start[I];
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
         const int jb = J * kTile;
+        const int cntJ = tm.cnt[J];
         __syncthreads();
         s_p[lane] = posa[jb + lane];
         s_q[lane] = qd[jb + lane];
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(64) void k_sym_field(int n, int nblk, const int4* _
         }
         double jx = 0.0, jy = 0.0, jz = 0.0;
         int kend;
-        if (!masked && !diag && ifull && jb + kTile <= n &&
+        if (!masked && !diag && ifull && cntJ == kTile &&
             tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
 #pragma unroll 2
             for (int k = 0; k < kTile; k++) {
@@ -566,7 +573,7 @@ __global__ __launch_bounds__(64) void k_sym_field(int n, int nblk, const int4* _
             const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
-                const bool ok = ivalid && (jb + jj < n) && !((m0 >> jj) & 1ull) &&
+                const bool ok = ivalid && (jj < cntJ) && !((m0 >> jj) & 1ull) &&
                                 !(diag && k == kTile / 2 && lane >= kTile / 2);
                 const double scale = ((mh >> jj) & 1ull) ? 0.5 : 1.0;
                 const double4 pj = s_p[jj];
@@ -593,7 +600,7 @@ __global__ __launch_bounds__(64) void k_sym_field(int n, int nblk, const int4* _
             }
             kend = k1 + 1;
         }
-        store3(Pj, S, col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
+        store3(Pj, S, slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
     }
     Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
@@ -628,7 +635,7 @@ __device__ __forceinline__ void elec_sym_step(const ElecSite& me, int my_orig, c
 }
 
 template <bool PBC, bool MUTUAL>
-__global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int4* __restrict__ work,
                                                  const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                  const double4* __restrict__ mu, const double* __restrict__ inva,
                                                  const int* __restrict__ orig, const int* __restrict__ exc_start,
@@ -647,12 +654,13 @@ __global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
     const int i = I * kTile + lane;
-    const bool ivalid = i < n;
+    const int cntI = tm.cnt[I];
+    const bool ivalid = lane < cntI;
     ElecSite me;
     load_site(me, posa[i], qd[i], mu[i], inva[i]);
     const int my_orig = orig[i];
     const float4 cI = tb_c[I], hI = tb_h[I];
-    const bool ifull = I * kTile + kTile <= n;
+    const bool ifull = cntI == kTile;
     ElecAcc acc = {0, 0, 0, 0, 0, 0};
     double en = 0.0;
     int e = exc_start[I];
@@ -660,6 +668,7 @@ __global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
         const int jb = J * kTile;
+        const int cntJ = tm.cnt[J];
         __syncthreads();
         s_p[lane] = posa[jb + lane];
         s_q[lane] = qd[jb + lane];
@@ -678,7 +687,7 @@ __global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __
         }
         ElecAcc jacc = {0, 0, 0, 0, 0, 0};
         int kend;
-        if (!masked && !diag && ifull && jb + kTile <= n &&
+        if (!masked && !diag && ifull && cntJ == kTile &&
             tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
             for (int k = 0; k < kTile; k++) {
                 const int jj = (lane + k) & (kTile - 1);
@@ -695,7 +704,7 @@ __global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __
                 const int jj = (lane + k) & (kTile - 1);
                 const bool z0 = (m0 >> jj) & 1ull, h = (mh >> jj) & 1ull, zi = (mi >> jj) & 1ull;
                 // a pair with both scales 0 contributes nothing (this also covers the self pair)
-                const bool ok = ivalid && (jb + jj < n) && !(z0 && zi) && !(diag && k == kTile / 2 && lane >= kTile / 2);
+                const bool ok = ivalid && (jj < cntJ) && !(z0 && zi) && !(diag && k == kTile / 2 && lane >= kTile / 2);
                 const double mS = z0 ? 0.0 : (h ? 0.5 : 1.0);
                 const double iS = zi ? 0.0 : 1.0;
                 ElecSite other;
@@ -706,7 +715,7 @@ __global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __
             }
             kend = k1 + 1;
         }
-        const size_t slot = col_tile(I, J) * kTile + ((lane + kend) & (kTile - 1));
+        const size_t slot = slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1));
         store3(PjF, S, slot, jacc.fx, jacc.fy, jacc.fz);
         store3(PjT, S, slot, jacc.tx, jacc.ty, jacc.tz);
     }
@@ -723,14 +732,14 @@ __global__ __launch_bounds__(64) void k_sym_elec(int n, int nblk, const int4* __
 // in wave order.  The order is fixed, so the result is deterministic.  out4 (double4 per
 // particle) or out3 (three doubles per particle) receives the sums.
 constexpr int kRedWaves = 8;
-__global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(int n, const int* __restrict__ row_start,
+__global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const int* __restrict__ row_start,
                                                                const double4* __restrict__ Pi,
                                                                const double* __restrict__ Pj, size_t S,
                                                                const double* __restrict__ scalars,
                                                                double4* __restrict__ out4, double* __restrict__ out3) {
-    if (scalars && solver_done(scalars)) return;
-    __shared__ double red[3][kRedWaves][kTile];
     const int B = blockIdx.x, l = threadIdx.x & (kTile - 1), v = threadIdx.x / kTile;
+    if (scalars && solver_done(scalars + SC_COUNT * tm.seg[B])) return;
+    __shared__ double red[3][kRedWaves][kTile];
     double sx = 0.0, sy = 0.0, sz = 0.0;
     for (int w = row_start[B] + v; w < row_start[B + 1]; w += kRedWaves) {
         const double4 p = Pi[(size_t)w * kTile + l];
@@ -738,10 +747,11 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(int n, const int*
         sy += p.y;
         sz += p.z;
     }
-    const size_t col = col_tile(0, B);
+    const size_t col = tm.col[B];
+    const int first = tm.first[B];
 #pragma unroll 4
-    for (int Ip = v; Ip <= B; Ip += kRedWaves) {
-        const size_t k = (col + Ip) * kTile + l;
+    for (int Ip = first + v; Ip <= B; Ip += kRedWaves) {
+        const size_t k = (col + (size_t)(Ip - first)) * kTile + l;
         sx += Pj[k];
         sy += Pj[S + k];
         sz += Pj[2 * S + k];
@@ -751,7 +761,7 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(int n, const int*
     red[2][v][l] = sz;
     __syncthreads();
     const int x = B * kTile + l;
-    if (v == 0 && x < n) {
+    if (v == 0) {
         for (int u = 1; u < kRedWaves; u++) {
             sx += red[0][u][l];
             sy += red[1][u][l];
@@ -768,206 +778,201 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(int n, const int*
 }
 
 // ------------------------------------------------------------------ solver vector kernels
+// The vector kernels run one 64-thread block per tile; each tile belongs to one system (segment)
+// of the context, whose solver scalars sit at scalars + SC_COUNT * s.  Padding slots carry
+// alpha = 0 and stay zero.  Norms are summed per tile (wave_sum, fixed order) into red[T] and
+// red[ntile + T], then per system by one 64-thread block over its tiles (fixed assignment and
+// order), so every result is deterministic.  A system whose solve stops adds 1 to done[0].
 
-// E0 = sum of split partials; mu = alpha*E0 (calculateInducedDipoles, ReferenceTholeDipoleForce.cpp:743-746).
-__global__ void k_field_init(int n, int npad, const double4* __restrict__ part,
-                             const double* __restrict__ alpha, double4* __restrict__ E0, double4* __restrict__ mu,
-                             float4* __restrict__ mu32) {
+__device__ __forceinline__ double inv_or_zero(double a) { return a > 0.0 ? 1.0 / a : 0.0; }
+
+// E0 = reduced field; mu = alpha*E0 (calculateInducedDipoles, ReferenceTholeDipoleForce.cpp:743-746).
+__global__ void k_field_init(int npad, const double4* __restrict__ part, const double* __restrict__ alpha,
+                             double4* __restrict__ E0, double4* __restrict__ mu, float4* __restrict__ mu32) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= npad) return;
-    double4 f = make_double4(0, 0, 0, 0), m = make_double4(0, 0, 0, 0);
-    if (i < n) {
-        f = part[i];
-        f.w = 0.0;
-        double a = alpha[i];
-        m = make_double4(f.x * a, f.y * a, f.z * a, 0.0);
-    }
+    double4 f = part[i];
+    f.w = 0.0;
+    double a = alpha[i];
+    double4 m = make_double4(f.x * a, f.y * a, f.z * a, 0.0);
     E0[i] = f;
     mu[i] = m;
     if (mu32) mu32[i] = make_float4((float)m.x, (float)m.y, (float)m.z, 0.f);
 }
 
-// Block partial sums for the solver norms: 256 threads, fixed-order tree in LDS.
-__device__ __forceinline__ void block_sum2(double a, double b, double* __restrict__ out, int stride) {
-    __shared__ double sa[256];
-    __shared__ double sb[256];
-    int t = threadIdx.x;
-    sa[t] = a;
-    sb[t] = b;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-        if (t < s) {
-            sa[t] += sa[t + s];
-            sb[t] += sb[t + s];
-        }
-        __syncthreads();
+__device__ __forceinline__ void tile_partials(double a, double b, double* __restrict__ red, int ntile) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (threadIdx.x == 0) {
+        red[blockIdx.x] = a;
+        red[ntile + blockIdx.x] = b;
     }
-    if (t == 0) {
-        out[blockIdx.x] = sa[0];
-        out[stride + blockIdx.x] = sb[0];
+}
+
+// Sums of the tile partials of system s = blockIdx.x over its tiles [t0, t1).
+__device__ __forceinline__ void segment_sums(const int4 sg, const double* __restrict__ red, int ntile, double& a,
+                                             double& b) {
+    a = 0.0;
+    b = 0.0;
+    for (int T = sg.x + (int)threadIdx.x; T < sg.y; T += kTile) {
+        a += red[T];
+        b += red[ntile + T];
     }
+    a = wave_sum(a);
+    b = wave_sum(b);
+}
+
+__device__ __forceinline__ void mark_done(double* __restrict__ sc, int* __restrict__ done) {
+    sc[SC_DONE] = 1.0;
+    atomicAdd(done, 1);
 }
 
 // One Jacobi step of convergeInducedDipolesByDIIS (ReferenceTholeDipoleForce.cpp:835-846):
 // mu_new = alpha*(E0 + T mu); partial sums of |mu_new - mu|^2.
-__global__ __launch_bounds__(256) void k_jacobi_update(int n, int npad, const double4* __restrict__ part,
-                                                       const double4* __restrict__ E0, const double* __restrict__ alpha,
-                                                       double4* __restrict__ mu, float4* __restrict__ mu32,
-                                                       const double* __restrict__ scalars, double* __restrict__ red,
-                                                       int red_stride) {
-    if (solver_done(scalars)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d2 = 0.0;
-    if (i < n) {
-        double4 f = part[i];
-        double4 e0 = E0[i];
-        double a = alpha[i];
-        double nx = (e0.x + f.x) * a, ny = (e0.y + f.y) * a, nz = (e0.z + f.z) * a;
-        double4 o = mu[i];
-        double dx = nx - o.x, dy = ny - o.y, dz = nz - o.z;
-        d2 = dx * dx + dy * dy + dz * dz;
-        mu[i] = make_double4(nx, ny, nz, 0.0);
-        if (mu32) mu32[i] = make_float4((float)nx, (float)ny, (float)nz, 0.f);
-    }
-    block_sum2(d2, 0.0, red, red_stride);
+__global__ __launch_bounds__(64) void k_jacobi_update(const int* __restrict__ tseg, const double4* __restrict__ part,
+                                                      const double4* __restrict__ E0, const double* __restrict__ alpha,
+                                                      double4* __restrict__ mu, float4* __restrict__ mu32,
+                                                      const double* __restrict__ scalars, double* __restrict__ red,
+                                                      int ntile) {
+    if (solver_done(scalars + SC_COUNT * tseg[blockIdx.x])) return;
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double4 f = part[i];
+    double4 e0 = E0[i];
+    double a = alpha[i];
+    double nx = (e0.x + f.x) * a, ny = (e0.y + f.y) * a, nz = (e0.z + f.z) * a;
+    double4 o = mu[i];
+    double dx = nx - o.x, dy = ny - o.y, dz = nz - o.z;
+    mu[i] = make_double4(nx, ny, nz, 0.0);
+    if (mu32) mu32[i] = make_float4((float)nx, (float)ny, (float)nz, 0.f);
+    tile_partials(dx * dx + dy * dy + dz * dz, 0.0, red, ntile);
 }
 
 // Jacobi bookkeeping (ReferenceTholeDipoleForce.cpp:829-850): eps = sqrt(sum/n), iteration count,
 // stop when iterations == max or !(eps > target).
-__global__ void k_jacobi_finalize(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
-    if (solver_done(scalars)) return;
+__global__ __launch_bounds__(64) void k_jacobi_finalize(const int4* __restrict__ seg, const double* __restrict__ red,
+                                                        int ntile, double* __restrict__ scalars, int* __restrict__ done) {
+    double* sc = scalars + SC_COUNT * blockIdx.x;
+    if (solver_done(sc)) return;
+    const int4 sg = seg[blockIdx.x];
+    double sum, unused;
+    segment_sums(sg, red, ntile, sum, unused);
     if (threadIdx.x != 0) return;
-    double s = 0.0;
-    for (int k = 0; k < nred; k++) s += red[k];
-    double eps = sqrt(s / n);
-    double it = scalars[SC_ITER] + 1.0;
-    scalars[SC_EPS] = eps;
-    scalars[SC_ITER] = it;
-    if (it >= scalars[SC_MAXIT] || !(eps > scalars[SC_TARGET])) scalars[SC_DONE] = 1.0;
+    double eps = sqrt(sum / sg.z);
+    double it = sc[SC_ITER] + 1.0;
+    sc[SC_EPS] = eps;
+    sc[SC_ITER] = it;
+    if (it >= sc[SC_MAXIT] || !(eps > sc[SC_TARGET])) mark_done(sc, done);
 }
 
 // ---- Jacobi-preconditioned CG on (alpha^-1 - T) mu = E0, stopped on the same RMS(alpha r) the
-// Jacobi iteration uses (alpha r is exactly the Jacobi update).
+// Jacobi iteration uses (alpha r is exactly the Jacobi update).  A particle with alpha = 0 keeps
+// mu = 0 (its z and p stay 0), as in the reference's update mu = alpha (E0 + T mu).
 // init: x = mu (= alpha E0), r = E0 - x/alpha + T x, z = alpha r, p = z.
-__global__ __launch_bounds__(256) void k_pcg_init(int n, int npad, const double4* __restrict__ part,
-                                                  const double4* __restrict__ E0, const double* __restrict__ alpha,
-                                                  const double4* __restrict__ x, double4* __restrict__ r,
-                                                  double4* __restrict__ p, float4* __restrict__ p32,
-                                                  double* __restrict__ red, int red_stride) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0.0, zz = 0.0;
-    if (i < npad) {
-        double4 rv = make_double4(0, 0, 0, 0), zv = make_double4(0, 0, 0, 0);
-        if (i < n) {
-            double4 f = part[i];
-            double4 e0 = E0[i], xv = x[i];
-            double a = alpha[i];
-            double ia = 1.0 / a;
-            rv = make_double4(e0.x - xv.x * ia + f.x, e0.y - xv.y * ia + f.y, e0.z - xv.z * ia + f.z, 0.0);
-            zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
-            rz = rv.x * zv.x + rv.y * zv.y + rv.z * zv.z;
-            zz = zv.x * zv.x + zv.y * zv.y + zv.z * zv.z;
-        }
-        r[i] = rv;
-        p[i] = zv;
-        if (p32) p32[i] = make_float4((float)zv.x, (float)zv.y, (float)zv.z, 0.f);
-    }
-    block_sum2(rz, zz, red, red_stride);
+__global__ __launch_bounds__(64) void k_pcg_init(const double4* __restrict__ part, const double4* __restrict__ E0,
+                                                 const double* __restrict__ alpha, const double4* __restrict__ x,
+                                                 double4* __restrict__ r, double4* __restrict__ p,
+                                                 float4* __restrict__ p32, double* __restrict__ red, int ntile) {
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double4 f = part[i];
+    double4 e0 = E0[i], xv = x[i];
+    double a = alpha[i];
+    double ia = inv_or_zero(a);
+    double4 rv = make_double4(e0.x - xv.x * ia + f.x, e0.y - xv.y * ia + f.y, e0.z - xv.z * ia + f.z, 0.0);
+    double4 zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
+    r[i] = rv;
+    p[i] = zv;
+    if (p32) p32[i] = make_float4((float)zv.x, (float)zv.y, (float)zv.z, 0.f);
+    tile_partials(rv.x * zv.x + rv.y * zv.y + rv.z * zv.z, zv.x * zv.x + zv.y * zv.y + zv.z * zv.z, red, ntile);
 }
 
-__global__ void k_pcg_init_finalize(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
+__global__ __launch_bounds__(64) void k_pcg_init_finalize(const int4* __restrict__ seg, const double* __restrict__ red,
+                                                          int ntile, double* __restrict__ scalars,
+                                                          int* __restrict__ done) {
+    double* sc = scalars + SC_COUNT * blockIdx.x;
+    const int4 sg = seg[blockIdx.x];
+    double rz, zz;
+    segment_sums(sg, red, ntile, rz, zz);
     if (threadIdx.x != 0) return;
-    double rz = 0.0, zz = 0.0;
-    for (int k = 0; k < nred; k++) {
-        rz += red[k];
-        zz += red[nred + k];
-    }
-    double eps = sqrt(zz / n);
-    scalars[SC_RZ] = rz;
-    scalars[SC_EPS] = eps;
-    scalars[SC_ITER] = 0.0;
-    scalars[SC_DONE] = (scalars[SC_MAXIT] <= 0.0 || !(eps > scalars[SC_TARGET])) ? 1.0 : 0.0;
+    double eps = sqrt(zz / sg.z);
+    sc[SC_RZ] = rz;
+    sc[SC_EPS] = eps;
+    sc[SC_ITER] = 0.0;
+    sc[SC_DONE] = 0.0;
+    if (sc[SC_MAXIT] <= 0.0 || !(eps > sc[SC_TARGET])) mark_done(sc, done);
 }
 
 // q = p/alpha - T p; partial p.q
-__global__ __launch_bounds__(256) void k_pcg_q(int n, int npad, const double4* __restrict__ part,
-                                               const double* __restrict__ alpha, const double4* __restrict__ p,
-                                               double4* __restrict__ q, const double* __restrict__ scalars,
-                                               double* __restrict__ red, int red_stride) {
-    if (solver_done(scalars)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double pq = 0.0;
-    if (i < n) {
-        double4 f = part[i];
-        double4 pv = p[i];
-        double ia = 1.0 / alpha[i];
-        double4 qv = make_double4(pv.x * ia - f.x, pv.y * ia - f.y, pv.z * ia - f.z, 0.0);
-        q[i] = qv;
-        pq = pv.x * qv.x + pv.y * qv.y + pv.z * qv.z;
-    }
-    block_sum2(pq, 0.0, red, red_stride);
+__global__ __launch_bounds__(64) void k_pcg_q(const int* __restrict__ tseg, const double4* __restrict__ part,
+                                              const double* __restrict__ alpha, const double4* __restrict__ p,
+                                              double4* __restrict__ q, const double* __restrict__ scalars,
+                                              double* __restrict__ red, int ntile) {
+    if (solver_done(scalars + SC_COUNT * tseg[blockIdx.x])) return;
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double4 f = part[i];
+    double4 pv = p[i];
+    double ia = inv_or_zero(alpha[i]);
+    double4 qv = make_double4(pv.x * ia - f.x, pv.y * ia - f.y, pv.z * ia - f.z, 0.0);
+    q[i] = qv;
+    tile_partials(pv.x * qv.x + pv.y * qv.y + pv.z * qv.z, 0.0, red, ntile);
 }
 
-__global__ void k_pcg_gamma(int nred, const double* __restrict__ red, double* __restrict__ scalars) {
-    if (solver_done(scalars)) return;
-    if (threadIdx.x != 0) return;
-    double pq = 0.0;
-    for (int k = 0; k < nred; k++) pq += red[k];
-    scalars[SC_GAMMA] = scalars[SC_RZ] / pq;
+__global__ __launch_bounds__(64) void k_pcg_gamma(const int4* __restrict__ seg, const double* __restrict__ red,
+                                                  int ntile, double* __restrict__ scalars) {
+    double* sc = scalars + SC_COUNT * blockIdx.x;
+    if (solver_done(sc)) return;
+    double pq, unused;
+    segment_sums(seg[blockIdx.x], red, ntile, pq, unused);
+    if (threadIdx.x == 0) sc[SC_GAMMA] = sc[SC_RZ] / pq;
 }
 
 // x += gamma p; r -= gamma q; z = alpha r; partial r.z and z.z
-__global__ __launch_bounds__(256) void k_pcg_update(int n, const double* __restrict__ alpha, const double4* __restrict__ p,
-                                                    const double4* __restrict__ q, double4* __restrict__ x,
-                                                    double4* __restrict__ r, double4* __restrict__ z,
-                                                    const double* __restrict__ scalars, double* __restrict__ red,
-                                                    int red_stride) {
-    if (solver_done(scalars)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double rz = 0.0, zz = 0.0;
-    if (i < n) {
-        double g = scalars[SC_GAMMA];
-        double4 pv = p[i], qv = q[i], xv = x[i], rv = r[i];
-        double a = alpha[i];
-        xv.x += g * pv.x; xv.y += g * pv.y; xv.z += g * pv.z;
-        rv.x -= g * qv.x; rv.y -= g * qv.y; rv.z -= g * qv.z;
-        double4 zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
-        x[i] = xv;
-        r[i] = rv;
-        z[i] = zv;
-        rz = rv.x * zv.x + rv.y * zv.y + rv.z * zv.z;
-        zz = zv.x * zv.x + zv.y * zv.y + zv.z * zv.z;
-    }
-    block_sum2(rz, zz, red, red_stride);
+__global__ __launch_bounds__(64) void k_pcg_update(const int* __restrict__ tseg, const double* __restrict__ alpha,
+                                                   const double4* __restrict__ p, const double4* __restrict__ q,
+                                                   double4* __restrict__ x, double4* __restrict__ r,
+                                                   double4* __restrict__ z, const double* __restrict__ scalars,
+                                                   double* __restrict__ red, int ntile) {
+    const double* sc = scalars + SC_COUNT * tseg[blockIdx.x];
+    if (solver_done(sc)) return;
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double g = sc[SC_GAMMA];
+    double4 pv = p[i], qv = q[i], xv = x[i], rv = r[i];
+    double a = alpha[i];
+    xv.x += g * pv.x; xv.y += g * pv.y; xv.z += g * pv.z;
+    rv.x -= g * qv.x; rv.y -= g * qv.y; rv.z -= g * qv.z;
+    double4 zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
+    x[i] = xv;
+    r[i] = rv;
+    z[i] = zv;
+    tile_partials(rv.x * zv.x + rv.y * zv.y + rv.z * zv.z, zv.x * zv.x + zv.y * zv.y + zv.z * zv.z, red, ntile);
 }
 
-__global__ void k_pcg_beta(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
-    if (solver_done(scalars)) return;
+__global__ __launch_bounds__(64) void k_pcg_beta(const int4* __restrict__ seg, const double* __restrict__ red,
+                                                 int ntile, double* __restrict__ scalars, int* __restrict__ done) {
+    double* sc = scalars + SC_COUNT * blockIdx.x;
+    if (solver_done(sc)) return;
+    const int4 sg = seg[blockIdx.x];
+    double rz, zz;
+    segment_sums(sg, red, ntile, rz, zz);
     if (threadIdx.x != 0) return;
-    double rz = 0.0, zz = 0.0;
-    for (int k = 0; k < nred; k++) {
-        rz += red[k];
-        zz += red[nred + k];
-    }
-    double eps = sqrt(zz / n);
-    double it = scalars[SC_ITER] + 1.0;
-    scalars[SC_BETA] = rz / scalars[SC_RZ];
-    scalars[SC_RZ] = rz;
-    scalars[SC_EPS] = eps;
-    scalars[SC_ITER] = it;
-    // Once DONE is set the remaining kernels of this iteration (the p update) return early;
-    // p is not needed after the last iteration.
-    if (it >= scalars[SC_MAXIT] || !(eps > scalars[SC_TARGET])) scalars[SC_DONE] = 1.0;
+    double eps = sqrt(zz / sg.z);
+    double it = sc[SC_ITER] + 1.0;
+    sc[SC_BETA] = rz / sc[SC_RZ];
+    sc[SC_RZ] = rz;
+    sc[SC_EPS] = eps;
+    sc[SC_ITER] = it;
+    // Once DONE is set the remaining kernels of this iteration (the p update) skip the system;
+    // p is not needed after its last iteration.
+    if (it >= sc[SC_MAXIT] || !(eps > sc[SC_TARGET])) mark_done(sc, done);
 }
 
 // p = z + beta p
-__global__ __launch_bounds__(256) void k_pcg_direction(int n, int npad, const double4* __restrict__ z,
-                                                       double4* __restrict__ p, float4* __restrict__ p32,
-                                                       const double* __restrict__ scalars) {
-    if (solver_done(scalars)) return;
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double b = scalars[SC_BETA];
+__global__ __launch_bounds__(64) void k_pcg_direction(const int* __restrict__ tseg, const double4* __restrict__ z,
+                                                      double4* __restrict__ p, float4* __restrict__ p32,
+                                                      const double* __restrict__ scalars) {
+    const double* sc = scalars + SC_COUNT * tseg[blockIdx.x];
+    if (solver_done(sc)) return;
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double b = sc[SC_BETA];
     double4 zv = z[i], pv = p[i];
     double4 np = make_double4(zv.x + b * pv.x, zv.y + b * pv.y, zv.z + b * pv.z, 0.0);
     p[i] = np;
@@ -976,11 +981,11 @@ __global__ __launch_bounds__(256) void k_pcg_direction(int n, int npad, const do
 
 // Extrapolated polarization (convergeInducedDipolesByExtrapolation, ReferenceTholeDipoleForce.cpp:876-887):
 // mu = alpha*(E0 + T mu); hist[o] = mu.
-__global__ void k_extrap_step(int n, int npad, const double4* __restrict__ part,
-                              const double4* __restrict__ E0, const double* __restrict__ alpha,
-                              double4* __restrict__ mu, float4* __restrict__ mu32, double4* __restrict__ hist_o) {
+__global__ void k_extrap_step(int npad, const double4* __restrict__ part, const double4* __restrict__ E0,
+                              const double* __restrict__ alpha, double4* __restrict__ mu, float4* __restrict__ mu32,
+                              double4* __restrict__ hist_o) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= npad) return;
     double4 f = part[i];
     double4 e0 = E0[i];
     double a = alpha[i];
@@ -991,10 +996,10 @@ __global__ void k_extrap_step(int n, int npad, const double4* __restrict__ part,
 }
 
 // mu = sum_o c_o hist[o]  (:890-895)
-__global__ void k_extrap_combine(int n, int npad, int order, const double4* __restrict__ hist,
-                                 const double* __restrict__ coef, double4* __restrict__ mu, float4* __restrict__ mu32) {
+__global__ void k_extrap_combine(int npad, int order, const double4* __restrict__ hist, const double* __restrict__ coef,
+                                 double4* __restrict__ mu, float4* __restrict__ mu32) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= npad) return;
     double x = 0, y = 0, z = 0;
     for (int o = 0; o < order; o++) {
         double4 h = hist[(size_t)o * npad + i];
@@ -1006,42 +1011,33 @@ __global__ void k_extrap_combine(int n, int npad, int order, const double4* __re
 }
 
 // final error of the extrapolated dipoles (:897-906): partial sums of |alpha(E0+T mu) - mu|^2
-__global__ __launch_bounds__(256) void k_extrap_error(int n, int npad, const double4* __restrict__ part,
-                                                      const double4* __restrict__ E0, const double* __restrict__ alpha,
-                                                      const double4* __restrict__ mu, double* __restrict__ red,
-                                                      int red_stride) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    double d2 = 0.0;
-    if (i < n) {
-        double4 f = part[i];
-        double4 e0 = E0[i], m = mu[i];
-        double a = alpha[i];
-        double dx = (e0.x + f.x) * a - m.x, dy = (e0.y + f.y) * a - m.y, dz = (e0.z + f.z) * a - m.z;
-        d2 = dx * dx + dy * dy + dz * dz;
-    }
-    block_sum2(d2, 0.0, red, red_stride);
+__global__ __launch_bounds__(64) void k_extrap_error(const double4* __restrict__ part, const double4* __restrict__ E0,
+                                                     const double* __restrict__ alpha, const double4* __restrict__ mu,
+                                                     double* __restrict__ red, int ntile) {
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double4 f = part[i];
+    double4 e0 = E0[i], m = mu[i];
+    double a = alpha[i];
+    double dx = (e0.x + f.x) * a - m.x, dy = (e0.y + f.y) * a - m.y, dz = (e0.z + f.z) * a - m.z;
+    tile_partials(dx * dx + dy * dy + dz * dz, 0.0, red, ntile);
 }
 
-__global__ void k_sum_eps(int n, int nred, const double* __restrict__ red, double* __restrict__ scalars) {
-    if (threadIdx.x != 0) return;
-    double s = 0.0;
-    for (int k = 0; k < nred; k++) s += red[k];
-    scalars[SC_EPS] = sqrt(s / n);
+__global__ __launch_bounds__(64) void k_sum_eps(const int4* __restrict__ seg, const double* __restrict__ red,
+                                                int ntile, double* __restrict__ scalars) {
+    const int4 sg = seg[blockIdx.x];
+    double sum, unused;
+    segment_sums(sg, red, ntile, sum, unused);
+    if (threadIdx.x == 0) scalars[SC_COUNT * blockIdx.x + SC_EPS] = sqrt(sum / sg.z);
 }
 
-// Forces/torques: sum split partials.  frc is [3n] doubles (accumulated later into the caller's array).
-__global__ void k_reduce_energy(int count, const double* __restrict__ part_e, double* __restrict__ out) {
-    __shared__ double s[256];
-    int t = threadIdx.x;
+// Energy of system s: the pair-kernel partials of its work items [w0, w1), fixed order.
+__global__ __launch_bounds__(64) void k_reduce_energy(const int2* __restrict__ segw, const double* __restrict__ part_e,
+                                                      double* __restrict__ out) {
+    const int2 w = segw[blockIdx.x];
     double a = 0.0;
-    for (int k = t; k < count; k += 256) a += part_e[k];
-    s[t] = a;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (t < w) s[t] += s[t + w];
-        __syncthreads();
-    }
-    if (t == 0) *out = s[0];
+    for (int k = w.x + (int)threadIdx.x; k < w.y; k += kTile) a += part_e[k];
+    a = wave_sum(a);
+    if (threadIdx.x == 0) out[blockIdx.x] = a;
 }
 
 // ------------------------------------------------------------------ torque -> force (a10)
@@ -1196,6 +1192,7 @@ __global__ void k_scatter_accumulate(int n, const int* __restrict__ perm, const 
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     int o = perm[k];
+    if (o < 0) return;
     dst[3 * o] += src[3 * k];
     dst[3 * o + 1] += src[3 * k + 1];
     dst[3 * o + 2] += src[3 * k + 2];
@@ -1310,15 +1307,15 @@ static BoxT64 mkbox(const double* b3) {
     return b;
 }
 
-void launch_load_positions(hipStream_t s, int n, int npad, const double* pos3, const int* perm, const double* thole,
+void launch_load_positions(hipStream_t s, int npad, const double* pos3, const int* perm, const double* thole,
                            double4* posa) {
-    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, pos3, perm, thole, posa);
+    hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, npad, pos3, perm, thole, posa);
 }
 
-void launch_tile_bounds(hipStream_t s, int n, int nblk, const double4* posa, const double* box3, bool pbc,
+void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h) {
-    hipLaunchKernelGGL(k_tile_bounds, dim3((nblk + 63) / 64), dim3(64), 0, s, n, nblk, posa, box3[0], box3[1], box3[2],
-                       pbc ? 1 : 0, tb_c, tb_h);
+    hipLaunchKernelGGL(k_tile_bounds, dim3((nblk + 63) / 64), dim3(64), 0, s, tcnt, nblk, posa, box3[0], box3[1],
+                       box3[2], pbc ? 1 : 0, tb_c, tb_h);
 }
 
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
@@ -1328,7 +1325,7 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
 
 void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj,
                        const double* scalars, double4* out4, double* out3) {
-    hipLaunchKernelGGL(k_col_reduce, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.n, L.row_start, Pi, Pj,
+    hipLaunchKernelGGL(k_col_reduce, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.tm, L.row_start, Pi, Pj,
                        L.slab, scalars, out4, out3);
 }
 
@@ -1336,10 +1333,10 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
                          const double* scalars, double4* Pi, double* Pj) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL(k_sym_matvec<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu, inva,
+        hipLaunchKernelGGL(k_sym_matvec<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu, inva,
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
     else
-        hipLaunchKernelGGL(k_sym_matvec<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu, inva,
+        hipLaunchKernelGGL(k_sym_matvec<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu, inva,
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
 }
 
@@ -1347,10 +1344,10 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
                              const float4* mu32, const double* inva, const double* scalars, double4* Pi, double* Pj) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
+        hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
                            mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
     else
-        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, mu,
+        hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
                            mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
 }
 
@@ -1358,10 +1355,10 @@ void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, c
                       double4* Pi, double* Pj, double4* out) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
-        hipLaunchKernelGGL(k_sym_field<true>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, inva,
+        hipLaunchKernelGGL(k_sym_field<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, inva,
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
     else
-        hipLaunchKernelGGL(k_sym_field<false>, dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, inva,
+        hipLaunchKernelGGL(k_sym_field<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, inva,
                            L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
     launch_col_reduce(s, L, Pi, Pj, nullptr, out, nullptr);
 }
@@ -1371,7 +1368,7 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const doub
                      double* PjT, double* part_e, double* frc, double4* torque) {
     BoxT64 b = mkbox(L.box);
 #define TDP_SELEC(P, M)                                                                                              \
-    hipLaunchKernelGGL((k_sym_elec<P, M>), dim3(L.nwork), dim3(kTile), 0, s, L.n, L.nblk, L.work, posa, qd, mu, inva, \
+    hipLaunchKernelGGL((k_sym_elec<P, M>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, mu, inva, \
                        orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, PiF, PjF, PiT, PjT, L.slab, part_e)
     if (L.pbc) {
         if (mutual) TDP_SELEC(true, true); else TDP_SELEC(true, false);
@@ -1383,61 +1380,58 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const doub
     launch_col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
 }
 
-void launch_field_init(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
-                       double4* E0, double4* mu, float4* mu32) {
-    hipLaunchKernelGGL(k_field_init, dim3((npad + 255) / 256), dim3(256), 0, s, n, npad, part, alpha, E0, mu,
+void launch_field_init(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha, double4* E0,
+                       double4* mu, float4* mu32) {
+    hipLaunchKernelGGL(k_field_init, dim3((G.npad + 255) / 256), dim3(256), 0, s, G.npad, part, alpha, E0, mu, mu32);
+}
+
+void launch_jacobi_update(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
+                          const double* alpha, double4* mu, float4* mu32, double* scalars, double* red) {
+    hipLaunchKernelGGL(k_jacobi_update, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, part, E0, alpha, mu, mu32,
+                       (const double*)scalars, red, G.ntile);
+    hipLaunchKernelGGL(k_jacobi_finalize, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile, scalars,
+                       G.done);
+}
+
+void launch_pcg_init(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0, const double* alpha,
+                     const double4* x, double4* r, double4* p, float4* p32, double* red, double* scalars) {
+    hipLaunchKernelGGL(k_pcg_init, dim3(G.ntile), dim3(kTile), 0, s, part, E0, alpha, x, r, p, p32, red, G.ntile);
+    hipLaunchKernelGGL(k_pcg_init_finalize, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile,
+                       scalars, G.done);
+}
+
+void launch_pcg_iteration_tail(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha,
+                               double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
+                               double* scalars) {
+    const double* sc = scalars;
+    hipLaunchKernelGGL(k_pcg_q, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, part, alpha, p, q, sc, red, G.ntile);
+    hipLaunchKernelGGL(k_pcg_gamma, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile, scalars);
+    hipLaunchKernelGGL(k_pcg_update, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, alpha, p, q, x, r, z, sc, red, G.ntile);
+    hipLaunchKernelGGL(k_pcg_beta, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile, scalars,
+                       G.done);
+    hipLaunchKernelGGL(k_pcg_direction, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, z, p, p32, sc);
+}
+
+void launch_extrap_step(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
+                        const double* alpha, double4* mu, float4* mu32, double4* hist_o) {
+    hipLaunchKernelGGL(k_extrap_step, dim3((G.npad + 255) / 256), dim3(256), 0, s, G.npad, part, E0, alpha, mu, mu32,
+                       hist_o);
+}
+
+void launch_extrap_combine(hipStream_t s, const SegLaunch& G, int order, const double4* hist, const double* coef,
+                           double4* mu, float4* mu32) {
+    hipLaunchKernelGGL(k_extrap_combine, dim3((G.npad + 255) / 256), dim3(256), 0, s, G.npad, order, hist, coef, mu,
                        mu32);
 }
 
-int red_blocks(int npad) { return (npad + 255) / 256; }
-
-void launch_jacobi_update(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
-                          const double* alpha, double4* mu, float4* mu32, const double* scalars, double* red) {
-    int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_jacobi_update, dim3(nb), dim3(256), 0, s, n, npad, part, E0, alpha, mu, mu32, scalars,
-                       red, nb);
-    hipLaunchKernelGGL(k_jacobi_finalize, dim3(1), dim3(64), 0, s, n, nb, red, (double*)scalars);
-}
-
-void launch_pcg_init(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
-                     const double* alpha, const double4* x, double4* r, double4* p, float4* p32, double* red,
-                     double* scalars) {
-    int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_pcg_init, dim3(nb), dim3(256), 0, s, n, npad, part, E0, alpha, x, r, p, p32, red, nb);
-    hipLaunchKernelGGL(k_pcg_init_finalize, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
-}
-
-void launch_pcg_iteration_tail(hipStream_t s, int n, int npad, const double4* part, const double* alpha,
-                               double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
-                               double* scalars) {
-    int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_pcg_q, dim3(nb), dim3(256), 0, s, n, npad, part, alpha, p, q, scalars, red, nb);
-    hipLaunchKernelGGL(k_pcg_gamma, dim3(1), dim3(64), 0, s, nb, red, scalars);
-    hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(256), 0, s, n, alpha, p, q, x, r, z, scalars, red, nb);
-    hipLaunchKernelGGL(k_pcg_beta, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
-    hipLaunchKernelGGL(k_pcg_direction, dim3(nb), dim3(256), 0, s, n, npad, z, p, p32, scalars);
-}
-
-void launch_extrap_step(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
-                        const double* alpha, double4* mu, float4* mu32, double4* hist_o) {
-    hipLaunchKernelGGL(k_extrap_step, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, part, E0, alpha, mu,
-                       mu32, hist_o);
-}
-
-void launch_extrap_combine(hipStream_t s, int n, int npad, int order, const double4* hist, const double* coef,
-                           double4* mu, float4* mu32) {
-    hipLaunchKernelGGL(k_extrap_combine, dim3((n + 255) / 256), dim3(256), 0, s, n, npad, order, hist, coef, mu, mu32);
-}
-
-void launch_extrap_error(hipStream_t s, int n, int npad, const double4* part, const double4* E0,
+void launch_extrap_error(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
                          const double* alpha, const double4* mu, double* red, double* scalars) {
-    int nb = red_blocks(npad);
-    hipLaunchKernelGGL(k_extrap_error, dim3(nb), dim3(256), 0, s, n, npad, part, E0, alpha, mu, red, nb);
-    hipLaunchKernelGGL(k_sum_eps, dim3(1), dim3(64), 0, s, n, nb, red, scalars);
+    hipLaunchKernelGGL(k_extrap_error, dim3(G.ntile), dim3(kTile), 0, s, part, E0, alpha, mu, red, G.ntile);
+    hipLaunchKernelGGL(k_sum_eps, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile, scalars);
 }
 
-void launch_reduce_energy(hipStream_t s, int count, const double* part_e, double* out) {
-    hipLaunchKernelGGL(k_reduce_energy, dim3(1), dim3(256), 0, s, count, part_e, out);
+void launch_reduce_energy(hipStream_t s, const SegLaunch& G, const double* part_e, double* out) {
+    hipLaunchKernelGGL(k_reduce_energy, dim3(G.nseg), dim3(kTile), 0, s, G.segw, part_e, out);
 }
 
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,

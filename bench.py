@@ -46,6 +46,7 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
+    p.add_argument("--no-clusters", action="store_true", help="skip the batched cluster-solve measurement")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="after the run, stream this many MiB through a plain device copy (y = x * 1 in float32) "
@@ -122,6 +123,86 @@ def cpu_baseline(s, mu, iterations, budget_s):
                         "system)" % (budget_s / 3.0, t_field, t_matvec, t_elec, iterations)))
 
 
+def cluster_systems():
+    """512 non-periodic water clusters, 20-400 molecules (60-1,200 sites) drawn log-uniformly, liquid
+    density, Mutual induction to 1e-7 D (2.08e-9 e*nm)."""
+    from tholedipoleplugin_amd import systems
+    sizes = systems.cluster_sizes_config3()
+    out = []
+    for k, m in enumerate(sizes):
+        c = systems.water_cluster(int(m), seed=20261206 + 1 + k)
+        c.force.setMutualInducedTargetEpsilon(1e-7 * DEBYE)
+        c.force.setMutualInducedMaxIterations(1000)
+        out.append(c)
+    return out
+
+
+def cluster_bench(torch, steps, warmup):
+    """Cluster solves per second: the whole set in one batched evaluation (mixed and FP64) against
+    one FP64 context per cluster evaluated in turn.  A solve is a full evaluation of a cluster
+    (field, induced dipoles to 1e-7 D, energy, forces)."""
+    from tholedipoleplugin_amd import _lib
+    cl = cluster_systems()
+    sizes = [c.n for c in cl]
+    dev = torch.device("cuda:0")
+    pos_h = np.concatenate([c.positions for c in cl])
+    pos = torch.tensor(pos_h.reshape(-1), dtype=torch.float64, device=dev)
+    frc = torch.zeros(pos.numel(), dtype=torch.float64, device=dev)
+    en = torch.zeros(len(cl), dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    res = {}
+    for prec in ("mixed", "double"):
+        b = _lib.HipBatchHandle([c.force for c in cl], precision=prec, solver="pcg")
+        for _ in range(warmup):
+            b.execute_device(pos.data_ptr(), frc.data_ptr(), en.data_ptr(), stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.execute_device(pos.data_ptr(), frc.data_ptr(), en.data_ptr(), stream)
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / steps
+        iters = [b.system_stats(g)["iterations"] for g in range(len(cl))]
+        e, f = b.execute(pos_h)
+        mu = b.dipoles(pos_h, None, _lib.DIPOLES_INDUCED)
+        b.close()
+        res[prec] = dict(seconds=dt, iters=iters, energy=e, forces=f, mu=mu)
+        print("# clusters batched %s: %.2f ms per set of %d, PCG iterations %d..%d" % (
+            prec, dt * 1e3, len(cl), min(iters), max(iters)), file=sys.stderr, flush=True)
+    # one FP64 context per cluster, evaluated in turn on device buffers
+    handles = [_lib.HipContextHandle(c.force, np.zeros(9), precision="double", solver="pcg") for c in cl]
+    offs = np.concatenate([[0], np.cumsum(sizes)])
+    def loop():
+        for k, h in enumerate(handles):
+            h.execute_device(pos.data_ptr() + 24 * int(offs[k]), None, frc.data_ptr() + 24 * int(offs[k]),
+                             en.data_ptr() + 8 * k, stream)
+    loop()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loop()
+    torch.cuda.synchronize()
+    dt_loop = time.perf_counter() - t0
+    for h in handles:
+        h.close()
+    print("# clusters one FP64 context each: %.2f ms per set" % (dt_loop * 1e3), file=sys.stderr, flush=True)
+    m, d = res["mixed"], res["double"]
+    ncl = len(cl)
+    return {
+        "workload": "512 water clusters, 20-400 molecules (60-1,200 sites, %d in all), NoCutoff all pairs, "
+                    "Mutual PCG to 1e-7 D" % int(offs[-1]),
+        "unit": "cluster solves/s",
+        "batched_mixed": ncl / m["seconds"],
+        "batched_fp64": ncl / d["seconds"],
+        "per_cluster_fp64_loop": ncl / dt_loop,
+        "batched_mixed_vs_loop": dt_loop / m["seconds"],
+        "batched_fp64_vs_loop": dt_loop / d["seconds"],
+        "mixed_vs_fp64_batched": d["seconds"] / m["seconds"],
+        "mixed_force_rel_rms_err_vs_fp64": float(np.sqrt(np.mean((m["forces"] - d["forces"]) ** 2)) /
+                                                 np.sqrt(np.mean(d["forces"] ** 2))),
+        "mixed_mu_max_err_vs_fp64_debye": float(np.max(np.abs(m["mu"] - d["mu"]))) / DEBYE,
+        "pcg_iterations_max": int(max(d["iters"])),
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -157,6 +238,9 @@ def main():
         err = float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
         mu_err = float(np.max(np.abs(mixed["mu"] - fp64["mu"]))) / DEBYE
         e_err = abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"])
+    clusters = None
+    if not args.no_clusters:
+        clusters = cluster_bench(torch, max(1, min(args.steps, 3)), 1)
     cpu = None
     if not args.no_cpu_baseline:
         from tholedipoleplugin_amd import _lib
@@ -198,6 +282,7 @@ def main():
                      "flops_per_pair": FLOPS_PER_PAIR_MATVEC, "pairs_per_launch": n * (n - 1) // 2,
                      "avg_launch_ms": avg_ms, "launches": st["matvec_launches"]},
         "cpu_baseline": cpu,
+        "clusters": clusters,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:

@@ -47,6 +47,8 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-clusters", action="store_true", help="skip the batched cluster-solve measurement")
+    p.add_argument("--clusters-only", action="store_true",
+                   help="run only the cluster-solve measurement and print its object (for profiling)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="after the run, stream this many MiB through a plain device copy (y = x * 1 in float32) "
@@ -213,6 +215,9 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.clusters_only:
+        print(json.dumps({"clusters": cluster_bench(torch, args.steps, args.warmup)}), flush=True)
+        return
     s = build_system()
     n = s.n
     print("# workload: %d sites, box %.2f nm; warmup %d, steps %d" % (n, s.box[0, 0], args.warmup, args.steps),

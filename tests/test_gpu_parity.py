@@ -262,6 +262,43 @@ def test_water12288_rows_full_size(precision):
     assert rel_rms(st.getForces()[rows], ref["force"]) <= F_TOL
 
 
+def test_saline96000_mixed_matches_double_and_rows():
+    """96,000 sites (31,978 waters + 33 Na+/Cl- pairs, 7.1 nm box, Thole a ~ U[0.3, 0.45]):
+    mixed and FP64 PCG solves (1e-10 e*nm) agree at every site to the bar, and the FP64 solution
+    satisfies mu = alpha (E0 + T mu) on sampled rows evaluated by the oracle over all partners."""
+    _say("case saline96000")
+    s = systems.config2()
+    s.force.setMutualInducedTargetEpsilon(1e-10)
+    from tholedipoleplugin_amd.context import System
+    out = {}
+    for precision in ("double", "mixed"):
+        sysm = System()
+        for m in s.masses:
+            sysm.addParticle(m)
+        sysm.setDefaultPeriodicBoxVectors(*s.box)
+        sysm.addForce(s.force)
+        ctx = Context(sysm, precision=precision, solver="pcg")
+        ctx.setPeriodicBoxVectors(*s.box)
+        ctx.setPositions(s.positions)
+        st = ctx.getState(getForces=True, getEnergy=True)
+        stats = ctx._kernel_for(s.force).solverStats()
+        _say("  %s stats %s" % (precision, stats))
+        assert stats["converged"] == 1
+        out[precision] = (st.getPotentialEnergy(), st.getForces(), s.force.getInducedDipoles(ctx))
+    (e64, f64, mu64), (em, fm, mum) = out["double"], out["mixed"]
+    _say("  max |mu_mixed - mu_double| = %.3e D" % (np.max(np.abs(mum - mu64)) / DEBYE))
+    assert np.max(np.abs(mum - mu64)) <= MU_TOL
+    assert rel_rms(fm, f64) <= F_TOL
+    assert abs(em - e64) <= E_TOL * abs(e64)
+    rows = np.random.default_rng(8).choice(s.n, 24, replace=False)
+    ref = oracle.rows(s.force, s.positions, mu64, rows, s.box, threads=8)
+    alpha = s.force.packed_arrays()["polarity"][rows][:, None]
+    resid = mu64[rows] - alpha * (ref["e0"] + ref["eind"])
+    _say("  max |mu - alpha (E0 + T mu)| on sampled rows: %.3e e*nm" % np.max(np.abs(resid)))
+    assert np.max(np.abs(resid)) <= MU_TOL
+    assert rel_rms(f64[rows], ref["force"]) <= F_TOL
+
+
 def half_box_system():
     """128 polarizable charges in a 4 nm periodic box: cluster A (64 sites on a 4x4x4 grid, 0.2 nm
     spacing, in the box octant nearest the origin, so that A and B fill one 64-site tile each

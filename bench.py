@@ -31,6 +31,7 @@ sys.path.insert(0, ROOT)
 # (2 x (6 for mu.Delta and its scaling + 12 for the three components) = 36 flops): 72 flops.
 FLOPS_PER_PAIR_MATVEC = 72.0
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
 # HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
@@ -49,6 +50,9 @@ def parse():
     p.add_argument("--no-clusters", action="store_true", help="skip the batched cluster-solve measurement")
     p.add_argument("--clusters-only", action="store_true",
                    help="run only the cluster-solve measurement and print its object (for profiling)")
+    p.add_argument("--no-dense", action="store_true", help="skip the dense cluster-response measurement")
+    p.add_argument("--dense-only", action="store_true",
+                   help="run only the dense cluster-response measurement and print its object (for profiling)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="after the run, stream this many MiB through a plain device copy (y = x * 1 in float32) "
@@ -205,6 +209,57 @@ def cluster_bench(torch, steps, warmup):
     }
 
 
+def dense_systems():
+    """256 non-periodic water clusters of 17-333 molecules (3N = 153-2,997 unknowns) drawn uniformly."""
+    from tholedipoleplugin_amd import systems
+    sizes = systems.cluster_sizes_config4()
+    return [systems.water_cluster(int(m), seed=20261207 + 1 + k) for k, m in enumerate(sizes)]
+
+
+def dense_bench(steps):
+    """Linear responses (E0 and unit fields along x, y, z) of 256 clusters to a relative residual
+    of 1e-10: 16-bit Cholesky on MFMA + FP64 refinement (FP16 and BF16) against batched FP64 CG.
+    A solve is the four responses of one cluster; timing covers the whole call (positions upload,
+    permanent field, assembly, factorization, refinement, download)."""
+    from tholedipoleplugin_amd import _lib
+    cl = dense_systems()
+    pos = np.concatenate([c.positions for c in cl])
+    b = _lib.HipBatchHandle([c.force for c in cl], precision="double", solver="pcg")
+    res = {}
+    for method in ("fp16", "bf16", "cg"):
+        b.responses(pos, method=method, tolerance=1e-10, max_steps=1000)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            X, st = b.responses(pos, method=method, tolerance=1e-10, max_steps=1000)
+        dt = (time.perf_counter() - t0) / steps
+        res[method] = dict(seconds=dt, X=X, stats=st)
+        print("# dense %s: %.2f ms per set of %d, %s" % (method, dt * 1e3, len(cl), st), file=sys.stderr, flush=True)
+    b.close()
+    unknowns = np.array([3 * c.n for c in cl], dtype=np.float64)
+    flops = float(np.sum(unknowns ** 3) / 3.0)
+    fac_ms = res["fp16"]["stats"]["factor_ms"]
+    cg = res["cg"]
+    out = {
+        "workload": "256 water clusters, 17-333 molecules (3N = 153-2,997 unknowns), 4 fields (E0, unit x, y, z), "
+                    "relative residual 1e-10",
+        "unit": "cluster solves/s",
+        "dense_fp16": len(cl) / res["fp16"]["seconds"],
+        "dense_bf16": len(cl) / res["bf16"]["seconds"],
+        "batched_fp64_cg": len(cl) / cg["seconds"],
+        "dense_fp16_vs_cg": cg["seconds"] / res["fp16"]["seconds"],
+        "refinement_steps_fp16": res["fp16"]["stats"]["steps"],
+        "refinement_steps_bf16": res["bf16"]["stats"]["steps"],
+        "cg_iterations_max": cg["stats"]["steps"],
+        "fp16_max_err_vs_cg_debye": float(np.max(np.abs(res["fp16"]["X"] - cg["X"]))) / DEBYE,
+        "bf16_max_err_vs_cg_debye": float(np.max(np.abs(res["bf16"]["X"] - cg["X"]))) / DEBYE,
+        "factor_roofline": {"bound": "mfma", "achieved": flops / (fac_ms * 1e-3) / 1e12,
+                            "peak": PEAK_FP16_TFLOPS, "unit": "TFLOP/s",
+                            "frac": flops / (fac_ms * 1e-3) / 1e12 / PEAK_FP16_TFLOPS,
+                            "flops": flops, "factor_ms": fac_ms, "assemble_ms": res["fp16"]["stats"]["assemble_ms"]},
+    }
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -217,6 +272,9 @@ def main():
         sys.exit(1)
     if args.clusters_only:
         print(json.dumps({"clusters": cluster_bench(torch, args.steps, args.warmup)}), flush=True)
+        return
+    if args.dense_only:
+        print(json.dumps({"dense_clusters": dense_bench(args.steps)}), flush=True)
         return
     s = build_system()
     n = s.n
@@ -246,6 +304,9 @@ def main():
     clusters = None
     if not args.no_clusters:
         clusters = cluster_bench(torch, max(1, min(args.steps, 3)), 1)
+    dense = None
+    if not args.no_dense:
+        dense = dense_bench(max(1, min(args.steps, 2)))
     cpu = None
     if not args.no_cpu_baseline:
         from tholedipoleplugin_amd import _lib
@@ -288,6 +349,7 @@ def main():
                      "avg_launch_ms": avg_ms, "launches": st["matvec_launches"]},
         "cpu_baseline": cpu,
         "clusters": clusters,
+        "dense_clusters": dense,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:

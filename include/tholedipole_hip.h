@@ -167,6 +167,26 @@ int tdp_batch_execute_device(tdp_context* ctx, const double* d_positions, double
 /* solver diagnostics of one system of the last evaluation */
 int tdp_batch_get_solver_stats(const tdp_context* ctx, int32_t system, tdp_solver_stats* stats);
 
+/* Linear responses of every system of a batch (no counterpart in the reference; it is the dense
+ * direct solve of small systems): the induced dipoles for four fields, the permanent field E0
+ * of calculateFixedDipoleField and unit uniform fields along x, y and z, i.e. the solutions of
+ * (alpha^-1 - T) mu = b with the Mutual operator of calculateInducedDipolePairIxn.
+ * method 0: FP16 Cholesky of D^1/2 (alpha^-1 - T) D^1/2 on MFMA (FP32 accumulation) refined in FP64
+ *           until |D^1/2 (b - A mu)| <= tolerance |D^1/2 b| for every system and field;
+ * method 1: the same with a BF16 factor;
+ * method 2: Jacobi-preconditioned FP64 CG per field, stopped at RMS(alpha r) <= tolerance RMS(alpha b).
+ * Systems of up to 1,024 particles (methods 0 and 1).  max_steps bounds refinement steps or CG
+ * iterations.  out: host [4][3N] (field, particle, component), caller's order. */
+typedef struct {
+    int32_t steps;                  /* refinement steps (0, 1) or the largest CG iteration count (2) */
+    double max_relative_residual;   /* largest final relative residual over systems and fields */
+    int32_t factor_failed;          /* nonzero: a pivot of the 16-bit factorization was not positive */
+    double assemble_ms;             /* GPU time of the 16-bit assembly (methods 0, 1; HIP events) */
+    double factor_ms;               /* GPU time of the factorization (methods 0, 1; HIP events) */
+} tdp_dense_stats;
+int tdp_batch_solve_responses(tdp_context* ctx, const double* positions, int32_t method, double tolerance,
+                              int32_t max_steps, double* out, tdp_dense_stats* stats);
+
 #ifdef __cplusplus
 }
 #endif

@@ -184,3 +184,60 @@ def test_batch_rejects_bad_input():
     f2.setPolarizationType(T.Direct)
     with pytest.raises(_lib.TdpError, match="polarization"):
         _lib.HipBatchHandle([f, f2])
+
+
+def _dense_members(sizes, seed0):
+    members = []
+    for k, m in enumerate(sizes):
+        c = systems.water_cluster(m, seed=seed0 + k)
+        members.append((c.force, np.asarray(c.positions, np.float64)))
+    return members
+
+
+@pytest.mark.parametrize("method", ["fp16", "bf16"])
+def test_dense_responses_match_cg_and_mutual(method):
+    """Dense 16-bit Cholesky + FP64 refinement: the E0 response equals the Mutual induced dipoles of
+    the batch PCG, and all four responses equal FP64 CG solves of the same fields."""
+    _say("case dense %s" % method)
+    members = _dense_members([5, 17, 40, 120, 333], 700)
+    for f, _ in members:
+        f.setMutualInducedTargetEpsilon(1e-12)
+        f.setMutualInducedMaxIterations(1000)
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    X, st = b.responses(pos, method=method, tolerance=1e-10)
+    _say("  dense stats %s" % st)
+    assert st["factor_failed"] == 0 and st["max_relative_residual"] <= 1e-10
+    Y, stc = b.responses(pos, method="cg", tolerance=1e-12, max_steps=1000)
+    _say("  cg stats %s" % stc)
+    b.execute(pos)
+    mu = b.dipoles(pos, None, _lib.DIPOLES_INDUCED)
+    b.close()
+    assert np.max(np.abs(X[0] - mu)) <= MU_TOL
+    for q in range(4):
+        assert np.max(np.abs(X[q] - Y[q])) <= MU_TOL, q
+
+
+def test_dense_responses_residual_by_oracle():
+    """The dense responses satisfy (alpha^-1 - T) mu = b with T applied by the CPU oracle
+    (rows of the induced field over all partners), for E0 and the three unit fields."""
+    _say("case dense residual via oracle")
+    members = _dense_members([9, 31], 800)
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    X, st = b.responses(pos, method="fp16", tolerance=1e-10)
+    b.close()
+    off = 0
+    for g, (f, p) in enumerate(members):
+        n = f.getNumParticles()
+        alpha = f.packed_arrays()["polarity"][:, None]
+        rows = np.arange(n)
+        fields = [None, np.array([1.0, 0, 0]), np.array([0, 1.0, 0]), np.array([0, 0, 1.0])]
+        for q in range(4):
+            x = X[q][off:off + n]
+            ref = oracle.rows(f, p, x, rows, None)
+            bq = ref["e0"] if q == 0 else np.tile(fields[q], (n, 1))
+            r = np.sqrt(alpha) * (bq - x / alpha + ref["eind"])
+            rel = np.linalg.norm(r) / np.linalg.norm(np.sqrt(alpha) * bq)
+            assert rel <= 2e-10, (g, q, rel)
+        off += n

@@ -29,7 +29,7 @@ EXPORTED_SYMBOLS = [
     "tdp_execute", "tdp_execute_device", "tdp_get_dipoles", "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
     "tdp_batch_create", "tdp_batch_num_systems", "tdp_batch_execute", "tdp_batch_execute_device",
-    "tdp_batch_get_solver_stats",
+    "tdp_batch_get_solver_stats", "tdp_batch_solve_responses",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -48,6 +48,11 @@ class ForceDesc(C.Structure):
 
 class Options(C.Structure):
     _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("device", C.c_int32)]
+
+
+class DenseStats(C.Structure):
+    _fields_ = [("steps", C.c_int32), ("max_relative_residual", C.c_double), ("factor_failed", C.c_int32),
+                ("assemble_ms", C.c_double), ("factor_ms", C.c_double)]
 
 
 class SolverStats(C.Structure):
@@ -95,6 +100,7 @@ def load():
     L.tdp_batch_execute.argtypes = [ctx_p, _dp, _dp, _dp]
     L.tdp_batch_execute_device.argtypes = [ctx_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
     L.tdp_batch_get_solver_stats.argtypes = [ctx_p, C.c_int32, C.POINTER(SolverStats)]
+    L.tdp_batch_solve_responses.argtypes = [ctx_p, _dp, C.c_int32, C.c_double, C.c_int32, _dp, C.POINTER(DenseStats)]
     for name in EXPORTED_SYMBOLS:
         if name not in ("tdp_destroy", "tdp_last_error", "tdp_last_create_error"):
             getattr(L, name).restype = C.c_int
@@ -267,3 +273,16 @@ class HipBatchHandle(HipContextHandle):
         st = SolverStats()
         self._check(self.L.tdp_batch_get_solver_stats(self.h, int(system), C.byref(st)))
         return dict(iterations=st.iterations, epsilon=st.epsilon, converged=st.converged)
+
+    RESPONSE_METHODS = {"fp16": 0, "bf16": 1, "cg": 2}
+
+    def responses(self, positions, method="fp16", tolerance=1e-10, max_steps=50):
+        """Induced dipoles of every system for E0 and unit fields along x, y, z: array [4, N, 3]."""
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        out = np.zeros(4 * pos.size)
+        st = DenseStats()
+        self._check(self.L.tdp_batch_solve_responses(self.h, _ptr(pos), self.RESPONSE_METHODS[method],
+                                                     float(tolerance), int(max_steps), _ptr(out), C.byref(st)))
+        return out.reshape(4, -1, 3), dict(steps=st.steps, max_relative_residual=st.max_relative_residual,
+                                           factor_failed=st.factor_failed, assemble_ms=st.assemble_ms,
+                                           factor_ms=st.factor_ms)

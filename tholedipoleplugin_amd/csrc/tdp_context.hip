@@ -88,6 +88,15 @@ struct tdp_context {
     int hist_order = 0;
     size_t out_bytes = 0;
     int *d_tcnt = nullptr, *d_tfirst = nullptr, *d_tseg = nullptr, *d_done = nullptr;
+    // ---- dense path (allocated on first use)
+    bool dense_ready = false;
+    int dense_max_nt = 0;
+    DenseSys* d_dsys = nullptr;
+    unsigned short* d_dmat = nullptr;
+    float* d_dinv = nullptr;
+    double *d_db = nullptr, *d_dx = nullptr, *d_dres = nullptr, *d_dy = nullptr, *d_drel = nullptr;
+    double4* d_dmu = nullptr;
+    int *d_dactive = nullptr, *d_dstatus = nullptr;
     size_t* d_tcol = nullptr;
     int4* d_seg = nullptr;
     int2* d_segw = nullptr;
@@ -585,16 +594,11 @@ int read_done(tdp_context* c, int& done) {
     return TDP_OK;
 }
 
-// setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764) from
-// positions already in d_pos3 (caller's order).  host_pos3 (may be NULL) are the same positions
-// on the host, used when the internal order is recomputed.
-int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) {
-    if (c->n == 0) {
-        std::fill(c->iterations.begin(), c->iterations.end(), 0);
-        std::fill(c->converged.begin(), c->converged.end(), 1);
-        std::fill(c->epsilon.begin(), c->epsilon.end(), c->polarization == TDP_DIRECT ? 1.0e50 : 0.0);
-        return TDP_OK;
-    }
+// setup() and the permanent field (ReferenceTholeDipoleForce.cpp:1117-1155, :735-746) from
+// positions already in d_pos3 (caller's order): internal order, lab-frame dipoles, E0 and
+// mu = alpha E0.  host_pos3 (may be NULL) are the same positions on the host, used when the
+// internal order is recomputed.
+int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
     hipStream_t s = c->stream;
     const int n = c->n, npad = c->npad;
     if (!c->ordered || c->evals_since_order >= kReorderInterval) {
@@ -622,41 +626,19 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
                          c->Emv);
     }
     launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
+    return TDP_OK;
+}
 
-    if (c->polarization == TDP_DIRECT) {
-        std::fill(c->iterations.begin(), c->iterations.end(), 0);
-        std::fill(c->converged.begin(), c->converged.end(), 1);
-        std::fill(c->epsilon.begin(), c->epsilon.end(), 1.0e50);  // the reference leaves its initial value (:24, :749-753)
-        return TDP_OK;
-    }
-    if (c->polarization == TDP_EXTRAPOLATED) {
-        // convergeInducedDipolesByExtrapolation (:853-909)
-        static const double opt4[4] = {-0.154, 0.017, 0.658, 0.474};
-        std::vector<double> coef = c->extrap;
-        if (coef.empty()) coef.assign(opt4, opt4 + 4);
-        int order = (int)coef.size();
-        if (order > c->hist_order) return fail(c, TDP_ERR_INVALID_ARGUMENT, "extrapolation history not allocated");
-        TDP_HIP(c, hipMemcpyAsync(c->d_coef, coef.data(), order * sizeof(double), hipMemcpyHostToDevice, s));
-        TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
-        for (int o = 1; o < order; o++) {
-            matvec(c, L, c->mu, nullptr);
-            launch_extrap_step(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
-                               c->hist + (size_t)o * npad);
-        }
-        launch_extrap_combine(s, G, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
-        matvec(c, L, c->mu, nullptr);
-        launch_extrap_error(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
-        int rc = read_solver_results(c, false);
-        if (rc) return rc;
-        std::fill(c->converged.begin(), c->converged.end(), 1);
-        std::fill(c->iterations.begin(), c->iterations.end(), order);
-        if (c->timing) harvest_timing(c);
-        return TDP_OK;
-    }
-    // Mutual: every system iterates until its own stop flag is set; a launch skips the tiles and
-    // work items of stopped systems.  The host checks the count of stopped systems every chunk.
+// Mutual solve of every system from the current E0 and mu (ReferenceTholeDipoleForce.cpp:820-851).
+int solve_mutual(tdp_context* c, const double* box) {
+    hipStream_t s = c->stream;
+    PairLaunch L = pair_launch(c, box);
+    SegLaunch G = seg_launch(c);
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    // every system iterates until its own stop flag is set; a launch skips the tiles and work
+    // items of stopped systems.  The host checks the count of stopped systems every chunk.
     const bool jacobi = c->solver == TDP_SOLVER_JACOBI;
-    const int chunk = n > 20000 ? 4 : 16;  // iterations enqueued between host checks
+    const int chunk = c->n > 20000 ? 4 : 16;  // iterations enqueued between host checks
     int rc = start_solve(c, jacobi);
     if (rc) return rc;
     int done = 0;
@@ -688,6 +670,54 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
     if ((rc = read_solver_results(c, jacobi))) return rc;
     if (c->timing) harvest_timing(c);
     return TDP_OK;
+}
+
+// setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764).
+int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) {
+    if (c->n == 0) {
+        std::fill(c->iterations.begin(), c->iterations.end(), 0);
+        std::fill(c->converged.begin(), c->converged.end(), 1);
+        std::fill(c->epsilon.begin(), c->epsilon.end(), c->polarization == TDP_DIRECT ? 1.0e50 : 0.0);
+        return TDP_OK;
+    }
+    int rc = prepare_field(c, box, host_pos3);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    const int npad = c->npad;
+    PairLaunch L = pair_launch(c, box);
+    SegLaunch G = seg_launch(c);
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    if (c->polarization == TDP_DIRECT) {
+        std::fill(c->iterations.begin(), c->iterations.end(), 0);
+        std::fill(c->converged.begin(), c->converged.end(), 1);
+        std::fill(c->epsilon.begin(), c->epsilon.end(), 1.0e50);  // the reference leaves its initial value (:24, :749-753)
+        return TDP_OK;
+    }
+    if (c->polarization == TDP_EXTRAPOLATED) {
+        // convergeInducedDipolesByExtrapolation (:853-909)
+        static const double opt4[4] = {-0.154, 0.017, 0.658, 0.474};
+        std::vector<double> coef = c->extrap;
+        if (coef.empty()) coef.assign(opt4, opt4 + 4);
+        int order = (int)coef.size();
+        if (order > c->hist_order) return fail(c, TDP_ERR_INVALID_ARGUMENT, "extrapolation history not allocated");
+        TDP_HIP(c, hipMemcpyAsync(c->d_coef, coef.data(), order * sizeof(double), hipMemcpyHostToDevice, s));
+        TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
+        for (int o = 1; o < order; o++) {
+            matvec(c, L, c->mu, nullptr);
+            launch_extrap_step(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr,
+                               c->hist + (size_t)o * npad);
+        }
+        launch_extrap_combine(s, G, order, c->hist, c->d_coef, c->mu, mixed ? c->vec32 : nullptr);
+        matvec(c, L, c->mu, nullptr);
+        launch_extrap_error(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->d_red, c->d_scalars);
+        int rc = read_solver_results(c, false);
+        if (rc) return rc;
+        std::fill(c->converged.begin(), c->converged.end(), 1);
+        std::fill(c->iterations.begin(), c->iterations.end(), order);
+        if (c->timing) harvest_timing(c);
+        return TDP_OK;
+    }
+    return solve_mutual(c, box);
 }
 
 // calculateElectrostatic + mapTorqueToForce (ReferenceTholeDipoleForce.cpp:118-124) into d_frc
@@ -925,6 +955,199 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     }
     if ((rc = upload_tables(c)) || (rc = upload_particles(c))) return bail(rc);
     *out = c;
+    return TDP_OK;
+}
+
+// Dense-path buffers of a batch context: the 16-bit matrices, diagonal-block inverses and the
+// refinement vectors of every system.
+int dense_alloc(tdp_context* c) {
+    if (c->dense_ready) return TDP_OK;
+    const int ns = c->nseg;
+    std::vector<DenseSys> sys(ns);
+    size_t moff = 0, voff = 0, doff = 0;
+    int maxnt = 0;
+    for (int g = 0; g < ns; g++) {
+        const int nt = c->seg_t1[g] - c->seg_t0[g];
+        if (nt > dense_max_tiles())
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "dense solve: system " + std::to_string(g) + " has more than " +
+                                                         std::to_string(64 * dense_max_tiles()) + " particles");
+        const size_t m = (size_t)192 * nt;
+        sys[g].nt = nt;
+        sys[g].t0 = c->seg_t0[g];
+        sys[g].moff = moff;
+        sys[g].voff = voff;
+        sys[g].doff = doff;
+        moff += m * m;
+        voff += 4 * m;
+        doff += (size_t)3 * nt * 64 * 64;
+        maxnt = std::max(maxnt, nt);
+    }
+    int rc;
+    if ((rc = dalloc(c, &c->d_dsys, ns)) || (rc = dalloc(c, &c->d_dmat, moff)) || (rc = dalloc(c, &c->d_dinv, doff)) ||
+        (rc = dalloc(c, &c->d_db, voff)) || (rc = dalloc(c, &c->d_dx, voff)) || (rc = dalloc(c, &c->d_dres, voff)) ||
+        (rc = dalloc(c, &c->d_dy, voff)) || (rc = dalloc(c, &c->d_drel, 4 * (size_t)ns)) ||
+        (rc = dalloc(c, &c->d_dmu, 4 * (size_t)c->npad)) || (rc = dalloc(c, &c->d_dactive, ns)) ||
+        (rc = dalloc(c, &c->d_dstatus, 1)))
+        return rc;
+    TDP_HIP(c, hipMemcpyAsync(c->d_dsys, sys.data(), ns * sizeof(DenseSys), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    c->dense_max_nt = maxnt;
+    c->dense_ready = true;
+    return TDP_OK;
+}
+
+// Induced dipoles of every system for four fields: E0 (the permanent field) and unit uniform
+// fields along x, y and z.  Low-precision Cholesky of A^ = I - D^1/2 T D^1/2 on MFMA, then FP64
+// iterative refinement with matrix-free FP64 residuals until every system's relative residual
+// |D^1/2 (b - A x)| / |D^1/2 b| is at most tol.  out: [4][3n] in the caller's order.
+int dense_responses(tdp_context* c, bool bf16, double tol, int max_steps, double* out, tdp_dense_stats* st) {
+    hipStream_t s = c->stream;
+    int rc = dense_alloc(c);
+    if (rc) return rc;
+    PairLaunch L = pair_launch(c, nullptr);
+    SegLaunch G = seg_launch(c);
+    const int ns = c->nseg, npad = c->npad;
+    int zero = 0;
+    TDP_HIP(c, hipMemcpyAsync(c->d_dstatus, &zero, sizeof zero, hipMemcpyHostToDevice, s));
+    hipEvent_t ev[3];
+    for (auto& e : ev) e = take_event(c);
+    TDP_HIP(c, hipEventRecord(ev[0], s));
+    launch_dense_assemble(s, bf16, ns, c->dense_max_nt, c->d_dsys, c->d_tcnt, c->posa, c->d_alpha, c->d_inva,
+                          c->d_exc_start, c->d_exc_J, c->d_exc_mask, c->d_dmat);
+    TDP_HIP(c, hipEventRecord(ev[1], s));
+    launch_dense_factor(s, bf16, ns, c->dense_max_nt, c->d_dsys, c->d_dmat, c->d_dinv, c->d_dstatus);
+    TDP_HIP(c, hipEventRecord(ev[2], s));
+    launch_dense_rhs(s, G, c->d_dsys, c->d_tcnt, c->E0, c->d_alpha, c->d_db, c->d_dx, c->d_dres);
+    std::vector<int> active(ns, 1);
+    std::vector<double> rel(4 * (size_t)ns, 0.0);
+    TDP_HIP(c, hipMemcpyAsync(c->d_dactive, active.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
+    double* pj = reinterpret_cast<double*>(pi + (size_t)c->nwork * 64);
+    int steps = 0;
+    double worst = 0.0;
+    for (; steps < max_steps;) {
+        launch_dense_solve(s, bf16, ns, c->d_dsys, c->d_dmat, c->d_dinv, c->d_dres, c->d_dy, c->d_dactive);
+        launch_dense_update(s, G, c->d_dsys, c->d_alpha, c->d_dy, c->d_dx, c->d_dmu, (size_t)npad, c->d_dactive);
+        steps++;
+        for (int q = 0; q < 4; q++) {
+            launch_sym_matvec64(s, L, c->posa, c->d_dmu + (size_t)q * npad, c->d_inva, nullptr, pi, pj);
+            launch_col_reduce(s, L, pi, pj, nullptr, c->Emv, nullptr);
+            launch_dense_residual(s, G, q, c->d_dsys, c->d_alpha, c->Emv, c->d_db, c->d_dx, c->d_dres, c->d_red,
+                                  c->d_drel);
+        }
+        TDP_HIP(c, hipMemcpyAsync(rel.data(), c->d_drel, rel.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        TDP_HIP(c, hipStreamSynchronize(s));
+        worst = 0.0;
+        int left = 0;
+        for (int g = 0; g < ns; g++) {
+            double w = 0.0;
+            for (int q = 0; q < 4; q++) w = std::max(w, rel[4 * g + q]);
+            worst = std::max(worst, w);
+            active[g] = w > tol ? 1 : 0;
+            left += active[g];
+        }
+        if (!left) break;
+        TDP_HIP(c, hipMemcpyAsync(c->d_dactive, active.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    }
+    int status = 0;
+    std::vector<double4> mu(4 * (size_t)npad);
+    TDP_HIP(c, hipMemcpyAsync(&status, c->d_dstatus, sizeof status, hipMemcpyDeviceToHost, s));
+    TDP_HIP(c, hipMemcpyAsync(mu.data(), c->d_dmu, mu.size() * sizeof(double4), hipMemcpyDeviceToHost, s));
+    TDP_HIP(c, hipStreamSynchronize(s));
+    if (out)
+        for (int q = 0; q < 4; q++)
+            for (int k = 0; k < npad; k++) {
+                const int o = c->perm[k];
+                if (o < 0) continue;
+                const double4 v = mu[(size_t)q * npad + k];
+                double* dst = out + (size_t)q * 3 * c->n + 3 * (size_t)o;
+                dst[0] = v.x;
+                dst[1] = v.y;
+                dst[2] = v.z;
+            }
+    float t_asm = 0.f, t_fac = 0.f;
+    hipEventElapsedTime(&t_asm, ev[0], ev[1]);
+    hipEventElapsedTime(&t_fac, ev[1], ev[2]);
+    for (auto& e : ev) c->ev_pool.push_back(e);
+    if (st) {
+        st->steps = steps;
+        st->max_relative_residual = worst;
+        st->factor_failed = status;
+        st->assemble_ms = t_asm;
+        st->factor_ms = t_fac;
+    }
+    return TDP_OK;
+}
+
+// The same four responses by Jacobi-preconditioned FP64 CG of every system (the comparator of the
+// dense path): the Mutual solver run once per field, stopped when RMS(alpha r) <= tol RMS(alpha b)
+// for the system.
+int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dense_stats* st) {
+    hipStream_t s = c->stream;
+    const int ns = c->nseg, npad = c->npad;
+    SegLaunch G = seg_launch(c);
+    std::vector<double4> e0(npad);
+    TDP_HIP(c, hipMemcpyAsync(e0.data(), c->E0, npad * sizeof(double4), hipMemcpyDeviceToHost, s));
+    TDP_HIP(c, hipStreamSynchronize(s));
+    std::vector<double4> e0save = e0;
+    const std::vector<int> maxit0 = c->seg_maxit;
+    const std::vector<double> target0 = c->seg_target;
+    const int solver0 = c->solver, prec0 = c->precision;
+    c->solver = TDP_SOLVER_PCG;
+    c->precision = TDP_PRECISION_DOUBLE;
+    int total_iter = 0;
+    double worst = 0.0;
+    int rc = TDP_OK;
+    std::vector<double4> mu(npad);
+    for (int q = 0; q < 4 && rc == TDP_OK; q++) {
+        if (q > 0) launch_uniform_field(s, npad, c->d_perm, q - 1, c->E0);
+        else TDP_HIP(c, hipMemcpyAsync(c->E0, e0save.data(), npad * sizeof(double4), hipMemcpyHostToDevice, s));
+        // per system: target = tol * RMS(alpha b)
+        for (int g = 0; g < ns; g++) {
+            double sum = 0.0;
+            for (int k = 64 * c->seg_t0[g]; k < 64 * c->seg_t1[g]; k++) {
+                const int o = c->perm[k];
+                if (o < 0) continue;
+                const double a = c->polarity[o];
+                const double4 b = e0save[k];
+                const double bx = q == 0 ? b.x : (q == 1), by = q == 0 ? b.y : (q == 2), bz = q == 0 ? b.z : (q == 3);
+                sum += a * a * (bx * bx + by * by + bz * bz);
+            }
+            c->seg_target[g] = tol * std::sqrt(sum / c->seg_n[g]);
+            c->seg_maxit[g] = max_iter;
+        }
+        launch_field_init(s, G, c->E0, c->d_alpha, c->E0, c->mu, nullptr);
+        rc = solve_mutual(c, nullptr);
+        if (rc) break;
+        TDP_HIP(c, hipMemcpyAsync(mu.data(), c->mu, npad * sizeof(double4), hipMemcpyDeviceToHost, s));
+        TDP_HIP(c, hipStreamSynchronize(s));
+        for (int g = 0; g < ns; g++) {
+            total_iter = std::max(total_iter, c->iterations[g]);
+            worst = std::max(worst, c->epsilon[g] / std::max(c->seg_target[g] / tol, 1e-300));
+        }
+        if (out)
+            for (int k = 0; k < npad; k++) {
+                const int o = c->perm[k];
+                if (o < 0) continue;
+                double* dst = out + (size_t)q * 3 * c->n + 3 * (size_t)o;
+                dst[0] = mu[k].x;
+                dst[1] = mu[k].y;
+                dst[2] = mu[k].z;
+            }
+    }
+    c->seg_maxit = maxit0;
+    c->seg_target = target0;
+    c->solver = solver0;
+    c->precision = prec0;
+    if (rc) return rc;
+    TDP_HIP(c, hipMemcpyAsync(c->E0, e0save.data(), npad * sizeof(double4), hipMemcpyHostToDevice, s));
+    if (st) {
+        st->steps = total_iter;
+        st->max_relative_residual = worst;
+        st->factor_failed = 0;
+        st->assemble_ms = 0.0;
+        st->factor_ms = 0.0;
+    }
     return TDP_OK;
 }
 
@@ -1226,6 +1449,21 @@ int tdp_batch_get_solver_stats(const tdp_context* cc, int32_t system, tdp_solver
 }
 
 int32_t tdp_batch_num_systems(const tdp_context* c) { return c ? c->nseg : 0; }
+
+int tdp_batch_solve_responses(tdp_context* c, const double* positions, int32_t method, double tolerance,
+                              int32_t max_steps, double* out, tdp_dense_stats* stats) {
+    if (!c || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (c->method != TDP_NO_CUTOFF) return fail(c, TDP_ERR_INVALID_ARGUMENT, "responses: NoCutoff systems only");
+    if (method < 0 || method > 2) return fail(c, TDP_ERR_INVALID_ARGUMENT, "responses: unknown method");
+    if (!(tolerance > 0.0) || max_steps <= 0) return fail(c, TDP_ERR_INVALID_ARGUMENT, "responses: bad tolerance");
+    hipSetDevice(c->device);
+    int rc;
+    if ((rc = upload_positions(c, positions))) return rc;
+    if (c->n == 0) return TDP_OK;
+    if ((rc = prepare_field(c, nullptr, positions))) return rc;
+    if (method == 2) return cg_responses(c, tolerance, max_steps, out, stats);
+    return dense_responses(c, method == 1, tolerance, max_steps, out, stats);
+}
 
 int tdp_batch_execute(tdp_context* c, const double* positions, double* forces, double* energies) {
     if (!c || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");

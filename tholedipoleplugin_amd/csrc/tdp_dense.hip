@@ -1,0 +1,516 @@
+// tdp_dense.hip -- dense direct solve of the induced-dipole equations of small systems (gfx950).
+//
+// For each system of a batch context the scaled matrix  A^ = I - D^1/2 T D^1/2  (D = diag(alpha),
+// T the Thole-damped dipole field tensor of calculateInducedDipolePairIxn,
+// ReferenceTholeDipoleForce.cpp:766-819, with the same iScale table) is assembled in FP16 or BF16,
+// factored as L L^T by a blocked left-looking Cholesky whose tile products run on MFMA with FP32
+// accumulation, and used as the preconditioner of an FP64 iterative refinement whose residuals are
+// formed matrix-free by the FP64 T*mu kernels.
+//
+// Layout of one system with nt particle tiles: unknown r = c * 64 nt + s (component c of local
+// slot s), so block R = c nt + I of 64 unknowns is component c of particle tile I, and a 64x64
+// block of A^ couples two particle tiles in one component pair.  m = 192 nt.  The factor is kept
+// in place: block (i, j), i > j, holds L_ij and block (j, i) its transpose; the inverse of each
+// diagonal block L_jj is kept in FP32.  Padding slots have alpha = 0 and identity rows.
+#include <hip/hip_runtime.h>
+#include <hip/hip_bf16.h>
+
+#include "tdp_device.hpp"
+#include "tdp_kernels.h"
+
+namespace tdp {
+namespace {
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef short short4_t __attribute__((ext_vector_type(4)));
+typedef float float16_t __attribute__((ext_vector_type(16)));
+
+constexpr int kB = 64;   // block size of the factorization (one particle tile)
+
+// Storage type of the factor: FP16 or BF16 bit patterns in 16-bit words.
+template <bool BF16>
+__device__ __forceinline__ unsigned short to_lp(float x) {
+    if (BF16) {
+        // round to nearest even on the upper 16 bits
+        unsigned u = __float_as_uint(x);
+        unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
+        return (unsigned short)(r >> 16);
+    }
+    _Float16 h = (_Float16)x;
+    return __builtin_bit_cast(unsigned short, h);
+}
+template <bool BF16>
+__device__ __forceinline__ float from_lp(unsigned short b) {
+    if (BF16) return __uint_as_float(((unsigned)b) << 16);
+    return (float)__builtin_bit_cast(_Float16, b);
+}
+
+// 32x32x8 MFMA with FP32 accumulation.  Operand lane layout (both A and B): lane l holds k =
+// 4 (l / 32) + t, t = 0..3, of row (A) / column (B) l % 32; accumulator element v of lane l is row
+// 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
+template <bool BF16>
+__device__ __forceinline__ float16_t mfma32(const unsigned short* a, const unsigned short* b, float16_t c) {
+    if (BF16) {
+        short4_t av = {(short)a[0], (short)a[1], (short)a[2], (short)a[3]};
+        short4_t bv = {(short)b[0], (short)b[1], (short)b[2], (short)b[3]};
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, c, 0, 0, 0);
+    }
+    half4_t av = {__builtin_bit_cast(_Float16, a[0]), __builtin_bit_cast(_Float16, a[1]),
+                  __builtin_bit_cast(_Float16, a[2]), __builtin_bit_cast(_Float16, a[3])};
+    half4_t bv = {__builtin_bit_cast(_Float16, b[0]), __builtin_bit_cast(_Float16, b[1]),
+                  __builtin_bit_cast(_Float16, b[2]), __builtin_bit_cast(_Float16, b[3])};
+    return __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ size_t blk_off(const DenseSys& d, int R, int C) {
+    const size_t m = (size_t)192 * d.nt;
+    return d.moff + (size_t)R * kB * m + (size_t)C * kB;
+}
+
+// ------------------------------------------------------------------ assembly
+// One workgroup per (system, particle tile pair I, J): the 3x3 tensors of its 64 x 64 particle
+// pairs, written into the 9 component blocks (c I, c' J).  A^_(ci)(c'j) = delta - s_i s_j T^cc'_ij
+// with s = sqrt(alpha) and T^cc' = iScale (rr5 D_c D_c' - rr3 delta_cc') (calculateInducedDipolePairIxn).
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_dense_assemble(const DenseSys* __restrict__ sys, const int* __restrict__ tcnt,
+                                                        const double4* __restrict__ posa,
+                                                        const double* __restrict__ alpha,
+                                                        const double* __restrict__ inva,
+                                                        const int* __restrict__ exc_start,
+                                                        const int* __restrict__ exc_J,
+                                                        const unsigned long long* __restrict__ exc_mask,
+                                                        unsigned short* __restrict__ mat) {
+    const DenseSys d = sys[blockIdx.z];
+    const int I = blockIdx.x, J = blockIdx.y;
+    if (I >= d.nt || J >= d.nt) return;
+    __shared__ unsigned long long s_mi[kB];
+    __shared__ double4 s_pj[kB];
+    __shared__ double s_aj[kB], s_iaj[kB];
+    const int t = threadIdx.x;
+    const int TI = d.t0 + I, TJ = d.t0 + J;
+    const int cntI = tcnt[TI], cntJ = tcnt[TJ];
+    if (t < kB) {
+        unsigned long long m = 0ull;
+        for (int e = exc_start[TI]; e < exc_start[TI + 1]; e++)
+            if (exc_J[e] == TJ) m = exc_mask[((size_t)e * 3 + 2) * kTile + t];
+        s_mi[t] = m;
+        s_pj[t] = posa[TJ * kTile + t];
+        s_aj[t] = alpha[TJ * kTile + t];
+        s_iaj[t] = inva[TJ * kTile + t];
+    }
+    __syncthreads();
+    const int j = t & (kB - 1);
+    const size_t m = (size_t)192 * d.nt;
+    for (int i = t >> 6; i < kB; i += 4) {
+        const double4 pi = posa[TI * kTile + i];
+        const double ai = alpha[TI * kTile + i];
+        const double4 pj = s_pj[j];
+        const bool valid = i < cntI && j < cntJ && !((s_mi[i] >> j) & 1ull);
+        double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
+        double r2 = valid ? dx * dx + dy * dy + dz * dz : 1.0;
+        double rr3, rr5;
+        damped_rr(r2, pi.w, pj.w, inva[TI * kTile + i], s_iaj[j], rr3, rr5);
+        const double sc = valid ? sqrt(ai * s_aj[j]) : 0.0;
+        const double D[3] = {dx, dy, dz};
+        const bool diag = TI == TJ && i == j;
+        for (int ci = 0; ci < 3; ci++)
+            for (int cj = 0; cj < 3; cj++) {
+                double tv = rr5 * D[ci] * D[cj] - (ci == cj ? rr3 : 0.0);
+                double v = (diag && ci == cj ? 1.0 : 0.0) - sc * tv;
+                const size_t row = (size_t)(ci * d.nt + I) * kB + i, col = (size_t)(cj * d.nt + J) * kB + j;
+                mat[d.moff + row * m + col] = to_lp<BF16>((float)v);
+            }
+    }
+}
+
+// ------------------------------------------------------------------ factorization
+// S = A^_ij - sum_{k<j} L_ik L_jk^T for one 64x64 block, on MFMA: four waves, each a 32x32 quarter.
+// Operand tiles are staged in LDS as 64 x 64 16-bit words (row stride 68 to spread the banks).
+constexpr int kLd = kB + 4;
+
+template <bool BF16>
+__device__ __forceinline__ void block_update(const DenseSys& d, int i, int j, unsigned short* s_a, unsigned short* s_b,
+                                             const unsigned short* __restrict__ mat, float16_t& acc) {
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int wr = w >> 1, wc = w & 1;
+    const size_t m = (size_t)192 * d.nt;
+    for (int k = 0; k < j; k++) {
+        __syncthreads();
+        // stage L_ik and L_jk: 4096 words each, 16 per thread
+        const unsigned short* gi = mat + blk_off(d, i, k);
+        const unsigned short* gj = mat + blk_off(d, j, k);
+        for (int e = t; e < kB * kB / 4; e += 256) {
+            const int r = e >> 4, c4 = (e & 15) * 4;
+            const ushort4 va = *reinterpret_cast<const ushort4*>(gi + (size_t)r * m + c4);
+            const ushort4 vb = *reinterpret_cast<const ushort4*>(gj + (size_t)r * m + c4);
+            unsigned short* pa = s_a + r * kLd + c4;
+            unsigned short* pb = s_b + r * kLd + c4;
+            pa[0] = va.x; pa[1] = va.y; pa[2] = va.z; pa[3] = va.w;
+            pb[0] = vb.x; pb[1] = vb.y; pb[2] = vb.z; pb[3] = vb.w;
+        }
+        __syncthreads();
+        const unsigned short* ra = s_a + (wr * 32 + (l & 31)) * kLd + 4 * (l >> 5);
+        const unsigned short* rb = s_b + (wc * 32 + (l & 31)) * kLd + 4 * (l >> 5);
+#pragma unroll
+        for (int kk = 0; kk < kB; kk += 8) acc = mfma32<BF16>(ra + kk, rb + kk, acc);
+    }
+}
+
+// Diagonal block j of every system with more than j blocks: S = A^_jj - sum L_jk L_jk^T, its
+// Cholesky factor in FP32 (column by column in LDS) and the inverse of that factor (kept in FP32
+// for the off-diagonal blocks and the triangular solves).
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_chol_diag(int j, const DenseSys* __restrict__ sys,
+                                                   const unsigned short* __restrict__ mat, float* __restrict__ dinv,
+                                                   int* __restrict__ status) {
+    const DenseSys d = sys[blockIdx.x];
+    const int nb = 3 * d.nt;
+    if (j >= nb) return;
+    __shared__ unsigned short s_a[kB * kLd], s_b[kB * kLd];
+    __shared__ float s_s[kB][kB + 1];
+    __shared__ float s_x[kB][kB + 1];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int wr = w >> 1, wc = w & 1;
+    float16_t acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    block_update<BF16>(d, j, j, s_a, s_b, mat, acc);
+    const size_t m = (size_t)192 * d.nt;
+    const unsigned short* g = mat + blk_off(d, j, j);
+    for (int v = 0; v < 16; v++) {
+        const int r = wr * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), c = wc * 32 + (l & 31);
+        s_s[r][c] = from_lp<BF16>(g[(size_t)r * m + c]) - acc[v];
+    }
+    __syncthreads();
+    // right-looking Cholesky of the 64x64 block, lower triangle
+    for (int p = 0; p < kB; p++) {
+        if (t == 0) {
+            float dv = s_s[p][p];
+            if (!(dv > 0.f)) {
+                atomicOr(status, 1);
+                dv = 1.f;
+            }
+            s_s[p][p] = sqrtf(dv);
+        }
+        __syncthreads();
+        const float piv = s_s[p][p];
+        for (int r = p + 1 + t; r < kB; r += 256) s_s[r][p] /= piv;
+        __syncthreads();
+        for (int e = t; e < kB * kB; e += 256) {
+            const int r = e >> 6, c = e & 63;
+            if (c > p && r >= c) s_s[r][c] -= s_s[r][p] * s_s[c][p];
+        }
+        __syncthreads();
+    }
+    // inverse of the lower factor: column c solved by thread c (forward substitution)
+    if (t < kB) {
+        const int c = t;
+        for (int r = 0; r < kB; r++) {
+            float v = r == c ? 1.f : 0.f;
+            for (int k = c; k < r; k++) v -= s_s[r][k] * s_x[k][c];
+            s_x[r][c] = r < c ? 0.f : v / s_s[r][r];
+        }
+    }
+    __syncthreads();
+    float* out = dinv + d.doff + (size_t)j * kB * kB;
+    for (int e = t; e < kB * kB; e += 256) out[e] = s_x[e >> 6][e & 63];
+}
+
+// Off-diagonal blocks (i, j), i > j: L_ij = (A^_ij - sum_{k<j} L_ik L_jk^T) L_jj^-T, stored in the
+// 16-bit format at (i, j) and transposed at (j, i).
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restrict__ sys,
+                                                  unsigned short* __restrict__ mat, const float* __restrict__ dinv) {
+    const DenseSys d = sys[blockIdx.y];
+    const int nb = 3 * d.nt;
+    const int i = j + 1 + blockIdx.x;
+    if (i >= nb) return;
+    __shared__ unsigned short s_a[kB * kLd], s_b[kB * kLd];
+    __shared__ float s_s[kB][kB + 1];
+    __shared__ float s_x[kB][kB + 1];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int wr = w >> 1, wc = w & 1;
+    float16_t acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    block_update<BF16>(d, i, j, s_a, s_b, mat, acc);
+    const size_t m = (size_t)192 * d.nt;
+    unsigned short* g = mat + blk_off(d, i, j);
+    for (int v = 0; v < 16; v++) {
+        const int r = wr * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), c = wc * 32 + (l & 31);
+        s_s[r][c] = from_lp<BF16>(g[(size_t)r * m + c]) - acc[v];
+    }
+    const float* inv = dinv + d.doff + (size_t)j * kB * kB;
+    for (int e = t; e < kB * kB; e += 256) s_x[e >> 6][e & 63] = inv[e];
+    __syncthreads();
+    // L_ij[r][c] = sum_k S[r][k] Linv[c][k]  (Linv lower: k <= c)
+    unsigned short* gt = mat + blk_off(d, j, i);
+    for (int e = t; e < kB * kB; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        float v = 0.f;
+        for (int k = 0; k <= c; k++) v += s_s[r][k] * s_x[c][k];
+        const unsigned short b = to_lp<BF16>(v);
+        g[(size_t)r * m + c] = b;
+        gt[(size_t)c * m + r] = b;
+    }
+}
+
+// ------------------------------------------------------------------ triangular solves
+// One workgroup per system solves L L^T y = rhs for the 4 right-hand sides (FP32 arithmetic on the
+// 16-bit factor), left-looking by blocks.  Thread (row r, split s) accumulates a quarter of the
+// columns of block row R for all 4 right-hand sides; y stays in LDS.  rhs and y are FP64 arrays of
+// m x 4 values (row-major: unknown, then right-hand side) at d.voff.
+constexpr int kMaxDenseTiles = 16;   // up to 1,024 particles per system: m <= 3,072
+
+template <bool BF16>
+__global__ __launch_bounds__(256) void k_dense_solve(const DenseSys* __restrict__ sys,
+                                                     const unsigned short* __restrict__ mat,
+                                                     const float* __restrict__ dinv, const double* __restrict__ rhs,
+                                                     double* __restrict__ y, const int* __restrict__ active) {
+    const DenseSys d = sys[blockIdx.x];
+    if (active && !active[blockIdx.x]) return;
+    const int nb = 3 * d.nt;
+    const size_t m = (size_t)192 * d.nt;
+    __shared__ float4 s_y[3 * kMaxDenseTiles * kB];   // 48 KB
+    __shared__ float4 s_p[4][kB];
+    const int t = threadIdx.x, r = t & 63, sp = t >> 6;
+    for (int e = t; e < (int)m; e += 256) {
+        const double* q = rhs + d.voff + 4 * (size_t)e;
+        s_y[e] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+    }
+    __syncthreads();
+    // forward: L y = rhs; L[R*64+r][col] = U[col][R*64+r] (upper copy, coalesced over r)
+    for (int R = 0; R < nb; R++) {
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        const unsigned short* col0 = mat + d.moff + (size_t)R * kB + r;
+        for (int c = sp; c < R * kB; c += 4) {
+            const float lv = from_lp<BF16>(col0[(size_t)c * m]);
+            const float4 yv = s_y[c];
+            a.x += lv * yv.x; a.y += lv * yv.y; a.z += lv * yv.z; a.w += lv * yv.w;
+        }
+        s_p[sp][r] = a;
+        __syncthreads();
+        if (sp == 0) {
+            float4 b = s_y[R * kB + r];
+            for (int u = 0; u < 4; u++) {
+                b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
+            }
+            s_p[0][r] = b;
+        }
+        __syncthreads();
+        if (sp == 0) {
+            const float* inv = dinv + d.doff + (size_t)R * kB * kB + (size_t)r * kB;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = 0; k <= r; k++) {
+                const float iv = inv[k];
+                const float4 b = s_p[0][k];
+                o.x += iv * b.x; o.y += iv * b.y; o.z += iv * b.z; o.w += iv * b.w;
+            }
+            s_y[R * kB + r] = o;
+        }
+        __syncthreads();
+    }
+    // backward: L^T x = y; (L^T)[R*64+r][col] = L[col][R*64+r] (lower storage, coalesced over r)
+    for (int R = nb - 1; R >= 0; R--) {
+        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+        const unsigned short* col0 = mat + d.moff + (size_t)R * kB + r;
+        for (int c = (R + 1) * kB + sp; c < (int)m; c += 4) {
+            const float lv = from_lp<BF16>(col0[(size_t)c * m]);
+            const float4 yv = s_y[c];
+            a.x += lv * yv.x; a.y += lv * yv.y; a.z += lv * yv.z; a.w += lv * yv.w;
+        }
+        s_p[sp][r] = a;
+        __syncthreads();
+        if (sp == 0) {
+            float4 b = s_y[R * kB + r];
+            for (int u = 0; u < 4; u++) {
+                b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
+            }
+            s_p[0][r] = b;
+        }
+        __syncthreads();
+        if (sp == 0) {
+            // x_R = Linv^T b: x[r] = sum_{k >= r} Linv[k][r] b[k]
+            const float* inv = dinv + d.doff + (size_t)R * kB * kB;
+            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (int k = r; k < kB; k++) {
+                const float iv = inv[(size_t)k * kB + r];
+                const float4 b = s_p[0][k];
+                o.x += iv * b.x; o.y += iv * b.y; o.z += iv * b.z; o.w += iv * b.w;
+            }
+            s_y[R * kB + r] = o;
+        }
+        __syncthreads();
+    }
+    for (int e = t; e < (int)m; e += 256) {
+        const float4 v = s_y[e];
+        double* q = y + d.voff + 4 * (size_t)e;
+        q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
+    }
+}
+
+// ------------------------------------------------------------------ refinement vectors
+// Slot i of tile T (system g, local tile I, lane s) is unknown c * 64 nt + 64 I + s.
+__device__ __forceinline__ size_t vec_index(const DenseSys& d, int T, int s, int c) {
+    return d.voff + 4 * ((size_t)c * kB * d.nt + (size_t)(T - d.t0) * kB + s);
+}
+
+// rhs^ = sqrt(alpha) [E0, e_x, e_y, e_z]; x^ = 0; r^ = rhs^
+__global__ __launch_bounds__(64) void k_dense_rhs(const int* __restrict__ tseg, const DenseSys* __restrict__ sys,
+                                                  const int* __restrict__ tcnt, const double4* __restrict__ E0,
+                                                  const double* __restrict__ alpha, double* __restrict__ b,
+                                                  double* __restrict__ x, double* __restrict__ res) {
+    const int T = blockIdx.x, s = threadIdx.x;
+    const DenseSys d = sys[tseg[T]];
+    const int i = T * kTile + s;
+    const bool valid = s < tcnt[T];
+    const double sa = valid ? sqrt(alpha[i]) : 0.0;
+    const double4 e = E0[i];
+    const double e0[3] = {e.x, e.y, e.z};
+    for (int c = 0; c < 3; c++) {
+        const size_t k = vec_index(d, T, s, c);
+        const double v[4] = {sa * e0[c], c == 0 ? sa : 0.0, c == 1 ? sa : 0.0, c == 2 ? sa : 0.0};
+        for (int q = 0; q < 4; q++) {
+            b[k + q] = v[q];
+            x[k + q] = 0.0;
+            res[k + q] = v[q];
+        }
+    }
+}
+
+// x^ += y (systems still refining); mu_q = sqrt(alpha) x^_q for right-hand side q (slot order)
+__global__ __launch_bounds__(64) void k_dense_update(const int* __restrict__ tseg, const DenseSys* __restrict__ sys,
+                                                     const double* __restrict__ alpha, const double* __restrict__ y,
+                                                     double* __restrict__ x, double4* __restrict__ mu4,
+                                                     size_t mu_stride, const int* __restrict__ active) {
+    const int T = blockIdx.x, s = threadIdx.x;
+    const int g = tseg[T];
+    const DenseSys d = sys[g];
+    const int i = T * kTile + s;
+    const double sa = sqrt(alpha[i]);
+    double m[4][3];
+    for (int c = 0; c < 3; c++) {
+        const size_t k = vec_index(d, T, s, c);
+        for (int q = 0; q < 4; q++) {
+            double v = x[k + q];
+            if (active[g]) {
+                v += y[k + q];
+                x[k + q] = v;
+            }
+            m[q][c] = sa * v;
+        }
+    }
+    for (int q = 0; q < 4; q++) mu4[q * mu_stride + i] = make_double4(m[q][0], m[q][1], m[q][2], 0.0);
+}
+
+// r^_q = rhs^_q - x^_q + sqrt(alpha) (T mu_q) for one right-hand side q; per-tile partial sums of
+// |r^_q|^2 and |rhs^_q|^2.
+__global__ __launch_bounds__(64) void k_dense_residual(int q, const int* __restrict__ tseg,
+                                                       const DenseSys* __restrict__ sys, const double* __restrict__ alpha,
+                                                       const double4* __restrict__ Tmu, const double* __restrict__ b,
+                                                       const double* __restrict__ x, double* __restrict__ res,
+                                                       double* __restrict__ red, int ntile) {
+    const int T = blockIdx.x, s = threadIdx.x;
+    const DenseSys d = sys[tseg[T]];
+    const int i = T * kTile + s;
+    const double sa = sqrt(alpha[i]);
+    const double4 f = Tmu[i];
+    const double fv[3] = {f.x, f.y, f.z};
+    double rr = 0.0, bb = 0.0;
+    for (int c = 0; c < 3; c++) {
+        const size_t k = vec_index(d, T, s, c) + q;
+        const double r = b[k] - x[k] + sa * fv[c];
+        res[k] = r;
+        rr += r * r;
+        bb += b[k] * b[k];
+    }
+    rr = wave_sum(rr);
+    bb = wave_sum(bb);
+    if (s == 0) {
+        red[T] = rr;
+        red[ntile + T] = bb;
+    }
+}
+
+// relative residual of system g for right-hand side q: out[4 g + q]
+__global__ __launch_bounds__(64) void k_dense_norms(int q, const int4* __restrict__ seg, const double* __restrict__ red,
+                                                    int ntile, double* __restrict__ out) {
+    const int4 sg = seg[blockIdx.x];
+    double a = 0.0, b = 0.0;
+    for (int T = sg.x + (int)threadIdx.x; T < sg.y; T += kTile) {
+        a += red[T];
+        b += red[ntile + T];
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    if (threadIdx.x == 0) out[4 * blockIdx.x + q] = b > 0.0 ? sqrt(a / b) : sqrt(a);
+}
+
+}  // namespace
+
+void launch_dense_assemble(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, const int* tcnt,
+                           const double4* posa, const double* alpha, const double* inva, const int* exc_start,
+                           const int* exc_J, const unsigned long long* exc_mask, unsigned short* mat) {
+    dim3 g(max_nt, max_nt, nsys);
+    if (bf16)
+        hipLaunchKernelGGL(k_dense_assemble<true>, g, dim3(256), 0, s, sys, tcnt, posa, alpha, inva, exc_start, exc_J,
+                           exc_mask, mat);
+    else
+        hipLaunchKernelGGL(k_dense_assemble<false>, g, dim3(256), 0, s, sys, tcnt, posa, alpha, inva, exc_start, exc_J,
+                           exc_mask, mat);
+}
+
+void launch_dense_factor(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, unsigned short* mat,
+                         float* dinv, int* status) {
+    const int nb = 3 * max_nt;
+    for (int j = 0; j < nb; j++) {
+        if (bf16) {
+            hipLaunchKernelGGL(k_chol_diag<true>, dim3(nsys), dim3(256), 0, s, j, sys, mat, dinv, status);
+            if (j + 1 < nb)
+                hipLaunchKernelGGL(k_chol_col<true>, dim3(nb - j - 1, nsys), dim3(256), 0, s, j, sys, mat, dinv);
+        } else {
+            hipLaunchKernelGGL(k_chol_diag<false>, dim3(nsys), dim3(256), 0, s, j, sys, mat, dinv, status);
+            if (j + 1 < nb)
+                hipLaunchKernelGGL(k_chol_col<false>, dim3(nb - j - 1, nsys), dim3(256), 0, s, j, sys, mat, dinv);
+        }
+    }
+}
+
+void launch_dense_solve(hipStream_t s, bool bf16, int nsys, const DenseSys* sys, const unsigned short* mat,
+                        const float* dinv, const double* rhs, double* y, const int* active) {
+    if (bf16)
+        hipLaunchKernelGGL(k_dense_solve<true>, dim3(nsys), dim3(256), 0, s, sys, mat, dinv, rhs, y, active);
+    else
+        hipLaunchKernelGGL(k_dense_solve<false>, dim3(nsys), dim3(256), 0, s, sys, mat, dinv, rhs, y, active);
+}
+
+void launch_dense_rhs(hipStream_t s, const SegLaunch& G, const DenseSys* sys, const int* tcnt, const double4* E0,
+                      const double* alpha, double* b, double* x, double* res) {
+    hipLaunchKernelGGL(k_dense_rhs, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, sys, tcnt, E0, alpha, b, x, res);
+}
+
+void launch_dense_update(hipStream_t s, const SegLaunch& G, const DenseSys* sys, const double* alpha, const double* y,
+                         double* x, double4* mu4, size_t mu_stride, const int* active) {
+    hipLaunchKernelGGL(k_dense_update, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, sys, alpha, y, x, mu4, mu_stride,
+                       active);
+}
+
+void launch_dense_residual(hipStream_t s, const SegLaunch& G, int q, const DenseSys* sys, const double* alpha,
+                           const double4* Tmu, const double* b, const double* x, double* res, double* red,
+                           double* rel) {
+    hipLaunchKernelGGL(k_dense_residual, dim3(G.ntile), dim3(kTile), 0, s, q, G.tseg, sys, alpha, Tmu, b, x, res, red,
+                       G.ntile);
+    hipLaunchKernelGGL(k_dense_norms, dim3(G.nseg), dim3(kTile), 0, s, q, G.seg, (const double*)red, G.ntile, rel);
+}
+
+int dense_max_tiles() { return kMaxDenseTiles; }
+
+// E0 = unit field along axis at every particle slot (padding slots 0)
+__global__ void k_uniform_field(int npad, const int* __restrict__ perm, int axis, double4* __restrict__ E0) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    const double v = perm[i] >= 0 ? 1.0 : 0.0;
+    E0[i] = make_double4(axis == 0 ? v : 0.0, axis == 1 ? v : 0.0, axis == 2 ? v : 0.0, 0.0);
+}
+
+void launch_uniform_field(hipStream_t s, int npad, const int* perm, int axis, double4* E0) {
+    hipLaunchKernelGGL(k_uniform_field, dim3((npad + 255) / 256), dim3(256), 0, s, npad, perm, axis, E0);
+}
+
+}  // namespace tdp

@@ -36,6 +36,14 @@ struct SegLaunch {
     int* done;
 };
 
+// One system of the dense path: nt particle tiles from tile t0; its 16-bit matrix (192 nt)^2 at
+// moff, its diagonal-block inverses (3 nt blocks of 64x64 FP32) at doff, its refinement vectors
+// ((192 nt) x 4 doubles) at voff.
+struct DenseSys {
+    int nt, t0;
+    size_t moff, voff, doff;
+};
+
 // Geometry of one all-pairs launch: nblk tiles of 64 particles.
 struct PairLaunch {
     int npad, nblk;
@@ -103,5 +111,23 @@ void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const
                       const double4* mu, bool pbc, const double* box3, double* out);
 void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd, const double4* mu,
                     const double* mass, double* out);
+
+// ---- dense path (tdp_dense.hip)
+int dense_max_tiles();
+void launch_dense_assemble(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, const int* tcnt,
+                           const double4* posa, const double* alpha, const double* inva, const int* exc_start,
+                           const int* exc_J, const unsigned long long* exc_mask, unsigned short* mat);
+void launch_dense_factor(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, unsigned short* mat,
+                         float* dinv, int* status);
+void launch_dense_solve(hipStream_t s, bool bf16, int nsys, const DenseSys* sys, const unsigned short* mat,
+                        const float* dinv, const double* rhs, double* y, const int* active);
+void launch_dense_rhs(hipStream_t s, const SegLaunch& G, const DenseSys* sys, const int* tcnt, const double4* E0,
+                      const double* alpha, double* b, double* x, double* res);
+void launch_dense_update(hipStream_t s, const SegLaunch& G, const DenseSys* sys, const double* alpha, const double* y,
+                         double* x, double4* mu4, size_t mu_stride, const int* active);
+void launch_dense_residual(hipStream_t s, const SegLaunch& G, int q, const DenseSys* sys, const double* alpha,
+                           const double4* Tmu, const double* b, const double* x, double* res, double* red,
+                           double* rel);
+void launch_uniform_field(hipStream_t s, int npad, const int* perm, int axis, double4* E0);
 
 }  // namespace tdp

@@ -49,17 +49,12 @@ __device__ __forceinline__ float from_lp(unsigned short b) {
 // 4 (l / 32) + t, t = 0..3, of row (A) / column (B) l % 32; accumulator element v of lane l is row
 // 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32.
 template <bool BF16>
-__device__ __forceinline__ float16_t mfma32(const unsigned short* a, const unsigned short* b, float16_t c) {
-    if (BF16) {
-        short4_t av = {(short)a[0], (short)a[1], (short)a[2], (short)a[3]};
-        short4_t bv = {(short)b[0], (short)b[1], (short)b[2], (short)b[3]};
-        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, c, 0, 0, 0);
-    }
-    half4_t av = {__builtin_bit_cast(_Float16, a[0]), __builtin_bit_cast(_Float16, a[1]),
-                  __builtin_bit_cast(_Float16, a[2]), __builtin_bit_cast(_Float16, a[3])};
-    half4_t bv = {__builtin_bit_cast(_Float16, b[0]), __builtin_bit_cast(_Float16, b[1]),
-                  __builtin_bit_cast(_Float16, b[2]), __builtin_bit_cast(_Float16, b[3])};
-    return __builtin_amdgcn_mfma_f32_32x32x8f16(av, bv, c, 0, 0, 0);
+__device__ __forceinline__ float16_t mfma32(uint2 a, uint2 b, float16_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(__builtin_bit_cast(short4_t, a), __builtin_bit_cast(short4_t, b),
+                                                       c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_32x32x8f16(__builtin_bit_cast(half4_t, a), __builtin_bit_cast(half4_t, b), c, 0,
+                                                0, 0);
 }
 
 __device__ __forceinline__ size_t blk_off(const DenseSys& d, int R, int C) {
@@ -125,34 +120,53 @@ __global__ __launch_bounds__(256) void k_dense_assemble(const DenseSys* __restri
 
 // ------------------------------------------------------------------ factorization
 // S = A^_ij - sum_{k<j} L_ik L_jk^T for one 64x64 block, on MFMA: four waves, each a 32x32 quarter.
-// Operand tiles are staged in LDS as 64 x 64 16-bit words (row stride 68 to spread the banks).
-constexpr int kLd = kB + 4;
+// Operand tiles are staged in LDS as 64 rows of 16 8-byte words (4 values each), row stride 17
+// words to spread the banks.
+constexpr int kLd2 = kB / 4 + 1;
+static_assert(2 * kB * (kB + 1) * sizeof(float) <= 4 * kB * kLd2 * sizeof(uint2), "LDS overlay");
 
 template <bool BF16>
-__device__ __forceinline__ void block_update(const DenseSys& d, int i, int j, unsigned short* s_a, unsigned short* s_b,
+__device__ __forceinline__ void block_update(const DenseSys& d, int i, int j, uint2* s_a, uint2* s_b,
                                              const unsigned short* __restrict__ mat, float16_t& acc) {
+    // s_a, s_b: two buffers each of 64 rows x kLd2 8-byte words; the tiles of step k + 1 are
+    // fetched into registers while step k's MFMAs run on the other buffer.
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int wr = w >> 1, wc = w & 1;
     const size_t m = (size_t)192 * d.nt;
-    for (int k = 0; k < j; k++) {
-        __syncthreads();
-        // stage L_ik and L_jk: 4096 words each, 16 per thread
-        const unsigned short* gi = mat + blk_off(d, i, k);
-        const unsigned short* gj = mat + blk_off(d, j, k);
-        for (int e = t; e < kB * kB / 4; e += 256) {
-            const int r = e >> 4, c4 = (e & 15) * 4;
-            const ushort4 va = *reinterpret_cast<const ushort4*>(gi + (size_t)r * m + c4);
-            const ushort4 vb = *reinterpret_cast<const ushort4*>(gj + (size_t)r * m + c4);
-            unsigned short* pa = s_a + r * kLd + c4;
-            unsigned short* pb = s_b + r * kLd + c4;
-            pa[0] = va.x; pa[1] = va.y; pa[2] = va.z; pa[3] = va.w;
-            pb[0] = vb.x; pb[1] = vb.y; pb[2] = vb.z; pb[3] = vb.w;
+    if (j == 0) return;
+    // 64 rows x 16 words per tile: thread t loads words (t, t + 256, t + 512, t + 768)
+    uint2 ra[4], rb[4];
+    auto fetch = [&](int k) {
+        const uint2* gi = reinterpret_cast<const uint2*>(mat + blk_off(d, i, k));
+        const uint2* gj = reinterpret_cast<const uint2*>(mat + blk_off(d, j, k));
+        const size_t mw = m / 4;
+        for (int u = 0; u < 4; u++) {
+            const int e = t + 256 * u, r = e >> 4, c = e & 15;
+            ra[u] = gi[(size_t)r * mw + c];
+            rb[u] = gj[(size_t)r * mw + c];
         }
-        __syncthreads();
-        const unsigned short* ra = s_a + (wr * 32 + (l & 31)) * kLd + 4 * (l >> 5);
-        const unsigned short* rb = s_b + (wc * 32 + (l & 31)) * kLd + 4 * (l >> 5);
+    };
+    auto stash = [&](int buf) {
+        uint2* pa = s_a + buf * kB * kLd2;
+        uint2* pb = s_b + buf * kB * kLd2;
+        for (int u = 0; u < 4; u++) {
+            const int e = t + 256 * u, r = e >> 4, c = e & 15;
+            pa[r * kLd2 + c] = ra[u];
+            pb[r * kLd2 + c] = rb[u];
+        }
+    };
+    fetch(0);
+    stash(0);
+    __syncthreads();
+    for (int k = 0; k < j; k++) {
+        const int buf = k & 1;
+        if (k + 1 < j) fetch(k + 1);
+        const uint2* pa = s_a + buf * kB * kLd2 + (wr * 32 + (l & 31)) * kLd2 + (l >> 5);
+        const uint2* pb = s_b + buf * kB * kLd2 + (wc * 32 + (l & 31)) * kLd2 + (l >> 5);
 #pragma unroll
-        for (int kk = 0; kk < kB; kk += 8) acc = mfma32<BF16>(ra + kk, rb + kk, acc);
+        for (int kk = 0; kk < kB / 4; kk += 2) acc = mfma32<BF16>(pa[kk], pb[kk], acc);
+        if (k + 1 < j) stash(buf ^ 1);
+        __syncthreads();
     }
 }
 
@@ -166,9 +180,12 @@ __global__ __launch_bounds__(256) void k_chol_diag(int j, const DenseSys* __rest
     const DenseSys d = sys[blockIdx.x];
     const int nb = 3 * d.nt;
     if (j >= nb) return;
-    __shared__ unsigned short s_a[kB * kLd], s_b[kB * kLd];
-    __shared__ float s_s[kB][kB + 1];
-    __shared__ float s_x[kB][kB + 1];
+    // the staging buffers of the update and the FP32 blocks used after it share one LDS area
+    __shared__ uint2 s_raw[4 * kB * kLd2];
+    uint2* s_a = s_raw;
+    uint2* s_b = s_raw + 2 * kB * kLd2;
+    float (*s_s)[kB + 1] = reinterpret_cast<float (*)[kB + 1]>(s_raw);
+    float (*s_x)[kB + 1] = s_s + kB;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int wr = w >> 1, wc = w & 1;
     float16_t acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -180,36 +197,36 @@ __global__ __launch_bounds__(256) void k_chol_diag(int j, const DenseSys* __rest
         s_s[r][c] = from_lp<BF16>(g[(size_t)r * m + c]) - acc[v];
     }
     __syncthreads();
-    // right-looking Cholesky of the 64x64 block, lower triangle
+    // Right-looking Cholesky of the 64x64 block (lower triangle of s_s) and, in the same sweep, the
+    // inverse of the factor (s_x, lower; Gauss-Jordan on the rows of the identity).
+    for (int e = t; e < kB * kB; e += 256) s_x[e >> 6][e & 63] = (e >> 6) == (e & 63) ? 1.f : 0.f;
+    __syncthreads();
     for (int p = 0; p < kB; p++) {
-        if (t == 0) {
-            float dv = s_s[p][p];
-            if (!(dv > 0.f)) {
-                atomicOr(status, 1);
-                dv = 1.f;
-            }
-            s_s[p][p] = sqrtf(dv);
+        float dv = s_s[p][p];
+        if (!(dv > 0.f)) {
+            if (t == 0) atomicOr(status, 1);
+            dv = 1.f;
+        }
+        const float piv = sqrtf(dv), ipiv = 1.f / piv;
+        __syncthreads();
+        // column p of L and row p of the inverse
+        if (t < kB) {
+            if (t > p) s_s[t][p] *= ipiv;
+            if (t == p) s_s[p][p] = piv;
+            if (t <= p) s_x[p][t] *= ipiv;
         }
         __syncthreads();
-        const float piv = s_s[p][p];
-        for (int r = p + 1 + t; r < kB; r += 256) s_s[r][p] /= piv;
-        __syncthreads();
+        // trailing update of S (p < c <= r) and elimination of column p from the rows below in X
         for (int e = t; e < kB * kB; e += 256) {
             const int r = e >> 6, c = e & 63;
-            if (c > p && r >= c) s_s[r][c] -= s_s[r][p] * s_s[c][p];
+            if (r > p) {
+                const float lrp = s_s[r][p];
+                if (c > p && c <= r) s_s[r][c] -= lrp * s_s[c][p];
+                if (c <= p) s_x[r][c] -= lrp * s_x[p][c];
+            }
         }
         __syncthreads();
     }
-    // inverse of the lower factor: column c solved by thread c (forward substitution)
-    if (t < kB) {
-        const int c = t;
-        for (int r = 0; r < kB; r++) {
-            float v = r == c ? 1.f : 0.f;
-            for (int k = c; k < r; k++) v -= s_s[r][k] * s_x[k][c];
-            s_x[r][c] = r < c ? 0.f : v / s_s[r][r];
-        }
-    }
-    __syncthreads();
     float* out = dinv + d.doff + (size_t)j * kB * kB;
     for (int e = t; e < kB * kB; e += 256) out[e] = s_x[e >> 6][e & 63];
 }
@@ -223,9 +240,12 @@ __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restr
     const int nb = 3 * d.nt;
     const int i = j + 1 + blockIdx.x;
     if (i >= nb) return;
-    __shared__ unsigned short s_a[kB * kLd], s_b[kB * kLd];
-    __shared__ float s_s[kB][kB + 1];
-    __shared__ float s_x[kB][kB + 1];
+    // the staging buffers of the update and the FP32 blocks used after it share one LDS area
+    __shared__ uint2 s_raw[4 * kB * kLd2];
+    uint2* s_a = s_raw;
+    uint2* s_b = s_raw + 2 * kB * kLd2;
+    float (*s_s)[kB + 1] = reinterpret_cast<float (*)[kB + 1]>(s_raw);
+    float (*s_x)[kB + 1] = s_s + kB;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int wr = w >> 1, wc = w & 1;
     float16_t acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -239,24 +259,59 @@ __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restr
     const float* inv = dinv + d.doff + (size_t)j * kB * kB;
     for (int e = t; e < kB * kB; e += 256) s_x[e >> 6][e & 63] = inv[e];
     __syncthreads();
-    // L_ij[r][c] = sum_k S[r][k] Linv[c][k]  (Linv lower: k <= c)
+    // L_ij[r][c] = sum_k S[r][k] Linv[c][k]  (Linv lower: k <= c); thread: rows 4 rb.., cols 4 cb..
     unsigned short* gt = mat + blk_off(d, j, i);
-    for (int e = t; e < kB * kB; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        float v = 0.f;
-        for (int k = 0; k <= c; k++) v += s_s[r][k] * s_x[c][k];
-        const unsigned short b = to_lp<BF16>(v);
-        g[(size_t)r * m + c] = b;
-        gt[(size_t)c * m + r] = b;
+    {
+        const int rb = (t >> 4) * 4, cb = (t & 15) * 4;
+        float o[4][4] = {};
+        for (int k = 0; k <= cb + 3; k++) {
+            float sv[4], xv[4];
+            for (int u = 0; u < 4; u++) {
+                sv[u] = s_s[rb + u][k];
+                xv[u] = s_x[cb + u][k];
+            }
+            for (int u = 0; u < 4; u++)
+                for (int v = 0; v < 4; v++) o[u][v] += sv[u] * xv[v];
+        }
+        for (int u = 0; u < 4; u++)
+            for (int v = 0; v < 4; v++) {
+                const unsigned short b = to_lp<BF16>(o[u][v]);
+                g[(size_t)(rb + u) * m + cb + v] = b;
+                gt[(size_t)(cb + v) * m + rb + u] = b;
+            }
     }
 }
 
 // ------------------------------------------------------------------ triangular solves
 // One workgroup per system solves L L^T y = rhs for the 4 right-hand sides (FP32 arithmetic on the
-// 16-bit factor), left-looking by blocks.  Thread (row r, split s) accumulates a quarter of the
-// columns of block row R for all 4 right-hand sides; y stays in LDS.  rhs and y are FP64 arrays of
-// m x 4 values (row-major: unknown, then right-hand side) at d.voff.
+// 16-bit factor), left-looking by blocks.  Row R*64 + r of L left of the diagonal block sits in the
+// lower blocks and row R*64 + r of L^T right of it in the upper (transposed) blocks, both
+// contiguous: thread (r, s) reads 16-byte pieces s, s + 4, ... of that row and accumulates all 4
+// right-hand sides; the four partial sums are added in a fixed order.  y stays in LDS.  rhs and y
+// are FP64 arrays of m x 4 values (unknown, then right-hand side) at d.voff.
 constexpr int kMaxDenseTiles = 16;   // up to 1,024 particles per system: m <= 3,072
+
+template <bool BF16>
+__device__ __forceinline__ float4 row_dot(const unsigned short* __restrict__ row, int c0, int c1, int sp,
+                                          const float4* __restrict__ s_y) {
+    // sum over columns [c0, c1) (multiples of 8) of row[c] * y[c], pieces of 8 columns
+    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c = c0 + 8 * sp; c < c1; c += 32) {
+        const uint4 v = *reinterpret_cast<const uint4*>(row + c);
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int h = 0; h < 4; h++) {
+            const float l0 = from_lp<BF16>((unsigned short)(w[h] & 0xffffu));
+            const float l1 = from_lp<BF16>((unsigned short)(w[h] >> 16));
+            const float4 y0 = s_y[c + 2 * h], y1 = s_y[c + 2 * h + 1];
+            a.x += l0 * y0.x + l1 * y1.x;
+            a.y += l0 * y0.y + l1 * y1.y;
+            a.z += l0 * y0.z + l1 * y1.z;
+            a.w += l0 * y0.w + l1 * y1.w;
+        }
+    }
+    return a;
+}
 
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_dense_solve(const DenseSys* __restrict__ sys,
@@ -266,79 +321,51 @@ __global__ __launch_bounds__(256) void k_dense_solve(const DenseSys* __restrict_
     const DenseSys d = sys[blockIdx.x];
     if (active && !active[blockIdx.x]) return;
     const int nb = 3 * d.nt;
-    const size_t m = (size_t)192 * d.nt;
+    const int m = 192 * d.nt;
     __shared__ float4 s_y[3 * kMaxDenseTiles * kB];   // 48 KB
     __shared__ float4 s_p[4][kB];
+    __shared__ float s_b[4][kB];
+    __shared__ float s_inv[kB][kB + 1];
     const int t = threadIdx.x, r = t & 63, sp = t >> 6;
-    for (int e = t; e < (int)m; e += 256) {
+    for (int e = t; e < m; e += 256) {
         const double* q = rhs + d.voff + 4 * (size_t)e;
         s_y[e] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
     }
     __syncthreads();
-    // forward: L y = rhs; L[R*64+r][col] = U[col][R*64+r] (upper copy, coalesced over r)
-    for (int R = 0; R < nb; R++) {
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-        const unsigned short* col0 = mat + d.moff + (size_t)R * kB + r;
-        for (int c = sp; c < R * kB; c += 4) {
-            const float lv = from_lp<BF16>(col0[(size_t)c * m]);
-            const float4 yv = s_y[c];
-            a.x += lv * yv.x; a.y += lv * yv.y; a.z += lv * yv.z; a.w += lv * yv.w;
-        }
-        s_p[sp][r] = a;
-        __syncthreads();
-        if (sp == 0) {
-            float4 b = s_y[R * kB + r];
-            for (int u = 0; u < 4; u++) {
-                b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
+    for (int pass = 0; pass < 2; pass++) {
+        // pass 0: L y = rhs, blocks R = 0..nb-1; pass 1: L^T x = y, blocks R = nb-1..0
+        for (int step = 0; step < nb; step++) {
+            const int R = pass == 0 ? step : nb - 1 - step;
+            const unsigned short* row = mat + d.moff + (size_t)(R * kB + r) * m;
+            const float4 a = pass == 0 ? row_dot<BF16>(row, 0, R * kB, sp, s_y)
+                                       : row_dot<BF16>(row, (R + 1) * kB, m, sp, s_y);
+            s_p[sp][r] = a;
+            __syncthreads();
+            if (sp == 0) {
+                float4 b = s_y[R * kB + r];
+                for (int u = 0; u < 4; u++) {
+                    b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
+                }
+                s_b[0][r] = b.x; s_b[1][r] = b.y; s_b[2][r] = b.z; s_b[3][r] = b.w;
             }
-            s_p[0][r] = b;
-        }
-        __syncthreads();
-        if (sp == 0) {
-            const float* inv = dinv + d.doff + (size_t)R * kB * kB + (size_t)r * kB;
-            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int k = 0; k <= r; k++) {
-                const float iv = inv[k];
-                const float4 b = s_p[0][k];
-                o.x += iv * b.x; o.y += iv * b.y; o.z += iv * b.z; o.w += iv * b.w;
-            }
-            s_y[R * kB + r] = o;
-        }
-        __syncthreads();
-    }
-    // backward: L^T x = y; (L^T)[R*64+r][col] = L[col][R*64+r] (lower storage, coalesced over r)
-    for (int R = nb - 1; R >= 0; R--) {
-        float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-        const unsigned short* col0 = mat + d.moff + (size_t)R * kB + r;
-        for (int c = (R + 1) * kB + sp; c < (int)m; c += 4) {
-            const float lv = from_lp<BF16>(col0[(size_t)c * m]);
-            const float4 yv = s_y[c];
-            a.x += lv * yv.x; a.y += lv * yv.y; a.z += lv * yv.z; a.w += lv * yv.w;
-        }
-        s_p[sp][r] = a;
-        __syncthreads();
-        if (sp == 0) {
-            float4 b = s_y[R * kB + r];
-            for (int u = 0; u < 4; u++) {
-                b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
-            }
-            s_p[0][r] = b;
-        }
-        __syncthreads();
-        if (sp == 0) {
-            // x_R = Linv^T b: x[r] = sum_{k >= r} Linv[k][r] b[k]
+            __syncthreads();
+            // diagonal block: thread (r, q = sp) applies Linv (pass 0) or Linv^T (pass 1) to field q
             const float* inv = dinv + d.doff + (size_t)R * kB * kB;
-            float4 o = make_float4(0.f, 0.f, 0.f, 0.f);
-            for (int k = r; k < kB; k++) {
-                const float iv = inv[(size_t)k * kB + r];
-                const float4 b = s_p[0][k];
-                o.x += iv * b.x; o.y += iv * b.y; o.z += iv * b.z; o.w += iv * b.w;
+            for (int e = t; e < kB * kB; e += 256) s_inv[e >> 6][e & 63] = inv[e];
+            __syncthreads();
+            float o = 0.f;
+            if (pass == 0) {
+                for (int k = 0; k <= r; k++) o += s_inv[r][k] * s_b[sp][k];
+            } else {
+                for (int k = r; k < kB; k++) o += s_inv[k][r] * s_b[sp][k];
             }
-            s_y[R * kB + r] = o;
+            __syncthreads();
+            float* yr = reinterpret_cast<float*>(&s_y[R * kB + r]);
+            yr[sp] = o;
+            __syncthreads();
         }
-        __syncthreads();
     }
-    for (int e = t; e < (int)m; e += 256) {
+    for (int e = t; e < m; e += 256) {
         const float4 v = s_y[e];
         double* q = y + d.voff + 4 * (size_t)e;
         q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;

@@ -734,7 +734,8 @@ void forces_and_energy(tdp_context* c, const double* box) {
     double4* pf = reinterpret_cast<double4*>(c->d_pbuf);
     double4* pt = c->d_pbuf2;
     const size_t off = (size_t)c->nwork * 64;
-    launch_sym_elec(s, L, c->polarization == TDP_MUTUAL, c->posa, c->qd, c->mu, c->d_inva, c->d_perm, pf,
+    launch_sym_elec(s, L, c->polarization == TDP_MUTUAL, c->precision == TDP_PRECISION_MIXED, c->posa, c->qd, c->mu,
+                    c->d_inva, c->d_perm, pf,
                     reinterpret_cast<double*>(pf + off), pt, reinterpret_cast<double*>(pt + off), c->d_part_e, c->d_frc,
                     c->torque);
     launch_reduce_energy(s, G, c->d_part_e, c->d_energy);

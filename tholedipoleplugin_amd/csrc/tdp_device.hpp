@@ -72,85 +72,105 @@ struct ElecOut {
     double tkx, tky, tkz;      // torque on K
 };
 
+// Inputs of one side of calculateElectrostaticPairIxn in precision T (positions enter through the
+// separation and the frame's axis choice).
+template <typename T>
+struct ElecSiteT {
+    T q;
+    T dx, dy, dz;   // lab-frame permanent dipole
+    T ux, uy, uz;   // induced dipole
+    T a;            // Thole parameter
+    T ia;           // 1/a
+};
+
+template <typename T>
+struct ElecOutT {
+    T e;                  // pair energy
+    T fx, fy, fz;         // force on K (force on I is the negative)
+    T tix, tiy, tiz;      // torque on I
+    T tkx, tky, tkz;      // torque on K
+};
+
 // calculateElectrostaticPairIxn + formQIRotationMatrix (ReferenceTholeDipoleForce.cpp:129-366,
-// :573-607) for the ordered pair (I,K), I < K, with _electric = _dielectric = 1.
+// :573-607) for the ordered pair (I,K), I < K, with _electric = _dielectric = 1, in precision T.
+// yz_differ: the raw positions of I and K differ in y or z (the frame's axis choice, :584-588).
 // delta = min-imaged posK - posI.  The QI frame has rows [X, Y, Z] with Z along delta; the
 // first rotated component is used as the bond-axis component and forces are rotated back with
 // the reference's index pattern (:150-159).  rInvVec[4] (read past a 4-element vector in the
 // reference, :196 vs :264-333) is taken as 1/r^4.  Thole factors: when |a_I a_K| <= 1e-5 or
 // a*u^3 >= 50 every factor is exactly 1 in FP64, so the exponential is skipped there; DAMP = false
 // is the specialisation for pairs known to lie beyond that range (all factors 1).
-template <bool MUTUAL, bool DAMP = true>
-__device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, double dx, double dy, double dz,
-                                          double mS, double iS, ElecOut& o) {
-    double r2 = dx * dx + dy * dy + dz * dz;
-    double rInv = rsqrt(r2);
-    double r = r2 * rInv;
+template <typename T, bool MUTUAL, bool DAMP>
+__device__ __forceinline__ void elec_pair_t(const ElecSiteT<T>& I, const ElecSiteT<T>& K, T dx, T dy, T dz,
+                                            bool yz_differ, T mS, T iS, ElecOutT<T>& o) {
+    T r2 = dx * dx + dy * dy + dz * dz;
+    T rInv = rsqrt_t(r2);
+    T r = r2 * rInv;
     // formQIRotationMatrix
-    double zx = dx * rInv, zy = dy * rInv, zz = dz * rInv;
-    double xx = zx, xy = zy, xz = zz;
-    if (I.y != K.y || I.z != K.z) xx += 1.0; else xy += 1.0;
-    double dt = zx * xx + zy * xy + zz * xz;
+    T zx = dx * rInv, zy = dy * rInv, zz = dz * rInv;
+    T xx = zx, xy = zy, xz = zz;
+    if (yz_differ) xx += T(1); else xy += T(1);
+    T dt = zx * xx + zy * xy + zz * xz;
     xx -= zx * dt; xy -= zy * dt; xz -= zz * dt;
-    double xn2 = xx * xx + xy * xy + xz * xz;
-    if (xn2 > 0.0) { double s = rsqrt(xn2); xx *= s; xy *= s; xz *= s; }
-    double yx = zy * xz - zz * xy;
-    double yy = zz * xx - zx * xz;
-    double yz = zx * xy - zy * xx;
+    T xn2 = xx * xx + xy * xy + xz * xz;
+    if (xn2 > T(0)) { T s = rsqrt_t(xn2); xx *= s; xy *= s; xz *= s; }
+    T yx = zy * xz - zz * xy;
+    T yy = zz * xx - zx * xz;
+    T yz = zx * xy - zy * xx;
     // rotated components: index 0 <- X, 1 <- Y, 2 <- Z (rows of the QI matrix)
-    double uI0 = xx * I.ux + xy * I.uy + xz * I.uz, uI1 = yx * I.ux + yy * I.uy + yz * I.uz,
+    T uI0 = xx * I.ux + xy * I.uy + xz * I.uz, uI1 = yx * I.ux + yy * I.uy + yz * I.uz,
            uI2 = zx * I.ux + zy * I.uy + zz * I.uz;
-    double uJ0 = xx * K.ux + xy * K.uy + xz * K.uz, uJ1 = yx * K.ux + yy * K.uy + yz * K.uz,
+    T uJ0 = xx * K.ux + xy * K.uy + xz * K.uz, uJ1 = yx * K.ux + yy * K.uy + yz * K.uz,
            uJ2 = zx * K.ux + zy * K.uy + zz * K.uz;
-    double QI0 = I.q, QJ0 = K.q;
-    double QI1 = xx * I.dx + xy * I.dy + xz * I.dz, QI2 = yx * I.dx + yy * I.dy + yz * I.dz,
+    T QI0 = I.q, QJ0 = K.q;
+    T QI1 = xx * I.dx + xy * I.dy + xz * I.dz, QI2 = yx * I.dx + yy * I.dy + yz * I.dz,
            QI3 = zx * I.dx + zy * I.dy + zz * I.dz;
-    double QJ1 = xx * K.dx + xy * K.dy + xz * K.dz, QJ2 = yx * K.dx + yy * K.dy + yz * K.dz,
+    T QJ1 = xx * K.dx + xy * K.dy + xz * K.dz, QJ2 = yx * K.dx + yy * K.dy + yz * K.dz,
            QJ3 = zx * K.dx + zy * K.dy + zz * K.dz;
 
-    double rI1 = rInv, rI2 = rI1 * rInv, rI3 = rI2 * rInv, rI4 = rI3 * rInv;
+    T rI1 = rInv, rI2 = rI1 * rInv, rI3 = rI2 * rInv, rI4 = rI3 * rInv;
 
-    double dmp = I.a * K.a;
-    double am = I.a < K.a ? I.a : K.a;
-    double expau3 = 0.0, au3 = 0.0;
-    if (DAMP && fabs(dmp) > 1.0e-5) {
-        double u = r * (I.ia * K.ia);
+    T dmp = I.a * K.a;
+    T am = I.a < K.a ? I.a : K.a;
+    T expau3 = T(0.0), au3 = T(0.0);
+    if (DAMP && fabs(dmp) > T(1.0e-5)) {
+        T u = r * (I.ia * K.ia);
         au3 = am * u * u * u;
-        if (au3 < 50.0) expau3 = exp(-au3);
+        if (au3 < T(50)) expau3 = exp_t(-au3);
     }
-    double thole_c = 1.0 - expau3;
-    double thole_d0 = 1.0 - expau3 * (1.0 + 1.5 * au3);
-    double thole_d1 = thole_c;
-    double dthole_c = thole_d0;
-    double dthole_d0 = 1.0 - expau3 * (1.0 + au3 + 1.5 * au3 * au3);
-    double dthole_d1 = 1.0 - expau3 * (1.0 + au3);
+    T thole_c = T(1.0) - expau3;
+    T thole_d0 = T(1.0) - expau3 * (T(1.0) + T(1.5) * au3);
+    T thole_d1 = thole_c;
+    T dthole_c = thole_d0;
+    T dthole_d0 = T(1.0) - expau3 * (T(1.0) + au3 + T(1.5) * au3 * au3);
+    T dthole_d1 = T(1.0) - expau3 * (T(1.0) + au3);
 
     // charge-charge (m=0)
-    double ePerm = rI1 * mS;
-    double dPerm = -0.5 * mS * rI2;
-    double Vij0 = ePerm * QJ0, Vji0 = ePerm * QI0;
-    double VijR0 = dPerm * QJ0, VjiR0 = dPerm * QI0;
+    T ePerm = rI1 * mS;
+    T dPerm = -T(0.5) * mS * rI2;
+    T Vij0 = ePerm * QJ0, Vji0 = ePerm * QI0;
+    T VijR0 = dPerm * QJ0, VjiR0 = dPerm * QI0;
     // C-D, C-Uind (m=0)
     ePerm = rI2 * mS;
-    double eU = rI2 * iS * thole_c;
+    T eU = rI2 * iS * thole_c;
     dPerm = -rI3 * mS;
-    double dU = -2.0 * rI3 * iS * dthole_c;
+    T dU = -T(2.0) * rI3 * iS * dthole_c;
     Vij0 += -(ePerm * QJ1 + eU * uJ0);
-    double Vji1 = -(ePerm * QI0);
+    T Vji1 = -(ePerm * QI0);
     VijR0 += -(dPerm * QJ1 + dU * uJ0);
-    double VjiR1 = -(dPerm * QI0);
-    double Vjid0 = -(eU * QI0);
+    T VjiR1 = -(dPerm * QI0);
+    T Vjid0 = -(eU * QI0);
     // D-C, Uind-C (m=0)
-    double Vij1 = ePerm * QJ0;
+    T Vij1 = ePerm * QJ0;
     Vji0 += ePerm * QI1 + eU * uI0;
-    double VijR1 = dPerm * QJ0;
+    T VijR1 = dPerm * QJ0;
     VjiR0 += dPerm * QI1 + dU * uI0;
-    double Vijd0 = eU * QJ0;
+    T Vijd0 = eU * QJ0;
     // D-D, D-Uind (m=0)
-    ePerm = -2.0 * rI3 * mS;
-    eU = -2.0 * rI3 * iS * thole_d0;
-    dPerm = 3.0 * rI4 * mS;
-    dU = 6.0 * rI4 * iS * dthole_d0;
+    ePerm = -T(2.0) * rI3 * mS;
+    eU = -T(2.0) * rI3 * iS * thole_d0;
+    dPerm = T(3.0) * rI4 * mS;
+    dU = T(6.0) * rI4 * iS * dthole_d0;
     Vij1 += ePerm * QJ1 + eU * uJ0;
     Vji1 += ePerm * QI1 + eU * uI0;
     VijR1 += dPerm * QJ1 + dU * uJ0;
@@ -160,49 +180,49 @@ __device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, 
     // D-D, D-Uind (m=1)
     ePerm = rI3 * mS;
     eU = rI3 * iS * thole_d1;
-    dPerm = -1.5 * rI4 * mS;
-    dU = -3.0 * rI4 * iS * dthole_d1;
-    double Vij2 = ePerm * QJ2 + eU * uJ1;
-    double Vji2 = ePerm * QI2 + eU * uI1;
-    double VijR2 = dPerm * QJ2 + dU * uJ1;
-    double VjiR2 = dPerm * QI2 + dU * uI1;
-    double Vijd1 = eU * QJ2;
-    double Vjid1 = eU * QI2;
-    double Vij3 = ePerm * QJ3 + eU * uJ2;
-    double Vji3 = ePerm * QI3 + eU * uI2;
-    double VijR3 = dPerm * QJ3 + dU * uJ2;
-    double VjiR3 = dPerm * QI3 + dU * uI2;
-    double Vijd2 = eU * QJ3;
-    double Vjid2 = eU * QI3;
+    dPerm = -T(1.5) * rI4 * mS;
+    dU = -T(3.0) * rI4 * iS * dthole_d1;
+    T Vij2 = ePerm * QJ2 + eU * uJ1;
+    T Vji2 = ePerm * QI2 + eU * uI1;
+    T VijR2 = dPerm * QJ2 + dU * uJ1;
+    T VjiR2 = dPerm * QI2 + dU * uI1;
+    T Vijd1 = eU * QJ2;
+    T Vjid1 = eU * QI2;
+    T Vij3 = ePerm * QJ3 + eU * uJ2;
+    T Vji3 = ePerm * QI3 + eU * uI2;
+    T VijR3 = dPerm * QJ3 + dU * uJ2;
+    T VjiR3 = dPerm * QI3 + dU * uI2;
+    T Vijd2 = eU * QJ3;
+    T Vjid2 = eU * QI3;
 
-    double energy = 0.5 * (QI0 * Vij0 + QJ0 * Vji0);
-    energy += 0.5 * (QI1 * Vij1 + QJ1 * Vji1);
-    energy += 0.5 * (QI2 * Vij2 + QJ2 * Vji2);
-    energy += 0.5 * (QI3 * Vij3 + QJ3 * Vji3);
-    double fIZ = QI0 * VijR0 + QI1 * VijR1 + QI2 * VijR2 + QI3 * VijR3;
-    double fJZ = QJ0 * VjiR0 + QJ1 * VjiR1 + QJ2 * VjiR2 + QJ3 * VjiR3;
+    T energy = T(0.5) * (QI0 * Vij0 + QJ0 * Vji0);
+    energy += T(0.5) * (QI1 * Vij1 + QJ1 * Vji1);
+    energy += T(0.5) * (QI2 * Vij2 + QJ2 * Vji2);
+    energy += T(0.5) * (QI3 * Vij3 + QJ3 * Vji3);
+    T fIZ = QI0 * VijR0 + QI1 * VijR1 + QI2 * VijR2 + QI3 * VijR3;
+    T fJZ = QJ0 * VjiR0 + QJ1 * VjiR1 + QJ2 * VjiR2 + QJ3 * VjiR3;
     // qiQIX = {0, Q3, 0, -Q1}, qiQIY = {0, -Q2, Q1, 0}, qiQIZ = {0, 0, -Q3, Q2}
-    double EIX = QI3 * Vij1 - QI1 * Vij3;
-    double EIY = -QI2 * Vij1 + QI1 * Vij2;
-    double EIZ = -QI3 * Vij2 + QI2 * Vij3;
-    double EJX = QJ3 * Vji1 - QJ1 * Vji3;
-    double EJY = -QJ2 * Vji1 + QJ1 * Vji2;
-    double EJZ = -QJ3 * Vji2 + QJ2 * Vji3;
-    double iEIX = uI2 * Vijd0 - uI0 * Vijd2;
-    double iEJX = uJ2 * Vjid0 - uJ0 * Vjid2;
-    double iEIY = uI0 * Vijd1 - uI1 * Vijd0;
-    double iEJY = uJ0 * Vjid1 - uJ1 * Vjid0;
+    T EIX = QI3 * Vij1 - QI1 * Vij3;
+    T EIY = -QI2 * Vij1 + QI1 * Vij2;
+    T EIZ = -QI3 * Vij2 + QI2 * Vij3;
+    T EJX = QJ3 * Vji1 - QJ1 * Vji3;
+    T EJY = -QJ2 * Vji1 + QJ1 * Vji2;
+    T EJZ = -QJ3 * Vji2 + QJ2 * Vji3;
+    T iEIX = uI2 * Vijd0 - uI0 * Vijd2;
+    T iEJX = uJ2 * Vjid0 - uJ0 * Vjid2;
+    T iEIY = uI0 * Vijd1 - uI1 * Vijd0;
+    T iEJY = uJ0 * Vjid1 - uJ1 * Vjid0;
     if (MUTUAL) {
-        double eC = -4.0 * rI3 * iS * thole_d0;
-        double dC = 6.0 * rI4 * iS * dthole_d0;
+        T eC = -T(4.0) * rI3 * iS * thole_d0;
+        T dC = T(6.0) * rI4 * iS * dthole_d0;
         iEIX += eC * uI2 * uJ0;
         iEJX += eC * uJ2 * uI0;
         iEIY -= eC * uI1 * uJ0;
         iEJY -= eC * uJ1 * uI0;
         fIZ += dC * uI0 * uJ0;
         fJZ += dC * uJ0 * uI0;
-        eC = 2.0 * rI3 * iS * thole_d1;
-        dC = -3.0 * rI4 * iS * dthole_d1;
+        eC = T(2.0) * rI3 * iS * thole_d1;
+        dC = -T(3.0) * rI4 * iS * dthole_d1;
         iEIX -= eC * uI0 * uJ2;
         iEJX -= eC * uJ0 * uI2;
         iEIY += eC * uI0 * uJ1;
@@ -210,9 +230,9 @@ __device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, 
         fIZ += dC * (uI1 * uJ1 + uI2 * uJ2);
         fJZ += dC * (uJ1 * uI1 + uJ2 * uI2);
     }
-    double qF0 = rInv * (EIY + EJY + iEIY + iEJY);
-    double qF1 = -rInv * (EIX + EJX + iEIX + iEJX);
-    double qF2 = -(fJZ + fIZ);
+    T qF0 = rInv * (EIY + EJY + iEIY + iEJY);
+    T qF1 = -rInv * (EIX + EJX + iEIX + iEJX);
+    T qF2 = -(fJZ + fIZ);
     // forceRotationMatrix rows: (Y_y, Z_y, X_y), (Y_z, Z_z, X_z), (Y_x, Z_x, X_x)
     o.e = energy;
     o.fx = yy * qF0 + zy * qF1 + xy * qF2;
@@ -224,6 +244,21 @@ __device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, 
     o.tkx = -(yy * EJX + zy * EJY + xy * EJZ);
     o.tky = -(yz * EJX + zz * EJY + xz * EJZ);
     o.tkz = -(yx * EJX + zx * EJY + xx * EJZ);
+}
+
+
+// FP64 form on full sites (the frame's axis choice compares the raw y and z).
+template <bool MUTUAL, bool DAMP = true>
+__device__ __forceinline__ void elec_pair(const ElecSite& I, const ElecSite& K, double dx, double dy, double dz,
+                                          double mS, double iS, ElecOut& o) {
+    const ElecSiteT<double> a = {I.q, I.dx, I.dy, I.dz, I.ux, I.uy, I.uz, I.a, I.ia};
+    const ElecSiteT<double> b = {K.q, K.dx, K.dy, K.dz, K.ux, K.uy, K.uz, K.a, K.ia};
+    ElecOutT<double> r;
+    elec_pair_t<double, MUTUAL, DAMP>(a, b, dx, dy, dz, I.y != K.y || I.z != K.z, mS, iS, r);
+    o.e = r.e;
+    o.fx = r.fx; o.fy = r.fy; o.fz = r.fz;
+    o.tix = r.tix; o.tiy = r.tiy; o.tiz = r.tiz;
+    o.tkx = r.tkx; o.tky = r.tky; o.tkz = r.tkz;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {

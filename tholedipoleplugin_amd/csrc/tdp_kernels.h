@@ -83,7 +83,7 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
                    const int4* axis, double4* qd);
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
                       double4* Pi, double* Pj, double4* out);
-void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
+void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed, const double4* posa, const double4* qd,
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
                      double* PjT, double* part_e, double* frc, double4* torque);
 void launch_field_init(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha, double4* E0,

@@ -634,7 +634,9 @@ __device__ __forceinline__ void elec_sym_step(const ElecSite& me, int my_orig, c
     en += o.e;
 }
 
-template <bool PBC, bool MUTUAL>
+// MIXED: tile pairs beyond the Thole range evaluate the pair terms in FP32 (separations, images and
+// the frame's axis choice in FP64, sums per tile flushed to FP64); every other tile stays FP64.
+template <bool PBC, bool MUTUAL, bool MIXED>
 __global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int4* __restrict__ work,
                                                  const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                  const double4* __restrict__ mu, const double* __restrict__ inva,
@@ -689,15 +691,53 @@ __global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int
         int kend;
         if (!masked && !diag && ifull && cntJ == kTile &&
             tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
-            for (int k = 0; k < kTile; k++) {
-                const int jj = (lane + k) & (kTile - 1);
-                ElecSite other;
-                load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
-                elec_sym_step<MUTUAL, false>(me, my_orig, other, s_o[jj], true, 1.0, 1.0, PBC, box, acc, jacc, en);
-                jacc.fx = rot1(jacc.fx); jacc.fy = rot1(jacc.fy); jacc.fz = rot1(jacc.fz);
-                jacc.tx = rot1(jacc.tx); jacc.ty = rot1(jacc.ty); jacc.tz = rot1(jacc.tz);
+            if (MIXED) {
+                const ElecSiteT<float> m32 = {(float)me.q, (float)me.dx, (float)me.dy, (float)me.dz, (float)me.ux,
+                                              (float)me.uy, (float)me.uz, 0.f, 0.f};
+                float ifx = 0.f, ify = 0.f, ifz = 0.f, itx = 0.f, ity = 0.f, itz = 0.f, e32 = 0.f;
+                float jfx = 0.f, jfy = 0.f, jfz = 0.f, jtx = 0.f, jty = 0.f, jtz = 0.f;
+                for (int k = 0; k < kTile; k++) {
+                    const int jj = (lane + k) & (kTile - 1);
+                    const double4 pj = s_p[jj], qj = s_q[jj], uj = s_u[jj];
+                    const bool lo = my_orig < s_o[jj];
+                    const double ax = lo ? me.x : pj.x, ay = lo ? me.y : pj.y, az = lo ? me.z : pj.z;
+                    const double bx = lo ? pj.x : me.x, by = lo ? pj.y : me.y, bz = lo ? pj.z : me.z;
+                    double ex = bx - ax, ey = by - ay, ez = bz - az;
+                    if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                    const ElecSiteT<float> o32 = {(float)qj.w, (float)qj.x, (float)qj.y, (float)qj.z, (float)uj.x,
+                                                  (float)uj.y, (float)uj.z, 0.f, 0.f};
+                    ElecOutT<float> o;
+                    elec_pair_t<float, MUTUAL, false>(lo ? m32 : o32, lo ? o32 : m32, (float)ex, (float)ey, (float)ez,
+                                                      me.y != pj.y || me.z != pj.z, 1.f, 1.f, o);
+                    const float sg = lo ? -1.f : 1.f;   // force on this lane's particle = sg f
+                    ifx += sg * o.fx; ify += sg * o.fy; ifz += sg * o.fz;
+                    itx += lo ? o.tix : o.tkx; ity += lo ? o.tiy : o.tky; itz += lo ? o.tiz : o.tkz;
+                    e32 += o.e;
+                    // j side lagging one step (rotation fused into the add)
+                    jfx = rot1(jfx) - sg * o.fx;
+                    jfy = rot1(jfy) - sg * o.fy;
+                    jfz = rot1(jfz) - sg * o.fz;
+                    jtx = rot1(jtx) + (lo ? o.tkx : o.tix);
+                    jty = rot1(jty) + (lo ? o.tky : o.tiy);
+                    jtz = rot1(jtz) + (lo ? o.tkz : o.tiz);
+                }
+                acc.fx += ifx; acc.fy += ify; acc.fz += ifz;
+                acc.tx += itx; acc.ty += ity; acc.tz += itz;
+                en += e32;
+                jacc.fx = jfx; jacc.fy = jfy; jacc.fz = jfz;
+                jacc.tx = jtx; jacc.ty = jty; jacc.tz = jtz;
+                kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
+            } else {
+                for (int k = 0; k < kTile; k++) {
+                    const int jj = (lane + k) & (kTile - 1);
+                    ElecSite other;
+                    load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
+                    elec_sym_step<MUTUAL, false>(me, my_orig, other, s_o[jj], true, 1.0, 1.0, PBC, box, acc, jacc, en);
+                    jacc.fx = rot1(jacc.fx); jacc.fy = rot1(jacc.fy); jacc.fz = rot1(jacc.fz);
+                    jacc.tx = rot1(jacc.tx); jacc.ty = rot1(jacc.ty); jacc.tz = rot1(jacc.tz);
+                }
+                kend = kTile;
             }
-            kend = kTile;
         } else {
             const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
             for (int k = k0; k <= k1; k++) {
@@ -1363,17 +1403,26 @@ void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, c
     launch_col_reduce(s, L, Pi, Pj, nullptr, out, nullptr);
 }
 
-void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, const double4* posa, const double4* qd,
+void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed, const double4* posa, const double4* qd,
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
                      double* PjT, double* part_e, double* frc, double4* torque) {
     BoxT64 b = mkbox(L.box);
-#define TDP_SELEC(P, M)                                                                                              \
-    hipLaunchKernelGGL((k_sym_elec<P, M>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, mu, inva, \
-                       orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, PiF, PjF, PiT, PjT, L.slab, part_e)
-    if (L.pbc) {
-        if (mutual) TDP_SELEC(true, true); else TDP_SELEC(true, false);
+#define TDP_SELEC(P, M, X)                                                                                           \
+    hipLaunchKernelGGL((k_sym_elec<P, M, X>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, mu, \
+                       inva, orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, PiF, PjF, PiT, PjT, L.slab,   \
+                       part_e)
+    if (mixed) {
+        if (L.pbc) {
+            if (mutual) TDP_SELEC(true, true, true); else TDP_SELEC(true, false, true);
+        } else {
+            if (mutual) TDP_SELEC(false, true, true); else TDP_SELEC(false, false, true);
+        }
     } else {
-        if (mutual) TDP_SELEC(false, true); else TDP_SELEC(false, false);
+        if (L.pbc) {
+            if (mutual) TDP_SELEC(true, true, false); else TDP_SELEC(true, false, false);
+        } else {
+            if (mutual) TDP_SELEC(false, true, false); else TDP_SELEC(false, false, false);
+        }
     }
 #undef TDP_SELEC
     launch_col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc);

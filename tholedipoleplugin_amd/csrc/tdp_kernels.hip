@@ -1070,14 +1070,22 @@ __global__ __launch_bounds__(64) void k_sum_eps(const int4* __restrict__ seg, co
     if (threadIdx.x == 0) scalars[SC_COUNT * blockIdx.x + SC_EPS] = sqrt(sum / sg.z);
 }
 
-// Energy of system s: the pair-kernel partials of its work items [w0, w1), fixed order.
-__global__ __launch_bounds__(64) void k_reduce_energy(const int2* __restrict__ segw, const double* __restrict__ part_e,
-                                                      double* __restrict__ out) {
+// Energy of system s: the pair-kernel partials of its work items [w0, w1) in a fixed order
+// (strided per thread, then a tree over the 256 threads).
+__global__ __launch_bounds__(256) void k_reduce_energy(const int2* __restrict__ segw, const double* __restrict__ part_e,
+                                                       double* __restrict__ out) {
+    __shared__ double sh[256];
     const int2 w = segw[blockIdx.x];
+    const int t = threadIdx.x;
     double a = 0.0;
-    for (int k = w.x + (int)threadIdx.x; k < w.y; k += kTile) a += part_e[k];
-    a = wave_sum(a);
-    if (threadIdx.x == 0) out[blockIdx.x] = a;
+    for (int k = w.x + t; k < w.y; k += 256) a += part_e[k];
+    sh[t] = a;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (t < h) sh[t] += sh[t + h];
+        __syncthreads();
+    }
+    if (t == 0) out[blockIdx.x] = sh[0];
 }
 
 // ------------------------------------------------------------------ torque -> force (a10)
@@ -1480,7 +1488,7 @@ void launch_extrap_error(hipStream_t s, const SegLaunch& G, const double4* part,
 }
 
 void launch_reduce_energy(hipStream_t s, const SegLaunch& G, const double* part_e, double* out) {
-    hipLaunchKernelGGL(k_reduce_energy, dim3(G.nseg), dim3(kTile), 0, s, G.segw, part_e, out);
+    hipLaunchKernelGGL(k_reduce_energy, dim3(G.nseg), dim3(256), 0, s, G.segw, part_e, out);
 }
 
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,

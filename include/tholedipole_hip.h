@@ -46,7 +46,7 @@ extern "C" {
 #define TDP_EXTRAPOLATED 2
 /* precision */
 #define TDP_PRECISION_DOUBLE 0       /* every pair term in FP64 */
-#define TDP_PRECISION_MIXED 1        /* T*mu matvecs in FP32 with FP64 accumulation; FP64 elsewhere */
+#define TDP_PRECISION_MIXED 1        /* far tile pairs of T*mu and energy/force in FP32, FP64 sums; FP64 elsewhere */
 /* solver for Mutual polarization */
 #define TDP_SOLVER_JACOBI 0          /* the reference's fixed-point iteration and stopping rule */
 #define TDP_SOLVER_PCG 1             /* Jacobi-preconditioned conjugate gradients, same stopping rule */

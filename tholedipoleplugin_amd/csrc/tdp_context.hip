@@ -70,7 +70,8 @@ struct tdp_context {
     double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
             *q = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
-    float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr;
+    float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr, *posl = nullptr;
+    double4* tb_o = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
     int nwork = 0;
     size_t ntri = 0;
@@ -477,6 +478,8 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.exc_mask = c->d_exc_mask;
     L.tb_c = c->tb_c;
     L.tb_h = c->tb_h;
+    L.tb_o = c->tb_o;
+    L.posl = c->posl;
     L.nwork = c->nwork;
     L.work = c->d_work;
     L.row_start = c->d_row_start;
@@ -618,7 +621,7 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
     SegLaunch G = seg_launch(c);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
-    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
     launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     {
         double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
@@ -948,7 +951,8 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_energy, ns)) || (rc = dalloc(c, &c->d_axis, npad)) ||
         (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
-        (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)))
+        (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)) ||
+        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
         c->hist_order = std::max(4, (int)c->extrap.size());

@@ -55,6 +55,8 @@ struct PairLaunch {
     const unsigned long long* exc_mask;   // [ntiles][3][64]
     const float4* tb_c;                   // [nblk] tile bounding-box centre, largest Thole parameter
     const float4* tb_h;                   // [nblk] tile bounding-box half extent, smallest Thole parameter
+    const double4* tb_o;                  // [nblk] tile origin (FP64) of the FP32 far-field paths
+    const float4* posl;                   // [npad] particle offset from its tile origin (FP32)
     // upper-triangle sweep: work item w = tiles (I, J0..J1) as int4 (I, J0, J1, 0); items of tile row I
     // are row_start[I] .. row_start[I+1]
     int nwork;
@@ -73,8 +75,9 @@ void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, co
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
                              const float4* mu32, const double* inva, const double* scalars, double4* Pi, double* Pj);
 
+// tb_o: tile origins (FP64), posl: FP32 offsets of the particles from their tile origin
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
-                        float4* tb_c, float4* tb_h);
+                        float4* tb_c, float4* tb_h, double4* tb_o, float4* posl);
 
 // posa: FP64 positions (internal order) and Thole parameter; slots with perm < 0 are padding.
 void launch_load_positions(hipStream_t s, int npad, const double* pos3, const int* perm, const double* thole,

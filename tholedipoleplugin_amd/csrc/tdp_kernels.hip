@@ -108,37 +108,49 @@ __global__ void k_load_positions(int npad, const double* __restrict__ pos3, cons
 }
 
 // ------------------------------------------------------------------ tile classification
-// Per tile of 64 particles (internal order): centre and half extent of its bounding box
-// (PBC: offsets from the tile's first particle wrapped to the minimum image, so a tile that
-// straddles the box edge stays compact), and the largest/smallest Thole parameter.
-__global__ void k_tile_bounds(const int* __restrict__ tcnt, int nblk, const double4* __restrict__ posa, double bx,
-                              double by, double bz, int pbc, float4* __restrict__ tb_c, float4* __restrict__ tb_h) {
-    int T = blockIdx.x * blockDim.x + threadIdx.x;
-    if (T >= nblk) return;
-    int b = T * kTile, e = b + tcnt[T];
-    double4 p0 = posa[b];
-    double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-    double amax = p0.w, amin = p0.w;
-    for (int i = b; i < e; i++) {
-        double4 p = posa[i];
-        double d[3] = {p.x - p0.x, p.y - p0.y, p.z - p0.z};
-        if (pbc) {
-            const double L[3] = {bx, by, bz};
-            for (int k = 0; k < 3; k++)
-                if (L[k] > 0) d[k] -= L[k] * floor(d[k] / L[k] + 0.5);
-        }
-        for (int k = 0; k < 3; k++) {
-            lo[k] = fmin(lo[k], d[k]);
-            hi[k] = fmax(hi[k], d[k]);
-        }
-        amax = fmax(amax, p.w);
-        amin = fmin(amin, p.w);
+// Per tile of 64 particles (internal order), one wavefront: centre and half extent of its bounding
+// box (PBC: offsets from the tile's first particle wrapped to the minimum image, so a tile that
+// straddles the box edge stays compact) and the largest/smallest Thole parameter.  Also the tile
+// frame of the FP32 far-field paths: origin o = first particle + box centre (FP64) and each
+// particle's offset u from it in FP32, so that x = o + u + n L with n the integer wrap of the
+// particle's offset from the first particle (|u| <= half extent per axis).
+__global__ __launch_bounds__(64) void k_tile_bounds(const int* __restrict__ tcnt, const double4* __restrict__ posa,
+                                                    double bx, double by, double bz, int pbc,
+                                                    float4* __restrict__ tb_c, float4* __restrict__ tb_h,
+                                                    double4* __restrict__ tb_o, float4* __restrict__ posl) {
+    const int T = blockIdx.x, lane = threadIdx.x;
+    const int i = T * kTile + lane;
+    const bool valid = lane < tcnt[T];
+    const double4 p0 = posa[T * kTile];
+    const double4 p = posa[i];
+    double d[3] = {p.x - p0.x, p.y - p0.y, p.z - p0.z};
+    if (pbc) {
+        const double L[3] = {bx, by, bz};
+        for (int k = 0; k < 3; k++)
+            if (L[k] > 0) d[k] -= L[k] * floor(d[k] / L[k] + 0.5);
     }
-    tb_c[T] = make_float4((float)(p0.x + 0.5 * (lo[0] + hi[0])), (float)(p0.y + 0.5 * (lo[1] + hi[1])),
-                          (float)(p0.z + 0.5 * (lo[2] + hi[2])), (float)amax);
-    // half extents rounded up so the box still contains every particle after the float conversion
-    tb_h[T] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0001f + 1e-6f, (float)(0.5 * (hi[1] - lo[1])) * 1.0001f + 1e-6f,
-                          (float)(0.5 * (hi[2] - lo[2])) * 1.0001f + 1e-6f, (float)amin);
+    double lo[3], hi[3];
+    for (int k = 0; k < 3; k++) lo[k] = hi[k] = valid ? d[k] : 0.0;   // the first particle has d = 0
+    double amax = valid ? p.w : p0.w, amin = amax;
+    for (int m = 1; m < kTile; m <<= 1) {
+        for (int k = 0; k < 3; k++) {
+            lo[k] = fmin(lo[k], __shfl_xor(lo[k], m));
+            hi[k] = fmax(hi[k], __shfl_xor(hi[k], m));
+        }
+        amax = fmax(amax, __shfl_xor(amax, m));
+        amin = fmin(amin, __shfl_xor(amin, m));
+    }
+    const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    posl[i] = valid ? make_float4((float)(d[0] - c[0]), (float)(d[1] - c[1]), (float)(d[2] - c[2]), 0.f)
+                    : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane == 0) {
+        tb_c[T] = make_float4((float)(p0.x + c[0]), (float)(p0.y + c[1]), (float)(p0.z + c[2]), (float)amax);
+        // half extents rounded up so the box still contains every particle after the float conversion
+        tb_h[T] = make_float4((float)(0.5 * (hi[0] - lo[0])) * 1.0001f + 1e-6f,
+                              (float)(0.5 * (hi[1] - lo[1])) * 1.0001f + 1e-6f,
+                              (float)(0.5 * (hi[2] - lo[2])) * 1.0001f + 1e-6f, (float)amin);
+        tb_o[T] = make_double4(p0.x + c[0], p0.y + c[1], p0.z + c[2], 0.0);
+    }
 }
 
 // True when every pair of tiles I and J lies beyond the Thole range, i.e. the reference's
@@ -340,6 +352,103 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
     Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
 
+// ------------------------------------------------------------------ FP32 far-field frames
+// For tiles I and J with origins oI, oJ (k_tile_bounds) let D = oJ - oI, reduced by the minimum-image
+// rule.  The separation of a pair (i, j) is then D + u_j - u_i up to whole boxes, with |u| bounded by
+// the tile half extents hI, hJ.  Per axis:
+//  - constant: |D| + hI + hJ < L/2.  Every pair takes the image of D, exactly as the FP64 rule
+//    d - L floor(d/L + 1/2) applied to the raw separation would pick it (the margin between the pair
+//    and the half box exceeds the rounding of the rule), and the imaged separation is u_j - s_i with
+//    the per-lane constant s_i = u_i - D (one FP32 subtraction per pair).
+//  - straddling: the pairs may fall on either side of the half box.  With h = copysign(L/2, D) and
+//    s_i = u_i - (D - h), y = u_j - s_i is the signed distance of the pair past the half box, and
+//    the imaged separation is y - copysign(L/2, y).  Pairs with |y| inside a band that exceeds the
+//    FP32 rounding of y cannot be decided in FP32: the tile pair is then redone from FP64
+//    positions (rare: a pair within ~1e-6 nm of half a box).
+//  - general (both boxes together span half a box or more): FP64 separations for the tile pair.
+// Non-periodic systems have constant frames only.
+struct FarFrame {
+    double shx, shy, shz;   // per-lane constant: s_i = u_i - sh
+    float hx, hy, hz;       // L/2 of the straddling axes
+    float bx, by, bz;       // undecidable band of the straddling axes
+    int smask;              // straddling axes (bit 0 x, 1 y, 2 z); -1: general
+};
+
+__device__ __forceinline__ double frame_axis(double D, float hI, float hJ, double L, double iL, bool pbc, int bit,
+                                             FarFrame& f, float& h, float& band) {
+    h = 0.f;
+    band = 0.f;
+    if (!pbc) return D;
+    D -= L * floor(D * iL + 0.5);
+    const double w = (double)hI + (double)hJ;
+    if (w >= 0.5 * L) {
+        f.smask = -1;
+        return D;
+    }
+    if (fabs(D) + w < 0.5 * L) return D;
+    if (f.smask >= 0) f.smask |= bit;
+    h = (float)(0.5 * L);
+    band = (float)((2.0 * hI + hJ + 1.0) * 0x1p-20);
+    return D - copysign(0.5 * L, D);
+}
+
+__device__ __forceinline__ FarFrame far_frame(double4 oI, double4 oJ, float4 hI, float4 hJ, bool pbc, const BoxT64& box) {
+    FarFrame f;
+    f.smask = 0;
+    f.shx = frame_axis(oJ.x - oI.x, hI.x, hJ.x, box.bx, box.ibx, pbc, 1, f, f.hx, f.bx);
+    f.shy = frame_axis(oJ.y - oI.y, hI.y, hJ.y, box.by, box.iby, pbc, 2, f, f.hy, f.by);
+    f.shz = frame_axis(oJ.z - oI.z, hI.z, hJ.z, box.bz, box.ibz, pbc, 4, f, f.hz, f.bz);
+    return f;
+}
+
+// imaged FP32 separation component on a constant (S = false) or straddling (S = true) axis
+template <bool S>
+__device__ __forceinline__ float far_sep(float uj, float si, float h, float band, bool& amb) {
+    float d = uj - si;
+    if (S) {
+        amb = amb || (fabsf(d) < band);
+        d -= copysignf(h, d);
+    }
+    return d;
+}
+
+// Far tile pair of T * mu in FP32 on frame SM (straddling-axis mask): s_l, s_u hold the j tile's
+// offsets and FP32 dipoles twice over (slot (lane + k) mod 64 is entry lane + k).  i side as two
+// sums per component (sum udj*d and sum rr3*u_j), j side lagging one step.
+template <int SM>
+__device__ __forceinline__ void far_matvec_frame(const float4* __restrict__ s_l, const float4* __restrict__ s_u,
+                                                 int lane, float si_x, float si_y, float si_z, const FarFrame& f,
+                                                 float4 uf, float3 uf3, float& tx, float& ty, float& tz, float& sx3,
+                                                 float& sy3, float& sz3, float& fx, float& fy, float& fz, bool& amb) {
+    const float hx = f.hx, hy = f.hy, hz = f.hz, bx = f.bx, by = f.by, bz = f.bz;
+#pragma unroll 4
+    for (int k = 0; k < kTile; k++) {
+        const int jj = lane + k;
+        const float4 pj = s_l[jj];
+        const float4 uj = s_u[jj];
+        const float dx = far_sep<(SM & 1) != 0>(pj.x, si_x, hx, bx, amb);
+        const float dy = far_sep<(SM & 2) != 0>(pj.y, si_y, hy, by, amb);
+        const float dz = far_sep<(SM & 4) != 0>(pj.z, si_z, hz, bz, amb);
+        float r2 = dx * dx + dy * dy + dz * dz;
+        float rinv = rsqrt_t(r2);
+        float rinv2 = rinv * rinv;
+        float rr3 = rinv * rinv2;
+        float rr5 = rr3 * rinv2;   // 1/r^5: the factor 3 sits in uf3 and in the flush of tx..tz
+        float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+        float udi = rr5 * (uf3.x * dx + uf3.y * dy + uf3.z * dz);
+        tx += udj * dx;
+        ty += udj * dy;
+        tz += udj * dz;
+        sx3 += rr3 * uj.x;
+        sy3 += rr3 * uj.y;
+        sz3 += rr3 * uj.z;
+        // before step k, lane l holds slot (l + k - 1); the rotation and the add form one DPP instruction
+        fx = rot1(fx) + (udi * dx - rr3 * uf.x);
+        fy = rot1(fy) + (udi * dy - rr3 * uf.y);
+        fz = rot1(fz) + (udi * dz - rr3 * uf.z);
+    }
+}
+
 // ------------------------------------------------------------------ T * mu, mixed precision (a7)
 // Tile pairs inside the Thole range, and the diagonal and exclusion tiles, are evaluated in FP64
 // as in k_sym_matvec.  Tile pairs beyond it, most of the pairs of a large system, carry only the
@@ -349,7 +458,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
 // largest terms, so keeping them in FP64 holds the operator's error to that of the FP32 far field;
 // all partial sums are FP64.
 template <bool PBC>
-__global__ __launch_bounds__(64) void k_sym_matvec_mixed(TileMap tm, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_sym_matvec_mixed(TileMap tm, int nblk, const int4* __restrict__ work,
                                                          const double4* __restrict__ posa,
                                                          const double4* __restrict__ mu,
                                                          const float4* __restrict__ mu32,
@@ -358,11 +467,14 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(TileMap tm, int nblk, c
                                                          const int* __restrict__ exc_J,
                                                          const unsigned long long* __restrict__ exc_mask,
                                                          const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
-                                                         BoxT64 box, const double* __restrict__ scalars,
+                                                         const double4* __restrict__ tb_o,
+                                                         const float4* __restrict__ posl, BoxT64 box, const double* __restrict__ scalars,
                                                          double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     // One LDS area holds either the near tile (FP64 dipoles and 1/a) or the far tile (FP32
     // dipoles).  The far tile is staged twice over, so that slot (lane + k) mod 64 is entry
     // lane + k: the step's address is the lane's base plus a constant offset.
+    // FP32 frames: offsets s_l (float4) at the start, dipoles s_uf behind the FP64 positions s_pf of
+    // the general far path, so a fallback from a frame to FP64 positions restages only s_pf.
     constexpr int kBytesNear = kTile * (2 * sizeof(double4) + sizeof(double));
     constexpr int kBytesFar = 2 * kTile * (sizeof(double4) + sizeof(float4));
     __shared__ __align__(16) unsigned char s_raw[kBytesNear > kBytesFar ? kBytesNear : kBytesFar];
@@ -371,6 +483,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(TileMap tm, int nblk, c
     double* s_ia = reinterpret_cast<double*>(s_u + kTile);
     double4* s_pf = reinterpret_cast<double4*>(s_raw);
     float4* s_uf = reinterpret_cast<float4*>(s_pf + 2 * kTile);
+    float4* s_l = reinterpret_cast<float4*>(s_raw);
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
@@ -384,6 +497,8 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(TileMap tm, int nblk, c
     const float4 uf = mu32[i];
     const float3 uf3 = make_float3(3.f * uf.x, 3.f * uf.y, 3.f * uf.z);
     const float4 cI = tb_c[I], hI = tb_h[I];
+    const double4 oI = tb_o[I];
+    const float4 li = posl[i];
     const bool ifull = cntI == kTile;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int e = exc_start[I];
@@ -406,43 +521,74 @@ __global__ __launch_bounds__(64) void k_sym_matvec_mixed(TileMap tm, int nblk, c
         int kend;
         __syncthreads();
         if (far) {
+            const FarFrame fr = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box);
             {
-                const double4 p = posa[jb + lane];
                 const float4 u = mu32[jb + lane];
-                s_pf[lane] = p;
-                s_pf[lane + kTile] = p;
                 s_uf[lane] = u;
                 s_uf[lane + kTile] = u;
             }
-            __syncthreads();
-            // i side as two sums per component (sum udj*d and sum rr3*u_j), j side lagging one step
             float tx = 0.f, ty = 0.f, tz = 0.f, sx3 = 0.f, sy3 = 0.f, sz3 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+            bool general = fr.smask < 0;
+            if (!general) {
+                const float4 l = posl[jb + lane];
+                s_l[lane] = l;
+                s_l[lane + kTile] = l;
+                __syncthreads();
+                const float si_x = (float)((double)li.x - fr.shx), si_y = (float)((double)li.y - fr.shy),
+                            si_z = (float)((double)li.z - fr.shz);
+                bool amb = false;
+#define TDP_FAR_CASE(M)                                                                                        \
+    case M:                                                                                                    \
+        far_matvec_frame<M>(s_l, s_uf, lane, si_x, si_y, si_z, fr, uf, uf3, tx, ty, tz, sx3, sy3, sz3, fx, fy, \
+                            fz, amb);                                                                          \
+        break;
+                switch (fr.smask) {
+                    TDP_FAR_CASE(0)
+                    TDP_FAR_CASE(1)
+                    TDP_FAR_CASE(2)
+                    TDP_FAR_CASE(3)
+                    TDP_FAR_CASE(4)
+                    TDP_FAR_CASE(5)
+                    TDP_FAR_CASE(6)
+                    TDP_FAR_CASE(7)
+                }
+#undef TDP_FAR_CASE
+                if (__builtin_amdgcn_ballot_w64(amb)) {
+                    general = true;
+                    tx = ty = tz = sx3 = sy3 = sz3 = fx = fy = fz = 0.f;
+                    __syncthreads();
+                }
+            }
+            if (general) {
+                const double4 p = posa[jb + lane];
+                s_pf[lane] = p;
+                s_pf[lane + kTile] = p;
+                __syncthreads();
 #pragma unroll 4
-            for (int k = 0; k < kTile; k++) {
-                const int jj = lane + k;
-                const double4 pj = s_pf[jj];
-                const float4 uj = s_uf[jj];
-                double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-                if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-                const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
-                float r2 = dx * dx + dy * dy + dz * dz;
-                float rinv = rsqrt_t(r2);
-                float rinv2 = rinv * rinv;
-                float rr3 = rinv * rinv2;
-                float rr5 = rr3 * rinv2;   // 1/r^5: the factor 3 sits in uf3 and in the flush of tx..tz
-                float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
-                float udi = rr5 * (uf3.x * dx + uf3.y * dy + uf3.z * dz);
-                tx += udj * dx;
-                ty += udj * dy;
-                tz += udj * dz;
-                sx3 += rr3 * uj.x;
-                sy3 += rr3 * uj.y;
-                sz3 += rr3 * uj.z;
-                // j side, lagging one step: before step k, lane l holds slot (l + k - 1); the
-                // rotation and the add form one DPP instruction
-                fx = rot1(fx) + (udi * dx - rr3 * uf.x);
-                fy = rot1(fy) + (udi * dy - rr3 * uf.y);
-                fz = rot1(fz) + (udi * dz - rr3 * uf.z);
+                for (int k = 0; k < kTile; k++) {
+                    const int jj = lane + k;
+                    const double4 pj = s_pf[jj];
+                    const float4 uj = s_uf[jj];
+                    double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+                    if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                    const float dx = (float)ex, dy = (float)ey, dz = (float)ez;
+                    float r2 = dx * dx + dy * dy + dz * dz;
+                    float rinv = rsqrt_t(r2);
+                    float rinv2 = rinv * rinv;
+                    float rr3 = rinv * rinv2;
+                    float rr5 = rr3 * rinv2;
+                    float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+                    float udi = rr5 * (uf3.x * dx + uf3.y * dy + uf3.z * dz);
+                    tx += udj * dx;
+                    ty += udj * dy;
+                    tz += udj * dz;
+                    sx3 += rr3 * uj.x;
+                    sy3 += rr3 * uj.y;
+                    sz3 += rr3 * uj.z;
+                    fx = rot1(fx) + (udi * dx - rr3 * uf.x);
+                    fy = rot1(fy) + (udi * dy - rr3 * uf.y);
+                    fz = rot1(fz) + (udi * dz - rr3 * uf.z);
+                }
             }
             ax += 3.0 * (double)tx - (double)sx3;
             ay += 3.0 * (double)ty - (double)sy3;
@@ -1361,9 +1507,9 @@ void launch_load_positions(hipStream_t s, int npad, const double* pos3, const in
 }
 
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
-                        float4* tb_c, float4* tb_h) {
-    hipLaunchKernelGGL(k_tile_bounds, dim3((nblk + 63) / 64), dim3(64), 0, s, tcnt, nblk, posa, box3[0], box3[1],
-                       box3[2], pbc ? 1 : 0, tb_c, tb_h);
+                        float4* tb_c, float4* tb_h, double4* tb_o, float4* posl) {
+    hipLaunchKernelGGL(k_tile_bounds, dim3(nblk), dim3(kTile), 0, s, tcnt, posa, box3[0], box3[1], box3[2],
+                       pbc ? 1 : 0, tb_c, tb_h, tb_o, posl);
 }
 
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
@@ -1393,10 +1539,12 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
-                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
+                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
+                           Pi, Pj, L.slab);
     else
         hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
-                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
+                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
+                           Pi, Pj, L.slab);
 }
 
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,

@@ -343,6 +343,48 @@ def test_half_box_images(precision):
     assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
 
 
+def dipolar_water(kind):
+    """Waters with molecular dipoles and axis frames, large enough for tile pairs beyond the Thole
+    range: a periodic box of 512 waters at liquid density (2.48 nm) or a 200-molecule cluster.  O:
+    Bisector (Z = H1, X = H2), H: ZThenX (Z = O, X = the other H); dipoles drawn per site type."""
+    from tholedipoleplugin_amd.context import System
+    if kind == "box":
+        s = systems.water_box(8, 512, 2.48, 31, method=T.PME)
+    else:
+        s = systems.water_cluster(200, seed=32)
+    f = s.force
+    rng = np.random.default_rng(33)
+    dO = rng.uniform(-0.01, 0.01, 3)
+    dH = rng.uniform(-0.005, 0.005, 3)
+    for m in range(f.getNumParticles() // 3):
+        o, h1, h2 = 3 * m, 3 * m + 1, 3 * m + 2
+        for i, d, ax, z, x in ((o, dO, T.Bisector, h1, h2), (h1, dH, T.ZThenX, o, h2), (h2, dH, T.ZThenX, o, h1)):
+            q, _, a, th = f.getParticleParameters(i)[:4]
+            f.setParticleParameters(i, q, list(d), a, th, ax, z, x, -1)
+    system = System()
+    for mass in s.masses:
+        system.addParticle(mass)
+    if s.box is not None:
+        system.setDefaultPeriodicBoxVectors(*s.box)
+    system.addForce(f)
+    return ref_systems.RefSystem("dipolar_" + kind, system, f, np.asarray(s.positions, np.float64), s.box)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("kind", ["box", "cluster"])
+def test_permanent_dipoles_far_tiles(kind, precision):
+    """Permanent dipoles, axis frames and torques on systems with far tile pairs (the FP32 far path
+    of the mixed energy pass and T*mu, FP64 otherwise) against the oracle at the parity bar."""
+    _say("case dipolar %s %s" % (kind, precision))
+    rs = tighten(dipolar_water(kind), eps=1e-12)
+    ctx, st, k = gpu_eval(rs, precision=precision, solver="pcg")
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box, target_eps=1e-13, max_iter=5000)
+    assert k.solverStats()["converged"] == 1
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= E_TOL * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+    assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
+
+
 def test_water12288_mixed_matches_double():
     """36,864 sites: the mixed-precision solve agrees with the FP64 one at every site (induced
     dipoles within 1e-6 D, forces within the force tolerance, energy within 1e-6), both solved

@@ -70,6 +70,7 @@ struct tdp_context {
     double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
             *q = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
+    bool has_perm = true;   // some particle has a nonzero molecular dipole
     float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr, *posl = nullptr;
     double4* tb_o = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
@@ -342,6 +343,7 @@ int upload_particles(tdp_context* c) {
     std::vector<int4> axis(npad, make_int4(5, -1, -1, -1));
     std::vector<int> perm(npad, -1);
     auto ren = [c](int a) { return a >= 0 ? c->inv[a] : a; };
+    bool has_perm = false;
     (void)n;
     for (int k = 0; k < npad; k++) {
         int i = c->perm[k];
@@ -351,7 +353,10 @@ int upload_particles(tdp_context* c) {
         thole[k] = c->thole[i];
         charge[k] = c->charge[i];
         inva[k] = c->thole[i] != 0.0 ? 1.0 / c->thole[i] : 0.0;
-        for (int d = 0; d < 3; d++) dmol[3 * k + d] = c->dmol[3 * i + d];
+        for (int d = 0; d < 3; d++) {
+            dmol[3 * k + d] = c->dmol[3 * i + d];
+            if (dmol[3 * k + d] != 0.0) has_perm = true;
+        }
         axis[k] = make_int4(c->axis_type[i], ren(c->atom_z[i]), ren(c->atom_x[i]), ren(c->atom_y[i]));
     }
     TDP_HIP(c, hipMemcpyAsync(c->d_alpha, alpha.data(), npad * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -362,6 +367,7 @@ int upload_particles(tdp_context* c) {
     TDP_HIP(c, hipMemcpyAsync(c->d_axis, axis.data(), npad * sizeof(int4), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipMemcpyAsync(c->d_perm, perm.data(), npad * sizeof(int), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));  // the host arrays go out of scope
+    c->has_perm = has_perm;
     return TDP_OK;
 }
 
@@ -470,6 +476,7 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.tm.col = c->d_tcol;
     L.nblk = c->nblk;
     L.pbc = c->method == TDP_PME;
+    L.perm = c->has_perm;
     L.box[0] = box9 ? box9[0] : 0.0;
     L.box[1] = box9 ? box9[4] : 0.0;
     L.box[2] = box9 ? box9[8] : 0.0;
@@ -742,7 +749,7 @@ void forces_and_energy(tdp_context* c, const double* box) {
                     reinterpret_cast<double*>(pf + off), pt, reinterpret_cast<double*>(pt + off), c->d_part_e, c->d_frc,
                     c->torque);
     launch_reduce_energy(s, G, c->d_part_e, c->d_energy);
-    if (!c->gidx.empty()) {
+    if (!c->gidx.empty() && c->has_perm) {   // without permanent dipoles every torque is zero
         launch_map_torque(s, npad, c->posa, c->d_axis, c->torque, c->contrib);
         launch_gather_torque(s, npad, c->d_goff, c->d_gidx, c->contrib, c->d_frc);
     }

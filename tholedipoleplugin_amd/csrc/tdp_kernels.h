@@ -49,6 +49,7 @@ struct PairLaunch {
     int npad, nblk;
     TileMap tm;
     bool pbc;
+    bool perm;     // some particle has a permanent dipole (else the energy pass forms no torques)
     double box[3];
     const int* exc_start;                 // [nblk+1]
     const int* exc_J;                     // [ntiles]
@@ -86,6 +87,7 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
                    const int4* axis, double4* qd);
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
                       double4* Pi, double* Pj, double4* out);
+// Energy, forces (accumulated into frc, 3 doubles per slot) and, when L.perm, torques (torque).
 void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed, const double4* posa, const double4* qd,
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
                      double* PjT, double* part_e, double* frc, double4* torque);

@@ -643,7 +643,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // calculateFixedDipoleField (ReferenceTholeDipoleForce.cpp:715-733) with each pair once, in the
 // sweep of k_sym_matvec: Delta = r_j - r_i,
 //   E_i -= m [Delta (rr3 q_j - rr5 d_j.Delta) + rr3 d_j],  E_j += m [Delta (rr3 q_i + rr5 d_i.Delta) - rr3 d_i].
-template <bool PBC>
+template <bool PBC, bool PERM>
 __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const int4* __restrict__ work,
                                                   const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                   const double* __restrict__ inva, const int* __restrict__ exc_start,
@@ -701,15 +701,25 @@ __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const in
                 double rinv = rsqrt(r2);
                 double rinv2 = rinv * rinv;
                 double rr3 = rinv * rinv2;
-                double rr5 = 3.0 * rr3 * rinv2;
-                double fj = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
-                double fi = rr3 * qi.w + rr5 * (qi.x * dx + qi.y * dy + qi.z * dz);
-                ax -= dx * fj + qj.x * rr3;
-                ay -= dy * fj + qj.y * rr3;
-                az -= dz * fj + qj.z * rr3;
-                jx += dx * fi - qi.x * rr3;
-                jy += dy * fi - qi.y * rr3;
-                jz += dz * fi - qi.z * rr3;
+                if (PERM) {
+                    double rr5 = 3.0 * rr3 * rinv2;
+                    double fj = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
+                    double fi = rr3 * qi.w + rr5 * (qi.x * dx + qi.y * dy + qi.z * dz);
+                    ax -= dx * fj + qj.x * rr3;
+                    ay -= dy * fj + qj.y * rr3;
+                    az -= dz * fj + qj.z * rr3;
+                    jx += dx * fi - qi.x * rr3;
+                    jy += dy * fi - qi.y * rr3;
+                    jz += dz * fi - qi.z * rr3;
+                } else {   // charges only
+                    const double fj = rr3 * qj.w, fi = rr3 * qi.w;
+                    ax -= dx * fj;
+                    ay -= dy * fj;
+                    az -= dz * fj;
+                    jx += dx * fi;
+                    jy += dy * fi;
+                    jz += dz * fi;
+                }
                 jx = rot1(jx);
                 jy = rot1(jy);
                 jz = rot1(jz);
@@ -732,14 +742,24 @@ __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const in
                 const double m = ok ? scale : 0.0;
                 rr3 = ok ? rr3 : 0.0;
                 rr5 = ok ? rr5 : 0.0;
-                double fj = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
-                double fi = rr3 * qi.w + rr5 * (qi.x * dx + qi.y * dy + qi.z * dz);
-                ax -= (dx * fj + qj.x * rr3) * m;
-                ay -= (dy * fj + qj.y * rr3) * m;
-                az -= (dz * fj + qj.z * rr3) * m;
-                jx += (dx * fi - qi.x * rr3) * m;
-                jy += (dy * fi - qi.y * rr3) * m;
-                jz += (dz * fi - qi.z * rr3) * m;
+                if (PERM) {
+                    double fj = rr3 * qj.w - rr5 * (qj.x * dx + qj.y * dy + qj.z * dz);
+                    double fi = rr3 * qi.w + rr5 * (qi.x * dx + qi.y * dy + qi.z * dz);
+                    ax -= (dx * fj + qj.x * rr3) * m;
+                    ay -= (dy * fj + qj.y * rr3) * m;
+                    az -= (dz * fj + qj.z * rr3) * m;
+                    jx += (dx * fi - qi.x * rr3) * m;
+                    jy += (dy * fi - qi.y * rr3) * m;
+                    jz += (dz * fi - qi.z * rr3) * m;
+                } else {   // charges only
+                    const double fj = rr3 * qj.w, fi = rr3 * qi.w;
+                    ax -= (dx * fj) * m;
+                    ay -= (dy * fj) * m;
+                    az -= (dz * fj) * m;
+                    jx += (dx * fi) * m;
+                    jy += (dy * fi) * m;
+                    jz += (dz * fi) * m;
+                }
                 jx = rot1(jx);
                 jy = rot1(jy);
                 jz = rot1(jz);
@@ -759,7 +779,7 @@ struct ElecAcc {
     double fx, fy, fz, tx, ty, tz;
 };
 
-template <bool MUTUAL, bool DAMP>
+template <bool MUTUAL, bool DAMP, bool PERM>
 __device__ __forceinline__ void elec_sym_step(const ElecSite& me, int my_orig, const ElecSite& other, int other_orig,
                                               bool ok, double mS, double iS, bool pbc, const BoxT64& box, ElecAcc& a,
                                               ElecAcc& b, double& en) {
@@ -770,26 +790,98 @@ __device__ __forceinline__ void elec_sym_step(const ElecSite& me, int my_orig, c
     if (pbc) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
     if (!ok) { dx = 1.0; dy = 0.0; dz = 0.0; mS = 0.0; iS = 0.0; }
     ElecOut o;
-    elec_pair<MUTUAL, DAMP>(A, B, dx, dy, dz, mS, iS, o);
+    elec_pair<MUTUAL, DAMP, PERM>(A, B, dx, dy, dz, mS, iS, o);
     if (!ok) { o.e = o.fx = o.fy = o.fz = o.tix = o.tiy = o.tiz = o.tkx = o.tky = o.tkz = 0.0; }
     const double s = lo ? -1.0 : 1.0;   // force on this lane's particle = s f
     a.fx += s * o.fx; a.fy += s * o.fy; a.fz += s * o.fz;
     b.fx -= s * o.fx; b.fy -= s * o.fy; b.fz -= s * o.fz;
-    a.tx += lo ? o.tix : o.tkx; a.ty += lo ? o.tiy : o.tky; a.tz += lo ? o.tiz : o.tkz;
-    b.tx += lo ? o.tkx : o.tix; b.ty += lo ? o.tky : o.tiy; b.tz += lo ? o.tkz : o.tiz;
+    if (PERM) {
+        a.tx += lo ? o.tix : o.tkx; a.ty += lo ? o.tiy : o.tky; a.tz += lo ? o.tiz : o.tkz;
+        b.tx += lo ? o.tkx : o.tix; b.ty += lo ? o.tky : o.tiy; b.tz += lo ? o.tkz : o.tiz;
+    }
     en += o.e;
 }
 
-// MIXED: tile pairs beyond the Thole range evaluate the pair terms in FP32 (separations, images and
-// the frame's axis choice in FP64, sums per tile flushed to FP64); every other tile stays FP64.
-template <bool PBC, bool MUTUAL, bool MIXED>
-__global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int4* __restrict__ work,
+// Rotation of the j-side sums of the energy pass by one lane (torques only with permanent dipoles).
+template <bool PERM>
+__device__ __forceinline__ void rot_acc(ElecAcc& b) {
+    b.fx = rot1(b.fx); b.fy = rot1(b.fy); b.fz = rot1(b.fz);
+    if (PERM) { b.tx = rot1(b.tx); b.ty = rot1(b.ty); b.tz = rot1(b.tz); }
+}
+
+// FP32 far tile pair of the energy pass: tile frame (offsets s_l) when fr.smask >= 0, else FP64
+// separations from s_p.  Pair orientation, the frame's axis choice (raw y and z compared in FP64)
+// and the force/torque signs follow elec_sym_step.  Returns false when a straddling pair could not
+// be decided in FP32 (the caller then repeats the tile pair with FP64 separations).
+template <bool PBC, bool MUTUAL, bool PERM, bool FRAME>
+__device__ __forceinline__ bool elec_far32(const double4* __restrict__ s_p, const float4* __restrict__ s_l,
+                                           const float4* __restrict__ s_qf, const float4* __restrict__ s_uf,
+                                           const int* __restrict__ s_o, int lane, const ElecSite& me, int my_orig,
+                                           float si_x, float si_y, float si_z, const FarFrame& fr, const BoxT64& box,
+                                           ElecAcc& acc, ElecAcc& jacc, double& en) {
+    const ElecSiteT<float> m32 = {(float)me.q, (float)me.dx, (float)me.dy, (float)me.dz, (float)me.ux,
+                                  (float)me.uy, (float)me.uz, 0.f, 0.f};
+    float ifx = 0.f, ify = 0.f, ifz = 0.f, itx = 0.f, ity = 0.f, itz = 0.f, e32 = 0.f;
+    float jfx = 0.f, jfy = 0.f, jfz = 0.f, jtx = 0.f, jty = 0.f, jtz = 0.f;
+    bool amb = false;
+    for (int k = 0; k < kTile; k++) {
+        const int jj = (lane + k) & (kTile - 1);
+        const double4 pj = s_p[jj];
+        const bool lo = my_orig < s_o[jj];
+        float dx, dy, dz;   // x_j - x_i, minimum image
+        if (FRAME) {
+            const float4 lj = s_l[jj];
+            dx = (fr.smask & 1) ? far_sep<true>(lj.x, si_x, fr.hx, fr.bx, amb) : lj.x - si_x;
+            dy = (fr.smask & 2) ? far_sep<true>(lj.y, si_y, fr.hy, fr.by, amb) : lj.y - si_y;
+            dz = (fr.smask & 4) ? far_sep<true>(lj.z, si_z, fr.hz, fr.bz, amb) : lj.z - si_z;
+        } else {
+            double ex = pj.x - me.x, ey = pj.y - me.y, ez = pj.z - me.z;
+            if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+            dx = (float)ex;
+            dy = (float)ey;
+            dz = (float)ez;
+        }
+        const float4 qj = s_qf[jj], uj = s_uf[jj];
+        const ElecSiteT<float> o32 = {qj.w, qj.x, qj.y, qj.z, uj.x, uj.y, uj.z, 0.f, 0.f};
+        const float sg = lo ? -1.f : 1.f;   // force on this lane's particle = sg f; separation A -> B = -sg d
+        ElecOutT<float> o;
+        elec_pair_t<float, MUTUAL, false, PERM>(lo ? m32 : o32, lo ? o32 : m32, -sg * dx, -sg * dy, -sg * dz,
+                                                me.y != pj.y || me.z != pj.z, 1.f, 1.f, o);
+        ifx += sg * o.fx; ify += sg * o.fy; ifz += sg * o.fz;
+        e32 += o.e;
+        // j side lagging one step (rotation fused into the add)
+        jfx = rot1(jfx) - sg * o.fx;
+        jfy = rot1(jfy) - sg * o.fy;
+        jfz = rot1(jfz) - sg * o.fz;
+        if (PERM) {
+            itx += lo ? o.tix : o.tkx; ity += lo ? o.tiy : o.tky; itz += lo ? o.tiz : o.tkz;
+            jtx = rot1(jtx) + (lo ? o.tkx : o.tix);
+            jty = rot1(jty) + (lo ? o.tky : o.tiy);
+            jtz = rot1(jtz) + (lo ? o.tkz : o.tiz);
+        }
+    }
+    if (FRAME && __builtin_amdgcn_ballot_w64(amb)) return false;
+    acc.fx += ifx; acc.fy += ify; acc.fz += ifz;
+    acc.tx += itx; acc.ty += ity; acc.tz += itz;
+    en += e32;
+    jacc.fx = jfx; jacc.fy = jfy; jacc.fz = jfz;
+    jacc.tx = jtx; jacc.ty = jty; jacc.tz = jtz;
+    return true;
+}
+
+// MIXED: tile pairs beyond the Thole range evaluate the pair terms in FP32 (separations from the tile
+// frames, or FP64 separations where a frame cannot decide an image; the frame's axis choice compares
+// FP64 positions; sums per tile flushed to FP64); every other tile stays FP64.  PERM = false: the
+// system has no permanent dipoles, so no torques are formed or written.
+template <bool PBC, bool MUTUAL, bool MIXED, bool PERM>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_sym_elec(TileMap tm, int nblk, const int4* __restrict__ work,
                                                  const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                  const double4* __restrict__ mu, const double* __restrict__ inva,
                                                  const int* __restrict__ orig, const int* __restrict__ exc_start,
                                                  const int* __restrict__ exc_J,
                                                  const unsigned long long* __restrict__ exc_mask,
                                                  const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                 const double4* __restrict__ tb_o, const float4* __restrict__ posl,
                                                  BoxT64 box, double4* __restrict__ PiF, double* __restrict__ PjF,
                                                  double4* __restrict__ PiT, double* __restrict__ PjT, size_t S,
                                                  double* __restrict__ part_e) {
@@ -798,6 +890,9 @@ __global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int
     __shared__ double4 s_u[kTile];
     __shared__ double s_ia[kTile];
     __shared__ int s_o[kTile];
+    __shared__ float4 s_l[MIXED ? kTile : 1];
+    __shared__ float4 s_qf[MIXED ? kTile : 1];
+    __shared__ float4 s_uf[MIXED ? kTile : 1];
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
@@ -808,6 +903,8 @@ __global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int
     load_site(me, posa[i], qd[i], mu[i], inva[i]);
     const int my_orig = orig[i];
     const float4 cI = tb_c[I], hI = tb_h[I];
+    const double4 oI = tb_o[I];
+    const float4 li = posl[i];
     const bool ifull = cntI == kTile;
     ElecAcc acc = {0, 0, 0, 0, 0, 0};
     double en = 0.0;
@@ -838,49 +935,31 @@ __global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int
         if (!masked && !diag && ifull && cntJ == kTile &&
             tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
             if (MIXED) {
-                const ElecSiteT<float> m32 = {(float)me.q, (float)me.dx, (float)me.dy, (float)me.dz, (float)me.ux,
-                                              (float)me.uy, (float)me.uz, 0.f, 0.f};
-                float ifx = 0.f, ify = 0.f, ifz = 0.f, itx = 0.f, ity = 0.f, itz = 0.f, e32 = 0.f;
-                float jfx = 0.f, jfy = 0.f, jfz = 0.f, jtx = 0.f, jty = 0.f, jtz = 0.f;
-                for (int k = 0; k < kTile; k++) {
-                    const int jj = (lane + k) & (kTile - 1);
-                    const double4 pj = s_p[jj], qj = s_q[jj], uj = s_u[jj];
-                    const bool lo = my_orig < s_o[jj];
-                    const double ax = lo ? me.x : pj.x, ay = lo ? me.y : pj.y, az = lo ? me.z : pj.z;
-                    const double bx = lo ? pj.x : me.x, by = lo ? pj.y : me.y, bz = lo ? pj.z : me.z;
-                    double ex = bx - ax, ey = by - ay, ez = bz - az;
-                    if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-                    const ElecSiteT<float> o32 = {(float)qj.w, (float)qj.x, (float)qj.y, (float)qj.z, (float)uj.x,
-                                                  (float)uj.y, (float)uj.z, 0.f, 0.f};
-                    ElecOutT<float> o;
-                    elec_pair_t<float, MUTUAL, false>(lo ? m32 : o32, lo ? o32 : m32, (float)ex, (float)ey, (float)ez,
-                                                      me.y != pj.y || me.z != pj.z, 1.f, 1.f, o);
-                    const float sg = lo ? -1.f : 1.f;   // force on this lane's particle = sg f
-                    ifx += sg * o.fx; ify += sg * o.fy; ifz += sg * o.fz;
-                    itx += lo ? o.tix : o.tkx; ity += lo ? o.tiy : o.tky; itz += lo ? o.tiz : o.tkz;
-                    e32 += o.e;
-                    // j side lagging one step (rotation fused into the add)
-                    jfx = rot1(jfx) - sg * o.fx;
-                    jfy = rot1(jfy) - sg * o.fy;
-                    jfz = rot1(jfz) - sg * o.fz;
-                    jtx = rot1(jtx) + (lo ? o.tkx : o.tix);
-                    jty = rot1(jty) + (lo ? o.tky : o.tiy);
-                    jtz = rot1(jtz) + (lo ? o.tkz : o.tiz);
+                const FarFrame fr = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box);
+                {
+                    const double4 q = s_q[lane], u = s_u[lane];
+                    s_qf[lane] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
+                    s_uf[lane] = make_float4((float)u.x, (float)u.y, (float)u.z, 0.f);
+                    s_l[lane] = posl[jb + lane];
                 }
-                acc.fx += ifx; acc.fy += ify; acc.fz += ifz;
-                acc.tx += itx; acc.ty += ity; acc.tz += itz;
-                en += e32;
-                jacc.fx = jfx; jacc.fy = jfy; jacc.fz = jfz;
-                jacc.tx = jtx; jacc.ty = jty; jacc.tz = jtz;
+                __syncthreads();
+                const float si_x = (float)((double)li.x - fr.shx), si_y = (float)((double)li.y - fr.shy),
+                            si_z = (float)((double)li.z - fr.shz);
+                const bool done = fr.smask >= 0 &&
+                                  elec_far32<PBC, MUTUAL, PERM, true>(s_p, s_l, s_qf, s_uf, s_o, lane, me, my_orig,
+                                                                      si_x, si_y, si_z, fr, box, acc, jacc, en);
+                if (!done)
+                    elec_far32<PBC, MUTUAL, PERM, false>(s_p, s_l, s_qf, s_uf, s_o, lane, me, my_orig, si_x, si_y,
+                                                         si_z, fr, box, acc, jacc, en);
                 kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
             } else {
                 for (int k = 0; k < kTile; k++) {
                     const int jj = (lane + k) & (kTile - 1);
                     ElecSite other;
                     load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
-                    elec_sym_step<MUTUAL, false>(me, my_orig, other, s_o[jj], true, 1.0, 1.0, PBC, box, acc, jacc, en);
-                    jacc.fx = rot1(jacc.fx); jacc.fy = rot1(jacc.fy); jacc.fz = rot1(jacc.fz);
-                    jacc.tx = rot1(jacc.tx); jacc.ty = rot1(jacc.ty); jacc.tz = rot1(jacc.tz);
+                    elec_sym_step<MUTUAL, false, PERM>(me, my_orig, other, s_o[jj], true, 1.0, 1.0, PBC, box, acc, jacc,
+                                                       en);
+                    rot_acc<PERM>(jacc);
                 }
                 kend = kTile;
             }
@@ -895,18 +974,17 @@ __global__ __launch_bounds__(64) void k_sym_elec(TileMap tm, int nblk, const int
                 const double iS = zi ? 0.0 : 1.0;
                 ElecSite other;
                 load_site(other, s_p[jj], s_q[jj], s_u[jj], s_ia[jj]);
-                elec_sym_step<MUTUAL, true>(me, my_orig, other, s_o[jj], ok, mS, iS, PBC, box, acc, jacc, en);
-                jacc.fx = rot1(jacc.fx); jacc.fy = rot1(jacc.fy); jacc.fz = rot1(jacc.fz);
-                jacc.tx = rot1(jacc.tx); jacc.ty = rot1(jacc.ty); jacc.tz = rot1(jacc.tz);
+                elec_sym_step<MUTUAL, true, PERM>(me, my_orig, other, s_o[jj], ok, mS, iS, PBC, box, acc, jacc, en);
+                rot_acc<PERM>(jacc);
             }
             kend = k1 + 1;
         }
         const size_t slot = slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1));
         store3(PjF, S, slot, jacc.fx, jacc.fy, jacc.fz);
-        store3(PjT, S, slot, jacc.tx, jacc.ty, jacc.tz);
+        if (PERM) store3(PjT, S, slot, jacc.tx, jacc.ty, jacc.tz);
     }
     PiF[(size_t)w * kTile + lane] = make_double4(acc.fx, acc.fy, acc.fz, 0.0);
-    PiT[(size_t)w * kTile + lane] = make_double4(acc.tx, acc.ty, acc.tz, 0.0);
+    if (PERM) PiT[(size_t)w * kTile + lane] = make_double4(acc.tx, acc.ty, acc.tz, 0.0);
     en = wave_sum(en);
     if (lane == 0) part_e[w] = en;
 }
@@ -1550,12 +1628,15 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
                       double4* Pi, double* Pj, double4* out) {
     BoxT64 b = mkbox(L.box);
-    if (L.pbc)
-        hipLaunchKernelGGL(k_sym_field<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
-    else
-        hipLaunchKernelGGL(k_sym_field<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab);
+#define TDP_SFIELD(P, Q)                                                                                          \
+    hipLaunchKernelGGL((k_sym_field<P, Q>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, inva, \
+                       L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab)
+    if (L.pbc) {
+        if (L.perm) TDP_SFIELD(true, true); else TDP_SFIELD(true, false);
+    } else {
+        if (L.perm) TDP_SFIELD(false, true); else TDP_SFIELD(false, false);
+    }
+#undef TDP_SFIELD
     launch_col_reduce(s, L, Pi, Pj, nullptr, out, nullptr);
 }
 
@@ -1563,26 +1644,26 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
                      double* PjT, double* part_e, double* frc, double4* torque) {
     BoxT64 b = mkbox(L.box);
-#define TDP_SELEC(P, M, X)                                                                                           \
-    hipLaunchKernelGGL((k_sym_elec<P, M, X>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, mu, \
-                       inva, orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, PiF, PjF, PiT, PjT, L.slab,   \
-                       part_e)
+#define TDP_SELEC(P, M, X, Q)                                                                                         \
+    hipLaunchKernelGGL((k_sym_elec<P, M, X, Q>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, \
+                       mu, inva, orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, PiF, PjF,  \
+                       PiT, PjT, L.slab, part_e)
+#define TDP_SELEC_Q(P, M, X)              \
+    if (L.perm) TDP_SELEC(P, M, X, true); \
+    else TDP_SELEC(P, M, X, false)
+#define TDP_SELEC_M(P, X)                                   \
+    if (mutual) { TDP_SELEC_Q(P, true, X); }                \
+    else { TDP_SELEC_Q(P, false, X); }
     if (mixed) {
-        if (L.pbc) {
-            if (mutual) TDP_SELEC(true, true, true); else TDP_SELEC(true, false, true);
-        } else {
-            if (mutual) TDP_SELEC(false, true, true); else TDP_SELEC(false, false, true);
-        }
+        if (L.pbc) { TDP_SELEC_M(true, true) } else { TDP_SELEC_M(false, true) }
     } else {
-        if (L.pbc) {
-            if (mutual) TDP_SELEC(true, true, false); else TDP_SELEC(true, false, false);
-        } else {
-            if (mutual) TDP_SELEC(false, true, false); else TDP_SELEC(false, false, false);
-        }
+        if (L.pbc) { TDP_SELEC_M(true, false) } else { TDP_SELEC_M(false, false) }
     }
+#undef TDP_SELEC_M
+#undef TDP_SELEC_Q
 #undef TDP_SELEC
     launch_col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc);
-    launch_col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
+    if (L.perm) launch_col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
 }
 
 void launch_field_init(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha, double4* E0,

@@ -449,6 +449,111 @@ __device__ __forceinline__ void far_matvec_frame(const float4* __restrict__ s_l,
     }
 }
 
+// Squared separation beyond which no pair of tiles I and J is damped by more than 3e-12 relative
+// (s = -gamma (r/(a_i a_j))^3 <= -30, so 1 - (1 - s) e^s differs from 1 by less than 3e-12): the
+// damping factors are 1 to FP32 precision there.  Bound from the tiles' largest and smallest a
+// with a 2% margin; 0 when no pair is damped, infinite when a zero a could meet a nonzero one.
+__device__ __forceinline__ float undamped_r2_fp32(float4 cI, float4 hI, float4 cJ, float4 hJ) {
+    const float dmax = cI.w * cJ.w, gmin = fminf(hI.w, hJ.w);
+    if (dmax == 0.f) return 0.f;
+    if (!(gmin > 0.f)) return __builtin_inff();
+    const float rd = dmax * cbrtf(30.f / gmin) * 1.02f;
+    return rd * rd;
+}
+
+// Near tile pair of T * mu in mixed precision (full, unmasked, off-diagonal tiles on frame SM).
+// Pass 1 (near_matvec_mixed32) runs the 64 steps in FP32 like a far tile pair, except steps in
+// which some pair lies within r2min (or a straddling pair cannot be decided in FP32): those are
+// returned in a step mask and only rotate the FP32 j sums.  Pass 2 (near_matvec_mixed64) evaluates
+// the marked steps in FP64 as k_sym_matvec does; in a step every lane serves a different j slot,
+// so its j terms are added to the slot's LDS accumulator s_jn (3 x 64 doubles) without conflicts
+// or rotation.
+template <int SM>
+__device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* __restrict__ s_l,
+                                                                  const float4* __restrict__ s_uf, int lane,
+                                                                  float si_x, float si_y, float si_z,
+                                                                  const FarFrame& f, float4 uf, float3 uf3,
+                                                                  float r2min, double& ax, double& ay, double& az,
+                                                                  double& jx, double& jy, double& jz) {
+    float tx = 0.f, ty = 0.f, tz = 0.f, sx3 = 0.f, sy3 = 0.f, sz3 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+    unsigned long long steps64 = 0ull;
+    for (int k = 0; k < kTile; k++) {
+        const int jj = lane + k;   // FP32 arrays are staged twice over
+        const float4 pl = s_l[jj];
+        bool amb = false;
+        const float dx = far_sep<(SM & 1) != 0>(pl.x, si_x, f.hx, f.bx, amb);
+        const float dy = far_sep<(SM & 2) != 0>(pl.y, si_y, f.hy, f.by, amb);
+        const float dz = far_sep<(SM & 4) != 0>(pl.z, si_z, f.hz, f.bz, amb);
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        if (__builtin_amdgcn_ballot_w64(amb || !(r2 > r2min))) {
+            steps64 |= 1ull << k;
+            fx = rot1(fx);
+            fy = rot1(fy);
+            fz = rot1(fz);
+            continue;
+        }
+        const float4 uj = s_uf[jj];
+        float rinv = rsqrt_t(r2);
+        float rinv2 = rinv * rinv;
+        float rr3 = rinv * rinv2;
+        float rr5 = rr3 * rinv2;   // 1/r^5: the factor 3 sits in uf3 and in the flush of tx..tz
+        float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
+        float udi = rr5 * (uf3.x * dx + uf3.y * dy + uf3.z * dz);
+        tx += udj * dx;
+        ty += udj * dy;
+        tz += udj * dz;
+        sx3 += rr3 * uj.x;
+        sy3 += rr3 * uj.y;
+        sz3 += rr3 * uj.z;
+        fx = rot1(fx) + (udi * dx - rr3 * uf.x);
+        fy = rot1(fy) + (udi * dy - rr3 * uf.y);
+        fz = rot1(fz) + (udi * dz - rr3 * uf.z);
+    }
+    ax += 3.0 * (double)tx - (double)sx3;
+    ay += 3.0 * (double)ty - (double)sy3;
+    az += 3.0 * (double)tz - (double)sz3;
+    // the FP32 j sums lag one step: after 64 steps lane l holds slot l + 63, one more rotation gives slot l
+    jx = (double)rot1(fx);
+    jy = (double)rot1(fy);
+    jz = (double)rot1(fz);
+    return steps64;
+}
+
+template <bool PBC>
+__device__ __forceinline__ void near_matvec_mixed64(unsigned long long steps64, const double4* __restrict__ s_p,
+                                                    const double4* __restrict__ s_u, const double* __restrict__ s_ia,
+                                                    double* __restrict__ s_jn, int lane, const double4& pi,
+                                                    const double4& ui, double iai, const BoxT64& box, double& ax,
+                                                    double& ay, double& az, double& jx, double& jy, double& jz) {
+    s_jn[lane] = 0.0;
+    s_jn[kTile + lane] = 0.0;
+    s_jn[2 * kTile + lane] = 0.0;
+    __syncthreads();
+    while (steps64) {
+        const int k = __builtin_ctzll(steps64);
+        steps64 &= steps64 - 1ull;
+        const int jd = (lane + k) & (kTile - 1);
+        const double4 pj = s_p[jd];
+        const double4 uj = s_u[jd];
+        double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+        double rr3, rr5;
+        damped_rr_wave(ex * ex + ey * ey + ez * ez, pi.w, pj.w, iai, s_ia[jd], rr3, rr5);
+        double udj = rr5 * (uj.x * ex + uj.y * ey + uj.z * ez);
+        double udi = rr5 * (ui.x * ex + ui.y * ey + ui.z * ez);
+        ax += udj * ex - rr3 * uj.x;
+        ay += udj * ey - rr3 * uj.y;
+        az += udj * ez - rr3 * uj.z;
+        s_jn[jd] += udi * ex - rr3 * ui.x;
+        s_jn[kTile + jd] += udi * ey - rr3 * ui.y;
+        s_jn[2 * kTile + jd] += udi * ez - rr3 * ui.z;
+    }
+    __syncthreads();
+    jx += s_jn[lane];
+    jy += s_jn[kTile + lane];
+    jz += s_jn[2 * kTile + lane];
+}
+
 // ------------------------------------------------------------------ T * mu, mixed precision (a7)
 // Tile pairs inside the Thole range, and the diagonal and exclusion tiles, are evaluated in FP64
 // as in k_sym_matvec.  Tile pairs beyond it, most of the pairs of a large system, carry only the
@@ -458,7 +563,7 @@ __device__ __forceinline__ void far_matvec_frame(const float4* __restrict__ s_l,
 // largest terms, so keeping them in FP64 holds the operator's error to that of the FP32 far field;
 // all partial sums are FP64.
 template <bool PBC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_sym_matvec_mixed(TileMap tm, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_sym_matvec_mixed(TileMap tm, int nblk, const int4* __restrict__ work,
                                                          const double4* __restrict__ posa,
                                                          const double4* __restrict__ mu,
                                                          const float4* __restrict__ mu32,
@@ -475,12 +580,17 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // lane + k: the step's address is the lane's base plus a constant offset.
     // FP32 frames: offsets s_l (float4) at the start, dipoles s_uf behind the FP64 positions s_pf of
     // the general far path, so a fallback from a frame to FP64 positions restages only s_pf.
+    // Near tiles in mixed steps add FP32 offsets and dipoles (twice over) behind the FP64 arrays.
     constexpr int kBytesNear = kTile * (2 * sizeof(double4) + sizeof(double));
+    constexpr int kBytesNearMixed = kBytesNear + 2 * kTile * 2 * sizeof(float4) + 3 * kTile * sizeof(double);
     constexpr int kBytesFar = 2 * kTile * (sizeof(double4) + sizeof(float4));
-    __shared__ __align__(16) unsigned char s_raw[kBytesNear > kBytesFar ? kBytesNear : kBytesFar];
+    __shared__ __align__(16) unsigned char s_raw[kBytesNearMixed > kBytesFar ? kBytesNearMixed : kBytesFar];
     double4* s_p = reinterpret_cast<double4*>(s_raw);
     double4* s_u = s_p + kTile;
     double* s_ia = reinterpret_cast<double*>(s_u + kTile);
+    float4* s_nl = reinterpret_cast<float4*>(s_raw + kBytesNear);
+    float4* s_nu = s_nl + 2 * kTile;
+    double* s_jn = reinterpret_cast<double*>(s_nu + 2 * kTile);
     double4* s_pf = reinterpret_cast<double4*>(s_raw);
     float4* s_uf = reinterpret_cast<float4*>(s_pf + 2 * kTile);
     float4* s_l = reinterpret_cast<float4*>(s_raw);
@@ -519,6 +629,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
                                               (float)box.bz);
         double jx, jy, jz;
         int kend;
+        FarFrame fr_n;
         __syncthreads();
         if (far) {
             const FarFrame fr = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box);
@@ -597,6 +708,42 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4))) void k_
             jy = (double)fy;
             jz = (double)fz;
             kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
+        } else if (!masked && !diag && ifull && cntJ == kTile &&
+                   (fr_n = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box)).smask >= 0) {
+            s_p[lane] = posa[jb + lane];
+            s_u[lane] = mu[jb + lane];
+            s_ia[lane] = inva[jb + lane];
+            {
+                const float4 l = posl[jb + lane];
+                const float4 u = mu32[jb + lane];
+                s_nl[lane] = l;
+                s_nl[lane + kTile] = l;
+                s_nu[lane] = u;
+                s_nu[lane + kTile] = u;
+            }
+            __syncthreads();
+            const float r2min = undamped_r2_fp32(cI, hI, tb_c[J], tb_h[J]);
+            const float si_x = (float)((double)li.x - fr_n.shx), si_y = (float)((double)li.y - fr_n.shy),
+                        si_z = (float)((double)li.z - fr_n.shz);
+            unsigned long long steps64 = 0ull;
+#define TDP_NEAR_CASE(M)                                                                                      \
+    case M:                                                                                                   \
+        steps64 = near_matvec_mixed32<M>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min, ax, ay, az, jx, \
+                                         jy, jz);                                                             \
+        break;
+            switch (fr_n.smask) {
+                TDP_NEAR_CASE(0)
+                TDP_NEAR_CASE(1)
+                TDP_NEAR_CASE(2)
+                TDP_NEAR_CASE(3)
+                TDP_NEAR_CASE(4)
+                TDP_NEAR_CASE(5)
+                TDP_NEAR_CASE(6)
+                TDP_NEAR_CASE(7)
+            }
+#undef TDP_NEAR_CASE
+            if (steps64) near_matvec_mixed64<PBC>(steps64, s_p, s_u, s_ia, s_jn, lane, pi, ui, iai, box, ax, ay, az, jx, jy, jz);
+            kend = kTile;
         } else {
             s_p[lane] = posa[jb + lane];
             s_u[lane] = mu[jb + lane];

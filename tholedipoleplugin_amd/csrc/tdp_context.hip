@@ -96,6 +96,8 @@ struct tdp_context {
     DenseSys* d_dsys = nullptr;
     unsigned short* d_dmat = nullptr;
     float* d_dinv = nullptr;
+    float* d_dacc = nullptr;        // FP32 running sums of the diagonal blocks during the factorization
+    size_t dense_dwords = 0;
     double *d_db = nullptr, *d_dx = nullptr, *d_dres = nullptr, *d_dy = nullptr, *d_drel = nullptr;
     double4* d_dmu = nullptr;
     int *d_dactive = nullptr, *d_dstatus = nullptr;
@@ -995,7 +997,7 @@ int dense_alloc(tdp_context* c) {
         maxnt = std::max(maxnt, nt);
     }
     int rc;
-    if ((rc = dalloc(c, &c->d_dsys, ns)) || (rc = dalloc(c, &c->d_dmat, moff)) || (rc = dalloc(c, &c->d_dinv, doff)) ||
+    if ((rc = dalloc(c, &c->d_dsys, ns)) || (rc = dalloc(c, &c->d_dmat, moff)) || (rc = dalloc(c, &c->d_dinv, doff)) || (rc = dalloc(c, &c->d_dacc, doff)) ||
         (rc = dalloc(c, &c->d_db, voff)) || (rc = dalloc(c, &c->d_dx, voff)) || (rc = dalloc(c, &c->d_dres, voff)) ||
         (rc = dalloc(c, &c->d_dy, voff)) || (rc = dalloc(c, &c->d_drel, 4 * (size_t)ns)) ||
         (rc = dalloc(c, &c->d_dmu, 4 * (size_t)c->npad)) || (rc = dalloc(c, &c->d_dactive, ns)) ||
@@ -1004,6 +1006,7 @@ int dense_alloc(tdp_context* c) {
     TDP_HIP(c, hipMemcpyAsync(c->d_dsys, sys.data(), ns * sizeof(DenseSys), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
     c->dense_max_nt = maxnt;
+    c->dense_dwords = doff;
     c->dense_ready = true;
     return TDP_OK;
 }
@@ -1027,7 +1030,8 @@ int dense_responses(tdp_context* c, bool bf16, double tol, int max_steps, double
     launch_dense_assemble(s, bf16, ns, c->dense_max_nt, c->d_dsys, c->d_tcnt, c->posa, c->d_alpha, c->d_inva,
                           c->d_exc_start, c->d_exc_J, c->d_exc_mask, c->d_dmat);
     TDP_HIP(c, hipEventRecord(ev[1], s));
-    launch_dense_factor(s, bf16, ns, c->dense_max_nt, c->d_dsys, c->d_dmat, c->d_dinv, c->d_dstatus);
+    TDP_HIP(c, hipMemsetAsync(c->d_dacc, 0, c->dense_dwords * sizeof(float), s));
+    launch_dense_factor(s, bf16, ns, c->dense_max_nt, c->d_dsys, c->d_dmat, c->d_dinv, c->d_dacc, c->d_dstatus);
     TDP_HIP(c, hipEventRecord(ev[2], s));
     launch_dense_rhs(s, G, c->d_dsys, c->d_tcnt, c->E0, c->d_alpha, c->d_db, c->d_dx, c->d_dres);
     std::vector<int> active(ns, 1);

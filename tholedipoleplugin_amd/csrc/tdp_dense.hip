@@ -170,72 +170,77 @@ __device__ __forceinline__ void block_update(const DenseSys& d, int i, int j, ui
     }
 }
 
-// Diagonal block j of every system with more than j blocks: S = A^_jj - sum L_jk L_jk^T, its
-// Cholesky factor in FP32 (column by column in LDS) and the inverse of that factor (kept in FP32
-// for the off-diagonal blocks and the triangular solves).
+// Diagonal block j of every system with more than j blocks: S = A^_jj - sum_{k<j} L_jk L_jk^T, its
+// Cholesky factor in FP32 and the inverse X of that factor (kept in FP32 for the off-diagonal blocks
+// and the triangular solves; the factor itself is not needed).  The sum over k < j arrives
+// precomputed: k_chol_col adds L_jk L_jk^T of every finished column k to the FP32 running block
+// dacc_j, so the serial chain of diagonal factorizations carries no block products.
+// Right-looking in registers: wave w holds rows 16 w .. 16 w + 15 of S and X, lane c column c.  At
+// pivot p the owner of row p scales it (by symmetry of the trailing S, row p past the diagonal is
+// column p of L) and scales row p of X; both go through LDS (double-buffered by the parity of p, one
+// barrier per pivot), and every wave updates S -= L_rp L_cp and, for its rows r > p, X_r -= L_rp X_p.
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_chol_diag(int j, const DenseSys* __restrict__ sys,
-                                                   const unsigned short* __restrict__ mat, float* __restrict__ dinv,
+                                                   const unsigned short* __restrict__ mat,
+                                                   const float* __restrict__ dacc, float* __restrict__ dinv,
                                                    int* __restrict__ status) {
     const DenseSys d = sys[blockIdx.x];
     const int nb = 3 * d.nt;
     if (j >= nb) return;
-    // the staging buffers of the update and the FP32 blocks used after it share one LDS area
-    __shared__ uint2 s_raw[4 * kB * kLd2];
-    uint2* s_a = s_raw;
-    uint2* s_b = s_raw + 2 * kB * kLd2;
-    float (*s_s)[kB + 1] = reinterpret_cast<float (*)[kB + 1]>(s_raw);
-    float (*s_x)[kB + 1] = s_s + kB;
-    const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    const int wr = w >> 1, wc = w & 1;
-    float16_t acc = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    block_update<BF16>(d, j, j, s_a, s_b, mat, acc);
+    __shared__ __align__(16) float s_col[2][kB];
+    __shared__ __align__(16) float s_xr[2][kB];
+    const int t = threadIdx.x, w = t >> 6, c = t & 63;
     const size_t m = (size_t)192 * d.nt;
     const unsigned short* g = mat + blk_off(d, j, j);
-    for (int v = 0; v < 16; v++) {
-        const int r = wr * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), c = wc * 32 + (l & 31);
-        s_s[r][c] = from_lp<BF16>(g[(size_t)r * m + c]) - acc[v];
+    const float* ac = dacc + d.doff + (size_t)j * kB * kB;
+    float S[16], X[16];
+#pragma unroll
+    for (int u = 0; u < 16; u++) {
+        const int r = 16 * w + u;
+        S[u] = from_lp<BF16>(g[(size_t)r * m + c]) - ac[r * kB + c];
+        X[u] = r == c ? 1.f : 0.f;
     }
-    __syncthreads();
-    // Right-looking Cholesky of the 64x64 block (lower triangle of s_s) and, in the same sweep, the
-    // inverse of the factor (s_x, lower; Gauss-Jordan on the rows of the identity).
-    for (int e = t; e < kB * kB; e += 256) s_x[e >> 6][e & 63] = (e >> 6) == (e & 63) ? 1.f : 0.f;
-    __syncthreads();
-    for (int p = 0; p < kB; p++) {
-        float dv = s_s[p][p];
-        if (!(dv > 0.f)) {
-            if (t == 0) atomicOr(status, 1);
-            dv = 1.f;
-        }
-        const float piv = sqrtf(dv), ipiv = 1.f / piv;
-        __syncthreads();
-        // column p of L and row p of the inverse
-        if (t < kB) {
-            if (t > p) s_s[t][p] *= ipiv;
-            if (t == p) s_s[p][p] = piv;
-            if (t <= p) s_x[p][t] *= ipiv;
-        }
-        __syncthreads();
-        // trailing update of S (p < c <= r) and elimination of column p from the rows below in X
-        for (int e = t; e < kB * kB; e += 256) {
-            const int r = e >> 6, c = e & 63;
-            if (r > p) {
-                const float lrp = s_s[r][p];
-                if (c > p && c <= r) s_s[r][c] -= lrp * s_s[c][p];
-                if (c <= p) s_x[r][c] -= lrp * s_x[p][c];
+    bool bad = false;
+    for (int wp = 0; wp < 4; wp++) {
+#pragma unroll
+        for (int up = 0; up < 16; up++) {
+            const int p = 16 * wp + up, b = p & 1;
+            if (w == wp) {
+                float dv = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(S[up]), p));   // S_pp
+                if (!(dv > 0.f)) {
+                    bad = true;
+                    dv = 1.f;
+                }
+                const float piv = sqrtf(dv), ip = 1.f / piv;
+                s_col[b][c] = c == p ? piv : S[up] * ip;   // L_cp for c > p
+                X[up] *= ip;
+                s_xr[b][c] = X[up];
+            }
+            __syncthreads();
+            const float lc = s_col[b][c], xp = s_xr[b][c];
+#pragma unroll
+            for (int u = 0; u < 16; u++) {
+                const int r = 16 * w + u;
+                const float lr = s_col[b][r];
+                S[u] -= lr * lc;              // rows or columns <= p are finished and never read again
+                if (r > p) X[u] -= lr * xp;   // X stays lower triangular
             }
         }
-        __syncthreads();
     }
+    if (bad && c == 0) atomicOr(status, 1);
     float* out = dinv + d.doff + (size_t)j * kB * kB;
-    for (int e = t; e < kB * kB; e += 256) out[e] = s_x[e >> 6][e & 63];
+#pragma unroll
+    for (int u = 0; u < 16; u++) out[(16 * w + u) * kB + c] = X[u];
 }
 
 // Off-diagonal blocks (i, j), i > j: L_ij = (A^_ij - sum_{k<j} L_ik L_jk^T) L_jj^-T, stored in the
 // 16-bit format at (i, j) and transposed at (j, i).
+// Each block also adds L_ij L_ij^T (from the stored 16-bit L_ij, on MFMA) to the FP32 running sum
+// dacc_i of its diagonal block; one workgroup per (i, j), and the columns j are successive launches.
 template <bool BF16>
 __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restrict__ sys,
-                                                  unsigned short* __restrict__ mat, const float* __restrict__ dinv) {
+                                                  unsigned short* __restrict__ mat, const float* __restrict__ dinv,
+                                                  float* __restrict__ dacc) {
     const DenseSys d = sys[blockIdx.y];
     const int nb = 3 * d.nt;
     const int i = j + 1 + blockIdx.x;
@@ -273,12 +278,31 @@ __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restr
             for (int u = 0; u < 4; u++)
                 for (int v = 0; v < 4; v++) o[u][v] += sv[u] * xv[v];
         }
+        unsigned short lb[4][4];
         for (int u = 0; u < 4; u++)
             for (int v = 0; v < 4; v++) {
                 const unsigned short b = to_lp<BF16>(o[u][v]);
+                lb[u][v] = b;
                 g[(size_t)(rb + u) * m + cb + v] = b;
                 gt[(size_t)(cb + v) * m + rb + u] = b;
             }
+        __syncthreads();   // s_s and s_x are read no more: their LDS now stages L_ij
+        for (int u = 0; u < 4; u++)
+            s_a[(rb + u) * kLd2 + (cb >> 2)] = make_uint2((unsigned)lb[u][0] | ((unsigned)lb[u][1] << 16),
+                                                          (unsigned)lb[u][2] | ((unsigned)lb[u][3] << 16));
+    }
+    __syncthreads();
+    {
+        float16_t acc2 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        const uint2* pa = s_a + (wr * 32 + (l & 31)) * kLd2 + (l >> 5);
+        const uint2* pb = s_a + (wc * 32 + (l & 31)) * kLd2 + (l >> 5);
+#pragma unroll
+        for (int kk = 0; kk < kB / 4; kk += 2) acc2 = mfma32<BF16>(pa[kk], pb[kk], acc2);
+        float* ac = dacc + d.doff + (size_t)i * kB * kB;
+        for (int v = 0; v < 16; v++) {
+            const int r = wr * 32 + 8 * (v >> 2) + 4 * (l >> 5) + (v & 3), c = wc * 32 + (l & 31);
+            ac[r * kB + c] += acc2[v];
+        }
     }
 }
 
@@ -286,17 +310,19 @@ __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restr
 // One workgroup per system solves L L^T y = rhs for the 4 right-hand sides (FP32 arithmetic on the
 // 16-bit factor), left-looking by blocks.  Row R*64 + r of L left of the diagonal block sits in the
 // lower blocks and row R*64 + r of L^T right of it in the upper (transposed) blocks, both
-// contiguous: thread (r, s) reads 16-byte pieces s, s + 4, ... of that row and accumulates all 4
-// right-hand sides; the four partial sums are added in a fixed order.  y stays in LDS.  rhs and y
+// contiguous: thread (r, s) reads 16-byte pieces s, s + 16, ... of that row and accumulates all 4
+// right-hand sides; the 16 partial sums are added in a fixed order.  y stays in LDS.  rhs and y
 // are FP64 arrays of m x 4 values (unknown, then right-hand side) at d.voff.
 constexpr int kMaxDenseTiles = 16;   // up to 1,024 particles per system: m <= 3,072
+constexpr int kSolveSplit = 16;      // column pieces of a row sum (waves per system)
+constexpr int kSolveThreads = kSolveSplit * kB;
 
 template <bool BF16>
 __device__ __forceinline__ float4 row_dot(const unsigned short* __restrict__ row, int c0, int c1, int sp,
                                           const float4* __restrict__ s_y) {
     // sum over columns [c0, c1) (multiples of 8) of row[c] * y[c], pieces of 8 columns
     float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int c = c0 + 8 * sp; c < c1; c += 32) {
+    for (int c = c0 + 8 * sp; c < c1; c += 8 * kSolveSplit) {
         const uint4 v = *reinterpret_cast<const uint4*>(row + c);
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
@@ -314,58 +340,68 @@ __device__ __forceinline__ float4 row_dot(const unsigned short* __restrict__ row
 }
 
 template <bool BF16>
-__global__ __launch_bounds__(256) void k_dense_solve(const DenseSys* __restrict__ sys,
-                                                     const unsigned short* __restrict__ mat,
-                                                     const float* __restrict__ dinv, const double* __restrict__ rhs,
-                                                     double* __restrict__ y, const int* __restrict__ active) {
+__global__ __launch_bounds__(kSolveThreads) void k_dense_solve(const DenseSys* __restrict__ sys,
+                                                               const unsigned short* __restrict__ mat,
+                                                               const float* __restrict__ dinv,
+                                                               const double* __restrict__ rhs, double* __restrict__ y,
+                                                               const int* __restrict__ active) {
     const DenseSys d = sys[blockIdx.x];
     if (active && !active[blockIdx.x]) return;
     const int nb = 3 * d.nt;
     const int m = 192 * d.nt;
     __shared__ float4 s_y[3 * kMaxDenseTiles * kB];   // 48 KB
-    __shared__ float4 s_p[4][kB];
+    __shared__ float4 s_p[kSolveSplit][kB];           // 16 KB
     __shared__ float s_b[4][kB];
     __shared__ float s_inv[kB][kB + 1];
     const int t = threadIdx.x, r = t & 63, sp = t >> 6;
-    for (int e = t; e < m; e += 256) {
+    for (int e = t; e < m; e += kSolveThreads) {
         const double* q = rhs + d.voff + 4 * (size_t)e;
         s_y[e] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
     }
     __syncthreads();
+    constexpr int kInvPer = kB * kB / kSolveThreads;   // diagonal-inverse words per thread
     for (int pass = 0; pass < 2; pass++) {
         // pass 0: L y = rhs, blocks R = 0..nb-1; pass 1: L^T x = y, blocks R = nb-1..0
         for (int step = 0; step < nb; step++) {
             const int R = pass == 0 ? step : nb - 1 - step;
+            // the step's diagonal-block inverse is fetched first, so its latency overlaps the row sums
+            const float* inv = dinv + d.doff + (size_t)R * kB * kB;
+            float iv[kInvPer];
+#pragma unroll
+            for (int u = 0; u < kInvPer; u++) iv[u] = inv[t + kSolveThreads * u];
             const unsigned short* row = mat + d.moff + (size_t)(R * kB + r) * m;
             const float4 a = pass == 0 ? row_dot<BF16>(row, 0, R * kB, sp, s_y)
                                        : row_dot<BF16>(row, (R + 1) * kB, m, sp, s_y);
             s_p[sp][r] = a;
+#pragma unroll
+            for (int u = 0; u < kInvPer; u++) {
+                const int e = t + kSolveThreads * u;
+                s_inv[e >> 6][e & 63] = iv[u];
+            }
             __syncthreads();
             if (sp == 0) {
                 float4 b = s_y[R * kB + r];
-                for (int u = 0; u < 4; u++) {
+                for (int u = 0; u < kSolveSplit; u++) {
                     b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
                 }
                 s_b[0][r] = b.x; s_b[1][r] = b.y; s_b[2][r] = b.z; s_b[3][r] = b.w;
             }
             __syncthreads();
-            // diagonal block: thread (r, q = sp) applies Linv (pass 0) or Linv^T (pass 1) to field q
-            const float* inv = dinv + d.doff + (size_t)R * kB * kB;
-            for (int e = t; e < kB * kB; e += 256) s_inv[e >> 6][e & 63] = inv[e];
-            __syncthreads();
-            float o = 0.f;
-            if (pass == 0) {
-                for (int k = 0; k <= r; k++) o += s_inv[r][k] * s_b[sp][k];
-            } else {
-                for (int k = r; k < kB; k++) o += s_inv[k][r] * s_b[sp][k];
+            // diagonal block: thread (r, q = sp < 4) applies Linv (pass 0) or Linv^T (pass 1) to field q
+            if (sp < 4) {
+                float o = 0.f;
+                if (pass == 0) {
+                    for (int k = 0; k <= r; k++) o += s_inv[r][k] * s_b[sp][k];
+                } else {
+                    for (int k = r; k < kB; k++) o += s_inv[k][r] * s_b[sp][k];
+                }
+                float* yr = reinterpret_cast<float*>(&s_y[R * kB + r]);
+                yr[sp] = o;
             }
-            __syncthreads();
-            float* yr = reinterpret_cast<float*>(&s_y[R * kB + r]);
-            yr[sp] = o;
             __syncthreads();
         }
     }
-    for (int e = t; e < m; e += 256) {
+    for (int e = t; e < m; e += kSolveThreads) {
         const float4 v = s_y[e];
         double* q = y + d.voff + 4 * (size_t)e;
         q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
@@ -484,17 +520,18 @@ void launch_dense_assemble(hipStream_t s, bool bf16, int nsys, int max_nt, const
 }
 
 void launch_dense_factor(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, unsigned short* mat,
-                         float* dinv, int* status) {
+                         float* dinv, float* dacc, int* status) {
     const int nb = 3 * max_nt;
     for (int j = 0; j < nb; j++) {
         if (bf16) {
-            hipLaunchKernelGGL(k_chol_diag<true>, dim3(nsys), dim3(256), 0, s, j, sys, mat, dinv, status);
+            hipLaunchKernelGGL(k_chol_diag<true>, dim3(nsys), dim3(256), 0, s, j, sys, mat, dacc, dinv, status);
             if (j + 1 < nb)
-                hipLaunchKernelGGL(k_chol_col<true>, dim3(nb - j - 1, nsys), dim3(256), 0, s, j, sys, mat, dinv);
+                hipLaunchKernelGGL(k_chol_col<true>, dim3(nb - j - 1, nsys), dim3(256), 0, s, j, sys, mat, dinv, dacc);
         } else {
-            hipLaunchKernelGGL(k_chol_diag<false>, dim3(nsys), dim3(256), 0, s, j, sys, mat, dinv, status);
+            hipLaunchKernelGGL(k_chol_diag<false>, dim3(nsys), dim3(256), 0, s, j, sys, mat, dacc, dinv, status);
             if (j + 1 < nb)
-                hipLaunchKernelGGL(k_chol_col<false>, dim3(nb - j - 1, nsys), dim3(256), 0, s, j, sys, mat, dinv);
+                hipLaunchKernelGGL(k_chol_col<false>, dim3(nb - j - 1, nsys), dim3(256), 0, s, j, sys, mat, dinv,
+                                   dacc);
         }
     }
 }
@@ -502,9 +539,9 @@ void launch_dense_factor(hipStream_t s, bool bf16, int nsys, int max_nt, const D
 void launch_dense_solve(hipStream_t s, bool bf16, int nsys, const DenseSys* sys, const unsigned short* mat,
                         const float* dinv, const double* rhs, double* y, const int* active) {
     if (bf16)
-        hipLaunchKernelGGL(k_dense_solve<true>, dim3(nsys), dim3(256), 0, s, sys, mat, dinv, rhs, y, active);
+        hipLaunchKernelGGL(k_dense_solve<true>, dim3(nsys), dim3(kSolveThreads), 0, s, sys, mat, dinv, rhs, y, active);
     else
-        hipLaunchKernelGGL(k_dense_solve<false>, dim3(nsys), dim3(256), 0, s, sys, mat, dinv, rhs, y, active);
+        hipLaunchKernelGGL(k_dense_solve<false>, dim3(nsys), dim3(kSolveThreads), 0, s, sys, mat, dinv, rhs, y, active);
 }
 
 void launch_dense_rhs(hipStream_t s, const SegLaunch& G, const DenseSys* sys, const int* tcnt, const double4* E0,

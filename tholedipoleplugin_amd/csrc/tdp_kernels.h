@@ -122,8 +122,9 @@ int dense_max_tiles();
 void launch_dense_assemble(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, const int* tcnt,
                            const double4* posa, const double* alpha, const double* inva, const int* exc_start,
                            const int* exc_J, const unsigned long long* exc_mask, unsigned short* mat);
+// dacc: FP32 running sums of the diagonal blocks (same layout as dinv), zero on entry
 void launch_dense_factor(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, unsigned short* mat,
-                         float* dinv, int* status);
+                         float* dinv, float* dacc, int* status);
 void launch_dense_solve(hipStream_t s, bool bf16, int nsys, const DenseSys* sys, const unsigned short* mat,
                         const float* dinv, const double* rhs, double* y, const int* active);
 void launch_dense_rhs(hipStream_t s, const SegLaunch& G, const DenseSys* sys, const int* tcnt, const double4* E0,

@@ -35,10 +35,10 @@ PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
 # HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1): 87.1 MB read + 347.4 MB
+# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1): 93.0 MB read + 347.4 MB
 # written (the tile partial slab, 255 MB, and the work-item partials).  Counters cannot be read
 # inside this process.
-MATVEC_HBM_BYTES = 434.5e6
+MATVEC_HBM_BYTES = 440.4e6
 
 
 def parse():

@@ -227,6 +227,39 @@ def test_device_entry_point_matches_host():
     assert r.returncode == 0 and "OK" in r.stdout
 
 
+@pytest.mark.parametrize("solver", ["pcg", "jacobi"])
+def test_iteration_graph_replays(solver, monkeypatch):
+    """Solver iterations replayed from captured HIP graphs give the same results as iterations
+    enqueued launch by launch (TDP_ITERATION_GRAPHS=0), over a sequence of evaluations that reuses
+    the graphs with new positions, a box change (new capture) and a repeat of the first
+    configuration."""
+    from tholedipoleplugin_amd import _lib
+    _say("case iteration graphs %s" % solver)
+    s = systems.config0()
+    s.force.setMutualInducedTargetEpsilon(1e-9)
+    s.force.setMutualInducedMaxIterations(500)
+    rng = np.random.default_rng(5)
+    p0 = np.asarray(s.positions, np.float64)
+    p1 = p0 + rng.uniform(-0.002, 0.002, p0.shape)
+    box0 = np.asarray(s.box, np.float64)
+    box1 = box0 * 1.001
+    seq = [(p0, box0), (p0, box0), (p1, box0), (p1, box1), (p0, box0)]
+    out = {}
+    for graphs in ("1", "0"):
+        monkeypatch.setenv("TDP_ITERATION_GRAPHS", graphs)
+        h = _lib.HipContextHandle(s.force, box0, precision="mixed", solver=solver)
+        res = []
+        for pos, box in seq:
+            e, f = h.execute(pos, box)
+            res.append((e, f.copy(), h.solver_stats()["iterations"]))
+        h.close()
+        out[graphs] = res
+    for (e1, f1, n1), (e0, f0, n0) in zip(out["1"], out["0"]):
+        assert n1 == n0
+        assert e1 == e0
+        assert np.array_equal(f1, f0)
+
+
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_water12288_rows_full_size(precision):
     """36,864 sites (12,288 waters, 4.95 nm box, reference PME semantics), PCG solved to an RMS

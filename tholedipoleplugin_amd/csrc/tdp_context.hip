@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -66,6 +67,18 @@ struct tdp_context {
     std::vector<int> goff, gidx;
     // ---- device state
     hipStream_t stream = nullptr;
+    // Captured chunks of solver iterations (hipGraph), replayed while the launch parameters hold:
+    // same chunk length and solver, same box, same device tables (graph_gen counts table uploads).
+    struct IterGraph {
+        int iters;
+        bool jacobi;
+        unsigned gen;
+        double box[3];
+        hipGraphExec_t exec;
+    };
+    std::vector<IterGraph> graphs;
+    unsigned graph_gen = 0;
+    bool use_graphs = true;
     std::vector<void*> allocations;
     double4 *posa = nullptr, *qd = nullptr, *mu = nullptr, *E0 = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
             *q = nullptr, *torque = nullptr,
@@ -106,6 +119,7 @@ struct tdp_context {
     int2* d_segw = nullptr;
     // ---- results of the last solve (per system)
     std::vector<int> iterations, converged;
+    int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     std::vector<double> epsilon;
     // ---- timing
     bool timing = false;
@@ -340,6 +354,7 @@ int dresize(tdp_context* c, T** ptr, size_t& cap, size_t count) {
 
 // Per-particle parameters in the internal order, with axis atoms renumbered.
 int upload_particles(tdp_context* c) {
+    c->graph_gen++;
     const int n = c->n, npad = c->npad;
     std::vector<double> alpha(npad, 0.0), inva(npad, 0.0), thole(npad, 0.0), charge(npad, 0.0), dmol(3 * npad, 0.0);
     std::vector<int4> axis(npad, make_int4(5, -1, -1, -1));
@@ -375,6 +390,7 @@ int upload_particles(tdp_context* c) {
 
 // Exclusion tiles and torque-gather lists of the current order.
 int upload_tables(tdp_context* c) {
+    c->graph_gen++;   // device tables may move: captured iteration graphs are stale
     build_tiles(c);
     build_torque_gather(c);
     const size_t ntiles = c->exc_J.size();
@@ -642,6 +658,54 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
 }
 
 // Mutual solve of every system from the current E0 and mu (ReferenceTholeDipoleForce.cpp:820-851).
+// Enqueues n solver iterations, as a replay of a captured graph of them when possible (the
+// iteration kernels then go to the device as one submission).  Events for per-launch timing are
+// not captured: with timing on, the launches are enqueued one by one.
+int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool jacobi, int n) {
+    hipStream_t s = c->stream;
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    auto enqueue = [&]() {
+        for (int k = 0; k < n; k++) {
+            if (jacobi) {
+                matvec(c, L, c->mu, c->d_scalars);
+                launch_jacobi_update(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr, c->d_scalars,
+                                     c->d_red);
+            } else {
+                matvec(c, L, c->p, c->d_scalars);
+                launch_pcg_iteration_tail(s, G, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr,
+                                          c->q, c->d_red, c->d_scalars);
+            }
+        }
+    };
+    if (!c->use_graphs || c->timing) {
+        enqueue();
+        return TDP_OK;
+    }
+    hipGraphExec_t ex = nullptr;
+    for (size_t k = 0; k < c->graphs.size();) {
+        auto& g = c->graphs[k];
+        if (g.gen != c->graph_gen || g.box[0] != L.box[0] || g.box[1] != L.box[1] || g.box[2] != L.box[2]) {
+            hipGraphExecDestroy(g.exec);
+            c->graphs.erase(c->graphs.begin() + k);
+            continue;
+        }
+        if (g.iters == n && g.jacobi == jacobi) ex = g.exec;
+        k++;
+    }
+    if (!ex) {
+        hipGraph_t graph = nullptr;
+        TDP_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        enqueue();
+        TDP_HIP(c, hipStreamEndCapture(s, &graph));
+        hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        c->graphs.push_back({n, jacobi, c->graph_gen, {L.box[0], L.box[1], L.box[2]}, ex});
+    }
+    TDP_HIP(c, hipGraphLaunch(ex, s));
+    return TDP_OK;
+}
+
 int solve_mutual(tdp_context* c, const double* box) {
     hipStream_t s = c->stream;
     PairLaunch L = pair_launch(c, box);
@@ -650,36 +714,28 @@ int solve_mutual(tdp_context* c, const double* box) {
     // every system iterates until its own stop flag is set; a launch skips the tiles and work
     // items of stopped systems.  The host checks the count of stopped systems every chunk.
     const bool jacobi = c->solver == TDP_SOLVER_JACOBI;
-    const int chunk = c->n > 20000 ? 4 : 16;  // iterations enqueued between host checks
+    // Iterations enqueued between host checks of the stop count.  The first chunk is the largest
+    // iteration count of the previous solve of this context (repeated evaluations of similar
+    // configurations stop after about as many), later chunks are short; launches enqueued after
+    // every system has stopped return at once.
+    const int chunk = c->n > 20000 ? 2 : 4;
+    const int first_chunk = c->last_max_iter > 0 ? c->last_max_iter : (c->n > 20000 ? 4 : 16);
     int rc = start_solve(c, jacobi);
     if (rc) return rc;
     int done = 0;
-    if (jacobi) {
-        if ((rc = read_done(c, done))) return rc;
-        while (done < c->nseg) {
-            for (int k = 0; k < chunk; k++) {
-                matvec(c, L, c->mu, c->d_scalars);
-                launch_jacobi_update(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr, c->d_scalars,
-                                     c->d_red);
-            }
-            if ((rc = read_done(c, done))) return rc;
-        }
-    } else {
+    if (!jacobi) {
         // x = mu; r = E0 - x/alpha + T x; z = alpha r; p = z
         matvec(c, L, c->mu, nullptr);
         launch_pcg_init(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p, mixed ? c->vec32 : nullptr, c->d_red,
                         c->d_scalars);
+    }
+    if ((rc = read_done(c, done))) return rc;
+    for (int pass = 0; done < c->nseg; pass++) {
+        if ((rc = run_iterations(c, L, G, jacobi, pass == 0 ? first_chunk : chunk))) return rc;
         if ((rc = read_done(c, done))) return rc;
-        while (done < c->nseg) {
-            for (int k = 0; k < chunk; k++) {
-                matvec(c, L, c->p, c->d_scalars);
-                launch_pcg_iteration_tail(s, G, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr,
-                                          c->q, c->d_red, c->d_scalars);
-            }
-            if ((rc = read_done(c, done))) return rc;
-        }
     }
     if ((rc = read_solver_results(c, jacobi))) return rc;
+    c->last_max_iter = *std::max_element(c->iterations.begin(), c->iterations.end());
     if (c->timing) harvest_timing(c);
     return TDP_OK;
 }
@@ -896,6 +952,10 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(TDP_ERR_HIP);
+    }
+    {
+        const char* g = std::getenv("TDP_ITERATION_GRAPHS");   // "0": enqueue iterations one launch at a time
+        c->use_graphs = !(g && g[0] == '0');
     }
     const int npad = c->npad, nb = c->nblk, ns = c->nseg;
     // Work items: tile row I of a system split into chunks of C tiles (J >= I, same system).
@@ -1265,6 +1325,7 @@ void tdp_destroy(tdp_context* c) {
         hipEventDestroy(pr.second);
     }
     for (auto e : c->ev_pool) hipEventDestroy(e);
+    for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
     for (void* p : c->allocations) hipFree(p);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

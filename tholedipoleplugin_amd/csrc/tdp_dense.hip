@@ -128,45 +128,57 @@ static_assert(2 * kB * (kB + 1) * sizeof(float) <= 4 * kB * kLd2 * sizeof(uint2)
 template <bool BF16>
 __device__ __forceinline__ void block_update(const DenseSys& d, int i, int j, uint2* s_a, uint2* s_b,
                                              const unsigned short* __restrict__ mat, float16_t& acc) {
-    // s_a, s_b: two buffers each of 64 rows x kLd2 8-byte words; the tiles of step k + 1 are
-    // fetched into registers while step k's MFMAs run on the other buffer.
+    // s_a, s_b: two buffers each of 64 rows x kLd2 8-byte words.  The tiles of the next kDepth steps
+    // are in flight in a register ring while step k's MFMAs run on one buffer and step k + 1's tiles
+    // are written to the other; the k loop is unrolled by kDepth so that the ring is indexed statically.
+    constexpr int kDepth = 4;
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int wr = w >> 1, wc = w & 1;
     const size_t m = (size_t)192 * d.nt;
     if (j == 0) return;
     // 64 rows x 16 words per tile: thread t loads words (t, t + 256, t + 512, t + 768)
-    uint2 ra[4], rb[4];
-    auto fetch = [&](int k) {
+    uint2 ra[kDepth][4], rb[kDepth][4];
+    auto fetch = [&](int k, uint2 (&xa)[4], uint2 (&xb)[4]) {
         const uint2* gi = reinterpret_cast<const uint2*>(mat + blk_off(d, i, k));
         const uint2* gj = reinterpret_cast<const uint2*>(mat + blk_off(d, j, k));
         const size_t mw = m / 4;
+#pragma unroll
         for (int u = 0; u < 4; u++) {
             const int e = t + 256 * u, r = e >> 4, c = e & 15;
-            ra[u] = gi[(size_t)r * mw + c];
-            rb[u] = gj[(size_t)r * mw + c];
+            xa[u] = gi[(size_t)r * mw + c];
+            xb[u] = gj[(size_t)r * mw + c];
         }
     };
-    auto stash = [&](int buf) {
+    auto stash = [&](int buf, const uint2 (&xa)[4], const uint2 (&xb)[4]) {
         uint2* pa = s_a + buf * kB * kLd2;
         uint2* pb = s_b + buf * kB * kLd2;
+#pragma unroll
         for (int u = 0; u < 4; u++) {
             const int e = t + 256 * u, r = e >> 4, c = e & 15;
-            pa[r * kLd2 + c] = ra[u];
-            pb[r * kLd2 + c] = rb[u];
+            pa[r * kLd2 + c] = xa[u];
+            pb[r * kLd2 + c] = xb[u];
         }
     };
-    fetch(0);
-    stash(0);
-    __syncthreads();
-    for (int k = 0; k < j; k++) {
-        const int buf = k & 1;
-        if (k + 1 < j) fetch(k + 1);
-        const uint2* pa = s_a + buf * kB * kLd2 + (wr * 32 + (l & 31)) * kLd2 + (l >> 5);
-        const uint2* pb = s_b + buf * kB * kLd2 + (wc * 32 + (l & 31)) * kLd2 + (l >> 5);
 #pragma unroll
-        for (int kk = 0; kk < kB / 4; kk += 2) acc = mfma32<BF16>(pa[kk], pb[kk], acc);
-        if (k + 1 < j) stash(buf ^ 1);
-        __syncthreads();
+    for (int q = 0; q < kDepth; q++)
+        if (q < j) fetch(q, ra[q], rb[q]);
+    stash(0, ra[0], rb[0]);
+    __syncthreads();
+    for (int k0 = 0; k0 < j; k0 += kDepth) {
+#pragma unroll
+        for (int q = 0; q < kDepth; q++) {
+            const int k = k0 + q;
+            if (k < j) {
+                const int buf = k & 1;
+                if (k + kDepth < j) fetch(k + kDepth, ra[q], rb[q]);   // slot q held tile k, already staged
+                const uint2* pa = s_a + buf * kB * kLd2 + (wr * 32 + (l & 31)) * kLd2 + (l >> 5);
+                const uint2* pb = s_b + buf * kB * kLd2 + (wc * 32 + (l & 31)) * kLd2 + (l >> 5);
+#pragma unroll
+                for (int kk = 0; kk < kB / 4; kk += 2) acc = mfma32<BF16>(pa[kk], pb[kk], acc);
+                if (k + 1 < j) stash(buf ^ 1, ra[(q + 1) % kDepth], rb[(q + 1) % kDepth]);
+                __syncthreads();
+            }
+        }
     }
 }
 
@@ -280,12 +292,14 @@ __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restr
         }
         unsigned short lb[4][4];
         for (int u = 0; u < 4; u++)
-            for (int v = 0; v < 4; v++) {
-                const unsigned short b = to_lp<BF16>(o[u][v]);
-                lb[u][v] = b;
-                g[(size_t)(rb + u) * m + cb + v] = b;
-                gt[(size_t)(cb + v) * m + rb + u] = b;
-            }
+            for (int v = 0; v < 4; v++) lb[u][v] = to_lp<BF16>(o[u][v]);
+        // four 16-bit values per 8-byte store: rows of L_ij and rows of its transpose
+        for (int u = 0; u < 4; u++)
+            *reinterpret_cast<uint2*>(g + (size_t)(rb + u) * m + cb) =
+                make_uint2((unsigned)lb[u][0] | ((unsigned)lb[u][1] << 16), (unsigned)lb[u][2] | ((unsigned)lb[u][3] << 16));
+        for (int v = 0; v < 4; v++)
+            *reinterpret_cast<uint2*>(gt + (size_t)(cb + v) * m + rb) =
+                make_uint2((unsigned)lb[0][v] | ((unsigned)lb[1][v] << 16), (unsigned)lb[2][v] | ((unsigned)lb[3][v] << 16));
         __syncthreads();   // s_s and s_x are read no more: their LDS now stages L_ij
         for (int u = 0; u < 4; u++)
             s_a[(rb + u) * kLd2 + (cb >> 2)] = make_uint2((unsigned)lb[u][0] | ((unsigned)lb[u][1] << 16),

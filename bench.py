@@ -52,6 +52,7 @@ def parse():
     p.add_argument("--clusters-only", action="store_true",
                    help="run only the cluster-solve measurement and print its object (for profiling)")
     p.add_argument("--no-dense", action="store_true", help="skip the dense cluster-response measurement")
+    p.add_argument("--no-saline", action="store_true", help="skip the 96,000-site saline box measurement")
     p.add_argument("--dense-only", action="store_true",
                    help="run only the dense cluster-response measurement and print its object (for profiling)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
@@ -73,6 +74,7 @@ def make_handle(s, precision):
 
 def time_path(torch, s, precision, steps, warmup, timing=False):
     h = make_handle(s, precision)
+    tiles_per_item = h.autotune(s.positions, s.box)   # schedule chosen on this configuration, untimed
     dev = torch.device("cuda:0")
     pos = torch.tensor(s.positions.reshape(-1), dtype=torch.float64, device=dev)
     frc = torch.zeros(pos.numel(), dtype=torch.float64, device=dev)
@@ -97,6 +99,7 @@ def time_path(torch, s, precision, steps, warmup, timing=False):
     e, f = h.execute(s.positions, s.box)
     mu = h.dipoles(s.positions, s.box, _lib.DIPOLES_INDUCED)
     h.close()
+    stats["tiles_per_item"] = tiles_per_item
     return dict(seconds=dt, ms_per_step=1000.0 * dt / steps, stats=stats, energy=e, forces=f, mu=mu)
 
 
@@ -160,6 +163,7 @@ def cluster_bench(torch, steps, warmup):
     res = {}
     for prec in ("mixed", "double"):
         b = _lib.HipBatchHandle([c.force for c in cl], precision=prec, solver="pcg")
+        tiles_per_item = b.autotune(pos_h)
         for _ in range(warmup):
             b.execute_device(pos.data_ptr(), frc.data_ptr(), en.data_ptr(), stream)
         torch.cuda.synchronize()
@@ -172,9 +176,9 @@ def cluster_bench(torch, steps, warmup):
         e, f = b.execute(pos_h)
         mu = b.dipoles(pos_h, None, _lib.DIPOLES_INDUCED)
         b.close()
-        res[prec] = dict(seconds=dt, iters=iters, energy=e, forces=f, mu=mu)
-        print("# clusters batched %s: %.2f ms per set of %d, PCG iterations %d..%d" % (
-            prec, dt * 1e3, len(cl), min(iters), max(iters)), file=sys.stderr, flush=True)
+        res[prec] = dict(seconds=dt, iters=iters, energy=e, forces=f, mu=mu, tiles_per_item=tiles_per_item)
+        print("# clusters batched %s: %.2f ms per set of %d, PCG iterations %d..%d, %d tiles per work item" % (
+            prec, dt * 1e3, len(cl), min(iters), max(iters), tiles_per_item), file=sys.stderr, flush=True)
     # one FP64 context per cluster, evaluated in turn on device buffers
     handles = [_lib.HipContextHandle(c.force, np.zeros(9), precision="double", solver="pcg") for c in cl]
     offs = np.concatenate([[0], np.cumsum(sizes)])
@@ -207,6 +211,7 @@ def cluster_bench(torch, steps, warmup):
                                                  np.sqrt(np.mean(d["forces"] ** 2))),
         "mixed_mu_max_err_vs_fp64_debye": float(np.max(np.abs(m["mu"] - d["mu"]))) / DEBYE,
         "pcg_iterations_max": int(max(d["iters"])),
+        "tiles_per_work_item": {"mixed": res["mixed"]["tiles_per_item"], "fp64": res["double"]["tiles_per_item"]},
     }
 
 
@@ -261,6 +266,32 @@ def dense_bench(steps):
     return out
 
 
+def saline_bench(torch, steps):
+    """96,000-site box of water with 0.15 M Na+/Cl- (7.1 nm, Thole a in [0.3, 0.45]), Mutual PCG to
+    1e-6 D: mixed and FP64 evaluations per second and the mixed result's error against FP64."""
+    from tholedipoleplugin_amd import systems
+    s = systems.config2()
+    mixed = time_path(torch, s, "mixed", steps, 1)
+    print("# saline96000 mixed: %.2f ms/step, %s" % (mixed["ms_per_step"], mixed["stats"]), file=sys.stderr,
+          flush=True)
+    fp64 = time_path(torch, s, "double", max(1, steps - 1), 1)
+    print("# saline96000 fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr,
+          flush=True)
+    a, b = mixed["forces"].reshape(-1), fp64["forces"].reshape(-1)
+    return {
+        "workload": "96,000 sites: 31,978 waters + 33 Na+/Cl- pairs, 7.1 nm box, min-image all pairs, Thole a in "
+                    "[0.3, 0.45], Mutual PCG to 1e-6 D",
+        "unit": "evals/s",
+        "mixed": 1000.0 / mixed["ms_per_step"],
+        "fp64": 1000.0 / fp64["ms_per_step"],
+        "mixed_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+        "pcg_iterations": mixed["stats"]["iterations"],
+        "mixed_force_rel_rms_err_vs_fp64": float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))),
+        "mixed_energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
+        "mixed_mu_max_err_vs_fp64_debye": float(np.max(np.abs(mixed["mu"] - fp64["mu"]))) / DEBYE,
+    }
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -311,6 +342,9 @@ def main():
     dense = None
     if not args.no_dense:
         dense = dense_bench(max(1, min(args.steps, 2)))
+    saline = None
+    if not args.no_saline:
+        saline = saline_bench(torch, max(2, min(args.steps, 3)))
     cpu = None
     if not args.no_cpu_baseline:
         from tholedipoleplugin_amd import _lib
@@ -336,7 +370,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": "water12288_36864_sites_box4.95nm_allpairs_min_image_mutual_1e-6D",
                    "sites": n, "solver": "pcg", "precision": "mixed",
-                   "pcg_iterations": mixed["stats"]["iterations"]},
+                   "pcg_iterations": mixed["stats"]["iterations"],
+                   "tiles_per_work_item": mixed["stats"]["tiles_per_item"]},
         "fp64_path": None if fp64 is None else {"evals_per_s": 1000.0 / fp64["ms_per_step"],
                                                 "ms_per_step": fp64["ms_per_step"],
                                                 "iterations": fp64["stats"]["iterations"]},
@@ -354,6 +389,7 @@ def main():
         "cpu_baseline": cpu,
         "clusters": clusters,
         "dense_clusters": dense,
+        "saline96000": saline,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:

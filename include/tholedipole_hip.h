@@ -149,6 +149,15 @@ int tdp_get_solver_stats(const tdp_context* ctx, tdp_solver_stats* stats);
  * context's stream around the launch; the totals are reported by tdp_get_solver_stats. */
 int tdp_set_timing(tdp_context* ctx, int32_t enable);
 
+/* Schedule autotuner (no counterpart in the reference): times the T*mu pass on the given positions
+ * (host [3N], caller's order; box host [9] or NULL as for tdp_execute) for each candidate length of
+ * the pair kernels' work items (1, 2, 3, 4, 6, 8, 12 or 16 tiles of one tile row) and keeps the
+ * fastest for the context; *tiles_per_item (may be NULL) receives it.  Without it a context uses
+ * min(3, max(1, tile pairs / 16,384)) tiles.  The choice changes only the order in
+ * which partial sums are added, i.e. results at the rounding level; after it, evaluations of the
+ * context are deterministic again. */
+int tdp_autotune(tdp_context* ctx, const double* positions, const double* box, int32_t* tiles_per_item);
+
 /* ---- Batches of independent systems.  No counterpart in the reference, whose kernel holds one
  * system: these entry points evaluate many NoCutoff systems (clusters) at once, each with the
  * same semantics as its own tdp_create context (same pair terms, scale maps, solver and stopping

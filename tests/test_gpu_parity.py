@@ -227,6 +227,28 @@ def test_device_entry_point_matches_host():
     assert r.returncode == 0 and "OK" in r.stdout
 
 
+def test_autotune_keeps_results():
+    """tdp_autotune picks one of its candidate work-item lengths and changes results only at the
+    rounding level; repeated evaluations after it are identical."""
+    from tholedipoleplugin_amd import _lib
+    _say("case autotune")
+    s = systems.config0()
+    s.force.setMutualInducedTargetEpsilon(1e-10)
+    p0 = np.asarray(s.positions, np.float64)
+    h0 = _lib.HipContextHandle(s.force, s.box, precision="double", solver="pcg")
+    e0, f0 = h0.execute(p0, s.box)
+    h0.close()
+    h = _lib.HipContextHandle(s.force, s.box, precision="double", solver="pcg")
+    C = h.autotune(p0, s.box)
+    assert C in (1, 2, 3, 4, 6, 8, 12, 16)
+    e1, f1 = h.execute(p0, s.box)
+    e2, f2 = h.execute(p0, s.box)
+    h.close()
+    assert e1 == e2 and np.array_equal(f1, f2)
+    assert abs(e1 - e0) <= 1e-12 * abs(e0)
+    assert rel_rms(f1, f0) <= 1e-10
+
+
 @pytest.mark.parametrize("solver", ["pcg", "jacobi"])
 def test_iteration_graph_replays(solver, monkeypatch):
     """Solver iterations replayed from captured HIP graphs give the same results as iterations

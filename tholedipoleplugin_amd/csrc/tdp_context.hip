@@ -62,6 +62,8 @@ struct tdp_context {
     bool ordered = false;
     int evals_since_order = 0;
     size_t cap_exc_J = 0, cap_exc_mask = 0, cap_gidx = 0;
+    size_t cap_work = 0, cap_pbuf = 0, cap_pbuf2 = 0, cap_part_e = 0;
+    int work_C = 0;   // tiles per upper-triangle work item
     std::vector<int> exc_start, exc_J;
     std::vector<unsigned long long> exc_mask;
     std::vector<int> goff, gidx;
@@ -856,6 +858,45 @@ int upload_positions(tdp_context* c, const double* positions) {
     return TDP_OK;
 }
 
+// Upper-triangle work items: tile row I of a system split into chunks of C tiles (J >= I, same
+// system), with the item partial buffers sized for them.  Short items keep the many one-wave
+// workgroups evenly spread over the CUs to the end of a launch; long items need fewer item
+// partials in the reduction.
+int set_work_items(tdp_context* c, int C) {
+    const int nb = c->nblk, ns = c->nseg;
+    std::vector<int4> work;
+    std::vector<int> row_start(nb + 1, 0);
+    for (int g = 0; g < ns; g++) {
+        c->seg_w0[g] = (int)work.size();
+        const int t1 = c->seg_t1[g];
+        for (int I = c->seg_t0[g]; I < t1; I++) {
+            row_start[I] = (int)work.size();
+            for (int J0 = I; J0 < t1; J0 += C) work.push_back(make_int4(I, J0, std::min(t1, J0 + C), 0));
+        }
+        c->seg_w1[g] = (int)work.size();
+    }
+    row_start[nb] = (int)work.size();
+    c->nwork = (int)work.size();
+    c->work_C = C;
+    c->graph_gen++;
+    std::vector<int2> segw(ns);
+    for (int g = 0; g < ns; g++) segw[g] = make_int2(c->seg_w0[g], c->seg_w1[g]);
+    const size_t pb = ((size_t)c->nwork + c->ntri) * 64;
+    int rc;
+    if ((rc = dresize(c, &c->d_work, c->cap_work, work.size())) ||
+        (rc = dresize(c, (double4**)&c->d_pbuf, c->cap_pbuf, pb)) || (rc = dresize(c, &c->d_pbuf2, c->cap_pbuf2, pb)) ||
+        (rc = dresize(c, &c->d_part_e, c->cap_part_e, (size_t)c->nwork + 1)))
+        return rc;
+    if (!c->d_row_start && (rc = dalloc(c, &c->d_row_start, nb + 1))) return rc;
+    if (!c->d_segw && (rc = dalloc(c, &c->d_segw, ns))) return rc;
+    TDP_HIP(c, hipMemcpyAsync(c->d_work, work.data(), work.size() * sizeof(int4), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_row_start, row_start.data(), row_start.size() * sizeof(int), hipMemcpyHostToDevice,
+                              c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_segw, segw.data(), ns * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    return TDP_OK;
+}
+
 // Tiles, work items and partial-slab layout of the systems (sizes in the caller's order).
 void build_layout(tdp_context* c, const std::vector<int>& sizes) {
     c->nseg = (int)sizes.size();
@@ -958,51 +999,26 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         c->use_graphs = !(g && g[0] == '0');
     }
     const int npad = c->npad, nb = c->nblk, ns = c->nseg;
-    // Work items: tile row I of a system split into chunks of C tiles (J >= I, same system).
-    // Short items (C = 3 on large systems) keep the many one-wave workgroups evenly spread over the
-    // CUs to the end of the launch; measured at 36,864 sites, C = 3 beats C = 20 by 25% on the
-    // T*mu pass, and C = 1 loses more in the reduction than it gains.
+    // Default work items: C = 3 tiles on large systems (measured at 36,864 sites, C = 3 beats C = 20
+    // by 25% on the T*mu pass, and C = 1 loses more in the reduction than it gains); tdp_autotune
+    // times the candidates on the caller's configuration instead.
+    if ((rc = set_work_items(c, (int)std::max<size_t>(1, std::min<size_t>(3, c->ntri / 16384))))) return bail(rc);
     {
-        const int C = (int)std::max<size_t>(1, std::min<size_t>(3, c->ntri / 16384));
-        std::vector<int4> work;
-        std::vector<int> row_start(nb + 1, 0);
-        for (int g = 0; g < ns; g++) {
-            c->seg_w0[g] = (int)work.size();
-            const int t1 = c->seg_t1[g];
-            for (int I = c->seg_t0[g]; I < t1; I++) {
-                row_start[I] = (int)work.size();
-                for (int J0 = I; J0 < t1; J0 += C) work.push_back(make_int4(I, J0, std::min(t1, J0 + C), 0));
-            }
-            c->seg_w1[g] = (int)work.size();
-        }
-        row_start[nb] = (int)work.size();
-        c->nwork = (int)work.size();
         std::vector<int4> seg(ns);
-        std::vector<int2> segw(ns);
-        for (int g = 0; g < ns; g++) {
-            seg[g] = make_int4(c->seg_t0[g], c->seg_t1[g], c->seg_n[g], 0);
-            segw[g] = make_int2(c->seg_w0[g], c->seg_w1[g]);
-        }
-        if ((rc = dalloc(c, &c->d_work, work.size())) || (rc = dalloc(c, &c->d_row_start, nb + 1)) ||
-            (rc = dalloc(c, (double4**)&c->d_pbuf, ((size_t)c->nwork + c->ntri) * 64)) ||
-            (rc = dalloc(c, &c->d_pbuf2, ((size_t)c->nwork + c->ntri) * 64)) ||
-            (rc = dalloc(c, &c->d_part_e, (size_t)c->nwork + 1)) || (rc = dalloc(c, &c->Emv, npad)) ||
-            (rc = dalloc(c, &c->d_tcnt, nb)) || (rc = dalloc(c, &c->d_tfirst, nb)) || (rc = dalloc(c, &c->d_tseg, nb)) ||
-            (rc = dalloc(c, &c->d_tcol, nb)) || (rc = dalloc(c, &c->d_seg, ns)) || (rc = dalloc(c, &c->d_segw, ns)) ||
+        for (int g = 0; g < ns; g++) seg[g] = make_int4(c->seg_t0[g], c->seg_t1[g], c->seg_n[g], 0);
+        if ((rc = dalloc(c, &c->Emv, npad)) || (rc = dalloc(c, &c->d_tcnt, nb)) || (rc = dalloc(c, &c->d_tfirst, nb)) ||
+            (rc = dalloc(c, &c->d_tseg, nb)) || (rc = dalloc(c, &c->d_tcol, nb)) || (rc = dalloc(c, &c->d_seg, ns)) ||
             (rc = dalloc(c, &c->d_done, 2)))
             return bail(rc);
         bool ok = true;
         auto up = [&](void* dst, const void* src, size_t bytes) {
             if (bytes && hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream) != hipSuccess) ok = false;
         };
-        up(c->d_work, work.data(), work.size() * sizeof(int4));
-        up(c->d_row_start, row_start.data(), row_start.size() * sizeof(int));
         up(c->d_tcnt, c->tcnt.data(), nb * sizeof(int));
         up(c->d_tfirst, c->tfirst.data(), nb * sizeof(int));
         up(c->d_tseg, c->tseg.data(), nb * sizeof(int));
         up(c->d_tcol, c->tcol.data(), nb * sizeof(size_t));
         up(c->d_seg, seg.data(), ns * sizeof(int4));
-        up(c->d_segw, segw.data(), ns * sizeof(int2));
         if (!ok || hipStreamSynchronize(c->stream) != hipSuccess) {
             c->err = "hipMemcpy of the tile tables failed";
             return bail(TDP_ERR_HIP);
@@ -1550,6 +1566,44 @@ int tdp_batch_execute(tdp_context* c, const double* positions, double* forces, d
 int tdp_batch_execute_device(tdp_context* c, const double* d_positions, double* d_forces, double* d_energies,
                              void* stream) {
     return tdp_execute_device(c, d_positions, nullptr, d_forces, d_energies, stream);
+}
+
+int tdp_autotune(tdp_context* c, const double* positions, const double* box, int32_t* tiles_per_item) {
+    if (!c || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    if (c->n > 0) {
+        if ((rc = upload_positions(c, positions))) return rc;
+        if ((rc = prepare_field(c, box, positions))) return rc;   // internal order, tile tables, mu = alpha E0
+        int longest = 1;
+        for (int g = 0; g < c->nseg; g++) longest = std::max(longest, c->seg_t1[g] - c->seg_t0[g]);
+        static const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16};
+        int best = c->work_C;
+        float best_ms = 0.f;
+        hipEvent_t e0 = take_event(c), e1 = take_event(c);
+        for (int C : cand) {
+            if (C > longest && C != 1) break;
+            if ((rc = set_work_items(c, C))) return rc;
+            PairLaunch L = pair_launch(c, box);
+            matvec(c, L, c->mu, nullptr);   // warm-up
+            TDP_HIP(c, hipEventRecord(e0, c->stream));
+            for (int k = 0; k < 3; k++) matvec(c, L, c->mu, nullptr);
+            TDP_HIP(c, hipEventRecord(e1, c->stream));
+            TDP_HIP(c, hipEventSynchronize(e1));
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (best_ms == 0.f || ms < best_ms) {
+                best_ms = ms;
+                best = C;
+            }
+        }
+        c->ev_pool.push_back(e0);
+        c->ev_pool.push_back(e1);
+        if ((rc = set_work_items(c, best))) return rc;
+    }
+    if (tiles_per_item) *tiles_per_item = c->work_C;
+    return TDP_OK;
 }
 
 int tdp_set_timing(tdp_context* c, int32_t enable) {

@@ -1,11 +1,11 @@
 // tdp_kernels.hip -- gfx950 kernels of the TholeDipoleForce force + induction path.
 //
-// Pair kernels use a "full" all-pairs formulation: a workgroup is one 64-lane wavefront that
-// owns a tile of 64 particles i (one per lane) and sweeps a contiguous range of j tiles; each
-// j tile is staged once into LDS with coalesced loads and then read back as wave-wide
-// broadcasts, so each lane accumulates its own particle's sums without atomics.  The j range
-// of one i tile is split over gridDim.y workgroups; the splits write separate partial rows that
-// the consumer kernel adds in a fixed order, so every result is deterministic.
+// Pair kernels visit every pair once: a workgroup is one 64-lane wavefront that owns a tile of
+// 64 particles i (one per lane) and sweeps a short range of j tiles J >= I (a work item); each j
+// tile is staged once into LDS with coalesced loads.  Lane l keeps the sums of its own particle in
+// registers and pairs with slot (l + k) mod 64 at step k, while the j-side sums rotate one lane per
+// step (DPP).  Work items write their i sums and each tile pair's j sums to separate partial
+// buffers that k_col_reduce adds in a fixed order, so every result is deterministic.
 //
 // Exclusions/scalings come from per-tile bit masks built on the host (tdp_context.hip):
 // for a tile pair (I,J) that holds at least one scaled pair, three 64-bit masks per lane mark
@@ -19,7 +19,6 @@ using namespace tdp;
 namespace {
 
 struct BoxT64 { double bx, by, bz, ibx, iby, ibz; };
-struct BoxT32 { float bx, by, bz, ibx, iby, ibz; };
 
 // Scalars shared between the solver kernels (device memory, see SolverScalars in tdp_kernels.h).
 __device__ __forceinline__ bool solver_done(const double* sc) { return sc[SC_DONE] != 0.0; }

@@ -149,6 +149,13 @@ int tdp_get_solver_stats(const tdp_context* ctx, tdp_solver_stats* stats);
  * context's stream around the launch; the totals are reported by tdp_get_solver_stats. */
 int tdp_set_timing(tdp_context* ctx, int32_t enable);
 
+/* Warm start (no counterpart in the reference, which starts every Mutual solve from alpha E0,
+ * ReferenceTholeDipoleForce.cpp:735-764): with enable != 0 each later Mutual solve of the context
+ * starts from the induced dipoles of its previous solve, as an MD loop would use it; the stopping
+ * rule is unchanged, so the result differs from a cold start by less than the solver tolerance.
+ * Enabling or disabling forgets the stored dipoles. */
+int tdp_set_warm_start(tdp_context* ctx, int32_t enable);
+
 /* Schedule autotuner (no counterpart in the reference): times the T*mu pass on the given positions
  * (host [3N], caller's order; box host [9] or NULL as for tdp_execute) for each candidate length of
  * the pair kernels' work items (1, 2, 3, 4, 6, 8, 12 or 16 tiles of one tile row) and keeps the

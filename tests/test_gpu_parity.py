@@ -227,6 +227,40 @@ def test_device_entry_point_matches_host():
     assert r.returncode == 0 and "OK" in r.stdout
 
 
+@pytest.mark.parametrize("solver", ["pcg", "jacobi"])
+def test_warm_start_mutual(solver):
+    """A warm-started Mutual solve (previous evaluation's dipoles as the initial guess) reaches the
+    cold-start dipoles within the bar in fewer iterations, over a short sequence of perturbed
+    configurations (the next configuration of an MD loop)."""
+    from tholedipoleplugin_amd import _lib
+    _say("case warm start %s" % solver)
+    s = systems.config0()
+    s.force.setMutualInducedTargetEpsilon(1e-9)
+    s.force.setMutualInducedMaxIterations(500)
+    rng = np.random.default_rng(8)
+    p = np.asarray(s.positions, np.float64)
+    seq = [p]
+    for _ in range(3):
+        seq.append(seq[-1] + rng.normal(0.0, 0.0005, p.shape))
+    hw = _lib.HipContextHandle(s.force, s.box, precision="double", solver=solver)
+    hw.set_warm_start(True)
+    hc = _lib.HipContextHandle(s.force, s.box, precision="double", solver=solver)
+    for k, pos in enumerate(seq):
+        ew, fw = hw.execute(pos, s.box)
+        ec, fc = hc.execute(pos, s.box)
+        nw, nc = hw.solver_stats()["iterations"], hc.solver_stats()["iterations"]
+        muw = hw.dipoles(pos, s.box, _lib.DIPOLES_INDUCED)
+        muc = hc.dipoles(pos, s.box, _lib.DIPOLES_INDUCED)
+        assert hw.solver_stats()["converged"] == 1
+        assert np.max(np.abs(muw - muc)) <= MU_TOL
+        assert abs(ew - ec) <= E_TOL * abs(ec)
+        assert rel_rms(fw, fc) <= F_TOL
+        if k > 0:
+            assert nw < nc, (k, nw, nc)
+    hw.close()
+    hc.close()
+
+
 def test_autotune_keeps_results():
     """tdp_autotune picks one of its candidate work-item lengths and changes results only at the
     rounding level; repeated evaluations after it are identical."""

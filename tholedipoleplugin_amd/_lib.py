@@ -28,7 +28,7 @@ EXPORTED_SYMBOLS = [
     "tdp_create", "tdp_destroy", "tdp_last_error", "tdp_last_create_error", "tdp_copy_parameters",
     "tdp_execute", "tdp_execute_device", "tdp_get_dipoles", "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
-    "tdp_autotune",
+    "tdp_autotune", "tdp_set_warm_start",
     "tdp_batch_create", "tdp_batch_num_systems", "tdp_batch_execute", "tdp_batch_execute_device",
     "tdp_batch_get_solver_stats", "tdp_batch_solve_responses",
 ]
@@ -97,6 +97,7 @@ def load():
     L.tdp_get_solver_stats.argtypes = [ctx_p, C.POINTER(SolverStats)]
     L.tdp_set_timing.argtypes = [ctx_p, C.c_int32]
     L.tdp_autotune.argtypes = [ctx_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
+    L.tdp_set_warm_start.argtypes = [ctx_p, C.c_int32]
     L.tdp_batch_create.argtypes = [C.POINTER(ForceDesc), C.c_int32, C.POINTER(Options), C.POINTER(ctx_p)]
     L.tdp_batch_num_systems.argtypes = [ctx_p]
     L.tdp_batch_execute.argtypes = [ctx_p, _dp, _dp, _dp]
@@ -239,6 +240,10 @@ class HipContextHandle:
 
     def set_timing(self, enable=True):
         self._check(self.L.tdp_set_timing(self.h, 1 if enable else 0))
+
+    def set_warm_start(self, enable=True):
+        """Later Mutual solves start from the previous solve's induced dipoles (tdp_set_warm_start)."""
+        self._check(self.L.tdp_set_warm_start(self.h, 1 if enable else 0))
 
     def autotune(self, positions, box=None):
         """Times the pair kernels' work-item lengths on these positions and keeps the fastest

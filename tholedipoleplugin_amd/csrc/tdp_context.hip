@@ -122,6 +122,9 @@ struct tdp_context {
     // ---- results of the last solve (per system)
     std::vector<int> iterations, converged;
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
+    // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
+    bool warm_start = false, have_prev = false;
+    double* d_mu_prev = nullptr;
     std::vector<double> epsilon;
     // ---- timing
     bool timing = false;
@@ -787,7 +790,15 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         if (c->timing) harvest_timing(c);
         return TDP_OK;
     }
-    return solve_mutual(c, box);
+    if (c->warm_start && c->have_prev)   // start from the previous solution instead of alpha E0
+        launch_gather_vec3(s, npad, c->d_perm, c->d_mu_prev, c->mu, mixed ? c->vec32 : nullptr);
+    rc = solve_mutual(c, box);
+    if (rc) return rc;
+    if (c->warm_start) {
+        launch_scatter_vec3(s, npad, c->d_perm, c->mu, c->d_mu_prev);
+        c->have_prev = true;
+    }
+    return TDP_OK;
 }
 
 // calculateElectrostatic + mapTorqueToForce (ReferenceTholeDipoleForce.cpp:118-124) into d_frc
@@ -1603,6 +1614,18 @@ int tdp_autotune(tdp_context* c, const double* positions, const double* box, int
         if ((rc = set_work_items(c, best))) return rc;
     }
     if (tiles_per_item) *tiles_per_item = c->work_C;
+    return TDP_OK;
+}
+
+int tdp_set_warm_start(tdp_context* c, int32_t enable) {
+    if (!c) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null context");
+    hipSetDevice(c->device);
+    if (enable && !c->d_mu_prev) {
+        int rc = dalloc(c, &c->d_mu_prev, 3 * (size_t)std::max(c->n, 1));
+        if (rc) return rc;
+    }
+    c->warm_start = enable != 0;
+    c->have_prev = false;
     return TDP_OK;
 }
 

@@ -83,6 +83,9 @@ void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4*
 // posa: FP64 positions (internal order) and Thole parameter; slots with perm < 0 are padding.
 void launch_load_positions(hipStream_t s, int npad, const double* pos3, const int* perm, const double* thole,
                            double4* posa);
+// caller's order (3 doubles per particle) <-> internal order (double4 per slot; dst32 may be null)
+void launch_gather_vec3(hipStream_t s, int npad, const int* perm, const double* src3, double4* dst4, float4* dst32);
+void launch_scatter_vec3(hipStream_t s, int npad, const int* perm, const double4* src4, double* dst3);
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,
                    const int4* axis, double4* qd);
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,

@@ -106,6 +106,30 @@ __global__ void k_load_positions(int npad, const double* __restrict__ pos3, cons
     posa[i] = p;
 }
 
+// Vectors between the caller's order (3 doubles per particle) and the internal order (double4 per
+// slot, with an optional FP32 copy); slots with perm < 0 are padding (zero).
+__global__ void k_gather_vec3(int npad, const int* __restrict__ perm, const double* __restrict__ src3,
+                              double4* __restrict__ dst4, float4* __restrict__ dst32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    const int o = perm[i];
+    const double3 v = o >= 0 ? make_double3(src3[3 * o], src3[3 * o + 1], src3[3 * o + 2]) : make_double3(0.0, 0.0, 0.0);
+    dst4[i] = make_double4(v.x, v.y, v.z, 0.0);
+    if (dst32) dst32[i] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+}
+
+__global__ void k_scatter_vec3(int npad, const int* __restrict__ perm, const double4* __restrict__ src4,
+                               double* __restrict__ dst3) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    const int o = perm[i];
+    if (o < 0) return;
+    const double4 v = src4[i];
+    dst3[3 * o] = v.x;
+    dst3[3 * o + 1] = v.y;
+    dst3[3 * o + 2] = v.z;
+}
+
 // ------------------------------------------------------------------ tile classification
 // Per tile of 64 particles (internal order), one wavefront: centre and half extent of its bounding
 // box (PBC: offsets from the tile's first particle wrapped to the minimum image, so a tile that
@@ -1734,6 +1758,14 @@ void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4*
                         float4* tb_c, float4* tb_h, double4* tb_o, float4* posl) {
     hipLaunchKernelGGL(k_tile_bounds, dim3(nblk), dim3(kTile), 0, s, tcnt, posa, box3[0], box3[1], box3[2],
                        pbc ? 1 : 0, tb_c, tb_h, tb_o, posl);
+}
+
+void launch_gather_vec3(hipStream_t s, int npad, const int* perm, const double* src3, double4* dst4, float4* dst32) {
+    hipLaunchKernelGGL(k_gather_vec3, dim3((npad + 255) / 256), dim3(256), 0, s, npad, perm, src3, dst4, dst32);
+}
+
+void launch_scatter_vec3(hipStream_t s, int npad, const int* perm, const double4* src4, double* dst3) {
+    hipLaunchKernelGGL(k_scatter_vec3, dim3((npad + 255) / 256), dim3(256), 0, s, npad, perm, src4, dst3);
 }
 
 void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol, const double* charge,

@@ -86,6 +86,7 @@ struct tdp_context {
             *q = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
     bool has_perm = true;   // some particle has a nonzero molecular dipole
+    float* d_slab32 = nullptr;   // mixed T*mu: FP32 tile partial slab (3 * ntri * 64)
     float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr, *posl = nullptr;
     double4* tb_o = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
@@ -564,15 +565,16 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
     }
     double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
     double* pj = reinterpret_cast<double*>(pi + (size_t)c->nwork * 64);
-    if (c->precision == TDP_PRECISION_MIXED)
-        launch_sym_matvec_mixed(c->stream, L, c->posa, v64, c->vec32, c->d_inva, scalars, pi, pj);
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    if (mixed)
+        launch_sym_matvec_mixed(c->stream, L, c->posa, v64, c->vec32, c->d_inva, scalars, pi, c->d_slab32);
     else
         launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pj);
     if (c->timing) {
         hipEventRecord(e1, c->stream);
         c->pending.push_back({e0, e1});
     }
-    launch_col_reduce(c->stream, L, pi, pj, scalars, c->Emv, nullptr);
+    launch_col_reduce(c->stream, L, pi, pj, scalars, c->Emv, nullptr, mixed ? c->d_slab32 : nullptr);
 }
 
 int check_box(tdp_context* c, const double* box) {
@@ -1048,7 +1050,9 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
         (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)) ||
-        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)))
+        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)) ||
+        (c->precision == TDP_PRECISION_MIXED &&
+         (rc = dalloc(c, &c->d_slab32, 3 * c->ntri * 64 + 1))))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
         c->hist_order = std::max(4, (int)c->extrap.size());

@@ -597,7 +597,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                                                          const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
                                                          const double4* __restrict__ tb_o,
                                                          const float4* __restrict__ posl, BoxT64 box, const double* __restrict__ scalars,
-                                                         double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
+                                                         double4* __restrict__ Pi, float* __restrict__ Pjf,
+                                                         size_t S) {
     // One LDS area holds either the near tile (FP64 dipoles and 1/a) or the far tile (FP32
     // dipoles).  The far tile is staged twice over, so that slot (lane + k) mod 64 is entry
     // lane + k: the step's address is the lane's base plus a constant offset.
@@ -803,7 +804,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             }
             kend = k1 + 1;
         }
-        store3(Pj, S, slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1)), jx, jy, jz);
+        // the j sums of a tile pair go to the FP32 slab: exact for far tile pairs (FP32 sums), one
+        // rounding of the FP64 sums of the other tile pairs
+        const size_t slot = slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1));
+        Pjf[slot] = (float)jx;
+        Pjf[S + slot] = (float)jy;
+        Pjf[2 * S + slot] = (float)jz;
     }
     Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
@@ -1166,9 +1172,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // in wave order.  The order is fixed, so the result is deterministic.  out4 (double4 per
 // particle) or out3 (three doubles per particle) receives the sums.
 constexpr int kRedWaves = 8;
+// F32: the tile partials are FP32 (the slab of the mixed T*mu pass, Pjf), else FP64 (Pj).
+template <bool F32>
 __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const int* __restrict__ row_start,
                                                                const double4* __restrict__ Pi,
-                                                               const double* __restrict__ Pj, size_t S,
+                                                               const double* __restrict__ Pj,
+                                                               const float* __restrict__ Pjf, size_t S,
                                                                const double* __restrict__ scalars,
                                                                double4* __restrict__ out4, double* __restrict__ out3) {
     const int B = blockIdx.x, l = threadIdx.x & (kTile - 1), v = threadIdx.x / kTile;
@@ -1186,9 +1195,15 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const
 #pragma unroll 4
     for (int Ip = first + v; Ip <= B; Ip += kRedWaves) {
         const size_t k = (col + (size_t)(Ip - first)) * kTile + l;
-        sx += Pj[k];
-        sy += Pj[S + k];
-        sz += Pj[2 * S + k];
+        if (F32) {
+            sx += (double)Pjf[k];
+            sy += (double)Pjf[S + k];
+            sz += (double)Pjf[2 * S + k];
+        } else {
+            sx += Pj[k];
+            sy += Pj[S + k];
+            sz += Pj[2 * S + k];
+        }
     }
     red[0][v][l] = sx;
     red[1][v][l] = sy;
@@ -1774,9 +1789,13 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
 }
 
 void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj,
-                       const double* scalars, double4* out4, double* out3) {
-    hipLaunchKernelGGL(k_col_reduce, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.tm, L.row_start, Pi, Pj,
-                       L.slab, scalars, out4, out3);
+                       const double* scalars, double4* out4, double* out3, const float* Pjf) {
+    if (Pjf)
+        hipLaunchKernelGGL(k_col_reduce<true>, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.tm, L.row_start, Pi, Pj,
+                           Pjf, L.slab, scalars, out4, out3);
+    else
+        hipLaunchKernelGGL(k_col_reduce<false>, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.tm, L.row_start, Pi, Pj,
+                           Pjf, L.slab, scalars, out4, out3);
 }
 
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
@@ -1791,16 +1810,16 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
 }
 
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
-                             const float4* mu32, const double* inva, const double* scalars, double4* Pi, double* Pj) {
+                             const float4* mu32, const double* inva, const double* scalars, double4* Pi, float* Pjf) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
                            mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
-                           Pi, Pj, L.slab);
+                           Pi, Pjf, L.slab);
     else
         hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
                            mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
-                           Pi, Pj, L.slab);
+                           Pi, Pjf, L.slab);
 }
 
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,

@@ -62,7 +62,7 @@ struct tdp_context {
     bool ordered = false;
     int evals_since_order = 0;
     size_t cap_exc_J = 0, cap_exc_mask = 0, cap_gidx = 0;
-    size_t cap_work = 0, cap_pbuf = 0, cap_pbuf2 = 0, cap_part_e = 0;
+    size_t cap_work = 0, cap_pbuf = 0, cap_pbuf2 = 0, cap_part_e = 0, cap_slab32 = 0;
     int work_C = 0;   // tiles per upper-triangle work item
     std::vector<int> exc_start, exc_J;
     std::vector<unsigned long long> exc_mask;
@@ -86,7 +86,7 @@ struct tdp_context {
             *q = nullptr, *torque = nullptr,
             *contrib = nullptr, *hist = nullptr;
     bool has_perm = true;   // some particle has a nonzero molecular dipole
-    float* d_slab32 = nullptr;   // mixed T*mu: FP32 tile partial slab (3 * ntri * 64)
+    float* d_slab32 = nullptr;   // mixed T*mu: FP32 tile partial slab (3 * ntri * 64), then item partials
     float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr, *posl = nullptr;
     double4* tb_o = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
@@ -567,7 +567,7 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
     double* pj = reinterpret_cast<double*>(pi + (size_t)c->nwork * 64);
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     if (mixed)
-        launch_sym_matvec_mixed(c->stream, L, c->posa, v64, c->vec32, c->d_inva, scalars, pi, c->d_slab32);
+        launch_sym_matvec_mixed(c->stream, L, c->posa, v64, c->vec32, c->d_inva, scalars, c->d_slab32);
     else
         launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pj);
     if (c->timing) {
@@ -898,7 +898,9 @@ int set_work_items(tdp_context* c, int C) {
     int rc;
     if ((rc = dresize(c, &c->d_work, c->cap_work, work.size())) ||
         (rc = dresize(c, (double4**)&c->d_pbuf, c->cap_pbuf, pb)) || (rc = dresize(c, &c->d_pbuf2, c->cap_pbuf2, pb)) ||
-        (rc = dresize(c, &c->d_part_e, c->cap_part_e, (size_t)c->nwork + 1)))
+        (rc = dresize(c, &c->d_part_e, c->cap_part_e, (size_t)c->nwork + 1)) ||
+        (c->precision == TDP_PRECISION_MIXED &&
+         (rc = dresize(c, &c->d_slab32, c->cap_slab32, 3 * ((size_t)c->ntri + c->nwork) * 64))))
         return rc;
     if (!c->d_row_start && (rc = dalloc(c, &c->d_row_start, nb + 1))) return rc;
     if (!c->d_segw && (rc = dalloc(c, &c->d_segw, ns))) return rc;
@@ -1050,9 +1052,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
         (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)) ||
-        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)) ||
-        (c->precision == TDP_PRECISION_MIXED &&
-         (rc = dalloc(c, &c->d_slab32, 3 * c->ntri * 64 + 1))))
+        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
         c->hist_order = std::max(4, (int)c->extrap.size());

@@ -69,14 +69,14 @@ struct PairLaunch {
 // Pairwise passes over the upper-triangle work items.  Pi: nwork*64 item partials; Pj: the
 // column-major tile partial slab, 3 * L.slab doubles.  The T*mu passes leave their partials for
 // launch_col_reduce, which writes T*mu to out4 (double4 per particle) or out3 (3 doubles).  The mixed
-// pass keeps its tile partials in an FP32 slab Pjf (3 * L.slab floats), which the reduction after it
-// reads (Pjf non-null) in place of Pj.
+// pass keeps its partials in FP32 in Pjf: the tile slab (3 * L.slab floats), then the item partials
+// (3 * nwork * 64 floats, SoA); the reduction after it reads them (Pjf non-null) in place of Pj, Pi.
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
                          const double* scalars, double4* Pi, double* Pj);
 void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj, const double* scalars,
                        double4* out4, double* out3, const float* Pjf = nullptr);
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
-                             const float4* mu32, const double* inva, const double* scalars, double4* Pi, float* Pjf);
+                             const float4* mu32, const double* inva, const double* scalars, float* Pjf);
 
 // tb_o: tile origins (FP64), posl: FP32 offsets of the particles from their tile origin
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,

@@ -597,8 +597,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                                                          const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
                                                          const double4* __restrict__ tb_o,
                                                          const float4* __restrict__ posl, BoxT64 box, const double* __restrict__ scalars,
-                                                         double4* __restrict__ Pi, float* __restrict__ Pjf,
-                                                         size_t S) {
+                                                         float* __restrict__ Pjf, size_t S) {
     // One LDS area holds either the near tile (FP64 dipoles and 1/a) or the far tile (FP32
     // dipoles).  The far tile is staged twice over, so that slot (lane + k) mod 64 is entry
     // lane + k: the step's address is the lane's base plus a constant offset.
@@ -811,7 +810,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
         Pjf[S + slot] = (float)jy;
         Pjf[2 * S + slot] = (float)jz;
     }
-    Pi[(size_t)w * kTile + lane] = make_double4(ax, ay, az, 0.0);
+    // item partials in FP32 behind the slab: component c of item w at Pjf[3 S + c NW + 64 w + lane]
+    const size_t NW = (size_t)gridDim.x * kTile, iw = 3 * S + (size_t)w * kTile + lane;
+    Pjf[iw] = (float)ax;
+    Pjf[iw + NW] = (float)ay;
+    Pjf[iw + 2 * NW] = (float)az;
 }
 
 
@@ -1172,7 +1175,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_
 // in wave order.  The order is fixed, so the result is deterministic.  out4 (double4 per
 // particle) or out3 (three doubles per particle) receives the sums.
 constexpr int kRedWaves = 8;
-// F32: the tile partials are FP32 (the slab of the mixed T*mu pass, Pjf), else FP64 (Pj).
+// F32: the partials are FP32 (the slab and item partials of the mixed T*mu pass in Pjf), else FP64
+// (Pj and Pi).
 template <bool F32>
 __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const int* __restrict__ row_start,
                                                                const double4* __restrict__ Pi,
@@ -1184,11 +1188,22 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const
     if (scalars && solver_done(scalars + SC_COUNT * tm.seg[B])) return;
     __shared__ double red[3][kRedWaves][kTile];
     double sx = 0.0, sy = 0.0, sz = 0.0;
-    for (int w = row_start[B] + v; w < row_start[B + 1]; w += kRedWaves) {
-        const double4 p = Pi[(size_t)w * kTile + l];
-        sx += p.x;
-        sy += p.y;
-        sz += p.z;
+    if (F32) {   // FP32 item partials behind the slab (k_sym_matvec_mixed)
+        const size_t NW = (size_t)row_start[gridDim.x] * kTile;   // all items (the grid is one block per tile)
+        const float* P = Pjf + 3 * S;
+        for (int w = row_start[B] + v; w < row_start[B + 1]; w += kRedWaves) {
+            const size_t k = (size_t)w * kTile + l;
+            sx += (double)P[k];
+            sy += (double)P[NW + k];
+            sz += (double)P[2 * NW + k];
+        }
+    } else {
+        for (int w = row_start[B] + v; w < row_start[B + 1]; w += kRedWaves) {
+            const double4 p = Pi[(size_t)w * kTile + l];
+            sx += p.x;
+            sy += p.y;
+            sz += p.z;
+        }
     }
     const size_t col = tm.col[B];
     const int first = tm.first[B];
@@ -1810,16 +1825,16 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
 }
 
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
-                             const float4* mu32, const double* inva, const double* scalars, double4* Pi, float* Pjf) {
+                             const float4* mu32, const double* inva, const double* scalars, float* Pjf) {
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
                            mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
-                           Pi, Pjf, L.slab);
+                           Pjf, L.slab);
     else
         hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
                            mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
-                           Pi, Pjf, L.slab);
+                           Pjf, L.slab);
 }
 
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,

@@ -36,6 +36,9 @@ __device__ __forceinline__ unsigned short to_lp(float x) {
         unsigned r = u + 0x7fffu + ((u >> 16) & 1u);
         return (unsigned short)(r >> 16);
     }
+    // keep x an FP32 value: otherwise the FP64 -> FP32 -> FP16 conversions of the callers fold into
+    // one FP64 -> FP16 conversion, which gfx950 has no instruction for (a long integer sequence)
+    asm volatile("" : "+v"(x));
     _Float16 h = (_Float16)x;
     return __builtin_bit_cast(unsigned short, h);
 }

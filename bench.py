@@ -35,10 +35,10 @@ PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
 # HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1): 93.0 MB read + 347.4 MB
-# written (the tile partial slab, 255 MB, and the work-item partials).  Counters cannot be read
-# inside this process.
-MATVEC_HBM_BYTES = 440.4e6
+# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 66.5 MB
+# read + 166.8 MB written (the FP32 tile partial slab, 128 MB, and the FP32 work-item partials).
+# Counters cannot be read inside this process.
+MATVEC_HBM_BYTES = 233.3e6
 
 
 def parse():
@@ -53,6 +53,8 @@ def parse():
                    help="run only the cluster-solve measurement and print its object (for profiling)")
     p.add_argument("--no-dense", action="store_true", help="skip the dense cluster-response measurement")
     p.add_argument("--no-saline", action="store_true", help="skip the 96,000-site saline box measurement")
+    p.add_argument("--no-autotune", action="store_true",
+                   help="keep each context's default schedule (no tdp_autotune before the warm-up)")
     p.add_argument("--dense-only", action="store_true",
                    help="run only the dense cluster-response measurement and print its object (for profiling)")
     p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
@@ -74,7 +76,8 @@ def make_handle(s, precision):
 
 def time_path(torch, s, precision, steps, warmup, timing=False):
     h = make_handle(s, precision)
-    tiles_per_item = h.autotune(s.positions, s.box)   # schedule chosen on this configuration, untimed
+    # schedule chosen on this configuration, untimed
+    tiles_per_item = 0 if NO_AUTOTUNE else h.autotune(s.positions, s.box)
     dev = torch.device("cuda:0")
     pos = torch.tensor(s.positions.reshape(-1), dtype=torch.float64, device=dev)
     frc = torch.zeros(pos.numel(), dtype=torch.float64, device=dev)
@@ -163,7 +166,7 @@ def cluster_bench(torch, steps, warmup):
     res = {}
     for prec in ("mixed", "double"):
         b = _lib.HipBatchHandle([c.force for c in cl], precision=prec, solver="pcg")
-        tiles_per_item = b.autotune(pos_h)
+        tiles_per_item = 0 if NO_AUTOTUNE else b.autotune(pos_h)
         for _ in range(warmup):
             b.execute_device(pos.data_ptr(), frc.data_ptr(), en.data_ptr(), stream)
         torch.cuda.synchronize()
@@ -292,8 +295,13 @@ def saline_bench(torch, steps):
     }
 
 
+NO_AUTOTUNE = False
+
+
 def main():
+    global NO_AUTOTUNE
     args = parse()
+    NO_AUTOTUNE = args.no_autotune
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (requested --gpus %d)" % args.gpus,
               file=sys.stderr)

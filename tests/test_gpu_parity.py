@@ -227,6 +227,49 @@ def test_device_entry_point_matches_host():
     assert r.returncode == 0 and "OK" in r.stdout
 
 
+def _random_system(n, periodic, seed):
+    """n polarizable charges (no exclusions) spread over a 1.6 nm box, at least 0.12 nm apart."""
+    from tholedipoleplugin_amd.context import System
+    rng = np.random.default_rng(seed)
+    L = 1.6
+    pos = []
+    while len(pos) < n:
+        x = rng.uniform(0.0, L, 3)
+        if all(np.linalg.norm((x - y) - L * np.round((x - y) / L)) > 0.12 for y in pos):
+            pos.append(x)
+    system = System()
+    f = TholeDipoleForce()
+    f.setNonbondedMethod(T.PME if periodic else T.NoCutoff)
+    f.setCutoffDistance(0.7)
+    f.setPolarizationType(T.Mutual)
+    for i in range(n):
+        system.addParticle(1.0)
+        f.addParticle(float(rng.uniform(-0.8, 0.8)), [0.0, 0.0, 0.0], float(rng.uniform(0.3e-3, 1.5e-3)),
+                      float(rng.uniform(0.3, 0.45)), T.NoAxisType, -1, -1, -1)
+    box = np.diag([L, L, L]) if periodic else None
+    if periodic:
+        system.setDefaultPeriodicBoxVectors(*box)
+    system.addForce(f)
+    return ref_systems.RefSystem("random%d" % n, system, f, np.array(pos).reshape(-1, 3), box)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+@pytest.mark.parametrize("n", [1, 2, 63, 64, 65, 129])
+def test_ragged_sizes(n, periodic):
+    """Systems of 1 and 2 particles and around the 64-particle tile boundary (a partial last tile,
+    exactly full tiles, one particle past), against the oracle at the bar, both precisions."""
+    _say("case ragged n=%d periodic=%d" % (n, periodic))
+    rs = tighten(_random_system(n, periodic, 40 + n), eps=1e-12)
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box, target_eps=1e-13, max_iter=5000)
+    for precision in ("double", "mixed"):
+        ctx, st, k = gpu_eval(rs, precision=precision, solver="pcg")
+        e = st.getPotentialEnergy()
+        assert abs(e - ref["energy"]) <= E_TOL * abs(ref["energy"]) + 1e-12
+        if n > 1:
+            assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+        assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
+
+
 @pytest.mark.parametrize("solver", ["pcg", "jacobi"])
 def test_warm_start_mutual(solver):
     """A warm-started Mutual solve (previous evaluation's dipoles as the initial guess) reaches the

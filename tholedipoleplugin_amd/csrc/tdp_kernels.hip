@@ -1053,7 +1053,7 @@ __device__ __forceinline__ bool elec_far32(const double4* __restrict__ s_p, cons
 // FP64 positions; sums per tile flushed to FP64); every other tile stays FP64.  PERM = false: the
 // system has no permanent dipoles, so no torques are formed or written.
 template <bool PBC, bool MUTUAL, bool MIXED, bool PERM>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void k_sym_elec(TileMap tm, int nblk, const int4* __restrict__ work,
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3))) void k_sym_elec(TileMap tm, int nblk, const int4* __restrict__ work,
                                                  const double4* __restrict__ posa, const double4* __restrict__ qd,
                                                  const double4* __restrict__ mu, const double* __restrict__ inva,
                                                  const int* __restrict__ orig, const int* __restrict__ exc_start,

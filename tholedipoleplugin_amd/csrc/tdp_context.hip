@@ -122,6 +122,9 @@ struct tdp_context {
     int2* d_segw = nullptr;
     // ---- results of the last solve (per system)
     std::vector<int> iterations, converged;
+    // pinned host copy of the stop count and the solver scalars, read in one round trip
+    void* h_stage = nullptr;
+    size_t h_stage_bytes = 0;
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
     bool warm_start = false, have_prev = false;
@@ -556,7 +559,8 @@ void harvest_timing(tdp_context* c) {
 
 // T * v for the solver: v64 is mu or p; in mixed precision vec32 holds its FP32 copy.  With
 // timing on, events bracket the pair kernel (not the reduction that follows it).
-void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const double* scalars) {
+void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const double* scalars,
+            const PcgQ& pq = PcgQ()) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         e0 = take_event(c);
@@ -574,7 +578,7 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         hipEventRecord(e1, c->stream);
         c->pending.push_back({e0, e1});
     }
-    launch_col_reduce(c->stream, L, pi, pj, scalars, c->Emv, nullptr, mixed ? c->d_slab32 : nullptr);
+    launch_col_reduce(c->stream, L, pi, pj, scalars, c->Emv, nullptr, mixed ? c->d_slab32 : nullptr, pq);
 }
 
 int check_box(tdp_context* c, const double* box) {
@@ -586,11 +590,34 @@ int check_box(tdp_context* c, const double* box) {
     return TDP_OK;
 }
 
-// Per-system solver results from the device scalars.
-int read_solver_results(tdp_context* c, bool jacobi) {
-    std::vector<double> sc((size_t)SC_COUNT * c->nseg);
-    TDP_HIP(c, hipMemcpyAsync(sc.data(), c->d_scalars, sc.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+// Copies the stop count and the solver scalars to the pinned staging buffer (one host round trip).
+int stage_solver_state(tdp_context* c, int& done) {
+    const size_t bytes = sizeof(double) * ((size_t)SC_COUNT * c->nseg + 1);
+    if (bytes > c->h_stage_bytes) {
+        if (c->h_stage) hipHostFree(c->h_stage);
+        c->h_stage = nullptr;
+        c->h_stage_bytes = 0;
+        TDP_HIP(c, hipHostMalloc(&c->h_stage, bytes, hipHostMallocDefault));
+        c->h_stage_bytes = bytes;
+    }
+    double* h = static_cast<double*>(c->h_stage);
+    TDP_HIP(c, hipMemcpyAsync(h, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(h + 1, c->d_scalars, sizeof(double) * SC_COUNT * c->nseg, hipMemcpyDeviceToHost,
+                              c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
+    done = *reinterpret_cast<const int*>(h);
+    return TDP_OK;
+}
+
+// Per-system solver results from the device scalars; staged: stage_solver_state has just copied
+// them (no launch since).
+int read_solver_results(tdp_context* c, bool jacobi, bool staged = false) {
+    if (!staged) {
+        int unused;
+        int rc = stage_solver_state(c, unused);
+        if (rc) return rc;
+    }
+    const double* sc = static_cast<const double*>(c->h_stage) + 1;
     for (int g = 0; g < c->nseg; g++) {
         const double* x = &sc[(size_t)SC_COUNT * g];
         c->iterations[g] = (int)x[SC_ITER];
@@ -623,11 +650,6 @@ int start_solve(tdp_context* c, bool jacobi) {
     return TDP_OK;
 }
 
-int read_done(tdp_context* c, int& done) {
-    TDP_HIP(c, hipMemcpyAsync(&done, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-    TDP_HIP(c, hipStreamSynchronize(c->stream));
-    return TDP_OK;
-}
 
 // setup() and the permanent field (ReferenceTholeDipoleForce.cpp:1117-1155, :735-746) from
 // positions already in d_pos3 (caller's order): internal order, lab-frame dipoles, E0 and
@@ -678,9 +700,14 @@ int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool
                 launch_jacobi_update(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr, c->d_scalars,
                                      c->d_red);
             } else {
-                matvec(c, L, c->p, c->d_scalars);
-                launch_pcg_iteration_tail(s, G, c->Emv, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr,
-                                          c->q, c->d_red, c->d_scalars);
+                PcgQ pq;
+                pq.p = c->p;
+                pq.alpha = c->d_alpha;
+                pq.q = c->q;
+                pq.red = c->d_red + 2 * (size_t)c->nblk;
+                matvec(c, L, c->p, c->d_scalars, pq);
+                launch_pcg_iteration_tail(s, G, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr, c->q,
+                                          c->d_red, c->d_scalars);
             }
         }
     };
@@ -736,12 +763,13 @@ int solve_mutual(tdp_context* c, const double* box) {
         launch_pcg_init(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p, mixed ? c->vec32 : nullptr, c->d_red,
                         c->d_scalars);
     }
-    if ((rc = read_done(c, done))) return rc;
-    for (int pass = 0; done < c->nseg; pass++) {
+    // no host check before the first chunk: systems already stopped make its launches return at once
+    for (int pass = 0;; pass++) {
         if ((rc = run_iterations(c, L, G, jacobi, pass == 0 ? first_chunk : chunk))) return rc;
-        if ((rc = read_done(c, done))) return rc;
+        if ((rc = stage_solver_state(c, done))) return rc;
+        if (done >= c->nseg) break;
     }
-    if ((rc = read_solver_results(c, jacobi))) return rc;
+    if ((rc = read_solver_results(c, jacobi, true))) return rc;
     c->last_max_iter = *std::max_element(c->iterations.begin(), c->iterations.end());
     if (c->timing) harvest_timing(c);
     return TDP_OK;
@@ -1046,7 +1074,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_pos3, 3 * (size_t)std::max(c->n, 1))) || (rc = dalloc(c, &c->d_frc, 3 * (size_t)npad)) ||
         (rc = dalloc(c, &c->d_alpha, npad)) || (rc = dalloc(c, &c->d_inva, npad)) ||
         (rc = dalloc(c, &c->d_thole, npad)) || (rc = dalloc(c, &c->d_charge, npad)) ||
-        (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 2 * (size_t)nb + 2)) ||
+        (rc = dalloc(c, &c->d_dmol, 3 * (size_t)npad)) || (rc = dalloc(c, &c->d_red, 3 * (size_t)nb + 2)) ||
         (rc = dalloc(c, &c->d_scalars, (size_t)SC_COUNT * ns)) || (rc = dalloc(c, &c->d_coef, 64)) ||
         (rc = dalloc(c, &c->d_energy, ns)) || (rc = dalloc(c, &c->d_axis, npad)) ||
         (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
@@ -1358,6 +1386,7 @@ void tdp_destroy(tdp_context* c) {
     for (auto e : c->ev_pool) hipEventDestroy(e);
     for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
     for (void* p : c->allocations) hipFree(p);
+    if (c->h_stage) hipHostFree(c->h_stage);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -12,9 +12,17 @@ enum SolverScalars {
     SC_TARGET = 3,  // target RMS dipole update (e*nm)
     SC_EPS = 4,     // current RMS dipole update
     SC_RZ = 5,      // PCG r.z
-    SC_GAMMA = 6,   // PCG step length
-    SC_BETA = 7,    // PCG direction update
-    SC_COUNT = 8
+    SC_RZOLD = 6,   // PCG r.z of the previous iteration (kept by the update kernel for the direction kernel)
+    SC_COUNT = 7
+};
+
+// Fused PCG step in the reduction of T*p: q = p/alpha - T p (alpha = 0: q = -T p) to q and the tile
+// sums of p.q to red[T]; off when q is null.
+struct PcgQ {
+    const double4* p = nullptr;
+    const double* alpha = nullptr;
+    double4* q = nullptr;
+    double* red = nullptr;
 };
 
 // Tiles of a context that may hold several independent systems (segments), each in a
@@ -74,7 +82,7 @@ struct PairLaunch {
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
                          const double* scalars, double4* Pi, double* Pj);
 void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj, const double* scalars,
-                       double4* out4, double* out3, const float* Pjf = nullptr);
+                       double4* out4, double* out3, const float* Pjf = nullptr, const PcgQ& pq = PcgQ());
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
                              const float4* mu32, const double* inva, const double* scalars, float* Pjf);
 
@@ -98,14 +106,15 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed
                      double* PjT, double* part_e, double* frc, double4* torque);
 void launch_field_init(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha, double4* E0,
                        double4* mu, float4* mu32);
-// red: 2 * ntile doubles; scalars: SC_COUNT per system
+// red: 3 * ntile doubles; scalars: SC_COUNT per system
 void launch_jacobi_update(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
                           const double* alpha, double4* mu, float4* mu32, double* scalars, double* red);
 void launch_pcg_init(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0, const double* alpha,
                      const double4* x, double4* r, double4* p, float4* p32, double* red, double* scalars);
-void launch_pcg_iteration_tail(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha,
-                               double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
-                               double* scalars);
+// The rest of a PCG iteration after the reduction with PcgQ (q and its p.q tile sums in
+// red + 2 ntile): update of x, r, z and the direction p, two launches.
+void launch_pcg_iteration_tail(hipStream_t s, const SegLaunch& G, const double* alpha, double4* x, double4* r,
+                               double4* z, double4* p, float4* p32, const double4* q, double* red, double* scalars);
 void launch_extrap_step(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
                         const double* alpha, double4* mu, float4* mu32, double4* hist_o);
 void launch_extrap_combine(hipStream_t s, const SegLaunch& G, int order, const double4* hist, const double* coef,

@@ -23,6 +23,8 @@ struct BoxT64 { double bx, by, bz, ibx, iby, ibz; };
 // Scalars shared between the solver kernels (device memory, see SolverScalars in tdp_kernels.h).
 __device__ __forceinline__ bool solver_done(const double* sc) { return sc[SC_DONE] != 0.0; }
 
+__device__ __forceinline__ double inv_or_zero(double a) { return a > 0.0 ? 1.0 / a : 0.0; }
+
 // ------------------------------------------------------------------ frame rotation (a3)
 // checkChiralCenterAtParticle + applyRotationMatrixToParticle
 // (ReferenceTholeDipoleForce.cpp:432-452, :477-546, called for Z>=0 && Z!=i at :556).
@@ -484,10 +486,23 @@ __device__ __forceinline__ float undamped_r2_fp32(float4 cI, float4 hI, float4 c
     return rd * rd;
 }
 
+// Squared separation beyond which the Thole factors of a pair may be evaluated in FP32 (s <= -7,
+// r beyond 2.6 damping lengths for gamma = 0.39): the closest pairs, whose large tensor terms
+// would carry the FP32 rounding of their separations, stay in FP64.  Same bound as
+// undamped_r2_fp32 with exponent 7; infinite when a zero a could meet a nonzero one.
+__device__ __forceinline__ float damped32_r2(float4 cI, float4 hI, float4 cJ, float4 hJ) {
+    const float dmax = cI.w * cJ.w, gmin = fminf(hI.w, hJ.w);
+    if (dmax == 0.f) return 0.f;
+    if (!(gmin > 0.f)) return __builtin_inff();
+    const float rd = dmax * cbrtf(7.f / gmin) * 1.02f;
+    return rd * rd;
+}
+
 // Near tile pair of T * mu in mixed precision (full, unmasked, off-diagonal tiles on frame SM).
-// Pass 1 (near_matvec_mixed32) runs the 64 steps in FP32 like a far tile pair, except steps in
-// which some pair lies within r2min (or a straddling pair cannot be decided in FP32): those are
-// returned in a step mask and only rotate the FP32 j sums.  Pass 2 (near_matvec_mixed64) evaluates
+// Pass 1 (near_matvec_mixed32) runs the 64 steps in FP32 like a far tile pair, with the Thole
+// factors in FP32 on steps where some pair lies within r2min (every a is then nonzero: r2min is
+// infinite otherwise), except steps in which some pair lies within r2d (or a straddling pair
+// cannot be decided in FP32): those are returned in a step mask and only rotate the FP32 j sums.  Pass 2 (near_matvec_mixed64) evaluates
 // the marked steps in FP64 as k_sym_matvec does; in a step every lane serves a different j slot,
 // so its j terms are added to the slot's LDS accumulator s_jn (3 x 64 doubles) without conflicts
 // or rotation.
@@ -496,7 +511,9 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
                                                                   const float4* __restrict__ s_uf, int lane,
                                                                   float si_x, float si_y, float si_z,
                                                                   const FarFrame& f, float4 uf, float3 uf3,
-                                                                  float r2min, double& ax, double& ay, double& az,
+                                                                  float r2min, float r2d,
+                                                                  const float2* __restrict__ s_na, float2 nai,
+                                                                  double& ax, double& ay, double& az,
                                                                   double& jx, double& jy, double& jz) {
     float tx = 0.f, ty = 0.f, tz = 0.f, sx3 = 0.f, sy3 = 0.f, sz3 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
     unsigned long long steps64 = 0ull;
@@ -508,7 +525,7 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
         const float dy = far_sep<(SM & 2) != 0>(pl.y, si_y, f.hy, f.by, amb);
         const float dz = far_sep<(SM & 4) != 0>(pl.z, si_z, f.hz, f.bz, amb);
         const float r2 = dx * dx + dy * dy + dz * dz;
-        if (__builtin_amdgcn_ballot_w64(amb || !(r2 > r2min))) {
+        if (__builtin_amdgcn_ballot_w64(amb || !(r2 > r2d))) {
             steps64 |= 1ull << k;
             fx = rot1(fx);
             fy = rot1(fy);
@@ -520,6 +537,17 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
         float rinv2 = rinv * rinv;
         float rr3 = rinv * rinv2;
         float rr5 = rr3 * rinv2;   // 1/r^5: the factor 3 sits in uf3 and in the flush of tx..tz
+        if (__builtin_amdgcn_ballot_w64(!(r2 > r2min))) {
+            // getAndScaleInverseRs in FP32: s = -min(a_i, a_j) (r / (a_i a_j))^3, rr3 *= 1 - e^s,
+            // rr5 *= 1 - (1 - s) e^s; for s <= -30 both factors round to 1
+            const float2 naj = s_na[jj];
+            const float rat = r2 * rinv * (nai.y * naj.y);
+            const float sv = -fminf(nai.x, naj.x) * (rat * rat * rat);
+            const float ev = __expf(sv);
+            const float l3 = 1.f - ev;
+            rr3 *= l3;
+            rr5 *= fmaf(sv, ev, l3);
+        }
         float udj = rr5 * (uj.x * dx + uj.y * dy + uj.z * dz);
         float udi = rr5 * (uf3.x * dx + uf3.y * dy + uf3.z * dz);
         tx += udj * dx;
@@ -605,7 +633,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // the general far path, so a fallback from a frame to FP64 positions restages only s_pf.
     // Near tiles in mixed steps add FP32 offsets and dipoles (twice over) behind the FP64 arrays.
     constexpr int kBytesNear = kTile * (2 * sizeof(double4) + sizeof(double));
-    constexpr int kBytesNearMixed = kBytesNear + 2 * kTile * 2 * sizeof(float4) + 3 * kTile * sizeof(double);
+    constexpr int kBytesNearMixed =
+        kBytesNear + 2 * kTile * 2 * sizeof(float4) + 3 * kTile * sizeof(double) + 2 * kTile * sizeof(float2);
     constexpr int kBytesFar = 2 * kTile * (sizeof(double4) + sizeof(float4));
     __shared__ __align__(16) unsigned char s_raw[kBytesNearMixed > kBytesFar ? kBytesNearMixed : kBytesFar];
     double4* s_p = reinterpret_cast<double4*>(s_raw);
@@ -614,6 +643,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     float4* s_nl = reinterpret_cast<float4*>(s_raw + kBytesNear);
     float4* s_nu = s_nl + 2 * kTile;
     double* s_jn = reinterpret_cast<double*>(s_nu + 2 * kTile);
+    float2* s_na = reinterpret_cast<float2*>(s_jn + 3 * kTile);   // (a, 1/a) of the j tile, twice over
     double4* s_pf = reinterpret_cast<double4*>(s_raw);
     float4* s_uf = reinterpret_cast<float4*>(s_pf + 2 * kTile);
     float4* s_l = reinterpret_cast<float4*>(s_raw);
@@ -632,6 +662,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const float4 cI = tb_c[I], hI = tb_h[I];
     const double4 oI = tb_o[I];
     const float4 li = posl[i];
+    const float2 nai = make_float2((float)pi.w, (float)iai);
     const bool ifull = cntI == kTile;
     double ax = 0.0, ay = 0.0, az = 0.0;
     int e = exc_start[I];
@@ -733,26 +764,32 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
         } else if (!masked && !diag && ifull && cntJ == kTile &&
                    (fr_n = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box)).smask >= 0) {
-            s_p[lane] = posa[jb + lane];
+            const double4 pj = posa[jb + lane];
+            const double iaj = inva[jb + lane];
+            s_p[lane] = pj;
             s_u[lane] = mu[jb + lane];
-            s_ia[lane] = inva[jb + lane];
+            s_ia[lane] = iaj;
             {
                 const float4 l = posl[jb + lane];
                 const float4 u = mu32[jb + lane];
+                const float2 na = make_float2((float)pj.w, (float)iaj);
                 s_nl[lane] = l;
                 s_nl[lane + kTile] = l;
                 s_nu[lane] = u;
                 s_nu[lane + kTile] = u;
+                s_na[lane] = na;
+                s_na[lane + kTile] = na;
             }
             __syncthreads();
             const float r2min = undamped_r2_fp32(cI, hI, tb_c[J], tb_h[J]);
+            const float r2d = damped32_r2(cI, hI, tb_c[J], tb_h[J]);
             const float si_x = (float)((double)li.x - fr_n.shx), si_y = (float)((double)li.y - fr_n.shy),
                         si_z = (float)((double)li.z - fr_n.shz);
             unsigned long long steps64 = 0ull;
 #define TDP_NEAR_CASE(M)                                                                                      \
     case M:                                                                                                   \
-        steps64 = near_matvec_mixed32<M>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min, ax, ay, az, jx, \
-                                         jy, jz);                                                             \
+        steps64 = near_matvec_mixed32<M>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min, r2d, s_na,   \
+                                         nai, ax, ay, az, jx, jy, jz);                                        \
         break;
             switch (fr_n.smask) {
                 TDP_NEAR_CASE(0)
@@ -1183,7 +1220,8 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const
                                                                const double* __restrict__ Pj,
                                                                const float* __restrict__ Pjf, size_t S,
                                                                const double* __restrict__ scalars,
-                                                               double4* __restrict__ out4, double* __restrict__ out3) {
+                                                               double4* __restrict__ out4, double* __restrict__ out3,
+                                                               PcgQ pq) {
     const int B = blockIdx.x, l = threadIdx.x & (kTile - 1), v = threadIdx.x / kTile;
     if (scalars && solver_done(scalars + SC_COUNT * tm.seg[B])) return;
     __shared__ double red[3][kRedWaves][kTile];
@@ -1231,7 +1269,14 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const
             sy += red[1][u][l];
             sz += red[2][u][l];
         }
-        if (out3) {
+        if (pq.q) {   // PCG: q = p/alpha - T p and the tile sum of p.q (wave 0 holds the whole tile)
+            const double4 p = pq.p[x];
+            const double ia = inv_or_zero(pq.alpha[x]);
+            const double4 q = make_double4(p.x * ia - sx, p.y * ia - sy, p.z * ia - sz, 0.0);
+            pq.q[x] = q;
+            const double d = wave_sum(p.x * q.x + p.y * q.y + p.z * q.z);
+            if (l == 0) pq.red[B] = d;
+        } else if (out3) {
             out3[3 * x] = sx;
             out3[3 * x + 1] = sy;
             out3[3 * x + 2] = sz;
@@ -1247,8 +1292,6 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const
 // alpha = 0 and stay zero.  Norms are summed per tile (wave_sum, fixed order) into red[T] and
 // red[ntile + T], then per system by one 64-thread block over its tiles (fixed assignment and
 // order), so every result is deterministic.  A system whose solve stops adds 1 to done[0].
-
-__device__ __forceinline__ double inv_or_zero(double a) { return a > 0.0 ? 1.0 / a : 0.0; }
 
 // E0 = reduced field; mu = alpha*E0 (calculateInducedDipoles, ReferenceTholeDipoleForce.cpp:743-746).
 __global__ void k_field_init(int npad, const double4* __restrict__ part, const double* __restrict__ alpha,
@@ -1273,7 +1316,14 @@ __device__ __forceinline__ void tile_partials(double a, double b, double* __rest
     }
 }
 
-// Sums of the tile partials of system s = blockIdx.x over its tiles [t0, t1).
+// Sums of the tile partials of system sg over its tiles [sg.x, sg.y), in a fixed order: every
+// workgroup that forms them gets the same values.
+__device__ __forceinline__ double segment_sum(const int4 sg, const double* __restrict__ red) {
+    double a = 0.0;
+    for (int T = sg.x + (int)threadIdx.x; T < sg.y; T += kTile) a += red[T];
+    return wave_sum(a);
+}
+
 __device__ __forceinline__ void segment_sums(const int4 sg, const double* __restrict__ red, int ntile, double& a,
                                              double& b) {
     a = 0.0;
@@ -1365,44 +1415,30 @@ __global__ __launch_bounds__(64) void k_pcg_init_finalize(const int4* __restrict
     if (sc[SC_MAXIT] <= 0.0 || !(eps > sc[SC_TARGET])) mark_done(sc, done);
 }
 
-// q = p/alpha - T p; partial p.q
-__global__ __launch_bounds__(64) void k_pcg_q(const int* __restrict__ tseg, const double4* __restrict__ part,
-                                              const double* __restrict__ alpha, const double4* __restrict__ p,
-                                              double4* __restrict__ q, const double* __restrict__ scalars,
-                                              double* __restrict__ red, int ntile) {
-    if (solver_done(scalars + SC_COUNT * tseg[blockIdx.x])) return;
-    const int i = blockIdx.x * kTile + threadIdx.x;
-    double4 f = part[i];
-    double4 pv = p[i];
-    double ia = inv_or_zero(alpha[i]);
-    double4 qv = make_double4(pv.x * ia - f.x, pv.y * ia - f.y, pv.z * ia - f.z, 0.0);
-    q[i] = qv;
-    tile_partials(pv.x * qv.x + pv.y * qv.y + pv.z * qv.z, 0.0, red, ntile);
-}
-
-__global__ __launch_bounds__(64) void k_pcg_gamma(const int4* __restrict__ seg, const double* __restrict__ red,
-                                                  int ntile, double* __restrict__ scalars) {
-    double* sc = scalars + SC_COUNT * blockIdx.x;
-    if (solver_done(sc)) return;
-    double pq, unused;
-    segment_sums(seg[blockIdx.x], red, ntile, pq, unused);
-    if (threadIdx.x == 0) sc[SC_GAMMA] = sc[SC_RZ] / pq;
-}
-
-// x += gamma p; r -= gamma q; z = alpha r; partial r.z and z.z
-__global__ __launch_bounds__(64) void k_pcg_update(const int* __restrict__ tseg, const double* __restrict__ alpha,
-                                                   const double4* __restrict__ p, const double4* __restrict__ q,
-                                                   double4* __restrict__ x, double4* __restrict__ r,
-                                                   double4* __restrict__ z, const double* __restrict__ scalars,
+// One PCG iteration after the reduction of T p, which left q = p/alpha - T p and the tile sums of
+// p.q in red_pq.  Every workgroup of a system sums the system's tile sums itself (segment_sum, the
+// same order in every workgroup, so all of them get the same value) instead of waiting for a
+// separate launch; the scalars that these sums replace are written by the system's first tile
+// only, into slots that no workgroup of the same launch reads.
+// x += gamma p; r -= gamma q; z = alpha r (gamma = r.z / p.q); partial r.z and z.z.
+__global__ __launch_bounds__(64) void k_pcg_update(const int* __restrict__ tseg, const int4* __restrict__ seg,
+                                                   const double* __restrict__ alpha, const double4* __restrict__ p,
+                                                   const double4* __restrict__ q, double4* __restrict__ x,
+                                                   double4* __restrict__ r, double4* __restrict__ z,
+                                                   double* __restrict__ scalars, const double* __restrict__ red_pq,
                                                    double* __restrict__ red, int ntile) {
-    const double* sc = scalars + SC_COUNT * tseg[blockIdx.x];
+    const int g = tseg[blockIdx.x];
+    double* sc = scalars + SC_COUNT * g;
     if (solver_done(sc)) return;
+    const int4 sg = seg[g];
+    const double rz = sc[SC_RZ];
+    const double gam = rz / segment_sum(sg, red_pq);
+    if ((int)blockIdx.x == sg.x && threadIdx.x == 0) sc[SC_RZOLD] = rz;   // for k_pcg_direction
     const int i = blockIdx.x * kTile + threadIdx.x;
-    double g = sc[SC_GAMMA];
     double4 pv = p[i], qv = q[i], xv = x[i], rv = r[i];
     double a = alpha[i];
-    xv.x += g * pv.x; xv.y += g * pv.y; xv.z += g * pv.z;
-    rv.x -= g * qv.x; rv.y -= g * qv.y; rv.z -= g * qv.z;
+    xv.x += gam * pv.x; xv.y += gam * pv.y; xv.z += gam * pv.z;
+    rv.x -= gam * qv.x; rv.y -= gam * qv.y; rv.z -= gam * qv.z;
     double4 zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
     x[i] = xv;
     r[i] = rv;
@@ -1410,37 +1446,35 @@ __global__ __launch_bounds__(64) void k_pcg_update(const int* __restrict__ tseg,
     tile_partials(rv.x * zv.x + rv.y * zv.y + rv.z * zv.z, zv.x * zv.x + zv.y * zv.y + zv.z * zv.z, red, ntile);
 }
 
-__global__ __launch_bounds__(64) void k_pcg_beta(const int4* __restrict__ seg, const double* __restrict__ red,
-                                                 int ntile, double* __restrict__ scalars, int* __restrict__ done) {
-    double* sc = scalars + SC_COUNT * blockIdx.x;
+// p = z + beta p (beta = new r.z / old r.z).  The system's first tile also records r.z, the RMS of
+// alpha r (the Jacobi update, eps) and the iteration count, and stops the solve when iterations ==
+// max or !(eps > target).  Workgroups of the same launch that then see the stop flag skip the p
+// update: p is not used after the last iteration.
+__global__ __launch_bounds__(64) void k_pcg_direction(const int* __restrict__ tseg, const int4* __restrict__ seg,
+                                                      const double4* __restrict__ z, double4* __restrict__ p,
+                                                      float4* __restrict__ p32, double* __restrict__ scalars,
+                                                      const double* __restrict__ red, int ntile,
+                                                      int* __restrict__ done) {
+    const int g = tseg[blockIdx.x];
+    double* sc = scalars + SC_COUNT * g;
     if (solver_done(sc)) return;
-    const int4 sg = seg[blockIdx.x];
+    const int4 sg = seg[g];
     double rz, zz;
     segment_sums(sg, red, ntile, rz, zz);
-    if (threadIdx.x != 0) return;
-    double eps = sqrt(zz / sg.z);
-    double it = sc[SC_ITER] + 1.0;
-    sc[SC_BETA] = rz / sc[SC_RZ];
-    sc[SC_RZ] = rz;
-    sc[SC_EPS] = eps;
-    sc[SC_ITER] = it;
-    // Once DONE is set the remaining kernels of this iteration (the p update) skip the system;
-    // p is not needed after its last iteration.
-    if (it >= sc[SC_MAXIT] || !(eps > sc[SC_TARGET])) mark_done(sc, done);
-}
-
-// p = z + beta p
-__global__ __launch_bounds__(64) void k_pcg_direction(const int* __restrict__ tseg, const double4* __restrict__ z,
-                                                      double4* __restrict__ p, float4* __restrict__ p32,
-                                                      const double* __restrict__ scalars) {
-    const double* sc = scalars + SC_COUNT * tseg[blockIdx.x];
-    if (solver_done(sc)) return;
+    const double b = rz / sc[SC_RZOLD];
     const int i = blockIdx.x * kTile + threadIdx.x;
-    double b = sc[SC_BETA];
     double4 zv = z[i], pv = p[i];
     double4 np = make_double4(zv.x + b * pv.x, zv.y + b * pv.y, zv.z + b * pv.z, 0.0);
     p[i] = np;
     if (p32) p32[i] = make_float4((float)np.x, (float)np.y, (float)np.z, 0.f);
+    if ((int)blockIdx.x == sg.x && threadIdx.x == 0) {
+        const double eps = sqrt(zz / sg.z);
+        const double it = sc[SC_ITER] + 1.0;
+        sc[SC_RZ] = rz;
+        sc[SC_EPS] = eps;
+        sc[SC_ITER] = it;
+        if (it >= sc[SC_MAXIT] || !(eps > sc[SC_TARGET])) mark_done(sc, done);
+    }
 }
 
 // Extrapolated polarization (convergeInducedDipolesByExtrapolation, ReferenceTholeDipoleForce.cpp:876-887):
@@ -1496,16 +1530,27 @@ __global__ __launch_bounds__(64) void k_sum_eps(const int4* __restrict__ seg, co
 
 // Energy of system s: the pair-kernel partials of its work items [w0, w1) in a fixed order
 // (strided per thread, then a tree over the 256 threads).
-__global__ __launch_bounds__(256) void k_reduce_energy(const int2* __restrict__ segw, const double* __restrict__ part_e,
-                                                       double* __restrict__ out) {
-    __shared__ double sh[256];
+constexpr int kEnergyThreads = 1024;
+__global__ __launch_bounds__(kEnergyThreads) void k_reduce_energy(const int2* __restrict__ segw,
+                                                                  const double* __restrict__ part_e,
+                                                                  double* __restrict__ out) {
+    __shared__ double sh[kEnergyThreads];
     const int2 w = segw[blockIdx.x];
     const int t = threadIdx.x;
-    double a = 0.0;
-    for (int k = w.x + t; k < w.y; k += 256) a += part_e[k];
-    sh[t] = a;
+    constexpr int T = kEnergyThreads;
+    // four independent loads in flight per thread, then a fixed-order tree
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+    int k = w.x + t;
+    for (; k + 3 * T < w.y; k += 4 * T) {
+        a0 += part_e[k];
+        a1 += part_e[k + T];
+        a2 += part_e[k + 2 * T];
+        a3 += part_e[k + 3 * T];
+    }
+    for (; k < w.y; k += T) a0 += part_e[k];
+    sh[t] = (a0 + a1) + (a2 + a3);
     __syncthreads();
-    for (int h = 128; h > 0; h >>= 1) {
+    for (int h = T / 2; h > 0; h >>= 1) {
         if (t < h) sh[t] += sh[t + h];
         __syncthreads();
     }
@@ -1804,13 +1849,13 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
 }
 
 void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, const double* Pj,
-                       const double* scalars, double4* out4, double* out3, const float* Pjf) {
+                       const double* scalars, double4* out4, double* out3, const float* Pjf, const PcgQ& pq) {
     if (Pjf)
         hipLaunchKernelGGL(k_col_reduce<true>, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.tm, L.row_start, Pi, Pj,
-                           Pjf, L.slab, scalars, out4, out3);
+                           Pjf, L.slab, scalars, out4, out3, pq);
     else
         hipLaunchKernelGGL(k_col_reduce<false>, dim3(L.nblk), dim3(kRedWaves * kTile), 0, s, L.tm, L.row_start, Pi, Pj,
-                           Pjf, L.slab, scalars, out4, out3);
+                           Pjf, L.slab, scalars, out4, out3, pq);
 }
 
 void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu, const double* inva,
@@ -1898,16 +1943,12 @@ void launch_pcg_init(hipStream_t s, const SegLaunch& G, const double4* part, con
                        scalars, G.done);
 }
 
-void launch_pcg_iteration_tail(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha,
-                               double4* x, double4* r, double4* z, double4* p, float4* p32, double4* q, double* red,
-                               double* scalars) {
-    const double* sc = scalars;
-    hipLaunchKernelGGL(k_pcg_q, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, part, alpha, p, q, sc, red, G.ntile);
-    hipLaunchKernelGGL(k_pcg_gamma, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile, scalars);
-    hipLaunchKernelGGL(k_pcg_update, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, alpha, p, q, x, r, z, sc, red, G.ntile);
-    hipLaunchKernelGGL(k_pcg_beta, dim3(G.nseg), dim3(kTile), 0, s, G.seg, (const double*)red, G.ntile, scalars,
-                       G.done);
-    hipLaunchKernelGGL(k_pcg_direction, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, z, p, p32, sc);
+void launch_pcg_iteration_tail(hipStream_t s, const SegLaunch& G, const double* alpha, double4* x, double4* r,
+                               double4* z, double4* p, float4* p32, const double4* q, double* red, double* scalars) {
+    hipLaunchKernelGGL(k_pcg_update, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, G.seg, alpha, (const double4*)p, q, x,
+                       r, z, scalars, (const double*)(red + 2 * (size_t)G.ntile), red, G.ntile);
+    hipLaunchKernelGGL(k_pcg_direction, dim3(G.ntile), dim3(kTile), 0, s, G.tseg, G.seg, (const double4*)z, p, p32,
+                       scalars, (const double*)red, G.ntile, G.done);
 }
 
 void launch_extrap_step(hipStream_t s, const SegLaunch& G, const double4* part, const double4* E0,
@@ -1929,7 +1970,7 @@ void launch_extrap_error(hipStream_t s, const SegLaunch& G, const double4* part,
 }
 
 void launch_reduce_energy(hipStream_t s, const SegLaunch& G, const double* part_e, double* out) {
-    hipLaunchKernelGGL(k_reduce_energy, dim3(G.nseg), dim3(256), 0, s, G.segw, part_e, out);
+    hipLaunchKernelGGL(k_reduce_energy, dim3(G.nseg), dim3(kEnergyThreads), 0, s, G.segw, part_e, out);
 }
 
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,

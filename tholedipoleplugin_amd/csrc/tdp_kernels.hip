@@ -132,13 +132,28 @@ __global__ void k_scatter_vec3(int npad, const int* __restrict__ perm, const dou
     dst3[3 * o + 2] = v.z;
 }
 
+// 32-bit key of a particle's raw (y, z): equal coordinates (-0 and +0 alike) give equal keys, so
+// different keys prove that the positions differ in y or z.  Mixing by the SplitMix64 finaliser.
+__device__ __forceinline__ unsigned yz_key(double y, double z) {
+    unsigned long long a = (unsigned long long)__double_as_longlong(y + 0.0);
+    const unsigned long long b = (unsigned long long)__double_as_longlong(z + 0.0);
+    a ^= b * 0x9E3779B97F4A7C15ull;
+    a ^= a >> 30;
+    a *= 0xBF58476D1CE4E5B9ull;
+    a ^= a >> 27;
+    a *= 0x94D049BB133111EBull;
+    a ^= a >> 31;
+    return (unsigned)a;
+}
+
 // ------------------------------------------------------------------ tile classification
 // Per tile of 64 particles (internal order), one wavefront: centre and half extent of its bounding
 // box (PBC: offsets from the tile's first particle wrapped to the minimum image, so a tile that
 // straddles the box edge stays compact) and the largest/smallest Thole parameter.  Also the tile
 // frame of the FP32 far-field paths: origin o = first particle + box centre (FP64) and each
 // particle's offset u from it in FP32, so that x = o + u + n L with n the integer wrap of the
-// particle's offset from the first particle (|u| <= half extent per axis).
+// particle's offset from the first particle (|u| <= half extent per axis); w carries the bits of
+// yz_key of the particle's raw position.
 __global__ __launch_bounds__(64) void k_tile_bounds(const int* __restrict__ tcnt, const double4* __restrict__ posa,
                                                     double bx, double by, double bz, int pbc,
                                                     float4* __restrict__ tb_c, float4* __restrict__ tb_h,
@@ -166,7 +181,8 @@ __global__ __launch_bounds__(64) void k_tile_bounds(const int* __restrict__ tcnt
         amin = fmin(amin, __shfl_xor(amin, m));
     }
     const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
-    posl[i] = valid ? make_float4((float)(d[0] - c[0]), (float)(d[1] - c[1]), (float)(d[2] - c[2]), 0.f)
+    posl[i] = valid ? make_float4((float)(d[0] - c[0]), (float)(d[1] - c[1]), (float)(d[2] - c[2]),
+                                  __uint_as_float(yz_key(p.y, p.z)))
                     : make_float4(0.f, 0.f, 0.f, 0.f);
     if (lane == 0) {
         tb_c[T] = make_float4((float)(p0.x + c[0]), (float)(p0.y + c[1]), (float)(p0.z + c[2]), (float)amax);
@@ -1085,6 +1101,97 @@ __device__ __forceinline__ bool elec_far32(const double4* __restrict__ s_p, cons
     return true;
 }
 
+// FP32 far tile pair of the energy pass without permanent dipoles (charges and induced dipoles, no
+// damping, unit scales) in the lab frame.  For such a pair calculateElectrostaticPairIxn
+// (ReferenceTholeDipoleForce.cpp:129-366) reduces to a closed form in the frame's X axis alone:
+// with d = x_B - x_A (A the lower caller index), Z = d/r and X = (e_x - Z_x Z)/|e_x - Z_x Z| (the
+// frame for raw positions that differ in y or z), a = mu_A, b = mu_B, w = q_A b - q_B a,
+//   energy = q_A q_B / r - (X.w) / (2 r^2),
+//   F      = w / r^3 + X (q_A q_B / r^2 - 3 (X.w) / r^3)
+//            [+ 6 ((X.b) a + (X.a) b) / r^4 + X (6 a.b - 30 (X.a)(X.b)) / r^4 for Mutual],
+// and the force on B is (F_y, F_z, F_x): the reference rotates the force back with the index
+// pattern at :150-159, which permutes the components (Y(Y.v) + Z(Z.v) = v - X(X.v) removes Y and
+// Z).  X.v = (v_x rho^2 - d_x (d_y v_y + d_z v_z)) / (r rho), rho^2 = d_y^2 + d_z^2, has no
+// cancellation when d is nearly along x.  The lane's own site is m, the other o, s = +1 when the
+// lane's particle is A; d, X and (X.m)(X.o) do not depend on the orientation, w and the a/b terms
+// change sign with it.  Returns false, with nothing accumulated, when a straddling pair cannot be
+// decided in FP32 (SM >= 0), when some pair's raw y and z may be equal (keys match; the reference
+// then builds the frame from e_y) or when a separation lies along x (rho = 0: no frame); the
+// caller then uses elec_far32.  SM: straddling-axis mask of the tile frame; -1: FP64 separations.
+template <bool PBC, bool MUTUAL, int SM>
+__device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, const float4* __restrict__ s_l,
+                                               const float4* __restrict__ s_uf, const int* __restrict__ s_o,
+                                               int lane, const ElecSite& me, int my_orig, unsigned key,
+                                               float si_x, float si_y, float si_z, const FarFrame& fr,
+                                               const BoxT64& box, ElecAcc& acc, ElecAcc& jacc, double& en) {
+    const float Qm = (float)me.q, mx = (float)me.ux, my = (float)me.uy, mz = (float)me.uz;
+    float ifx = 0.f, ify = 0.f, ifz = 0.f, e32 = 0.f, jfx = 0.f, jfy = 0.f, jfz = 0.f;
+    bool amb = false, bad = false;
+    for (int k = 0; k < kTile; k++) {
+        const int jj = (lane + k) & (kTile - 1);
+        float dx, dy, dz;   // x_other - x_me, minimum image
+        if (SM >= 0) {
+            const float4 lj = s_l[jj];
+            dx = far_sep<(SM & 1) != 0>(lj.x, si_x, fr.hx, fr.bx, amb);
+            dy = far_sep<(SM & 2) != 0>(lj.y, si_y, fr.hy, fr.by, amb);
+            dz = far_sep<(SM & 4) != 0>(lj.z, si_z, fr.hz, fr.bz, amb);
+            bad = bad || __float_as_uint(lj.w) == key;
+        } else {
+            const double4 pj = s_p[jj];
+            double ex = pj.x - me.x, ey = pj.y - me.y, ez = pj.z - me.z;
+            if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+            dx = (float)ex;
+            dy = (float)ey;
+            dz = (float)ez;
+            bad = bad || !(me.y != pj.y || me.z != pj.z);
+        }
+        const float4 oq = s_uf[jj];   // induced dipole, charge
+        const float s = my_orig < s_o[jj] ? 1.f : -1.f;
+        const float rho2 = dy * dy + dz * dz;
+        bad = bad || !(rho2 > 0.f);
+        const float r2 = dx * dx + rho2;
+        const float rinv = rsqrt_t(r2);
+        const float rI2 = rinv * rinv, rI3 = rI2 * rinv;
+        const float kx = rinv * rsqrt_t(rho2);
+        const float Xm = (mx * rho2 - dx * (dy * my + dz * mz)) * kx;
+        const float Xo = (oq.x * rho2 - dx * (dy * oq.y + dz * oq.z)) * kx;
+        const float ex = -dx * kx;
+        const float Xx = rho2 * kx, Xy = ex * dy, Xz = ex * dz;
+        const float QQ = Qm * oq.w;
+        const float Xw = Qm * Xo - oq.w * Xm;
+        float am, ao, inner;
+        if (MUTUAL) {
+            const float rI4 = rI2 * rI2;
+            const float A = s * (6.f * rI4);
+            am = A * Xo - rI3 * oq.w;
+            ao = A * Xm + rI3 * Qm;
+            inner = rI2 * QQ + rI4 * (6.f * (mx * oq.x + my * oq.y + mz * oq.z) - 30.f * Xm * Xo);
+        } else {
+            am = -rI3 * oq.w;
+            ao = rI3 * Qm;
+            inner = rI2 * QQ;
+        }
+        const float lam = s * inner - 3.f * rI3 * Xw;
+        const float Gx = mx * am + oq.x * ao + Xx * lam;
+        const float Gy = my * am + oq.y * ao + Xy * lam;
+        const float Gz = mz * am + oq.z * ao + Xz * lam;
+        e32 += rinv * QQ - 0.5f * s * rI2 * Xw;
+        // force on the lane's particle -(G_y, G_z, G_x), on the other +(G_y, G_z, G_x); j side lagging one step
+        ifx -= Gy;
+        ify -= Gz;
+        ifz -= Gx;
+        jfx = rot1(jfx) + Gy;
+        jfy = rot1(jfy) + Gz;
+        jfz = rot1(jfz) + Gx;
+    }
+    if (__builtin_amdgcn_ballot_w64(amb || bad)) return false;
+    acc.fx += ifx; acc.fy += ify; acc.fz += ifz;
+    en += e32;
+    jacc.fx = jfx; jacc.fy = jfy; jacc.fz = jfz;
+    jacc.tx = jacc.ty = jacc.tz = 0.0;
+    return true;
+}
+
 // MIXED: tile pairs beyond the Thole range evaluate the pair terms in FP32 (separations from the tile
 // frames, or FP64 separations where a frame cannot decide an image; the frame's axis choice compares
 // FP64 positions; sums per tile flushed to FP64); every other tile stays FP64.  PERM = false: the
@@ -1155,15 +1262,41 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 {
                     const double4 q = s_q[lane], u = s_u[lane];
                     s_qf[lane] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
-                    s_uf[lane] = make_float4((float)u.x, (float)u.y, (float)u.z, 0.f);
+                    s_uf[lane] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)q.w);
                     s_l[lane] = posl[jb + lane];
                 }
                 __syncthreads();
                 const float si_x = (float)((double)li.x - fr.shx), si_y = (float)((double)li.y - fr.shy),
                             si_z = (float)((double)li.z - fr.shz);
-                const bool done = fr.smask >= 0 &&
-                                  elec_far32<PBC, MUTUAL, PERM, true>(s_p, s_l, s_qf, s_uf, s_o, lane, me, my_orig,
-                                                                      si_x, si_y, si_z, fr, box, acc, jacc, en);
+                bool done = false;
+                if (!PERM) {   // closed form in the lab frame; elec_far32 where it cannot decide
+                    const unsigned key = __float_as_uint(li.w);
+#define TDP_LAB_CASE(M)                                                                                        \
+    case M:                                                                                                    \
+        done = elec_far32_lab<PBC, MUTUAL, M>(s_p, s_l, s_uf, s_o, lane, me, my_orig, key, si_x, si_y, si_z, fr, \
+                                              box, acc, jacc, en);                                             \
+        break;
+                    switch (fr.smask) {
+                        TDP_LAB_CASE(0)
+                        TDP_LAB_CASE(1)
+                        TDP_LAB_CASE(2)
+                        TDP_LAB_CASE(3)
+                        TDP_LAB_CASE(4)
+                        TDP_LAB_CASE(5)
+                        TDP_LAB_CASE(6)
+                        TDP_LAB_CASE(7)
+                        default:
+                            break;
+                    }
+#undef TDP_LAB_CASE
+                    if (!done)
+                        done = elec_far32_lab<PBC, MUTUAL, -1>(s_p, s_l, s_uf, s_o, lane, me, my_orig, key, si_x, si_y,
+                                                               si_z, fr, box, acc, jacc, en);
+                } else {
+                    done = fr.smask >= 0 &&
+                           elec_far32<PBC, MUTUAL, PERM, true>(s_p, s_l, s_qf, s_uf, s_o, lane, me, my_orig, si_x,
+                                                               si_y, si_z, fr, box, acc, jacc, en);
+                }
                 if (!done)
                     elec_far32<PBC, MUTUAL, PERM, false>(s_p, s_l, s_qf, s_uf, s_o, lane, me, my_orig, si_x, si_y,
                                                          si_z, fr, box, acc, jacc, en);

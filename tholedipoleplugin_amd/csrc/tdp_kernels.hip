@@ -1192,6 +1192,101 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
     return true;
 }
 
+// Near tile pair of the energy pass without permanent dipoles in mixed precision (full, unmasked,
+// off-diagonal tiles on frame SM), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
+// a u^3 >= 7) in FP32 by the closed form of elec_far32_lab with the Thole factors of
+// calculateElectrostaticPairIxn (:205-220): with e = exp(-a u^3), thole_c = 1 - e, thole_d0 =
+// 1 - e (1 + 1.5 a u^3), dthole_d0 = 1 - e (1 + a u^3 + 1.5 (a u^3)^2), dthole_d1 = 1 - e (1 + a u^3),
+//   energy = q_A q_B / r - thole_c (X.w) / (2 r^2),
+//   F      = thole_c w / r^3 + X (q_A q_B / r^2 - (2 thole_d0 + thole_c) (X.w) / r^3)
+//            [+ c6 ((X.b) a + (X.a) b) / r^4 + X (6 dthole_d1 a.b - (12 dthole_d0 + 6 dthole_d1 + 2 c6)
+//               (X.a)(X.b)) / r^4 for Mutual, c6 = 4 thole_d0 + 2 thole_c].
+// Steps with a closer pair, an undecidable straddling pair, possibly equal raw y and z or a
+// separation along x are returned in a step mask for the FP64 pass (their j sums only rotate).
+template <bool PBC, bool MUTUAL, int SM>
+__device__ __forceinline__ unsigned long long elec_near32_lab(const float4* __restrict__ s_l,
+                                                              const float4* __restrict__ s_uf,
+                                                              const float2* __restrict__ s_na,
+                                                              const int* __restrict__ s_o, int lane,
+                                                              const ElecSite& me, int my_orig, unsigned key,
+                                                              float si_x, float si_y, float si_z, const FarFrame& fr,
+                                                              float r2d, ElecAcc& acc, ElecAcc& jacc, double& en) {
+    const float Qm = (float)me.q, mx = (float)me.ux, my = (float)me.uy, mz = (float)me.uz;
+    const float am_ = (float)me.a, iam = (float)me.ia;
+    float ifx = 0.f, ify = 0.f, ifz = 0.f, e32 = 0.f, jfx = 0.f, jfy = 0.f, jfz = 0.f;
+    unsigned long long steps64 = 0ull;
+    for (int k = 0; k < kTile; k++) {
+        const int jj = (lane + k) & (kTile - 1);
+        const float4 lj = s_l[jj];
+        bool amb = false;
+        const float dx = far_sep<(SM & 1) != 0>(lj.x, si_x, fr.hx, fr.bx, amb);
+        const float dy = far_sep<(SM & 2) != 0>(lj.y, si_y, fr.hy, fr.by, amb);
+        const float dz = far_sep<(SM & 4) != 0>(lj.z, si_z, fr.hz, fr.bz, amb);
+        const float rho2 = dy * dy + dz * dz;
+        const float r2 = dx * dx + rho2;
+        if (__builtin_amdgcn_ballot_w64(amb || __float_as_uint(lj.w) == key || !(rho2 > 0.f) || !(r2 > r2d))) {
+            steps64 |= 1ull << k;
+            jfx = rot1(jfx);
+            jfy = rot1(jfy);
+            jfz = rot1(jfz);
+            continue;
+        }
+        const float4 oq = s_uf[jj];
+        const float2 na = s_na[jj];
+        const float s = my_orig < s_o[jj] ? 1.f : -1.f;
+        const float rinv = rsqrt_t(r2);
+        const float rI2 = rinv * rinv, rI3 = rI2 * rinv;
+        const float uu = r2 * rinv * (iam * na.y);
+        const float au3 = fminf(fminf(am_, na.x) * (uu * uu * uu), 100.f);
+        const float ev = __expf(-au3);
+        const float thc = 1.f - ev;
+        const float th0 = 1.f - ev * fmaf(1.5f, au3, 1.f);
+        const float kx = rinv * rsqrt_t(rho2);
+        const float Xm = (mx * rho2 - dx * (dy * my + dz * mz)) * kx;
+        const float Xo = (oq.x * rho2 - dx * (dy * oq.y + dz * oq.z)) * kx;
+        const float ex = -dx * kx;
+        const float Xx = rho2 * kx, Xy = ex * dy, Xz = ex * dz;
+        const float QQ = Qm * oq.w;
+        const float Xw = Qm * Xo - oq.w * Xm;
+        const float t3 = rI3 * thc;
+        float amc, aoc, inner;
+        if (MUTUAL) {
+            const float rI4 = rI2 * rI2;
+            const float dd1 = 1.f - ev * (1.f + au3);
+            const float dd0 = dd1 - ev * (1.5f * au3 * au3);
+            const float c6 = 4.f * th0 + 2.f * thc;
+            const float A = s * c6 * rI4;
+            amc = A * Xo - t3 * oq.w;
+            aoc = A * Xm + t3 * Qm;
+            inner = rI2 * QQ + rI4 * (6.f * dd1 * (mx * oq.x + my * oq.y + mz * oq.z) -
+                                      (12.f * dd0 + 6.f * dd1 + 2.f * c6) * Xm * Xo);
+        } else {
+            amc = -t3 * oq.w;
+            aoc = t3 * Qm;
+            inner = rI2 * QQ;
+        }
+        const float lam = s * inner - rI3 * (2.f * th0 + thc) * Xw;
+        const float Gx = mx * amc + oq.x * aoc + Xx * lam;
+        const float Gy = my * amc + oq.y * aoc + Xy * lam;
+        const float Gz = mz * amc + oq.z * aoc + Xz * lam;
+        e32 += rinv * QQ - 0.5f * s * rI2 * thc * Xw;
+        ifx -= Gy;
+        ify -= Gz;
+        ifz -= Gx;
+        jfx = rot1(jfx) + Gy;
+        jfy = rot1(jfy) + Gz;
+        jfz = rot1(jfz) + Gx;
+    }
+    acc.fx += ifx; acc.fy += ify; acc.fz += ifz;
+    en += e32;
+    // the j sums lag one step: after 64 steps lane l holds slot l + 63, one more rotation gives slot l
+    jacc.fx = (double)rot1(jfx);
+    jacc.fy = (double)rot1(jfy);
+    jacc.fz = (double)rot1(jfz);
+    jacc.tx = jacc.ty = jacc.tz = 0.0;
+    return steps64;
+}
+
 // MIXED: tile pairs beyond the Thole range evaluate the pair terms in FP32 (separations from the tile
 // frames, or FP64 separations where a frame cannot decide an image; the frame's axis choice compares
 // FP64 positions; sums per tile flushed to FP64); every other tile stays FP64.  PERM = false: the
@@ -1216,6 +1311,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
     __shared__ float4 s_l[MIXED ? kTile : 1];
     __shared__ float4 s_qf[MIXED ? kTile : 1];
     __shared__ float4 s_uf[MIXED ? kTile : 1];
+    __shared__ float2 s_na[MIXED && !PERM ? kTile : 1];      // (a, 1/a) of the j tile, near mixed steps
+    __shared__ double s_jn[MIXED && !PERM ? 3 * kTile : 1];  // per-slot j sums of the FP64 steps
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
     const int I = wk.x, J0 = wk.y, J1 = wk.z;
@@ -1255,6 +1352,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
         }
         ElecAcc jacc = {0, 0, 0, 0, 0, 0};
         int kend;
+        FarFrame fr_n;
         if (!masked && !diag && ifull && cntJ == kTile &&
             tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
             if (MIXED) {
@@ -1312,6 +1410,59 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 }
                 kend = kTile;
             }
+        } else if (MIXED && !PERM && !masked && !diag && ifull && cntJ == kTile &&
+                   (fr_n = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box)).smask >= 0) {
+            // near tile pair, mixed: FP32 steps by the closed form, the rest in FP64 (pass 2)
+            {
+                const double4 q = s_q[lane], u = s_u[lane];
+                s_uf[lane] = make_float4((float)u.x, (float)u.y, (float)u.z, (float)q.w);
+                s_l[lane] = posl[jb + lane];
+                s_na[lane] = make_float2((float)s_p[lane].w, (float)s_ia[lane]);
+                s_jn[lane] = 0.0;
+                s_jn[kTile + lane] = 0.0;
+                s_jn[2 * kTile + lane] = 0.0;
+            }
+            __syncthreads();
+            const float r2d = damped32_r2(cI, hI, tb_c[J], tb_h[J]);
+            const float si_x = (float)((double)li.x - fr_n.shx), si_y = (float)((double)li.y - fr_n.shy),
+                        si_z = (float)((double)li.z - fr_n.shz);
+            const unsigned key = __float_as_uint(li.w);
+            unsigned long long steps64 = 0ull;
+#define TDP_ENEAR_CASE(M)                                                                                       \
+    case M:                                                                                                     \
+        steps64 = elec_near32_lab<PBC, MUTUAL, M>(s_l, s_uf, s_na, s_o, lane, me, my_orig, key, si_x, si_y, si_z, \
+                                                  fr_n, r2d, acc, jacc, en);                                    \
+        break;
+            switch (fr_n.smask) {
+                TDP_ENEAR_CASE(0)
+                TDP_ENEAR_CASE(1)
+                TDP_ENEAR_CASE(2)
+                TDP_ENEAR_CASE(3)
+                TDP_ENEAR_CASE(4)
+                TDP_ENEAR_CASE(5)
+                TDP_ENEAR_CASE(6)
+                TDP_ENEAR_CASE(7)
+            }
+#undef TDP_ENEAR_CASE
+            // pass 2: the marked steps in FP64; in a step every lane serves a different j slot, whose
+            // sums collect in s_jn without conflicts
+            while (steps64) {
+                const int k = __builtin_ctzll(steps64);
+                steps64 &= steps64 - 1ull;
+                const int jd = (lane + k) & (kTile - 1);
+                ElecSite other;
+                load_site(other, s_p[jd], s_q[jd], s_u[jd], s_ia[jd]);
+                ElecAcc jb1 = {0, 0, 0, 0, 0, 0};
+                elec_sym_step<MUTUAL, true, PERM>(me, my_orig, other, s_o[jd], true, 1.0, 1.0, PBC, box, acc, jb1, en);
+                s_jn[jd] += jb1.fx;
+                s_jn[kTile + jd] += jb1.fy;
+                s_jn[2 * kTile + jd] += jb1.fz;
+            }
+            __syncthreads();
+            jacc.fx += s_jn[lane];
+            jacc.fy += s_jn[kTile + lane];
+            jacc.fz += s_jn[2 * kTile + lane];
+            kend = kTile;
         } else {
             const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
             for (int k = k0; k <= k1; k++) {

@@ -44,8 +44,8 @@ MATVEC_HBM_BYTES = 233.3e6
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=1)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-fp64", action="store_true")
     p.add_argument("--no-clusters", action="store_true", help="skip the batched cluster-solve measurement")
@@ -274,10 +274,10 @@ def saline_bench(torch, steps):
     1e-6 D: mixed and FP64 evaluations per second and the mixed result's error against FP64."""
     from tholedipoleplugin_amd import systems
     s = systems.config2()
-    mixed = time_path(torch, s, "mixed", steps, 1)
+    mixed = time_path(torch, s, "mixed", steps, 2)
     print("# saline96000 mixed: %.2f ms/step, %s" % (mixed["ms_per_step"], mixed["stats"]), file=sys.stderr,
           flush=True)
-    fp64 = time_path(torch, s, "double", max(1, steps - 1), 1)
+    fp64 = time_path(torch, s, "double", steps, 2)
     print("# saline96000 fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr,
           flush=True)
     a, b = mixed["forces"].reshape(-1), fp64["forces"].reshape(-1)
@@ -336,7 +336,7 @@ def main():
         st["matvec_launches"], passes, avg_ms, achieved_tflops), file=sys.stderr, flush=True)
     fp64 = None
     if not args.no_fp64:
-        fp64 = time_path(torch, s, "double", max(1, args.steps // 2), min(args.warmup, 1))
+        fp64 = time_path(torch, s, "double", args.steps, args.warmup)
         print("# fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr, flush=True)
     err = mu_err = e_err = None
     if fp64 is not None:
@@ -352,7 +352,7 @@ def main():
         dense = dense_bench(max(1, min(args.steps, 2)))
     saline = None
     if not args.no_saline:
-        saline = saline_bench(torch, max(2, min(args.steps, 3)))
+        saline = saline_bench(torch, max(2, min(args.steps, 8)))
     cpu = None
     if not args.no_cpu_baseline:
         from tholedipoleplugin_amd import _lib

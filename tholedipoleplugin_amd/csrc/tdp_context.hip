@@ -848,7 +848,7 @@ void forces_and_energy(tdp_context* c, const double* box) {
     launch_sym_elec(s, L, c->polarization == TDP_MUTUAL, c->precision == TDP_PRECISION_MIXED, c->posa, c->qd, c->mu,
                     c->d_inva, c->d_perm, pf,
                     reinterpret_cast<double*>(pf + off), pt, reinterpret_cast<double*>(pt + off), c->d_part_e, c->d_frc,
-                    c->torque);
+                    c->torque, c->d_slab32);
     launch_reduce_energy(s, G, c->d_part_e, c->d_energy);
     if (!c->gidx.empty() && c->has_perm) {   // without permanent dipoles every torque is zero
         launch_map_torque(s, npad, c->posa, c->d_axis, c->torque, c->contrib);

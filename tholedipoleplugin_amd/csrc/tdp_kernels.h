@@ -101,9 +101,11 @@ void launch_rotate(hipStream_t s, int n, const double4* posa, const double* dmol
 void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* qd, const double* inva,
                       double4* Pi, double* Pj, double4* out);
 // Energy, forces (accumulated into frc, 3 doubles per slot) and, when L.perm, torques (torque).
+// mixed without permanent dipoles: the force partials go to slab32 (FP32, the layout of
+// launch_sym_matvec_mixed's Pjf) instead of PiF/PjF.
 void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed, const double4* posa, const double4* qd,
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
-                     double* PjT, double* part_e, double* frc, double4* torque);
+                     double* PjT, double* part_e, double* frc, double4* torque, float* slab32);
 void launch_field_init(hipStream_t s, const SegLaunch& G, const double4* part, const double* alpha, double4* E0,
                        double4* mu, float4* mu32);
 // red: 3 * ntile doubles; scalars: SC_COUNT per system

@@ -502,15 +502,15 @@ __device__ __forceinline__ float undamped_r2_fp32(float4 cI, float4 hI, float4 c
     return rd * rd;
 }
 
-// Squared separation beyond which the Thole factors of a pair may be evaluated in FP32 (s <= -7,
-// r beyond 2.6 damping lengths for gamma = 0.39): the closest pairs, whose large tensor terms
+// Squared separation beyond which the Thole factors of a pair may be evaluated in FP32 (s <= -4,
+// r beyond 2.2 damping lengths for gamma = 0.39): the closest pairs, whose large tensor terms
 // would carry the FP32 rounding of their separations, stay in FP64.  Same bound as
-// undamped_r2_fp32 with exponent 7; infinite when a zero a could meet a nonzero one.
+// undamped_r2_fp32 with exponent 4; infinite when a zero a could meet a nonzero one.
 __device__ __forceinline__ float damped32_r2(float4 cI, float4 hI, float4 cJ, float4 hJ) {
     const float dmax = cI.w * cJ.w, gmin = fminf(hI.w, hJ.w);
     if (dmax == 0.f) return 0.f;
     if (!(gmin > 0.f)) return __builtin_inff();
-    const float rd = dmax * cbrtf(7.f / gmin) * 1.02f;
+    const float rd = dmax * cbrtf(4.f / gmin) * 1.02f;
     return rd * rd;
 }
 
@@ -1194,7 +1194,7 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
 
 // Near tile pair of the energy pass without permanent dipoles in mixed precision (full, unmasked,
 // off-diagonal tiles on frame SM), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
-// a u^3 >= 7) in FP32 by the closed form of elec_far32_lab with the Thole factors of
+// a u^3 >= 4) in FP32 by the closed form of elec_far32_lab with the Thole factors of
 // calculateElectrostaticPairIxn (:205-220): with e = exp(-a u^3), thole_c = 1 - e, thole_d0 =
 // 1 - e (1 + 1.5 a u^3), dthole_d0 = 1 - e (1 + a u^3 + 1.5 (a u^3)^2), dthole_d1 = 1 - e (1 + a u^3),
 //   energy = q_A q_B / r - thole_c (X.w) / (2 r^2),
@@ -1302,7 +1302,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                                                  const double4* __restrict__ tb_o, const float4* __restrict__ posl,
                                                  BoxT64 box, double4* __restrict__ PiF, double* __restrict__ PjF,
                                                  double4* __restrict__ PiT, double* __restrict__ PjT, size_t S,
-                                                 double* __restrict__ part_e) {
+                                                 double* __restrict__ part_e, float* __restrict__ Pf32) {
+    // MIXED without permanent dipoles: the force partials go to Pf32 in FP32, in the layout of the
+    // mixed T*mu pass (tile slab, then the item partials), one rounding per tile pair or work item
+    constexpr bool kF32Out = MIXED && !PERM;
     __shared__ double4 s_p[kTile];
     __shared__ double4 s_q[kTile];
     __shared__ double4 s_u[kTile];
@@ -1480,10 +1483,23 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
             kend = k1 + 1;
         }
         const size_t slot = slab_tile(tm, I, J) * kTile + ((lane + kend) & (kTile - 1));
-        store3(PjF, S, slot, jacc.fx, jacc.fy, jacc.fz);
+        if (kF32Out) {
+            Pf32[slot] = (float)jacc.fx;
+            Pf32[S + slot] = (float)jacc.fy;
+            Pf32[2 * S + slot] = (float)jacc.fz;
+        } else {
+            store3(PjF, S, slot, jacc.fx, jacc.fy, jacc.fz);
+        }
         if (PERM) store3(PjT, S, slot, jacc.tx, jacc.ty, jacc.tz);
     }
-    PiF[(size_t)w * kTile + lane] = make_double4(acc.fx, acc.fy, acc.fz, 0.0);
+    if (kF32Out) {
+        const size_t NW = (size_t)gridDim.x * kTile, iw = 3 * S + (size_t)w * kTile + lane;
+        Pf32[iw] = (float)acc.fx;
+        Pf32[iw + NW] = (float)acc.fy;
+        Pf32[iw + 2 * NW] = (float)acc.fz;
+    } else {
+        PiF[(size_t)w * kTile + lane] = make_double4(acc.fx, acc.fy, acc.fz, 0.0);
+    }
     if (PERM) PiT[(size_t)w * kTile + lane] = make_double4(acc.tx, acc.ty, acc.tz, 0.0);
     en = wave_sum(en);
     if (lane == 0) part_e[w] = en;
@@ -2183,12 +2199,12 @@ void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, c
 
 void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed, const double4* posa, const double4* qd,
                      const double4* mu, const double* inva, const int* orig, double4* PiF, double* PjF, double4* PiT,
-                     double* PjT, double* part_e, double* frc, double4* torque) {
+                     double* PjT, double* part_e, double* frc, double4* torque, float* slab32) {
     BoxT64 b = mkbox(L.box);
 #define TDP_SELEC(P, M, X, Q)                                                                                         \
     hipLaunchKernelGGL((k_sym_elec<P, M, X, Q>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, \
                        mu, inva, orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, PiF, PjF,  \
-                       PiT, PjT, L.slab, part_e)
+                       PiT, PjT, L.slab, part_e, slab32)
 #define TDP_SELEC_Q(P, M, X)              \
     if (L.perm) TDP_SELEC(P, M, X, true); \
     else TDP_SELEC(P, M, X, false)
@@ -2203,7 +2219,7 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed
 #undef TDP_SELEC_M
 #undef TDP_SELEC_Q
 #undef TDP_SELEC
-    launch_col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc);
+    launch_col_reduce(s, L, PiF, PjF, nullptr, nullptr, frc, mixed && !L.perm ? slab32 : nullptr);
     if (L.perm) launch_col_reduce(s, L, PiT, PjT, nullptr, torque, nullptr);
 }
 

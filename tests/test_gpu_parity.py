@@ -463,6 +463,51 @@ def half_box_system():
     return ref_systems.RefSystem("halfbox", system, f, pos, box)
 
 
+def collinear_grid_system(periodic):
+    """192 polarizable charges in three 64-site clusters on one exact grid (4x4x4, 0.2 nm spacing):
+    A, C = A + (0.9, 0, 0) and B = A + (1.8, 0, 0) nm.  Every site of C and B has partners in the
+    other clusters with bit-identical raw y and z, so those pairs are separated exactly along x: the
+    reference then builds the QI frame from e_y instead of e_x (ReferenceTholeDipoleForce.cpp:584-588),
+    and the closed form of the FP32 energy pass has no frame (rho = 0).  A and B are one tile each
+    and beyond the Thole range of each other (far tile pair); C is within it of both (near tile
+    pairs)."""
+    from tholedipoleplugin_amd.context import System
+    L = 4.0
+    g = np.arange(4) * 0.2
+    a = np.array([[x, y, z] for x in g for y in g for z in g]) + 0.3
+    pos = np.concatenate([a, a + np.array([0.9, 0.0, 0.0]), a + np.array([1.8, 0.0, 0.0])])
+    rng = np.random.default_rng(12)
+    q = rng.choice([-0.5, 0.5], len(pos))
+    system = System()
+    f = TholeDipoleForce()
+    f.setNonbondedMethod(T.PME if periodic else T.NoCutoff)
+    f.setPolarizationType(T.Mutual)
+    for i in range(len(pos)):
+        system.addParticle(1.0)
+        f.addParticle(float(q[i]), [0.0, 0.0, 0.0], 1.0e-3, 0.39, T.NoAxisType, -1, -1, -1)
+    box = np.diag([L, L, L]) if periodic else None
+    if periodic:
+        system.setDefaultPeriodicBoxVectors(*box)
+    system.addForce(f)
+    return ref_systems.RefSystem("collinear", system, f, pos, box)
+
+
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+@pytest.mark.parametrize("periodic", [False, True])
+def test_collinear_pairs_equal_yz(periodic, precision):
+    """Pairs with identical raw y and z (the reference's e_y frame) and separations exactly along x,
+    on far and near tile pairs, against the oracle at the bar: the FP32 closed form of the mixed
+    energy pass must hand these pairs to the explicit frame."""
+    _say("case collinear periodic=%d %s" % (periodic, precision))
+    rs = tighten(collinear_grid_system(periodic), eps=1e-12)
+    ctx, st, k = gpu_eval(rs, precision=precision, solver="pcg")
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box, target_eps=1e-13, max_iter=5000)
+    assert k.solverStats()["converged"] == 1
+    assert abs(st.getPotentialEnergy() - ref["energy"]) <= E_TOL * abs(ref["energy"])
+    assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL
+    assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
+
+
 @pytest.mark.parametrize("precision", ["double", "mixed"])
 def test_half_box_images(precision):
     _say("case half box %s" % precision)

@@ -88,6 +88,7 @@ struct tdp_context {
     bool has_perm = true;   // some particle has a nonzero molecular dipole
     float* d_slab32 = nullptr;   // mixed T*mu: FP32 tile partial slab (3 * ntri * 64), then item partials
     float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr, *posl = nullptr;
+    TilePair* d_tp = nullptr;   // [ntri] tile pair classification, recomputed every evaluation
     double4* tb_o = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
     int nwork = 0;
@@ -514,6 +515,7 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.tb_h = c->tb_h;
     L.tb_o = c->tb_o;
     L.posl = c->posl;
+    L.tp = c->d_tp;
     L.nwork = c->nwork;
     L.work = c->d_work;
     L.row_start = c->d_row_start;
@@ -676,6 +678,7 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
     launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    launch_tile_pairs(s, L, c->d_tp);
     launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     {
         double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
@@ -1080,7 +1083,8 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
         (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)) ||
-        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)))
+        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)) ||
+        (rc = dalloc(c, &c->d_tp, c->ntri)))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
         c->hist_order = std::max(4, (int)c->extrap.size());

@@ -53,6 +53,19 @@ struct DenseSys {
 };
 
 // Geometry of one all-pairs launch: nblk tiles of 64 particles.
+// Classification of tile pair (I, J), I <= J, computed once per evaluation (launch_tile_pairs) and
+// read by the pair kernels: the FP32 frame (shifts sh, undecidable bands b of the straddling axes),
+// whether every pair lies beyond the Thole range, and the squared radii beyond which pairs carry
+// no damping in FP32 (r2min) or may be evaluated in FP32 with their damping (r2d).  Indexed like
+// the tile partial slab.
+struct TilePair {
+    double shx, shy, shz;
+    float bx, by, bz;
+    unsigned flags;   // bits 0-2: straddling axes; kTpGeneral: no FP32 frame; kTpFar: beyond the Thole range
+    float r2min, r2d;
+};
+constexpr unsigned kTpStraddle = 7u, kTpGeneral = 8u, kTpFar = 16u;
+
 struct PairLaunch {
     int npad, nblk;
     TileMap tm;
@@ -66,6 +79,7 @@ struct PairLaunch {
     const float4* tb_h;                   // [nblk] tile bounding-box half extent, smallest Thole parameter
     const double4* tb_o;                  // [nblk] tile origin (FP64) of the FP32 far-field paths
     const float4* posl;                   // [npad] particle offset from its tile origin (FP32)
+    const TilePair* tp;                   // [ntri] tile pair classification (launch_tile_pairs)
     // upper-triangle sweep: work item w = tiles (I, J0..J1) as int4 (I, J0, J1, 0); items of tile row I
     // are row_start[I] .. row_start[I+1]
     int nwork;
@@ -86,6 +100,8 @@ void launch_col_reduce(hipStream_t s, const PairLaunch& L, const double4* Pi, co
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
                              const float4* mu32, const double* inva, const double* scalars, float* Pjf);
 
+// Classification of every tile pair of the work items into L.tp (after launch_tile_bounds).
+void launch_tile_pairs(hipStream_t s, const PairLaunch& L, TilePair* tp);
 // tb_o: tile origins (FP64), posl: FP32 offsets of the particles from their tile origin
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h, double4* tb_o, float4* posl);

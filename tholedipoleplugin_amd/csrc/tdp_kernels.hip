@@ -290,7 +290,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
                                                    const double* __restrict__ inva, const int* __restrict__ exc_start,
                                                    const int* __restrict__ exc_J,
                                                    const unsigned long long* __restrict__ exc_mask,
-                                                   const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                   const float4* __restrict__ tb_c, const float4* __restrict__ tb_h, const TilePair* __restrict__ tp,
                                                    BoxT64 box, const double* __restrict__ scalars,
                                                    double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     __shared__ double4 s_p[kTile];
@@ -313,6 +313,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
+        const TilePair tpr = tp[slab_tile(tm, I, J)];
         const int jb = J * kTile;
         const int cntJ = tm.cnt[J];
         __syncthreads();
@@ -330,7 +331,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
         double jx = 0.0, jy = 0.0, jz = 0.0;   // j side of slot (lane + k) mod 64
         int kend;
         if (!masked && !diag && ifull && cntJ == kTile &&
-            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
+            (tpr.flags & kTpFar)) {
 #pragma unroll 4
             for (int k = 0; k < kTile; k++) {
                 const int jj = (lane + k) & (kTile - 1);
@@ -442,6 +443,22 @@ __device__ __forceinline__ FarFrame far_frame(double4 oI, double4 oJ, float4 hI,
     return f;
 }
 
+// The frame of a tile pair from its record (the same values far_frame gives).
+__device__ __forceinline__ FarFrame frame_of(const TilePair& t, const BoxT64& box) {
+    FarFrame f;
+    f.shx = t.shx;
+    f.shy = t.shy;
+    f.shz = t.shz;
+    f.hx = (t.flags & 1u) ? (float)(0.5 * box.bx) : 0.f;
+    f.hy = (t.flags & 2u) ? (float)(0.5 * box.by) : 0.f;
+    f.hz = (t.flags & 4u) ? (float)(0.5 * box.bz) : 0.f;
+    f.bx = t.bx;
+    f.by = t.by;
+    f.bz = t.bz;
+    f.smask = (t.flags & kTpGeneral) ? -1 : (int)(t.flags & kTpStraddle);
+    return f;
+}
+
 // imaged FP32 separation component on a constant (S = false) or straddling (S = true) axis
 template <bool S>
 __device__ __forceinline__ float far_sep(float uj, float si, float h, float band, bool& amb) {
@@ -512,6 +529,39 @@ __device__ __forceinline__ float damped32_r2(float4 cI, float4 hI, float4 cJ, fl
     if (!(gmin > 0.f)) return __builtin_inff();
     const float rd = dmax * cbrtf(4.f / gmin) * 1.02f;
     return rd * rd;
+}
+
+// Classification of the tile pairs of the work items (one thread per work item, over its tiles J).
+// Positions are fixed during an evaluation, so the pair kernels read these records instead of
+// recomputing the bounds, frames and radii on every pass.
+template <bool PBC>
+__global__ __launch_bounds__(64) void k_tile_pairs(TileMap tm, const int4* __restrict__ work, int nwork,
+                                                   const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                   const double4* __restrict__ tb_o, BoxT64 box,
+                                                   TilePair* __restrict__ out) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwork) return;
+    const int4 wk = work[w];
+    const int I = wk.x;
+    const float4 cI = tb_c[I], hI = tb_h[I];
+    const double4 oI = tb_o[I];
+    for (int J = wk.y; J < wk.z; J++) {
+        const float4 cJ = tb_c[J], hJ = tb_h[J];
+        const FarFrame fr = far_frame(oI, tb_o[J], hI, hJ, PBC, box);
+        TilePair t;
+        t.shx = fr.shx;
+        t.shy = fr.shy;
+        t.shz = fr.shz;
+        t.bx = fr.bx;
+        t.by = fr.by;
+        t.bz = fr.bz;
+        unsigned fl = fr.smask < 0 ? kTpGeneral : (unsigned)fr.smask;
+        if (tiles_beyond_damping(cI, hI, cJ, hJ, PBC, (float)box.bx, (float)box.by, (float)box.bz)) fl |= kTpFar;
+        t.flags = fl;
+        t.r2min = undamped_r2_fp32(cI, hI, cJ, hJ);
+        t.r2d = damped32_r2(cI, hI, cJ, hJ);
+        out[slab_tile(tm, I, J)] = t;
+    }
 }
 
 // Near tile pair of T * mu in mixed precision (full, unmasked, off-diagonal tiles on frame SM).
@@ -638,7 +688,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                                                          const int* __restrict__ exc_start,
                                                          const int* __restrict__ exc_J,
                                                          const unsigned long long* __restrict__ exc_mask,
-                                                         const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                         const float4* __restrict__ tb_c, const float4* __restrict__ tb_h, const TilePair* __restrict__ tp,
                                                          const double4* __restrict__ tb_o,
                                                          const float4* __restrict__ posl, BoxT64 box, const double* __restrict__ scalars,
                                                          float* __restrict__ Pjf, size_t S) {
@@ -685,6 +735,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
+        const TilePair tpr = tp[slab_tile(tm, I, J)];
         const int jb = J * kTile;
         const int cntJ = tm.cnt[J];
         const bool diag = J == I;
@@ -695,14 +746,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             e++;
         }
         const bool far = !masked && !diag && ifull && cntJ == kTile &&
-                         tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by,
-                                              (float)box.bz);
+                         (tpr.flags & kTpFar);
         double jx, jy, jz;
         int kend;
         FarFrame fr_n;
         __syncthreads();
         if (far) {
-            const FarFrame fr = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box);
+            const FarFrame fr = frame_of(tpr, box);
             {
                 const float4 u = mu32[jb + lane];
                 s_uf[lane] = u;
@@ -779,7 +829,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             jz = (double)fz;
             kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
         } else if (!masked && !diag && ifull && cntJ == kTile &&
-                   (fr_n = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box)).smask >= 0) {
+                   (fr_n = frame_of(tpr, box)).smask >= 0) {
             const double4 pj = posa[jb + lane];
             const double iaj = inva[jb + lane];
             s_p[lane] = pj;
@@ -797,8 +847,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 s_na[lane + kTile] = na;
             }
             __syncthreads();
-            const float r2min = undamped_r2_fp32(cI, hI, tb_c[J], tb_h[J]);
-            const float r2d = damped32_r2(cI, hI, tb_c[J], tb_h[J]);
+            const float r2min = tpr.r2min;
+            const float r2d = tpr.r2d;
             const float si_x = (float)((double)li.x - fr_n.shx), si_y = (float)((double)li.y - fr_n.shy),
                         si_z = (float)((double)li.z - fr_n.shz);
             unsigned long long steps64 = 0ull;
@@ -881,7 +931,7 @@ __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const in
                                                   const double* __restrict__ inva, const int* __restrict__ exc_start,
                                                   const int* __restrict__ exc_J,
                                                   const unsigned long long* __restrict__ exc_mask,
-                                                  const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                  const float4* __restrict__ tb_c, const float4* __restrict__ tb_h, const TilePair* __restrict__ tp,
                                                   BoxT64 box, double4* __restrict__ Pi, double* __restrict__ Pj,
                                                   size_t S) {
     __shared__ double4 s_p[kTile];
@@ -903,6 +953,7 @@ __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const in
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
+        const TilePair tpr = tp[slab_tile(tm, I, J)];
         const int jb = J * kTile;
         const int cntJ = tm.cnt[J];
         __syncthreads();
@@ -921,7 +972,7 @@ __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const in
         double jx = 0.0, jy = 0.0, jz = 0.0;
         int kend;
         if (!masked && !diag && ifull && cntJ == kTile &&
-            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
+            (tpr.flags & kTpFar)) {
 #pragma unroll 2
             for (int k = 0; k < kTile; k++) {
                 const int jj = (lane + k) & (kTile - 1);
@@ -1298,7 +1349,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                                                  const int* __restrict__ orig, const int* __restrict__ exc_start,
                                                  const int* __restrict__ exc_J,
                                                  const unsigned long long* __restrict__ exc_mask,
-                                                 const float4* __restrict__ tb_c, const float4* __restrict__ tb_h,
+                                                 const float4* __restrict__ tb_c, const float4* __restrict__ tb_h, const TilePair* __restrict__ tp,
                                                  const double4* __restrict__ tb_o, const float4* __restrict__ posl,
                                                  BoxT64 box, double4* __restrict__ PiF, double* __restrict__ PjF,
                                                  double4* __restrict__ PiT, double* __restrict__ PjT, size_t S,
@@ -1335,6 +1386,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
     const int eEnd = exc_start[I + 1];
     while (e < eEnd && exc_J[e] < J0) e++;
     for (int J = J0; J < J1; J++) {
+        const TilePair tpr = tp[slab_tile(tm, I, J)];
         const int jb = J * kTile;
         const int cntJ = tm.cnt[J];
         __syncthreads();
@@ -1357,9 +1409,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
         int kend;
         FarFrame fr_n;
         if (!masked && !diag && ifull && cntJ == kTile &&
-            tiles_beyond_damping(cI, hI, tb_c[J], tb_h[J], PBC, (float)box.bx, (float)box.by, (float)box.bz)) {
+            (tpr.flags & kTpFar)) {
             if (MIXED) {
-                const FarFrame fr = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box);
+                const FarFrame fr = frame_of(tpr, box);
                 {
                     const double4 q = s_q[lane], u = s_u[lane];
                     s_qf[lane] = make_float4((float)q.x, (float)q.y, (float)q.z, (float)q.w);
@@ -1414,7 +1466,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 kend = kTile;
             }
         } else if (MIXED && !PERM && !masked && !diag && ifull && cntJ == kTile &&
-                   (fr_n = far_frame(oI, tb_o[J], hI, tb_h[J], PBC, box)).smask >= 0) {
+                   (fr_n = frame_of(tpr, box)).smask >= 0) {
             // near tile pair, mixed: FP32 steps by the closed form, the rest in FP64 (pass 2)
             {
                 const double4 q = s_q[lane], u = s_u[lane];
@@ -1426,7 +1478,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 s_jn[2 * kTile + lane] = 0.0;
             }
             __syncthreads();
-            const float r2d = damped32_r2(cI, hI, tb_c[J], tb_h[J]);
+            const float r2d = tpr.r2d;
             const float si_x = (float)((double)li.x - fr_n.shx), si_y = (float)((double)li.y - fr_n.shy),
                         si_z = (float)((double)li.z - fr_n.shz);
             const unsigned key = __float_as_uint(li.w);
@@ -2129,6 +2181,17 @@ void launch_load_positions(hipStream_t s, int npad, const double* pos3, const in
     hipLaunchKernelGGL(k_load_positions, dim3((npad + 255) / 256), dim3(256), 0, s, npad, pos3, perm, thole, posa);
 }
 
+void launch_tile_pairs(hipStream_t s, const PairLaunch& L, TilePair* tp) {
+    if (L.nwork <= 0) return;
+    BoxT64 b = mkbox(L.box);
+    const dim3 grid((L.nwork + 63) / 64);
+    if (L.pbc)
+        hipLaunchKernelGGL(k_tile_pairs<true>, grid, dim3(64), 0, s, L.tm, L.work, L.nwork, L.tb_c, L.tb_h, L.tb_o, b, tp);
+    else
+        hipLaunchKernelGGL(k_tile_pairs<false>, grid, dim3(64), 0, s, L.tm, L.work, L.nwork, L.tb_c, L.tb_h, L.tb_o, b,
+                           tp);
+}
+
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
                         float4* tb_c, float4* tb_h, double4* tb_o, float4* posl) {
     hipLaunchKernelGGL(k_tile_bounds, dim3(nblk), dim3(kTile), 0, s, tcnt, posa, box3[0], box3[1], box3[2],
@@ -2163,10 +2226,10 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, b, scalars, Pi, Pj, L.slab);
     else
         hipLaunchKernelGGL(k_sym_matvec<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, scalars, Pi, Pj, L.slab);
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, b, scalars, Pi, Pj, L.slab);
 }
 
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,
@@ -2174,11 +2237,11 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec_mixed<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
-                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
+                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, L.tb_o, L.posl, b, scalars,
                            Pjf, L.slab);
     else
         hipLaunchKernelGGL(k_sym_matvec_mixed<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu,
-                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, scalars,
+                           mu32, inva, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, L.tb_o, L.posl, b, scalars,
                            Pjf, L.slab);
 }
 
@@ -2187,7 +2250,7 @@ void launch_sym_field(hipStream_t s, const PairLaunch& L, const double4* posa, c
     BoxT64 b = mkbox(L.box);
 #define TDP_SFIELD(P, Q)                                                                                          \
     hipLaunchKernelGGL((k_sym_field<P, Q>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, inva, \
-                       L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, b, Pi, Pj, L.slab)
+                       L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, b, Pi, Pj, L.slab)
     if (L.pbc) {
         if (L.perm) TDP_SFIELD(true, true); else TDP_SFIELD(true, false);
     } else {
@@ -2203,7 +2266,7 @@ void launch_sym_elec(hipStream_t s, const PairLaunch& L, bool mutual, bool mixed
     BoxT64 b = mkbox(L.box);
 #define TDP_SELEC(P, M, X, Q)                                                                                         \
     hipLaunchKernelGGL((k_sym_elec<P, M, X, Q>), dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, qd, \
-                       mu, inva, orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tb_o, L.posl, b, PiF, PjF,  \
+                       mu, inva, orig, L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, L.tb_o, L.posl, b, PiF, PjF,  \
                        PiT, PjT, L.slab, part_e, slab32)
 #define TDP_SELEC_Q(P, M, X)              \
     if (L.perm) TDP_SELEC(P, M, X, true); \

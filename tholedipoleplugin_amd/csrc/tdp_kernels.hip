@@ -577,8 +577,7 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
                                                                   const float4* __restrict__ s_uf, int lane,
                                                                   float si_x, float si_y, float si_z,
                                                                   const FarFrame& f, float4 uf, float3 uf3,
-                                                                  float r2min, float r2d,
-                                                                  const float2* __restrict__ s_na, float2 nai,
+                                                                  float r2min, float r2d, float2 nai,
                                                                   double& ax, double& ay, double& az,
                                                                   double& jx, double& jy, double& jz) {
     float tx = 0.f, ty = 0.f, tz = 0.f, sx3 = 0.f, sy3 = 0.f, sz3 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
@@ -606,9 +605,9 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
         if (__builtin_amdgcn_ballot_w64(!(r2 > r2min))) {
             // getAndScaleInverseRs in FP32: s = -min(a_i, a_j) (r / (a_i a_j))^3, rr3 *= 1 - e^s,
             // rr5 *= 1 - (1 - s) e^s; for s <= -30 both factors round to 1
-            const float2 naj = s_na[jj];
-            const float rat = r2 * rinv * (nai.y * naj.y);
-            const float sv = -fminf(nai.x, naj.x) * (rat * rat * rat);
+            // staged: 1/a_j in the offsets' w, a_j in the dipoles' w
+            const float rat = r2 * rinv * (nai.y * pl.w);
+            const float sv = -fminf(nai.x, uj.w) * (rat * rat * rat);
             const float ev = __expf(sv);
             const float l3 = 1.f - ev;
             rr3 *= l3;
@@ -699,8 +698,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     // the general far path, so a fallback from a frame to FP64 positions restages only s_pf.
     // Near tiles in mixed steps add FP32 offsets and dipoles (twice over) behind the FP64 arrays.
     constexpr int kBytesNear = kTile * (2 * sizeof(double4) + sizeof(double));
-    constexpr int kBytesNearMixed =
-        kBytesNear + 2 * kTile * 2 * sizeof(float4) + 3 * kTile * sizeof(double) + 2 * kTile * sizeof(float2);
+    constexpr int kBytesNearMixed = kBytesNear + 2 * kTile * 2 * sizeof(float4) + 3 * kTile * sizeof(double);
     constexpr int kBytesFar = 2 * kTile * (sizeof(double4) + sizeof(float4));
     __shared__ __align__(16) unsigned char s_raw[kBytesNearMixed > kBytesFar ? kBytesNearMixed : kBytesFar];
     double4* s_p = reinterpret_cast<double4*>(s_raw);
@@ -709,7 +707,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
     float4* s_nl = reinterpret_cast<float4*>(s_raw + kBytesNear);
     float4* s_nu = s_nl + 2 * kTile;
     double* s_jn = reinterpret_cast<double*>(s_nu + 2 * kTile);
-    float2* s_na = reinterpret_cast<float2*>(s_jn + 3 * kTile);   // (a, 1/a) of the j tile, twice over
     double4* s_pf = reinterpret_cast<double4*>(s_raw);
     float4* s_uf = reinterpret_cast<float4*>(s_pf + 2 * kTile);
     float4* s_l = reinterpret_cast<float4*>(s_raw);
@@ -836,15 +833,14 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             s_u[lane] = mu[jb + lane];
             s_ia[lane] = iaj;
             {
-                const float4 l = posl[jb + lane];
-                const float4 u = mu32[jb + lane];
-                const float2 na = make_float2((float)pj.w, (float)iaj);
+                float4 l = posl[jb + lane];
+                float4 u = mu32[jb + lane];
+                l.w = (float)iaj;   // the Thole parameters of the FP32 damped steps ride in the unused w
+                u.w = (float)pj.w;
                 s_nl[lane] = l;
                 s_nl[lane + kTile] = l;
                 s_nu[lane] = u;
                 s_nu[lane + kTile] = u;
-                s_na[lane] = na;
-                s_na[lane + kTile] = na;
             }
             __syncthreads();
             const float r2min = tpr.r2min;
@@ -854,8 +850,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             unsigned long long steps64 = 0ull;
 #define TDP_NEAR_CASE(M)                                                                                      \
     case M:                                                                                                   \
-        steps64 = near_matvec_mixed32<M>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min, r2d, s_na,   \
-                                         nai, ax, ay, az, jx, jy, jz);                                        \
+        steps64 = near_matvec_mixed32<M>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min, r2d, nai, ax, \
+                                         ay, az, jx, jy, jz);                                                 \
         break;
             switch (fr_n.smask) {
                 TDP_NEAR_CASE(0)

@@ -1638,8 +1638,9 @@ __global__ __launch_bounds__(kRedWaves * 64) void k_col_reduce(TileMap tm, const
 // The vector kernels run one 64-thread block per tile; each tile belongs to one system (segment)
 // of the context, whose solver scalars sit at scalars + SC_COUNT * s.  Padding slots carry
 // alpha = 0 and stay zero.  Norms are summed per tile (wave_sum, fixed order) into red[T] and
-// red[ntile + T], then per system by one 64-thread block over its tiles (fixed assignment and
-// order), so every result is deterministic.  A system whose solve stops adds 1 to done[0].
+// red[ntile + T] (the PCG p.q sums into red[2 ntile + T]), then per system over its tiles in a
+// fixed assignment and order, by one 64-thread block or by every block of the next kernel alike
+// (segment_sum), so every result is deterministic.  A system whose solve stops adds 1 to done[0].
 
 // E0 = reduced field; mu = alpha*E0 (calculateInducedDipoles, ReferenceTholeDipoleForce.cpp:743-746).
 __global__ void k_field_init(int npad, const double4* __restrict__ part, const double* __restrict__ alpha,

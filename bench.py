@@ -35,10 +35,10 @@ PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
 # HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 66.5 MB
+# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 98.5 MB
 # read + 166.8 MB written (the FP32 tile partial slab, 128 MB, and the FP32 work-item partials).
 # Counters cannot be read inside this process.
-MATVEC_HBM_BYTES = 233.3e6
+MATVEC_HBM_BYTES = 265.3e6
 
 
 def parse():
@@ -53,6 +53,7 @@ def parse():
                    help="run only the cluster-solve measurement and print its object (for profiling)")
     p.add_argument("--no-dense", action="store_true", help="skip the dense cluster-response measurement")
     p.add_argument("--no-saline", action="store_true", help="skip the 96,000-site saline box measurement")
+    p.add_argument("--no-md", action="store_true", help="skip the warm-started trajectory measurement")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep each context's default schedule (no tdp_autotune before the warm-up)")
     p.add_argument("--dense-only", action="store_true",
@@ -295,6 +296,47 @@ def saline_bench(torch, steps):
     }
 
 
+def md_bench(torch, s, steps):
+    """Evaluations along a trajectory-like sequence of configurations of the 36,864-site box: before
+    each evaluation every coordinate moves by a seeded Gaussian step of 0.0005 nm (about the
+    displacement of one ~1 fs MD step), and each Mutual solve starts from the previous evaluation's
+    dipoles (tdp_set_warm_start), as an integrator loop would run it.  Mixed precision, PCG."""
+    h = make_handle(s, "mixed")
+    if not NO_AUTOTUNE:
+        h.autotune(s.positions, s.box)
+    h.set_warm_start(True)
+    dev = torch.device("cuda:0")
+    pos = torch.tensor(s.positions.reshape(-1), dtype=torch.float64, device=dev)
+    frc = torch.zeros(pos.numel(), dtype=torch.float64, device=dev)
+    en = torch.zeros(1, dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(20261210)
+    warm = 3
+    noise = torch.randn((warm + steps, pos.numel()), generator=gen, device=dev, dtype=torch.float64) * 5e-4
+    for k in range(warm):
+        pos.add_(noise[k])
+        h.execute_device(pos.data_ptr(), s.box, frc.data_ptr(), en.data_ptr(), stream)
+    torch.cuda.synchronize()
+    iters = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        pos.add_(noise[warm + k])
+        h.execute_device(pos.data_ptr(), s.box, frc.data_ptr(), en.data_ptr(), stream)
+        iters.append(h.solver_stats()["iterations"])
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    h.close()
+    out = {"workload": "36,864-site box, every coordinate moved by N(0, 0.0005 nm) before each evaluation, "
+                       "Mutual solve warm-started from the previous dipoles, mixed PCG to 1e-6 D",
+           "unit": "evals/s", "evals_per_s": steps / dt, "ms_per_step": 1000.0 * dt / steps,
+           "pcg_iterations_mean": float(np.mean(iters)), "pcg_iterations_max": int(np.max(iters)), "steps": steps}
+    print("# md warm start: %.2f ms/step, PCG iterations %.1f (max %d)" % (out["ms_per_step"], out["pcg_iterations_mean"],
+                                                                           out["pcg_iterations_max"]),
+          file=sys.stderr, flush=True)
+    return out
+
+
 NO_AUTOTUNE = False
 
 
@@ -350,6 +392,9 @@ def main():
     dense = None
     if not args.no_dense:
         dense = dense_bench(max(1, min(args.steps, 2)))
+    md = None
+    if not args.no_md:
+        md = md_bench(torch, s, args.steps)
     saline = None
     if not args.no_saline:
         saline = saline_bench(torch, max(2, min(args.steps, 8)))
@@ -398,6 +443,7 @@ def main():
         "clusters": clusters,
         "dense_clusters": dense,
         "saline96000": saline,
+        "md_warm_start": md,
     }
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:

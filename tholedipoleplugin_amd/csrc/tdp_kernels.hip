@@ -181,9 +181,11 @@ __global__ __launch_bounds__(64) void k_tile_bounds(const int* __restrict__ tcnt
         amin = fmin(amin, __shfl_xor(amin, m));
     }
     const double c[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    // padding slots sit 1e6 nm from the tile origin: their FP32 pairs stay finite (and carry zero
+    // charge, dipole and Thole parameters), so the FP32 paths need no masks on partial tiles
     posl[i] = valid ? make_float4((float)(d[0] - c[0]), (float)(d[1] - c[1]), (float)(d[2] - c[2]),
                                   __uint_as_float(yz_key(p.y, p.z)))
-                    : make_float4(0.f, 0.f, 0.f, 0.f);
+                    : make_float4(1e6f, 1e6f, 1e6f, 0.f);
     if (lane == 0) {
         tb_c[T] = make_float4((float)(p0.x + c[0]), (float)(p0.y + c[1]), (float)(p0.z + c[2]), (float)amax);
         // half extents rounded up so the box still contains every particle after the float conversion
@@ -564,7 +566,7 @@ __global__ __launch_bounds__(64) void k_tile_pairs(TileMap tm, const int4* __res
     }
 }
 
-// Near tile pair of T * mu in mixed precision (full, unmasked, off-diagonal tiles on frame SM).
+// Near tile pair of T * mu in mixed precision (unmasked, off-diagonal tiles on frame SM, full or partial).
 // Pass 1 (near_matvec_mixed32) runs the 64 steps in FP32 like a far tile pair, with the Thole
 // factors in FP32 on steps where some pair lies within r2min (every a is then nonzero: r2min is
 // infinite otherwise), except steps in which some pair lies within r2d (or a straddling pair
@@ -638,9 +640,10 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
 template <bool PBC>
 __device__ __forceinline__ void near_matvec_mixed64(unsigned long long steps64, const double4* __restrict__ s_p,
                                                     const double4* __restrict__ s_u, const double* __restrict__ s_ia,
-                                                    double* __restrict__ s_jn, int lane, const double4& pi,
-                                                    const double4& ui, double iai, const BoxT64& box, double& ax,
-                                                    double& ay, double& az, double& jx, double& jy, double& jz) {
+                                                    double* __restrict__ s_jn, int lane, bool ivalid, int cntJ,
+                                                    const double4& pi, const double4& ui, double iai,
+                                                    const BoxT64& box, double& ax, double& ay, double& az,
+                                                    double& jx, double& jy, double& jz) {
     s_jn[lane] = 0.0;
     s_jn[kTile + lane] = 0.0;
     s_jn[2 * kTile + lane] = 0.0;
@@ -653,8 +656,13 @@ __device__ __forceinline__ void near_matvec_mixed64(unsigned long long steps64, 
         const double4 uj = s_u[jd];
         double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
         if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+        const bool ok = ivalid && jd < cntJ;   // padding slots of partial tiles sit at the origin
         double rr3, rr5;
-        damped_rr_wave(ex * ex + ey * ey + ez * ez, pi.w, pj.w, iai, s_ia[jd], rr3, rr5);
+        damped_rr_wave(ok ? ex * ex + ey * ey + ez * ez : 1.0, pi.w, pj.w, iai, s_ia[jd], rr3, rr5);
+        if (!ok) {
+            rr3 = 0.0;
+            rr5 = 0.0;
+        }
         double udj = rr5 * (uj.x * ex + uj.y * ey + uj.z * ez);
         double udi = rr5 * (ui.x * ex + ui.y * ey + ui.z * ez);
         ax += udj * ex - rr3 * uj.x;
@@ -825,8 +833,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             jy = (double)fy;
             jz = (double)fz;
             kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
-        } else if (!masked && !diag && ifull && cntJ == kTile &&
-                   (fr_n = frame_of(tpr, box)).smask >= 0) {
+        } else if (!masked && !diag && (fr_n = frame_of(tpr, box)).smask >= 0) {
+            // near tile pair, or a far one with a partial tile: padding slots need no masks in FP32
+            // (their staged offsets lie 1e6 nm away) and are masked in the FP64 pass
             const double4 pj = posa[jb + lane];
             const double iaj = inva[jb + lane];
             s_p[lane] = pj;
@@ -864,7 +873,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
                 TDP_NEAR_CASE(7)
             }
 #undef TDP_NEAR_CASE
-            if (steps64) near_matvec_mixed64<PBC>(steps64, s_p, s_u, s_ia, s_jn, lane, pi, ui, iai, box, ax, ay, az, jx, jy, jz);
+            if (steps64)
+                near_matvec_mixed64<PBC>(steps64, s_p, s_u, s_ia, s_jn, lane, ivalid, cntJ, pi, ui, iai, box, ax, ay, az,
+                                         jx, jy, jz);
             kend = kTile;
         } else {
             s_p[lane] = posa[jb + lane];

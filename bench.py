@@ -388,7 +388,7 @@ def main():
         e_err = abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"])
     clusters = None
     if not args.no_clusters:
-        clusters = cluster_bench(torch, max(1, min(args.steps, 3)), 1)
+        clusters = cluster_bench(torch, max(1, min(args.steps, 10)), 3)
     dense = None
     if not args.no_dense:
         dense = dense_bench(max(1, min(args.steps, 2)))

@@ -1250,8 +1250,8 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
     return true;
 }
 
-// Near tile pair of the energy pass without permanent dipoles in mixed precision (full, unmasked,
-// off-diagonal tiles on frame SM), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
+// Near tile pair of the energy pass without permanent dipoles in mixed precision (unmasked,
+// off-diagonal tiles on frame SM, full or partial), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
 // a u^3 >= 4) in FP32 by the closed form of elec_far32_lab with the Thole factors of
 // calculateElectrostaticPairIxn (:205-220): with e = exp(-a u^3), thole_c = 1 - e, thole_d0 =
 // 1 - e (1 + 1.5 a u^3), dthole_d0 = 1 - e (1 + a u^3 + 1.5 (a u^3)^2), dthole_d1 = 1 - e (1 + a u^3),
@@ -1472,8 +1472,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 }
                 kend = kTile;
             }
-        } else if (MIXED && !PERM && !masked && !diag && ifull && cntJ == kTile &&
-                   (fr_n = frame_of(tpr, box)).smask >= 0) {
+        } else if (MIXED && !PERM && !masked && !diag && (fr_n = frame_of(tpr, box)).smask >= 0) {
+            // full or partial tiles: padding carries no charge or dipole and its staged offsets lie
+            // 1e6 nm away (finite FP32 terms); the FP64 steps mask it
             // near tile pair, mixed: FP32 steps by the closed form, the rest in FP64 (pass 2)
             {
                 const double4 q = s_q[lane], u = s_u[lane];
@@ -1515,7 +1516,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 ElecSite other;
                 load_site(other, s_p[jd], s_q[jd], s_u[jd], s_ia[jd]);
                 ElecAcc jb1 = {0, 0, 0, 0, 0, 0};
-                elec_sym_step<MUTUAL, true, PERM>(me, my_orig, other, s_o[jd], true, 1.0, 1.0, PBC, box, acc, jb1, en);
+                elec_sym_step<MUTUAL, true, PERM>(me, my_orig, other, s_o[jd], ivalid && jd < cntJ, 1.0, 1.0, PBC, box,
+                                                  acc, jb1, en);
                 s_jn[jd] += jb1.fx;
                 s_jn[kTile + jd] += jb1.fy;
                 s_jn[2 * kTile + jd] += jb1.fz;

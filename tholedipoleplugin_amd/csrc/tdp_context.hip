@@ -1627,27 +1627,37 @@ int tdp_autotune(tdp_context* c, const double* positions, const double* box, int
         int longest = 1;
         for (int g = 0; g < c->nseg; g++) longest = std::max(longest, c->seg_t1[g] - c->seg_t0[g]);
         static const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16};
-        int best = c->work_C;
-        float best_ms = 0.f;
+        constexpr int kCand = sizeof(cand) / sizeof(cand[0]);
+        float cand_ms[kCand];
+        for (int q = 0; q < kCand; q++) cand_ms[q] = 0.f;
         hipEvent_t e0 = take_event(c), e1 = take_event(c);
-        for (int C : cand) {
-            if (C > longest && C != 1) break;
-            if ((rc = set_work_items(c, C))) return rc;
-            PairLaunch L = pair_launch(c, box);
-            matvec(c, L, c->mu, nullptr);   // warm-up
-            TDP_HIP(c, hipEventRecord(e0, c->stream));
-            for (int k = 0; k < 3; k++) matvec(c, L, c->mu, nullptr);
-            TDP_HIP(c, hipEventRecord(e1, c->stream));
-            TDP_HIP(c, hipEventSynchronize(e1));
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
-            if (best_ms == 0.f || ms < best_ms) {
-                best_ms = ms;
-                best = C;
+        // two rounds over the candidates, each timed on three passes; a candidate's time is the smaller
+        // of its two, so that one disturbed measurement does not decide the schedule
+        for (int round = 0; round < 2; round++) {
+            for (int q = 0; q < kCand; q++) {
+                const int C = cand[q];
+                if (C > longest && C != 1) break;
+                if ((rc = set_work_items(c, C))) return rc;
+                PairLaunch L = pair_launch(c, box);
+                matvec(c, L, c->mu, nullptr);   // warm-up
+                TDP_HIP(c, hipEventRecord(e0, c->stream));
+                for (int k = 0; k < 3; k++) matvec(c, L, c->mu, nullptr);
+                TDP_HIP(c, hipEventRecord(e1, c->stream));
+                TDP_HIP(c, hipEventSynchronize(e1));
+                float ms = 0.f;
+                hipEventElapsedTime(&ms, e0, e1);
+                if (round == 0 || ms < cand_ms[q]) cand_ms[q] = ms;
             }
         }
         c->ev_pool.push_back(e0);
         c->ev_pool.push_back(e1);
+        int best = c->work_C;
+        float best_ms = 0.f;
+        for (int q = 0; q < kCand; q++)
+            if (cand_ms[q] > 0.f && (best_ms == 0.f || cand_ms[q] < best_ms)) {
+                best_ms = cand_ms[q];
+                best = cand[q];
+            }
         if ((rc = set_work_items(c, best))) return rc;
     }
     if (tiles_per_item) *tiles_per_item = c->work_C;

@@ -574,12 +574,15 @@ __global__ __launch_bounds__(64) void k_tile_pairs(TileMap tm, const int4* __res
 // the marked steps in FP64 as k_sym_matvec does; in a step every lane serves a different j slot,
 // so its j terms are added to the slot's LDS accumulator s_jn (3 x 64 doubles) without conflicts
 // or rotation.
-template <int SM>
+// MASKED: the tile pair holds excluded pairs (bit s of mskip: the pair with slot s has iScale 0);
+// a step with such a pair goes to the FP64 pass, which skips it.
+template <int SM, bool MASKED>
 __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* __restrict__ s_l,
                                                                   const float4* __restrict__ s_uf, int lane,
                                                                   float si_x, float si_y, float si_z,
                                                                   const FarFrame& f, float4 uf, float3 uf3,
                                                                   float r2min, float r2d, float2 nai,
+                                                                  unsigned long long mskip,
                                                                   double& ax, double& ay, double& az,
                                                                   double& jx, double& jy, double& jz) {
     float tx = 0.f, ty = 0.f, tz = 0.f, sx3 = 0.f, sy3 = 0.f, sz3 = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
@@ -592,6 +595,7 @@ __device__ __forceinline__ unsigned long long near_matvec_mixed32(const float4* 
         const float dy = far_sep<(SM & 2) != 0>(pl.y, si_y, f.hy, f.by, amb);
         const float dz = far_sep<(SM & 4) != 0>(pl.z, si_z, f.hz, f.bz, amb);
         const float r2 = dx * dx + dy * dy + dz * dz;
+        if (MASKED) amb = amb || ((mskip >> (jj & (kTile - 1))) & 1ull);
         if (__builtin_amdgcn_ballot_w64(amb || !(r2 > r2d))) {
             steps64 |= 1ull << k;
             fx = rot1(fx);
@@ -641,6 +645,7 @@ template <bool PBC>
 __device__ __forceinline__ void near_matvec_mixed64(unsigned long long steps64, const double4* __restrict__ s_p,
                                                     const double4* __restrict__ s_u, const double* __restrict__ s_ia,
                                                     double* __restrict__ s_jn, int lane, bool ivalid, int cntJ,
+                                                    unsigned long long mskip,
                                                     const double4& pi, const double4& ui, double iai,
                                                     const BoxT64& box, double& ax, double& ay, double& az,
                                                     double& jx, double& jy, double& jz) {
@@ -656,7 +661,8 @@ __device__ __forceinline__ void near_matvec_mixed64(unsigned long long steps64, 
         const double4 uj = s_u[jd];
         double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
         if (PBC) min_image(ex, ey, ez, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
-        const bool ok = ivalid && jd < cntJ;   // padding slots of partial tiles sit at the origin
+        // padding slots of partial tiles sit at the origin; excluded pairs (iScale 0) are skipped
+        const bool ok = ivalid && jd < cntJ && !((mskip >> jd) & 1ull);
         double rr3, rr5;
         damped_rr_wave(ok ? ex * ex + ey * ey + ez * ez : 1.0, pi.w, pj.w, iai, s_ia[jd], rr3, rr5);
         if (!ok) {
@@ -833,9 +839,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             jy = (double)fy;
             jz = (double)fz;
             kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
-        } else if (!masked && !diag && (fr_n = frame_of(tpr, box)).smask >= 0) {
-            // near tile pair, or a far one with a partial tile: padding slots need no masks in FP32
-            // (their staged offsets lie 1e6 nm away) and are masked in the FP64 pass
+        } else if (!diag && (fr_n = frame_of(tpr, box)).smask >= 0) {
+            // near tile pair, or a far one with a partial tile or excluded pairs: padding slots need no
+            // masks in FP32 (their staged offsets lie 1e6 nm away) and are masked in the FP64 pass, as
+            // are excluded pairs, whose steps all go to the FP64 pass
             const double4 pj = posa[jb + lane];
             const double iaj = inva[jb + lane];
             s_p[lane] = pj;
@@ -857,10 +864,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             const float si_x = (float)((double)li.x - fr_n.shx), si_y = (float)((double)li.y - fr_n.shy),
                         si_z = (float)((double)li.z - fr_n.shz);
             unsigned long long steps64 = 0ull;
-#define TDP_NEAR_CASE(M)                                                                                      \
-    case M:                                                                                                   \
-        steps64 = near_matvec_mixed32<M>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min, r2d, nai, ax, \
-                                         ay, az, jx, jy, jz);                                                 \
+#define TDP_NEAR_CASE(M)                                                                                       \
+    case M:                                                                                                    \
+        steps64 = masked ? near_matvec_mixed32<M, true>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min,   \
+                                                        r2d, nai, mi, ax, ay, az, jx, jy, jz)                   \
+                         : near_matvec_mixed32<M, false>(s_nl, s_nu, lane, si_x, si_y, si_z, fr_n, uf, uf3, r2min,  \
+                                                         r2d, nai, 0ull, ax, ay, az, jx, jy, jz);               \
         break;
             switch (fr_n.smask) {
                 TDP_NEAR_CASE(0)
@@ -874,8 +883,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             }
 #undef TDP_NEAR_CASE
             if (steps64)
-                near_matvec_mixed64<PBC>(steps64, s_p, s_u, s_ia, s_jn, lane, ivalid, cntJ, pi, ui, iai, box, ax, ay, az,
-                                         jx, jy, jz);
+                near_matvec_mixed64<PBC>(steps64, s_p, s_u, s_ia, s_jn, lane, ivalid, cntJ, mi, pi, ui, iai, box, ax, ay,
+                                         az, jx, jy, jz);
             kend = kTile;
         } else {
             s_p[lane] = posa[jb + lane];

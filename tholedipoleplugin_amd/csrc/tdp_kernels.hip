@@ -1270,14 +1270,17 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
 //               (X.a)(X.b)) / r^4 for Mutual, c6 = 4 thole_d0 + 2 thole_c].
 // Steps with a closer pair, an undecidable straddling pair, possibly equal raw y and z or a
 // separation along x are returned in a step mask for the FP64 pass (their j sums only rotate).
-template <bool PBC, bool MUTUAL, int SM>
+// MASKED: the tile pair holds scaled pairs (bit s of mscaled: the pair with slot s has mScale or
+// iScale below 1); a step with such a pair goes to the FP64 pass, which applies the scales.
+template <bool PBC, bool MUTUAL, int SM, bool MASKED>
 __device__ __forceinline__ unsigned long long elec_near32_lab(const float4* __restrict__ s_l,
                                                               const float4* __restrict__ s_uf,
                                                               const float2* __restrict__ s_na,
                                                               const int* __restrict__ s_o, int lane,
                                                               const ElecSite& me, int my_orig, unsigned key,
                                                               float si_x, float si_y, float si_z, const FarFrame& fr,
-                                                              float r2d, ElecAcc& acc, ElecAcc& jacc, double& en) {
+                                                              float r2d, unsigned long long mscaled, ElecAcc& acc,
+                                                              ElecAcc& jacc, double& en) {
     const float Qm = (float)me.q, mx = (float)me.ux, my = (float)me.uy, mz = (float)me.uz;
     const float am_ = (float)me.a, iam = (float)me.ia;
     float ifx = 0.f, ify = 0.f, ifz = 0.f, e32 = 0.f, jfx = 0.f, jfy = 0.f, jfz = 0.f;
@@ -1291,6 +1294,7 @@ __device__ __forceinline__ unsigned long long elec_near32_lab(const float4* __re
         const float dz = far_sep<(SM & 4) != 0>(lj.z, si_z, fr.hz, fr.bz, amb);
         const float rho2 = dy * dy + dz * dz;
         const float r2 = dx * dx + rho2;
+        if (MASKED) amb = amb || ((mscaled >> jj) & 1ull);
         if (__builtin_amdgcn_ballot_w64(amb || __float_as_uint(lj.w) == key || !(rho2 > 0.f) || !(r2 > r2d))) {
             steps64 |= 1ull << k;
             jfx = rot1(jfx);
@@ -1481,9 +1485,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 }
                 kend = kTile;
             }
-        } else if (MIXED && !PERM && !masked && !diag && (fr_n = frame_of(tpr, box)).smask >= 0) {
+        } else if (MIXED && !PERM && !diag && (fr_n = frame_of(tpr, box)).smask >= 0) {
             // full or partial tiles: padding carries no charge or dipole and its staged offsets lie
-            // 1e6 nm away (finite FP32 terms); the FP64 steps mask it
+            // 1e6 nm away (finite FP32 terms); the FP64 steps mask it.  Steps with scaled pairs go to
+            // the FP64 steps, which apply the scales.
             // near tile pair, mixed: FP32 steps by the closed form, the rest in FP64 (pass 2)
             {
                 const double4 q = s_q[lane], u = s_u[lane];
@@ -1500,10 +1505,12 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                         si_z = (float)((double)li.z - fr_n.shz);
             const unsigned key = __float_as_uint(li.w);
             unsigned long long steps64 = 0ull;
-#define TDP_ENEAR_CASE(M)                                                                                       \
-    case M:                                                                                                     \
-        steps64 = elec_near32_lab<PBC, MUTUAL, M>(s_l, s_uf, s_na, s_o, lane, me, my_orig, key, si_x, si_y, si_z, \
-                                                  fr_n, r2d, acc, jacc, en);                                    \
+#define TDP_ENEAR_CASE(M)                                                                                        \
+    case M:                                                                                                      \
+        steps64 = masked ? elec_near32_lab<PBC, MUTUAL, M, true>(s_l, s_uf, s_na, s_o, lane, me, my_orig, key, si_x, \
+                                                                 si_y, si_z, fr_n, r2d, m0 | mh | mi, acc, jacc, en) \
+                         : elec_near32_lab<PBC, MUTUAL, M, false>(s_l, s_uf, s_na, s_o, lane, me, my_orig, key,     \
+                                                                  si_x, si_y, si_z, fr_n, r2d, 0ull, acc, jacc, en); \
         break;
             switch (fr_n.smask) {
                 TDP_ENEAR_CASE(0)
@@ -1525,8 +1532,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                 ElecSite other;
                 load_site(other, s_p[jd], s_q[jd], s_u[jd], s_ia[jd]);
                 ElecAcc jb1 = {0, 0, 0, 0, 0, 0};
-                elec_sym_step<MUTUAL, true, PERM>(me, my_orig, other, s_o[jd], ivalid && jd < cntJ, 1.0, 1.0, PBC, box,
-                                                  acc, jb1, en);
+                // the scales of the pair as in the FP64 path below (a pair with both scales 0 is skipped)
+                const bool z0 = (m0 >> jd) & 1ull, zh = (mh >> jd) & 1ull, zi = (mi >> jd) & 1ull;
+                elec_sym_step<MUTUAL, true, PERM>(me, my_orig, other, s_o[jd], ivalid && jd < cntJ && !(z0 && zi),
+                                                  z0 ? 0.0 : (zh ? 0.5 : 1.0), zi ? 0.0 : 1.0, PBC, box, acc, jb1,
+                                                  en);
                 s_jn[jd] += jb1.fx;
                 s_jn[kTile + jd] += jb1.fy;
                 s_jn[2 * kTile + jd] += jb1.fz;

@@ -324,36 +324,40 @@ __global__ __launch_bounds__(256) void k_chol_col(int j, const DenseSys* __restr
 }
 
 // ------------------------------------------------------------------ triangular solves
-// One workgroup per system solves L L^T y = rhs for the 4 right-hand sides (FP32 arithmetic on the
-// 16-bit factor), left-looking by blocks.  Row R*64 + r of L left of the diagonal block sits in the
-// lower blocks and row R*64 + r of L^T right of it in the upper (transposed) blocks, both
-// contiguous: thread (r, s) reads 16-byte pieces s, s + 16, ... of that row and accumulates all 4
-// right-hand sides; the 16 partial sums are added in a fixed order.  y stays in LDS.  rhs and y
-// are FP64 arrays of m x 4 values (unknown, then right-hand side) at d.voff.
+// One workgroup of 16 waves per system solves L L^T y = rhs for the 4 right-hand sides, left-looking
+// by blocks.  Step R forms b_R - (row block R of L or L^T, off the diagonal) y on MFMA
+// (v_mfma_f32_16x16x32 f16/bf16, FP32 accumulation): the 16-bit factor rows are the A operand,
+// read once from memory, and y the B operand, split into a high and a low 16-bit part per field
+// (columns 0-3 and 4-7; columns 8-15 are zero), so the products carry y to about 2^-22 (FP16) or
+// 2^-16 (BF16) relative, below the rounding of the factor itself.  Row R*64 + r of L left of the
+// diagonal block sits in the lower blocks and row R*64 + r of L^T right of it in the upper
+// (transposed) blocks, both contiguous.  The 32-column chunks of the row block are dealt to the
+// waves in turn; each wave covers all 64 rows (four 16-row tiles) of its chunks and its partial
+// sums are added in a fixed order, then the step applies the FP32 inverse of the diagonal block.
+// y stays in LDS (FP32, and its 16-bit parts).  Each field is scaled by a power of two that brings
+// its largest right-hand side entry into [1, 2) (exact; undone on output), so that the 16-bit parts
+// of the small corrections of later refinement steps stay within the FP16 exponent range.  rhs and
+// y are FP64 arrays of m x 4 values (unknown, then right-hand side) at d.voff.
 constexpr int kMaxDenseTiles = 16;   // up to 1,024 particles per system: m <= 3,072
-constexpr int kSolveSplit = 16;      // column pieces of a row sum (waves per system)
-constexpr int kSolveThreads = kSolveSplit * kB;
+constexpr int kMaxM = 192 * kMaxDenseTiles;
+constexpr int kSolveWaves = 16;
+constexpr int kSolveThreads = kSolveWaves * 64;
+constexpr int kSolveRound = 2;       // chunks per wave whose factor loads are in flight together
 
+typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// 16x16x32 MFMA with FP32 accumulation.  Lane l holds row l % 16 of A and column l % 16 of B, with
+// the same 8 values of k (group l / 16) in both; accumulator element v of lane l is row
+// 4 (l / 16) + v, column l % 16.
 template <bool BF16>
-__device__ __forceinline__ float4 row_dot(const unsigned short* __restrict__ row, int c0, int c1, int sp,
-                                          const float4* __restrict__ s_y) {
-    // sum over columns [c0, c1) (multiples of 8) of row[c] * y[c], pieces of 8 columns
-    float4 a = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (int c = c0 + 8 * sp; c < c1; c += 8 * kSolveSplit) {
-        const uint4 v = *reinterpret_cast<const uint4*>(row + c);
-        const unsigned w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int h = 0; h < 4; h++) {
-            const float l0 = from_lp<BF16>((unsigned short)(w[h] & 0xffffu));
-            const float l1 = from_lp<BF16>((unsigned short)(w[h] >> 16));
-            const float4 y0 = s_y[c + 2 * h], y1 = s_y[c + 2 * h + 1];
-            a.x += l0 * y0.x + l1 * y1.x;
-            a.y += l0 * y0.y + l1 * y1.y;
-            a.z += l0 * y0.z + l1 * y1.z;
-            a.w += l0 * y0.w + l1 * y1.w;
-        }
-    }
-    return a;
+__device__ __forceinline__ float4_t mfma16(uint4 a, uint4 b, float4_t c) {
+    if (BF16)
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b),
+                                                       c, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(half8_t, a), __builtin_bit_cast(half8_t, b), c, 0,
+                                                  0, 0);
 }
 
 template <bool BF16>
@@ -366,62 +370,158 @@ __global__ __launch_bounds__(kSolveThreads) void k_dense_solve(const DenseSys* _
     if (active && !active[blockIdx.x]) return;
     const int nb = 3 * d.nt;
     const int m = 192 * d.nt;
-    __shared__ float4 s_y[3 * kMaxDenseTiles * kB];   // 48 KB
-    __shared__ float4 s_p[kSolveSplit][kB];           // 16 KB
-    __shared__ float s_b[4][kB];
+    __shared__ float4 s_y[kMaxM];                               // 48 KB: y, then x, 4 fields
+    __shared__ __align__(16) unsigned short s_h[8][kMaxM];      // 48 KB: high parts (fields 0-3), low parts
+    __shared__ float s_part[kSolveWaves][8][kB + 1];            // 32.5 KB: per-wave partial products
     __shared__ float s_inv[kB][kB + 1];
-    const int t = threadIdx.x, r = t & 63, sp = t >> 6;
+    __shared__ float s_b[4][kB];
+    __shared__ float s_max[kSolveWaves][4];
+    const int t = threadIdx.x, w = t >> 6, l = t & 63;
+    const int n = l & 15, g = l >> 4;
+    float4 mx = make_float4(0.f, 0.f, 0.f, 0.f);
     for (int e = t; e < m; e += kSolveThreads) {
         const double* q = rhs + d.voff + 4 * (size_t)e;
-        s_y[e] = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+        const float4 v = make_float4((float)q[0], (float)q[1], (float)q[2], (float)q[3]);
+        s_y[e] = v;
+        mx = make_float4(fmaxf(mx.x, fabsf(v.x)), fmaxf(mx.y, fabsf(v.y)), fmaxf(mx.z, fabsf(v.z)),
+                         fmaxf(mx.w, fabsf(v.w)));
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        mx.x = fmaxf(mx.x, __shfl_xor(mx.x, o));
+        mx.y = fmaxf(mx.y, __shfl_xor(mx.y, o));
+        mx.z = fmaxf(mx.z, __shfl_xor(mx.z, o));
+        mx.w = fmaxf(mx.w, __shfl_xor(mx.w, o));
+    }
+    if (l == 0) {
+        s_max[w][0] = mx.x; s_max[w][1] = mx.y; s_max[w][2] = mx.z; s_max[w][3] = mx.w;
+    }
+    __syncthreads();
+    // per field: 2^-e with the largest |rhs| in [2^e, 2^(e+1)) (1 for a zero field)
+    float sc[4], isc[4];
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+        float v = 0.f;
+        for (int u = 0; u < kSolveWaves; u++) v = fmaxf(v, s_max[u][f]);
+        const int e = v > 0.f ? ilogbf(v) : 0;
+        sc[f] = ldexpf(1.f, -e);
+        isc[f] = ldexpf(1.f, e);
+    }
+    for (int e = t; e < m; e += kSolveThreads) {
+        const float4 v = s_y[e];
+        s_y[e] = make_float4(v.x * sc[0], v.y * sc[1], v.z * sc[2], v.w * sc[3]);
     }
     __syncthreads();
     constexpr int kInvPer = kB * kB / kSolveThreads;   // diagonal-inverse words per thread
-    for (int pass = 0; pass < 2; pass++) {
-        // pass 0: L y = rhs, blocks R = 0..nb-1; pass 1: L^T x = y, blocks R = nb-1..0
-        for (int step = 0; step < nb; step++) {
-            const int R = pass == 0 ? step : nb - 1 - step;
-            // the step's diagonal-block inverse is fetched first, so its latency overlaps the row sums
-            const float* inv = dinv + d.doff + (size_t)R * kB * kB;
-            float iv[kInvPer];
+    // Steps 0..nb-1: L y = rhs, blocks R = 0..nb-1; steps nb..2nb-1: L^T x = y, blocks R = nb-1..0.
+    // Step q reads the columns [c0, c0 + 32 nch) of row block R: left of the diagonal (forward) or
+    // right of it (backward).  The factor does not depend on y, so the loads of a step's first round
+    // of chunks and of its diagonal inverse are issued at the end of the step before, ahead of its
+    // barriers.  Lane (n, g) loads row 16 rt + n of the row block, columns 32 ch + 8 g .. + 7.
+    auto geom = [&](int q, int& R, int& c0, int& nch) {
+        R = q < nb ? q : 2 * nb - 1 - q;
+        c0 = q < nb ? 0 : (R + 1) * kB;
+        nch = (q < nb ? R * kB : m - c0) >> 5;
+    };
+    uint4 a[kSolveRound][4];
+    float iv[kInvPer];
+    auto issue = [&](int R, int c0, int nch, int ch0) {
+        const unsigned short* arow = mat + d.moff + (size_t)(R * kB + n) * m + c0 + 8 * g;
 #pragma unroll
-            for (int u = 0; u < kInvPer; u++) iv[u] = inv[t + kSolveThreads * u];
-            const unsigned short* row = mat + d.moff + (size_t)(R * kB + r) * m;
-            const float4 a = pass == 0 ? row_dot<BF16>(row, 0, R * kB, sp, s_y)
-                                       : row_dot<BF16>(row, (R + 1) * kB, m, sp, s_y);
-            s_p[sp][r] = a;
+        for (int u = 0; u < kSolveRound; u++) {
+            const int ch = ch0 + kSolveWaves * u;
+            if (ch < nch) {
 #pragma unroll
-            for (int u = 0; u < kInvPer; u++) {
-                const int e = t + kSolveThreads * u;
-                s_inv[e >> 6][e & 63] = iv[u];
+                for (int rt = 0; rt < 4; rt++)
+                    a[u][rt] = *reinterpret_cast<const uint4*>(arow + (size_t)(16 * rt) * m + 32 * ch);
             }
-            __syncthreads();
-            if (sp == 0) {
-                float4 b = s_y[R * kB + r];
-                for (int u = 0; u < kSolveSplit; u++) {
-                    b.x -= s_p[u][r].x; b.y -= s_p[u][r].y; b.z -= s_p[u][r].z; b.w -= s_p[u][r].w;
-                }
-                s_b[0][r] = b.x; s_b[1][r] = b.y; s_b[2][r] = b.z; s_b[3][r] = b.w;
-            }
-            __syncthreads();
-            // diagonal block: thread (r, q = sp < 4) applies Linv (pass 0) or Linv^T (pass 1) to field q
-            if (sp < 4) {
-                float o = 0.f;
-                if (pass == 0) {
-                    for (int k = 0; k <= r; k++) o += s_inv[r][k] * s_b[sp][k];
-                } else {
-                    for (int k = r; k < kB; k++) o += s_inv[k][r] * s_b[sp][k];
-                }
-                float* yr = reinterpret_cast<float*>(&s_y[R * kB + r]);
-                yr[sp] = o;
-            }
-            __syncthreads();
         }
+    };
+    auto issue_inv = [&](int R) {
+        const float* inv = dinv + d.doff + (size_t)R * kB * kB;
+#pragma unroll
+        for (int u = 0; u < kInvPer; u++) iv[u] = inv[t + kSolveThreads * u];
+    };
+    {
+        int R, c0, nch;
+        geom(0, R, c0, nch);
+        issue(R, c0, nch, w);
+        issue_inv(R);
+    }
+    for (int q = 0; q < 2 * nb; q++) {
+        const int pass = q < nb ? 0 : 1;
+        int R, c0, nch;
+        geom(q, R, c0, nch);
+        const unsigned short* brow = &s_h[n & 7][c0 + 8 * g];
+        float4_t acc[4];
+#pragma unroll
+        for (int rt = 0; rt < 4; rt++) acc[rt] = float4_t{0.f, 0.f, 0.f, 0.f};
+        for (int ch0 = w;;) {
+#pragma unroll
+            for (int u = 0; u < kSolveRound; u++) {
+                const int ch = ch0 + kSolveWaves * u;   // uniform over the wave
+                if (ch < nch) {
+                    uint4 bv = *reinterpret_cast<const uint4*>(brow + 32 * ch);
+                    if (n >= 8) bv = make_uint4(0u, 0u, 0u, 0u);
+#pragma unroll
+                    for (int rt = 0; rt < 4; rt++) acc[rt] = mfma16<BF16>(a[u][rt], bv, acc[rt]);
+                }
+            }
+            ch0 += kSolveWaves * kSolveRound;
+            if (ch0 >= nch) break;
+            issue(R, c0, nch, ch0);
+        }
+        if (n < 8) {
+#pragma unroll
+            for (int rt = 0; rt < 4; rt++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) s_part[w][n][16 * rt + 4 * g + v] = acc[rt][v];
+        }
+#pragma unroll
+        for (int u = 0; u < kInvPer; u++) {
+            const int e = t + kSolveThreads * u;
+            s_inv[e >> 6][e & 63] = iv[u];
+        }
+        if (q + 1 < 2 * nb) {
+            int R1, c01, nch1;
+            geom(q + 1, R1, c01, nch1);
+            issue(R1, c01, nch1, w);
+            issue_inv(R1);
+        }
+        __syncthreads();
+        if (t < 4 * kB) {
+            const int r = t & 63, f = t >> 6;
+            float sum = 0.f;
+            for (int u = 0; u < kSolveWaves; u++) sum += s_part[u][f][r] + s_part[u][f + 4][r];
+            s_b[f][r] = reinterpret_cast<const float*>(&s_y[R * kB + r])[f] - sum;
+        }
+        __syncthreads();
+        // diagonal block: thread (r, f) applies Linv (forward) or Linv^T (backward) to field f
+        if (t < 4 * kB) {
+            const int r = t & 63, f = t >> 6;
+            // the inverse is lower triangular with explicit zeros above the diagonal, so the sums run
+            // over all 64 columns (a fixed count the compiler unrolls, with the LDS reads batched)
+            float o = 0.f;
+            if (pass == 0) {
+#pragma unroll
+                for (int k = 0; k < kB; k++) o += s_inv[r][k] * s_b[f][k];
+            } else {
+#pragma unroll
+                for (int k = 0; k < kB; k++) o += s_inv[k][r] * s_b[f][k];
+            }
+            reinterpret_cast<float*>(&s_y[R * kB + r])[f] = o;
+            const unsigned short hi = to_lp<BF16>(o);
+            s_h[f][R * kB + r] = hi;
+            s_h[4 + f][R * kB + r] = to_lp<BF16>(o - from_lp<BF16>(hi));
+        }
+        __syncthreads();
     }
     for (int e = t; e < m; e += kSolveThreads) {
         const float4 v = s_y[e];
         double* q = y + d.voff + 4 * (size_t)e;
-        q[0] = v.x; q[1] = v.y; q[2] = v.z; q[3] = v.w;
+        q[0] = (double)v.x * isc[0];
+        q[1] = (double)v.y * isc[1];
+        q[2] = (double)v.z * isc[2];
+        q[3] = (double)v.w * isc[3];
     }
 }
 

@@ -218,6 +218,36 @@ def test_dense_responses_match_cg_and_mutual(method):
         assert np.max(np.abs(X[q] - Y[q])) <= MU_TOL, q
 
 
+
+@pytest.mark.parametrize("method", ["fp16", "bf16"])
+def test_dense_zero_field_and_size_limit(method):
+    """A system without charges has E0 = 0: its E0 response is exactly zero (the solve scales each
+    field by a power of two from its largest entry, and a zero field keeps scale 1), and its unit
+    field responses equal FP64 CG.  A system past 1,024 particles is refused with an error before
+    any work."""
+    _say("case dense zero field %s" % method)
+    members = _dense_members([6, 25], 900)
+    f0 = members[0][0]
+    for k in range(f0.getNumParticles()):
+        q, d, a, th, ax, z, x, y = f0.getParticleParameters(k)
+        f0.setParticleParameters(k, 0.0, d, a, th, ax, z, x, y)
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    X, st = b.responses(pos, method=method, tolerance=1e-10)
+    Y, _ = b.responses(pos, method="cg", tolerance=1e-12, max_steps=1000)
+    b.close()
+    _say("  dense stats %s" % st)
+    n0 = f0.getNumParticles()
+    assert st["factor_failed"] == 0 and st["max_relative_residual"] <= 1e-10
+    assert np.all(X[0][:n0] == 0.0)
+    for q in range(4):
+        assert np.max(np.abs(X[q] - Y[q])) <= MU_TOL, q
+    big = _dense_members([342], 950)   # 1,026 particles
+    bb = _lib.HipBatchHandle([big[0][0]], precision="double", solver="pcg")
+    with pytest.raises(_lib.TdpError, match="more than 1024 particles"):
+        bb.responses(big[0][1], method=method, tolerance=1e-10)
+    bb.close()
+
 def test_dense_responses_residual_by_oracle():
     """The dense responses satisfy (alpha^-1 - T) mu = b with T applied by the CPU oracle
     (rows of the induced field over all partners), for E0 and the three unit fields."""

@@ -36,9 +36,9 @@ def lab_form(qm, qo, m, o, d, am, ao, s, mutual, damped):
     dd1 = 1.0 - ev * (1.0 + au3)                # dthole_d1
     dd0 = dd1 - ev * 1.5 * au3 * au3            # dthole_d0
     kx = rinv / np.sqrt(rho2)
-    Xm = (m[0] * rho2 - dx * (dy * m[1] + dz * m[2])) * kx
-    Xo = (o[0] * rho2 - dx * (dy * o[1] + dz * o[2])) * kx
     X = np.array([rho2 * kx, -dx * dy * kx, -dx * dz * kx])
+    Xm = X @ m
+    Xo = X @ o
     qq = qm * qo
     xw = qm * Xo - qo * Xm
     t3 = rI3 * thc

@@ -1178,8 +1178,8 @@ __device__ __forceinline__ bool elec_far32(const double4* __restrict__ s_p, cons
 //            [+ 6 ((X.b) a + (X.a) b) / r^4 + X (6 a.b - 30 (X.a)(X.b)) / r^4 for Mutual],
 // and the force on B is (F_y, F_z, F_x): the reference rotates the force back with the index
 // pattern at :150-159, which permutes the components (Y(Y.v) + Z(Z.v) = v - X(X.v) removes Y and
-// Z).  X.v = (v_x rho^2 - d_x (d_y v_y + d_z v_z)) / (r rho), rho^2 = d_y^2 + d_z^2, has no
-// cancellation when d is nearly along x.  The lane's own site is m, the other o, s = +1 when the
+// Z).  X = (rho^2, -d_x d_y, -d_x d_z) / (r rho), rho^2 = d_y^2 + d_z^2, is formed from products
+// only, with no cancellation when d is nearly along x.  The lane's own site is m, the other o, s = +1 when the
 // lane's particle is A; d, X and (X.m)(X.o) do not depend on the orientation, w and the a/b terms
 // change sign with it.  Returns false, with nothing accumulated, when a straddling pair cannot be
 // decided in FP32 (SM >= 0), when some pair's raw y and z may be equal (keys match; the reference
@@ -1220,10 +1220,11 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
         const float rinv = rsqrt_t(r2);
         const float rI2 = rinv * rinv, rI3 = rI2 * rinv;
         const float kx = rinv * rsqrt_t(rho2);
-        const float Xm = (mx * rho2 - dx * (dy * my + dz * mz)) * kx;
-        const float Xo = (oq.x * rho2 - dx * (dy * oq.y + dz * oq.z)) * kx;
+        // X = (rho^2, -d_x d_y, -d_x d_z) / (r rho): products only, no cancellation; X.v as a dot product
         const float ex = -dx * kx;
         const float Xx = rho2 * kx, Xy = ex * dy, Xz = ex * dz;
+        const float Xm = Xx * mx + Xy * my + Xz * mz;
+        const float Xo = Xx * oq.x + Xy * oq.y + Xz * oq.z;
         const float QQ = Qm * oq.w;
         const float Xw = Qm * Xo - oq.w * Xm;
         float am, ao, inner;
@@ -1313,10 +1314,11 @@ __device__ __forceinline__ unsigned long long elec_near32_lab(const float4* __re
         const float thc = 1.f - ev;
         const float th0 = 1.f - ev * fmaf(1.5f, au3, 1.f);
         const float kx = rinv * rsqrt_t(rho2);
-        const float Xm = (mx * rho2 - dx * (dy * my + dz * mz)) * kx;
-        const float Xo = (oq.x * rho2 - dx * (dy * oq.y + dz * oq.z)) * kx;
+        // X = (rho^2, -d_x d_y, -d_x d_z) / (r rho): products only, no cancellation; X.v as a dot product
         const float ex = -dx * kx;
         const float Xx = rho2 * kx, Xy = ex * dy, Xz = ex * dz;
+        const float Xm = Xx * mx + Xy * my + Xz * mz;
+        const float Xo = Xx * oq.x + Xy * oq.y + Xz * oq.z;
         const float QQ = Qm * oq.w;
         const float Xw = Qm * Xo - oq.w * Xm;
         const float t3 = rI3 * thc;

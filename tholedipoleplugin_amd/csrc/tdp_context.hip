@@ -117,6 +117,7 @@ struct tdp_context {
     size_t dense_dwords = 0;
     double *d_db = nullptr, *d_dx = nullptr, *d_dres = nullptr, *d_dy = nullptr, *d_drel = nullptr;
     double4* d_dmu = nullptr;
+    double* d_rout = nullptr;   // responses in the caller's order, [4][n][3]
     int *d_dactive = nullptr, *d_dstatus = nullptr;
     size_t* d_tcol = nullptr;
     int4* d_seg = nullptr;
@@ -1134,6 +1135,20 @@ int dense_alloc(tdp_context* c) {
     return TDP_OK;
 }
 
+// Fields q = 0..nfield-1 (nfield <= 4) of double4 vectors in the internal order (src + q npad) to the
+// caller's order out[q][n][3]: one gather per field on the device, then one download.
+int download_fields(tdp_context* c, const double4* src, int nfield, double* out) {
+    hipStream_t s = c->stream;
+    const size_t n3 = (size_t)3 * c->n;
+    int rc;
+    if (!c->d_rout && (rc = dalloc(c, &c->d_rout, 4 * n3))) return rc;
+    for (int q = 0; q < nfield; q++)
+        launch_scatter_vec3(s, c->npad, c->d_perm, src + (size_t)q * c->npad, c->d_rout + q * n3);
+    TDP_HIP(c, hipMemcpyAsync(out, c->d_rout, nfield * n3 * sizeof(double), hipMemcpyDeviceToHost, s));
+    TDP_HIP(c, hipStreamSynchronize(s));
+    return TDP_OK;
+}
+
 // Induced dipoles of every system for four fields: E0 (the permanent field) and unit uniform
 // fields along x, y and z.  Low-precision Cholesky of A^ = I - D^1/2 T D^1/2 on MFMA, then FP64
 // iterative refinement with matrix-free FP64 residuals until every system's relative residual
@@ -1189,21 +1204,12 @@ int dense_responses(tdp_context* c, bool bf16, double tol, int max_steps, double
         TDP_HIP(c, hipMemcpyAsync(c->d_dactive, active.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
     }
     int status = 0;
-    std::vector<double4> mu(4 * (size_t)npad);
     TDP_HIP(c, hipMemcpyAsync(&status, c->d_dstatus, sizeof status, hipMemcpyDeviceToHost, s));
-    TDP_HIP(c, hipMemcpyAsync(mu.data(), c->d_dmu, mu.size() * sizeof(double4), hipMemcpyDeviceToHost, s));
-    TDP_HIP(c, hipStreamSynchronize(s));
-    if (out)
-        for (int q = 0; q < 4; q++)
-            for (int k = 0; k < npad; k++) {
-                const int o = c->perm[k];
-                if (o < 0) continue;
-                const double4 v = mu[(size_t)q * npad + k];
-                double* dst = out + (size_t)q * 3 * c->n + 3 * (size_t)o;
-                dst[0] = v.x;
-                dst[1] = v.y;
-                dst[2] = v.z;
-            }
+    if (out) {
+        if ((rc = download_fields(c, c->d_dmu, 4, out))) return rc;
+    } else {
+        TDP_HIP(c, hipStreamSynchronize(s));
+    }
     float t_asm = 0.f, t_fac = 0.f;
     hipEventElapsedTime(&t_asm, ev[0], ev[1]);
     hipEventElapsedTime(&t_fac, ev[1], ev[2]);
@@ -1237,7 +1243,7 @@ int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dens
     int total_iter = 0;
     double worst = 0.0;
     int rc = TDP_OK;
-    std::vector<double4> mu(npad);
+    if (out && !c->d_rout && (rc = dalloc(c, &c->d_rout, (size_t)12 * c->n))) return rc;
     for (int q = 0; q < 4 && rc == TDP_OK; q++) {
         if (q > 0) launch_uniform_field(s, npad, c->d_perm, q - 1, c->E0);
         else TDP_HIP(c, hipMemcpyAsync(c->E0, e0save.data(), npad * sizeof(double4), hipMemcpyHostToDevice, s));
@@ -1258,21 +1264,12 @@ int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dens
         launch_field_init(s, G, c->E0, c->d_alpha, c->E0, c->mu, nullptr);
         rc = solve_mutual(c, nullptr);
         if (rc) break;
-        TDP_HIP(c, hipMemcpyAsync(mu.data(), c->mu, npad * sizeof(double4), hipMemcpyDeviceToHost, s));
-        TDP_HIP(c, hipStreamSynchronize(s));
         for (int g = 0; g < ns; g++) {
             total_iter = std::max(total_iter, c->iterations[g]);
             worst = std::max(worst, c->epsilon[g] / std::max(c->seg_target[g] / tol, 1e-300));
         }
-        if (out)
-            for (int k = 0; k < npad; k++) {
-                const int o = c->perm[k];
-                if (o < 0) continue;
-                double* dst = out + (size_t)q * 3 * c->n + 3 * (size_t)o;
-                dst[0] = mu[k].x;
-                dst[1] = mu[k].y;
-                dst[2] = mu[k].z;
-            }
+        // field q in the caller's order; downloaded with the others after the last solve
+        if (out) launch_scatter_vec3(s, npad, c->d_perm, c->mu, c->d_rout + (size_t)q * 3 * c->n);
     }
     c->seg_maxit = maxit0;
     c->seg_target = target0;
@@ -1280,6 +1277,8 @@ int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dens
     c->precision = prec0;
     if (rc) return rc;
     TDP_HIP(c, hipMemcpyAsync(c->E0, e0save.data(), npad * sizeof(double4), hipMemcpyHostToDevice, s));
+    if (out) TDP_HIP(c, hipMemcpyAsync(out, c->d_rout, (size_t)12 * c->n * sizeof(double), hipMemcpyDeviceToHost, s));
+    TDP_HIP(c, hipStreamSynchronize(s));
     if (st) {
         st->steps = total_iter;
         st->max_relative_residual = worst;

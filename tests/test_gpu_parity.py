@@ -270,6 +270,69 @@ def test_ragged_sizes(n, periodic):
         assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL
 
 
+
+def _grouped_system(ngroup, periodic, seed):
+    """Groups of a center and five satellites 0.1-0.2 nm from it, the centers at least 0.45 nm apart
+    in a 2.2 nm box.  The center's 1-2 list holds its satellites, so by list position the pairs take
+    mScale 0, 0, 0, 0.5 and 1; each satellite's 1-2 list holds the center and its 1-3 list the other
+    satellites, so the satellite pairs take iScale 0 (first in the list) or 1."""
+    from tholedipoleplugin_amd.context import System
+    rng = np.random.default_rng(seed)
+    L = 2.2
+    centers = []
+    while len(centers) < ngroup:
+        x = rng.uniform(0.0, L, 3)
+        if all(np.linalg.norm((x - y) - L * np.round((x - y) / L)) > 0.45 for y in centers):
+            centers.append(x)
+    pos = []
+    for c in centers:
+        sats = []
+        while len(sats) < 5:
+            u = rng.normal(size=3)
+            x = c + u / np.linalg.norm(u) * rng.uniform(0.1, 0.2)
+            if all(np.linalg.norm(x - y) > 0.1 for y in sats):
+                sats.append(x)
+        pos += [c] + sats
+    system = System()
+    f = TholeDipoleForce()
+    f.setNonbondedMethod(T.PME if periodic else T.NoCutoff)
+    f.setCutoffDistance(0.7)
+    f.setPolarizationType(T.Mutual)
+    for i in range(6 * ngroup):
+        system.addParticle(1.0)
+        f.addParticle(float(rng.uniform(-0.8, 0.8)), [0.0, 0.0, 0.0], float(rng.uniform(0.3e-3, 1.0e-3)),
+                      float(rng.uniform(0.3, 0.45)), T.NoAxisType, -1, -1, -1)
+    for g in range(ngroup):
+        c = 6 * g
+        sat = [c + 1 + k for k in range(5)]
+        f.setCovalentMap(c, T.Covalent12, sat)
+        for k, sk in enumerate(sat):
+            f.setCovalentMap(sk, T.Covalent12, [c])
+            f.setCovalentMap(sk, T.Covalent13, [x for x in sat[k + 1:] + sat[:k]])
+    box = np.diag([L, L, L]) if periodic else None
+    if periodic:
+        system.setDefaultPeriodicBoxVectors(*box)
+    system.addForce(f)
+    return ref_systems.RefSystem("grouped%d" % ngroup, system, f, np.array(pos).reshape(-1, 3), box)
+
+
+@pytest.mark.parametrize("periodic", [False, True])
+def test_scaled_pairs_both_precisions(periodic):
+    """Excluded and half-scaled pairs (mScale 0 and 0.5, iScale 0) inside near tile pairs: the
+    mixed paths defer every step with such a pair to FP64, which applies the scales; both
+    precisions against the oracle at the bar."""
+    _say("case scaled pairs periodic=%d" % periodic)
+    rs = tighten(_grouped_system(50, periodic, 61 + periodic), eps=1e-12)
+    assert tuple(oracle.pair_scales(rs.force, 0, 4)) == (0.5, 1.0)   # center, fourth satellite
+    assert tuple(oracle.pair_scales(rs.force, 1, 2)) == (1.0, 0.0)   # satellites, first in the 1-3 list
+    ref = oracle.force_energy(rs.force, rs.positions, rs.box, target_eps=1e-13, max_iter=5000)
+    for precision in ("double", "mixed"):
+        ctx, st, k = gpu_eval(rs, precision=precision, solver="pcg")
+        e = st.getPotentialEnergy()
+        assert abs(e - ref["energy"]) <= E_TOL * abs(ref["energy"]), (precision, e, ref["energy"])
+        assert rel_rms(st.getForces(), ref["forces"]) <= F_TOL, precision
+        assert np.max(np.abs(rs.force.getInducedDipoles(ctx) - ref["induced"])) <= MU_TOL, precision
+
 @pytest.mark.parametrize("solver", ["pcg", "jacobi"])
 def test_warm_start_mutual(solver):
     """A warm-started Mutual solve (previous evaluation's dipoles as the initial guess) reaches the

@@ -685,13 +685,14 @@ __device__ __forceinline__ void near_matvec_mixed64(unsigned long long steps64, 
 }
 
 // ------------------------------------------------------------------ T * mu, mixed precision (a7)
-// Tile pairs inside the Thole range, and the diagonal and exclusion tiles, are evaluated in FP64
-// as in k_sym_matvec.  Tile pairs beyond it, most of the pairs of a large system, carry only the
-// undamped 1/r^3 and 3/r^5 terms: their separations and minimum images are formed in FP64 exactly
-// as in k_sym_matvec (so both precisions pick the same image, even for a pair exactly half a box
-// apart), and the tensor terms are evaluated in FP32 with FP32 dipoles.  The near pairs carry the
-// largest terms, so keeping them in FP64 holds the operator's error to that of the FP32 far field;
-// all partial sums are FP64.
+// Tile pairs beyond the Thole range, most of the pairs of a large system, carry only the undamped
+// 1/r^3 and 3/r^5 terms and run in FP32 with FP32 dipoles, on the tile pair's FP32 frame (which
+// picks the image of the FP64 rule for every pair, or sends the tile pair to FP64 separations).
+// On the other off-diagonal tile pairs, a step whose pairs all lie beyond a u^3 >= 4 also runs in
+// FP32, with FP32 Thole factors; the steps with a closer pair or an excluded or scaled pair are
+// evaluated afterwards in FP64 (near_matvec_mixed64), as are the diagonal tiles.  The closest pairs
+// carry the largest terms, so keeping them in FP64 holds the operator's error near that of the FP32
+// far field.  Tile and item partials are stored in FP32, the reductions run in FP64.
 template <bool PBC>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_sym_matvec_mixed(TileMap tm, int nblk, const int4* __restrict__ work,
                                                          const double4* __restrict__ posa,

@@ -566,7 +566,7 @@ __global__ __launch_bounds__(64) void k_tile_pairs(TileMap tm, const int4* __res
     }
 }
 
-// Near tile pair of T * mu in mixed precision (unmasked, off-diagonal tiles on frame SM, full or partial).
+// Near tile pair of T * mu in mixed precision (off-diagonal tiles on frame SM, full or partial).
 // Pass 1 (near_matvec_mixed32) runs the 64 steps in FP32 like a far tile pair, with the Thole
 // factors in FP32 on steps where some pair lies within r2min (every a is then nonzero: r2min is
 // infinite otherwise), except steps in which some pair lies within r2d (or a straddling pair
@@ -1261,8 +1261,8 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
     return true;
 }
 
-// Near tile pair of the energy pass without permanent dipoles in mixed precision (unmasked,
-// off-diagonal tiles on frame SM, full or partial), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
+// Near tile pair of the energy pass without permanent dipoles in mixed precision (off-diagonal
+// tiles on frame SM, full or partial), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
 // a u^3 >= 4) in FP32 by the closed form of elec_far32_lab with the Thole factors of
 // calculateElectrostaticPairIxn (:205-220): with e = exp(-a u^3), thole_c = 1 - e, thole_d0 =
 // 1 - e (1 + 1.5 a u^3), dthole_d0 = 1 - e (1 + a u^3 + 1.5 (a u^3)^2), dthole_d1 = 1 - e (1 + a u^3),

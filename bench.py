@@ -38,7 +38,7 @@ DEBYE = 0.020819434  # e*nm
 # WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 98.5 MB
 # read + 166.8 MB written (the FP32 tile partial slab, 128 MB, and the FP32 work-item partials).
 # Counters cannot be read inside this process.
-MATVEC_HBM_BYTES = 265.3e6
+MATVEC_HBM_BYTES = 265.6e6
 
 
 def parse():

@@ -35,7 +35,7 @@ PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
 # HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 98.5 MB
+# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 98.8 MB
 # read + 166.8 MB written (the FP32 tile partial slab, 128 MB, and the FP32 work-item partials).
 # Counters cannot be read inside this process.
 MATVEC_HBM_BYTES = 265.6e6

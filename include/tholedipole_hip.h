@@ -192,7 +192,8 @@ int tdp_batch_get_solver_stats(const tdp_context* ctx, int32_t system, tdp_solve
  * method 1: the same with a BF16 factor;
  * method 2: Jacobi-preconditioned FP64 CG per field, stopped at RMS(alpha r) <= tolerance RMS(alpha b).
  * Systems of up to 1,024 particles (methods 0 and 1).  max_steps bounds refinement steps or CG
- * iterations.  out: host [4][3N] (field, particle, component), caller's order. */
+ * iterations.  out: host [4][3N] (field, particle, component), caller's order, or NULL for the
+ * statistics only. */
 typedef struct {
     int32_t steps;                  /* refinement steps (0, 1) or the largest CG iteration count (2) */
     double max_relative_residual;   /* largest final relative residual over systems and fields */

@@ -36,6 +36,8 @@ extern "C" {
 #define TDP_ERR_BOX_TOO_SMALL 4      /* box < 2*cutoff (ReferenceTholeDipoleKernels.cpp:170-173) */
 #define TDP_ERR_PARTICLE_COUNT 5     /* copyParameters with a different particle count */
 #define TDP_ERR_NO_DEVICE 6          /* no usable gfx950 device */
+#define TDP_ERR_NOT_CONVERGED 7      /* a solve ended above its tolerance (responses; Mutual solves when
+                                        TDP_FLAG_REQUIRE_CONVERGENCE is set); results are still written */
 
 /* nonbonded_method (TholeDipoleForce.h:58-70) */
 #define TDP_NO_CUTOFF 0
@@ -85,10 +87,17 @@ typedef struct tdp_force_desc {
     double default_box[9];            /* System default periodic box vectors (row-major a,b,c) */
 } tdp_force_desc;
 
+/* option flags (tdp_options.flags) */
+#define TDP_FLAG_REQUIRE_CONVERGENCE 1   /* tdp_execute, tdp_execute_device and tdp_get_dipoles return
+                                            TDP_ERR_NOT_CONVERGED when a Mutual solve stops at its iteration
+                                            limit above its target; the reference records this silently
+                                            (ReferenceTholeDipoleForce.cpp:849-850) */
+
 typedef struct tdp_options {
     int32_t precision;                /* TDP_PRECISION_* */
     int32_t solver;                   /* TDP_SOLVER_* */
     int32_t device;                   /* HIP device ordinal */
+    int32_t flags;                    /* TDP_FLAG_* bits, 0 for the reference's behaviour */
 } tdp_options;
 
 typedef struct tdp_solver_stats {
@@ -113,14 +122,17 @@ int tdp_copy_parameters(tdp_context* ctx, const tdp_force_desc* desc);
 
 /* Replaces execute (TholeDipoleKernels.h:68, ReferenceTholeDipoleKernels.cpp:201-215).
  * positions: host [3n]; box: host [9] (ignored for NoCutoff); forces: host [3n], accumulated
- * (may be NULL when include_forces == 0); energy: host scalar (may be NULL). */
+ * (may be NULL when include_forces == 0); energy: host scalar (may be NULL).  With include_forces
+ * == 0 (or forces NULL) the torque mapping is skipped and forces are left untouched; with
+ * include_energy == 0 (or energy NULL) the energy reduction is skipped; with neither, only the
+ * induced dipoles are solved. */
 int tdp_execute(tdp_context* ctx, const double* positions, const double* box, int32_t include_forces,
                 int32_t include_energy, double* forces, double* energy);
 
 /* Same evaluation with device-resident buffers on the caller's HIP stream (a hipStream_t
  * passed as void*, NULL = the null stream): d_positions [3n] doubles, d_forces [3n] doubles
- * accumulated on the device, d_energy one double written on the device (may be NULL; a batch
- * context writes one energy per system). */
+ * accumulated on the device (may be NULL: no forces), d_energy one double written on the device
+ * (may be NULL: no energy; a batch context writes one energy per system). */
 int tdp_execute_device(tdp_context* ctx, const double* d_positions, const double* box, double* d_forces,
                        double* d_energy, void* stream);
 

@@ -58,6 +58,7 @@ def lib():
         L.tdpo_pass_slice.restype = C.c_double
         L.tdpo_pass_slice.argtypes = [C.POINTER(_Sys), _dp, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.tdpo_rows.argtypes = [C.POINTER(_Sys), _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp]
+        L.tdpo_evaluate_at.argtypes = [C.POINTER(_Sys), _dp, _dp, _dp, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -213,6 +214,20 @@ def rows(force, positions, mu, row_idx, box=None, threads=8):
     _check(L.tdpo_rows(C.byref(os_.s), _p(pos), _p(m), _p(r, _ip), k, _p(e0), _p(ei), _p(f), _p(t), _p(en)))
     L.tdpo_set_threads(1)
     return dict(e0=e0.reshape(k, 3), eind=ei.reshape(k, 3), force=f.reshape(k, 3), torque=t.reshape(k, 3), energy=en)
+
+
+def evaluate_at(force, positions, mu, box=None, threads=8):
+    """The reference's pair passes at given induced dipoles: dict(e0, eind, forces, energy)."""
+    L = lib()
+    L.tdpo_set_threads(int(threads))
+    os_ = OracleSystem(force, box)
+    n = os_.n
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    m = np.ascontiguousarray(mu, np.float64).reshape(-1)
+    e0 = np.zeros(3 * n); ei = np.zeros(3 * n); f = np.zeros(3 * n); e = C.c_double(0.0)
+    _check(L.tdpo_evaluate_at(C.byref(os_.s), _p(pos), _p(m), _p(e0), _p(ei), _p(f), C.byref(e)))
+    L.tdpo_set_threads(1)
+    return dict(e0=e0.reshape(n, 3), eind=ei.reshape(n, 3), forces=f.reshape(n, 3), energy=e.value)
 
 
 def time_pass_slices(force, positions, mu, box, which, slices, threads=1):

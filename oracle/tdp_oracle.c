@@ -822,6 +822,35 @@ int tdpo_force_energy(const tdpo_system* s, const double* pos, double* forces, t
     return 0;
 }
 
+/* The three pair passes of calculateForceAndEnergy (ReferenceTholeDipoleForce.cpp:100-127) at
+ * GIVEN induced dipoles mu (3n) instead of solved ones: fixed field (:715-733), T*mu (:782-818),
+ * calculateElectrostatic (:368-393) and mapTorqueToForce (:1092-1115).  Outputs (may be NULL):
+ * e0 and eind (3n, overwritten), forces (3n, accumulated), energy. */
+int tdpo_evaluate_at(const tdpo_system* s, const double* pos, const double* mu, double* e0, double* eind,
+                     double* forces, double* energy) {
+    int n = s->n;
+    int rc = check_cov_lists(s);
+    if (rc) return rc;
+    pdata* p = (pdata*)malloc(sizeof(pdata) * (size_t)(n > 0 ? n : 1));
+    rc = load_and_rotate(s, pos, p);
+    if (rc) { free(p); return rc; }
+    if (e0) fixed_field(s, p, e0);
+    if (eind) induced_field(s, p, mu, eind);
+    if (forces || energy) {
+        double* tq = (double*)calloc(3 * (size_t)(n > 0 ? n : 1), sizeof(double));
+        double* f = (double*)calloc(3 * (size_t)(n > 0 ? n : 1), sizeof(double));
+        double e = electrostatic(s, p, mu, tq, f);
+        map_torques(s, p, tq, f);
+        if (forces)
+            for (int k = 0; k < 3 * n; k++) forces[k] += f[k];
+        if (energy) *energy = e;
+        free(tq);
+        free(f);
+    }
+    free(p);
+    return 0;
+}
+
 int tdpo_induced_dipoles(const tdpo_system* s, const double* pos, double* induced, double* labdip,
                          double* fixed_field_out, tdpo_stats* st) {
     int n = s->n;

@@ -70,6 +70,12 @@ int tdpo_force_energy(const tdpo_system* s, const double* pos, double* forces, t
 int tdpo_induced_dipoles(const tdpo_system* s, const double* pos, double* induced, double* labdip,
                          double* fixed_field, tdpo_stats* st);
 
+/* The reference's three pair passes at GIVEN induced dipoles mu (3n): fixed field, T*mu, and the
+ * electrostatic energy/forces with torques mapped (ReferenceTholeDipoleForce.cpp:100-127 without the
+ * solve).  e0, eind (3n) overwritten; forces (3n) accumulated; energy; any may be NULL. */
+int tdpo_evaluate_at(const tdpo_system* s, const double* pos, const double* mu, double* e0, double* eind,
+                     double* forces, double* energy);
+
 /* calculateElectrostaticPotential (ReferenceTholeDipoleForce.cpp:1315-1372). */
 int tdpo_potential(const tdpo_system* s, const double* pos, const double* grid, int ngrid, double* out);
 

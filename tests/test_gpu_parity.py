@@ -6,6 +6,8 @@ to agree far tighter; the assertions use the bar.  Solves compared here run to a
 (1e-12 e*nm) unless a test checks the reference's own stopping rule, so that the comparison is
 not dominated by solver truncation.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -423,14 +425,14 @@ def test_iteration_graph_replays(solver, monkeypatch):
 
 
 @pytest.mark.parametrize("precision", ["double", "mixed"])
-def test_water12288_rows_full_size(precision):
+def test_water12288_every_site_full_size(precision):
     """36,864 sites (12,288 waters, 4.95 nm box, reference PME semantics), PCG solved to an RMS
     dipole update of 1e-9 e*nm (well below the 1e-6 D bar, so that solver truncation does not
-    hide the operator's error).  Checks at full size: the solve converged; on sampled rows the
-    fixed point mu_i = alpha_i (E0_i + (T mu)_i), evaluated by the oracle in FP64 over all 36,863
-    partners with the GPU's dipoles, holds to 1e-6 D; the GPU forces on those rows match the
-    oracle's rows to the force tolerance."""
-    _say("case water12288 %s" % precision)
+    hide the operator's error).  The oracle evaluates the reference's three pair passes in FP64
+    over all 6.8e8 pairs with the GPU's induced dipoles (tdpo_evaluate_at, on the host's threads);
+    at EVERY site the fixed point mu_i = alpha_i (E0_i + (T mu)_i) holds to 1e-6 D, the forces match
+    to the force tolerance, and the total energy matches to the energy tolerance."""
+    _say("case water12288 every site %s" % precision)
     s = systems.config1()
     s.force.setMutualInducedTargetEpsilon(1e-9)
     from tholedipoleplugin_amd.context import System
@@ -448,13 +450,18 @@ def test_water12288_rows_full_size(precision):
     _say("  stats %s" % stats)
     assert stats["converged"] == 1
     mu = s.force.getInducedDipoles(ctx)
-    rows = np.random.default_rng(7).choice(s.n, 48, replace=False)
-    ref = oracle.rows(s.force, s.positions, mu, rows, s.box, threads=8)
-    alpha = s.force.packed_arrays()["polarity"][rows][:, None]
-    resid = mu[rows] - alpha * (ref["e0"] + ref["eind"])
-    _say("  max |mu - alpha (E0 + T mu)| on sampled rows: %.3e e*nm" % np.max(np.abs(resid)))
+    threads = max(1, min(16, os.cpu_count() or 1))
+    _say("  oracle passes over all pairs on %d threads" % threads)
+    ref = oracle.evaluate_at(s.force, s.positions, mu, s.box, threads=threads)
+    alpha = s.force.packed_arrays()["polarity"][:, None]
+    resid = mu - alpha * (ref["e0"] + ref["eind"])
+    de = abs(st.getPotentialEnergy() - ref["energy"]) / abs(ref["energy"])
+    df = rel_rms(st.getForces(), ref["forces"])
+    _say("  every site: max |mu - alpha (E0 + T mu)| %.3e e*nm, force rel RMS %.3e, energy rel %.3e"
+         % (np.max(np.abs(resid)), df, de))
     assert np.max(np.abs(resid)) <= MU_TOL
-    assert rel_rms(st.getForces()[rows], ref["force"]) <= F_TOL
+    assert df <= F_TOL
+    assert de <= E_TOL
 
 
 def test_saline96000_mixed_matches_double_and_rows():

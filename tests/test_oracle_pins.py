@@ -466,3 +466,174 @@ def test_reference_zonly_finite_difference_check_does_not_hold():
     e2 = oracle.force_energy(rs.force, rs.positions - F * step)["energy"]
     e3 = oracle.force_energy(rs.force, rs.positions + F * step)["energy"]
     assert abs(e2 - (e3 + norm * delta)) > 1e-3 * max(abs(e2), 1.0)
+
+
+# Closed-form lab frames (applyRotationMatrixToParticle, ReferenceTholeDipoleForce.cpp:477-546).
+# Each case places the axis atoms along hand-chosen directions (at unequal distances, so the
+# normalisations matter) and states the frame rows X, Y, Z that the definitions give, worked out by
+# hand; the lab dipole is d_x X + d_y Y + d_z Z.
+_S2, _S3, _S6 = math.sqrt(2.0), math.sqrt(3.0), math.sqrt(6.0)
+_FRAME_CASES = {
+    # Z = (z^ + x^)/|..|; X = x^ orthogonalised to Z; Y = Z x X
+    "bisector": (T.Bisector, [(0, 0, 1), (1, 0, 0), None],
+                 [(1 / _S2, 0, -1 / _S2), (0, 1, 0), (1 / _S2, 0, 1 / _S2)]),
+    # X = (x^ + y^)/|..| orthogonalised to Z = z^
+    "zbisect": (T.ZBisect, [(0, 0, 1), (1, 0, 0), (0, 1, 0)],
+                [(1 / _S2, 1 / _S2, 0), (-1 / _S2, 1 / _S2, 0), (0, 0, 1)]),
+    # X atom along (1,0,1)/sqrt2: x^ + y^ = (0.5, 1/sqrt2, 0.5)*sqrt2/sqrt2 ... orthogonalised to z^
+    "zbisect_tilted": (T.ZBisect, [(0, 0, 1), (1 / _S2, 0, 1 / _S2), (0, 1, 0)],
+                       [(1 / _S3, _S2 / _S3, 0), (-_S2 / _S3, 1 / _S3, 0), (0, 0, 1)]),
+    # Z = (z^ + x^ + y^)/sqrt3; X = x^ - Z/sqrt3 = (2,-1,-1)/3 normalised
+    "threefold": (T.ThreeFold, [(0, 0, 1), (1, 0, 0), (0, 1, 0)],
+                  [(2 / _S6, -1 / _S6, -1 / _S6), (0, 1 / _S2, -1 / _S2), (1 / _S3, 1 / _S3, 1 / _S3)]),
+    # |z_x| = 0.6 < 0.866: X from e_x
+    "zonly_ex": (T.ZOnly, [(0.6, 0, 0.8), None, None], [(0.8, 0, -0.6), (0, 1, 0), (0.6, 0, 0.8)]),
+    # |z_x| = 0.9 >= 0.866: X from e_y
+    "zonly_ey": (T.ZOnly, [(0.9, 0, math.sqrt(1 - 0.81)), None, None],
+                 [(0, 1, 0), (-math.sqrt(1 - 0.81), 0, 0.9), (0.9, 0, math.sqrt(1 - 0.81))]),
+}
+
+
+def _frame_system(axis, dirs, dmol, origin, dists=(0.1, 0.137, 0.081)):
+    f = T()
+    zi = 1
+    xi = 2 if axis != T.ZOnly else 2    # ZOnly still needs a valid X index (mapTorqueToForce reads it)
+    yi = 3 if dirs[2] is not None else -1
+    f.addParticle(0.0, list(dmol), 0.001, 0.39, axis, zi, xi, yi)
+    for _ in range(3):
+        f.addParticle(0.0, [0, 0, 0], 0.001, 0.39)
+    pos = np.zeros((4, 3))
+    pos[0] = origin
+    for k in range(3):
+        d = dirs[k] if dirs[k] is not None else (0.3, -0.5, 0.2)   # unused atom: generic spot
+        pos[k + 1] = origin + dists[k] * np.asarray(d, float) / np.linalg.norm(d)
+    return f, pos
+
+
+@pytest.mark.parametrize("case", sorted(_FRAME_CASES))
+def test_lab_frame_closed_forms(case):
+    axis, dirs, rows = _FRAME_CASES[case]
+    dmol = np.array([0.011, -0.023, 0.037])
+    f, pos = _frame_system(axis, dirs, dmol, np.array([0.3, -0.2, 0.5]))
+    R = np.asarray(rows, float)
+    assert np.allclose(R @ R.T, np.eye(3), atol=1e-15)      # the hand-derived rows form a frame
+    assert np.linalg.det(R) > 0.0
+    expect = dmol @ R
+    lab = oracle.lab_dipoles(f, pos)[0]
+    assert np.allclose(lab, expect, atol=1e-15, rtol=0), (lab, expect)
+
+
+@pytest.mark.parametrize("axis", [T.Bisector, T.ZBisect, T.ThreeFold, T.ZThenX])
+def test_lab_frame_rotates_with_the_molecule(axis):
+    """A rigid rotation of the whole geometry rotates the lab dipole with it (frames built from
+    atom positions only; ZOnly is excluded because its X comes from the lab axes)."""
+    rng = np.random.default_rng(7)
+    dmol = np.array([0.019, 0.007, -0.031])
+    dirs = [(0.2, 0.1, 1.0), (1.0, -0.3, 0.2), (-0.1, 1.0, 0.4)]
+    f, pos = _frame_system(axis, dirs, dmol, np.zeros(3))
+    a = oracle.lab_dipoles(f, pos)[0]
+    q, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    if np.linalg.det(q) < 0:
+        q[:, 0] *= -1
+    b = oracle.lab_dipoles(f, pos @ q.T)[0]
+    assert np.allclose(b, q @ a, atol=1e-15)
+
+
+# Torque -> force (mapTorqueToForceForParticle, ReferenceTholeDipoleForce.cpp:915-1090).  With a
+# uniform external field F the energy of the frame-rotated dipole is U(r) = -d_lab(r).F and its
+# torque is d_lab x F; the mapped forces must be -dU/dr of every atom of the frame, by central
+# differences of the lab dipole.  ZOnly builds its X axis from the lab axes, so only a dipole along
+# its Z axis rotates with the atoms; that case uses one.
+_FD_GEOMETRY = [(0.2, 0.1, 1.0), (1.0, -0.3, 0.2), (-0.1, 1.0, 0.4)]
+
+
+@pytest.mark.parametrize("axis", [T.ZThenX, T.Bisector, T.ZBisect, T.ZOnly])
+def test_torque_mapping_is_minus_gradient(axis):
+    dmol = np.array([0.0, 0.0, 0.029]) if axis == T.ZOnly else np.array([0.017, -0.011, 0.029])
+    f, pos = _frame_system(axis, _FD_GEOMETRY, dmol, np.array([0.05, 0.02, -0.03]))
+    field = np.array([0.7, -1.3, 0.4])
+    d = oracle.lab_dipoles(f, pos)[0]
+    tq = np.zeros((4, 3))
+    tq[0] = np.cross(d, field)
+    F = oracle.map_torques(f, pos, tq)
+    h = 1e-6
+    used = [0, 1, 2] + ([3] if axis in (T.ZBisect, T.ThreeFold) else [])
+    if axis == T.ZOnly:
+        used = [0, 1]
+    grad = np.zeros((4, 3))
+    for k in used:
+        for c in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[k, c] += h
+            p2[k, c] -= h
+            u1 = -oracle.lab_dipoles(f, p1)[0] @ field
+            u2 = -oracle.lab_dipoles(f, p2)[0] @ field
+            grad[k, c] = (u1 - u2) / (2 * h)
+    scale = np.max(np.abs(grad))
+    assert scale > 1e-3
+    assert np.allclose(F, -grad, atol=1e-7 * scale, rtol=0), (F, -grad)
+
+
+def _c3v_threefold(theta_deg):
+    th = math.radians(theta_deg)
+    dirs = [(math.sin(th) * math.cos(a), math.sin(th) * math.sin(a), math.cos(th))
+            for a in (0.0, 2 * math.pi / 3, 4 * math.pi / 3)]
+    return _frame_system(T.ThreeFold, dirs, np.zeros(3), np.array([0.02, -0.01, 0.03]), dists=(0.1, 0.1, 0.1))
+
+
+@pytest.mark.parametrize("theta", [70.0, 109.47])
+def test_threefold_mapping_reproduces_perpendicular_torque(theta):
+    """ThreeFold (:1053-1080) is Tinker's symmetric average over the three axis atoms, which is
+    not the gradient of the frame energy at a general geometry (the pair of checks below fails
+    there, for the reference's arithmetic as restated).  At a C3v geometry (three axis atoms at one
+    polar angle, 120 degrees apart around z, equal distances) each torque perpendicular to the
+    symmetry axis is reproduced exactly by the mapped forces, with zero net force; a swapped
+    projection or a wrong factor in any of du, dv, dw breaks the reproduction."""
+    f, pos = _c3v_threefold(theta)
+    rng = np.random.default_rng(11)
+    for _ in range(4):
+        t = rng.normal(size=3)
+        t[2] = 0.0
+        tq = np.zeros((4, 3))
+        tq[0] = t
+        F = oracle.map_torques(f, pos, tq)
+        assert np.allclose(F.sum(0), 0.0, atol=1e-14)
+        rep = np.cross(pos - pos[0], F).sum(0)
+        assert np.allclose(rep, t, atol=1e-13 * np.abs(t).max()), (rep, t)
+
+
+def test_threefold_mapping_is_not_the_gradient_at_general_geometry():
+    """Records the reference's behaviour: at a general geometry the ThreeFold forces are not
+    -dU/dr of the frame energy U = -d_lab.F (they differ by O(1) of the gradient)."""
+    dmol = np.array([0.017, -0.011, 0.029])
+    f, pos = _frame_system(T.ThreeFold, _FD_GEOMETRY, dmol, np.array([0.05, 0.02, -0.03]))
+    field = np.array([0.7, -1.3, 0.4])
+    d = oracle.lab_dipoles(f, pos)[0]
+    tq = np.zeros((4, 3))
+    tq[0] = np.cross(d, field)
+    F = oracle.map_torques(f, pos, tq)
+    h = 1e-6
+    grad = np.zeros((4, 3))
+    for k in range(4):
+        for c in range(3):
+            p1, p2 = pos.copy(), pos.copy()
+            p1[k, c] += h
+            p2[k, c] -= h
+            grad[k, c] = (-(oracle.lab_dipoles(f, p1)[0] @ field) + oracle.lab_dipoles(f, p2)[0] @ field) / (2 * h)
+    assert np.abs(F + grad).max() > 0.1 * np.abs(grad).max()
+    assert np.allclose(F.sum(0), 0.0, atol=1e-14)
+
+
+def test_evaluate_at_matches_force_energy():
+    """tdpo_evaluate_at (the three pair passes at given dipoles) at the oracle's own solved dipoles
+    reproduces tdpo_force_energy's energy, forces and fixed field, and mu = alpha (E0 + T mu)."""
+    rs = ref_systems.water4(T.Mutual)
+    rs.force.setMutualInducedTargetEpsilon(1e-14)
+    rs.force.setMutualInducedMaxIterations(2000)
+    full = oracle.force_energy(rs.force, rs.positions, rs.box)
+    at = oracle.evaluate_at(rs.force, rs.positions, full["induced"], rs.box, threads=2)
+    assert abs(at["energy"] - full["energy"]) <= 1e-13 * abs(full["energy"])
+    assert np.allclose(at["forces"], full["forces"], rtol=0, atol=1e-12 * np.abs(full["forces"]).max())
+    assert np.allclose(at["e0"], full["fixed_field"], rtol=0, atol=1e-12 * np.abs(full["fixed_field"]).max())
+    alpha = rs.force.packed_arrays()["polarity"][:, None]
+    assert np.max(np.abs(full["induced"] - alpha * (at["e0"] + at["eind"]))) <= 1e-12

@@ -18,6 +18,9 @@ TDP_ERR_NOT_PME = 3
 TDP_ERR_BOX_TOO_SMALL = 4
 TDP_ERR_PARTICLE_COUNT = 5
 TDP_ERR_NO_DEVICE = 6
+TDP_ERR_NOT_CONVERGED = 7
+
+FLAG_REQUIRE_CONVERGENCE = 1
 
 PRECISION = {"double": 0, "mixed": 1}
 SOLVER = {"jacobi": 0, "pcg": 1}
@@ -48,7 +51,7 @@ class ForceDesc(C.Structure):
 
 
 class Options(C.Structure):
-    _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("device", C.c_int32)]
+    _fields_ = [("precision", C.c_int32), ("solver", C.c_int32), ("device", C.c_int32), ("flags", C.c_int32)]
 
 
 class DenseStats(C.Structure):
@@ -149,11 +152,11 @@ class PackedForce:
 class HipContextHandle:
     """Owns one tdp_context; methods raise TdpError on a nonzero status."""
 
-    def __init__(self, force, default_box, precision="double", solver="jacobi", device=0):
+    def __init__(self, force, default_box, precision="double", solver="jacobi", device=0, flags=0):
         self.L = load()
         self.n = force.getNumParticles()
         pf = PackedForce(force, default_box)
-        opts = Options(PRECISION[precision], SOLVER[solver], int(device))
+        opts = Options(PRECISION[precision], SOLVER[solver], int(device), int(flags))
         h = C.c_void_p()
         rc = self.L.tdp_create(C.byref(pf.desc), C.byref(opts), C.byref(h))
         if rc != TDP_OK:
@@ -187,13 +190,13 @@ class HipContextHandle:
         pf = PackedForce(force, default_box)
         self._check(self.L.tdp_copy_parameters(self.h, C.byref(pf.desc)))
 
-    def execute(self, positions, box, forces=None):
+    def execute(self, positions, box, forces=None, include_forces=True, include_energy=True):
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
         f = np.zeros(pos.size) if forces is None else forces
         e = C.c_double(0.0)
         b = self._box(box)
-        self._check(self.L.tdp_execute(self.h, _ptr(pos), None if b is None else _ptr(b), 1, 1, _ptr(f),
-                                       C.byref(e)))
+        self._check(self.L.tdp_execute(self.h, _ptr(pos), None if b is None else _ptr(b), int(bool(include_forces)),
+                                       int(bool(include_energy)), _ptr(f), C.byref(e)))
         return e.value, f.reshape(-1, 3)
 
     def execute_device(self, d_positions_ptr, box, d_forces_ptr, d_energy_ptr, stream_ptr):
@@ -259,13 +262,13 @@ class HipBatchHandle(HipContextHandle):
     """Owns one batch context over several NoCutoff systems (tdp_batch_create); particles are
     numbered consecutively across the systems."""
 
-    def __init__(self, forces, precision="double", solver="pcg", device=0):
+    def __init__(self, forces, precision="double", solver="pcg", device=0, flags=0):
         self.L = load()
         self.packed = [PackedForce(f, np.zeros(9)) for f in forces]
         self.sizes = [f.getNumParticles() for f in forces]
         self.n = int(sum(self.sizes))
         descs = (ForceDesc * len(forces))(*[pf.desc for pf in self.packed])
-        opts = Options(PRECISION[precision], SOLVER[solver], int(device))
+        opts = Options(PRECISION[precision], SOLVER[solver], int(device), int(flags))
         h = C.c_void_p()
         rc = self.L.tdp_batch_create(descs, len(forces), C.byref(opts), C.byref(h))
         if rc != TDP_OK:
@@ -292,13 +295,17 @@ class HipBatchHandle(HipContextHandle):
 
     RESPONSE_METHODS = {"fp16": 0, "bf16": 1, "cg": 2}
 
-    def responses(self, positions, method="fp16", tolerance=1e-10, max_steps=50):
-        """Induced dipoles of every system for E0 and unit fields along x, y, z: array [4, N, 3]."""
+    def responses(self, positions, method="fp16", tolerance=1e-10, max_steps=50, check=True):
+        """Induced dipoles of every system for E0 and unit fields along x, y, z: array [4, N, 3], and the
+        statistics.  With check=False a nonzero status is returned as a third value instead of raised."""
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
         out = np.zeros(4 * pos.size)
         st = DenseStats()
-        self._check(self.L.tdp_batch_solve_responses(self.h, _ptr(pos), self.RESPONSE_METHODS[method],
-                                                     float(tolerance), int(max_steps), _ptr(out), C.byref(st)))
-        return out.reshape(4, -1, 3), dict(steps=st.steps, max_relative_residual=st.max_relative_residual,
-                                           factor_failed=st.factor_failed, assemble_ms=st.assemble_ms,
-                                           factor_ms=st.factor_ms)
+        rc = self.L.tdp_batch_solve_responses(self.h, _ptr(pos), self.RESPONSE_METHODS[method], float(tolerance),
+                                              int(max_steps), _ptr(out), C.byref(st))
+        stats = dict(steps=st.steps, max_relative_residual=st.max_relative_residual, factor_failed=st.factor_failed,
+                     assemble_ms=st.assemble_ms, factor_ms=st.factor_ms)
+        if not check:
+            return out.reshape(4, -1, 3), stats, rc
+        self._check(rc)
+        return out.reshape(4, -1, 3), stats

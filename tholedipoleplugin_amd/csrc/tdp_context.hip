@@ -18,6 +18,7 @@
 
 #include "../../include/tholedipole_hip.h"
 #include "tdp_kernels.h"
+#include "tdp_tables.h"
 
 using namespace tdp;
 
@@ -26,12 +27,6 @@ namespace {
 thread_local std::string g_create_error;
 
 }  // namespace
-
-// A pair whose scales differ from 1, in the caller's indices (i < j).
-struct ExcPair {
-    int i, j;
-    double m, is;
-};
 
 // Evaluations between recomputations of the internal spatial order.
 constexpr int kReorderInterval = 100;
@@ -50,6 +45,7 @@ struct tdp_context {
     std::vector<size_t> tcol;
     int method = TDP_NO_CUTOFF, polarization = TDP_MUTUAL, precision = TDP_PRECISION_DOUBLE;
     int solver = TDP_SOLVER_JACOBI, device = 0;
+    int flags = 0;   // TDP_FLAG_* of the options
     double cutoff = 1.0, ewald_tol = 1e-4, pme_alpha = 0.0;
     int grid[3] = {0, 0, 0};
     std::vector<double> extrap;
@@ -70,10 +66,12 @@ struct tdp_context {
     // ---- device state
     hipStream_t stream = nullptr;
     // Captured chunks of solver iterations (hipGraph), replayed while the launch parameters hold:
-    // same chunk length and solver, same box, same device tables (graph_gen counts table uploads).
+    // same chunk length, solver and precision (the pair kernel and the vectors it reads), same box,
+    // same device tables (graph_gen counts table uploads).
     struct IterGraph {
         int iters;
         bool jacobi;
+        int precision;
         unsigned gen;
         double box[3];
         hipGraphExec_t exec;
@@ -169,8 +167,8 @@ int dalloc(tdp_context* c, T** ptr, size_t count) {
     return TDP_OK;
 }
 
-const double kMScale[5] = {0.0, 0.0, 0.0, 0.5, 1.0};  // ReferenceTholeDipoleForce.cpp:37-41
-const double kIScale[5] = {0.0, 1.0, 1.0, 1.0, 1.0};  // ReferenceTholeDipoleForce.cpp:43-47
+// max that keeps a NaN (std::max drops a NaN second argument)
+double nan_max(double a, double b) { return (std::isnan(a) || std::isnan(b)) ? NAN : std::max(a, b); }
 
 // Validation that TholeDipoleForceImpl::initialize performs (TholeDipoleForceImpl.cpp:52-108),
 // plus the inputs the reference math would dereference out of bounds.
@@ -240,79 +238,18 @@ void copy_particles(tdp_context* c, const tdp_force_desc* d) {
     c->atom_y.assign(d->atom_y, d->atom_y + n);
 }
 
-// setupScaleMaps/getScaleFactor (ReferenceTholeDipoleForce.cpp:609-644): pair (i<j) takes its
-// scales from particle i's own lists only: mScale from list 0 at the position of j, iScale from
-// list 1, the last occurrence of a duplicated atom winning.  Pairs whose scales are both 1 are
-// plain pairs; the rest are kept (in the caller's indices) for the exclusion tiles.
+// setupScaleMaps/getScaleFactor (ReferenceTholeDipoleForce.cpp:609-644) -> the scaled pairs.
 void build_exclusion_pairs(tdp_context* c, const tdp_force_desc* d) {
-    const int n = c->n;
-    std::unordered_map<long long, std::pair<double, double>> pairs;
-    auto key = [n](int i, int j) { return (long long)i * n + j; };
-    for (int i = 0; i < n; i++) {
-        for (int t = 0; t < 2; t++) {
-            int b = d->covalent_offsets[4 * i + t], e = d->covalent_offsets[4 * i + t + 1];
-            for (int p = b; p < e; p++) {
-                int a = d->covalent_atoms[p];
-                if (a <= i || a >= n) continue;  // never looked up by the reference
-                auto it = pairs.find(key(i, a));
-                if (it == pairs.end()) it = pairs.emplace(key(i, a), std::make_pair(1.0, 1.0)).first;
-                if (t == 0) it->second.first = kMScale[p - b];
-                else it->second.second = kIScale[p - b];
-            }
-        }
-    }
-    c->exc_pairs.clear();
-    for (auto& kv : pairs) {
-        if (kv.second.first == 1.0 && kv.second.second == 1.0) continue;
-        c->exc_pairs.push_back({(int)(kv.first / n), (int)(kv.first % n), kv.second.first, kv.second.second});
-    }
-    std::sort(c->exc_pairs.begin(), c->exc_pairs.end(),
-              [](const ExcPair& a, const ExcPair& b) { return a.i != b.i ? a.i < b.i : a.j < b.j; });
+    c->exc_pairs = scaled_pairs(c->n, d->covalent_offsets, d->covalent_atoms);
 }
 
-// Exclusion tiles in the internal order: for a tile pair (I,J) holding a scaled pair, three
-// 64-bit masks per row mark mScale == 0, mScale == 0.5 and iScale == 0; self pairs set all three.
+// Exclusion tiles of the current internal order (tdp_tables.cpp).
 void build_tiles(tdp_context* c) {
-    const int n = c->n, nblk = c->nblk;
-    std::vector<std::unordered_map<int, int>> tile_of(nblk);
-    std::vector<unsigned long long> masks;  // [tile][3][64]
-    std::vector<std::pair<int, int>> tile_ij;
-    auto tile = [&](int I, int J) {
-        auto it = tile_of[I].find(J);
-        if (it != tile_of[I].end()) return it->second;
-        int id = (int)tile_ij.size();
-        tile_of[I][J] = id;
-        tile_ij.push_back({I, J});
-        masks.resize(masks.size() + 3 * 64, 0ull);
-        return id;
-    };
-    auto setbits = [&](int i, int j, double m, double is) {
-        int t = tile(i / 64, j / 64);
-        int row = i % 64, col = j % 64;
-        unsigned long long bit = 1ull << col;
-        if (m == 0.0) masks[((size_t)t * 3 + 0) * 64 + row] |= bit;
-        if (m == 0.5) masks[((size_t)t * 3 + 1) * 64 + row] |= bit;
-        if (is == 0.0) masks[((size_t)t * 3 + 2) * 64 + row] |= bit;
-    };
-    for (int i = 0; i < n; i++) setbits(c->inv[i], c->inv[i], 0.0, 0.0);
-    for (const ExcPair& e : c->exc_pairs) {
-        int a = c->inv[e.i], b = c->inv[e.j];
-        setbits(a, b, e.m, e.is);
-        setbits(b, a, e.m, e.is);
-    }
-    std::vector<int> order(tile_ij.size());
-    for (size_t k = 0; k < order.size(); k++) order[k] = (int)k;
-    std::sort(order.begin(), order.end(), [&](int a, int b) { return tile_ij[a] < tile_ij[b]; });
-    c->exc_start.assign(nblk + 1, 0);
-    c->exc_J.resize(order.size());
-    c->exc_mask.resize(masks.size());
-    for (size_t k = 0; k < order.size(); k++) {
-        int t = order[k];
-        c->exc_start[tile_ij[t].first + 1]++;
-        c->exc_J[k] = tile_ij[t].second;
-        std::memcpy(&c->exc_mask[k * 3 * 64], &masks[(size_t)t * 3 * 64], 3 * 64 * sizeof(unsigned long long));
-    }
-    for (int I = 0; I < nblk; I++) c->exc_start[I + 1] += c->exc_start[I];
+    ExclusionTiles t;
+    exclusion_tiles(c->n, c->nblk, c->inv.data(), c->exc_pairs, t);
+    c->exc_start = std::move(t.start);
+    c->exc_J = std::move(t.J);
+    c->exc_mask = std::move(t.mask);
 }
 
 // Gather lists for mapTorqueToForce (ReferenceTholeDipoleForce.cpp:1092-1115): each particle's
@@ -727,7 +664,7 @@ int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool
             c->graphs.erase(c->graphs.begin() + k);
             continue;
         }
-        if (g.iters == n && g.jacobi == jacobi) ex = g.exec;
+        if (g.iters == n && g.jacobi == jacobi && g.precision == c->precision) ex = g.exec;
         k++;
     }
     if (!ex) {
@@ -738,7 +675,7 @@ int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool
         hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
         hipGraphDestroy(graph);
         if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
-        c->graphs.push_back({n, jacobi, c->graph_gen, {L.box[0], L.box[1], L.box[2]}, ex});
+        c->graphs.push_back({n, jacobi, c->precision, c->graph_gen, {L.box[0], L.box[1], L.box[2]}, ex});
     }
     TDP_HIP(c, hipGraphLaunch(ex, s));
     return TDP_OK;
@@ -836,13 +773,16 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
 }
 
 // calculateElectrostatic + mapTorqueToForce (ReferenceTholeDipoleForce.cpp:118-124) into d_frc
-// and d_energy.
-void forces_and_energy(tdp_context* c, const double* box) {
+// and d_energy.  The pair pass forms energies and forces together; a caller that wants no forces
+// skips the torque mapping, one that wants no energy skips the energy reduction (includeForces,
+// includeEnergy of TholeDipoleKernels.h:64-68).
+void forces_and_energy(tdp_context* c, const double* box, bool want_forces = true, bool want_energy = true) {
     hipStream_t s = c->stream;
     if (c->n == 0) {
         hipMemsetAsync(c->d_energy, 0, sizeof(double), s);
         return;
     }
+    if (!want_forces && !want_energy) return;
     const int npad = c->npad;
     PairLaunch L = pair_launch(c, box);
     SegLaunch G = seg_launch(c);
@@ -853,11 +793,26 @@ void forces_and_energy(tdp_context* c, const double* box) {
                     c->d_inva, c->d_perm, pf,
                     reinterpret_cast<double*>(pf + off), pt, reinterpret_cast<double*>(pt + off), c->d_part_e, c->d_frc,
                     c->torque, c->d_slab32);
-    launch_reduce_energy(s, G, c->d_part_e, c->d_energy);
-    if (!c->gidx.empty() && c->has_perm) {   // without permanent dipoles every torque is zero
+    if (want_energy) launch_reduce_energy(s, G, c->d_part_e, c->d_energy);
+    if (want_forces && !c->gidx.empty() && c->has_perm) {   // without permanent dipoles every torque is zero
         launch_map_torque(s, npad, c->posa, c->d_axis, c->torque, c->contrib);
         launch_gather_torque(s, npad, c->d_goff, c->d_gidx, c->contrib, c->d_frc);
     }
+}
+
+// With TDP_FLAG_REQUIRE_CONVERGENCE, a Mutual solve of any system that stopped at its iteration
+// limit above its target is an error (the reference records non-convergence silently,
+// ReferenceTholeDipoleForce.cpp:849-850); the results of the evaluation are still written.
+int convergence_status(tdp_context* c) {
+    if (!(c->flags & TDP_FLAG_REQUIRE_CONVERGENCE) || c->polarization != TDP_MUTUAL) return TDP_OK;
+    for (int g = 0; g < c->nseg; g++)
+        if (!c->converged[g]) {
+            char buf[200];
+            snprintf(buf, sizeof buf, "Mutual induced dipoles did not converge: system %d stopped after %d iterations "
+                     "at epsilon %.3e (target %.3e)", g, c->iterations[g], c->epsilon[g], c->seg_target[g]);
+            return fail(c, TDP_ERR_NOT_CONVERGED, buf);
+        }
+    return TDP_OK;
 }
 
 int ensure_out(tdp_context* c, size_t bytes) {
@@ -999,6 +954,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     c->device = opts ? opts->device : 0;
     c->precision = opts ? opts->precision : TDP_PRECISION_DOUBLE;
     c->solver = opts ? opts->solver : TDP_SOLVER_JACOBI;
+    c->flags = opts ? opts->flags : 0;
     if (c->device < 0 || c->device >= ndev) {
         delete c;
         return fail(nullptr, TDP_ERR_NO_DEVICE, "invalid device ordinal");
@@ -1149,6 +1105,8 @@ int download_fields(tdp_context* c, const double4* src, int nfield, double* out)
     return TDP_OK;
 }
 
+int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dense_stats* st);
+
 // Induced dipoles of every system for four fields: E0 (the permanent field) and unit uniform
 // fields along x, y and z.  Low-precision Cholesky of A^ = I - D^1/2 T D^1/2 on MFMA, then FP64
 // iterative refinement with matrix-free FP64 residuals until every system's relative residual
@@ -1195,9 +1153,9 @@ int dense_responses(tdp_context* c, bool bf16, double tol, int max_steps, double
         int left = 0;
         for (int g = 0; g < ns; g++) {
             double w = 0.0;
-            for (int q = 0; q < 4; q++) w = std::max(w, rel[4 * g + q]);
-            worst = std::max(worst, w);
-            active[g] = w > tol ? 1 : 0;
+            for (int q = 0; q < 4; q++) w = nan_max(w, rel[4 * g + q]);
+            worst = nan_max(worst, w);
+            active[g] = w <= tol ? 0 : 1;   // a NaN residual keeps the system active
             left += active[g];
         }
         if (!left) break;
@@ -1221,7 +1179,21 @@ int dense_responses(tdp_context* c, bool bf16, double tol, int max_steps, double
         st->assemble_ms = t_asm;
         st->factor_ms = t_fac;
     }
-    return TDP_OK;
+    if (worst <= tol) return TDP_OK;   // accepted on the FP64 residual, whatever the factor's pivots were
+    // The refinement did not reach the tolerance (an indefinite or badly rounded factor, or too few
+    // steps): the whole batch is solved again by FP64 CG, and fails if that does not converge either.
+    tdp_dense_stats cg{};
+    const int rc_cg = cg_responses(c, tol, std::max(max_steps, 200), out, &cg);
+    if (st) {
+        st->steps = steps + cg.steps;
+        st->max_relative_residual = cg.max_relative_residual;
+    }
+    if (rc_cg == TDP_ERR_NOT_CONVERGED)
+        return fail(c, TDP_ERR_NOT_CONVERGED,
+                    std::string("responses: refinement of the ") + (bf16 ? "BF16" : "FP16") +
+                        " factor and the FP64 CG fallback both ended above the tolerance" +
+                        (status ? " (a pivot of the factorization was not positive)" : ""));
+    return rc_cg;
 }
 
 // The same four responses by Jacobi-preconditioned FP64 CG of every system (the comparator of the
@@ -1266,7 +1238,7 @@ int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dens
         if (rc) break;
         for (int g = 0; g < ns; g++) {
             total_iter = std::max(total_iter, c->iterations[g]);
-            worst = std::max(worst, c->epsilon[g] / std::max(c->seg_target[g] / tol, 1e-300));
+            worst = nan_max(worst, c->epsilon[g] / std::max(c->seg_target[g] / tol, 1e-300));
         }
         // field q in the caller's order; downloaded with the others after the last solve
         if (out) launch_scatter_vec3(s, npad, c->d_perm, c->mu, c->d_rout + (size_t)q * 3 * c->n);
@@ -1286,6 +1258,9 @@ int cg_responses(tdp_context* c, double tol, int max_iter, double* out, tdp_dens
         st->assemble_ms = 0.0;
         st->factor_ms = 0.0;
     }
+    if (!(worst <= tol))
+        return fail(c, TDP_ERR_NOT_CONVERGED, "responses: FP64 CG ended above the tolerance after " +
+                                                  std::to_string(total_iter) + " iterations");
     return TDP_OK;
 }
 
@@ -1417,10 +1392,10 @@ int tdp_execute(tdp_context* c, const double* positions, const double* box, int3
     hipSetDevice(c->device);
     if ((rc = upload_positions(c, positions))) return rc;
     if ((rc = setup_and_solve(c, box, positions))) return rc;
-    forces_and_energy(c, box);
-    if ((rc = download_results(c, include_forces ? forces : nullptr, energy))) return rc;
-    (void)include_energy;  // both are always computed, as in the reference (ReferenceTholeDipoleKernels.cpp:201-215)
-    return TDP_OK;
+    const bool want_f = include_forces && forces, want_e = include_energy && energy;
+    forces_and_energy(c, box, want_f, want_e);
+    if ((rc = download_results(c, want_f ? forces : nullptr, want_e ? energy : nullptr))) return rc;
+    return convergence_status(c);
 }
 
 int tdp_execute_device(tdp_context* c, const double* d_positions, const double* box, double* d_forces,
@@ -1440,14 +1415,14 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
         c->ev_pool.push_back(ev);
         return rc;
     }
-    forces_and_energy(c, box);
+    forces_and_energy(c, box, d_forces != nullptr, d_energy != nullptr);
     if (d_forces && c->n > 0) launch_scatter_accumulate(c->stream, c->npad, c->d_perm, c->d_frc, d_forces);
     if (d_energy)
         TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     TDP_HIP(c, hipEventRecord(ev, c->stream));
     TDP_HIP(c, hipStreamWaitEvent(user, ev, 0));
     c->ev_pool.push_back(ev);
-    return TDP_OK;
+    return convergence_status(c);
 }
 
 int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, int32_t kind, double* out) {
@@ -1472,7 +1447,7 @@ int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, 
         for (int k = 0; k < 3; k++)
             out[3 * o + k] = kind == TDP_DIPOLES_INDUCED ? a[k] : (kind == TDP_DIPOLES_LAB_PERMANENT ? b[k] : b[k] + a[k]);
     }
-    return TDP_OK;
+    return convergence_status(c);
 }
 
 int tdp_get_electrostatic_potential(tdp_context* c, const double* positions, const double* box, const double* grid,

@@ -136,6 +136,25 @@ int tdp_execute(tdp_context* ctx, const double* positions, const double* box, in
 int tdp_execute_device(tdp_context* ctx, const double* d_positions, const double* box, double* d_forces,
                        double* d_energy, void* stream);
 
+/* No counterpart in the reference: tdp_execute_device for callers that cannot allocate device or
+ * pinned memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  Same arguments and results as tdp_execute_device; the evaluation's partial sums
+ * live in the library's static device scratch of TDP_NOALLOC_SCRATCH_BYTES (one copy per GPU, held
+ * until the process ends) instead of the context's buffers, and the call makes no hipMalloc,
+ * hipHostMalloc, hipFree, event creation or graph instantiation.  Limits and differences:
+ *   - contexts whose partial sums exceed the scratch are rejected with TDP_ERR_INVALID_ARGUMENT
+ *     (about 7,000 particles in one system at the default schedule);
+ *   - the context's internal spatial order is kept as it is (the caller's order until a
+ *     tdp_execute* call has computed one), so results match tdp_execute_device to rounding;
+ *   - solver iterations are enqueued one launch at a time (no captured graphs), still with one host
+ *     read of the stop count per chunk of iterations for a Mutual solve;
+ *   - timing must be off (TDP_ERR_INVALID_ARGUMENT otherwise).
+ * Every call on one GPU uses that GPU's one scratch: calls that overlap on one GPU (several
+ * streams or threads) race; serialise them. */
+#define TDP_NOALLOC_SCRATCH_BYTES (64u << 20)
+int tdp_execute_device_noalloc(tdp_context* ctx, const double* d_positions, const double* box, double* d_forces,
+                               double* d_energy, void* stream);
+
 /* Replaces getInducedDipoles / getLabFramePermanentDipoles / getTotalDipoles
  * (TholeDipoleKernels.h:91-105, ReferenceTholeDipoleKernels.cpp:217-282): recomputes from the
  * given positions, like the reference.  out: host [3n]. */

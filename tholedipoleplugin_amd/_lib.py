@@ -29,7 +29,8 @@ DIPOLES_INDUCED, DIPOLES_LAB_PERMANENT, DIPOLES_TOTAL = 0, 1, 2
 # Every symbol include/tholedipole_hip.h declares.
 EXPORTED_SYMBOLS = [
     "tdp_create", "tdp_destroy", "tdp_last_error", "tdp_last_create_error", "tdp_copy_parameters",
-    "tdp_execute", "tdp_execute_device", "tdp_get_dipoles", "tdp_get_electrostatic_potential",
+    "tdp_execute", "tdp_execute_device", "tdp_execute_device_noalloc", "tdp_get_dipoles",
+    "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
     "tdp_autotune", "tdp_set_warm_start",
     "tdp_batch_create", "tdp_batch_num_systems", "tdp_batch_execute", "tdp_batch_execute_device",
@@ -93,6 +94,7 @@ def load():
     L.tdp_copy_parameters.argtypes = [ctx_p, C.POINTER(ForceDesc)]
     L.tdp_execute.argtypes = [ctx_p, _dp, _dp, C.c_int32, C.c_int32, _dp, _dp]
     L.tdp_execute_device.argtypes = [ctx_p, C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.tdp_execute_device_noalloc.argtypes = [ctx_p, C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.tdp_get_dipoles.argtypes = [ctx_p, _dp, _dp, C.c_int32, _dp]
     L.tdp_get_electrostatic_potential.argtypes = [ctx_p, _dp, _dp, _dp, C.c_int32, _dp]
     L.tdp_get_system_multipole_moments.argtypes = [ctx_p, _dp, _dp, _dp, _dp]
@@ -204,6 +206,18 @@ class HipContextHandle:
         self._check(self.L.tdp_execute_device(self.h, C.c_void_p(d_positions_ptr), None if b is None else _ptr(b),
                                               C.c_void_p(d_forces_ptr), C.c_void_p(d_energy_ptr),
                                               C.c_void_p(stream_ptr)))
+
+    def execute_device_noalloc(self, d_positions_ptr, box, d_forces_ptr, d_energy_ptr, stream_ptr, check=True):
+        """tdp_execute_device_noalloc: the device entry point with its partial sums in the library's static
+        scratch (no allocation).  With check=False the status is returned instead of raised."""
+        b = self._box(box)
+        rc = self.L.tdp_execute_device_noalloc(self.h, C.c_void_p(d_positions_ptr), None if b is None else _ptr(b),
+                                               C.c_void_p(d_forces_ptr), C.c_void_p(d_energy_ptr),
+                                               C.c_void_p(stream_ptr))
+        if not check:
+            return rc
+        self._check(rc)
+        return rc
 
     def dipoles(self, positions, box, kind):
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1)

@@ -12,6 +12,7 @@
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -25,6 +26,47 @@ using namespace tdp;
 namespace {
 
 thread_local std::string g_create_error;
+
+}  // namespace
+
+// Scratch of tdp_execute_device_noalloc: the partial sums of one evaluation (pair kernels' work-item
+// partials and tile slabs, torque and energy partials) for callers that cannot allocate.  One copy
+// per GPU, held until the process ends; calls that overlap on one GPU share it and race.
+constexpr size_t kNoallocScratchBytes = (size_t)TDP_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};   // device address of scratch, per device ordinal
+
+// Device address of this device's scratch (the device must be current); called at context
+// creation, once per device.
+hipError_t scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (g_scratch_ptr[device]) return hipSuccess;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_ptr[device] = ptr;
+    return hipSuccess;
+}
+
+void* scratch_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    return g_scratch_ptr[device];
+}
+
+// Zero fill of n 16-byte words (the scratch is cleared by a kernel, not a memset, so that a caller
+// may capture the call).
+__global__ void k_zero16(uint4* __restrict__ p, size_t n) {
+    const uint4 z = make_uint4(0u, 0u, 0u, 0u);
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = z;
+}
 
 }  // namespace
 
@@ -102,7 +144,7 @@ struct tdp_context {
     int4* d_axis = nullptr;
     int *d_exc_start = nullptr, *d_exc_J = nullptr, *d_goff = nullptr, *d_gidx = nullptr, *d_perm = nullptr;
     unsigned long long* d_exc_mask = nullptr;
-    int hist_order = 0;
+    int hist_order = 0, extrap_order = 0;
     size_t out_bytes = 0;
     int *d_tcnt = nullptr, *d_tfirst = nullptr, *d_tseg = nullptr, *d_done = nullptr;
     // ---- dense path (allocated on first use)
@@ -124,7 +166,8 @@ struct tdp_context {
     std::vector<int> iterations, converged;
     // pinned host copy of the stop count and the solver scalars, read in one round trip
     void* h_stage = nullptr;
-    size_t h_stage_bytes = 0;
+    // set for the duration of a tdp_execute_device_noalloc call: no reorder, no graph capture, no timing
+    bool noalloc = false;
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
     bool warm_start = false, have_prev = false;
@@ -502,7 +545,8 @@ void harvest_timing(tdp_context* c) {
 void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const double* scalars,
             const PcgQ& pq = PcgQ()) {
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
+    const bool timed = c->timing && !c->noalloc;
+    if (timed) {
         e0 = take_event(c);
         e1 = take_event(c);
         hipEventRecord(e0, c->stream);
@@ -514,7 +558,7 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         launch_sym_matvec_mixed(c->stream, L, c->posa, v64, c->vec32, c->d_inva, scalars, c->d_slab32);
     else
         launch_sym_matvec64(c->stream, L, c->posa, v64, c->d_inva, scalars, pi, pj);
-    if (c->timing) {
+    if (timed) {
         hipEventRecord(e1, c->stream);
         c->pending.push_back({e0, e1});
     }
@@ -530,16 +574,16 @@ int check_box(tdp_context* c, const double* box) {
     return TDP_OK;
 }
 
+// Pinned staging buffer of a context, allocated once at create: the download area (the stop count,
+// then the solver scalars of every system) and the upload area (the solver scalars, then the stop
+// count and a zero).  Every copy through it is followed by a stream synchronisation before the host
+// touches it again.
+size_t stage_download_doubles(const tdp_context* c) { return (size_t)SC_COUNT * c->nseg + 1; }
+size_t stage_upload_doubles(const tdp_context* c) { return (size_t)SC_COUNT * c->nseg + 2; }
+double* stage_upload(tdp_context* c) { return static_cast<double*>(c->h_stage) + stage_download_doubles(c); }
+
 // Copies the stop count and the solver scalars to the pinned staging buffer (one host round trip).
 int stage_solver_state(tdp_context* c, int& done) {
-    const size_t bytes = sizeof(double) * ((size_t)SC_COUNT * c->nseg + 1);
-    if (bytes > c->h_stage_bytes) {
-        if (c->h_stage) hipHostFree(c->h_stage);
-        c->h_stage = nullptr;
-        c->h_stage_bytes = 0;
-        TDP_HIP(c, hipHostMalloc(&c->h_stage, bytes, hipHostMallocDefault));
-        c->h_stage_bytes = bytes;
-    }
     double* h = static_cast<double*>(c->h_stage);
     TDP_HIP(c, hipMemcpyAsync(h, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     TDP_HIP(c, hipMemcpyAsync(h + 1, c->d_scalars, sizeof(double) * SC_COUNT * c->nseg, hipMemcpyDeviceToHost,
@@ -571,7 +615,9 @@ int read_solver_results(tdp_context* c, bool jacobi, bool staged = false) {
 // Starts a Mutual solve: per-system iteration limit and target, epsilon 1, and for the Jacobi
 // iteration the reference's loop condition (:829-832) as the initial stop flag.
 int start_solve(tdp_context* c, bool jacobi) {
-    std::vector<double> sc((size_t)SC_COUNT * c->nseg, 0.0);
+    double* sc = stage_upload(c);
+    const size_t nsc = (size_t)SC_COUNT * c->nseg;
+    std::fill(sc, sc + nsc, 0.0);
     int done = 0;
     for (int g = 0; g < c->nseg; g++) {
         double* x = &sc[(size_t)SC_COUNT * g];
@@ -584,9 +630,11 @@ int start_solve(tdp_context* c, bool jacobi) {
         }
     }
     if (!jacobi) done = 0;   // the CG start kernel sets every system's flag afresh
-    TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc.data(), sc.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    int zero[2] = {done, 0};
-    TDP_HIP(c, hipMemcpyAsync(c->d_done, zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc, nsc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int* counts = reinterpret_cast<int*>(sc + nsc);
+    counts[0] = done;
+    counts[1] = 0;
+    TDP_HIP(c, hipMemcpyAsync(c->d_done, counts, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
     return TDP_OK;
 }
 
@@ -598,7 +646,9 @@ int start_solve(tdp_context* c, bool jacobi) {
 int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
     hipStream_t s = c->stream;
     const int n = c->n, npad = c->npad;
-    if (!c->ordered || c->evals_since_order >= kReorderInterval) {
+    // (an allocation-free call keeps the current order: recomputing it needs host work and may grow
+    // the exclusion tables)
+    if (!c->noalloc && (!c->ordered || c->evals_since_order >= kReorderInterval)) {
         std::vector<double> tmp;
         const double* hp = host_pos3;
         if (!hp) {
@@ -652,7 +702,7 @@ int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool
             }
         }
     };
-    if (!c->use_graphs || c->timing) {
+    if (!c->use_graphs || c->timing || c->noalloc) {
         enqueue();
         return TDP_OK;
     }
@@ -712,7 +762,7 @@ int solve_mutual(tdp_context* c, const double* box) {
     }
     if ((rc = read_solver_results(c, jacobi, true))) return rc;
     c->last_max_iter = *std::max_element(c->iterations.begin(), c->iterations.end());
-    if (c->timing) harvest_timing(c);
+    if (c->timing && !c->noalloc) harvest_timing(c);
     return TDP_OK;
 }
 
@@ -738,13 +788,8 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         return TDP_OK;
     }
     if (c->polarization == TDP_EXTRAPOLATED) {
-        // convergeInducedDipolesByExtrapolation (:853-909)
-        static const double opt4[4] = {-0.154, 0.017, 0.658, 0.474};
-        std::vector<double> coef = c->extrap;
-        if (coef.empty()) coef.assign(opt4, opt4 + 4);
-        int order = (int)coef.size();
-        if (order > c->hist_order) return fail(c, TDP_ERR_INVALID_ARGUMENT, "extrapolation history not allocated");
-        TDP_HIP(c, hipMemcpyAsync(c->d_coef, coef.data(), order * sizeof(double), hipMemcpyHostToDevice, s));
+        // convergeInducedDipolesByExtrapolation (:853-909); the coefficients are on the device since create
+        const int order = c->extrap_order;
         TDP_HIP(c, hipMemcpyAsync(c->hist, c->mu, npad * sizeof(double4), hipMemcpyDeviceToDevice, s));
         for (int o = 1; o < order; o++) {
             matvec(c, L, c->mu, nullptr);
@@ -758,7 +803,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         if (rc) return rc;
         std::fill(c->converged.begin(), c->converged.end(), 1);
         std::fill(c->iterations.begin(), c->iterations.end(), order);
-        if (c->timing) harvest_timing(c);
+        if (c->timing && !c->noalloc) harvest_timing(c);
         return TDP_OK;
     }
     if (c->warm_start && c->have_prev)   // start from the previous solution instead of alpha E0
@@ -1044,8 +1089,41 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_tp, c->ntri)))
         return bail(rc);
     if (c->polarization == TDP_EXTRAPOLATED) {
-        c->hist_order = std::max(4, (int)c->extrap.size());
+        static const double opt4[4] = {-0.154, 0.017, 0.658, 0.474};   // TholeDipoleForce.cpp default
+        std::vector<double> coef = c->extrap;
+        if (coef.empty()) coef.assign(opt4, opt4 + 4);
+        if (coef.size() > 64) {
+            c->err = "TholeDipoleForce: more than 64 extrapolation coefficients";
+            return bail(TDP_ERR_INVALID_ARGUMENT);
+        }
+        c->extrap_order = (int)coef.size();
+        c->hist_order = c->extrap_order;
         if ((rc = dalloc(c, &c->hist, (size_t)c->hist_order * npad))) return bail(rc);
+        if (hipMemcpyAsync(c->d_coef, coef.data(), coef.size() * sizeof(double), hipMemcpyHostToDevice, c->stream) !=
+                hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess) {
+            c->err = "hipMemcpy of the extrapolation coefficients failed";
+            return bail(TDP_ERR_HIP);
+        }
+    }
+    // pinned staging of the solver state and the events of one evaluation, so that an evaluation
+    // allocates nothing
+    if (hipHostMalloc(&c->h_stage, sizeof(double) * (stage_download_doubles(c) + stage_upload_doubles(c)),
+                      hipHostMallocDefault) != hipSuccess) {
+        c->err = "hipHostMalloc of the staging buffer failed";
+        return bail(TDP_ERR_HIP);
+    }
+    for (int k = 0; k < 4; k++) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) {
+            c->err = "hipEventCreate failed";
+            return bail(TDP_ERR_HIP);
+        }
+        c->ev_pool.push_back(e);
+    }
+    if (scratch_init(c->device) != hipSuccess) {
+        c->err = "hipGetSymbolAddress of the allocation-free scratch failed";
+        return bail(TDP_ERR_HIP);
     }
     if ((rc = upload_tables(c)) || (rc = upload_particles(c))) return bail(rc);
     *out = c;
@@ -1422,6 +1500,72 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
     TDP_HIP(c, hipEventRecord(ev, c->stream));
     TDP_HIP(c, hipStreamWaitEvent(user, ev, 0));
     c->ev_pool.push_back(ev);
+    return convergence_status(c);
+}
+
+int tdp_execute_device_noalloc(tdp_context* c, const double* d_positions, const double* box, double* d_forces,
+                               double* d_energy, void* stream) {
+    if (!c) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null context");
+    if (c->n > 0 && !d_positions) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null positions");
+    if (c->timing) return fail(c, TDP_ERR_INVALID_ARGUMENT, "allocation-free evaluation: disable timing first");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    unsigned char* base = static_cast<unsigned char*>(scratch_ptr(c->device));
+    if (!base) return fail(c, TDP_ERR_HIP, "allocation-free evaluation: no scratch address for this device");
+    // the context's evaluation partials, carved from the scratch in 256-byte aligned pieces
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pb = ((size_t)c->nwork + c->ntri) * 64 * sizeof(double4);
+    const size_t pe = ((size_t)c->nwork + 1) * sizeof(double);
+    const size_t p32 = c->precision == TDP_PRECISION_MIXED ? 3 * ((size_t)c->ntri + c->nwork) * 64 * sizeof(float) : 0;
+    const size_t need = up(pb) + up(pb) + up(pe) + up(p32);
+    if (need > kNoallocScratchBytes) {
+        char buf[240];
+        snprintf(buf, sizeof buf, "allocation-free evaluation: the partial sums of this context need %zu bytes, more "
+                 "than the %zu bytes of scratch (about 7,000 particles in one system)", need, kNoallocScratchBytes);
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, buf);
+    }
+    hipStream_t user = reinterpret_cast<hipStream_t>(stream);
+    hipEvent_t ev = take_event(c);   // from the pool filled at create
+    TDP_HIP(c, hipEventRecord(ev, user));
+    TDP_HIP(c, hipStreamWaitEvent(c->stream, ev, 0));
+    void* saved_pbuf = c->d_pbuf;
+    double4* saved_pbuf2 = c->d_pbuf2;
+    double* saved_part_e = c->d_part_e;
+    float* saved_slab32 = c->d_slab32;
+    c->d_pbuf = base;
+    c->d_pbuf2 = reinterpret_cast<double4*>(base + up(pb));
+    c->d_part_e = reinterpret_cast<double*>(base + 2 * up(pb));
+    c->d_slab32 = p32 ? reinterpret_cast<float*>(base + 2 * up(pb) + up(pe)) : nullptr;
+    c->noalloc = true;
+    // the previous call's contents: start from zero, as a fresh context's buffers do
+    hipLaunchKernelGGL(k_zero16, dim3(1024), dim3(256), 0, c->stream, reinterpret_cast<uint4*>(base), need / 16);
+    rc = TDP_OK;
+    if (c->n > 0) {
+        hipError_t e = hipMemcpyAsync(c->d_pos3, d_positions, 3 * c->n * sizeof(double), hipMemcpyDeviceToDevice,
+                                      c->stream);
+        if (e != hipSuccess) rc = fail(c, TDP_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+    }
+    if (!rc) rc = setup_and_solve(c, box, nullptr);
+    if (!rc) {
+        forces_and_energy(c, box, d_forces != nullptr, d_energy != nullptr);
+        if (d_forces && c->n > 0) launch_scatter_accumulate(c->stream, c->npad, c->d_perm, c->d_frc, d_forces);
+        if (d_energy) {
+            hipError_t e = hipMemcpyAsync(d_energy, c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToDevice,
+                                          c->stream);
+            if (e != hipSuccess) rc = fail(c, TDP_ERR_HIP, std::string("hipMemcpyAsync: ") + hipGetErrorString(e));
+        }
+    }
+    // the caller's stream waits for this call's work, which reads the scratch until it ends
+    hipEventRecord(ev, c->stream);
+    hipStreamWaitEvent(user, ev, 0);
+    c->ev_pool.push_back(ev);
+    c->noalloc = false;
+    c->d_pbuf = saved_pbuf;
+    c->d_pbuf2 = saved_pbuf2;
+    c->d_part_e = saved_part_e;
+    c->d_slab32 = saved_slab32;
+    if (rc) return rc;
     return convergence_status(c);
 }
 

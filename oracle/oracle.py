@@ -27,6 +27,22 @@ class _Sys(C.Structure):
                 ("extrap", _dp), ("n_extrap", C.c_int)]
 
 
+class _Ewald(C.Structure):
+    _fields_ = [("cutoff", C.c_double), ("alpha", C.c_double), ("grid", C.c_int * 3), ("order", C.c_int),
+                ("kmax", C.c_int * 3)]
+
+
+def _ewald(cutoff, alpha, grid=(0, 0, 0), order=5, kmax=None):
+    e = _Ewald()
+    e.cutoff = float(cutoff)
+    e.alpha = float(alpha)
+    for k in range(3):
+        e.grid[k] = int(grid[k])
+        e.kmax[k] = int(kmax[k]) if kmax is not None else 0
+    e.order = int(order)
+    return e
+
+
 class _Stats(C.Structure):
     _fields_ = [("energy", C.c_double), ("iterations", C.c_int), ("epsilon", C.c_double), ("converged", C.c_int)]
 
@@ -59,6 +75,14 @@ def lib():
         L.tdpo_pass_slice.argtypes = [C.POINTER(_Sys), _dp, _dp, C.c_int, C.c_int, C.c_int, _dp]
         L.tdpo_rows.argtypes = [C.POINTER(_Sys), _dp, _dp, _ip, C.c_int, _dp, _dp, _dp, _dp, _dp]
         L.tdpo_evaluate_at.argtypes = [C.POINTER(_Sys), _dp, _dp, _dp, _dp, _dp, _dp]
+        L.tdpo_ewald_fields.argtypes = [C.POINTER(_Sys), C.POINTER(_Ewald), _dp, _dp, _dp, _dp]
+        L.tdpo_ewald_energy_at.argtypes = [C.POINTER(_Sys), C.POINTER(_Ewald), _dp, _dp, _dp, _dp, _dp]
+        L.tdpo_ewald_evaluate.argtypes = [C.POINTER(_Sys), C.POINTER(_Ewald), _dp, _dp, _dp, _dp, _dp,
+                                          C.POINTER(_Stats)]
+        L.tdpo_fft3.argtypes = [_dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.tdpo_fft3.restype = None
+        L.tdpo_bspline.argtypes = [C.c_double, C.c_int, _dp, _dp, _dp]
+        L.tdpo_bspline.restype = None
         _lib = L
     return _lib
 
@@ -228,6 +252,69 @@ def evaluate_at(force, positions, mu, box=None, threads=8):
     _check(L.tdpo_evaluate_at(C.byref(os_.s), _p(pos), _p(m), _p(e0), _p(ei), _p(f), C.byref(e)))
     L.tdpo_set_threads(1)
     return dict(e0=e0.reshape(n, 3), eind=ei.reshape(n, 3), forces=f.reshape(n, 3), energy=e.value)
+
+
+# ---- Ewald sums (tdp_ewald.c): the model of the HIP library's Ewald option
+def ewald_evaluate(force, positions, box, cutoff, alpha, grid=(0, 0, 0), order=5, kmax=None, threads=8, **kw):
+    """E0, the induced-dipole solve, energy and forces by Ewald summation: dict(energy, forces,
+    induced, e0, iterations, epsilon, converged)."""
+    L = lib()
+    L.tdpo_set_threads(int(threads))
+    os_ = OracleSystem(force, box, **kw)
+    n = os_.n
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    f = np.zeros(3 * n); mu = np.zeros(3 * n); e0 = np.zeros(3 * n); en = C.c_double(0.0)
+    st = _Stats()
+    e = _ewald(cutoff, alpha, grid, order, kmax)
+    _check(L.tdpo_ewald_evaluate(C.byref(os_.s), C.byref(e), _p(pos), _p(f), C.byref(en), _p(mu), _p(e0),
+                                 C.byref(st)))
+    L.tdpo_set_threads(1)
+    return dict(energy=en.value, forces=f.reshape(n, 3), induced=mu.reshape(n, 3), e0=e0.reshape(n, 3),
+                iterations=st.iterations, epsilon=st.epsilon, converged=st.converged)
+
+
+def ewald_energy_at(force, positions, mu, box, cutoff, alpha, grid=(0, 0, 0), order=5, kmax=None, threads=8):
+    """Energy, forces and E0 by Ewald summation at GIVEN induced dipoles: dict(energy, forces, e0)."""
+    L = lib()
+    L.tdpo_set_threads(int(threads))
+    os_ = OracleSystem(force, box)
+    n = os_.n
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    m = np.ascontiguousarray(mu, np.float64).reshape(-1)
+    f = np.zeros(3 * n); e0 = np.zeros(3 * n); en = C.c_double(0.0)
+    e = _ewald(cutoff, alpha, grid, order, kmax)
+    _check(L.tdpo_ewald_energy_at(C.byref(os_.s), C.byref(e), _p(pos), _p(m), _p(f), C.byref(en), _p(e0)))
+    L.tdpo_set_threads(1)
+    return dict(energy=en.value, forces=f.reshape(n, 3), e0=e0.reshape(n, 3))
+
+
+def ewald_fields(force, positions, mu, box, cutoff, alpha, grid=(0, 0, 0), order=5, kmax=None, threads=8):
+    """E0 and T mu (with the Ewald self-field) by Ewald summation at given dipoles: (e0, tmu)."""
+    L = lib()
+    L.tdpo_set_threads(int(threads))
+    os_ = OracleSystem(force, box)
+    n = os_.n
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    m = np.ascontiguousarray(mu, np.float64).reshape(-1)
+    e0 = np.zeros(3 * n); tm = np.zeros(3 * n)
+    e = _ewald(cutoff, alpha, grid, order, kmax)
+    _check(L.tdpo_ewald_fields(C.byref(os_.s), C.byref(e), _p(pos), _p(m), _p(e0), _p(tm)))
+    L.tdpo_set_threads(1)
+    return e0.reshape(n, 3), tm.reshape(n, 3)
+
+
+def fft3(a, sign):
+    """Unnormalised 3D DFT of a complex array (sign -1 forward, +1 inverse)."""
+    b = np.ascontiguousarray(a, np.complex128).copy()
+    nx, ny, nz = b.shape
+    lib().tdpo_fft3(b.ctypes.data_as(_dp), nx, ny, nz, int(sign))
+    return b
+
+
+def bspline(w, order):
+    th = np.zeros(order); d1 = np.zeros(order); d2 = np.zeros(order)
+    lib().tdpo_bspline(float(w), int(order), _p(th), _p(d1), _p(d2))
+    return th, d1, d2
 
 
 def time_pass_slices(force, positions, mu, box, which, slices, threads=1):

@@ -113,6 +113,33 @@ int tdpo_lab_dipoles(const tdpo_system* s, const double* pos, double* labdip);
 /* mapTorqueToForceForParticle for every particle (ReferenceTholeDipoleForce.cpp:1092-1115). */
 int tdpo_map_torques(const tdpo_system* s, const double* pos, const double* torques, double* forces);
 
+/* ---- Ewald sums (tdp_ewald.c): charges and induced dipoles with Thole damping summed over all
+ * periodic images; the model is described at the top of tdp_ewald.c.  Requires method PME, an
+ * orthorhombic box (s->box), zero molecular dipoles, Direct or Mutual polarization. */
+typedef struct {
+    double cutoff;   /* real-space cutoff, nm (at most half the box) */
+    double alpha;    /* Ewald splitting parameter, 1/nm */
+    int grid[3];     /* smooth PME grid */
+    int order;       /* B-spline order (5) */
+    int kmax[3];     /* kmax[0] > 0: exact Ewald sum over |n_d| <= kmax[d] instead of smooth PME */
+} tdpo_ewald;
+
+/* E0 (3n) and T mu (3n, including the Ewald self-field) at given dipoles; either may be NULL. */
+int tdpo_ewald_fields(const tdpo_system* s, const tdpo_ewald* e, const double* pos, const double* mu, double* e0,
+                      double* tmu);
+/* Energy and forces (accumulated, 3n) at GIVEN induced dipoles mu; e0_out (3n) may be NULL. */
+int tdpo_ewald_energy_at(const tdpo_system* s, const tdpo_ewald* e, const double* pos, const double* mu,
+                         double* forces, double* energy, double* e0_out);
+/* Full evaluation: E0, the induced-dipole solve (PCG, stop at RMS(alpha r) <= target_eps), energy and
+ * forces (accumulated).  Any output may be NULL. */
+int tdpo_ewald_evaluate(const tdpo_system* s, const tdpo_ewald* e, const double* pos, double* forces,
+                        double* energy, double* induced, double* e0_out, tdpo_stats* st);
+/* Building blocks exported for the pinning tests: 3D unnormalised complex DFT in place
+ * (re_im: interleaved complex, x slowest; sign -1 forward, +1 inverse) and the order-p B-spline
+ * weights th[j] = M_p(w + p-1-j) with first and second derivatives (d1, d2 may be NULL). */
+void tdpo_fft3(double* re_im, int nx, int ny, int nz, int sign);
+void tdpo_bspline(double w, int p, double* th, double* d1, double* d2);
+
 #ifdef __cplusplus
 }
 #endif

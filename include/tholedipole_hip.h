@@ -92,6 +92,14 @@ typedef struct tdp_force_desc {
                                             TDP_ERR_NOT_CONVERGED when a Mutual solve stops at its iteration
                                             limit above its target; the reference records this silently
                                             (ReferenceTholeDipoleForce.cpp:849-850) */
+#define TDP_FLAG_EWALD 2                 /* PME method: Ewald-summed electrostatics (no counterpart in the
+                                            reference, whose PME method is the minimum image over all pairs,
+                                            ReferenceTholeDipoleForce.cpp:1375-1423): pairs within the cutoff
+                                            in real space plus smooth PME of order 5 on the grid and Ewald
+                                            parameter of tdp_get_pme_parameters, for charges and induced
+                                            dipoles (molecular dipoles zero), Direct or Mutual polarization,
+                                            rectangular boxes.  Energy: U_qq - 1/2 sum mu.E0; forces: -grad
+                                            of it at fixed mu (the exact gradient: U is stationary in mu) */
 
 typedef struct tdp_options {
     int32_t precision;                /* TDP_PRECISION_* */
@@ -155,6 +163,14 @@ int tdp_execute_device(tdp_context* ctx, const double* d_positions, const double
 int tdp_execute_device_noalloc(tdp_context* ctx, const double* d_positions, const double* box, double* d_forces,
                                double* d_energy, void* stream);
 
+/* No counterpart in the reference (its PME method has no cutoff): every pair i < j (caller indices)
+ * whose minimum-image distance the Ewald option's real-space passes find below the cutoff, for the
+ * given positions: pairs[2k], pairs[2k+1] for k < min(*count, capacity), in no particular order;
+ * *count receives the number of pairs.  The criterion is evaluated in FP64 with correctly rounded
+ * operations: per axis d = x_i - x_j, d -= L * floor(d / L + 0.5); r2 = (dx^2 + dy^2) + dz^2 < cutoff^2. */
+int tdp_get_neighbor_pairs(tdp_context* ctx, const double* positions, const double* box, int64_t capacity,
+                           int32_t* pairs, int64_t* count);
+
 /* Replaces getInducedDipoles / getLabFramePermanentDipoles / getTotalDipoles
  * (TholeDipoleKernels.h:91-105, ReferenceTholeDipoleKernels.cpp:217-282): recomputes from the
  * given positions, like the reference.  out: host [3n]. */
@@ -169,7 +185,10 @@ int tdp_get_electrostatic_potential(tdp_context* ctx, const double* positions, c
 int tdp_get_system_multipole_moments(tdp_context* ctx, const double* positions, const double* box,
                                      const double* masses, double* out);
 
-/* Replaces getPMEParameters (TholeDipoleKernels.h:84, ReferenceTholeDipoleKernels.cpp:341-348). */
+/* Replaces getPMEParameters (TholeDipoleKernels.h:84, ReferenceTholeDipoleKernels.cpp:341-348): the
+ * values given to setPMEParameters, or when alpha or the grid is 0 the closed form of OpenMM's
+ * calcPMEParameters (alpha = sqrt(-ln(2 tol))/cutoff, grid sizes rounded up to prime factors 2, 3, 5
+ * and 7); the Ewald option computes with these. */
 int tdp_get_pme_parameters(const tdp_context* ctx, double* alpha, int32_t* nx, int32_t* ny, int32_t* nz);
 
 /* Solver diagnostics of the last evaluation (the reference records these silently,

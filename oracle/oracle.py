@@ -83,6 +83,8 @@ def lib():
         L.tdpo_fft3.restype = None
         L.tdpo_bspline.argtypes = [C.c_double, C.c_int, _dp, _dp, _dp]
         L.tdpo_bspline.restype = None
+        L.tdpo_cutoff_pairs.argtypes = [C.POINTER(_Sys), _dp, C.c_double, C.c_longlong, _ip]
+        L.tdpo_cutoff_pairs.restype = C.c_longlong
         _lib = L
     return _lib
 
@@ -301,6 +303,17 @@ def ewald_fields(force, positions, mu, box, cutoff, alpha, grid=(0, 0, 0), order
     _check(L.tdpo_ewald_fields(C.byref(os_.s), C.byref(e), _p(pos), _p(m), _p(e0), _p(tm)))
     L.tdpo_set_threads(1)
     return e0.reshape(n, 3), tm.reshape(n, 3)
+
+
+def cutoff_pairs(force, positions, box, cutoff):
+    """Pairs (i < j) within the cutoff by the library's FP64 neighbour criterion, i then j ascending."""
+    L = lib()
+    os_ = OracleSystem(force, box)
+    pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+    n = L.tdpo_cutoff_pairs(C.byref(os_.s), _p(pos), float(cutoff), 0, None)
+    out = np.zeros(2 * max(n, 1), np.int32)
+    L.tdpo_cutoff_pairs(C.byref(os_.s), _p(pos), float(cutoff), n, _p(out, _ip))
+    return out[:2 * n].reshape(-1, 2).astype(np.int64)
 
 
 def fft3(a, sign):

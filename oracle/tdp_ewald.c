@@ -312,8 +312,8 @@ static void pair_scale(const ewctx* c, int i, int j, double* m, double* is, int*
  * lambda3/5/7).  Index n multiplies the n-th kernel: 1/r, 1/r^3, 3/r^5, 15/r^7. */
 typedef struct { double qq[2], qm[3], mm[4]; } pair_coef;
 
-static void real_coeffs(const ewctx* c, double r, double ai, double aj, double m, double is, pair_coef* pc) {
-    const int inside = r < c->rc;
+static void real_coeffs(const ewctx* c, double r, int inside, double ai, double aj, double m, double is,
+                        pair_coef* pc) {
     const double a = c->a, ri = 1.0 / r, r2i = ri * ri;
     double B[4] = {0, 0, 0, 0};
     if (inside) {
@@ -360,15 +360,16 @@ typedef struct {
     int mutual;         /* dipole-dipole forces */
 } real_out;
 
-static void real_pair(const ewctx* c, int i, int j, const double* d, double r, real_out* o, double* e0,
+static void real_pair(const ewctx* c, int i, int j, const double* d, double r2, real_out* o, double* e0,
                       double* tmu, double* f, double* uqq) {
     const tdpo_system* s = c->s;
     double m, is;
     int listed;
     pair_scale(c, i, j, &m, &is, &listed);
-    if (!(r < c->rc) && !listed) return;
+    const int inside = r2 < c->rc * c->rc;
+    if (!inside && !listed) return;
     pair_coef pc;
-    real_coeffs(c, r, s->thole[i], s->thole[j], m, is, &pc);
+    real_coeffs(c, sqrt(r2), inside, s->thole[i], s->thole[j], m, is, &pc);
     const double qi = s->charge[i], qj = s->charge[j];
     if (e0)
         for (int k = 0; k < 3; k++) {
@@ -476,8 +477,8 @@ static void real_space(const ewctx* c, real_out* o) {
                                 double d[3];
                                 for (int k = 0; k < 3; k++) d[k] = c->pos[3 * i + k] - c->pos[3 * j + k];
                                 min_image(c, d);
-                                const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                                if (r < c->rc) real_pair(c, i, j, d, r, o, e0, tmu, f, o->uqq ? &uqq : NULL);
+                                const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+                                if (r2 < c->rc * c->rc) real_pair(c, i, j, d, r2, o, e0, tmu, f, o->uqq ? &uqq : NULL);
                             }
                         }
             } else {
@@ -485,8 +486,8 @@ static void real_space(const ewctx* c, real_out* o) {
                     double d[3];
                     for (int k = 0; k < 3; k++) d[k] = c->pos[3 * i + k] - c->pos[3 * j + k];
                     min_image(c, d);
-                    const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-                    if (r < c->rc) real_pair(c, i, j, d, r, o, e0, tmu, f, o->uqq ? &uqq : NULL);
+                    const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+                    if (r2 < c->rc * c->rc) real_pair(c, i, j, d, r2, o, e0, tmu, f, o->uqq ? &uqq : NULL);
                 }
             }
         }
@@ -502,8 +503,8 @@ static void real_space(const ewctx* c, real_out* o) {
         double d[3];
         for (int q = 0; q < 3; q++) d[q] = c->pos[3 * i + q] - c->pos[3 * j + q];
         min_image(c, d);
-        const double r = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        if (!(r < c->rc)) real_pair(c, i, j, d, r, o, e0, tmu, f, o->uqq ? &uqq : NULL);
+        const double r2 = (d[0] * d[0] + d[1] * d[1]) + d[2] * d[2];
+        if (!(r2 < c->rc * c->rc)) real_pair(c, i, j, d, r2, o, e0, tmu, f, o->uqq ? &uqq : NULL);
     }
     for (int t = 0; t < nt; t++) {
         const double* a = acc + (size_t)t * 3 * N3;
@@ -862,4 +863,29 @@ int tdpo_ewald_evaluate(const tdpo_system* s, const tdpo_ewald* e, const double*
     free(mu);
     ew_free(&c);
     return 0;
+}
+
+/* Every pair i < j whose minimum-image distance is below the cutoff, by the criterion of the HIP
+ * library's real-space passes (correctly rounded FP64 operations in this order, built without
+ * contraction): per axis d = x_i - x_j, d -= L * floor(d / L + 0.5); (dx^2 + dy^2) + dz^2 < rc^2.
+ * pairs[2k], pairs[2k+1] for k < min(count, cap), i ascending then j; returns the count. */
+long long tdpo_cutoff_pairs(const tdpo_system* s, const double* pos, double cutoff, long long cap, int* pairs) {
+    const double rc2 = cutoff * cutoff;
+    long long cnt = 0;
+    for (int i = 0; i < s->n; i++)
+        for (int j = i + 1; j < s->n; j++) {
+            double d[3];
+            for (int k = 0; k < 3; k++) {
+                d[k] = pos[3 * i + k] - pos[3 * j + k];
+                d[k] -= s->box[k] * floor(d[k] / s->box[k] + 0.5);
+            }
+            if ((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2] < rc2) {
+                if (cnt < cap) {
+                    pairs[2 * cnt] = i;
+                    pairs[2 * cnt + 1] = j;
+                }
+                cnt++;
+            }
+        }
+    return cnt;
 }

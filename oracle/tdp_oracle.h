@@ -138,6 +138,9 @@ int tdpo_ewald_evaluate(const tdpo_system* s, const tdpo_ewald* e, const double*
  * (re_im: interleaved complex, x slowest; sign -1 forward, +1 inverse) and the order-p B-spline
  * weights th[j] = M_p(w + p-1-j) with first and second derivatives (d1, d2 may be NULL). */
 void tdpo_fft3(double* re_im, int nx, int ny, int nz, int sign);
+/* Every pair i < j within the cutoff by the HIP library's FP64 neighbour criterion (see tdp_ewald.c);
+ * returns the count, writes the first cap pairs. */
+long long tdpo_cutoff_pairs(const tdpo_system* s, const double* pos, double cutoff, long long cap, int* pairs);
 void tdpo_bspline(double w, int p, double* th, double* d1, double* d2);
 
 #ifdef __cplusplus

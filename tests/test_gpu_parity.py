@@ -213,7 +213,18 @@ def test_error_paths():
     a, nx, ny, nz = w2.force.getPMEParametersInContext(w2ctx)
     tol, rc, L = 1e-4, 0.7, 1.8643
     assert abs(a - np.sqrt(-np.log(2 * tol)) / rc) < 1e-12
-    assert nx == int(np.ceil(2 * a * L / (3 * tol ** 0.2)))
+
+    def legal(n):   # smallest size >= n with prime factors 2, 3, 5, 7 (OpenMM's FFT sizes)
+        while True:
+            m = n
+            for p in (2, 3, 5, 7):
+                while m % p == 0:
+                    m //= p
+            if m == 1:
+                return n
+            n += 1
+    n0 = int(np.ceil(2 * a * L / (3 * tol ** 0.2)))
+    assert nx == legal(max(6, n0)) and (n0, nx) == (33, 35)
 
 
 def test_device_entry_point_matches_host():

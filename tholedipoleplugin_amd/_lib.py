@@ -21,6 +21,7 @@ TDP_ERR_NO_DEVICE = 6
 TDP_ERR_NOT_CONVERGED = 7
 
 FLAG_REQUIRE_CONVERGENCE = 1
+FLAG_EWALD = 2
 
 PRECISION = {"double": 0, "mixed": 1}
 SOLVER = {"jacobi": 0, "pcg": 1}
@@ -29,7 +30,7 @@ DIPOLES_INDUCED, DIPOLES_LAB_PERMANENT, DIPOLES_TOTAL = 0, 1, 2
 # Every symbol include/tholedipole_hip.h declares.
 EXPORTED_SYMBOLS = [
     "tdp_create", "tdp_destroy", "tdp_last_error", "tdp_last_create_error", "tdp_copy_parameters",
-    "tdp_execute", "tdp_execute_device", "tdp_execute_device_noalloc", "tdp_get_dipoles",
+    "tdp_execute", "tdp_execute_device", "tdp_execute_device_noalloc", "tdp_get_neighbor_pairs", "tdp_get_dipoles",
     "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
     "tdp_autotune", "tdp_set_warm_start",
@@ -96,6 +97,7 @@ def load():
     L.tdp_execute_device.argtypes = [ctx_p, C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.tdp_execute_device_noalloc.argtypes = [ctx_p, C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.tdp_get_dipoles.argtypes = [ctx_p, _dp, _dp, C.c_int32, _dp]
+    L.tdp_get_neighbor_pairs.argtypes = [ctx_p, _dp, _dp, C.c_int64, _ip, C.POINTER(C.c_int64)]
     L.tdp_get_electrostatic_potential.argtypes = [ctx_p, _dp, _dp, _dp, C.c_int32, _dp]
     L.tdp_get_system_multipole_moments.argtypes = [ctx_p, _dp, _dp, _dp, _dp]
     L.tdp_get_pme_parameters.argtypes = [ctx_p, _dp, _ip, _ip, _ip]
@@ -218,6 +220,19 @@ class HipContextHandle:
             return rc
         self._check(rc)
         return rc
+
+    def neighbor_pairs(self, positions, box):
+        """Pairs (i < j) within the cutoff as the Ewald option's real-space passes find them, sorted."""
+        pos = np.ascontiguousarray(positions, np.float64).reshape(-1)
+        b = self._box(box)
+        cnt = C.c_int64(0)
+        self._check(self.L.tdp_get_neighbor_pairs(self.h, _ptr(pos), None if b is None else _ptr(b), 0, None,
+                                                  C.byref(cnt)))
+        out = np.zeros(2 * max(cnt.value, 1), np.int32)
+        self._check(self.L.tdp_get_neighbor_pairs(self.h, _ptr(pos), None if b is None else _ptr(b), cnt.value,
+                                                  _ptr(out, _ip), C.byref(cnt)))
+        p = out[:2 * cnt.value].reshape(-1, 2).astype(np.int64)
+        return p[np.lexsort((p[:, 1], p[:, 0]))]
 
     def dipoles(self, positions, box, kind):
         pos = np.ascontiguousarray(positions, np.float64).reshape(-1)

@@ -75,10 +75,11 @@ class HipCalcTholeDipoleForceKernel:
 
     Name = "CalcTholeDipoleForce"
 
-    def __init__(self, precision="double", solver="jacobi", device=0):
+    def __init__(self, precision="double", solver="jacobi", device=0, flags=0):
         self.precision = precision
         self.solver = solver
         self.device = device
+        self.flags = flags
         self.handle: Optional[_lib.HipContextHandle] = None
         self.force = None
 
@@ -87,7 +88,8 @@ class HipCalcTholeDipoleForceKernel:
         self.default_box = system.getDefaultPeriodicBoxVectors()
         self.masses = np.asarray([system.getParticleMass(i) for i in range(system.getNumParticles())], np.float64)
         try:
-            self.handle = _lib.HipContextHandle(self.force, self.default_box, self.precision, self.solver, self.device)
+            self.handle = _lib.HipContextHandle(self.force, self.default_box, self.precision, self.solver, self.device,
+                                                self.flags)
         except _lib.TdpError as e:
             raise OpenMMException(str(e)) from None
 
@@ -133,13 +135,17 @@ class Context:
     """Holds positions/box and one HIP kernel per TholeDipoleForce of the System.
 
     ``precision`` is "double" or "mixed", ``solver`` "jacobi" (the reference iteration) or "pcg".
+    ``ewald=True`` sums the PME method's electrostatics by Ewald splitting (real space within the
+    cutoff, smooth PME) instead of the reference's minimum image over all pairs (TDP_FLAG_EWALD).
     """
 
-    def __init__(self, system: System, precision: str = "double", solver: str = "jacobi", device: int = 0):
+    def __init__(self, system: System, precision: str = "double", solver: str = "jacobi", device: int = 0,
+                 ewald: bool = False):
         self._system = system
         self._precision = precision
         self._solver = solver
         self._device = device
+        self._flags = _lib.FLAG_EWALD if ewald else 0
         self._positions = np.zeros((system.getNumParticles(), 3))
         self._box = system.getDefaultPeriodicBoxVectors()
         self._kernels = {}
@@ -150,7 +156,7 @@ class Context:
         for f in self._system.getForces():
             if isinstance(f, TholeDipoleForce):
                 validate_force(f, self._system.getNumParticles(), self._system.getDefaultPeriodicBoxVectors())
-                k = HipCalcTholeDipoleForceKernel(self._precision, self._solver, self._device)
+                k = HipCalcTholeDipoleForceKernel(self._precision, self._solver, self._device, self._flags)
                 k.initialize(self._system, f)
                 self._kernels[id(f)] = (f, k)
 

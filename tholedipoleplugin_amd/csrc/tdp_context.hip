@@ -168,6 +168,27 @@ struct tdp_context {
     void* h_stage = nullptr;
     // set for the duration of a tdp_execute_device_noalloc call: no reorder, no graph capture, no timing
     bool noalloc = false;
+    // ---- Ewald option (TDP_FLAG_EWALD): parameters, FFT plans, grids, neighbour tiles, scaled pairs
+    bool ewald = false;
+    double ew_alpha = 0.0, ew_box[3] = {0.0, 0.0, 0.0};
+    double ew_r64 = 0.25;   // mixed precision: real-space pairs closer than this stay in FP64 (nm)
+    int ew_K[3] = {0, 0, 0};
+    int ew_radix[3][kEwMaxStages] = {};
+    int ew_nst[3] = {0, 0, 0};
+    double ew_qsum = 0.0;
+    double2* d_tw64[3] = {nullptr, nullptr, nullptr};
+    float2* d_tw32[3] = {nullptr, nullptr, nullptr};
+    double* d_G64 = nullptr;
+    float* d_G32 = nullptr;
+    unsigned long long* d_qfix = nullptr;
+    double2* d_cgrid = nullptr;
+    double *d_phiq = nullptr, *d_phiv = nullptr;
+    int *d_nbr = nullptr, *d_nbr_cnt = nullptr;
+    int *d_sc_off = nullptr, *d_sc_j = nullptr;
+    double2* d_sc_s = nullptr;
+    size_t cap_sc = 0;
+    double *d_epart = nullptr, *d_eout = nullptr;
+    double4* d_rs = nullptr;
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
     bool warm_start = false, have_prev = false;
@@ -379,6 +400,8 @@ int upload_particles(tdp_context* c) {
     return TDP_OK;
 }
 
+int ew_upload_scaled(tdp_context* c);
+
 // Exclusion tiles and torque-gather lists of the current order.
 int upload_tables(tdp_context* c) {
     c->graph_gen++;   // device tables may move: captured iteration graphs are stale
@@ -400,6 +423,7 @@ int upload_tables(tdp_context* c) {
     if (!c->gidx.empty())
         TDP_HIP(c, hipMemcpyAsync(c->d_gidx, c->gidx.data(), c->gidx.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->ewald) return ew_upload_scaled(c);
     return TDP_OK;
 }
 
@@ -476,6 +500,175 @@ int reorder(tdp_context* c, const double* host_pos3, const double* box) {
     return TDP_OK;
 }
 
+// ---------------------------------------------------------------- Ewald option helpers
+// Smallest n >= n0 whose prime factors are 2, 3, 5 and 7 (the FFT sizes OpenMM accepts).
+int legal_fft_size(int n0) {
+    for (int n = std::max(n0, 1);; n++) {
+        int m = n;
+        for (int p : {2, 3, 5, 7})
+            while (m % p == 0) m /= p;
+        if (m == 1) return n;
+    }
+}
+
+// alpha and grid of the PME method: setPMEParameters' values, or when alpha or the grid is 0, the
+// closed form of OpenMM's NonbondedForceImpl::calcPMEParameters (the reference asks it for them,
+// ReferenceTholeDipoleKernels.cpp:143-152): alpha = sqrt(-ln(2 tol))/cutoff,
+// n_d = legal size >= max(6, ceil(2 alpha L_d / (3 tol^(1/5)))).
+void pme_parameters(const tdp_context* c, double& a, int g[3]) {
+    a = c->pme_alpha;
+    for (int k = 0; k < 3; k++) g[k] = c->grid[k];
+    if (g[0] == 0 || a == 0.0) {
+        const double tol = c->ewald_tol;
+        a = std::sqrt(-std::log(2.0 * tol)) / c->cutoff;
+        const double L[3] = {c->default_box[0], c->default_box[4], c->default_box[8]};
+        for (int k = 0; k < 3; k++)
+            g[k] = legal_fft_size(std::max(6, (int)std::ceil(2 * a * L[k] / (3 * std::pow(tol, 0.2)))));
+    }
+}
+
+// Radix stages of an FFT of length n (4s first, then 2, 3, 5, 7); false if another factor remains.
+bool fft_plan(int n, int* r, int& nst) {
+    nst = 0;
+    int m = n;
+    while (m % 4 == 0 && nst < kEwMaxStages) { r[nst++] = 4; m /= 4; }
+    for (int p : {2, 3, 5, 7})
+        while (m % p == 0 && nst < kEwMaxStages) { r[nst++] = p; m /= p; }
+    return m == 1 && n > 1;
+}
+
+// |b(m)|^2 of the order-5 Euler exponential spline of one axis (Essmann et al. 1995, eq. 4.4):
+// 1 / |sum_{k=0}^{3} M_5(k+1) e^{2 pi i m k / K}|^2, the vanishing denominator at m = K/2 replaced by
+// the mean of its neighbours.
+std::vector<double> bspline_moduli5(int K) {
+    const double M5[4] = {1.0 / 24.0, 11.0 / 24.0, 11.0 / 24.0, 1.0 / 24.0};   // M_5(1..4)
+    std::vector<double> mod(K);
+    for (int m = 0; m < K; m++) {
+        double re = 0.0, im = 0.0;
+        for (int k = 0; k < 4; k++) {
+            re += M5[k] * std::cos(2.0 * M_PI * m * k / K);
+            im += M5[k] * std::sin(2.0 * M_PI * m * k / K);
+        }
+        mod[m] = re * re + im * im;
+    }
+    std::vector<double> fixed = mod;
+    for (int m = 0; m < K; m++)
+        if (mod[m] < 1e-7) fixed[m] = 0.5 * (mod[(m - 1 + K) % K] + mod[(m + 1) % K]);
+    for (int m = 0; m < K; m++) fixed[m] = 1.0 / fixed[m];
+    return fixed;
+}
+
+// Influence function of the box (when it changed): G(m) = exp(-pi^2 |m/L|^2 / a^2) / (pi V |m/L|^2)
+// times the three B-spline moduli, G(0) = 0, uploaded in FP64 and FP32.
+int ew_update_box(tdp_context* c, const double* box9) {
+    const double L[3] = {box9[0], box9[4], box9[8]};
+    if (L[0] == c->ew_box[0] && L[1] == c->ew_box[1] && L[2] == c->ew_box[2]) return TDP_OK;
+    const int K0 = c->ew_K[0], K1 = c->ew_K[1], K2 = c->ew_K[2];
+    const std::vector<double> b0 = bspline_moduli5(K0), b1 = bspline_moduli5(K1), b2 = bspline_moduli5(K2);
+    const size_t ng = (size_t)K0 * K1 * K2;
+    std::vector<double> G(ng);
+    std::vector<float> G32(ng);
+    const double V = L[0] * L[1] * L[2], f = M_PI * M_PI / (c->ew_alpha * c->ew_alpha);
+    for (int x = 0; x < K0; x++)
+        for (int y = 0; y < K1; y++)
+            for (int z = 0; z < K2; z++) {
+                const int mx = x <= K0 / 2 ? x : x - K0, my = y <= K1 / 2 ? y : y - K1, mz = z <= K2 / 2 ? z : z - K2;
+                const double kx = mx / L[0], ky = my / L[1], kz = mz / L[2];
+                const double m2 = kx * kx + ky * ky + kz * kz;
+                const size_t g = ((size_t)x * K1 + y) * K2 + z;
+                G[g] = (x == 0 && y == 0 && z == 0) ? 0.0 : std::exp(-f * m2) / (M_PI * V * m2) * b0[x] * b1[y] * b2[z];
+                G32[g] = (float)G[g];
+            }
+    TDP_HIP(c, hipStreamSynchronize(c->stream));   // no kernel may still read the old table
+    TDP_HIP(c, hipMemcpyAsync(c->d_G64, G.data(), ng * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_G32, G32.data(), ng * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 3; k++) c->ew_box[k] = L[k];
+    c->graph_gen++;
+    return TDP_OK;
+}
+
+EwLaunch ew_launch(const tdp_context* c, const double* box9) {
+    EwLaunch E;
+    E.npad = c->npad;
+    E.nblk = c->nblk;
+    E.tcnt = c->d_tcnt;
+    for (int k = 0; k < 3; k++) E.L[k] = box9[4 * k];
+    E.rc2 = c->cutoff * c->cutoff;
+    const float cm = (float)(c->cutoff * 1.0001 + 1e-4);
+    E.cut2_tiles = cm * cm;
+    const double a = c->ew_alpha, sp = 1.0 / std::sqrt(M_PI);
+    E.alpha = a;
+    E.c1 = 2.0 * a * sp;
+    E.c2 = 4.0 * a * a * a * sp;
+    E.c3 = 8.0 * a * a * a * a * a * sp;
+    E.self_field = 4.0 * a * a * a * sp / 3.0;
+    E.self_energy = -a * sp;
+    E.tb_c = c->tb_c;
+    E.tb_h = c->tb_h;
+    E.nbr = c->d_nbr;
+    E.nbr_cnt = c->d_nbr_cnt;
+    E.exc_start = c->d_exc_start;
+    E.exc_J = c->d_exc_J;
+    E.exc_mask = c->d_exc_mask;
+    E.sc_off = c->d_sc_off;
+    E.sc_j = c->d_sc_j;
+    E.sc_s = c->d_sc_s;
+    for (int k = 0; k < 3; k++) {
+        E.K[k] = c->ew_K[k];
+        E.ax[k].n = c->ew_K[k];
+        E.ax[k].nst = c->ew_nst[k];
+        for (int q = 0; q < kEwMaxStages; q++) E.ax[k].r[q] = c->ew_radix[k][q];
+        E.ax[k].tw64 = c->d_tw64[k];
+        E.ax[k].tw32 = c->d_tw32[k];
+    }
+    E.G64 = c->d_G64;
+    E.G32 = c->d_G32;
+    E.qfix = c->d_qfix;
+    E.cgrid = c->d_cgrid;
+    E.phi_q = c->d_phiq;
+    E.phi_v = c->d_phiv;
+    E.rs = c->d_rs;
+    // reciprocal space in FP64 in both precisions: FP32 grids and FFTs put 2.4e-8 e*nm into the
+    // fixed point of the 36,864-site box, above the 1e-6 D (2.08e-8 e*nm) bar
+    E.mixed = false;
+    E.pairs32 = c->precision == TDP_PRECISION_MIXED;
+    E.r64sq = c->ew_r64 * c->ew_r64;
+    return E;
+}
+
+// Scaled partners of every slot (both members of each scaled pair), sorted by slot and partner.
+int ew_upload_scaled(tdp_context* c) {
+    struct E3 { int slot, partner; double m, is; };
+    std::vector<E3> es;
+    es.reserve(2 * c->exc_pairs.size());
+    for (const ExcPair& e : c->exc_pairs) {
+        const int a = c->inv[e.i], b = c->inv[e.j];
+        es.push_back({a, b, e.m, e.is});
+        es.push_back({b, a, e.m, e.is});
+    }
+    std::sort(es.begin(), es.end(), [](const E3& x, const E3& y) { return x.slot != y.slot ? x.slot < y.slot : x.partner < y.partner; });
+    std::vector<int> off(c->npad + 1, 0), js(es.size());
+    std::vector<double2> ss(es.size());
+    for (size_t k = 0; k < es.size(); k++) {
+        off[es[k].slot + 1]++;
+        js[k] = es[k].partner;
+        ss[k] = make_double2(es[k].m, es[k].is);
+    }
+    for (int k = 0; k < c->npad; k++) off[k + 1] += off[k];
+    int rc;
+    size_t cap_j = c->cap_sc, cap_s = c->cap_sc;
+    if ((rc = dresize(c, &c->d_sc_j, cap_j, es.size())) || (rc = dresize(c, &c->d_sc_s, cap_s, es.size()))) return rc;
+    c->cap_sc = std::max(cap_j, (size_t)1);
+    TDP_HIP(c, hipMemcpyAsync(c->d_sc_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    if (!es.empty()) {
+        TDP_HIP(c, hipMemcpyAsync(c->d_sc_j, js.data(), js.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+        TDP_HIP(c, hipMemcpyAsync(c->d_sc_s, ss.data(), ss.size() * sizeof(double2), hipMemcpyHostToDevice, c->stream));
+    }
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    return TDP_OK;
+}
+
 PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     PairLaunch L;
     L.npad = c->npad;
@@ -549,7 +742,20 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
     if (timed) {
         e0 = take_event(c);
         e1 = take_event(c);
-        hipEventRecord(e0, c->stream);
+        if (!c->ewald) hipEventRecord(e0, c->stream);
+    }
+    if (c->ewald) {   // reciprocal potential of v, then the real-space pass with the gather in its epilogue
+        const double box9[9] = {L.box[0], 0, 0, 0, L.box[1], 0, 0, 0, L.box[2]};
+        const EwLaunch E = ew_launch(c, box9);
+        launch_ew_recip(c->stream, E, c->posa, nullptr, v64, E.phi_v, scalars);
+        if (timed) hipEventRecord(e0, c->stream);
+        launch_ew_pairs(c->stream, E, kEwTmu, false, c->posa, c->d_inva, c->d_charge, v64, nullptr, scalars, c->Emv, pq,
+                        nullptr, nullptr, nullptr);
+        if (timed) {
+            hipEventRecord(e1, c->stream);
+            c->pending.push_back({e0, e1});
+        }
+        return;
     }
     double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
     double* pj = reinterpret_cast<double*>(pi + (size_t)c->nwork * 64);
@@ -571,6 +777,10 @@ int check_box(tdp_context* c, const double* box) {
     double minAllowed = 1.999999 * c->cutoff;  // ReferenceTholeDipoleKernels.cpp:170-173
     if (box[0] < minAllowed || box[4] < minAllowed || box[8] < minAllowed)
         return fail(c, TDP_ERR_BOX_TOO_SMALL, "The periodic box size has decreased to less than twice the nonbonded cutoff.");
+    if (c->ewald && (box[1] != 0.0 || box[2] != 0.0 || box[3] != 0.0 || box[5] != 0.0 || box[6] != 0.0 || box[7] != 0.0))
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, "the Ewald option needs a rectangular box");
+    if (c->ewald && (2.0 * c->cutoff > box[0] || 2.0 * c->cutoff > box[4] || 2.0 * c->cutoff > box[8]))
+        return fail(c, TDP_ERR_BOX_TOO_SMALL, "the Ewald option needs a cutoff of at most half the box");
     return TDP_OK;
 }
 
@@ -666,6 +876,19 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
     launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    if (c->ewald) {
+        // neighbour tiles, the charges' potential grid, E0 = real space + reciprocal + scaled pairs
+        int rc = ew_update_box(c, box);
+        if (rc) return rc;
+        const EwLaunch E = ew_launch(c, box);
+        launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
+        launch_ew_neighbors(s, E);
+        launch_ew_recip(s, E, c->posa, c->d_charge, nullptr, E.phi_q, nullptr);
+        launch_ew_pairs(s, E, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv, PcgQ(),
+                        c->d_epart, nullptr, nullptr);
+        launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
+        return TDP_OK;
+    }
     launch_tile_pairs(s, L, c->d_tp);
     launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     {
@@ -829,6 +1052,18 @@ void forces_and_energy(tdp_context* c, const double* box, bool want_forces = tru
     }
     if (!want_forces && !want_energy) return;
     const int npad = c->npad;
+    if (c->ewald) {   // the dipoles' potential grid, then forces and energy per slot
+        const EwLaunch E = ew_launch(c, box);
+        launch_ew_recip(s, E, c->posa, nullptr, c->mu, E.phi_v, nullptr);
+        launch_ew_pairs(s, E, kEwForce, c->polarization == TDP_MUTUAL, c->posa, c->d_inva, c->d_charge, c->mu, c->E0,
+                        nullptr, nullptr, PcgQ(), c->d_epart, c->d_frc, c->d_eout);
+        if (want_energy) {
+            const double V = box[0] * box[4] * box[8];
+            const double bg = -M_PI * c->ew_qsum * c->ew_qsum / (2.0 * V * c->ew_alpha * c->ew_alpha);
+            launch_ew_energy(s, npad, c->d_eout, bg, c->d_energy);
+        }
+        return;
+    }
     PairLaunch L = pair_launch(c, box);
     SegLaunch G = seg_launch(c);
     double4* pf = reinterpret_cast<double4*>(c->d_pbuf);
@@ -928,11 +1163,13 @@ int set_work_items(tdp_context* c, int C) {
     for (int g = 0; g < ns; g++) segw[g] = make_int2(c->seg_w0[g], c->seg_w1[g]);
     const size_t pb = ((size_t)c->nwork + c->ntri) * 64;
     int rc;
+    // (the Ewald option's passes keep no pair partials: its contexts get token buffers)
+    const size_t pbn = c->ewald ? 1 : pb;
     if ((rc = dresize(c, &c->d_work, c->cap_work, work.size())) ||
-        (rc = dresize(c, (double4**)&c->d_pbuf, c->cap_pbuf, pb)) || (rc = dresize(c, &c->d_pbuf2, c->cap_pbuf2, pb)) ||
-        (rc = dresize(c, &c->d_part_e, c->cap_part_e, (size_t)c->nwork + 1)) ||
+        (rc = dresize(c, (double4**)&c->d_pbuf, c->cap_pbuf, pbn)) || (rc = dresize(c, &c->d_pbuf2, c->cap_pbuf2, pbn)) ||
+        (rc = dresize(c, &c->d_part_e, c->cap_part_e, c->ewald ? 1 : (size_t)c->nwork + 1)) ||
         (c->precision == TDP_PRECISION_MIXED &&
-         (rc = dresize(c, &c->d_slab32, c->cap_slab32, 3 * ((size_t)c->ntri + c->nwork) * 64))))
+         (rc = dresize(c, &c->d_slab32, c->cap_slab32, c->ewald ? 1 : 3 * ((size_t)c->ntri + c->nwork) * 64))))
         return rc;
     if (!c->d_row_start && (rc = dalloc(c, &c->d_row_start, nb + 1))) return rc;
     if (!c->d_segw && (rc = dalloc(c, &c->d_segw, ns))) return rc;
@@ -1038,6 +1275,45 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     c->epsilon.assign(c->nseg, 1.0e50);
     copy_particles(c, d);
     build_exclusion_pairs(c, d);
+    if (c->flags & TDP_FLAG_EWALD) {
+        c->ewald = true;
+        const char* why = nullptr;
+        const double* b = c->default_box;
+        if (c->method != TDP_PME) why = "the Ewald option needs the PME method";
+        else if (c->nseg != 1) why = "the Ewald option is not available for batches";
+        else if (c->polarization == TDP_EXTRAPOLATED) why = "the Ewald option supports Direct and Mutual polarization";
+        else if (b[1] != 0.0 || b[2] != 0.0 || b[3] != 0.0 || b[5] != 0.0 || b[6] != 0.0 || b[7] != 0.0)
+            why = "the Ewald option needs a rectangular box";
+        for (double v : c->dmol)
+            if (v != 0.0) why = "the Ewald option supports charges and induced dipoles only (molecular dipoles must be zero)";
+        if (!why) {
+            double a;
+            int g[3];
+            pme_parameters(c, a, g);
+            c->ew_alpha = a;
+            for (int k = 0; k < 3; k++) {
+                c->ew_K[k] = g[k];
+                if (g[k] < kEwOrder || !fft_plan(g[k], c->ew_radix[k], c->ew_nst[k]))
+                    why = "the Ewald option needs PME grid sizes of at least 5 with prime factors 2, 3, 5 and 7 only";
+            }
+            if (!(a > 0.0)) why = "the Ewald option needs a positive Ewald parameter";
+        }
+        if (why) {
+            c->err = std::string("TholeDipoleForce: ") + why;
+            return bail(TDP_ERR_INVALID_ARGUMENT);
+        }
+        c->ew_qsum = 0.0;
+        for (double q : c->charge) c->ew_qsum += q;
+        // FP64 radius of the mixed real-space passes: where a u^3 reaches 4 for the strongest damping
+        // (u = r / (a_i a_j)), and at least 0.25 nm
+        double amax = 0.0, amin = 1e300;
+        for (double a : c->thole) {
+            amax = std::max(amax, a);
+            if (a > 0.0) amin = std::min(amin, a);
+        }
+        c->ew_r64 = 0.25;
+        if (amax > 0.0) c->ew_r64 = std::max(0.25, std::cbrt(4.0 / amin) * amax * amax);
+    }
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         c->err = "hipStreamCreate failed";
         return bail(TDP_ERR_HIP);
@@ -1086,8 +1362,33 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
         (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)) ||
         (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)) ||
-        (rc = dalloc(c, &c->d_tp, c->ntri)))
+        (rc = dalloc(c, &c->d_tp, c->ewald ? 1 : c->ntri)))
         return bail(rc);
+    if (c->ewald) {
+        const size_t ng = (size_t)c->ew_K[0] * c->ew_K[1] * c->ew_K[2];
+        if ((rc = dalloc(c, &c->d_nbr, (size_t)nb * nb)) || (rc = dalloc(c, &c->d_nbr_cnt, nb)) ||
+            (rc = dalloc(c, &c->d_qfix, ng)) || (rc = dalloc(c, &c->d_cgrid, ng)) || (rc = dalloc(c, &c->d_phiq, ng)) ||
+            (rc = dalloc(c, &c->d_phiv, ng)) || (rc = dalloc(c, &c->d_G64, ng)) || (rc = dalloc(c, &c->d_G32, ng)) ||
+            (rc = dalloc(c, &c->d_sc_off, (size_t)npad + 1)) || (rc = dalloc(c, &c->d_epart, npad)) ||
+            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, npad)))
+            return bail(rc);
+        for (int k = 0; k < 3; k++) {
+            const int K = c->ew_K[k];
+            std::vector<double2> t64(K);
+            std::vector<float2> t32(K);
+            for (int q = 0; q < K; q++) {
+                const double ang = -2.0 * M_PI * q / K;
+                t64[q] = make_double2(std::cos(ang), std::sin(ang));
+                t32[q] = make_float2((float)t64[q].x, (float)t64[q].y);
+            }
+            if ((rc = dalloc(c, &c->d_tw64[k], K)) || (rc = dalloc(c, &c->d_tw32[k], K))) return bail(rc);
+            if (hipMemcpy(c->d_tw64[k], t64.data(), K * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+                hipMemcpy(c->d_tw32[k], t32.data(), K * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+                c->err = "hipMemcpy of the FFT twiddles failed";
+                return bail(TDP_ERR_HIP);
+            }
+        }
+    }
     if (c->polarization == TDP_EXTRAPOLATED) {
         static const double opt4[4] = {-0.154, 0.017, 0.658, 0.474};   // TholeDipoleForce.cpp default
         std::vector<double> coef = c->extrap;
@@ -1457,7 +1758,15 @@ int tdp_copy_parameters(tdp_context* c, const tdp_force_desc* d) {
     if (rc) return rc;
     TDP_HIP(c, hipSetDevice(c->device));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
+    if (c->ewald)
+        for (int k = 0; k < 3 * d->num_particles; k++)
+            if (d->dipoles[k] != 0.0)
+                return fail(c, TDP_ERR_INVALID_ARGUMENT,
+                            "TholeDipoleForce: the Ewald option supports charges and induced dipoles only (molecular "
+                            "dipoles must be zero)");
     copy_particles(c, d);
+    c->ew_qsum = 0.0;
+    for (double q : c->charge) c->ew_qsum += q;
     if ((rc = upload_tables(c))) return rc;  // axis atoms may have changed: torque gather lists
     return upload_particles(c);
 }
@@ -1508,6 +1817,7 @@ int tdp_execute_device_noalloc(tdp_context* c, const double* d_positions, const 
     if (!c) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null context");
     if (c->n > 0 && !d_positions) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null positions");
     if (c->timing) return fail(c, TDP_ERR_INVALID_ARGUMENT, "allocation-free evaluation: disable timing first");
+    if (c->ewald) return fail(c, TDP_ERR_INVALID_ARGUMENT, "allocation-free evaluation: not available with the Ewald option");
     int rc = check_box(c, box);
     if (rc) return rc;
     hipSetDevice(c->device);
@@ -1569,6 +1879,46 @@ int tdp_execute_device_noalloc(tdp_context* c, const double* d_positions, const 
     return convergence_status(c);
 }
 
+int tdp_get_neighbor_pairs(tdp_context* c, const double* positions, const double* box, int64_t capacity,
+                           int32_t* pairs, int64_t* count) {
+    if (!c || !count || (!positions && c->n > 0) || capacity < 0 || (capacity > 0 && !pairs))
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->ewald) return fail(c, TDP_ERR_INVALID_ARGUMENT, "neighbour pairs: only the Ewald option has a cutoff");
+    int rc = check_box(c, box);
+    if (rc) return rc;
+    hipSetDevice(c->device);
+    *count = 0;
+    if (c->n == 0) return TDP_OK;
+    if ((rc = upload_positions(c, positions))) return rc;
+    hipStream_t s = c->stream;
+    if (!c->ordered || c->evals_since_order >= kReorderInterval) {
+        TDP_HIP(c, hipStreamSynchronize(s));
+        if ((rc = reorder(c, positions, box))) return rc;
+    }
+    PairLaunch L = pair_launch(c, box);
+    launch_load_positions(s, c->npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    const EwLaunch E = ew_launch(c, box);
+    launch_ew_neighbors(s, E);
+    int* d_pairs = nullptr;
+    unsigned long long* d_count = nullptr;
+    if ((rc = dalloc(c, &d_pairs, 2 * (size_t)std::max<int64_t>(capacity, 1))) || (rc = dalloc(c, &d_count, 1))) return rc;
+    launch_ew_dump_pairs(s, E, c->posa, c->d_perm, d_pairs, capacity, d_count);
+    unsigned long long n = 0;
+    TDP_HIP(c, hipMemcpyAsync(&n, d_count, sizeof n, hipMemcpyDeviceToHost, s));
+    TDP_HIP(c, hipStreamSynchronize(s));
+    if (capacity > 0)
+        TDP_HIP(c, hipMemcpy(pairs, d_pairs, 2 * sizeof(int32_t) * (size_t)std::min<int64_t>((int64_t)n, capacity),
+                             hipMemcpyDeviceToHost));
+    for (void* p : {(void*)d_pairs, (void*)d_count}) {
+        auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
+        if (it != c->allocations.end()) c->allocations.erase(it);
+        hipFree(p);
+    }
+    *count = (int64_t)n;
+    return TDP_OK;
+}
+
 int tdp_get_dipoles(tdp_context* c, const double* positions, const double* box, int32_t kind, double* out) {
     if (!c || !out || (!positions && c->n > 0)) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
     if (kind < 0 || kind > 2) return fail(c, TDP_ERR_INVALID_ARGUMENT, "unknown dipole kind");
@@ -1599,6 +1949,8 @@ int tdp_get_electrostatic_potential(tdp_context* c, const double* positions, con
     if (!c || (!positions && c->n > 0) || (num_grid > 0 && (!grid || !out)) || num_grid < 0)
         return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
     if (c->nseg != 1) return fail(c, TDP_ERR_INVALID_ARGUMENT, "getElectrostaticPotential: not available for a batch");
+    if (c->ewald)
+        return fail(c, TDP_ERR_INVALID_ARGUMENT, "getElectrostaticPotential: not available with the Ewald option");
     int rc = check_box(c, box);
     if (rc) return rc;
     hipSetDevice(c->device);
@@ -1662,17 +2014,9 @@ int tdp_get_pme_parameters(const tdp_context* cc, double* alpha, int32_t* nx, in
     if (!c || !alpha || !nx || !ny || !nz) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
     if (c->method != TDP_PME)
         return fail(c, TDP_ERR_NOT_PME, "getPMEParametersInContext: This Context is not using PME");
-    double a = c->pme_alpha;
-    int g[3] = {c->grid[0], c->grid[1], c->grid[2]};
-    if (g[0] == 0 || a == 0.0) {
-        // The reference asks OpenMM's NonbondedForceImpl::calcPMEParameters (ReferenceTholeDipoleKernels.cpp:143-152)
-        // for a fresh NonbondedForce with this tolerance and cutoff; OpenMM's published closed form:
-        // alpha = sqrt(-ln(2 tol))/cutoff, n_d = max(6, ceil(2 alpha L_d / (3 tol^(1/5)))).
-        double tol = c->ewald_tol;
-        a = std::sqrt(-std::log(2.0 * tol)) / c->cutoff;
-        const double L[3] = {c->default_box[0], c->default_box[4], c->default_box[8]};
-        for (int k = 0; k < 3; k++) g[k] = std::max(6, (int)std::ceil(2 * a * L[k] / (3 * std::pow(tol, 0.2))));
-    }
+    double a;
+    int g[3];
+    pme_parameters(c, a, g);
     *alpha = a;
     *nx = g[0];
     *ny = g[1];
@@ -1739,7 +2083,7 @@ int tdp_autotune(tdp_context* c, const double* positions, const double* box, int
     int rc = check_box(c, box);
     if (rc) return rc;
     hipSetDevice(c->device);
-    if (c->n > 0) {
+    if (c->n > 0 && !c->ewald) {   // (the Ewald option's passes have no work-item schedule)
         if ((rc = upload_positions(c, positions))) return rc;
         if ((rc = prepare_field(c, box, positions))) return rc;   // internal order, tile tables, mu = alpha E0
         int longest = 1;

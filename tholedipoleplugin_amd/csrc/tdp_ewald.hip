@@ -1,0 +1,923 @@
+// tdp_ewald.hip -- the Ewald option of the periodic method on gfx950: charges and induced dipoles
+// with Thole damping summed over all periodic images.
+//
+// The reference's PME method is the minimum image over all pairs
+// (ReferenceTholeDipoleForce.cpp:1375-1423); this option keeps its pair terms (fixed field :678-733,
+// induced field :766-818, Thole damping :646-676, scale maps :609-644) and sums them by Ewald
+// splitting instead:
+//   real space     pairs whose minimum-image distance is below the cutoff, over neighbour tiles:
+//                  every tile row I owns its particles' sums (full neighbour list, no j-side
+//                  partials), four waves share a row and add their sums in a fixed order;
+//   reciprocal     smooth PME of order 5: charges and dipoles spread into a fixed-point int64 grid
+//                  (scale 2^40, order-independent and therefore deterministic), a 3D FFT by
+//                  Stockham passes in LDS, the influence function folded into the x pass, and the
+//                  potential gathered with B-spline derivatives in the epilogue of the real-space
+//                  kernel;
+//   corrections    the Ewald self-field and self-energy, and the pairs with a scale below 1 (masked
+//                  out of the neighbour pass) evaluated per slot in FP64 from their own list with
+//                  (scale * lambda - 1) times the bare kernel added to the screened one.
+// Screened radial functions B_0 = erfc(a r)/r, B_n = ((2n-1) B_{n-1} + (2a^2)^n/(a sqrt(pi))
+// e^{-a^2 r^2}) / r^2; damped kernels C_n = B_n + (lambda_{2n+1} - 1) b_n with b = 1/r, 1/r^3, 3/r^5,
+// 15/r^7 and the reference's Thole factors lambda3 = 1 - e^s, lambda5 = 1 - (1-s) e^s plus
+// lambda7 = 1 - (1 - s + 3/5 s^2) e^s for the gradient of the dipole field.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "tdp_device.hpp"
+#include "tdp_kernels.h"
+
+namespace tdp {
+
+namespace {
+
+constexpr int kEwWaves = 8;                  // waves per tile row in the real-space kernel
+constexpr double kFix = 1099511627776.0;     // 2^40: fixed-point scale of the spreading grid
+constexpr double kInvFix = 1.0 / 1099511627776.0;
+
+// ------------------------------------------------------------------ exact FP64 pair geometry
+// The neighbour criterion is evaluated with correctly rounded operations in a fixed order and
+// no contraction, so that the host can reproduce it bit for bit: d = x_i - x_j, then per axis
+// d -= L * floor(d / L + 0.5), r2 = (dx^2 + dy^2) + dz^2, inside = r2 < rc^2.
+__device__ __forceinline__ double ew_image(double d, double L) {
+    return __dsub_rn(d, __dmul_rn(L, floor(__dadd_rn(__ddiv_rn(d, L), 0.5))));
+}
+__device__ __forceinline__ double ew_r2(double dx, double dy, double dz) {
+    return __dadd_rn(__dadd_rn(__dmul_rn(dx, dx), __dmul_rn(dy, dy)), __dmul_rn(dz, dz));
+}
+
+// B-spline weights of order 5 for grid coordinate u: th[j] = M_5(w + 4 - j) at grid point
+// floor(u) - 4 + j, first and second derivatives with respect to u.
+template <typename T>
+__device__ __forceinline__ void ew_bspline(T w, T th[kEwOrder], T d1[kEwOrder], T d2[kEwOrder]) {
+    T m1[kEwOrder], m2[kEwOrder], m3[kEwOrder];
+    // levels 2..5 of M_n(x) = (x M_{n-1}(x) + (n - x) M_{n-1}(x-1)) / (n-1), index j <-> M_n(w + n-1-j)
+    m1[0] = T(1) - w;   // n = 2
+    m1[1] = w;
+    // n = 3
+    m2[0] = T(0.5) * (T(1) - w) * m1[0];
+    m2[1] = T(0.5) * ((w + T(1)) * m1[0] + (T(2) - w) * m1[1]);
+    m2[2] = T(0.5) * w * m1[1];
+    // n = 4
+    const T t3 = T(1) / T(3);
+    m3[0] = t3 * (T(1) - w) * m2[0];
+    m3[1] = t3 * ((w + T(2)) * m2[0] + (T(2) - w) * m2[1]);
+    m3[2] = t3 * ((w + T(1)) * m2[1] + (T(3) - w) * m2[2]);
+    m3[3] = t3 * w * m2[2];
+    // n = 5
+    th[0] = T(0.25) * (T(1) - w) * m3[0];
+    th[1] = T(0.25) * ((w + T(3)) * m3[0] + (T(2) - w) * m3[1]);
+    th[2] = T(0.25) * ((w + T(2)) * m3[1] + (T(3) - w) * m3[2]);
+    th[3] = T(0.25) * ((w + T(1)) * m3[2] + (T(4) - w) * m3[3]);
+    th[4] = T(0.25) * w * m3[3];
+    // M_5' = M_4(x) - M_4(x-1): d1[j] = m3[j-1] - m3[j]
+    d1[0] = -m3[0];
+    d1[1] = m3[0] - m3[1];
+    d1[2] = m3[1] - m3[2];
+    d1[3] = m3[2] - m3[3];
+    d1[4] = m3[3];
+    // M_5'' = M_3(x) - 2 M_3(x-1) + M_3(x-2): d2[j] = m2[j-2] - 2 m2[j-1] + m2[j]
+    d2[0] = m2[0];
+    d2[1] = m2[1] - T(2) * m2[0];
+    d2[2] = m2[2] - T(2) * m2[1] + m2[0];
+    d2[3] = -T(2) * m2[2] + m2[1];
+    d2[4] = m2[2];
+}
+
+// Grid coordinate of one axis: u = K * frac(x / L) in [0, K), base grid index floor(u) - 4.
+__device__ __forceinline__ void ew_grid_coord(double x, double L, int K, int& base, double& w) {
+    double f = x / L;
+    f -= floor(f);
+    const double u = f * K;
+    int iu = (int)floor(u);
+    w = u - iu;
+    if (iu >= K) iu -= K;
+    if (!(iu >= 0 && iu < K)) {   // a coordinate that is not finite: keep the indices in the grid
+        iu = 0;
+        w = 0.0;
+    }
+    base = iu - (kEwOrder - 1);
+}
+
+__device__ __forceinline__ int ew_wrap(int g, int K) { return g < 0 ? g + K : (g >= K ? g - K : g); }
+
+template <typename T> struct Cplx;
+template <> struct Cplx<double> { using type = double2; };
+template <> struct Cplx<float> { using type = float2; };
+
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+    C r;
+    r.x = a.x * b.x - a.y * b.y;
+    r.y = a.x * b.y + a.y * b.x;
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ typename Cplx<T>::type twiddle(const EwAxis& A, int q, bool inverse);
+template <>
+__device__ __forceinline__ double2 twiddle<double>(const EwAxis& A, int q, bool inverse) {
+    double2 w = A.tw64[q];
+    if (inverse) w.y = -w.y;
+    return w;
+}
+template <>
+__device__ __forceinline__ float2 twiddle<float>(const EwAxis& A, int q, bool inverse) {
+    float2 w = A.tw32[q];
+    if (inverse) w.y = -w.y;
+    return w;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ neighbour tiles
+// Tile row I lists every tile J whose bounding box (minimum image of the centres, FP32 with a
+// margin) comes within the cutoff of I's: a superset of the tiles holding a pair within the cutoff.
+__global__ __launch_bounds__(64) void k_ew_neighbors(EwLaunch E) {
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const float4 cI = E.tb_c[I], hI = E.tb_h[I];
+    const float bx = (float)E.L[0], by = (float)E.L[1], bz = (float)E.L[2];
+    int count = 0;
+    for (int J0 = 0; J0 < E.nblk; J0 += kTile) {
+        const int J = J0 + lane;
+        bool ok = false;
+        if (J < E.nblk && E.tcnt[J] > 0) {
+            const float4 cJ = E.tb_c[J], hJ = E.tb_h[J];
+            float dx = cJ.x - cI.x, dy = cJ.y - cI.y, dz = cJ.z - cI.z;
+            dx -= bx * rintf(dx / bx);
+            dy -= by * rintf(dy / by);
+            dz -= bz * rintf(dz / bz);
+            dx = fmaxf(0.f, fabsf(dx) - hI.x - hJ.x);
+            dy = fmaxf(0.f, fabsf(dy) - hI.y - hJ.y);
+            dz = fmaxf(0.f, fabsf(dz) - hI.z - hJ.z);
+            ok = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
+        }
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (ok) E.nbr[(size_t)I * E.nblk + count + before] = J;
+        count += __popcll(m);
+    }
+    if (lane == 0) E.nbr_cnt[I] = count;
+}
+
+// ------------------------------------------------------------------ spreading
+// One thread per slot: q theta_x theta_y theta_z + sum_d v_d (K_d/L_d) theta'_d ... into the
+// fixed-point grid.  Slots with neither charge nor dipole add nothing.
+__global__ __launch_bounds__(64) void k_ew_spread(EwLaunch E, const double4* __restrict__ posa,
+                                                  const double* __restrict__ charge, const double4* __restrict__ vec,
+                                                  const double* __restrict__ scalars) {
+    if (scalars && scalars[SC_DONE] != 0.0) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= E.npad) return;
+    const double q = charge ? charge[i] : 0.0;
+    const double4 v = vec ? vec[i] : make_double4(0.0, 0.0, 0.0, 0.0);
+    if (q == 0.0 && v.x == 0.0 && v.y == 0.0 && v.z == 0.0) return;
+    const double4 p = posa[i];
+    const double x[3] = {p.x, p.y, p.z};
+    int base[3];
+    double th[3][kEwOrder], d1[3][kEwOrder], d2[kEwOrder];
+    for (int k = 0; k < 3; k++) {
+        double w;
+        ew_grid_coord(x[k], E.L[k], E.K[k], base[k], w);
+        ew_bspline<double>(w, th[k], d1[k], d2);
+    }
+    const double vx = v.x * E.K[0] / E.L[0], vy = v.y * E.K[1] / E.L[1], vz = v.z * E.K[2] / E.L[2];
+    for (int a = 0; a < kEwOrder; a++) {
+        const int gx = ew_wrap(base[0] + a, E.K[0]);
+        for (int b = 0; b < kEwOrder; b++) {
+            const int gy = ew_wrap(base[1] + b, E.K[1]);
+            const double txy = th[0][a] * th[1][b];
+            const double cq = q * txy + vx * d1[0][a] * th[1][b] + vy * th[0][a] * d1[1][b];
+            const double cz = vz * txy;
+            unsigned long long* row = E.qfix + ((size_t)gx * E.K[1] + gy) * E.K[2];
+            for (int c = 0; c < kEwOrder; c++) {
+                const int gz = ew_wrap(base[2] + c, E.K[2]);
+                const double val = cq * th[2][c] + cz * d1[2][c];
+                atomicAdd(row + gz, (unsigned long long)__double2ll_rn(val * kFix));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ FFT passes
+// Batched 1D FFTs along one axis of the K0 x K1 x K2 grid (z fastest), LB lines per workgroup in
+// LDS, Stockham radix stages (autosorting, ping-pong buffers).  Stage with radix R after the
+// stages so far of total span Ns: butterfly j takes x[j + r n/R] times w^(r (j mod Ns) n/(Ns R)),
+// forms the R-point DFT and writes it to (j div Ns) Ns R + (j mod Ns) + k Ns.
+// Pass kinds: 0 z forward from the fixed-point grid (which it clears), 1 y forward, 2 x forward,
+// influence function, x inverse, 3 y inverse, 4 z inverse to the real potential grid.
+// One radix-R stage of the lines in LDS (R known at compile time, so the butterfly stays in registers).
+template <typename T, int R>
+__device__ __forceinline__ void ew_fft_stage(const typename Cplx<T>::type* __restrict__ src,
+                                             typename Cplx<T>::type* __restrict__ dst, int LB, int n, int Ns,
+                                             const EwAxis& A, bool inverse) {
+    using C = typename Cplx<T>::type;
+    const int nb = n / R;
+    const int span = n / (Ns * R);
+    for (int idx = threadIdx.x; idx < LB * nb; idx += blockDim.x) {
+        const int line = idx / nb, j = idx - line * nb;
+        const C* x = src + line * n;
+        C* y = dst + line * n;
+        const int k = j % Ns;
+        const int tstep = k * span;
+        C v[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            C t = x[j + r * nb];
+            if (r > 0 && tstep) t = cmul(t, twiddle<T>(A, (r * tstep) % n, inverse));
+            v[r] = t;
+        }
+        const int base = (j / Ns) * Ns * R + k;
+#pragma unroll
+        for (int kk = 0; kk < R; kk++) {
+            C acc = v[0];
+#pragma unroll
+            for (int r = 1; r < R; r++) {
+                const C t = cmul(v[r], twiddle<T>(A, ((r * kk) % R) * nb, inverse));
+                acc.x += t.x;
+                acc.y += t.y;
+            }
+            y[base + kk * Ns] = acc;
+        }
+    }
+}
+
+template <typename T>
+__device__ void ew_fft_lines(typename Cplx<T>::type* a, typename Cplx<T>::type* b, int LB, const EwAxis& A,
+                             bool inverse, typename Cplx<T>::type*& out) {
+    using C = typename Cplx<T>::type;
+    const int n = A.n;
+    int Ns = 1;
+    C* src = a;
+    C* dst = b;
+    for (int st = 0; st < A.nst; st++) {
+        const int R = A.r[st];
+        switch (R) {
+            case 2: ew_fft_stage<T, 2>(src, dst, LB, n, Ns, A, inverse); break;
+            case 3: ew_fft_stage<T, 3>(src, dst, LB, n, Ns, A, inverse); break;
+            case 4: ew_fft_stage<T, 4>(src, dst, LB, n, Ns, A, inverse); break;
+            case 5: ew_fft_stage<T, 5>(src, dst, LB, n, Ns, A, inverse); break;
+            default: ew_fft_stage<T, 7>(src, dst, LB, n, Ns, A, inverse); break;
+        }
+        __syncthreads();
+        C* t = src;
+        src = dst;
+        dst = t;
+        Ns *= R;
+    }
+    out = src;
+}
+
+template <typename T, int KIND>
+__global__ __launch_bounds__(256) void k_ew_fft(EwLaunch E, int LB, void* phi_out, const double* __restrict__ scalars) {
+    using C = typename Cplx<T>::type;
+    if (scalars && scalars[SC_DONE] != 0.0) return;
+    extern __shared__ unsigned char smem[];
+    const int axis = (KIND == 0 || KIND == 4) ? 2 : ((KIND == 1 || KIND == 3) ? 1 : 0);
+    const EwAxis& A = E.ax[axis];
+    const int n = A.n;
+    const int K1 = E.K[1], K2 = E.K[2];
+    const int nlines = axis == 2 ? E.K[0] * K1 : (axis == 1 ? E.K[0] * K2 : K1 * K2);
+    const size_t estride = axis == 2 ? 1 : (axis == 1 ? (size_t)K2 : (size_t)K1 * K2);
+    C* bufA = reinterpret_cast<C*>(smem);
+    C* bufB = bufA + LB * n;
+    C* grid = reinterpret_cast<C*>(E.cgrid);
+    const int L0 = blockIdx.x * LB;
+    const int lb = min(LB, nlines - L0);
+    auto line_base = [&](int Lidx) -> size_t {
+        if (axis == 2) return (size_t)Lidx * K2;
+        if (axis == 1) return (size_t)(Lidx / K2) * K1 * K2 + (Lidx % K2);
+        return (size_t)Lidx;
+    };
+    // load (z lines: elements of a line are contiguous; y and x lines: consecutive lines are)
+    for (int idx = threadIdx.x; idx < lb * n; idx += blockDim.x) {
+        int b, e;
+        if (axis == 2) { b = idx / n; e = idx - b * n; }
+        else { e = idx / lb; b = idx - e * lb; }
+        const size_t g = line_base(L0 + b) + (size_t)e * estride;
+        C v;
+        if (KIND == 0) {
+            const long long f = (long long)E.qfix[g];
+            v.x = (T)((double)f * kInvFix);
+            v.y = T(0);
+            E.qfix[g] = 0ull;
+        } else {
+            v = grid[g];
+        }
+        bufA[b * n + e] = v;
+    }
+    __syncthreads();
+    C* res;
+    ew_fft_lines<T>(bufA, bufB, lb, A, KIND >= 3, res);
+    if (KIND == 2) {   // influence function, then the inverse transform along x
+        for (int idx = threadIdx.x; idx < lb * n; idx += blockDim.x) {
+            const int e = idx / lb, b = idx - e * lb;
+            const size_t g = line_base(L0 + b) + (size_t)e * estride;
+            const T gf = E.mixed ? (T)E.G32[g] : (T)E.G64[g];
+            C v = res[b * n + e];
+            v.x *= gf;
+            v.y *= gf;
+            res[b * n + e] = v;
+        }
+        __syncthreads();
+        C* other = res == bufA ? bufB : bufA;
+        ew_fft_lines<T>(res, other, lb, A, true, res);
+    }
+    for (int idx = threadIdx.x; idx < lb * n; idx += blockDim.x) {
+        int b, e;
+        if (axis == 2) { b = idx / n; e = idx - b * n; }
+        else { e = idx / lb; b = idx - e * lb; }
+        const size_t g = line_base(L0 + b) + (size_t)e * estride;
+        if (KIND == 4) reinterpret_cast<T*>(phi_out)[g] = res[b * n + e].x;
+        else grid[g] = res[b * n + e];
+    }
+}
+
+// ------------------------------------------------------------------ reciprocal gather
+// Potential derivatives at slot x from a potential grid: g1 = grad_u phi (3), g2 = second
+// derivatives in u (xx, yy, zz, xy, xz, yz), phi itself.  NEED: 0 phi and g1, 1 also g2.
+template <typename T, int NEED>
+__device__ __forceinline__ void ew_gather(const EwLaunch& E, const T* __restrict__ phi, double4 p, double& ph,
+                                          double g1[3], double g2[6]) {
+    int base[3];
+    double th[3][kEwOrder], d1[3][kEwOrder], d2[3][kEwOrder];
+    const double x[3] = {p.x, p.y, p.z};
+    for (int k = 0; k < 3; k++) {
+        double w;
+        ew_grid_coord(x[k], E.L[k], E.K[k], base[k], w);
+        ew_bspline<double>(w, th[k], d1[k], d2[k]);
+    }
+    ph = 0.0;
+    for (int k = 0; k < 3; k++) g1[k] = 0.0;
+    for (int k = 0; k < 6; k++) g2[k] = 0.0;
+#pragma unroll 1
+    for (int a = 0; a < kEwOrder; a++) {
+        const int gx = ew_wrap(base[0] + a, E.K[0]);
+#pragma unroll 1
+        for (int b = 0; b < kEwOrder; b++) {
+            const int gy = ew_wrap(base[1] + b, E.K[1]);
+            const T* row = phi + ((size_t)gx * E.K[1] + gy) * E.K[2];
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0;   // sum_c v th_c, v d1_c, v d2_c
+            for (int c = 0; c < kEwOrder; c++) {
+                const double v = (double)row[ew_wrap(base[2] + c, E.K[2])];
+                s0 += v * th[2][c];
+                s1 += v * d1[2][c];
+                if (NEED) s2 += v * d2[2][c];
+            }
+            const double txy = th[0][a] * th[1][b];
+            ph += txy * s0;
+            g1[0] += d1[0][a] * th[1][b] * s0;
+            g1[1] += th[0][a] * d1[1][b] * s0;
+            g1[2] += txy * s1;
+            if (NEED) {
+                g2[0] += d2[0][a] * th[1][b] * s0;
+                g2[1] += th[0][a] * d2[1][b] * s0;
+                g2[2] += txy * s2;
+                g2[3] += d1[0][a] * d1[1][b] * s0;
+                g2[4] += d1[0][a] * th[1][b] * s1;
+                g2[5] += th[0][a] * d1[1][b] * s1;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ real space
+// Minimum image with the bit-exact result of d - L floor(d / L + 0.5): floor(d (1/L) + 0.5) is the
+// same integer unless d/L + 0.5 lies within 1e-12 of one, where the correctly rounded division
+// decides.
+__device__ __forceinline__ double ew_image_fast(double d, double L, double iL) {
+    const double t = d * iL + 0.5;
+    double n = floor(t);
+    if (fabs(t - rint(t)) < 1e-12) n = floor(__dadd_rn(__ddiv_rn(d, L), 0.5));
+    return __dsub_rn(d, __dmul_rn(L, n));
+}
+
+// Screened and damped radial coefficients of one pair (B_0..B_3, bare b_1..b_3, Thole factors).
+template <typename T>
+struct EwR {
+    T B0, B1, B2, B3, b1, b2, b3, l3, l5, l7;
+};
+
+// FP64: erfc and the exponentials of the math library.
+__device__ __forceinline__ void ew_radial(const EwLaunch& E, double r2, double ai, double aj, double iai, double iaj,
+                                          bool inside, bool need_l7, EwR<double>& R) {
+    const double ri = rsqrt(r2);
+    const double r = r2 * ri;
+    const double ri2 = ri * ri;
+    R.b1 = ri * ri2;
+    R.b2 = 3.0 * R.b1 * ri2;
+    R.b3 = 5.0 * R.b2 * ri2;
+    if (inside) {
+        const double ex = exp(-E.alpha * E.alpha * r2);
+        R.B0 = erfc(E.alpha * r) * ri;
+        R.B1 = (R.B0 + E.c1 * ex) * ri2;
+        R.B2 = (3.0 * R.B1 + E.c2 * ex) * ri2;
+        R.B3 = (5.0 * R.B2 + E.c3 * ex) * ri2;
+    } else {
+        R.B0 = R.B1 = R.B2 = R.B3 = 0.0;
+    }
+    R.l3 = R.l5 = R.l7 = 1.0;
+    const double g = ai < aj ? ai : aj;
+    const double u = r * (iai * iaj);
+    const double s = -g * (u * u * u);
+    if (ai * aj != 0.0 && s > -50.0) {
+        const double es = exp(s);
+        R.l3 = 1.0 - es;
+        R.l5 = 1.0 - (1.0 - s) * es;
+        if (need_l7) R.l7 = 1.0 - (1.0 - s + 0.6 * s * s) * es;
+    }
+}
+
+// FP32 (pairs inside the cutoff): erfc(x) by Abramowitz & Stegun 7.1.26 (Hastings),
+// t = 1/(1 + p x), erfc = t (a1 + t (a2 + t (a3 + t (a4 + t a5)))) e^{-x^2}, absolute error below
+// 1.5e-7, sharing e^{-x^2} with B_1..B_3; the Thole exponential only where some lane is damped.
+__device__ __forceinline__ void ew_radial32(float alpha, float c1, float c2, float c3, float r2, float ai, float aj,
+                                            float iai, float iaj, bool need_l7, EwR<float>& R) {
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float r = r2 * ri;
+    const float ri2 = ri * ri;
+    R.b1 = ri * ri2;
+    R.b2 = 3.f * R.b1 * ri2;
+    R.b3 = 5.f * R.b2 * ri2;
+    const float x = alpha * r;
+    const float ex = expf(-x * x);
+    const float t = __builtin_amdgcn_rcpf(1.f + 0.3275911f * x);
+    const float poly = t * (0.254829592f + t * (-0.284496736f + t * (1.421413741f + t * (-1.453152027f + t * 1.061405429f))));
+    R.B0 = poly * ex * ri;
+    R.B1 = (R.B0 + c1 * ex) * ri2;
+    R.B2 = (3.f * R.B1 + c2 * ex) * ri2;
+    R.B3 = (5.f * R.B2 + c3 * ex) * ri2;
+    R.l3 = R.l5 = R.l7 = 1.f;
+    const float g = ai < aj ? ai : aj;
+    const float u = r * (iai * iaj);
+    const float s = -g * (u * u * u);
+    const bool damped = ai * aj != 0.f && s > -50.f;
+    if (__builtin_amdgcn_ballot_w64(damped)) {
+        const float es = damped ? __expf(s) : 0.f;
+        R.l3 = 1.f - es;
+        R.l5 = 1.f - (1.f - s) * es;
+        if (need_l7) R.l7 = 1.f - (1.f - s + 0.6f * s * s) * es;
+    }
+}
+
+// The i-side terms of one unscaled pair inside the cutoff (d = x_i - x_j): the field of charge qj
+// (E0), the field of dipole vj (T v), or the force on i and half the pair's charge-charge energy.
+template <typename T, int MODE, bool MUTUAL>
+__device__ __forceinline__ void ew_pair_terms(const EwR<T>& R, T dx, T dy, T dz, T qi, T vix, T viy, T viz, T qj,
+                                              T vjx, T vjy, T vjz, T& sx, T& sy, T& sz, T& se) {
+    const T C1 = R.B1 + (R.l3 - T(1)) * R.b1;
+    const T C2 = R.B2 + (R.l5 - T(1)) * R.b2;
+    if (MODE == kEwE0) {
+        const T f = qj * C1;
+        sx += f * dx;
+        sy += f * dy;
+        sz += f * dz;
+    } else if (MODE == kEwTmu) {
+        const T vd = vjx * dx + vjy * dy + vjz * dz;
+        sx += C2 * vd * dx - C1 * vjx;
+        sy += C2 * vd * dy - C1 * vjy;
+        sz += C2 * vd * dz - C1 * vjz;
+    } else {
+        // g = grad_d U; the force on i is -g
+        const T qq = qi * qj * R.B1;
+        T gx = -qq * dx, gy = -qq * dy, gz = -qq * dz;
+        const T ux = qi * vjx - qj * vix, uy = qi * vjy - qj * viy, uz = qi * vjz - qj * viz;
+        const T ud = ux * dx + uy * dy + uz * dz;
+        gx += C1 * ux - C2 * ud * dx;
+        gy += C1 * uy - C2 * ud * dy;
+        gz += C1 * uz - C2 * ud * dz;
+        if (MUTUAL) {
+            const T C3 = R.B3 + (R.l7 - T(1)) * R.b3;
+            const T mid = vix * dx + viy * dy + viz * dz;
+            const T mjd = vjx * dx + vjy * dy + vjz * dz;
+            const T mm = vix * vjx + viy * vjy + viz * vjz;
+            const T cd = -C2 * mm + C3 * mid * mjd;
+            gx += cd * dx - C2 * (vix * mjd + vjx * mid);
+            gy += cd * dy - C2 * (viy * mjd + vjy * mid);
+            gz += cd * dz - C2 * (viz * mjd + vjz * mid);
+        }
+        sx -= gx;
+        sy -= gy;
+        sz -= gz;
+        se += T(0.5) * qi * qj * R.B0;
+    }
+}
+
+// Row-owner real-space pass over the neighbour tiles of tile row I: one workgroup of kEwWaves
+// waves per row, the tiles dealt round-robin to the waves, each wave staging its own tile in its
+// own LDS slice (no workgroup barrier until the row's sums are added).  In a staged tile only the
+// slots within the cutoff (plus a margin) of row I's bounding box are visited, and a step whose 64
+// pairs all lie beyond the cutoff is skipped.  MIXED: pairs beyond E.r64sq are evaluated in FP32
+// (tile sums in FP32, added to the FP64 row sums); the steps that hold a closer pair are marked and
+// evaluated afterwards in FP64 for those pairs.  Output: the slot's real-space sums (field or force,
+// and the charge-charge energy) in rs.
+template <int MODE, bool MUTUAL, bool MIXED>
+__global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const double4* __restrict__ posa,
+                                                            const double* __restrict__ inva,
+                                                            const double* __restrict__ charge,
+                                                            const double4* __restrict__ vec,
+                                                            const double* __restrict__ scalars,
+                                                            double4* __restrict__ rs) {
+    if (scalars && scalars[SC_DONE] != 0.0) return;
+    const int I = blockIdx.x;
+    const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
+    __shared__ double4 s_p[kEwWaves][kTile];
+    __shared__ double4 s_v[kEwWaves][kTile];   // charge in w
+    __shared__ double s_ia[kEwWaves][kTile];
+    const int i = I * kTile + lane;
+    const bool ivalid = lane < E.tcnt[I];
+    const double4 pi = posa[i];
+    const double iai = inva[i];
+    const double qi = charge[i];
+    const double4 vi = (MODE == kEwE0) ? make_double4(0.0, 0.0, 0.0, 0.0) : vec[i];
+    const float4 cI = E.tb_c[I], hI = E.tb_h[I];
+    const float bxf = (float)E.L[0], byf = (float)E.L[1], bzf = (float)E.L[2];
+    const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
+    double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0;
+    const int cnt = E.nbr_cnt[I];
+    const int* nb = E.nbr + (size_t)I * E.nblk;
+    const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
+    for (int t = w; t < cnt; t += kEwWaves) {
+        const int J = nb[t];
+        const int j = J * kTile + lane;
+        const double4 pl = posa[j];
+        {
+            double4 vj = (MODE == kEwE0) ? make_double4(0.0, 0.0, 0.0, 0.0) : vec[j];
+            vj.w = charge[j];
+            s_p[w][lane] = pl;
+            s_v[w][lane] = vj;
+            s_ia[w][lane] = inva[j];
+        }
+        // the slice is this wave's own: its LDS operations complete in order
+        __builtin_amdgcn_wave_barrier();
+        const int cntJ = E.tcnt[J];
+        bool near_box = false;   // slot lane of tile J within the cutoff (plus margin) of row I's box
+        if (lane < cntJ) {
+            float dx = (float)pl.x - cI.x, dy = (float)pl.y - cI.y, dz = (float)pl.z - cI.z;
+            dx -= bxf * rintf(dx / bxf);
+            dy -= byf * rintf(dy / byf);
+            dz -= bzf * rintf(dz / bzf);
+            dx = fmaxf(0.f, fabsf(dx) - hI.x);
+            dy = fmaxf(0.f, fabsf(dy) - hI.y);
+            dz = fmaxf(0.f, fabsf(dz) - hI.z);
+            near_box = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
+        }
+        unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
+        // scaled pairs of this tile pair are masked (the epilogue evaluates them)
+        unsigned long long mask = 0ull;
+        for (int e = eS; e < eE; e++) {
+            if (E.exc_J[e] != J) continue;
+            const unsigned long long* m = E.exc_mask + (size_t)e * 3 * kTile;
+            if (MODE == kEwE0) mask = m[lane] | m[kTile + lane];
+            else if (MODE == kEwTmu) mask = m[2 * kTile + lane];
+            else mask = m[lane] | m[kTile + lane] | m[2 * kTile + lane];
+            break;
+        }
+        if (J == I) mask |= 1ull << lane;
+        if (!ivalid) mask = ~0ull;
+        unsigned long long near_steps = 0ull;
+        if (MIXED) {
+            float fx = 0.f, fy = 0.f, fz = 0.f, fe = 0.f;
+            const float alpha32 = (float)E.alpha, c1f = (float)E.c1, c2f = (float)E.c2, c3f = (float)E.c3;
+            unsigned long long v = visit;
+            while (v) {
+                const int k = __builtin_ctzll(v);
+                v &= v - 1ull;
+                const double4 pj = s_p[w][k];
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const double r2x = ew_r2(dx, dy, dz);
+                const bool inside = !((mask >> k) & 1ull) && r2x < E.rc2;
+                const bool near64 = inside && r2x < E.r64sq;
+                if (__builtin_amdgcn_ballot_w64(near64)) near_steps |= 1ull << k;
+                const bool far32 = inside && !near64;
+                if (!__builtin_amdgcn_ballot_w64(far32)) continue;
+                const double4 vj = s_v[w][k];
+                const float gx = (float)dx, gy = (float)dy, gz = (float)dz;
+                const float r2f = far32 ? gx * gx + gy * gy + gz * gz : 1.f;
+                EwR<float> R;
+                ew_radial32(alpha32, c1f, c2f, c3f, r2f, (float)pi.w, (float)pj.w, (float)iai, (float)s_ia[w][k],
+                            MODE == kEwForce && MUTUAL, R);
+                float tx = 0.f, ty = 0.f, tz = 0.f, te = 0.f;
+                ew_pair_terms<float, MODE, MUTUAL>(R, gx, gy, gz, (float)qi, (float)vi.x, (float)vi.y, (float)vi.z,
+                                                   (float)vj.w, (float)vj.x, (float)vj.y, (float)vj.z, tx, ty, tz, te);
+                if (far32) {
+                    fx += tx;
+                    fy += ty;
+                    fz += tz;
+                    fe += te;
+                }
+            }
+            ax += (double)fx;
+            ay += (double)fy;
+            az += (double)fz;
+            ae += (double)fe;
+        }
+        double sx = 0.0, sy = 0.0, sz = 0.0, se = 0.0;
+        unsigned long long v = MIXED ? near_steps : visit;
+        while (v) {
+            const int k = __builtin_ctzll(v);
+            v &= v - 1ull;
+            const double4 pj = s_p[w][k];
+            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+            const double r2x = ew_r2(dx, dy, dz);
+            bool sel = !((mask >> k) & 1ull) && r2x < E.rc2;
+            if (MIXED) sel = sel && r2x < E.r64sq;
+            if (!__builtin_amdgcn_ballot_w64(sel)) continue;
+            const double4 vj = s_v[w][k];
+            EwR<double> R;
+            ew_radial(E, sel ? r2x : 1.0, pi.w, pj.w, iai, s_ia[w][k], true, MODE == kEwForce && MUTUAL, R);
+            double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
+            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, vj.w, vj.x, vj.y, vj.z, tx, ty,
+                                                tz, te);
+            if (sel) {
+                sx += tx;
+                sy += ty;
+                sz += tz;
+                se += te;
+            }
+        }
+        ax += sx;
+        ay += sy;
+        az += sz;
+        ae += se;
+        __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next tile is staged
+    }
+    // the row's sums over the waves, in wave order (the staging slices are reused)
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(&s_p[0][0]);   // [kEwWaves][4][kTile] doubles: 16 KB of the 32 KB
+    red[(w * 4 + 0) * kTile + lane] = ax;
+    red[(w * 4 + 1) * kTile + lane] = ay;
+    red[(w * 4 + 2) * kTile + lane] = az;
+    red[(w * 4 + 3) * kTile + lane] = ae;
+    __syncthreads();
+    if (w != 0) return;
+    for (int u = 1; u < kEwWaves; u++) {
+        ax += red[(u * 4 + 0) * kTile + lane];
+        ay += red[(u * 4 + 1) * kTile + lane];
+        az += red[(u * 4 + 2) * kTile + lane];
+        ae += red[(u * 4 + 3) * kTile + lane];
+    }
+    rs[i] = make_double4(ax, ay, az, ae);
+}
+
+// Epilogue of a pass for the 64 slots of tile I (one wave per tile): the real-space sums, the
+// scaled pairs of each slot in FP64 with their scales ([inside] B_n + (s lambda - 1) b_n), the
+// reciprocal field (E = -(K/L) grad_u phi) or force (q E + grad E . v) gathered from the potential
+// grids, and the self terms; then the outputs of the mode.
+template <int MODE, bool MUTUAL>
+__global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* __restrict__ posa,
+                                                    const double* __restrict__ inva, const double* __restrict__ charge,
+                                                    const double4* __restrict__ vec, const double4* __restrict__ E0,
+                                                    const double* __restrict__ scalars,
+                                                    const double4* __restrict__ rs, double4* __restrict__ out4, PcgQ pq,
+                                                    double* __restrict__ epart, double* __restrict__ frc,
+                                                    double* __restrict__ eout) {
+    if (scalars && scalars[SC_DONE] != 0.0) return;
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int i = I * kTile + lane;
+    const bool valid = lane < E.tcnt[I];
+    const double4 pi = posa[i];
+    const double iai = inva[i];
+    const double qi = charge[i];
+    const double4 vi = (MODE == kEwE0) ? make_double4(0.0, 0.0, 0.0, 0.0) : vec[i];
+    const double4 r0 = rs[i];
+    double ax = r0.x, ay = r0.y, az = r0.z, ae = r0.w;
+    const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
+    if (valid) {
+        for (int q = E.sc_off[i]; q < E.sc_off[i + 1]; q++) {
+            const int j = E.sc_j[q];
+            const double2 sc = E.sc_s[q];
+            const double m = sc.x, is = sc.y;
+            if (MODE == kEwE0 && m == 1.0) continue;
+            if (MODE == kEwTmu && is == 1.0) continue;
+            const double4 pj = posa[j];
+            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+            const double r2 = ew_r2(dx, dy, dz);
+            const bool inside = r2 < E.rc2;
+            EwR<double> R;
+            ew_radial(E, r2, pi.w, pj.w, iai, inva[j], inside, true, R);
+            const bool cm = inside || m != 1.0, ci = inside || is != 1.0;
+            const double Cqm1 = R.B1 + (cm ? (m * R.l3 - 1.0) * R.b1 : 0.0);
+            const double Cqm2 = R.B2 + (cm ? (m * R.l5 - 1.0) * R.b2 : 0.0);
+            const double Cmm1 = R.B1 + (ci ? (is * R.l3 - 1.0) * R.b1 : 0.0);
+            const double Cmm2 = R.B2 + (ci ? (is * R.l5 - 1.0) * R.b2 : 0.0);
+            const double qj = charge[j];
+            if (MODE == kEwE0) {
+                ax += qj * Cqm1 * dx;
+                ay += qj * Cqm1 * dy;
+                az += qj * Cqm1 * dz;
+            } else if (MODE == kEwTmu) {
+                const double4 vj = vec[j];
+                const double vd = vj.x * dx + vj.y * dy + vj.z * dz;
+                ax += Cmm2 * vd * dx - Cmm1 * vj.x;
+                ay += Cmm2 * vd * dy - Cmm1 * vj.y;
+                az += Cmm2 * vd * dz - Cmm1 * vj.z;
+            } else {
+                const double4 vj = vec[j];
+                const double b0 = R.b1 * r2;   // 1/r
+                const double Cqq0 = R.B0 + (cm ? (m - 1.0) * b0 : 0.0);
+                const double Cqq1 = R.B1 + (cm ? (m - 1.0) * R.b1 : 0.0);
+                double gx = -qi * qj * Cqq1 * dx, gy = -qi * qj * Cqq1 * dy, gz = -qi * qj * Cqq1 * dz;
+                const double ux = qi * vj.x - qj * vi.x, uy = qi * vj.y - qj * vi.y, uz = qi * vj.z - qj * vi.z;
+                const double ud = ux * dx + uy * dy + uz * dz;
+                gx += Cqm1 * ux - Cqm2 * ud * dx;
+                gy += Cqm1 * uy - Cqm2 * ud * dy;
+                gz += Cqm1 * uz - Cqm2 * ud * dz;
+                if (MUTUAL) {
+                    const double Cmm3 = R.B3 + (ci ? (is * R.l7 - 1.0) * R.b3 : 0.0);
+                    const double mid = vi.x * dx + vi.y * dy + vi.z * dz;
+                    const double mjd = vj.x * dx + vj.y * dy + vj.z * dz;
+                    const double mm = vi.x * vj.x + vi.y * vj.y + vi.z * vj.z;
+                    const double cd = -Cmm2 * mm + Cmm3 * mid * mjd;
+                    gx += cd * dx - Cmm2 * (vi.x * mjd + vj.x * mid);
+                    gy += cd * dy - Cmm2 * (vi.y * mjd + vj.y * mid);
+                    gz += cd * dz - Cmm2 * (vi.z * mjd + vj.z * mid);
+                }
+                ax -= gx;
+                ay -= gy;
+                az -= gz;
+                ae += 0.5 * qi * qj * Cqq0;
+            }
+        }
+    }
+    const double sxk = E.K[0] / E.L[0], syk = E.K[1] / E.L[1], szk = E.K[2] / E.L[2];
+    double ph, g1[3], g2[6];
+    if (MODE == kEwE0) {
+        ew_gather<double, 0>(E, reinterpret_cast<const double*>(E.phi_q), pi, ph, g1, g2);
+        ax -= g1[0] * sxk;
+        ay -= g1[1] * syk;
+        az -= g1[2] * szk;
+        if (valid) {
+            out4[i] = make_double4(ax, ay, az, 0.0);
+            epart[i] = 0.5 * qi * ph + E.self_energy * qi * qi;
+        } else {
+            out4[i] = make_double4(0.0, 0.0, 0.0, 0.0);
+            epart[i] = 0.0;
+        }
+    } else if (MODE == kEwTmu) {
+        ew_gather<double, 0>(E, reinterpret_cast<const double*>(E.phi_v), pi, ph, g1, g2);
+        ax += -g1[0] * sxk + E.self_field * vi.x;
+        ay += -g1[1] * syk + E.self_field * vi.y;
+        az += -g1[2] * szk + E.self_field * vi.z;
+        if (!valid) ax = ay = az = 0.0;
+        if (pq.q) {   // PCG: q = p/alpha - T p and the tile sum of p.q
+            const double4 p = pq.p[i];
+            const double a = pq.alpha[i];
+            const double ia = a > 0.0 ? 1.0 / a : 0.0;
+            const double4 qv = make_double4(p.x * ia - ax, p.y * ia - ay, p.z * ia - az, 0.0);
+            pq.q[i] = qv;
+            const double d = wave_sum(p.x * qv.x + p.y * qv.y + p.z * qv.z);
+            if (lane == 0) pq.red[I] = d;
+        } else {
+            out4[i] = make_double4(ax, ay, az, 0.0);
+        }
+    } else {
+        double gv[3], hv[6], phv;
+        ew_gather<double, 1>(E, reinterpret_cast<const double*>(E.phi_q), pi, ph, g1, g2);
+        ew_gather<double, 1>(E, reinterpret_cast<const double*>(E.phi_v), pi, phv, gv, hv);
+        const double Ex = -(g1[0] + gv[0]) * sxk, Ey = -(g1[1] + gv[1]) * syk, Ez = -(g1[2] + gv[2]) * szk;
+        double h[6];
+        for (int k = 0; k < 6; k++) h[k] = g2[k] + (MUTUAL ? hv[k] : 0.0);
+        const double hxx = -h[0] * sxk * sxk, hyy = -h[1] * syk * syk, hzz = -h[2] * szk * szk;
+        const double hxy = -h[3] * sxk * syk, hxz = -h[4] * sxk * szk, hyz = -h[5] * syk * szk;
+        ax += qi * Ex + hxx * vi.x + hxy * vi.y + hxz * vi.z;
+        ay += qi * Ey + hxy * vi.x + hyy * vi.y + hyz * vi.z;
+        az += qi * Ez + hxz * vi.x + hyz * vi.y + hzz * vi.z;
+        if (valid) {
+            const double4 e0 = E0[i];
+            frc[3 * i] = ax;
+            frc[3 * i + 1] = ay;
+            frc[3 * i + 2] = az;
+            eout[i] = ae + epart[i] - 0.5 * (vi.x * e0.x + vi.y * e0.y + vi.z * e0.z);
+        } else {
+            frc[3 * i] = frc[3 * i + 1] = frc[3 * i + 2] = 0.0;
+            eout[i] = 0.0;
+        }
+    }
+}
+
+// Energy: slots summed in a fixed order by one workgroup, plus the background term.
+__global__ __launch_bounds__(256) void k_ew_energy(int npad, const double* __restrict__ eout, double background,
+                                                   double* __restrict__ energy) {
+    __shared__ double s[4];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < npad; i += blockDim.x) acc += eout[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) energy[0] = ((s[0] + s[1]) + (s[2] + s[3])) + background;
+}
+
+// Every pair within the cutoff, from the side of its lower caller index.
+__global__ __launch_bounds__(64) void k_ew_dump_pairs(EwLaunch E, const double4* __restrict__ posa,
+                                                      const int* __restrict__ perm, int* __restrict__ pairs,
+                                                      long long cap, unsigned long long* __restrict__ count) {
+    const int I = blockIdx.x, lane = threadIdx.x;
+    const int i = I * kTile + lane;
+    const bool ivalid = lane < E.tcnt[I];
+    const double4 pi = posa[i];
+    const int oi = perm[i];
+    const int cnt = E.nbr_cnt[I];
+    for (int t = 0; t < cnt; t++) {
+        const int J = E.nbr[(size_t)I * E.nblk + t];
+        for (int k = 0; k < E.tcnt[J]; k++) {
+            const int j = J * kTile + k;
+            const int oj = perm[j];
+            if (!ivalid || oj <= oi) continue;
+            const double4 pj = posa[j];
+            const double dx = ew_image(__dsub_rn(pi.x, pj.x), E.L[0]);
+            const double dy = ew_image(__dsub_rn(pi.y, pj.y), E.L[1]);
+            const double dz = ew_image(__dsub_rn(pi.z, pj.z), E.L[2]);
+            if (!(ew_r2(dx, dy, dz) < E.rc2)) continue;
+            const unsigned long long at = atomicAdd(count, 1ull);
+            if ((long long)at < cap) {
+                pairs[2 * at] = oi;
+                pairs[2 * at + 1] = oj;
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_ew_neighbors(hipStream_t s, const EwLaunch& E) {
+    hipLaunchKernelGGL(k_ew_neighbors, dim3(E.nblk), dim3(64), 0, s, E);
+}
+
+namespace {
+template <typename T>
+void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const double* scalars) {
+    using C = typename Cplx<T>::type;
+    const int axis = (kind == 0 || kind == 4) ? 2 : ((kind == 1 || kind == 3) ? 1 : 0);
+    const int n = E.K[axis];
+    const int nlines = axis == 2 ? E.K[0] * E.K[1] : (axis == 1 ? E.K[0] * E.K[2] : E.K[1] * E.K[2]);
+    // lines per workgroup: at most 24 KB per LDS buffer, and enough workgroups (512) to spread over the CUs
+    int LB = (int)(24576 / sizeof(C)) / n;
+    LB = std::min(LB, std::max(1, nlines / 512));
+    LB = std::max(1, std::min(LB, 32));
+    const size_t shm = 2 * (size_t)LB * n * sizeof(C);
+    const dim3 grid((nlines + LB - 1) / LB);
+    switch (kind) {
+        case 0: hipLaunchKernelGGL((k_ew_fft<T, 0>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
+        case 1: hipLaunchKernelGGL((k_ew_fft<T, 1>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
+        case 2: hipLaunchKernelGGL((k_ew_fft<T, 2>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
+        case 3: hipLaunchKernelGGL((k_ew_fft<T, 3>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
+        default: hipLaunchKernelGGL((k_ew_fft<T, 4>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
+    }
+}
+}  // namespace
+
+void launch_ew_recip(hipStream_t s, const EwLaunch& E, const double4* posa, const double* charge, const double4* vec,
+                     void* phi, const double* scalars) {
+    hipLaunchKernelGGL(k_ew_spread, dim3((E.npad + 63) / 64), dim3(64), 0, s, E, posa, charge, vec, scalars);
+    for (int kind = 0; kind < 5; kind++) {
+        if (E.mixed) fft_pass<float>(s, E, kind, phi, scalars);
+        else fft_pass<double>(s, E, kind, phi, scalars);
+    }
+}
+
+void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, const double4* posa, const double* inva,
+                     const double* charge, const double4* vec, const double4* E0, const double* scalars, double4* out4,
+                     const PcgQ& pq, double* epart, double* frc, double* eout) {
+    const dim3 grid(E.nblk), block(kEwWaves * 64);
+#define TDP_EW_PAIRS(M, MU, MX) \
+    hipLaunchKernelGGL((k_ew_pairs<M, MU, MX>), grid, block, 0, s, E, posa, inva, charge, vec, scalars, E.rs)
+#define TDP_EW_EPI(M, MU)                                                                                      \
+    hipLaunchKernelGGL((k_ew_epilogue<M, MU>), grid, dim3(64), 0, s, E, posa, inva, charge, vec, E0, scalars, E.rs, \
+                       out4, pq, epart, frc, eout)
+    // the permanent field stays in FP64 in both precisions (cancellation of the molecules' charge fields)
+    if (mode == kEwE0) {
+        TDP_EW_PAIRS(kEwE0, false, false);
+        TDP_EW_EPI(kEwE0, false);
+    } else if (mode == kEwTmu) {
+        if (E.pairs32) TDP_EW_PAIRS(kEwTmu, false, true);
+        else TDP_EW_PAIRS(kEwTmu, false, false);
+        TDP_EW_EPI(kEwTmu, false);
+    } else if (mutual) {
+        if (E.pairs32) TDP_EW_PAIRS(kEwForce, true, true);
+        else TDP_EW_PAIRS(kEwForce, true, false);
+        TDP_EW_EPI(kEwForce, true);
+    } else {
+        if (E.pairs32) TDP_EW_PAIRS(kEwForce, false, true);
+        else TDP_EW_PAIRS(kEwForce, false, false);
+        TDP_EW_EPI(kEwForce, false);
+    }
+#undef TDP_EW_PAIRS
+#undef TDP_EW_EPI
+}
+
+void launch_ew_energy(hipStream_t s, int npad, const double* eout, double background, double* energy) {
+    hipLaunchKernelGGL(k_ew_energy, dim3(1), dim3(256), 0, s, npad, eout, background, energy);
+}
+
+void launch_ew_dump_pairs(hipStream_t s, const EwLaunch& E, const double4* posa, const int* perm, int* pairs,
+                          long long cap, unsigned long long* count) {
+    hipLaunchKernelGGL(k_ew_dump_pairs, dim3(E.nblk), dim3(64), 0, s, E, posa, perm, pairs, cap, count);
+}
+
+}  // namespace tdp

@@ -149,6 +149,74 @@ void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const
 void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd, const double4* mu,
                     const double* mass, double* out);
 
+// ---- Ewald option (tdp_ewald.hip): real space within the cutoff over neighbour tiles, smooth PME
+// (order 5) in reciprocal space.
+constexpr int kEwOrder = 5;
+constexpr int kEwMaxStages = 12;
+// FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
+struct EwAxis {
+    int n, nst;
+    int r[kEwMaxStages];
+    const double2* tw64;
+    const float2* tw32;
+};
+
+// Real-space pass modes: the charges' field (E0), the dipoles' field (T v), or energy and forces.
+enum EwMode { kEwE0 = 0, kEwTmu = 1, kEwForce = 2 };
+
+struct EwLaunch {
+    int npad, nblk;
+    const int* tcnt;
+    double L[3];            // box diagonal
+    double rc2;             // cutoff squared (FP64 neighbour criterion)
+    double r64sq;           // mixed precision: pairs closer than this (squared) stay in FP64
+    bool pairs32;           // mixed precision: real-space pairs beyond r64 in FP32 (T v and forces)
+    float cut2_tiles;       // tile culling radius squared (cutoff plus margin), FP32
+    double alpha, c1, c2, c3, self_field, self_energy;   // Ewald constants
+    const float4* tb_c;
+    const float4* tb_h;
+    int* nbr;               // [nblk][nblk] neighbour tiles of each tile row (J ascending, J == I included)
+    int* nbr_cnt;           // [nblk]
+    const int* exc_start;
+    const int* exc_J;
+    const unsigned long long* exc_mask;
+    const int* sc_off;      // [npad+1] scaled partners of each slot (internal order)
+    const int* sc_j;
+    const double2* sc_s;    // (mScale, iScale)
+    int K[3];
+    EwAxis ax[3];
+    const double* G64;      // influence function [K0][K1][K2]
+    const float* G32;
+    unsigned long long* qfix;   // fixed-point spreading grid (int64, scale 2^40), zero between uses
+    void* cgrid;            // complex work grid (double2 or float2)
+    void* phi_q;            // potential grid of the charges (kept from the E0 pass)
+    void* phi_v;            // potential grid of the last dipole pass
+    double4* rs;            // [npad] real-space sums of the last pass (field or force, energy in w)
+    bool mixed;             // FP32 grids and FFTs (FP64 spreading and sums)
+};
+
+// Neighbour tiles of every tile row (after launch_tile_bounds).
+void launch_ew_neighbors(hipStream_t s, const EwLaunch& E);
+// Reciprocal-space potential grid of the charges (vec == null) or of the dipoles v (charge == null)
+// into phi (E.phi_q or E.phi_v): spreading, forward FFT, influence function, inverse FFT.
+// Skipped when scalars says the solve has stopped.
+void launch_ew_recip(hipStream_t s, const EwLaunch& E, const double4* posa, const double* charge,
+                     const double4* vec, void* phi, const double* scalars);
+// Real-space pass + reciprocal gather + self and scaled-pair terms for the 64 slots of every tile.
+// kEwE0: out4 = E0, epart = 1/2 q phi_q - a/sqrt(pi) q^2.  kEwTmu: out4 = T v, or with pq the PCG
+// q and p.q tile sums.  kEwForce (v = mu): frc (3 doubles per slot, overwritten) and eout (energy
+// per slot, with epart and -1/2 mu.E0).
+void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, const double4* posa,
+                     const double* inva, const double* charge, const double4* vec, const double4* E0,
+                     const double* scalars, double4* out4, const PcgQ& pq, double* epart, double* frc,
+                     double* eout);
+// Energy of the system: sum of eout over slots in a fixed order, plus the background term.
+void launch_ew_energy(hipStream_t s, int npad, const double* eout, double background, double* energy);
+// Every pair (i < j, caller indices) within the cutoff, as the real-space passes decide it:
+// pairs[2k], pairs[2k+1]; count[0] (may exceed cap: pairs past cap are not written).
+void launch_ew_dump_pairs(hipStream_t s, const EwLaunch& E, const double4* posa, const int* perm, int* pairs,
+                          long long cap, unsigned long long* count);
+
 // ---- dense path (tdp_dense.hip)
 int dense_max_tiles();
 void launch_dense_assemble(hipStream_t s, bool bf16, int nsys, int max_nt, const DenseSys* sys, const int* tcnt,

@@ -189,6 +189,16 @@ struct tdp_context {
     size_t cap_sc = 0;
     double *d_epart = nullptr, *d_eout = nullptr;
     double4* d_rs = nullptr;
+    double4* d_rsw = nullptr;
+    // stored real-space blocks of T (sliced ELL lists per tile row and wave, built by the E0 pass)
+    bool ew_stored = false;
+    int ew_cap32 = 0, ew_cap64 = 0;
+    int *d_idx32 = nullptr, *d_idx64 = nullptr, *d_w32 = nullptr, *d_w64 = nullptr, *d_need = nullptr;
+    float4* d_a32 = nullptr;
+    float2* d_b32 = nullptr;
+    double4* d_a64 = nullptr;
+    double2* d_b64 = nullptr;
+    void* h_need = nullptr;   // pinned copy of d_need
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
     bool warm_start = false, have_prev = false;
@@ -588,6 +598,49 @@ int ew_update_box(tdp_context* c, const double* box9) {
     return TDP_OK;
 }
 
+// (Re)allocates the stored T lists for the given per-wave capacities.  Returns TDP_OK with
+// ew_stored false when the device cannot hold them (the T v passes then stay matrix-free).
+int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
+    auto release = [c](void* p) {
+        if (!p) return;
+        auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
+        if (it != c->allocations.end()) c->allocations.erase(it);
+        hipFree(p);
+    };
+    release(c->d_idx32); release(c->d_a32); release(c->d_b32);
+    release(c->d_idx64); release(c->d_a64); release(c->d_b64);
+    c->d_idx32 = c->d_idx64 = nullptr;
+    c->d_a32 = nullptr;
+    c->d_b32 = nullptr;
+    c->d_a64 = nullptr;
+    c->d_b64 = nullptr;
+    c->ew_stored = false;
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    const size_t rows = (size_t)c->nblk * kEwWaves * 64;   // 64 slots per tile
+    const size_t n32 = mixed ? rows * cap32 : 1, n64 = rows * cap64;
+    void* p[6] = {};
+    const size_t bytes[6] = {n32 * sizeof(int), n32 * sizeof(float4), n32 * sizeof(float2),
+                             n64 * sizeof(int), n64 * sizeof(double4), n64 * sizeof(double2)};
+    for (int k = 0; k < 6; k++)
+        if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
+            for (int q = 0; q < k; q++) hipFree(p[q]);
+            (void)hipGetLastError();
+            return TDP_OK;   // matrix-free fallback
+        }
+    for (int k = 0; k < 6; k++) c->allocations.push_back(p[k]);
+    c->d_idx32 = (int*)p[0];
+    c->d_a32 = (float4*)p[1];
+    c->d_b32 = (float2*)p[2];
+    c->d_idx64 = (int*)p[3];
+    c->d_a64 = (double4*)p[4];
+    c->d_b64 = (double2*)p[5];
+    c->ew_cap32 = mixed ? cap32 : 0;
+    c->ew_cap64 = cap64;
+    c->ew_stored = true;
+    c->graph_gen++;
+    return TDP_OK;
+}
+
 EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     EwLaunch E;
     E.npad = c->npad;
@@ -629,6 +682,18 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.phi_q = c->d_phiq;
     E.phi_v = c->d_phiv;
     E.rs = c->d_rs;
+    E.rsw = c->d_rsw;
+    E.cap32 = c->ew_cap32;
+    E.cap64 = c->ew_cap64;
+    E.idx32 = c->ew_stored ? c->d_idx32 : nullptr;
+    E.a32 = c->d_a32;
+    E.b32 = c->d_b32;
+    E.idx64 = c->ew_stored ? c->d_idx64 : nullptr;
+    E.a64 = c->d_a64;
+    E.b64 = c->d_b64;
+    E.w32 = c->d_w32;
+    E.w64 = c->d_w64;
+    E.need = c->d_need;
     // reciprocal space in FP64 in both precisions: FP32 grids and FFTs put 2.4e-8 e*nm into the
     // fixed point of the 36,864-site box, above the 1e-6 D (2.08e-8 e*nm) bar
     E.mixed = false;
@@ -749,8 +814,12 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         const EwLaunch E = ew_launch(c, box9);
         launch_ew_recip(c->stream, E, c->posa, nullptr, v64, E.phi_v, scalars);
         if (timed) hipEventRecord(e0, c->stream);
-        launch_ew_pairs(c->stream, E, kEwTmu, false, c->posa, c->d_inva, c->d_charge, v64, nullptr, scalars, c->Emv, pq,
-                        nullptr, nullptr, nullptr);
+        if (c->ew_stored)
+            launch_ew_tmu_stored(c->stream, E, c->posa, c->d_inva, c->d_charge, v64,
+                                 c->precision == TDP_PRECISION_MIXED ? c->vec32 : nullptr, scalars, c->Emv, pq);
+        else
+            launch_ew_pairs(c->stream, E, kEwTmu, false, c->posa, c->d_inva, c->d_charge, v64, nullptr, scalars, c->Emv,
+                            pq, nullptr, nullptr, nullptr);
         if (timed) {
             hipEventRecord(e1, c->stream);
             c->pending.push_back({e0, e1});
@@ -884,8 +953,23 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
         launch_ew_neighbors(s, E);
         launch_ew_recip(s, E, c->posa, c->d_charge, nullptr, E.phi_q, nullptr);
-        launch_ew_pairs(s, E, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv, PcgQ(),
-                        c->d_epart, nullptr, nullptr);
+        for (int attempt = 0;; attempt++) {
+            const EwLaunch Eb = ew_launch(c, box);
+            if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 2 * sizeof(int), s));
+            launch_ew_pairs(s, Eb, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv,
+                            PcgQ(), c->d_epart, nullptr, nullptr);
+            if (!c->ew_stored) break;
+            // the lists must have held every pair: else grow them and build again
+            int* hn = static_cast<int*>(c->h_need);
+            TDP_HIP(c, hipMemcpyAsync(hn, c->d_need, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            TDP_HIP(c, hipStreamSynchronize(s));
+            const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
+            if ((!mixed_lists || hn[0] <= c->ew_cap32) && hn[1] <= c->ew_cap64) break;
+            if (attempt > 0) return fail(c, TDP_ERR_HIP, "Ewald option: the stored T lists could not be sized");
+            const int c32 = mixed_lists ? std::max(c->ew_cap32, hn[0] + hn[0] / 4 + 8) : 0;
+            const int c64 = std::max(c->ew_cap64, hn[1] + hn[1] / 4 + 8);
+            if ((rc = ew_alloc_lists(c, c32, c64))) return rc;
+        }
         launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
         return TDP_OK;
     }
@@ -1370,8 +1454,26 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             (rc = dalloc(c, &c->d_qfix, ng)) || (rc = dalloc(c, &c->d_cgrid, ng)) || (rc = dalloc(c, &c->d_phiq, ng)) ||
             (rc = dalloc(c, &c->d_phiv, ng)) || (rc = dalloc(c, &c->d_G64, ng)) || (rc = dalloc(c, &c->d_G32, ng)) ||
             (rc = dalloc(c, &c->d_sc_off, (size_t)npad + 1)) || (rc = dalloc(c, &c->d_epart, npad)) ||
-            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, npad)))
+            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, npad)) ||
+            (rc = dalloc(c, &c->d_rsw, (size_t)npad * kEwWaves)) ||
+            (rc = dalloc(c, &c->d_w32, (size_t)nb * kEwWaves)) || (rc = dalloc(c, &c->d_w64, (size_t)nb * kEwWaves)) ||
+            (rc = dalloc(c, &c->d_need, 2)))
             return bail(rc);
+        if (hipHostMalloc(&c->h_need, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+            c->err = "hipHostMalloc failed";
+            return bail(TDP_ERR_HIP);
+        }
+        {   // list capacities per wave from the mean density: 1.5x the expected pairs within the cutoff
+            const double V = c->default_box[0] * c->default_box[4] * c->default_box[8];
+            const double rho = V > 0.0 ? c->n / V : 0.0;
+            const double ball = 4.0 / 3.0 * M_PI;
+            const double est = rho * ball * c->cutoff * c->cutoff * c->cutoff;
+            const double est64 = rho * ball * c->ew_r64 * c->ew_r64 * c->ew_r64;
+            const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
+            const int c32 = mixed_lists ? (int)std::ceil(1.5 * est / kEwWaves) + 32 : 0;
+            const int c64 = mixed_lists ? (int)std::ceil(1.5 * est64 / kEwWaves) + 16 : (int)std::ceil(1.5 * est / kEwWaves) + 32;
+            if ((rc = ew_alloc_lists(c, c32, c64))) return bail(rc);
+        }
         for (int k = 0; k < 3; k++) {
             const int K = c->ew_K[k];
             std::vector<double2> t64(K);
@@ -1744,6 +1846,7 @@ void tdp_destroy(tdp_context* c) {
     for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
     for (void* p : c->allocations) hipFree(p);
     if (c->h_stage) hipHostFree(c->h_stage);
+    if (c->h_need) hipHostFree(c->h_need);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 
 #include "tdp_device.hpp"
 #include "tdp_kernels.h"
@@ -31,7 +32,6 @@ namespace tdp {
 
 namespace {
 
-constexpr int kEwWaves = 8;                  // waves per tile row in the real-space kernel
 constexpr double kFix = 1099511627776.0;     // 2^40: fixed-point scale of the spreading grid
 constexpr double kInvFix = 1.0 / 1099511627776.0;
 
@@ -161,41 +161,90 @@ __global__ __launch_bounds__(64) void k_ew_neighbors(EwLaunch E) {
 }
 
 // ------------------------------------------------------------------ spreading
-// One thread per slot: q theta_x theta_y theta_z + sum_d v_d (K_d/L_d) theta'_d ... into the
-// fixed-point grid.  Slots with neither charge nor dipole add nothing.
+// One workgroup per tile: each slot's contribution q theta_x theta_y theta_z + sum_d v_d (K_d/L_d)
+// theta'_d ... is added in fixed point (an associative integer sum, so the grid does not depend on
+// the order of the additions) into an LDS box of kSpB^3 grid points around the tile, and the box's
+// nonzero points are then added to the global grid (a compact tile touches about 13^3 points, so
+// the global atomics fall from 125 per slot to a fifth of that).  A tile whose points do not fit in
+// the box adds its contributions to the global grid directly.
+constexpr int kSpB = 16;
+
 __global__ __launch_bounds__(64) void k_ew_spread(EwLaunch E, const double4* __restrict__ posa,
                                                   const double* __restrict__ charge, const double4* __restrict__ vec,
                                                   const double* __restrict__ scalars) {
     if (scalars && scalars[SC_DONE] != 0.0) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= E.npad) return;
+    __shared__ unsigned long long box[kSpB * kSpB * kSpB];
+    const int T = blockIdx.x, lane = threadIdx.x;
+    const int i = T * kTile + lane;
     const double q = charge ? charge[i] : 0.0;
     const double4 v = vec ? vec[i] : make_double4(0.0, 0.0, 0.0, 0.0);
-    if (q == 0.0 && v.x == 0.0 && v.y == 0.0 && v.z == 0.0) return;
+    const bool live = lane < E.tcnt[T] && !(q == 0.0 && v.x == 0.0 && v.y == 0.0 && v.z == 0.0);
     const double4 p = posa[i];
     const double x[3] = {p.x, p.y, p.z};
-    int base[3];
+    int base[3], rel[3], ref[3], lo[3], ext[3];
     double th[3][kEwOrder], d1[3][kEwOrder], d2[kEwOrder];
+    bool fits = true;
     for (int k = 0; k < 3; k++) {
         double w;
         ew_grid_coord(x[k], E.L[k], E.K[k], base[k], w);
         ew_bspline<double>(w, th[k], d1[k], d2);
+        // offsets from the first slot's base index, taken across the periodic wrap
+        ref[k] = __shfl(base[k], 0, 64);
+        int r = base[k] - ref[k];
+        if (r > E.K[k] / 2) r -= E.K[k];
+        if (r < -E.K[k] / 2) r += E.K[k];
+        rel[k] = r;
+        int mn = live ? r : INT_MAX, mx = live ? r : INT_MIN;
+        for (int off = 32; off > 0; off >>= 1) {
+            mn = min(mn, __shfl_xor(mn, off, 64));
+            mx = max(mx, __shfl_xor(mx, off, 64));
+        }
+        lo[k] = mn;
+        ext[k] = mx - mn + kEwOrder;
+        if (mn == INT_MAX || ext[k] > kSpB || ext[k] > E.K[k]) fits = false;
     }
     const double vx = v.x * E.K[0] / E.L[0], vy = v.y * E.K[1] / E.L[1], vz = v.z * E.K[2] / E.L[2];
-    for (int a = 0; a < kEwOrder; a++) {
-        const int gx = ew_wrap(base[0] + a, E.K[0]);
-        for (int b = 0; b < kEwOrder; b++) {
-            const int gy = ew_wrap(base[1] + b, E.K[1]);
-            const double txy = th[0][a] * th[1][b];
-            const double cq = q * txy + vx * d1[0][a] * th[1][b] + vy * th[0][a] * d1[1][b];
-            const double cz = vz * txy;
-            unsigned long long* row = E.qfix + ((size_t)gx * E.K[1] + gy) * E.K[2];
-            for (int c = 0; c < kEwOrder; c++) {
-                const int gz = ew_wrap(base[2] + c, E.K[2]);
-                const double val = cq * th[2][c] + cz * d1[2][c];
-                atomicAdd(row + gz, (unsigned long long)__double2ll_rn(val * kFix));
+    if (!fits) {   // (an empty tile also lands here and adds nothing)
+        if (!live) return;
+        for (int a = 0; a < kEwOrder; a++) {
+            const int gx = ew_wrap(base[0] + a, E.K[0]);
+            for (int b = 0; b < kEwOrder; b++) {
+                const int gy = ew_wrap(base[1] + b, E.K[1]);
+                const double txy = th[0][a] * th[1][b];
+                const double cq = q * txy + vx * d1[0][a] * th[1][b] + vy * th[0][a] * d1[1][b];
+                const double cz = vz * txy;
+                unsigned long long* row = E.qfix + ((size_t)gx * E.K[1] + gy) * E.K[2];
+                for (int c = 0; c < kEwOrder; c++) {
+                    const int gz = ew_wrap(base[2] + c, E.K[2]);
+                    atomicAdd(row + gz, (unsigned long long)__double2ll_rn((cq * th[2][c] + cz * d1[2][c]) * kFix));
+                }
             }
         }
+        return;
+    }
+    const int ncell = ext[0] * ext[1] * ext[2];
+    for (int c = lane; c < ncell; c += kTile) box[c] = 0ull;
+    __syncthreads();
+    if (live) {
+        const int o0 = rel[0] - lo[0], o1 = rel[1] - lo[1], o2 = rel[2] - lo[2];
+        for (int a = 0; a < kEwOrder; a++)
+            for (int b = 0; b < kEwOrder; b++) {
+                const double txy = th[0][a] * th[1][b];
+                const double cq = q * txy + vx * d1[0][a] * th[1][b] + vy * th[0][a] * d1[1][b];
+                const double cz = vz * txy;
+                unsigned long long* row = box + ((o0 + a) * ext[1] + (o1 + b)) * ext[2] + o2;
+                for (int c = 0; c < kEwOrder; c++)
+                    atomicAdd(row + c, (unsigned long long)__double2ll_rn((cq * th[2][c] + cz * d1[2][c]) * kFix));
+            }
+    }
+    __syncthreads();
+    for (int c = lane; c < ncell; c += kTile) {
+        const unsigned long long val = box[c];
+        if (!val) continue;
+        const int cz = c % ext[2], cy = (c / ext[2]) % ext[1], cx = c / (ext[2] * ext[1]);
+        int g[3] = {ref[0] + lo[0] + cx, ref[1] + lo[1] + cy, ref[2] + lo[2] + cz};
+        for (int k = 0; k < 3; k++) g[k] = ((g[k] % E.K[k]) + E.K[k]) % E.K[k];
+        atomicAdd(E.qfix + ((size_t)g[0] * E.K[1] + g[1]) * E.K[2] + g[2], val);
     }
 }
 
@@ -537,6 +586,8 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     const int cnt = E.nbr_cnt[I];
     const int* nb = E.nbr + (size_t)I * E.nblk;
     const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
+    const bool build = MODE == kEwE0 && E.idx64 != nullptr;   // store T's blocks while forming E0
+    int n32 = 0, n64 = 0;
     for (int t = w; t < cnt; t += kEwWaves) {
         const int J = nb[t];
         const int j = J * kTile + lane;
@@ -563,18 +614,27 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             near_box = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
         }
         unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
-        // scaled pairs of this tile pair are masked (the epilogue evaluates them)
-        unsigned long long mask = 0ull;
+        // scaled pairs of this tile pair are masked (the epilogue evaluates them); the E0 pass also
+        // needs the mask of T's pairs (iScale 0) for the stored blocks
+        unsigned long long mask = 0ull, maskT = 0ull;
         for (int e = eS; e < eE; e++) {
             if (E.exc_J[e] != J) continue;
             const unsigned long long* m = E.exc_mask + (size_t)e * 3 * kTile;
-            if (MODE == kEwE0) mask = m[lane] | m[kTile + lane];
-            else if (MODE == kEwTmu) mask = m[2 * kTile + lane];
-            else mask = m[lane] | m[kTile + lane] | m[2 * kTile + lane];
+            if (MODE == kEwE0) {
+                mask = m[lane] | m[kTile + lane];
+                maskT = m[2 * kTile + lane];
+            } else if (MODE == kEwTmu) {
+                mask = m[2 * kTile + lane];
+            } else {
+                mask = m[lane] | m[kTile + lane] | m[2 * kTile + lane];
+            }
             break;
         }
-        if (J == I) mask |= 1ull << lane;
-        if (!ivalid) mask = ~0ull;
+        if (J == I) {
+            mask |= 1ull << lane;
+            maskT |= 1ull << lane;
+        }
+        if (!ivalid) mask = maskT = ~0ull;
         unsigned long long near_steps = 0ull;
         if (MIXED) {
             float fx = 0.f, fy = 0.f, fz = 0.f, fe = 0.f;
@@ -626,10 +686,12 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             const double r2x = ew_r2(dx, dy, dz);
             bool sel = !((mask >> k) & 1ull) && r2x < E.rc2;
             if (MIXED) sel = sel && r2x < E.r64sq;
-            if (!__builtin_amdgcn_ballot_w64(sel)) continue;
+            const bool store = build && !((maskT >> k) & 1ull) && r2x < E.rc2;
+            if (!__builtin_amdgcn_ballot_w64(sel || store)) continue;
             const double4 vj = s_v[w][k];
             EwR<double> R;
-            ew_radial(E, sel ? r2x : 1.0, pi.w, pj.w, iai, s_ia[w][k], true, MODE == kEwForce && MUTUAL, R);
+            ew_radial(E, (sel || store) ? r2x : 1.0, pi.w, pj.w, iai, s_ia[w][k], true, MODE == kEwForce && MUTUAL,
+                      R);
             double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
             ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, vj.w, vj.x, vj.y, vj.z, tx, ty,
                                                 tz, te);
@@ -639,12 +701,64 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
                 sz += tz;
                 se += te;
             }
+            if (store) {   // the block T_ij = C2 d d^T - C1 of the T v passes (iScale 1, damped)
+                const double C1 = R.B1 + (R.l3 - 1.0) * R.b1;
+                const double C2 = R.B2 + (R.l5 - 1.0) * R.b2;
+                const double txx = C2 * dx * dx - C1, tyy = C2 * dy * dy - C1, tzz = C2 * dz * dz - C1;
+                const double txy = C2 * dx * dy, txz = C2 * dx * dz, tyz = C2 * dy * dz;
+                const int jj = J * kTile + k;
+                if (E.pairs32 && r2x >= E.r64sq) {
+                    if (n32 < E.cap32) {
+                        const size_t at = (((size_t)I * kEwWaves + w) * E.cap32 + n32) * kTile + lane;
+                        E.idx32[at] = jj;
+                        E.a32[at] = make_float4((float)txx, (float)tyy, (float)tzz, (float)txy);
+                        E.b32[at] = make_float2((float)txz, (float)tyz);
+                    }
+                    n32++;
+                } else {
+                    if (n64 < E.cap64) {
+                        const size_t at = (((size_t)I * kEwWaves + w) * E.cap64 + n64) * kTile + lane;
+                        E.idx64[at] = jj;
+                        E.a64[at] = make_double4(txx, tyy, tzz, txy);
+                        E.b64[at] = make_double2(txz, tyz);
+                    }
+                    n64++;
+                }
+            }
         }
         ax += sx;
         ay += sy;
         az += sz;
         ae += se;
         __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next tile is staged
+    }
+    if (build) {   // list widths of this row and wave; the largest needed width for the capacity check
+        int m32 = n32, m64 = n64;
+        for (int off = 32; off > 0; off >>= 1) {
+            m32 = max(m32, __shfl_xor(m32, off, 64));
+            m64 = max(m64, __shfl_xor(m64, off, 64));
+        }
+        // the T v pass reads every lane's list up to the wave's width: pad the shorter ones with
+        // zero blocks (on the slot itself)
+        const int w32c = min(m32, E.cap32), w64c = min(m64, E.cap64);
+        for (int n = n32; n < w32c; n++) {
+            const size_t at = (((size_t)I * kEwWaves + w) * E.cap32 + n) * kTile + lane;
+            E.idx32[at] = i;
+            E.a32[at] = make_float4(0.f, 0.f, 0.f, 0.f);
+            E.b32[at] = make_float2(0.f, 0.f);
+        }
+        for (int n = n64; n < w64c; n++) {
+            const size_t at = (((size_t)I * kEwWaves + w) * E.cap64 + n) * kTile + lane;
+            E.idx64[at] = i;
+            E.a64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
+            E.b64[at] = make_double2(0.0, 0.0);
+        }
+        if (lane == 0) {
+            E.w32[I * kEwWaves + w] = w32c;
+            E.w64[I * kEwWaves + w] = w64c;
+            atomicMax(&E.need[0], m32);
+            atomicMax(&E.need[1], m64);
+        }
     }
     // the row's sums over the waves, in wave order (the staging slices are reused)
     __syncthreads();
@@ -664,11 +778,59 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     rs[i] = make_double4(ax, ay, az, ae);
 }
 
+// T v of the real-space pairs from the stored blocks: one wave per list (tile row I, list w), so that
+// the nblk * kEwWaves waves spread evenly over the CUs; FP32 lists on the FP32 copy of v, FP64 lists
+// on v.  Each wave writes its partial sums; the epilogue adds the kEwWaves partials of a slot in
+// list order.  Memory bound: 28 bytes per FP32 entry, 52 per FP64 entry, plus the gathered v_j.
+template <bool MIXED>
+__global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __restrict__ vec,
+                                                const float4* __restrict__ v32, const double* __restrict__ scalars,
+                                                double4* __restrict__ part) {
+    if (scalars && scalars[SC_DONE] != 0.0) return;
+    const int row = blockIdx.x, lane = threadIdx.x;   // row = I kEwWaves + w
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (MIXED) {
+        const int n32 = E.w32[row];
+        const size_t base = (size_t)row * E.cap32 * kTile + lane;
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+#pragma unroll 4
+        for (int n = 0; n < n32; n++) {
+            const size_t at = base + (size_t)n * kTile;
+            const int j = E.idx32[at];
+            const float4 a = E.a32[at];
+            const float2 b = E.b32[at];
+            const float4 v = v32[j];
+            fx += a.x * v.x + a.w * v.y + b.x * v.z;
+            fy += a.w * v.x + a.y * v.y + b.y * v.z;
+            fz += b.x * v.x + b.y * v.y + a.z * v.z;
+        }
+        ax = fx;
+        ay = fy;
+        az = fz;
+    }
+    {
+        const int n64 = E.w64[row];
+        const size_t base = (size_t)row * E.cap64 * kTile + lane;
+#pragma unroll 2
+        for (int n = 0; n < n64; n++) {
+            const size_t at = base + (size_t)n * kTile;
+            const int j = E.idx64[at];
+            const double4 a = E.a64[at];
+            const double2 b = E.b64[at];
+            const double4 v = vec[j];
+            ax += a.x * v.x + a.w * v.y + b.x * v.z;
+            ay += a.w * v.x + a.y * v.y + b.y * v.z;
+            az += b.x * v.x + b.y * v.y + a.z * v.z;
+        }
+    }
+    part[(size_t)row * kTile + lane] = make_double4(ax, ay, az, 0.0);
+}
+
 // Epilogue of a pass for the 64 slots of tile I (one wave per tile): the real-space sums, the
 // scaled pairs of each slot in FP64 with their scales ([inside] B_n + (s lambda - 1) b_n), the
 // reciprocal field (E = -(K/L) grad_u phi) or force (q E + grad E . v) gathered from the potential
 // grids, and the self terms; then the outputs of the mode.
-template <int MODE, bool MUTUAL>
+template <int MODE, bool MUTUAL, bool PARTS = false>
 __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* __restrict__ posa,
                                                     const double* __restrict__ inva, const double* __restrict__ charge,
                                                     const double4* __restrict__ vec, const double4* __restrict__ E0,
@@ -684,8 +846,26 @@ __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* _
     const double iai = inva[i];
     const double qi = charge[i];
     const double4 vi = (MODE == kEwE0) ? make_double4(0.0, 0.0, 0.0, 0.0) : vec[i];
-    const double4 r0 = rs[i];
-    double ax = r0.x, ay = r0.y, az = r0.z, ae = r0.w;
+    double ax, ay, az, ae;
+    if (PARTS) {   // the kEwWaves list partials of the stored T v pass, in list order
+        const double4 r0 = rs[(size_t)(I * kEwWaves) * kTile + lane];
+        ax = r0.x;
+        ay = r0.y;
+        az = r0.z;
+        ae = 0.0;
+        for (int u = 1; u < kEwWaves; u++) {
+            const double4 r = rs[(size_t)(I * kEwWaves + u) * kTile + lane];
+            ax += r.x;
+            ay += r.y;
+            az += r.z;
+        }
+    } else {
+        const double4 r0 = rs[i];
+        ax = r0.x;
+        ay = r0.y;
+        az = r0.z;
+        ae = r0.w;
+    }
     const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
     if (valid) {
         for (int q = E.sc_off[i]; q < E.sc_off[i + 1]; q++) {
@@ -874,7 +1054,7 @@ void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const doubl
 
 void launch_ew_recip(hipStream_t s, const EwLaunch& E, const double4* posa, const double* charge, const double4* vec,
                      void* phi, const double* scalars) {
-    hipLaunchKernelGGL(k_ew_spread, dim3((E.npad + 63) / 64), dim3(64), 0, s, E, posa, charge, vec, scalars);
+    hipLaunchKernelGGL(k_ew_spread, dim3(E.nblk), dim3(64), 0, s, E, posa, charge, vec, scalars);
     for (int kind = 0; kind < 5; kind++) {
         if (E.mixed) fft_pass<float>(s, E, kind, phi, scalars);
         else fft_pass<double>(s, E, kind, phi, scalars);
@@ -909,6 +1089,19 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
     }
 #undef TDP_EW_PAIRS
 #undef TDP_EW_EPI
+}
+
+void launch_ew_tmu_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
+                          const double* charge, const double4* vec, const float4* v32, const double* scalars,
+                          double4* out4, const PcgQ& pq) {
+    const dim3 lists(E.nblk * kEwWaves);
+    if (E.pairs32)
+        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);
+    else
+        hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);
+    hipLaunchKernelGGL((k_ew_epilogue<kEwTmu, false, true>), dim3(E.nblk), dim3(64), 0, s, E, posa, inva, charge, vec,
+                       (const double4*)nullptr, scalars, E.rsw, out4, pq, (double*)nullptr, (double*)nullptr,
+                       (double*)nullptr);
 }
 
 void launch_ew_energy(hipStream_t s, int npad, const double* eout, double background, double* energy) {

@@ -153,6 +153,7 @@ void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd
 // (order 5) in reciprocal space.
 constexpr int kEwOrder = 5;
 constexpr int kEwMaxStages = 12;
+constexpr int kEwWaves = 8;   // waves per tile row in the real-space passes
 // FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
 struct EwAxis {
     int n, nst;
@@ -192,6 +193,22 @@ struct EwLaunch {
     void* phi_q;            // potential grid of the charges (kept from the E0 pass)
     void* phi_v;            // potential grid of the last dipole pass
     double4* rs;            // [npad] real-space sums of the last pass (field or force, energy in w)
+    double4* rsw;           // [nblk kEwWaves 64] partial T v sums of the stored lists
+    // Stored real-space blocks of T (built by the E0 pass, applied by every T v pass): per tile row I
+    // and wave w a list of (j, T_ij) for each of the row's 64 slots, sliced ELL: entry n of slot
+    // lane at ((I kEwWaves + w) cap + n) 64 + lane; T_ij as (xx, yy, zz, xy) and (xz, yz).  FP64
+    // lists hold every pair in FP64 mode and the pairs closer than r64 in mixed precision; FP32
+    // lists hold the other pairs in mixed precision.
+    int cap32, cap64;
+    int* idx32;
+    float4* a32;
+    float2* b32;
+    int* idx64;
+    double4* a64;
+    double2* b64;
+    int* w32;               // [nblk kEwWaves] list widths
+    int* w64;
+    int* need;              // [2] largest width seen (FP32, FP64), for the capacity check
     bool mixed;             // FP32 grids and FFTs (FP64 spreading and sums)
 };
 
@@ -210,6 +227,11 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
                      const double* inva, const double* charge, const double4* vec, const double4* E0,
                      const double* scalars, double4* out4, const PcgQ& pq, double* epart, double* frc,
                      double* eout);
+// T v of the solver from the stored blocks (v in FP64, and its FP32 copy v32 for the FP32 lists),
+// then the epilogue of kEwTmu (reciprocal gather, scaled pairs, self field, PCG q).
+void launch_ew_tmu_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
+                          const double* charge, const double4* vec, const float4* v32, const double* scalars,
+                          double4* out4, const PcgQ& pq);
 // Energy of the system: sum of eout over slots in a fixed order, plus the background term.
 void launch_ew_energy(hipStream_t s, int npad, const double* eout, double background, double* energy);
 // Every pair (i < j, caller indices) within the cutoff, as the real-space passes decide it:

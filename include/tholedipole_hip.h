@@ -171,6 +171,26 @@ int tdp_execute_device_noalloc(tdp_context* ctx, const double* d_positions, cons
 int tdp_get_neighbor_pairs(tdp_context* ctx, const double* positions, const double* box, int64_t capacity,
                            int32_t* pairs, int64_t* count);
 
+/* No counterpart in the reference: the Ewald option's layout after the last evaluation, for
+ * diagnostics and the memory-traffic figure of the stored T v pass.  entries: real-space blocks of
+ * T held per direction (i, j) and (j, i), FP32 and FP64; slots: the same lists with their padding
+ * to the width of a wave (what one T v pass streams); stored 0 when the lists did not fit in
+ * memory and the T v passes run matrix-free. */
+typedef struct tdp_ewald_info {
+    double alpha;                     /* Ewald parameter, nm^-1 */
+    int32_t grid[3];                  /* PME grid */
+    double cutoff;                    /* real-space cutoff, nm */
+    double r_fp64;                    /* mixed precision: pairs closer than this stay in FP64 */
+    int32_t stored;
+    int64_t entries32, entries64;
+    int64_t slots32, slots64;
+    int32_t bytes_per_entry32, bytes_per_entry64;
+    int32_t tiles;                    /* tiles of 64 slots */
+    int32_t max_neighbor_tiles;       /* largest neighbour-tile count of a tile */
+    double mean_neighbor_tiles;
+} tdp_ewald_info;
+int tdp_get_ewald_info(const tdp_context* ctx, tdp_ewald_info* info);
+
 /* Replaces getInducedDipoles / getLabFramePermanentDipoles / getTotalDipoles
  * (TholeDipoleKernels.h:91-105, ReferenceTholeDipoleKernels.cpp:217-282): recomputes from the
  * given positions, like the reference.  out: host [3n]. */

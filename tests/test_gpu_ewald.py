@@ -102,6 +102,42 @@ def test_neighbor_pairs_bit_exact():
         assert got.shape == ref.shape and np.array_equal(got, ref), name
 
 
+@pytest.mark.parametrize("precision", ["double", "mixed"])
+def test_stored_lists_hold_every_pair(precision):
+    """The stored real-space blocks of T: one entry per direction of every pair within the cutoff
+    whose iScale is nonzero (the oracle's scale maps), split at the FP64 radius in mixed precision
+    with the neighbour criterion's own FP64 r^2; the padded lists hold at least as many slots."""
+    _say("case ewald stored lists %s" % precision)
+    import ctypes as C
+    s = _box648()
+    h = _lib.HipContextHandle(s.force, s.box, precision=precision, solver="pcg", flags=_lib.FLAG_EWALD)
+    h.execute(s.positions, s.box)
+    info = h.ewald_info()
+    h.close()
+    pairs = oracle.cutoff_pairs(s.force, s.positions, s.box, s.force.getCutoffDistance())
+    os_ = oracle.OracleSystem(s.force)
+    L = oracle.lib()
+    m, sc = C.c_double(), C.c_double()
+    keep = np.zeros(len(pairs), bool)
+    for k, (i, j) in enumerate(pairs):
+        assert L.tdpo_pair_scales(C.byref(os_.s), int(i), int(j), C.byref(m), C.byref(sc)) == 0
+        keep[k] = sc.value != 0.0
+    box = np.diag(s.box)
+    d = s.positions[pairs[:, 0]] - s.positions[pairs[:, 1]]
+    d -= box * np.floor(d / box + 0.5)
+    r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+    near = r2 < info["r_fp64"] * info["r_fp64"] if precision == "mixed" else np.ones(len(pairs), bool)
+    want64 = 2 * int(np.count_nonzero(keep & near))
+    want32 = 2 * int(np.count_nonzero(keep & ~near))
+    _say("  %s: %d FP32 + %d FP64 entries (expected %d + %d), slots %d + %d, %d pairs with iScale 0"
+         % (precision, info["entries32"], info["entries64"], want32, want64, info["slots32"], info["slots64"],
+            int(np.count_nonzero(~keep))))
+    assert info["stored"] == 1
+    assert info["entries32"] == want32 and info["entries64"] == want64
+    assert info["slots32"] >= want32 and info["slots64"] >= want64
+    assert np.count_nonzero(~keep) > 0 and (precision == "double" or want32 > want64 > 0)
+
+
 @pytest.mark.parametrize("polarization", [T.Mutual, T.Direct])
 def test_box648_fp64_matches_oracle(polarization):
     _say("case ewald box648 fp64 %d" % polarization)

@@ -30,7 +30,8 @@ DIPOLES_INDUCED, DIPOLES_LAB_PERMANENT, DIPOLES_TOTAL = 0, 1, 2
 # Every symbol include/tholedipole_hip.h declares.
 EXPORTED_SYMBOLS = [
     "tdp_create", "tdp_destroy", "tdp_last_error", "tdp_last_create_error", "tdp_copy_parameters",
-    "tdp_execute", "tdp_execute_device", "tdp_execute_device_noalloc", "tdp_get_neighbor_pairs", "tdp_get_dipoles",
+    "tdp_execute", "tdp_execute_device", "tdp_execute_device_noalloc", "tdp_get_neighbor_pairs", "tdp_get_ewald_info",
+    "tdp_get_dipoles",
     "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
     "tdp_autotune", "tdp_set_warm_start",
@@ -66,6 +67,13 @@ class SolverStats(C.Structure):
                 ("matvec_ms", C.c_double), ("matvec_launches", C.c_int32)]
 
 
+class EwaldInfo(C.Structure):
+    _fields_ = [("alpha", C.c_double), ("grid", C.c_int32 * 3), ("cutoff", C.c_double), ("r_fp64", C.c_double),
+                ("stored", C.c_int32), ("entries32", C.c_int64), ("entries64", C.c_int64), ("slots32", C.c_int64),
+                ("slots64", C.c_int64), ("bytes_per_entry32", C.c_int32), ("bytes_per_entry64", C.c_int32),
+                ("tiles", C.c_int32), ("max_neighbor_tiles", C.c_int32), ("mean_neighbor_tiles", C.c_double)]
+
+
 _lib = None
 
 
@@ -98,6 +106,7 @@ def load():
     L.tdp_execute_device_noalloc.argtypes = [ctx_p, C.c_void_p, _dp, C.c_void_p, C.c_void_p, C.c_void_p]
     L.tdp_get_dipoles.argtypes = [ctx_p, _dp, _dp, C.c_int32, _dp]
     L.tdp_get_neighbor_pairs.argtypes = [ctx_p, _dp, _dp, C.c_int64, _ip, C.POINTER(C.c_int64)]
+    L.tdp_get_ewald_info.argtypes = [ctx_p, C.POINTER(EwaldInfo)]
     L.tdp_get_electrostatic_potential.argtypes = [ctx_p, _dp, _dp, _dp, C.c_int32, _dp]
     L.tdp_get_system_multipole_moments.argtypes = [ctx_p, _dp, _dp, _dp, _dp]
     L.tdp_get_pme_parameters.argtypes = [ctx_p, _dp, _ip, _ip, _ip]
@@ -263,6 +272,12 @@ class HipContextHandle:
         a = C.c_double(); nx = C.c_int32(); ny = C.c_int32(); nz = C.c_int32()
         self._check(self.L.tdp_get_pme_parameters(self.h, C.byref(a), C.byref(nx), C.byref(ny), C.byref(nz)))
         return a.value, nx.value, ny.value, nz.value
+
+    def ewald_info(self):
+        """Layout of the Ewald option after the last evaluation (stored T entries and padded slots)."""
+        info = EwaldInfo()
+        self._check(self.L.tdp_get_ewald_info(self.h, C.byref(info)))
+        return {k: (list(getattr(info, k)) if k == "grid" else getattr(info, k)) for k, _ in EwaldInfo._fields_}
 
     def solver_stats(self):
         st = SolverStats()

@@ -193,11 +193,12 @@ struct tdp_context {
     // stored real-space blocks of T (sliced ELL lists per tile row and wave, built by the E0 pass)
     bool ew_stored = false;
     int ew_cap32 = 0, ew_cap64 = 0;
-    int *d_idx32 = nullptr, *d_idx64 = nullptr, *d_w32 = nullptr, *d_w64 = nullptr, *d_need = nullptr;
-    float4* d_a32 = nullptr;
-    float2* d_b32 = nullptr;
-    double4* d_a64 = nullptr;
-    double2* d_b64 = nullptr;
+    unsigned short* d_idx16 = nullptr;
+    int *d_idx64 = nullptr, *d_w32 = nullptr, *d_w64 = nullptr, *d_need = nullptr;
+    int ew_lds_tiles = 0;   // neighbour tiles the stored T v pass stages in LDS (0: gathers from HBM)
+    float4* d_t32 = nullptr;
+    double4* d_t64 = nullptr;
+    int64_t ew_entries[2] = {0, 0}, ew_slots[2] = {0, 0};   // stored entries and padded slots of the last build
     void* h_need = nullptr;   // pinned copy of d_need
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
@@ -598,6 +599,21 @@ int ew_update_box(tdp_context* c, const double* box9) {
     return TDP_OK;
 }
 
+// Per-particle terms of the Ewald option: the net charge (background energy) and the FP64 radius of
+// the mixed real-space passes, where a u^3 reaches 4 for the strongest damping (u = r / (a_i a_j)),
+// at least 0.25 nm.  The radius splits the stored T lists: the next build uses it.
+void ew_particle_terms(tdp_context* c) {
+    c->ew_qsum = 0.0;
+    for (double q : c->charge) c->ew_qsum += q;
+    double amax = 0.0, amin = 1e300;
+    for (double a : c->thole) {
+        amax = std::max(amax, a);
+        if (a > 0.0) amin = std::min(amin, a);
+    }
+    c->ew_r64 = 0.25;
+    if (amax > 0.0) c->ew_r64 = std::max(0.25, std::cbrt(4.0 / amin) * amax * amax);
+}
+
 // (Re)allocates the stored T lists for the given per-wave capacities.  Returns TDP_OK with
 // ew_stored false when the device cannot hold them (the T v passes then stay matrix-free).
 int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
@@ -607,33 +623,31 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
         if (it != c->allocations.end()) c->allocations.erase(it);
         hipFree(p);
     };
-    release(c->d_idx32); release(c->d_a32); release(c->d_b32);
-    release(c->d_idx64); release(c->d_a64); release(c->d_b64);
-    c->d_idx32 = c->d_idx64 = nullptr;
-    c->d_a32 = nullptr;
-    c->d_b32 = nullptr;
-    c->d_a64 = nullptr;
-    c->d_b64 = nullptr;
+    release(c->d_idx16); release(c->d_t32);
+    release(c->d_idx64); release(c->d_t64);
+    c->d_idx16 = nullptr;
+    c->d_idx64 = nullptr;
+    c->d_t32 = nullptr;
+    c->d_t64 = nullptr;
     c->ew_stored = false;
+    cap32 = (cap32 + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU;
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     const size_t rows = (size_t)c->nblk * kEwWaves * 64;   // 64 slots per tile
     const size_t n32 = mixed ? rows * cap32 : 1, n64 = rows * cap64;
-    void* p[6] = {};
-    const size_t bytes[6] = {n32 * sizeof(int), n32 * sizeof(float4), n32 * sizeof(float2),
-                             n64 * sizeof(int), n64 * sizeof(double4), n64 * sizeof(double2)};
-    for (int k = 0; k < 6; k++)
+    void* p[4] = {};
+    const size_t bytes[4] = {n32 * sizeof(unsigned short), n32 * sizeof(float4), n64 * sizeof(int),
+                             n64 * sizeof(double4)};
+    for (int k = 0; k < 4; k++)
         if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
             for (int q = 0; q < k; q++) hipFree(p[q]);
             (void)hipGetLastError();
             return TDP_OK;   // matrix-free fallback
         }
-    for (int k = 0; k < 6; k++) c->allocations.push_back(p[k]);
-    c->d_idx32 = (int*)p[0];
-    c->d_a32 = (float4*)p[1];
-    c->d_b32 = (float2*)p[2];
-    c->d_idx64 = (int*)p[3];
-    c->d_a64 = (double4*)p[4];
-    c->d_b64 = (double2*)p[5];
+    for (int k = 0; k < 4; k++) c->allocations.push_back(p[k]);
+    c->d_idx16 = (unsigned short*)p[0];
+    c->d_t32 = (float4*)p[1];
+    c->d_idx64 = (int*)p[2];
+    c->d_t64 = (double4*)p[3];
     c->ew_cap32 = mixed ? cap32 : 0;
     c->ew_cap64 = cap64;
     c->ew_stored = true;
@@ -685,15 +699,14 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.rsw = c->d_rsw;
     E.cap32 = c->ew_cap32;
     E.cap64 = c->ew_cap64;
-    E.idx32 = c->ew_stored ? c->d_idx32 : nullptr;
-    E.a32 = c->d_a32;
-    E.b32 = c->d_b32;
+    E.idx16 = c->d_idx16;
+    E.t32 = c->d_t32;
     E.idx64 = c->ew_stored ? c->d_idx64 : nullptr;
-    E.a64 = c->d_a64;
-    E.b64 = c->d_b64;
+    E.t64 = c->d_t64;
     E.w32 = c->d_w32;
     E.w64 = c->d_w64;
     E.need = c->d_need;
+    E.lds_tiles = c->ew_lds_tiles;
     // reciprocal space in FP64 in both precisions: FP32 grids and FFTs put 2.4e-8 e*nm into the
     // fixed point of the 36,864-site box, above the 1e-6 D (2.08e-8 e*nm) bar
     E.mixed = false;
@@ -955,15 +968,34 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         launch_ew_recip(s, E, c->posa, c->d_charge, nullptr, E.phi_q, nullptr);
         for (int attempt = 0;; attempt++) {
             const EwLaunch Eb = ew_launch(c, box);
-            if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 2 * sizeof(int), s));
+            if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 7 * sizeof(int), s));
             launch_ew_pairs(s, Eb, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv,
                             PcgQ(), c->d_epart, nullptr, nullptr);
             if (!c->ew_stored) break;
             // the lists must have held every pair: else grow them and build again
             int* hn = static_cast<int*>(c->h_need);
-            TDP_HIP(c, hipMemcpyAsync(hn, c->d_need, 2 * sizeof(int), hipMemcpyDeviceToHost, s));
+            TDP_HIP(c, hipMemcpyAsync(hn, c->d_need, 7 * sizeof(int), hipMemcpyDeviceToHost, s));
             TDP_HIP(c, hipStreamSynchronize(s));
+            for (int k = 0; k < 2; k++) {
+                c->ew_entries[k] = hn[2 + k];
+                c->ew_slots[k] = (int64_t)hn[4 + k] * 64;
+            }
             const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
+            if (mixed_lists && hn[6] > kEwMaxLocalTiles) {
+                // the FP32 list indices cannot address this many neighbour tiles: matrix-free T v
+                // passes from now on (this pass's E0 is complete)
+                c->ew_stored = false;
+                c->graph_gen++;
+                break;
+            }
+            {   // LDS copy of v for the FP32 lists: the wave's slots of up to lds_tiles tiles, 64 KB at most
+                int lt = (hn[6] + 7) / 8 * 8;
+                if ((size_t)lt * 8 * sizeof(float4) > 65536) lt = 0;
+                if (lt != c->ew_lds_tiles) {
+                    c->ew_lds_tiles = lt;
+                    c->graph_gen++;
+                }
+            }
             if ((!mixed_lists || hn[0] <= c->ew_cap32) && hn[1] <= c->ew_cap64) break;
             if (attempt > 0) return fail(c, TDP_ERR_HIP, "Ewald option: the stored T lists could not be sized");
             const int c32 = mixed_lists ? std::max(c->ew_cap32, hn[0] + hn[0] / 4 + 8) : 0;
@@ -1386,17 +1418,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             c->err = std::string("TholeDipoleForce: ") + why;
             return bail(TDP_ERR_INVALID_ARGUMENT);
         }
-        c->ew_qsum = 0.0;
-        for (double q : c->charge) c->ew_qsum += q;
-        // FP64 radius of the mixed real-space passes: where a u^3 reaches 4 for the strongest damping
-        // (u = r / (a_i a_j)), and at least 0.25 nm
-        double amax = 0.0, amin = 1e300;
-        for (double a : c->thole) {
-            amax = std::max(amax, a);
-            if (a > 0.0) amin = std::min(amin, a);
-        }
-        c->ew_r64 = 0.25;
-        if (amax > 0.0) c->ew_r64 = std::max(0.25, std::cbrt(4.0 / amin) * amax * amax);
+        ew_particle_terms(c);
     }
     if (hipStreamCreate(&c->stream) != hipSuccess) {
         c->err = "hipStreamCreate failed";
@@ -1457,9 +1479,9 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, npad)) ||
             (rc = dalloc(c, &c->d_rsw, (size_t)npad * kEwWaves)) ||
             (rc = dalloc(c, &c->d_w32, (size_t)nb * kEwWaves)) || (rc = dalloc(c, &c->d_w64, (size_t)nb * kEwWaves)) ||
-            (rc = dalloc(c, &c->d_need, 2)))
+            (rc = dalloc(c, &c->d_need, 7)))
             return bail(rc);
-        if (hipHostMalloc(&c->h_need, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&c->h_need, 7 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
             c->err = "hipHostMalloc failed";
             return bail(TDP_ERR_HIP);
         }
@@ -1868,8 +1890,7 @@ int tdp_copy_parameters(tdp_context* c, const tdp_force_desc* d) {
                             "TholeDipoleForce: the Ewald option supports charges and induced dipoles only (molecular "
                             "dipoles must be zero)");
     copy_particles(c, d);
-    c->ew_qsum = 0.0;
-    for (double q : c->charge) c->ew_qsum += q;
+    if (c->ewald) ew_particle_terms(c);
     if ((rc = upload_tables(c))) return rc;  // axis atoms may have changed: torque gather lists
     return upload_particles(c);
 }
@@ -2124,6 +2145,36 @@ int tdp_get_pme_parameters(const tdp_context* cc, double* alpha, int32_t* nx, in
     *nx = g[0];
     *ny = g[1];
     *nz = g[2];
+    return TDP_OK;
+}
+
+int tdp_get_ewald_info(const tdp_context* cc, tdp_ewald_info* info) {
+    tdp_context* c = const_cast<tdp_context*>(cc);
+    if (!c || !info) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    if (!c->ewald) return fail(c, TDP_ERR_INVALID_ARGUMENT, "the context does not use the Ewald option");
+    *info = tdp_ewald_info{};
+    info->alpha = c->ew_alpha;
+    for (int k = 0; k < 3; k++) info->grid[k] = c->ew_K[k];
+    info->cutoff = c->cutoff;
+    info->r_fp64 = c->precision == TDP_PRECISION_MIXED ? c->ew_r64 : c->cutoff;
+    info->stored = c->ew_stored ? 1 : 0;
+    info->entries32 = c->ew_entries[0];
+    info->entries64 = c->ew_entries[1];
+    info->slots32 = c->ew_slots[0];
+    info->slots64 = c->ew_slots[1];
+    info->bytes_per_entry32 = (int32_t)(sizeof(unsigned short) + sizeof(float4));
+    info->bytes_per_entry64 = (int32_t)(sizeof(int) + sizeof(double4));
+    info->tiles = c->nblk;
+    TDP_HIP(c, hipSetDevice(c->device));
+    std::vector<int> cnt(c->nblk);
+    TDP_HIP(c, hipMemcpyAsync(cnt.data(), c->d_nbr_cnt, c->nblk * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    double sum = 0.0;
+    for (int v : cnt) {
+        info->max_neighbor_tiles = std::max(info->max_neighbor_tiles, v);
+        sum += v;
+    }
+    info->mean_neighbor_tiles = c->nblk > 0 ? sum / c->nblk : 0.0;
     return TDP_OK;
 }
 

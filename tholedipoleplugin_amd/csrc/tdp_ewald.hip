@@ -509,6 +509,13 @@ __device__ __forceinline__ void ew_radial32(float alpha, float c1, float c2, flo
     }
 }
 
+// Bits w, w + kEwWaves, ..., w + 7 kEwWaves of x packed into bits 0-7 (the slots of wave w in a
+// tile's 64-bit mask): the spread bits times sum_i 2^(7 i + 7) land, without carries, in the top byte.
+__device__ __forceinline__ unsigned long long ew_slots_of_wave(unsigned long long x, int w) {
+    x = (x >> w) & 0x0101010101010101ull;
+    return (x * 0x0102040810204080ull) >> 56;
+}
+
 // The i-side terms of one unscaled pair inside the cutoff (d = x_i - x_j): the field of charge qj
 // (E0), the field of dipole vj (T v), or the force on i and half the pair's charge-charge energy.
 template <typename T, int MODE, bool MUTUAL>
@@ -553,13 +560,15 @@ __device__ __forceinline__ void ew_pair_terms(const EwR<T>& R, T dx, T dy, T dz,
 }
 
 // Row-owner real-space pass over the neighbour tiles of tile row I: one workgroup of kEwWaves
-// waves per row, the tiles dealt round-robin to the waves, each wave staging its own tile in its
-// own LDS slice (no workgroup barrier until the row's sums are added).  In a staged tile only the
-// slots within the cutoff (plus a margin) of row I's bounding box are visited, and a step whose 64
-// pairs all lie beyond the cutoff is skipped.  MIXED: pairs beyond E.r64sq are evaluated in FP32
-// (tile sums in FP32, added to the FP64 row sums); the steps that hold a closer pair are marked and
-// evaluated afterwards in FP64 for those pairs.  Output: the slot's real-space sums (field or force,
-// and the charge-charge energy) in rs.
+// waves per row.  Every wave visits every neighbour tile J and takes its 8 slots
+// k = w + kEwWaves m, eight tiles at a time staged in the wave's own LDS slice, so that each wave
+// sees an even spatial sample of the row's neighbourhood and the waves' stored lists have nearly
+// equal lengths per slot (their padding to the longest is small).  Only the slots within the
+// cutoff (plus a margin) of row I's bounding box are visited, and a step whose 64 pairs all lie
+// beyond the cutoff is skipped.  MIXED: pairs beyond E.r64sq are evaluated in FP32 (group sums in
+// FP32, added to the FP64 row sums); the steps that hold a closer pair are marked and evaluated
+// afterwards in FP64 for those pairs.  Output: the slot's real-space sums (field or force, and the
+// charge-charge energy) in rs.
 template <int MODE, bool MUTUAL, bool MIXED>
 __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const double4* __restrict__ posa,
                                                             const double* __restrict__ inva,
@@ -570,6 +579,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     if (scalars && scalars[SC_DONE] != 0.0) return;
     const int I = blockIdx.x;
     const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
+    static_assert(kEwWaves * 8 == kTile, "each wave takes 8 slots of a tile");
     __shared__ double4 s_p[kEwWaves][kTile];
     __shared__ double4 s_v[kEwWaves][kTile];   // charge in w
     __shared__ double s_ia[kEwWaves][kTile];
@@ -587,52 +597,71 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     const int* nb = E.nbr + (size_t)I * E.nblk;
     const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
     const bool build = MODE == kEwE0 && E.idx64 != nullptr;   // store T's blocks while forming E0
+    // (FP32 list indices address at most kEwMaxLocalTiles neighbour tiles: the host checks the
+    // largest count and does not use the lists beyond it)
     int n32 = 0, n64 = 0;
-    for (int t = w; t < cnt; t += kEwWaves) {
-        const int J = nb[t];
-        const int j = J * kTile + lane;
-        const double4 pl = posa[j];
-        {
-            double4 vj = (MODE == kEwE0) ? make_double4(0.0, 0.0, 0.0, 0.0) : vec[j];
-            vj.w = charge[j];
-            s_p[w][lane] = pl;
-            s_v[w][lane] = vj;
-            s_ia[w][lane] = inva[j];
+    for (int t0 = 0; t0 < cnt; t0 += 8) {
+        // a group of up to 8 neighbour tiles: lane 8g + m holds slot k = w + kEwWaves m of tile
+        // t0 + g and tests it against row I's bounding box (cutoff plus margin)
+        const int g8 = lane >> 3, m8 = lane & 7;
+        int Jl = 0;
+        double4 pl = make_double4(0.0, 0.0, 0.0, 0.0), vl = make_double4(0.0, 0.0, 0.0, 0.0);
+        double ial = 0.0;
+        bool near_box = false;
+        if (t0 + g8 < cnt) {
+            Jl = nb[t0 + g8];
+            const int ks = w + kEwWaves * m8;
+            if (ks < E.tcnt[Jl]) {
+                const int j = Jl * kTile + ks;
+                pl = posa[j];
+                if (MODE != kEwE0) vl = vec[j];
+                vl.w = charge[j];
+                ial = inva[j];
+                float dx = (float)pl.x - cI.x, dy = (float)pl.y - cI.y, dz = (float)pl.z - cI.z;
+                dx -= bxf * rintf(dx / bxf);
+                dy -= byf * rintf(dy / byf);
+                dz -= bzf * rintf(dz / bzf);
+                dx = fmaxf(0.f, fabsf(dx) - hI.x);
+                dy = fmaxf(0.f, fabsf(dy) - hI.y);
+                dz = fmaxf(0.f, fabsf(dz) - hI.z);
+                near_box = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
+            }
         }
+        const unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
+        if (!visit) continue;
+        s_p[w][lane] = pl;
+        s_v[w][lane] = vl;
+        s_ia[w][lane] = ial;
         // the slice is this wave's own: its LDS operations complete in order
         __builtin_amdgcn_wave_barrier();
-        const int cntJ = E.tcnt[J];
-        bool near_box = false;   // slot lane of tile J within the cutoff (plus margin) of row I's box
-        if (lane < cntJ) {
-            float dx = (float)pl.x - cI.x, dy = (float)pl.y - cI.y, dz = (float)pl.z - cI.z;
-            dx -= bxf * rintf(dx / bxf);
-            dy -= byf * rintf(dy / byf);
-            dz -= bzf * rintf(dz / bzf);
-            dx = fmaxf(0.f, fabsf(dx) - hI.x);
-            dy = fmaxf(0.f, fabsf(dy) - hI.y);
-            dz = fmaxf(0.f, fabsf(dz) - hI.z);
-            near_box = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
-        }
-        unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
-        // scaled pairs of this tile pair are masked (the epilogue evaluates them); the E0 pass also
-        // needs the mask of T's pairs (iScale 0) for the stored blocks
+        // scaled pairs of these tile pairs are masked (the epilogue evaluates them); the E0 pass also
+        // needs the mask of T's pairs (iScale 0) for the stored blocks.  Bit 8g + m of the group's
+        // masks is slot w + kEwWaves m of tile t0 + g.
         unsigned long long mask = 0ull, maskT = 0ull;
-        for (int e = eS; e < eE; e++) {
-            if (E.exc_J[e] != J) continue;
-            const unsigned long long* m = E.exc_mask + (size_t)e * 3 * kTile;
-            if (MODE == kEwE0) {
-                mask = m[lane] | m[kTile + lane];
-                maskT = m[2 * kTile + lane];
-            } else if (MODE == kEwTmu) {
-                mask = m[2 * kTile + lane];
-            } else {
-                mask = m[lane] | m[kTile + lane] | m[2 * kTile + lane];
+        for (int g = 0; g < 8; g++) {
+            if (!((visit >> (8 * g)) & 0xffull)) continue;
+            const int J = __builtin_amdgcn_readlane(Jl, 8 * g);
+            for (int e = eS; e < eE; e++) {
+                if (E.exc_J[e] != J) continue;
+                const unsigned long long* mm = E.exc_mask + (size_t)e * 3 * kTile;
+                unsigned long long mk, mkT = 0ull;
+                if (MODE == kEwE0) {
+                    mk = mm[lane] | mm[kTile + lane];
+                    mkT = mm[2 * kTile + lane];
+                } else if (MODE == kEwTmu) {
+                    mk = mm[2 * kTile + lane];
+                } else {
+                    mk = mm[lane] | mm[kTile + lane] | mm[2 * kTile + lane];
+                }
+                mask |= ew_slots_of_wave(mk, w) << (8 * g);
+                maskT |= ew_slots_of_wave(mkT, w) << (8 * g);
+                break;
             }
-            break;
-        }
-        if (J == I) {
-            mask |= 1ull << lane;
-            maskT |= 1ull << lane;
+            if (J == I && ((lane - w) & (kEwWaves - 1)) == 0 && lane >= w) {
+                const unsigned long long self = 1ull << (8 * g + (lane - w) / kEwWaves);
+                mask |= self;
+                maskT |= self;
+            }
         }
         if (!ivalid) mask = maskT = ~0ull;
         unsigned long long near_steps = 0ull;
@@ -641,27 +670,28 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             const float alpha32 = (float)E.alpha, c1f = (float)E.c1, c2f = (float)E.c2, c3f = (float)E.c3;
             unsigned long long v = visit;
             while (v) {
-                const int k = __builtin_ctzll(v);
+                const int m = __builtin_ctzll(v);
                 v &= v - 1ull;
-                const double4 pj = s_p[w][k];
-                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
-                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
-                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const int k = m;   // bit of the group's masks
+                const double pjx = s_p[w][m].x, pjy = s_p[w][m].y, pjz = s_p[w][m].z;
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pjx), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pjy), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pjz), E.L[2], iL2);
                 const double r2x = ew_r2(dx, dy, dz);
                 const bool inside = !((mask >> k) & 1ull) && r2x < E.rc2;
                 const bool near64 = inside && r2x < E.r64sq;
-                if (__builtin_amdgcn_ballot_w64(near64)) near_steps |= 1ull << k;
+                if (__builtin_amdgcn_ballot_w64(near64)) near_steps |= 1ull << m;
                 const bool far32 = inside && !near64;
                 if (!__builtin_amdgcn_ballot_w64(far32)) continue;
-                const double4 vj = s_v[w][k];
                 const float gx = (float)dx, gy = (float)dy, gz = (float)dz;
                 const float r2f = far32 ? gx * gx + gy * gy + gz * gz : 1.f;
                 EwR<float> R;
-                ew_radial32(alpha32, c1f, c2f, c3f, r2f, (float)pi.w, (float)pj.w, (float)iai, (float)s_ia[w][k],
-                            MODE == kEwForce && MUTUAL, R);
+                ew_radial32(alpha32, c1f, c2f, c3f, r2f, (float)pi.w, (float)s_p[w][m].w, (float)iai,
+                            (float)s_ia[w][m], MODE == kEwForce && MUTUAL, R);
                 float tx = 0.f, ty = 0.f, tz = 0.f, te = 0.f;
                 ew_pair_terms<float, MODE, MUTUAL>(R, gx, gy, gz, (float)qi, (float)vi.x, (float)vi.y, (float)vi.z,
-                                                   (float)vj.w, (float)vj.x, (float)vj.y, (float)vj.z, tx, ty, tz, te);
+                                                   (float)s_v[w][m].w, (float)s_v[w][m].x,
+                                                   (float)s_v[w][m].y, (float)s_v[w][m].z, tx, ty, tz, te);
                 if (far32) {
                     fx += tx;
                     fy += ty;
@@ -677,24 +707,25 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         double sx = 0.0, sy = 0.0, sz = 0.0, se = 0.0;
         unsigned long long v = MIXED ? near_steps : visit;
         while (v) {
-            const int k = __builtin_ctzll(v);
+            const int m = __builtin_ctzll(v);
             v &= v - 1ull;
-            const double4 pj = s_p[w][k];
-            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
-            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
-            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+            const int k = m;   // bit of the group's masks
+            const double pjx = s_p[w][m].x, pjy = s_p[w][m].y, pjz = s_p[w][m].z;
+            const double dx = ew_image_fast(__dsub_rn(pi.x, pjx), E.L[0], iL0);
+            const double dy = ew_image_fast(__dsub_rn(pi.y, pjy), E.L[1], iL1);
+            const double dz = ew_image_fast(__dsub_rn(pi.z, pjz), E.L[2], iL2);
             const double r2x = ew_r2(dx, dy, dz);
             bool sel = !((mask >> k) & 1ull) && r2x < E.rc2;
             if (MIXED) sel = sel && r2x < E.r64sq;
             const bool store = build && !((maskT >> k) & 1ull) && r2x < E.rc2;
             if (!__builtin_amdgcn_ballot_w64(sel || store)) continue;
-            const double4 vj = s_v[w][k];
             EwR<double> R;
-            ew_radial(E, (sel || store) ? r2x : 1.0, pi.w, pj.w, iai, s_ia[w][k], true, MODE == kEwForce && MUTUAL,
-                      R);
+            ew_radial(E, (sel || store) ? r2x : 1.0, pi.w, s_p[w][m].w, iai, s_ia[w][m], true,
+                      MODE == kEwForce && MUTUAL, R);
             double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
-            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, vj.w, vj.x, vj.y, vj.z, tx, ty,
-                                                tz, te);
+            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, s_v[w][m].w,
+                                                s_v[w][m].x, s_v[w][m].y, s_v[w][m].z, tx, ty, tz,
+                                                te);
             if (sel) {
                 sx += tx;
                 sy += ty;
@@ -704,23 +735,20 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             if (store) {   // the block T_ij = C2 d d^T - C1 of the T v passes (iScale 1, damped)
                 const double C1 = R.B1 + (R.l3 - 1.0) * R.b1;
                 const double C2 = R.B2 + (R.l5 - 1.0) * R.b2;
-                const double txx = C2 * dx * dx - C1, tyy = C2 * dy * dy - C1, tzz = C2 * dz * dz - C1;
-                const double txy = C2 * dx * dy, txz = C2 * dx * dz, tyz = C2 * dy * dz;
-                const int jj = J * kTile + k;
+                const double g = sqrt(fabs(C2));
                 if (E.pairs32 && r2x >= E.r64sq) {
                     if (n32 < E.cap32) {
                         const size_t at = (((size_t)I * kEwWaves + w) * E.cap32 + n32) * kTile + lane;
-                        E.idx32[at] = jj;
-                        E.a32[at] = make_float4((float)txx, (float)tyy, (float)tzz, (float)txy);
-                        E.b32[at] = make_float2((float)txz, (float)tyz);
+                        E.idx16[at] = (unsigned short)((8 * t0 + m) | (C2 < 0.0 ? 0x8000 : 0));
+                        E.t32[at] = make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz));
                     }
                     n32++;
                 } else {
                     if (n64 < E.cap64) {
                         const size_t at = (((size_t)I * kEwWaves + w) * E.cap64 + n64) * kTile + lane;
-                        E.idx64[at] = jj;
-                        E.a64[at] = make_double4(txx, tyy, tzz, txy);
-                        E.b64[at] = make_double2(txz, tyz);
+                        E.idx64[at] = (__builtin_amdgcn_readlane(Jl, m) * kTile + w + kEwWaves * (m & 7)) |
+                                      (C2 < 0.0 ? (int)0x80000000u : 0);
+                        E.t64[at] = make_double4(-C1, g * dx, g * dy, g * dz);
                     }
                     n64++;
                 }
@@ -730,34 +758,40 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         ay += sy;
         az += sz;
         ae += se;
-        __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next tile is staged
+        __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next group is staged
     }
     if (build) {   // list widths of this row and wave; the largest needed width for the capacity check
-        int m32 = n32, m64 = n64;
+        int m32 = n32, m64 = n64, s32 = n32, s64 = n64;
         for (int off = 32; off > 0; off >>= 1) {
             m32 = max(m32, __shfl_xor(m32, off, 64));
             m64 = max(m64, __shfl_xor(m64, off, 64));
+            s32 += __shfl_xor(s32, off, 64);
+            s64 += __shfl_xor(s64, off, 64);
         }
+        m32 = (m32 + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU;
         // the T v pass reads every lane's list up to the wave's width: pad the shorter ones with
         // zero blocks (on the slot itself)
         const int w32c = min(m32, E.cap32), w64c = min(m64, E.cap64);
         for (int n = n32; n < w32c; n++) {
             const size_t at = (((size_t)I * kEwWaves + w) * E.cap32 + n) * kTile + lane;
-            E.idx32[at] = i;
-            E.a32[at] = make_float4(0.f, 0.f, 0.f, 0.f);
-            E.b32[at] = make_float2(0.f, 0.f);
+            E.idx16[at] = 0;
+            E.t32[at] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         for (int n = n64; n < w64c; n++) {
             const size_t at = (((size_t)I * kEwWaves + w) * E.cap64 + n) * kTile + lane;
             E.idx64[at] = i;
-            E.a64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
-            E.b64[at] = make_double2(0.0, 0.0);
+            E.t64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
         if (lane == 0) {
             E.w32[I * kEwWaves + w] = w32c;
             E.w64[I * kEwWaves + w] = w64c;
             atomicMax(&E.need[0], m32);
             atomicMax(&E.need[1], m64);
+            atomicAdd(&E.need[2], s32);
+            atomicAdd(&E.need[3], s64);
+            atomicAdd(&E.need[4], w32c);
+            atomicAdd(&E.need[5], w64c);
+            if (w == 0) atomicMax(&E.need[6], cnt);
         }
     }
     // the row's sums over the waves, in wave order (the staging slices are reused)
@@ -778,31 +812,96 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     rs[i] = make_double4(ax, ay, az, ae);
 }
 
+typedef float ew_f4 __attribute__((ext_vector_type(4)));
+
+// FP32 lists of one wave: the sum over its entries of T_ij v_j, v_j from the wave's LDS copy
+// (LDS) or gathered through the neighbour-tile list.  Software pipeline over steps of kEwSpmvU
+// entries, three register sets by step mod 3: while step s is summed, the indices and blocks of
+// steps s+1 and s+2 are in flight (the memory counter completes in order, so waiting for step s
+// never waits for a later step).  Entries past the end of the list are read at the last entry
+// (cached) and added with weight zero: no branch in the loop.
+template <bool LDS>
+__device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, int lane, int n32,
+                                             const float4* __restrict__ s_v, const float4* __restrict__ v32,
+                                             const int* __restrict__ nb, float& fx, float& fy, float& fz) {
+    const int ns = n32 / kEwSpmvU;
+    const unsigned short* ip = E.idx16 + (size_t)row * E.cap32 * kTile + lane;
+    const ew_f4* tp = reinterpret_cast<const ew_f4*>(E.t32 + (size_t)row * E.cap32 * kTile + lane);
+    const int last = n32 - 1;
+    unsigned short j[3][kEwSpmvU];
+    ew_f4 t[3][kEwSpmvU];
+    auto load = [&](int set, int step) {
+#pragma unroll
+        for (int u = 0; u < kEwSpmvU; u++) {
+            const size_t at = (size_t)min(step * kEwSpmvU + u, last) * kTile;
+            j[set][u] = __builtin_nontemporal_load(ip + at);
+            t[set][u] = __builtin_nontemporal_load(tp + at);
+        }
+    };
+    auto sum = [&](int set, float on) {
+#pragma unroll
+        for (int u = 0; u < kEwSpmvU; u++) {
+            const int loc = j[set][u] & 0x7fff;
+            const float4 b = LDS ? s_v[loc] : v32[nb[loc >> 3] * kTile + w + kEwWaves * (loc & 7)];
+            const ew_f4 a = t[set][u];
+            float p = a.y * b.x + a.z * b.y + a.w * b.z;
+            p = (j[set][u] & 0x8000) ? -p : p;
+            const float c = on * a.x;
+            p *= on;
+            fx += c * b.x + p * a.y;
+            fy += c * b.y + p * a.z;
+            fz += c * b.z + p * a.w;
+        }
+    };
+    load(0, 0);
+    load(1, 1);
+    for (int s0 = 0; s0 < ns; s0 += 3) {
+        load(2, s0 + 2);
+        __builtin_amdgcn_sched_barrier(0);
+        sum(0, 1.f);
+        __builtin_amdgcn_sched_barrier(0);
+        load(0, s0 + 3);
+        __builtin_amdgcn_sched_barrier(0);
+        sum(1, s0 + 1 < ns ? 1.f : 0.f);
+        __builtin_amdgcn_sched_barrier(0);
+        load(1, s0 + 4);
+        __builtin_amdgcn_sched_barrier(0);
+        sum(2, s0 + 2 < ns ? 1.f : 0.f);
+        __builtin_amdgcn_sched_barrier(0);
+    }
+}
+
 // T v of the real-space pairs from the stored blocks: one wave per list (tile row I, list w), so that
-// the nblk * kEwWaves waves spread evenly over the CUs; FP32 lists on the FP32 copy of v, FP64 lists
-// on v.  Each wave writes its partial sums; the epilogue adds the kEwWaves partials of a slot in
-// list order.  Memory bound: 28 bytes per FP32 entry, 52 per FP64 entry, plus the gathered v_j.
+// the nblk * kEwWaves waves spread evenly over the CUs.  The FP32 lists read v_j from an LDS copy
+// of the wave's slots of the row's neighbour tiles (E.lds_tiles of them, 16 bytes each: staged
+// once, then every gather is an LDS read); the FP64 lists gather v_j from HBM (L2).  Each wave
+// writes its partial sums; the epilogue adds the kEwWaves partials of a slot in list order.  Memory
+// bound: 18 bytes per FP32 entry and 36 per FP64 entry stream from HBM once per pass.
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __restrict__ vec,
                                                 const float4* __restrict__ v32, const double* __restrict__ scalars,
                                                 double4* __restrict__ part) {
+    extern __shared__ float4 s_v[];   // [E.lds_tiles * 8]
     if (scalars && scalars[SC_DONE] != 0.0) return;
     const int row = blockIdx.x, lane = threadIdx.x;   // row = I kEwWaves + w
+    const int I = row / kEwWaves, w = row % kEwWaves;
     double ax = 0.0, ay = 0.0, az = 0.0;
     if (MIXED) {
-        const int n32 = E.w32[row];
-        const size_t base = (size_t)row * E.cap32 * kTile + lane;
+        const int n32 = E.w32[row];   // a multiple of kEwSpmvU
+        const int* nb = E.nbr + (size_t)I * E.nblk;
         float fx = 0.f, fy = 0.f, fz = 0.f;
-#pragma unroll 4
-        for (int n = 0; n < n32; n++) {
-            const size_t at = base + (size_t)n * kTile;
-            const int j = E.idx32[at];
-            const float4 a = E.a32[at];
-            const float2 b = E.b32[at];
-            const float4 v = v32[j];
-            fx += a.x * v.x + a.w * v.y + b.x * v.z;
-            fy += a.w * v.x + a.y * v.y + b.y * v.z;
-            fz += b.x * v.x + b.y * v.y + a.z * v.z;
+        if (n32 > 0) {
+            if (E.lds_tiles > 0) {
+                // entry 8 t + m: slot w + kEwWaves m of tile nb[t] (the wave's own LDS: its
+                // operations complete in order)
+                const int cnt = E.nbr_cnt[I];
+                for (int e = lane; e < 8 * cnt; e += kTile)
+                    s_v[e] = v32[nb[e >> 3] * kTile + w + kEwWaves * (e & 7)];
+                __builtin_amdgcn_wave_barrier();
+                ew_spmv_fp32<true>(E, row, w, lane, n32, s_v, v32, nb, fx, fy, fz);
+            } else {
+                ew_spmv_fp32<false>(E, row, w, lane, n32, s_v, v32, nb, fx, fy, fz);
+            }
         }
         ax = fx;
         ay = fy;
@@ -815,12 +914,13 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
         for (int n = 0; n < n64; n++) {
             const size_t at = base + (size_t)n * kTile;
             const int j = E.idx64[at];
-            const double4 a = E.a64[at];
-            const double2 b = E.b64[at];
-            const double4 v = vec[j];
-            ax += a.x * v.x + a.w * v.y + b.x * v.z;
-            ay += a.w * v.x + a.y * v.y + b.y * v.z;
-            az += b.x * v.x + b.y * v.y + a.z * v.z;
+            const double4 t = E.t64[at];
+            const double4 v = vec[j & 0x7fffffff];
+            double p = t.y * v.x + t.z * v.y + t.w * v.z;
+            p = j < 0 ? -p : p;
+            ax += t.x * v.x + p * t.y;
+            ay += t.x * v.y + p * t.z;
+            az += t.x * v.z + p * t.w;
         }
     }
     part[(size_t)row * kTile + lane] = make_double4(ax, ay, az, 0.0);
@@ -1096,7 +1196,8 @@ void launch_ew_tmu_stored(hipStream_t s, const EwLaunch& E, const double4* posa,
                           double4* out4, const PcgQ& pq) {
     const dim3 lists(E.nblk * kEwWaves);
     if (E.pairs32)
-        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);
+        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), (size_t)E.lds_tiles * 8 * sizeof(float4), s, E, vec, v32,
+                           scalars, E.rsw);
     else
         hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);
     hipLaunchKernelGGL((k_ew_epilogue<kEwTmu, false, true>), dim3(E.nblk), dim3(64), 0, s, E, posa, inva, charge, vec,

@@ -154,6 +154,8 @@ void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd
 constexpr int kEwOrder = 5;
 constexpr int kEwMaxStages = 12;
 constexpr int kEwWaves = 8;   // waves per tile row in the real-space passes
+constexpr int kEwSpmvU = 4;   // entries per step of the stored T v pass (FP32 lists)
+constexpr int kEwMaxLocalTiles = 4096;   // neighbour tiles an FP32 list index can address
 // FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
 struct EwAxis {
     int n, nst;
@@ -196,19 +198,24 @@ struct EwLaunch {
     double4* rsw;           // [nblk kEwWaves 64] partial T v sums of the stored lists
     // Stored real-space blocks of T (built by the E0 pass, applied by every T v pass): per tile row I
     // and wave w a list of (j, T_ij) for each of the row's 64 slots, sliced ELL: entry n of slot
-    // lane at ((I kEwWaves + w) cap + n) 64 + lane; T_ij as (xx, yy, zz, xy) and (xz, yz).  FP64
-    // lists hold every pair in FP64 mode and the pairs closer than r64 in mixed precision; FP32
-    // lists hold the other pairs in mixed precision.
+    // lane at ((I kEwWaves + w) cap + n) 64 + lane.  T_ij = C2 d d^T - C1 1 is held as
+    // (-C1, u) with u = sqrt(|C2|) d and the sign of C2 in the index's top bit, so that
+    // T_ij v = -C1 v + sign(C2) u (u . v).  FP64 lists (every pair in FP64 mode, the pairs closer
+    // than r64 in mixed precision) index j itself (int32, sign in bit 31): 4 + 32 bytes per entry.
+    // FP32 lists (the other pairs in mixed precision) index j by its place among the wave's slots
+    // of the row's neighbour tiles, 8 t + m for slot w + kEwWaves m of tile nbr[I][t] (uint16, sign
+    // in bit 15; rows with at most 4096 neighbour tiles): 2 + 16 bytes per entry; their widths are
+    // a multiple of kEwSpmvU.
     int cap32, cap64;
-    int* idx32;
-    float4* a32;
-    float2* b32;
+    unsigned short* idx16;
+    float4* t32;
     int* idx64;
-    double4* a64;
-    double2* b64;
+    double4* t64;
     int* w32;               // [nblk kEwWaves] list widths
     int* w64;
-    int* need;              // [2] largest width seen (FP32, FP64), for the capacity check
+    int* need;              // [7] largest width (FP32, FP64), entries (FP32, FP64), summed widths
+                            // (FP32, FP64), largest neighbour-tile count
+    int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
     bool mixed;             // FP32 grids and FFTs (FP64 spreading and sums)
 };
 

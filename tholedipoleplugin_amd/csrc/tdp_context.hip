@@ -1476,7 +1476,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             (rc = dalloc(c, &c->d_qfix, ng)) || (rc = dalloc(c, &c->d_cgrid, ng)) || (rc = dalloc(c, &c->d_phiq, ng)) ||
             (rc = dalloc(c, &c->d_phiv, ng)) || (rc = dalloc(c, &c->d_G64, ng)) || (rc = dalloc(c, &c->d_G32, ng)) ||
             (rc = dalloc(c, &c->d_sc_off, (size_t)npad + 1)) || (rc = dalloc(c, &c->d_epart, npad)) ||
-            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, npad)) ||
+            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, (size_t)npad * kEwParts)) ||
             (rc = dalloc(c, &c->d_rsw, (size_t)npad * kEwWaves)) ||
             (rc = dalloc(c, &c->d_w32, (size_t)nb * kEwWaves)) || (rc = dalloc(c, &c->d_w64, (size_t)nb * kEwWaves)) ||
             (rc = dalloc(c, &c->d_need, 7)))
@@ -1493,7 +1493,8 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             const double est64 = rho * ball * c->ew_r64 * c->ew_r64 * c->ew_r64;
             const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
             const int c32 = mixed_lists ? (int)std::ceil(1.5 * est / kEwWaves) + 32 : 0;
-            const int c64 = mixed_lists ? (int)std::ceil(1.5 * est64 / kEwWaves) + 16 : (int)std::ceil(1.5 * est / kEwWaves) + 32;
+            const int c64 = mixed_lists ? (int)std::ceil(1.5 * est64 / kEwWaves) + 16
+                                        : (int)std::ceil(1.5 * est / kEwWaves) + 32;
             if ((rc = ew_alloc_lists(c, c32, c64))) return bail(rc);
         }
         for (int k = 0; k < 3; k++) {

@@ -559,16 +559,18 @@ __device__ __forceinline__ void ew_pair_terms(const EwR<T>& R, T dx, T dy, T dz,
     }
 }
 
-// Row-owner real-space pass over the neighbour tiles of tile row I: one workgroup of kEwWaves
-// waves per row.  Every wave visits every neighbour tile J and takes its 8 slots
-// k = w + kEwWaves m, eight tiles at a time staged in the wave's own LDS slice, so that each wave
-// sees an even spatial sample of the row's neighbourhood and the waves' stored lists have nearly
-// equal lengths per slot (their padding to the longest is small).  Only the slots within the
-// cutoff (plus a margin) of row I's bounding box are visited, and a step whose 64 pairs all lie
-// beyond the cutoff is skipped.  MIXED: pairs beyond E.r64sq are evaluated in FP32 (group sums in
-// FP32, added to the FP64 row sums); the steps that hold a closer pair are marked and evaluated
-// afterwards in FP64 for those pairs.  Output: the slot's real-space sums (field or force, and the
-// charge-charge energy) in rs.
+// Row-owner real-space pass over the neighbour tiles of tile row I: NP workgroups of kEwWaves waves
+// per row, part h taking the neighbour tiles h, h + NP, ... (the T v and force passes use two parts:
+// twice as many workgroups as rows fill the CUs more evenly, one 8-wave workgroup per CU at these
+// register counts; the E0 pass, which builds the stored lists, one, for longer lists).  Every wave
+// visits every tile of its part and takes its 8 slots k = w + kEwWaves m, eight tiles at a time
+// staged in the wave's own LDS slice, so that each wave sees an even spatial sample of the row's
+// neighbourhood and the waves' stored lists have nearly equal lengths per slot (their padding to
+// the longest is small).  Only the slots within the cutoff (plus a margin) of row I's bounding box
+// are visited, and a step whose 64 pairs all lie beyond the cutoff is skipped.  MIXED: pairs beyond
+// E.r64sq are evaluated in FP32 (group sums in FP32, added to the FP64 row sums); the steps that
+// hold a closer pair are marked and evaluated afterwards in FP64 for those pairs.  Output: the
+// part's real-space sums for the slot (field or force, and the charge-charge energy) in rs.
 template <int MODE, bool MUTUAL, bool MIXED>
 __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const double4* __restrict__ posa,
                                                             const double* __restrict__ inva,
@@ -577,7 +579,8 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
                                                             const double* __restrict__ scalars,
                                                             double4* __restrict__ rs) {
     if (scalars && scalars[SC_DONE] != 0.0) return;
-    const int I = blockIdx.x;
+    constexpr int NP = ew_parts(MODE);
+    const int I = blockIdx.x / NP, h = blockIdx.x % NP;
     const int lane = threadIdx.x & (kTile - 1), w = threadIdx.x / kTile;
     static_assert(kEwWaves * 8 == kTile, "each wave takes 8 slots of a tile");
     __shared__ double4 s_p[kEwWaves][kTile];
@@ -593,9 +596,11 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     const float bxf = (float)E.L[0], byf = (float)E.L[1], bzf = (float)E.L[2];
     const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
     double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0;
-    const int cnt = E.nbr_cnt[I];
-    const int* nb = E.nbr + (size_t)I * E.nblk;
+    // this part's neighbour tiles: t = NP l + h of the row's list, l < cnt
+    const int cnt = (E.nbr_cnt[I] - h + NP - 1) / NP;
+    const int* nb = E.nbr + (size_t)I * E.nblk + h;
     const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
+    const int list = I * kEwWaves + w;   // (the E0 pass, which builds the lists, has one part)
     const bool build = MODE == kEwE0 && E.idx64 != nullptr;   // store T's blocks while forming E0
     // (FP32 list indices address at most kEwMaxLocalTiles neighbour tiles: the host checks the
     // largest count and does not use the lists beyond it)
@@ -609,7 +614,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         double ial = 0.0;
         bool near_box = false;
         if (t0 + g8 < cnt) {
-            Jl = nb[t0 + g8];
+            Jl = nb[NP * (t0 + g8)];
             const int ks = w + kEwWaves * m8;
             if (ks < E.tcnt[Jl]) {
                 const int j = Jl * kTile + ks;
@@ -738,14 +743,14 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
                 const double g = sqrt(fabs(C2));
                 if (E.pairs32 && r2x >= E.r64sq) {
                     if (n32 < E.cap32) {
-                        const size_t at = (((size_t)I * kEwWaves + w) * E.cap32 + n32) * kTile + lane;
+                        const size_t at = ((size_t)list * E.cap32 + n32) * kTile + lane;
                         E.idx16[at] = (unsigned short)((8 * t0 + m) | (C2 < 0.0 ? 0x8000 : 0));
                         E.t32[at] = make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz));
                     }
                     n32++;
                 } else {
                     if (n64 < E.cap64) {
-                        const size_t at = (((size_t)I * kEwWaves + w) * E.cap64 + n64) * kTile + lane;
+                        const size_t at = ((size_t)list * E.cap64 + n64) * kTile + lane;
                         E.idx64[at] = (__builtin_amdgcn_readlane(Jl, m) * kTile + w + kEwWaves * (m & 7)) |
                                       (C2 < 0.0 ? (int)0x80000000u : 0);
                         E.t64[at] = make_double4(-C1, g * dx, g * dy, g * dz);
@@ -773,18 +778,18 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         // zero blocks (on the slot itself)
         const int w32c = min(m32, E.cap32), w64c = min(m64, E.cap64);
         for (int n = n32; n < w32c; n++) {
-            const size_t at = (((size_t)I * kEwWaves + w) * E.cap32 + n) * kTile + lane;
+            const size_t at = ((size_t)list * E.cap32 + n) * kTile + lane;
             E.idx16[at] = 0;
             E.t32[at] = make_float4(0.f, 0.f, 0.f, 0.f);
         }
         for (int n = n64; n < w64c; n++) {
-            const size_t at = (((size_t)I * kEwWaves + w) * E.cap64 + n) * kTile + lane;
+            const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
             E.idx64[at] = i;
             E.t64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
         }
         if (lane == 0) {
-            E.w32[I * kEwWaves + w] = w32c;
-            E.w64[I * kEwWaves + w] = w64c;
+            E.w32[list] = w32c;
+            E.w64[list] = w64c;
             atomicMax(&E.need[0], m32);
             atomicMax(&E.need[1], m64);
             atomicAdd(&E.need[2], s32);
@@ -809,7 +814,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         az += red[(u * 4 + 2) * kTile + lane];
         ae += red[(u * 4 + 3) * kTile + lane];
     }
-    rs[i] = make_double4(ax, ay, az, ae);
+    rs[(size_t)h * E.npad + i] = make_double4(ax, ay, az, ae);
 }
 
 typedef float ew_f4 __attribute__((ext_vector_type(4)));
@@ -871,7 +876,7 @@ __device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, 
     }
 }
 
-// T v of the real-space pairs from the stored blocks: one wave per list (tile row I, list w), so that
+// T v of the real-space pairs from the stored blocks: one wave per list (tile row I, wave w), so that
 // the nblk * kEwWaves waves spread evenly over the CUs.  The FP32 lists read v_j from an LDS copy
 // of the wave's slots of the row's neighbour tiles (E.lds_tiles of them, 16 bytes each: staged
 // once, then every gather is an LDS read); the FP64 lists gather v_j from HBM (L2).  Each wave
@@ -892,7 +897,7 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
         float fx = 0.f, fy = 0.f, fz = 0.f;
         if (n32 > 0) {
             if (E.lds_tiles > 0) {
-                // entry 8 t + m: slot w + kEwWaves m of tile nb[t] (the wave's own LDS: its
+                // entry 8 t + m: slot w + kEwWaves m of neighbour tile t (the wave's own LDS: its
                 // operations complete in order)
                 const int cnt = E.nbr_cnt[I];
                 for (int e = lane; e < 8 * cnt; e += kTile)
@@ -959,12 +964,19 @@ __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* _
             ay += r.y;
             az += r.z;
         }
-    } else {
+    } else {   // the row parts' sums, in part order
         const double4 r0 = rs[i];
         ax = r0.x;
         ay = r0.y;
         az = r0.z;
         ae = r0.w;
+        for (int h = 1; h < ew_parts(MODE); h++) {
+            const double4 r = rs[(size_t)h * E.npad + i];
+            ax += r.x;
+            ay += r.y;
+            az += r.z;
+            ae += r.w;
+        }
     }
     const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
     if (valid) {
@@ -1164,11 +1176,11 @@ void launch_ew_recip(hipStream_t s, const EwLaunch& E, const double4* posa, cons
 void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, const double4* posa, const double* inva,
                      const double* charge, const double4* vec, const double4* E0, const double* scalars, double4* out4,
                      const PcgQ& pq, double* epart, double* frc, double* eout) {
-    const dim3 grid(E.nblk), block(kEwWaves * 64);
+    const dim3 grid(E.nblk * (mode == kEwE0 ? ew_parts(kEwE0) : kEwParts)), block(kEwWaves * 64), rows(E.nblk);
 #define TDP_EW_PAIRS(M, MU, MX) \
     hipLaunchKernelGGL((k_ew_pairs<M, MU, MX>), grid, block, 0, s, E, posa, inva, charge, vec, scalars, E.rs)
 #define TDP_EW_EPI(M, MU)                                                                                      \
-    hipLaunchKernelGGL((k_ew_epilogue<M, MU>), grid, dim3(64), 0, s, E, posa, inva, charge, vec, E0, scalars, E.rs, \
+    hipLaunchKernelGGL((k_ew_epilogue<M, MU>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0, scalars, E.rs, \
                        out4, pq, epart, frc, eout)
     // the permanent field stays in FP64 in both precisions (cancellation of the molecules' charge fields)
     if (mode == kEwE0) {

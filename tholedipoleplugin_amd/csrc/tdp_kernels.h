@@ -153,7 +153,10 @@ void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd
 // (order 5) in reciprocal space.
 constexpr int kEwOrder = 5;
 constexpr int kEwMaxStages = 12;
-constexpr int kEwWaves = 8;   // waves per tile row in the real-space passes
+constexpr int kEwWaves = 8;   // waves per workgroup of the real-space passes
+constexpr int kEwParts = 2;   // workgroups per tile row of the T v and force passes (part h takes the
+                              // neighbour tiles h, h + kEwParts, ...); the E0 pass has one
+constexpr int ew_parts(int mode) { return mode == 0 ? 1 : kEwParts; }   // mode: EwMode (kEwE0 is 0)
 constexpr int kEwSpmvU = 4;   // entries per step of the stored T v pass (FP32 lists)
 constexpr int kEwMaxLocalTiles = 4096;   // neighbour tiles an FP32 list index can address
 // FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
@@ -194,7 +197,8 @@ struct EwLaunch {
     void* cgrid;            // complex work grid (double2 or float2)
     void* phi_q;            // potential grid of the charges (kept from the E0 pass)
     void* phi_v;            // potential grid of the last dipole pass
-    double4* rs;            // [npad] real-space sums of the last pass (field or force, energy in w)
+    double4* rs;            // [kEwParts npad] real-space sums of the last pass per row part (field or force,
+                            // energy in w)
     double4* rsw;           // [nblk kEwWaves 64] partial T v sums of the stored lists
     // Stored real-space blocks of T (built by the E0 pass, applied by every T v pass): per tile row I
     // and wave w a list of (j, T_ij) for each of the row's 64 slots, sliced ELL: entry n of slot

@@ -1,18 +1,23 @@
 """Benchmark: force + induction evaluations per second of the 36,864-site water box.
 
-Workload: 12,288 rigid 3-site waters (36,864 sites) in a 4.95 nm cubic box, PME method with the
-reference's semantics (all pairs, minimum image), Mutual polarization solved to an RMS dipole
-update of 1e-6 D (2.08e-8 e*nm).  One step = one complete evaluation (frame rotation, permanent
-field, induced-dipole solve, energy, forces, torque mapping) with positions and forces resident
-in HBM (tdp_execute_device).
+Workload: 12,288 rigid 3-site waters (36,864 sites) in a 4.95 nm cubic box, PME with a 1.0 nm
+cutoff, a 64^3 grid and B-splines of order 5 (the Ewald option: real space within the cutoff,
+smooth PME in reciprocal space, Ewald parameter from the error tolerance 1e-4), Mutual polarization
+solved by PCG to an RMS dipole update of 1e-6 D (2.08e-8 e*nm).  One step = one complete
+evaluation (neighbour tiles, permanent field with the stored real-space blocks of T, reciprocal
+space, induced-dipole solve, energy, forces) with positions and forces resident in HBM
+(tdp_execute_device).
 
-The reported value is the mixed-precision path (T*mu matvecs with tile pairs beyond the Thole
-range in FP32 and the rest in FP64, all partial sums in FP64, PCG with FP64 norms); the FP64
-path with the same solver is timed beside it.  The roofline entry covers the dominant kernel, the
-T*mu pass (k_sym_matvec_mixed, the pair terms), timed with HIP events on the context's stream over
-a timed region of its own; its fixed-order reduction is a separate, HBM-bound kernel.  Its ceiling is the compute roofline at
-MI355X's published dense FP32 rate: the pair arithmetic runs on the vector ALUs.  The CPU baseline is the oracle (a C restatement of the reference loops)
-timed on a bounded sample of the same workload's pair passes.
+The reported value is the mixed-precision path (real-space pairs beyond the Thole range in FP32,
+near pairs, reciprocal space, sums and PCG norms in FP64); the same evaluation fully in FP64 is
+timed beside it.  The roofline entry covers the T*mu kernel of the solve (k_ew_spmv, the stored
+real-space blocks applied to the dipoles), timed with HIP events on the context's stream over a
+timed region of its own; it is memory bound (ceiling: MI355X's 8 TB/s).  The CPU baseline is the
+oracle's Ewald evaluation (C, FP64, cell lists, SPME) of the same system on the host's threads.
+
+Also measured: the same box with the reference's PME semantics (minimum image over all pairs,
+no cutoff), the 96,000-site saline box in both models, a warm-started trajectory, the batched
+cluster solve and the dense cluster responses.
 """
 import argparse
 import json
@@ -25,20 +30,22 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-# Algorithmic work of one T*mu pass: per unordered pair (N(N-1)/2 of them) the reference forms the
-# minimum-image separation and the damped rr3/rr5 once (3 sub + 15 min-image + 5 r^2 + 1 rsqrt
-# + 5 powers + 7 damping-test = 36 flops) and accumulates both directions
-# (2 x (6 for mu.Delta and its scaling + 12 for the three components) = 36 flops): 72 flops.
+# All-pairs (reference semantics) T*mu pass: per unordered pair (N(N-1)/2 of them) the minimum-image
+# separation and the damped rr3/rr5 once (3 sub + 15 min-image + 5 r^2 + 1 rsqrt + 5 powers + 7
+# damping-test = 36 flops) and both directions accumulated (2 x 18 = 36 flops): 72 flops.
 FLOPS_PER_PAIR_MATVEC = 72.0
 PEAK_FP32_TFLOPS = 157.3
 PEAK_FP16_TFLOPS = 2500.0
 PEAK_HBM_GBS = 8000.0
 DEBYE = 0.020819434  # e*nm
-# HBM bytes per launch of k_sym_matvec_mixed on this workload, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 98.8 MB
+# HBM bytes per launch of k_sym_matvec_mixed (all-pairs model) on this box, from rocprofv3 FETCH_SIZE
+# and WRITE_SIZE counters scaled by a 512 MiB copy probe (fetch x2, write x1), default schedule: 98.8 MB
 # read + 166.8 MB written (the FP32 tile partial slab, 128 MB, and the FP32 work-item partials).
 # Counters cannot be read inside this process.
 MATVEC_HBM_BYTES = 265.6e6
+# HBM bytes per launch of k_ew_spmv<true> on this box from the same counters and scaling (None
+# until measured).
+SPMV_HBM_BYTES = None
 
 
 def parse():
@@ -54,11 +61,16 @@ def parse():
     p.add_argument("--no-dense", action="store_true", help="skip the dense cluster-response measurement")
     p.add_argument("--no-saline", action="store_true", help="skip the 96,000-site saline box measurement")
     p.add_argument("--no-md", action="store_true", help="skip the warm-started trajectory measurement")
+    p.add_argument("--no-allpairs", action="store_true",
+                   help="skip the all-pairs (reference PME semantics) measurement of the 36,864-site box")
+    p.add_argument("--pmode-only", action="store_true",
+                   help="run only the Ewald evaluation of the 36,864-site box (mixed and FP64) and print its line")
     p.add_argument("--no-autotune", action="store_true",
                    help="keep each context's default schedule (no tdp_autotune before the warm-up)")
     p.add_argument("--dense-only", action="store_true",
                    help="run only the dense cluster-response measurement and print its object (for profiling)")
-    p.add_argument("--cpu-seconds", type=float, default=12.0, help="approximate CPU time budget per pass sample")
+    p.add_argument("--cpu-seconds", type=float, default=4.0,
+                   help="approximate CPU time budget per pass sample of the all-pairs CPU baseline")
     p.add_argument("--copy-probe-mib", type=int, default=0,
                    help="after the run, stream this many MiB through a plain device copy (y = x * 1 in float32) "
                         "three times: the reference kernel for scaling HBM byte counters")
@@ -70,15 +82,16 @@ def build_system():
     return systems.config1()
 
 
-def make_handle(s, precision):
+def make_handle(s, precision, ewald=False):
     from tholedipoleplugin_amd import _lib
-    return _lib.HipContextHandle(s.force, s.box, precision=precision, solver="pcg", device=0)
+    return _lib.HipContextHandle(s.force, s.box, precision=precision, solver="pcg", device=0,
+                                 flags=_lib.FLAG_EWALD if ewald else 0)
 
 
-def time_path(torch, s, precision, steps, warmup, timing=False):
-    h = make_handle(s, precision)
-    # schedule chosen on this configuration, untimed
-    tiles_per_item = 0 if NO_AUTOTUNE else h.autotune(s.positions, s.box)
+def time_path(torch, s, precision, steps, warmup, timing=False, ewald=False):
+    h = make_handle(s, precision, ewald)
+    # schedule chosen on this configuration, untimed (the all-pairs passes' work-item length)
+    tiles_per_item = 0 if (NO_AUTOTUNE or ewald) else h.autotune(s.positions, s.box)
     dev = torch.device("cuda:0")
     pos = torch.tensor(s.positions.reshape(-1), dtype=torch.float64, device=dev)
     frc = torch.zeros(pos.numel(), dtype=torch.float64, device=dev)
@@ -102,9 +115,10 @@ def time_path(torch, s, precision, steps, warmup, timing=False):
     from tholedipoleplugin_amd import _lib
     e, f = h.execute(s.positions, s.box)
     mu = h.dipoles(s.positions, s.box, _lib.DIPOLES_INDUCED)
+    info = h.ewald_info() if ewald else None
     h.close()
     stats["tiles_per_item"] = tiles_per_item
-    return dict(seconds=dt, ms_per_step=1000.0 * dt / steps, stats=stats, energy=e, forces=f, mu=mu)
+    return dict(seconds=dt, ms_per_step=1000.0 * dt / steps, stats=stats, energy=e, forces=f, mu=mu, info=info)
 
 
 def cpu_baseline(s, mu, iterations, budget_s):
@@ -114,7 +128,7 @@ def cpu_baseline(s, mu, iterations, budget_s):
     import oracle
     n = s.n
     total_pairs = n * (n - 1) / 2.0
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads = cpu_threads()
     # calibrate the slice size on the first pass, then spread slices over the row range
     rates = {}
     for which in (0, 1, 2):
@@ -130,11 +144,39 @@ def cpu_baseline(s, mu, iterations, budget_s):
     t_matvec = total_pairs / rates[1]
     t_elec = total_pairs / rates[2]
     t_eval = t_field + iterations * t_matvec + t_elec
-    return dict(value=1.0 / t_eval, unit="evals/s", cores=threads, kind="port",
-                sample=("oracle pair passes over 8 row slices per pass (~%.0f s CPU each) of the 36,864-site box; "
-                        "per-pass times %.1f s field, %.1f s T*mu, %.1f s energy/force, extrapolated per eval with "
-                        "%d T*mu passes (the GPU PCG count; the reference's Jacobi iteration diverges on this "
-                        "system)" % (budget_s / 3.0, t_field, t_matvec, t_elec, iterations)))
+    return dict(value=1.0 / t_eval, unit="evals/s", cores=threads, kind="port", host_cpus=os.cpu_count(),
+                sample=("oracle pair passes of the 36,864-site box (all pairs, minimum image), each timed on 8 "
+                        "slices of rows holding about %.0f s of work: per-pass times %.1f s field, %.1f s T*mu, "
+                        "%.1f s energy/force, per eval t_field + %d t_T*mu + t_energy (the GPU PCG pass count; the "
+                        "reference's Jacobi iteration diverges on this system)"
+                        % (budget_s / 3.0, t_field, t_matvec, t_elec, iterations)))
+
+
+def cpu_threads():
+    """Host threads for the CPU baselines: every CPU the process may use, at most 16 (the share of one
+    GPU on the GPU hosts)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline_ewald(s, alpha, grid):
+    """The oracle's Ewald evaluation (FP64 C: cell-list real space, SPME of order 5, PCG to the same
+    tolerance) of the whole system, once, on the host threads."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = cpu_threads()
+    t0 = time.perf_counter()
+    r = oracle.ewald_evaluate(s.force, s.positions, s.box, s.force.getCutoffDistance(), alpha, grid=grid, order=5,
+                              threads=threads)
+    dt = time.perf_counter() - t0
+    return dict(value=1.0 / dt, unit="evals/s", cores=threads, kind="port", host_cpus=os.cpu_count(),
+                sample="one full evaluation of the 36,864-site box by the oracle (FP64; cell-list real space, SPME "
+                       "%dx%dx%d order 5, PCG %d iterations to 1e-6 D): %.1f s on %d threads"
+                       % (grid[0], grid[1], grid[2], r["iterations"], dt, threads),
+                energy=r["energy"], forces=r["forces"], induced=r["induced"])
 
 
 def cluster_systems():
@@ -272,38 +314,40 @@ def dense_bench(steps):
 
 def saline_bench(torch, steps):
     """96,000-site box of water with 0.15 M Na+/Cl- (7.1 nm, Thole a in [0.3, 0.45]), Mutual PCG to
-    1e-6 D: mixed and FP64 evaluations per second and the mixed result's error against FP64."""
+    1e-6 D, with the Ewald option (cutoff 1.0 nm, 96^3 grid, order 5) and with the reference's
+    all-pairs semantics: mixed and FP64 evaluations per second and the mixed result's error against
+    FP64."""
     from tholedipoleplugin_amd import systems
     s = systems.config2()
-    mixed = time_path(torch, s, "mixed", steps, 2)
-    print("# saline96000 mixed: %.2f ms/step, %s" % (mixed["ms_per_step"], mixed["stats"]), file=sys.stderr,
-          flush=True)
-    fp64 = time_path(torch, s, "double", steps, 2)
-    print("# saline96000 fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr,
-          flush=True)
-    a, b = mixed["forces"].reshape(-1), fp64["forces"].reshape(-1)
-    return {
-        "workload": "96,000 sites: 31,978 waters + 33 Na+/Cl- pairs, 7.1 nm box, min-image all pairs, Thole a in "
-                    "[0.3, 0.45], Mutual PCG to 1e-6 D",
-        "unit": "evals/s",
-        "mixed": 1000.0 / mixed["ms_per_step"],
-        "fp64": 1000.0 / fp64["ms_per_step"],
-        "mixed_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
-        "pcg_iterations": mixed["stats"]["iterations"],
-        "mixed_force_rel_rms_err_vs_fp64": float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))),
-        "mixed_energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
-        "mixed_mu_max_err_vs_fp64_debye": float(np.max(np.abs(mixed["mu"] - fp64["mu"]))) / DEBYE,
-    }
+    out = {"workload": "96,000 sites: 31,978 waters + 33 Na+/Cl- pairs, 7.1 nm box, Thole a in [0.3, 0.45], "
+                       "Mutual PCG to 1e-6 D", "unit": "evals/s"}
+    for model, ewald in (("ewald_cutoff1.0_grid96", True), ("allpairs_min_image", False)):
+        mixed = time_path(torch, s, "mixed", steps, 2, ewald=ewald)
+        print("# saline96000 %s mixed: %.2f ms/step, %s" % (model, mixed["ms_per_step"], mixed["stats"]),
+              file=sys.stderr, flush=True)
+        fp64 = time_path(torch, s, "double", steps, 2, ewald=ewald)
+        print("# saline96000 %s fp64: %.2f ms/step, %s" % (model, fp64["ms_per_step"], fp64["stats"]),
+              file=sys.stderr, flush=True)
+        a, b = mixed["forces"].reshape(-1), fp64["forces"].reshape(-1)
+        out[model] = {
+            "mixed": 1000.0 / mixed["ms_per_step"],
+            "fp64": 1000.0 / fp64["ms_per_step"],
+            "mixed_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+            "pcg_iterations": mixed["stats"]["iterations"],
+            "mixed_force_rel_rms_err_vs_fp64": float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2))),
+            "mixed_energy_rel_err_vs_fp64": abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"]),
+            "mixed_mu_max_err_vs_fp64_debye": float(np.max(np.abs(mixed["mu"] - fp64["mu"]))) / DEBYE,
+        }
+    return out
 
 
 def md_bench(torch, s, steps):
     """Evaluations along a trajectory-like sequence of configurations of the 36,864-site box: before
     each evaluation every coordinate moves by a seeded Gaussian step of 0.0005 nm (about the
     displacement of one ~1 fs MD step), and each Mutual solve starts from the previous evaluation's
-    dipoles (tdp_set_warm_start), as an integrator loop would run it.  Mixed precision, PCG."""
-    h = make_handle(s, "mixed")
-    if not NO_AUTOTUNE:
-        h.autotune(s.positions, s.box)
+    dipoles (tdp_set_warm_start), as an integrator loop would run it.  Ewald option, mixed precision,
+    PCG."""
+    h = make_handle(s, "mixed", ewald=True)
     h.set_warm_start(True)
     dev = torch.device("cuda:0")
     pos = torch.tensor(s.positions.reshape(-1), dtype=torch.float64, device=dev)
@@ -327,8 +371,9 @@ def md_bench(torch, s, steps):
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     h.close()
-    out = {"workload": "36,864-site box, every coordinate moved by N(0, 0.0005 nm) before each evaluation, "
-                       "Mutual solve warm-started from the previous dipoles, mixed PCG to 1e-6 D",
+    out = {"workload": "36,864-site box (Ewald option, cutoff 1.0 nm, 64^3 grid), every coordinate moved by "
+                       "N(0, 0.0005 nm) before each evaluation, Mutual solve warm-started from the previous "
+                       "dipoles, mixed PCG to 1e-6 D",
            "unit": "evals/s", "evals_per_s": steps / dt, "ms_per_step": 1000.0 * dt / steps,
            "pcg_iterations_mean": float(np.mean(iters)), "pcg_iterations_max": int(np.max(iters)), "steps": steps}
     print("# md warm start: %.2f ms/step, PCG iterations %.1f (max %d)" % (out["ms_per_step"], out["pcg_iterations_mean"],
@@ -338,6 +383,78 @@ def md_bench(torch, s, steps):
 
 
 NO_AUTOTUNE = False
+
+
+def errors(a, b):
+    """Mixed result a against FP64 result b: force relative RMS, energy relative, dipole max (D)."""
+    fa, fb = a["forces"].reshape(-1), b["forces"].reshape(-1)
+    return (float(np.sqrt(np.mean((fa - fb) ** 2)) / np.sqrt(np.mean(fb ** 2))),
+            abs(a["energy"] - b["energy"]) / abs(b["energy"]),
+            float(np.max(np.abs(a["mu"] - b["mu"]))) / DEBYE)
+
+
+def spmv_roofline(prof, n):
+    """HBM roofline of the stored-block T*mu kernel from the timed pass: algorithmic bytes per launch
+    (each stored block read once: 18 bytes per FP32 entry, 36 per FP64 entry; v once in FP32 and
+    FP64; the list partials written once) over its average launch time."""
+    st, info = prof["stats"], prof["info"]
+    launches = max(st["matvec_full_launches"], 1)
+    avg_ms = st["matvec_ms_full"] / launches
+    lists = info["tiles"] * 8
+    other = n * (16 + 32) + lists * 64 * 32
+    algo = info["entries32"] * info["bytes_per_entry32"] + info["entries64"] * info["bytes_per_entry64"] + other
+    streamed = info["slots32"] * info["bytes_per_entry32"] + info["slots64"] * info["bytes_per_entry64"] + other
+    achieved = algo / (avg_ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": achieved / PEAK_HBM_GBS, "traffic": SPMV_HBM_BYTES,
+            "kernel": "k_ew_spmv<true> (stored real-space blocks of T applied to the dipoles, one PCG T*mu)",
+            "algorithmic_bytes": algo, "streamed_bytes_with_padding": streamed,
+            "entries_fp32": info["entries32"], "entries_fp64": info["entries64"],
+            "bytes_per_entry": {"fp32": info["bytes_per_entry32"], "fp64": info["bytes_per_entry64"]},
+            "avg_launch_ms": avg_ms, "full_launches": st["matvec_full_launches"],
+            "launches": st["matvec_launches"]}
+
+
+def allpairs_bench(torch, s, steps, warmup, no_fp64, no_cpu, cpu_seconds):
+    """The same box with the reference's PME semantics (minimum image over all pairs, no cutoff):
+    mixed and FP64 evaluations per second, the T*mu pass against the FP32 compute roofline, and the
+    oracle's pair passes as CPU baseline."""
+    n = s.n
+    mixed = time_path(torch, s, "mixed", steps, warmup)
+    print("# allpairs mixed: %.2f ms/step, %s" % (mixed["ms_per_step"], mixed["stats"]), file=sys.stderr, flush=True)
+    n_prof = max(1, min(steps, 3))
+    prof = time_path(torch, s, "mixed", n_prof, 0, timing=True)
+    st = prof["stats"]
+    avg_ms = st["matvec_ms_full"] / max(st["matvec_full_launches"], 1)
+    flops = FLOPS_PER_PAIR_MATVEC * n * (n - 1) / 2.0
+    achieved_tflops = flops / (avg_ms * 1e-3) / 1e12
+    out = {"workload": "36,864 sites, all pairs under the minimum image (the reference's PME semantics), Mutual PCG "
+                       "to 1e-6 D", "unit": "evals/s", "mixed": 1000.0 / mixed["ms_per_step"],
+           "ms_per_step": mixed["ms_per_step"], "pcg_iterations": mixed["stats"]["iterations"],
+           "tiles_per_work_item": mixed["stats"]["tiles_per_item"],
+           "roofline": {"bound": "valu", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": MATVEC_HBM_BYTES,
+                        "kernel": "k_sym_matvec_mixed<true> (pair terms of one T*mu pass)",
+                        "ceiling": "the published dense FP32 rate; the pair terms run on the vector ALUs",
+                        "flops_per_pair": FLOPS_PER_PAIR_MATVEC, "pairs_per_launch": n * (n - 1) // 2,
+                        "avg_launch_ms": avg_ms, "full_launches": st["matvec_full_launches"]}}
+    if not no_fp64:
+        fp64 = time_path(torch, s, "double", steps, warmup)
+        print("# allpairs fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr,
+              flush=True)
+        e = errors(mixed, fp64)
+        out.update({"fp64": 1000.0 / fp64["ms_per_step"], "mixed_vs_fp64": fp64["ms_per_step"] / mixed["ms_per_step"],
+                    "mixed_force_rel_rms_err_vs_fp64": e[0], "mixed_energy_rel_err_vs_fp64": e[1],
+                    "mixed_mu_max_err_vs_fp64_debye": e[2]})
+    if not no_cpu:
+        from tholedipoleplugin_amd import _lib
+        h = _lib.HipContextHandle(s.force, s.box, precision="double", solver="pcg")
+        mu = h.dipoles(s.positions, s.box, _lib.DIPOLES_INDUCED)
+        iters = h.solver_stats()["iterations"] + 1
+        h.close()
+        out["cpu_baseline"] = cpu_baseline(s, mu, iters, cpu_seconds * 3.0)
+        print("# allpairs cpu baseline: %s" % out["cpu_baseline"], file=sys.stderr, flush=True)
+    return out
 
 
 def main():
@@ -360,57 +477,40 @@ def main():
         return
     s = build_system()
     n = s.n
-    print("# workload: %d sites, box %.2f nm; warmup %d, steps %d" % (n, s.box[0, 0], args.warmup, args.steps),
+    h = make_handle(s, "double", ewald=True)
+    alpha, nx, ny, nz = h.pme_parameters()
+    h.close()
+    print("# workload: %d sites, box %.2f nm, cutoff %.2f nm, alpha %.4f /nm, grid %dx%dx%d; warmup %d, steps %d"
+          % (n, s.box[0, 0], s.force.getCutoffDistance(), alpha, nx, ny, nz, args.warmup, args.steps),
           file=sys.stderr, flush=True)
-    mixed = time_path(torch, s, "mixed", args.steps, args.warmup, timing=False)
-    print("# mixed: %.2f ms/step, %s" % (mixed["ms_per_step"], mixed["stats"]), file=sys.stderr, flush=True)
+    mixed = time_path(torch, s, "mixed", args.steps, args.warmup, ewald=True)
+    print("# ewald mixed: %.2f ms/step, %s" % (mixed["ms_per_step"], mixed["stats"]), file=sys.stderr, flush=True)
     # separate timed pass for the kernel roofline (events around every T*mu launch)
-    n_prof = max(1, min(args.steps, 3))
-    prof = time_path(torch, s, "mixed", n_prof, 0, timing=True)
-    st = prof["stats"]
-    # PCG makes iterations + 1 full passes per solve; launches enqueued after the solve stopped
-    # return at once (~13 us each) and are not passes, but their time stays in the total
-    passes = n_prof * (st["iterations"] + 1)
-    avg_ms = st["matvec_ms"] / max(passes, 1)
-    flops = FLOPS_PER_PAIR_MATVEC * n * (n - 1) / 2.0
-    achieved_tflops = flops / (avg_ms * 1e-3) / 1e12
-    print("# matvec: %d launches, %d full passes, avg %.3f ms per pass, %.1f TFLOP/s algorithmic" % (
-        st["matvec_launches"], passes, avg_ms, achieved_tflops), file=sys.stderr, flush=True)
+    prof = time_path(torch, s, "mixed", max(1, min(args.steps, 3)), 0, timing=True, ewald=True)
+    roof = spmv_roofline(prof, n)
+    print("# T*mu (k_ew_spmv): %d full launches, avg %.1f us, %.0f GB/s algorithmic (%.2f of peak)"
+          % (roof["full_launches"], roof["avg_launch_ms"] * 1e3, roof["achieved"], roof["frac"]), file=sys.stderr,
+          flush=True)
     fp64 = None
     if not args.no_fp64:
-        fp64 = time_path(torch, s, "double", args.steps, args.warmup)
-        print("# fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr, flush=True)
-    err = mu_err = e_err = None
-    if fp64 is not None:
-        a, b = mixed["forces"].reshape(-1), fp64["forces"].reshape(-1)
-        err = float(np.sqrt(np.mean((a - b) ** 2)) / np.sqrt(np.mean(b ** 2)))
-        mu_err = float(np.max(np.abs(mixed["mu"] - fp64["mu"]))) / DEBYE
-        e_err = abs(mixed["energy"] - fp64["energy"]) / abs(fp64["energy"])
-    clusters = None
-    if not args.no_clusters:
-        clusters = cluster_bench(torch, max(1, min(args.steps, 10)), 3)
-    dense = None
-    if not args.no_dense:
-        dense = dense_bench(max(1, min(args.steps, 2)))
-    md = None
-    if not args.no_md:
-        md = md_bench(torch, s, args.steps)
-    saline = None
-    if not args.no_saline:
-        saline = saline_bench(torch, max(2, min(args.steps, 8)))
+        fp64 = time_path(torch, s, "double", args.steps, args.warmup, ewald=True)
+        print("# ewald fp64: %.2f ms/step, %s" % (fp64["ms_per_step"], fp64["stats"]), file=sys.stderr, flush=True)
+    err = (None, None, None) if fp64 is None else errors(mixed, fp64)
     cpu = None
     if not args.no_cpu_baseline:
-        from tholedipoleplugin_amd import _lib
-        h = _lib.HipContextHandle(s.force, s.box, precision="double", solver="pcg")
-        mu = h.dipoles(s.positions, s.box, _lib.DIPOLES_INDUCED)
-        iters = h.solver_stats()["iterations"] + 1
-        h.close()
-        cpu = cpu_baseline(s, mu, iters, args.cpu_seconds * 3.0)
+        cpu = cpu_baseline_ewald(s, alpha, (nx, ny, nz))
+        ref = {"forces": cpu.pop("forces"), "energy": cpu.pop("energy"), "mu": cpu.pop("induced")}
+        e = errors(mixed, ref)
+        cpu["mixed_force_rel_rms_err_vs_oracle"], cpu["mixed_energy_rel_err_vs_oracle"] = e[0], e[1]
+        cpu["mixed_mu_max_err_vs_oracle_debye"] = e[2]
+        if fp64 is not None:
+            e = errors(fp64, ref)
+            cpu["fp64_force_rel_rms_err_vs_oracle"], cpu["fp64_energy_rel_err_vs_oracle"] = e[0], e[1]
+            cpu["fp64_mu_max_err_vs_oracle_debye"] = e[2]
         print("# cpu baseline: %s" % cpu, file=sys.stderr, flush=True)
-    value = 1000.0 / mixed["ms_per_step"]
     out = {
         "metric": "force+induction evals/s (36k sites) & cluster solves/s, mixed vs FP64 @<=1e-5 err",
-        "value": value,
+        "value": 1000.0 / mixed["ms_per_step"],
         "unit": "evals/s",
         "n_gpus": 1,
         "steps": args.steps,
@@ -421,30 +521,27 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32+fp64",
         "data": "synthetic",
-        "config": {"workload": "water12288_36864_sites_box4.95nm_allpairs_min_image_mutual_1e-6D",
-                   "sites": n, "solver": "pcg", "precision": "mixed",
-                   "pcg_iterations": mixed["stats"]["iterations"],
-                   "tiles_per_work_item": mixed["stats"]["tiles_per_item"]},
+        "config": {"workload": "water12288_36864_sites_box4.95nm_pme_cutoff1.0nm_grid64_order5_mutual_1e-6D",
+                   "sites": n, "model": "Ewald option (real space within the cutoff + smooth PME)",
+                   "alpha_per_nm": alpha, "grid": [nx, ny, nz], "solver": "pcg", "precision": "mixed",
+                   "pcg_iterations": mixed["stats"]["iterations"]},
         "fp64_path": None if fp64 is None else {"evals_per_s": 1000.0 / fp64["ms_per_step"],
                                                 "ms_per_step": fp64["ms_per_step"],
                                                 "iterations": fp64["stats"]["iterations"]},
         "mixed_vs_fp64_speedup": None if fp64 is None else fp64["ms_per_step"] / mixed["ms_per_step"],
-        "mixed_force_rel_rms_err_vs_fp64": err,
-        "mixed_energy_rel_err_vs_fp64": e_err,
-        "mixed_mu_max_err_vs_fp64_debye": mu_err,
-        "roofline": {"bound": "mfma", "achieved": achieved_tflops, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / PEAK_FP32_TFLOPS, "traffic": MATVEC_HBM_BYTES,
-                     "kernel": "k_sym_matvec_mixed<true> (pair terms of one T*mu pass)",
-                     "ceiling": "compute roofline at the published dense FP32 rate; the pair terms run on the "
-                                "vector ALUs (no MFMA instruction)",
-                     "flops_per_pair": FLOPS_PER_PAIR_MATVEC, "pairs_per_launch": n * (n - 1) // 2,
-                     "avg_launch_ms": avg_ms, "launches": st["matvec_launches"], "full_passes": passes},
+        "mixed_force_rel_rms_err_vs_fp64": err[0],
+        "mixed_energy_rel_err_vs_fp64": err[1],
+        "mixed_mu_max_err_vs_fp64_debye": err[2],
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "clusters": clusters,
-        "dense_clusters": dense,
-        "saline96000": saline,
-        "md_warm_start": md,
     }
+    if not args.pmode_only:
+        out["allpairs_min_image"] = None if args.no_allpairs else allpairs_bench(
+            torch, s, args.steps, args.warmup, args.no_fp64, args.no_cpu_baseline, args.cpu_seconds)
+        out["clusters"] = None if args.no_clusters else cluster_bench(torch, max(1, min(args.steps, 10)), 3)
+        out["dense_clusters"] = None if args.no_dense else dense_bench(max(1, min(args.steps, 2)))
+        out["md_warm_start"] = None if args.no_md else md_bench(torch, s, args.steps)
+        out["saline96000"] = None if args.no_saline else saline_bench(torch, max(2, min(args.steps, 6)))
     print(json.dumps(out), flush=True)
     if args.copy_probe_mib > 0:
         x = torch.ones(args.copy_probe_mib * (1 << 20) // 4, dtype=torch.float32, device="cuda:0")

@@ -114,6 +114,9 @@ typedef struct tdp_solver_stats {
     int32_t converged;
     double matvec_ms;                 /* GPU time of the T*mu kernels since timing was enabled (HIP events) */
     int32_t matvec_launches;          /* T*mu launches timed since timing was enabled */
+    double matvec_ms_full;            /* the same over the full passes: launches taking at least half the
+                                         median (launches after a solve stopped return at once) */
+    int32_t matvec_full_launches;
 } tdp_solver_stats;
 
 /* Replaces CalcTholeDipoleForceKernel::initialize (TholeDipoleKernels.h:59,

@@ -64,7 +64,8 @@ class DenseStats(C.Structure):
 
 class SolverStats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("epsilon", C.c_double), ("converged", C.c_int32),
-                ("matvec_ms", C.c_double), ("matvec_launches", C.c_int32)]
+                ("matvec_ms", C.c_double), ("matvec_launches", C.c_int32), ("matvec_ms_full", C.c_double),
+                ("matvec_full_launches", C.c_int32)]
 
 
 class EwaldInfo(C.Structure):
@@ -283,7 +284,8 @@ class HipContextHandle:
         st = SolverStats()
         self._check(self.L.tdp_get_solver_stats(self.h, C.byref(st)))
         return dict(iterations=st.iterations, epsilon=st.epsilon, converged=st.converged,
-                    matvec_ms=st.matvec_ms, matvec_launches=st.matvec_launches)
+                    matvec_ms=st.matvec_ms, matvec_launches=st.matvec_launches, matvec_ms_full=st.matvec_ms_full,
+                    matvec_full_launches=st.matvec_full_launches)
 
     def set_timing(self, enable=True):
         self._check(self.L.tdp_set_timing(self.h, 1 if enable else 0))

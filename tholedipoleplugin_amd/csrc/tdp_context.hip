@@ -209,6 +209,7 @@ struct tdp_context {
     bool timing = false;
     double matvec_ms = 0.0;
     int matvec_launches = 0;
+    std::vector<float> matvec_each;   // every timed launch, ms
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
     std::string err;
@@ -804,6 +805,7 @@ void harvest_timing(tdp_context* c) {
         if (hipEventElapsedTime(&ms, pr.first, pr.second) == hipSuccess) {
             c->matvec_ms += ms;
             c->matvec_launches++;
+            c->matvec_each.push_back(ms);
         }
         c->ev_pool.push_back(pr.first);
         c->ev_pool.push_back(pr.second);
@@ -826,17 +828,18 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         const double box9[9] = {L.box[0], 0, 0, 0, L.box[1], 0, 0, 0, L.box[2]};
         const EwLaunch E = ew_launch(c, box9);
         launch_ew_recip(c->stream, E, c->posa, nullptr, v64, E.phi_v, scalars);
+        // timed: the stored-block T v kernel (or the matrix-free pass with its epilogue)
         if (timed) hipEventRecord(e0, c->stream);
-        if (c->ew_stored)
-            launch_ew_tmu_stored(c->stream, E, c->posa, c->d_inva, c->d_charge, v64,
-                                 c->precision == TDP_PRECISION_MIXED ? c->vec32 : nullptr, scalars, c->Emv, pq);
-        else
+        if (c->ew_stored) {
+            launch_ew_spmv(c->stream, E, v64, c->precision == TDP_PRECISION_MIXED ? c->vec32 : nullptr, scalars);
+            if (timed) hipEventRecord(e1, c->stream);
+            launch_ew_tmu_epilogue(c->stream, E, c->posa, c->d_inva, c->d_charge, v64, scalars, c->Emv, pq);
+        } else {
             launch_ew_pairs(c->stream, E, kEwTmu, false, c->posa, c->d_inva, c->d_charge, v64, nullptr, scalars, c->Emv,
                             pq, nullptr, nullptr, nullptr);
-        if (timed) {
-            hipEventRecord(e1, c->stream);
-            c->pending.push_back({e0, e1});
+            if (timed) hipEventRecord(e1, c->stream);
         }
+        if (timed) c->pending.push_back({e0, e1});
         return;
     }
     double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
@@ -2191,6 +2194,20 @@ int tdp_get_solver_stats(const tdp_context* cc, tdp_solver_stats* st) {
     st->converged = c->converged[0];
     st->matvec_ms = c->matvec_ms;
     st->matvec_launches = c->matvec_launches;
+    // launches enqueued after a solve stopped return at once: the full passes are those taking at
+    // least half the median launch
+    st->matvec_ms_full = 0.0;
+    st->matvec_full_launches = 0;
+    if (!c->matvec_each.empty()) {
+        std::vector<float> t = c->matvec_each;
+        std::nth_element(t.begin(), t.begin() + t.size() / 2, t.end());
+        const float half = 0.5f * t[t.size() / 2];
+        for (float ms : c->matvec_each)
+            if (ms >= half) {
+                st->matvec_ms_full += ms;
+                st->matvec_full_launches++;
+            }
+    }
     return TDP_OK;
 }
 
@@ -2301,6 +2318,7 @@ int tdp_set_timing(tdp_context* c, int32_t enable) {
     c->timing = enable != 0;
     c->matvec_ms = 0.0;
     c->matvec_launches = 0;
+    c->matvec_each.clear();
     return TDP_OK;
 }
 

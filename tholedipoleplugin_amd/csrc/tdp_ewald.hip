@@ -1203,15 +1203,18 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
 #undef TDP_EW_EPI
 }
 
-void launch_ew_tmu_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
-                          const double* charge, const double4* vec, const float4* v32, const double* scalars,
-                          double4* out4, const PcgQ& pq) {
+void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars) {
     const dim3 lists(E.nblk * kEwWaves);
     if (E.pairs32)
         hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), (size_t)E.lds_tiles * 8 * sizeof(float4), s, E, vec, v32,
                            scalars, E.rsw);
     else
         hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);
+}
+
+void launch_ew_tmu_epilogue(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
+                            const double* charge, const double4* vec, const double* scalars, double4* out4,
+                            const PcgQ& pq) {
     hipLaunchKernelGGL((k_ew_epilogue<kEwTmu, false, true>), dim3(E.nblk), dim3(64), 0, s, E, posa, inva, charge, vec,
                        (const double4*)nullptr, scalars, E.rsw, out4, pq, (double*)nullptr, (double*)nullptr,
                        (double*)nullptr);

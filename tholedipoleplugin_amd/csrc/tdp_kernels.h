@@ -239,10 +239,12 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
                      const double* scalars, double4* out4, const PcgQ& pq, double* epart, double* frc,
                      double* eout);
 // T v of the solver from the stored blocks (v in FP64, and its FP32 copy v32 for the FP32 lists),
-// then the epilogue of kEwTmu (reciprocal gather, scaled pairs, self field, PCG q).
-void launch_ew_tmu_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
-                          const double* charge, const double4* vec, const float4* v32, const double* scalars,
-                          double4* out4, const PcgQ& pq);
+// into the list partials E.rsw ...
+void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars);
+// ... then the epilogue of kEwTmu over them (reciprocal gather, scaled pairs, self field, PCG q).
+void launch_ew_tmu_epilogue(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
+                            const double* charge, const double4* vec, const double* scalars, double4* out4,
+                            const PcgQ& pq);
 // Energy of the system: sum of eout over slots in a fixed order, plus the background term.
 void launch_ew_energy(hipStream_t s, int npad, const double* eout, double background, double* energy);
 // Every pair (i < j, caller indices) within the cutoff, as the real-space passes decide it:

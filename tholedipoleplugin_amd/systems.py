@@ -171,10 +171,17 @@ def config0(seed: int = 20261203) -> SyntheticSystem:
     return s
 
 
+def pme_alpha(force) -> float:
+    """The Ewald parameter OpenMM derives from the error tolerance: sqrt(-ln(2 tol)) / cutoff."""
+    return math.sqrt(-math.log(2.0 * force.getEwaldErrorTolerance())) / force.getCutoffDistance()
+
+
 def config1(seed: int = 20261204) -> SyntheticSystem:
     """12,288 waters (36,864 sites) in a 4.95 nm box: seeded 12,288 of the 24^3 lattice points."""
     s = water_box(24, 12288, 4.95, seed, method=TholeDipoleForce.PME, cutoff=1.0, name="water12288")
-    s.force.setPMEParameters(0.0, 64, 64, 64)
+    # the Ewald parameter of the force's error tolerance with the stated 64^3 grid (alpha 0 would
+    # make the grid follow the tolerance too)
+    s.force.setPMEParameters(pme_alpha(s.force), 64, 64, 64)
     s.force.setMutualInducedTargetEpsilon(1e-6 * DEBYE_ENM)
     s.force.setMutualInducedMaxIterations(1000)
     return s
@@ -184,7 +191,7 @@ def config2(seed: int = 20261205) -> SyntheticSystem:
     """96,000 sites: 31,978 waters + 33 Na+/Cl- pairs in a 7.1 nm box, Thole a ~ U[0.3, 0.45]."""
     s = water_box(32, 31978, 7.1, seed, method=TholeDipoleForce.PME, cutoff=1.0, name="saline96000",
                   n_ion_pairs=33, thole_range=(0.3, 0.45))
-    s.force.setPMEParameters(0.0, 96, 96, 96)
+    s.force.setPMEParameters(pme_alpha(s.force), 96, 96, 96)
     s.force.setMutualInducedTargetEpsilon(1e-6 * DEBYE_ENM)
     s.force.setMutualInducedMaxIterations(1000)
     return s

@@ -43,9 +43,9 @@ DEBYE = 0.020819434  # e*nm
 # read + 166.8 MB written (the FP32 tile partial slab, 128 MB, and the FP32 work-item partials).
 # Counters cannot be read inside this process.
 MATVEC_HBM_BYTES = 265.6e6
-# HBM bytes per launch of k_ew_spmv<true> on this box from the same counters and scaling (None
-# until measured).
-SPMV_HBM_BYTES = None
+# HBM bytes per launch of k_ew_spmv<true> on this box (64^3 grid) from the same counters and scaling:
+# 997.2 MB read (the stored lists with their padding to the wave width, 997.6 MB) + 9.3 MB written.
+SPMV_HBM_BYTES = 1006.5e6
 
 
 def parse():

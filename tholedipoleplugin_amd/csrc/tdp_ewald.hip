@@ -586,6 +586,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     __shared__ double4 s_p[kEwWaves][kTile];
     __shared__ double4 s_v[kEwWaves][kTile];   // charge in w
     __shared__ double s_ia[kEwWaves][kTile];
+    __shared__ int s_j[MODE == kEwE0 ? kEwWaves : 1][kTile];   // slot index of a group slot (list build)
     const int i = I * kTile + lane;
     const bool ivalid = lane < E.tcnt[I];
     const double4 pi = posa[i];
@@ -637,6 +638,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         s_p[w][lane] = pl;
         s_v[w][lane] = vl;
         s_ia[w][lane] = ial;
+        if (MODE == kEwE0) s_j[w][lane] = Jl * kTile + w + kEwWaves * m8;
         // the slice is this wave's own: its LDS operations complete in order
         __builtin_amdgcn_wave_barrier();
         // scaled pairs of these tile pairs are masked (the epilogue evaluates them); the E0 pass also
@@ -669,40 +671,51 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             }
         }
         if (!ivalid) mask = maskT = ~0ull;
-        unsigned long long near_steps = 0ull;
+        // Phase A (one step per visited slot, all lanes): the FP64 pair criterion, giving each lane
+        // the bits of the pairs it evaluates.  E0: mA = the field's pairs, mB = the stored blocks;
+        // mixed T v / force: mA = pairs beyond r64 (FP32), mB = closer pairs (FP64); FP64: mB.
+        unsigned long long mA = 0ull, mB = 0ull;
+        {
+            unsigned long long v = visit;
+            while (v) {
+                const int q = __builtin_ctzll(v);
+                v &= v - 1ull;
+                const double4 pj = s_p[w][q];
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const double r2x = ew_r2(dx, dy, dz);
+                const unsigned long long bit = 1ull << q;
+                const bool in = r2x < E.rc2;
+                if (MODE == kEwE0) {
+                    if (in && !(mask & bit)) mA |= bit;
+                    if (build && in && !(maskT & bit)) mB |= bit;
+                } else if (in && !(mask & bit)) {
+                    if (MIXED && r2x >= E.r64sq) mA |= bit;
+                    else mB |= bit;
+                }
+            }
+        }
+        // Phase B: each lane runs through its own pairs (ascending q, the same order for every
+        // pass), so that the radial functions are evaluated for pairs inside the cutoff only.
         if (MIXED) {
             float fx = 0.f, fy = 0.f, fz = 0.f, fe = 0.f;
             const float alpha32 = (float)E.alpha, c1f = (float)E.c1, c2f = (float)E.c2, c3f = (float)E.c3;
-            unsigned long long v = visit;
+            unsigned long long v = mA;
             while (v) {
-                const int m = __builtin_ctzll(v);
+                const int q = __builtin_ctzll(v);
                 v &= v - 1ull;
-                const int k = m;   // bit of the group's masks
-                const double pjx = s_p[w][m].x, pjy = s_p[w][m].y, pjz = s_p[w][m].z;
-                const double dx = ew_image_fast(__dsub_rn(pi.x, pjx), E.L[0], iL0);
-                const double dy = ew_image_fast(__dsub_rn(pi.y, pjy), E.L[1], iL1);
-                const double dz = ew_image_fast(__dsub_rn(pi.z, pjz), E.L[2], iL2);
-                const double r2x = ew_r2(dx, dy, dz);
-                const bool inside = !((mask >> k) & 1ull) && r2x < E.rc2;
-                const bool near64 = inside && r2x < E.r64sq;
-                if (__builtin_amdgcn_ballot_w64(near64)) near_steps |= 1ull << m;
-                const bool far32 = inside && !near64;
-                if (!__builtin_amdgcn_ballot_w64(far32)) continue;
+                const double4 pj = s_p[w][q];
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const double4 vj = s_v[w][q];
                 const float gx = (float)dx, gy = (float)dy, gz = (float)dz;
-                const float r2f = far32 ? gx * gx + gy * gy + gz * gz : 1.f;
                 EwR<float> R;
-                ew_radial32(alpha32, c1f, c2f, c3f, r2f, (float)pi.w, (float)s_p[w][m].w, (float)iai,
-                            (float)s_ia[w][m], MODE == kEwForce && MUTUAL, R);
-                float tx = 0.f, ty = 0.f, tz = 0.f, te = 0.f;
+                ew_radial32(alpha32, c1f, c2f, c3f, gx * gx + gy * gy + gz * gz, (float)pi.w, (float)pj.w, (float)iai,
+                            (float)s_ia[w][q], MODE == kEwForce && MUTUAL, R);
                 ew_pair_terms<float, MODE, MUTUAL>(R, gx, gy, gz, (float)qi, (float)vi.x, (float)vi.y, (float)vi.z,
-                                                   (float)s_v[w][m].w, (float)s_v[w][m].x,
-                                                   (float)s_v[w][m].y, (float)s_v[w][m].z, tx, ty, tz, te);
-                if (far32) {
-                    fx += tx;
-                    fy += ty;
-                    fz += tz;
-                    fe += te;
-                }
+                                                   (float)vj.w, (float)vj.x, (float)vj.y, (float)vj.z, fx, fy, fz, fe);
             }
             ax += (double)fx;
             ay += (double)fy;
@@ -710,49 +723,42 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             ae += (double)fe;
         }
         double sx = 0.0, sy = 0.0, sz = 0.0, se = 0.0;
-        unsigned long long v = MIXED ? near_steps : visit;
+        unsigned long long v = MODE == kEwE0 ? (mA | mB) : mB;
         while (v) {
-            const int m = __builtin_ctzll(v);
+            const int q = __builtin_ctzll(v);
             v &= v - 1ull;
-            const int k = m;   // bit of the group's masks
-            const double pjx = s_p[w][m].x, pjy = s_p[w][m].y, pjz = s_p[w][m].z;
-            const double dx = ew_image_fast(__dsub_rn(pi.x, pjx), E.L[0], iL0);
-            const double dy = ew_image_fast(__dsub_rn(pi.y, pjy), E.L[1], iL1);
-            const double dz = ew_image_fast(__dsub_rn(pi.z, pjz), E.L[2], iL2);
+            const double4 pj = s_p[w][q];
+            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
             const double r2x = ew_r2(dx, dy, dz);
-            bool sel = !((mask >> k) & 1ull) && r2x < E.rc2;
-            if (MIXED) sel = sel && r2x < E.r64sq;
-            const bool store = build && !((maskT >> k) & 1ull) && r2x < E.rc2;
-            if (!__builtin_amdgcn_ballot_w64(sel || store)) continue;
+            const double4 vj = s_v[w][q];
             EwR<double> R;
-            ew_radial(E, (sel || store) ? r2x : 1.0, pi.w, s_p[w][m].w, iai, s_ia[w][m], true,
-                      MODE == kEwForce && MUTUAL, R);
+            ew_radial(E, r2x, pi.w, pj.w, iai, s_ia[w][q], true, MODE == kEwForce && MUTUAL, R);
             double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
-            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, s_v[w][m].w,
-                                                s_v[w][m].x, s_v[w][m].y, s_v[w][m].z, tx, ty, tz,
+            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, vj.w, vj.x, vj.y, vj.z, tx, ty, tz,
                                                 te);
-            if (sel) {
+            if (MODE != kEwE0 || ((mA >> q) & 1ull)) {
                 sx += tx;
                 sy += ty;
                 sz += tz;
                 se += te;
             }
-            if (store) {   // the block T_ij = C2 d d^T - C1 of the T v passes (iScale 1, damped)
+            if (MODE == kEwE0 && ((mB >> q) & 1ull)) {   // the block T_ij = C2 d d^T - C1 of the T v passes
                 const double C1 = R.B1 + (R.l3 - 1.0) * R.b1;
                 const double C2 = R.B2 + (R.l5 - 1.0) * R.b2;
                 const double g = sqrt(fabs(C2));
                 if (E.pairs32 && r2x >= E.r64sq) {
                     if (n32 < E.cap32) {
                         const size_t at = ((size_t)list * E.cap32 + n32) * kTile + lane;
-                        E.idx16[at] = (unsigned short)((8 * t0 + m) | (C2 < 0.0 ? 0x8000 : 0));
+                        E.idx16[at] = (unsigned short)((8 * t0 + q) | (C2 < 0.0 ? 0x8000 : 0));
                         E.t32[at] = make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz));
                     }
                     n32++;
                 } else {
                     if (n64 < E.cap64) {
                         const size_t at = ((size_t)list * E.cap64 + n64) * kTile + lane;
-                        E.idx64[at] = (__builtin_amdgcn_readlane(Jl, m) * kTile + w + kEwWaves * (m & 7)) |
-                                      (C2 < 0.0 ? (int)0x80000000u : 0);
+                        E.idx64[at] = s_j[w][q] | (C2 < 0.0 ? (int)0x80000000u : 0);
                         E.t64[at] = make_double4(-C1, g * dx, g * dy, g * dz);
                     }
                     n64++;

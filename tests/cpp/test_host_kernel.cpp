@@ -6,10 +6,10 @@
 #include <iostream>
 #include <vector>
 
-#include "../../oracle/tdp_oracle.h"
-#include "../../tholedipoleplugin_amd/host/TholeDipoleForce.h"
+#include "cases.hpp"
 
 using namespace TholeDipolePlugin;
+using namespace tdp_test;
 using OpenMM::Vec3;
 
 #define ASSERT(cond)                                                                       \
@@ -19,70 +19,6 @@ using OpenMM::Vec3;
             return 1;                                                                      \
         }                                                                                  \
     } while (0)
-
-// Two ammonia molecules (TestReferenceTholeDipoleForce.cpp:62-199), NoCutoff.
-static void ammonia(TholeDipoleForce& f, std::vector<Vec3>& pos) {
-    f.setNonbondedMethod(TholeDipoleForce::NoCutoff);
-    f.setPolarizationType(TholeDipoleForce::Mutual);
-    f.setMutualInducedTargetEpsilon(1e-12);
-    f.setMutualInducedMaxIterations(1000);
-    std::vector<double> dN = {8.3832254e-3, 0.0, 3.4232474e-3}, dH = {-1.7388763e-3, 0.0, -4.6837475e-3};
-    f.addParticle(-5.796e-1, dN, 1.073e-3, 3.9e-1, 2, 1, 2, 3);
-    f.addParticle(1.932e-1, dH, 4.96e-4, 3.9e-1, 2, 0, 2, 3);
-    f.addParticle(1.932e-1, dH, 4.96e-4, 3.9e-1, 2, 0, 1, 3);
-    f.addParticle(1.932e-1, dH, 4.96e-4, 3.9e-1, 2, 0, 1, 2);
-    f.addParticle(-5.796e-1, dN, 1.073e-3, 3.9e-1, 2, 5, 6, 7);
-    f.addParticle(1.932e-1, dH, 4.96e-4, 3.9e-1, 2, 4, 6, 7);
-    f.addParticle(1.932e-1, dH, 4.96e-4, 3.9e-1, 2, 4, 5, 7);
-    f.addParticle(1.932e-1, dH, 4.96e-4, 3.9e-1, 2, 4, 5, 6);
-    for (int m = 0; m < 2; m++) {
-        int n0 = 4 * m;
-        f.setCovalentMap(n0, TholeDipoleForce::Covalent12, {n0 + 1, n0 + 2, n0 + 3});
-        for (int h = 1; h <= 3; h++) {
-            f.setCovalentMap(n0 + h, TholeDipoleForce::Covalent12, {n0});
-            std::vector<int> others;
-            for (int k = 1; k <= 3; k++)
-                if (k != h) others.push_back(n0 + k);
-            f.setCovalentMap(n0 + h, TholeDipoleForce::Covalent13, others);
-        }
-    }
-    pos = {Vec3(1.5927280e-01, 1.7000000e-06, 1.6491000e-03), Vec3(2.0805540e-01, -8.1258800e-02, 3.7282500e-02),
-           Vec3(2.0843610e-01, 8.0953200e-02, 3.7462200e-02), Vec3(1.7280780e-01, 2.0730000e-04, -9.8741700e-02),
-           Vec3(-1.6743680e-01, 1.5900000e-05, -6.6149000e-03), Vec3(-2.0428260e-01, 8.1071500e-02, 4.1343900e-02),
-           Vec3(-6.7308300e-02, 1.2800000e-05, 1.0623300e-02), Vec3(-2.0426290e-01, -8.1231400e-02, 4.1033500e-02)};
-}
-
-struct OracleInput {
-    std::vector<double> q, d, a, t;
-    std::vector<int> ax, z, x, y, off, atoms;
-    tdpo_system s;
-};
-
-static void pack(const TholeDipoleForce& f, OracleInput& o) {
-    int n = f.getNumParticles();
-    o.off.assign(4 * n + 1, 0);
-    for (int i = 0; i < n; i++) {
-        double q, a, t;
-        int at, z, x, y;
-        std::vector<double> d;
-        f.getParticleParameters(i, q, d, a, t, at, z, x, y);
-        o.q.push_back(q); o.a.push_back(a); o.t.push_back(t);
-        o.d.insert(o.d.end(), d.begin(), d.end());
-        o.ax.push_back(at); o.z.push_back(z); o.x.push_back(x); o.y.push_back(y);
-        std::vector<std::vector<int> > lists;
-        f.getCovalentMaps(i, lists);
-        for (int k = 0; k < 4; k++) {
-            o.off[4 * i + k] = (int) o.atoms.size();
-            o.atoms.insert(o.atoms.end(), lists[k].begin(), lists[k].end());
-        }
-    }
-    o.off[4 * n] = (int) o.atoms.size();
-    o.atoms.push_back(0);
-    o.s = tdpo_system{n, o.q.data(), o.d.data(), o.a.data(), o.t.data(), o.ax.data(), o.z.data(), o.x.data(),
-                      o.y.data(), o.off.data(), o.atoms.data(), f.getNonbondedMethod(), {0, 0, 0},
-                      f.getPolarizationType(), f.getMutualInducedMaxIterations(), f.getMutualInducedTargetEpsilon(),
-                      nullptr, 0};
-}
 
 int main() {
     try {

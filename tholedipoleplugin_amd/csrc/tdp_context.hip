@@ -198,6 +198,7 @@ struct tdp_context {
     int ew_lds_tiles = 0;   // neighbour tiles the stored T v pass stages in LDS (0: gathers from HBM)
     float4* d_t32 = nullptr;
     double4* d_t64 = nullptr;
+    unsigned* d_cand = nullptr;   // the E0 pass's pair candidates
     int64_t ew_entries[2] = {0, 0}, ew_slots[2] = {0, 0};   // stored entries and padded slots of the last build
     void* h_need = nullptr;   // pinned copy of d_need
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
@@ -626,29 +627,34 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
     };
     release(c->d_idx16); release(c->d_t32);
     release(c->d_idx64); release(c->d_t64);
+    release(c->d_cand);
     c->d_idx16 = nullptr;
     c->d_idx64 = nullptr;
     c->d_t32 = nullptr;
     c->d_t64 = nullptr;
+    c->d_cand = nullptr;
     c->ew_stored = false;
-    cap32 = (cap32 + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU;
+    // capacities in whole candidate blocks of 8 (a multiple of kEwSpmvU)
+    cap32 = (cap32 + 7) / 8 * 8;
+    cap64 = (cap64 + 7) / 8 * 8;
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     const size_t rows = (size_t)c->nblk * kEwWaves * 64;   // 64 slots per tile
     const size_t n32 = mixed ? rows * cap32 : 1, n64 = rows * cap64;
-    void* p[4] = {};
-    const size_t bytes[4] = {n32 * sizeof(unsigned short), n32 * sizeof(float4), n64 * sizeof(int),
-                             n64 * sizeof(double4)};
-    for (int k = 0; k < 4; k++)
+    void* p[5] = {};
+    const size_t bytes[5] = {n32 * sizeof(unsigned short), n32 * sizeof(float4), n64 * sizeof(int),
+                             n64 * sizeof(double4), (mixed ? n32 : n64) * sizeof(unsigned)};
+    for (int k = 0; k < 5; k++)
         if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
             for (int q = 0; q < k; q++) hipFree(p[q]);
             (void)hipGetLastError();
             return TDP_OK;   // matrix-free fallback
         }
-    for (int k = 0; k < 4; k++) c->allocations.push_back(p[k]);
+    for (int k = 0; k < 5; k++) c->allocations.push_back(p[k]);
     c->d_idx16 = (unsigned short*)p[0];
     c->d_t32 = (float4*)p[1];
     c->d_idx64 = (int*)p[2];
     c->d_t64 = (double4*)p[3];
+    c->d_cand = (unsigned*)p[4];
     c->ew_cap32 = mixed ? cap32 : 0;
     c->ew_cap64 = cap64;
     c->ew_stored = true;
@@ -704,6 +710,7 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.t32 = c->d_t32;
     E.idx64 = c->ew_stored ? c->d_idx64 : nullptr;
     E.t64 = c->d_t64;
+    E.cand = c->d_cand;
     E.w32 = c->d_w32;
     E.w64 = c->d_w64;
     E.need = c->d_need;
@@ -972,8 +979,11 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         for (int attempt = 0;; attempt++) {
             const EwLaunch Eb = ew_launch(c, box);
             if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 7 * sizeof(int), s));
-            launch_ew_pairs(s, Eb, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv,
-                            PcgQ(), c->d_epart, nullptr, nullptr);
+            if (c->ew_stored)
+                launch_ew_e0_stored(s, Eb, c->posa, c->d_inva, c->d_charge, c->Emv, c->d_epart);
+            else
+                launch_ew_pairs(s, Eb, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv,
+                                PcgQ(), c->d_epart, nullptr, nullptr);
             if (!c->ew_stored) break;
             // the lists must have held every pair: else grow them and build again
             int* hn = static_cast<int*>(c->h_need);
@@ -1479,7 +1489,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             (rc = dalloc(c, &c->d_qfix, ng)) || (rc = dalloc(c, &c->d_cgrid, ng)) || (rc = dalloc(c, &c->d_phiq, ng)) ||
             (rc = dalloc(c, &c->d_phiv, ng)) || (rc = dalloc(c, &c->d_G64, ng)) || (rc = dalloc(c, &c->d_G32, ng)) ||
             (rc = dalloc(c, &c->d_sc_off, (size_t)npad + 1)) || (rc = dalloc(c, &c->d_epart, npad)) ||
-            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, (size_t)npad * kEwParts)) ||
+            (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, (size_t)npad * std::max(kEwParts, kEwE0Parts))) ||
             (rc = dalloc(c, &c->d_rsw, (size_t)npad * kEwWaves)) ||
             (rc = dalloc(c, &c->d_w32, (size_t)nb * kEwWaves)) || (rc = dalloc(c, &c->d_w64, (size_t)nb * kEwWaves)) ||
             (rc = dalloc(c, &c->d_need, 7)))

@@ -823,6 +823,230 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     rs[(size_t)h * E.npad + i] = make_double4(ax, ay, az, ae);
 }
 
+// The E0 pass when the blocks of T are stored, in two phases per wave (list (I, w)).
+// Phase A visits, as k_ew_pairs does, the wave's slots of the row's neighbour tiles that lie near
+// row I's box and records for each lane the pairs inside the cutoff: candidate 8 t + m (slot
+// w + kEwWaves m of neighbour tile t) with flags (in the field sum, stored, FP64 block), in the
+// lane's own 32-byte blocks of the candidate buffer (8 candidates per block, so that a lane's
+// stores fill whole sectors).  Phase B runs through each lane's candidates densely, every lane busy:
+// the FP64 radial functions are evaluated for pairs inside the cutoff only (a step of k_ew_pairs
+// keeps the lanes of its pairs beyond the cutoff idle), and the block of candidate n goes to row n of
+// the dense list (FP32 in mixed precision, FP64 otherwise), so that the 64 lanes store one row
+// together; a row holds a zero block where candidate n is not a block of that list or lies past
+// the lane's end.  In mixed precision the FP64 blocks of the closest pairs are appended to the
+// lane's FP64 list.  The permanent field of the row is summed as k_ew_pairs sums it (FP64, the
+// same pairs in the same order within a lane).
+constexpr unsigned kCandSel = 1u << 16, kCandStore = 1u << 17, kCandNear = 1u << 18;
+
+template <bool MIXED>
+__global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, const double4* __restrict__ posa,
+                                                                 const double* __restrict__ inva,
+                                                                 const double* __restrict__ charge,
+                                                                 double4* __restrict__ rs) {
+    // workgroup h of row I: waves w = kEwE0Waves h + wl (kEwWaves / kEwE0Waves workgroups per row, so
+    // that several fit on a CU at this register count)
+    const int I = blockIdx.x / kEwE0Parts, h = blockIdx.x % kEwE0Parts;
+    const int lane = threadIdx.x & (kTile - 1), wl = threadIdx.x / kTile, w = kEwE0Waves * h + wl;
+    __shared__ double4 s_p[kEwE0Waves][kTile];
+    __shared__ double red[kEwE0Waves * 3 * kTile];
+    const int i = I * kTile + lane;
+    const bool ivalid = lane < E.tcnt[I];
+    const double4 pi = posa[i];
+    const double iai = inva[i];
+    const float4 cI = E.tb_c[I], hI = E.tb_h[I];
+    const float bxf = (float)E.L[0], byf = (float)E.L[1], bzf = (float)E.L[2];
+    const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
+    const int cnt = E.nbr_cnt[I];
+    const int* nb = E.nbr + (size_t)I * E.nblk;
+    const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
+    const int list = I * kEwWaves + w;
+    const int capd = MIXED ? E.cap32 : E.cap64;   // the dense list's (and the candidates') capacity, a multiple of 8
+    unsigned* cb = E.cand + (size_t)list * capd * kTile + lane * 8;   // candidate n at cb[(n / 8) 512 + n % 8]
+    int nc = 0;
+    // ---- phase A
+    for (int t0 = 0; t0 < cnt; t0 += 8) {
+        const int g8 = lane >> 3, m8 = lane & 7;
+        int Jl = 0;
+        double4 pl = make_double4(0.0, 0.0, 0.0, 0.0);
+        bool near_box = false;
+        if (t0 + g8 < cnt) {
+            Jl = nb[t0 + g8];
+            const int ks = w + kEwWaves * m8;
+            if (ks < E.tcnt[Jl]) {
+                pl = posa[Jl * kTile + ks];
+                float dx = (float)pl.x - cI.x, dy = (float)pl.y - cI.y, dz = (float)pl.z - cI.z;
+                dx -= bxf * rintf(dx / bxf);
+                dy -= byf * rintf(dy / byf);
+                dz -= bzf * rintf(dz / bzf);
+                dx = fmaxf(0.f, fabsf(dx) - hI.x);
+                dy = fmaxf(0.f, fabsf(dy) - hI.y);
+                dz = fmaxf(0.f, fabsf(dz) - hI.z);
+                near_box = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
+            }
+        }
+        const unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
+        if (!visit) continue;
+        s_p[wl][lane] = pl;
+        __builtin_amdgcn_wave_barrier();   // the slice is this wave's own: its LDS operations complete in order
+        // masks of the group (bit 8 g + m): pairs scaled in the field (m0, mhalf: the epilogue
+        // evaluates them) and pairs without a block of T (i0)
+        unsigned long long mask = 0ull, maskT = 0ull;
+        for (int g = 0; g < 8; g++) {
+            if (!((visit >> (8 * g)) & 0xffull)) continue;
+            const int J = __builtin_amdgcn_readlane(Jl, 8 * g);
+            for (int e = eS; e < eE; e++) {
+                if (E.exc_J[e] != J) continue;
+                const unsigned long long* mm = E.exc_mask + (size_t)e * 3 * kTile;
+                mask |= ew_slots_of_wave(mm[lane] | mm[kTile + lane], w) << (8 * g);
+                maskT |= ew_slots_of_wave(mm[2 * kTile + lane], w) << (8 * g);
+                break;
+            }
+            if (J == I && ((lane - w) & (kEwWaves - 1)) == 0 && lane >= w) {
+                const unsigned long long self = 1ull << (8 * g + (lane - w) / kEwWaves);
+                mask |= self;
+                maskT |= self;
+            }
+        }
+        if (!ivalid) mask = maskT = ~0ull;
+        unsigned long long v = visit;
+        while (v) {
+            const int q = __builtin_ctzll(v);
+            v &= v - 1ull;
+            const double4 pj = s_p[wl][q];
+            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+            const double r2x = ew_r2(dx, dy, dz);
+            const bool in = r2x < E.rc2;
+            const bool sel = in && !((mask >> q) & 1ull);
+            const bool store = in && !((maskT >> q) & 1ull);
+            if (sel || store) {
+                if (nc < capd)
+                    cb[(nc >> 3) * 512 + (nc & 7)] = (unsigned)(8 * t0 + q) | (sel ? kCandSel : 0u) |
+                                                     (store ? kCandStore : 0u) |
+                                                     ((MIXED && r2x >= E.r64sq) ? 0u : kCandNear);
+                nc++;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next group is staged
+    }
+    // ---- phase B
+    int mc = nc;
+    for (int off = 32; off > 0; off >>= 1) mc = max(mc, __shfl_xor(mc, off, 64));
+    const int wd = min((mc + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU, capd);   // dense list width
+    const int nl = min(nc, capd);
+    auto cand = [&](int n) -> unsigned {
+        const int k = min(n, capd - 1);
+        return cb[(k >> 3) * 512 + (k & 7)];
+    };
+    auto slot = [&](unsigned c) -> int {
+        const int loc = (int)(c & 0xffffu);
+        return nb[loc >> 3] * kTile + w + kEwWaves * (loc & 7);
+    };
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int n64 = 0, s32 = 0, s64 = 0;
+    // software pipeline: the slot data of candidate n+1 and candidate n+2 itself are in flight while
+    // pair n is evaluated
+    unsigned c0 = cand(0), c1 = cand(1);
+    int j0 = 0 < nl ? slot(c0) : i;
+    double4 p0 = posa[j0];
+    double q0 = charge[j0], a0 = inva[j0];
+    for (int n = 0; n < wd; n++) {
+        const unsigned c2 = cand(n + 2);
+        const int j1 = n + 1 < nl ? slot(c1) : i;
+        const double4 p1 = posa[j1];
+        const double q1 = charge[j1], a1 = inva[j1];
+        const bool act = n < nl;
+        const unsigned c = act ? c0 : 0u;
+        double C1 = 0.0, C2 = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+        if (act) {
+            dx = ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
+            dy = ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
+            dz = ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
+            EwR<double> R;
+            ew_radial(E, ew_r2(dx, dy, dz), pi.w, p0.w, iai, a0, true, false, R);
+            C1 = R.B1 + (R.l3 - 1.0) * R.b1;
+            C2 = R.B2 + (R.l5 - 1.0) * R.b2;
+            if (c & kCandSel) {
+                const double f = q0 * C1;
+                sx += f * dx;
+                sy += f * dy;
+                sz += f * dz;
+            }
+        }
+        // the block T_ij = C2 d d^T - C1 as (-C1, sqrt|C2| d), the sign of C2 with the index
+        const bool st = (c & kCandStore) != 0u;
+        const double g = st ? sqrt(fabs(C2)) : 0.0;
+        const bool dense = st && (MIXED ? !(c & kCandNear) : true);
+        if (MIXED) {
+            const size_t at = ((size_t)list * E.cap32 + n) * kTile + lane;
+            E.idx16[at] = dense ? (unsigned short)((c & 0xffffu) | (C2 < 0.0 ? 0x8000u : 0u)) : (unsigned short)0;
+            E.t32[at] = dense ? make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz))
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+            s32 += dense ? 1 : 0;
+            if (st && (c & kCandNear)) {
+                if (n64 < E.cap64) {
+                    const size_t at64 = ((size_t)list * E.cap64 + n64) * kTile + lane;
+                    E.idx64[at64] = j0 | (C2 < 0.0 ? (int)0x80000000u : 0);
+                    E.t64[at64] = make_double4(-C1, g * dx, g * dy, g * dz);
+                }
+                n64++;
+            }
+        } else {
+            const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
+            E.idx64[at] = dense ? (j0 | (C2 < 0.0 ? (int)0x80000000u : 0)) : i;
+            E.t64[at] = dense ? make_double4(-C1, g * dx, g * dy, g * dz) : make_double4(0.0, 0.0, 0.0, 0.0);
+            s64 += dense ? 1 : 0;
+        }
+        c0 = c1;
+        c1 = c2;
+        j0 = j1;
+        p0 = p1;
+        q0 = q1;
+        a0 = a1;
+    }
+    // list widths and counts; the FP64 list of mixed precision padded with zero blocks
+    int m64 = n64;
+    for (int off = 32; off > 0; off >>= 1) m64 = max(m64, __shfl_xor(m64, off, 64));
+    int s32w = s32, s64w = s64 + (MIXED ? n64 : 0);
+    for (int off = 32; off > 0; off >>= 1) {
+        s32w += __shfl_xor(s32w, off, 64);
+        s64w += __shfl_xor(s64w, off, 64);
+    }
+    int w64c = 0;
+    if (MIXED) {
+        w64c = min(m64, E.cap64);
+        for (int n = n64; n < w64c; n++) {
+            const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
+            E.idx64[at] = i;
+            E.t64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
+        }
+    }
+    if (lane == 0) {
+        E.w32[list] = MIXED ? wd : 0;
+        E.w64[list] = MIXED ? w64c : wd;
+        atomicMax(&E.need[0], MIXED ? mc : 0);
+        atomicMax(&E.need[1], MIXED ? m64 : mc);
+        atomicAdd(&E.need[2], s32w);
+        atomicAdd(&E.need[3], s64w);
+        atomicAdd(&E.need[4], MIXED ? wd : 0);
+        atomicAdd(&E.need[5], MIXED ? w64c : wd);
+        if (w == 0) atomicMax(&E.need[6], cnt);
+    }
+    // the part's field over its waves, in wave order (the epilogue adds the parts in order)
+    red[(wl * 3 + 0) * kTile + lane] = sx;
+    red[(wl * 3 + 1) * kTile + lane] = sy;
+    red[(wl * 3 + 2) * kTile + lane] = sz;
+    __syncthreads();
+    if (wl != 0) return;
+    for (int u = 1; u < kEwE0Waves; u++) {
+        sx += red[(u * 3 + 0) * kTile + lane];
+        sy += red[(u * 3 + 1) * kTile + lane];
+        sz += red[(u * 3 + 2) * kTile + lane];
+    }
+    rs[(size_t)h * E.npad + i] = make_double4(sx, sy, sz, 0.0);
+}
+
 typedef float ew_f4 __attribute__((ext_vector_type(4)));
 
 // FP32 lists of one wave: the sum over its entries of T_ij v_j, v_j from the wave's LDS copy
@@ -976,7 +1200,7 @@ __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* _
         ay = r0.y;
         az = r0.z;
         ae = r0.w;
-        for (int h = 1; h < ew_parts(MODE); h++) {
+        for (int h = 1; h < (MODE == kEwE0 ? E.e0_parts : ew_parts(MODE)); h++) {
             const double4 r = rs[(size_t)h * E.npad + i];
             ax += r.x;
             ay += r.y;
@@ -1207,6 +1431,20 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
     }
 #undef TDP_EW_PAIRS
 #undef TDP_EW_EPI
+}
+
+void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
+                         const double* charge, double4* out4, double* epart) {
+    EwLaunch Ep = E;
+    Ep.e0_parts = kEwE0Parts;
+    const dim3 grid(E.nblk * kEwE0Parts), block(kEwE0Waves * 64);
+    if (E.pairs32)
+        hipLaunchKernelGGL((k_ew_e0_build<true>), grid, block, 0, s, Ep, posa, inva, charge, E.rs);
+    else
+        hipLaunchKernelGGL((k_ew_e0_build<false>), grid, block, 0, s, Ep, posa, inva, charge, E.rs);
+    hipLaunchKernelGGL((k_ew_epilogue<kEwE0, false>), dim3(E.nblk), dim3(64), 0, s, Ep, posa, inva, charge,
+                       (const double4*)nullptr, (const double4*)nullptr, (const double*)nullptr, E.rs, out4, PcgQ(),
+                       epart, (double*)nullptr, (double*)nullptr);
 }
 
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars) {

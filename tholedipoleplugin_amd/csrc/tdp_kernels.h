@@ -157,6 +157,8 @@ constexpr int kEwWaves = 8;   // waves per workgroup of the real-space passes
 constexpr int kEwParts = 2;   // workgroups per tile row of the T v and force passes (part h takes the
                               // neighbour tiles h, h + kEwParts, ...); the E0 pass has one
 constexpr int ew_parts(int mode) { return mode == 0 ? 1 : kEwParts; }   // mode: EwMode (kEwE0 is 0)
+constexpr int kEwE0Waves = 4;   // waves per workgroup of the list-building E0 pass
+constexpr int kEwE0Parts = kEwWaves / kEwE0Waves;   // its workgroups per tile row
 constexpr int kEwSpmvU = 4;   // entries per step of the stored T v pass (FP32 lists)
 constexpr int kEwMaxLocalTiles = 4096;   // neighbour tiles an FP32 list index can address
 // FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
@@ -220,6 +222,9 @@ struct EwLaunch {
     int* need;              // [7] largest width (FP32, FP64), entries (FP32, FP64), summed widths
                             // (FP32, FP64), largest neighbour-tile count
     int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
+    int e0_parts = 1;       // row parts whose sums the E0 epilogue adds (kEwE0Parts after the list build)
+    unsigned* cand;         // [nblk kEwWaves 64 capd] the E0 pass's pair candidates (capd: cap32 in mixed
+                            // precision, cap64 otherwise), per lane in blocks of 8
     bool mixed;             // FP32 grids and FFTs (FP64 spreading and sums)
 };
 
@@ -238,6 +243,10 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
                      const double* inva, const double* charge, const double4* vec, const double4* E0,
                      const double* scalars, double4* out4, const PcgQ& pq, double* epart, double* frc,
                      double* eout);
+// The E0 pass that also builds the stored blocks (when the lists are stored): the real-space field
+// and the lists, then the epilogue of kEwE0 (reciprocal gather, scaled pairs) into out4 and epart.
+void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
+                         const double* charge, double4* out4, double* epart);
 // T v of the solver from the stored blocks (v in FP64, and its FP32 copy v32 for the FP32 lists),
 // into the list partials E.rsw ...
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars);

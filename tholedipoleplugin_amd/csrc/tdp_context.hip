@@ -198,7 +198,8 @@ struct tdp_context {
     int ew_lds_tiles = 0;   // neighbour tiles the stored T v pass stages in LDS (0: gathers from HBM)
     float4* d_t32 = nullptr;
     double4* d_t64 = nullptr;
-    unsigned* d_cand = nullptr;   // the E0 pass's pair candidates
+    unsigned *d_cand32 = nullptr, *d_cand64 = nullptr;   // the E0 pass's pair candidates
+    int2* d_ccnt = nullptr;
     int64_t ew_entries[2] = {0, 0}, ew_slots[2] = {0, 0};   // stored entries and padded slots of the last build
     void* h_need = nullptr;   // pinned copy of d_need
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
@@ -627,12 +628,13 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
     };
     release(c->d_idx16); release(c->d_t32);
     release(c->d_idx64); release(c->d_t64);
-    release(c->d_cand);
+    release(c->d_cand32); release(c->d_cand64); release(c->d_ccnt);
     c->d_idx16 = nullptr;
     c->d_idx64 = nullptr;
     c->d_t32 = nullptr;
     c->d_t64 = nullptr;
-    c->d_cand = nullptr;
+    c->d_cand32 = c->d_cand64 = nullptr;
+    c->d_ccnt = nullptr;
     c->ew_stored = false;
     // capacities in whole candidate blocks of 8 (a multiple of kEwSpmvU)
     cap32 = (cap32 + 7) / 8 * 8;
@@ -640,21 +642,23 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     const size_t rows = (size_t)c->nblk * kEwWaves * 64;   // 64 slots per tile
     const size_t n32 = mixed ? rows * cap32 : 1, n64 = rows * cap64;
-    void* p[5] = {};
-    const size_t bytes[5] = {n32 * sizeof(unsigned short), n32 * sizeof(float4), n64 * sizeof(int),
-                             n64 * sizeof(double4), (mixed ? n32 : n64) * sizeof(unsigned)};
-    for (int k = 0; k < 5; k++)
+    void* p[7] = {};
+    const size_t bytes[7] = {n32 * sizeof(unsigned short), n32 * sizeof(float4), n64 * sizeof(int),
+                             n64 * sizeof(double4), n32 * sizeof(unsigned), n64 * sizeof(unsigned), rows * sizeof(int2)};
+    for (int k = 0; k < 7; k++)
         if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
             for (int q = 0; q < k; q++) hipFree(p[q]);
             (void)hipGetLastError();
             return TDP_OK;   // matrix-free fallback
         }
-    for (int k = 0; k < 5; k++) c->allocations.push_back(p[k]);
+    for (int k = 0; k < 7; k++) c->allocations.push_back(p[k]);
     c->d_idx16 = (unsigned short*)p[0];
     c->d_t32 = (float4*)p[1];
     c->d_idx64 = (int*)p[2];
     c->d_t64 = (double4*)p[3];
-    c->d_cand = (unsigned*)p[4];
+    c->d_cand32 = (unsigned*)p[4];
+    c->d_cand64 = (unsigned*)p[5];
+    c->d_ccnt = (int2*)p[6];
     c->ew_cap32 = mixed ? cap32 : 0;
     c->ew_cap64 = cap64;
     c->ew_stored = true;
@@ -710,7 +714,9 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.t32 = c->d_t32;
     E.idx64 = c->ew_stored ? c->d_idx64 : nullptr;
     E.t64 = c->d_t64;
-    E.cand = c->d_cand;
+    E.cand32 = c->d_cand32;
+    E.cand64 = c->d_cand64;
+    E.ccnt = c->d_ccnt;
     E.w32 = c->d_w32;
     E.w64 = c->d_w64;
     E.need = c->d_need;
@@ -1184,8 +1190,12 @@ void forces_and_energy(tdp_context* c, const double* box, bool want_forces = tru
     if (c->ewald) {   // the dipoles' potential grid, then forces and energy per slot
         const EwLaunch E = ew_launch(c, box);
         launch_ew_recip(s, E, c->posa, nullptr, c->mu, E.phi_v, nullptr);
-        launch_ew_pairs(s, E, kEwForce, c->polarization == TDP_MUTUAL, c->posa, c->d_inva, c->d_charge, c->mu, c->E0,
-                        nullptr, nullptr, PcgQ(), c->d_epart, c->d_frc, c->d_eout);
+        if (c->ew_stored)   // the E0 pass of this evaluation left the pair candidates
+            launch_ew_force_stored(s, E, c->polarization == TDP_MUTUAL, c->posa, c->d_inva, c->d_charge, c->mu, c->E0,
+                                   c->d_epart, c->d_frc, c->d_eout);
+        else
+            launch_ew_pairs(s, E, kEwForce, c->polarization == TDP_MUTUAL, c->posa, c->d_inva, c->d_charge, c->mu,
+                            c->E0, nullptr, nullptr, PcgQ(), c->d_epart, c->d_frc, c->d_eout);
         if (want_energy) {
             const double V = box[0] * box[4] * box[8];
             const double bg = -M_PI * c->ew_qsum * c->ew_qsum / (2.0 * V * c->ew_alpha * c->ew_alpha);

@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <type_traits>
 
 #include "tdp_device.hpp"
 #include "tdp_kernels.h"
@@ -825,18 +826,25 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
 
 // The E0 pass when the blocks of T are stored, in two phases per wave (list (I, w)).
 // Phase A visits, as k_ew_pairs does, the wave's slots of the row's neighbour tiles that lie near
-// row I's box and records for each lane the pairs inside the cutoff: candidate 8 t + m (slot
-// w + kEwWaves m of neighbour tile t) with flags (in the field sum, stored, FP64 block), in the
-// lane's own 32-byte blocks of the candidate buffer (8 candidates per block, so that a lane's
-// stores fill whole sectors).  Phase B runs through each lane's candidates densely, every lane busy:
-// the FP64 radial functions are evaluated for pairs inside the cutoff only (a step of k_ew_pairs
-// keeps the lanes of its pairs beyond the cutoff idle), and the block of candidate n goes to row n of
-// the dense list (FP32 in mixed precision, FP64 otherwise), so that the 64 lanes store one row
-// together; a row holds a zero block where candidate n is not a block of that list or lies past
-// the lane's end.  In mixed precision the FP64 blocks of the closest pairs are appended to the
-// lane's FP64 list.  The permanent field of the row is summed as k_ew_pairs sums it (FP64, the
-// same pairs in the same order within a lane).
-constexpr unsigned kCandSel = 1u << 16, kCandStore = 1u << 17, kCandNear = 1u << 18;
+// row I's box and records for each lane the pairs inside the cutoff as candidates 8 t + m (slot
+// w + kEwWaves m of neighbour tile t) with two flags (in the field sum, stored as a block of T),
+// in two streams: the pairs whose blocks are FP32 (beyond r64 in mixed precision) and the others
+// (FP64).  A lane's candidates go to its own 32-byte blocks of the stream (8 per block, so that the
+// lane's stores fill whole sectors).  Phase B runs through each lane's streams densely, every lane
+// busy: the FP64 radial functions are evaluated for pairs inside the cutoff only (a step of
+// k_ew_pairs keeps the lanes of its pairs beyond the cutoff idle), and the block of candidate n of
+// a stream goes to row n of that stream's list, so that the 64 lanes store one row together (zero
+// blocks where candidate n is not stored or lies past the lane's end).  The candidates and their
+// counts stay for the force pass (k_ew_force_cand).
+constexpr unsigned kCandSel = 1u << 16, kCandStore = 1u << 17;
+
+__device__ __forceinline__ unsigned* ew_cand_base(unsigned* c, int list, int cap, int lane) {
+    return c + (size_t)list * cap * kTile + lane * 8;   // candidate n at [(n / 8) 512 + n % 8]
+}
+__device__ __forceinline__ int ew_cand_slot(const int* nb, unsigned c, int w) {
+    const int loc = (int)(c & 0xffffu);
+    return nb[loc >> 3] * kTile + w + kEwWaves * (loc & 7);
+}
 
 template <bool MIXED>
 __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, const double4* __restrict__ posa,
@@ -860,9 +868,9 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const int* nb = E.nbr + (size_t)I * E.nblk;
     const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
     const int list = I * kEwWaves + w;
-    const int capd = MIXED ? E.cap32 : E.cap64;   // the dense list's (and the candidates') capacity, a multiple of 8
-    unsigned* cb = E.cand + (size_t)list * capd * kTile + lane * 8;   // candidate n at cb[(n / 8) 512 + n % 8]
-    int nc = 0;
+    unsigned* cf = ew_cand_base(E.cand32, list, E.cap32, lane);   // FP32 stream (mixed precision)
+    unsigned* cn = ew_cand_base(E.cand64, list, E.cap64, lane);   // FP64 stream
+    int nf = 0, nn = 0;
     // ---- phase A
     for (int t0 = 0; t0 < cnt; t0 += 8) {
         const int g8 = lane >> 3, m8 = lane & 7;
@@ -921,116 +929,102 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             const bool sel = in && !((mask >> q) & 1ull);
             const bool store = in && !((maskT >> q) & 1ull);
             if (sel || store) {
-                if (nc < capd)
-                    cb[(nc >> 3) * 512 + (nc & 7)] = (unsigned)(8 * t0 + q) | (sel ? kCandSel : 0u) |
-                                                     (store ? kCandStore : 0u) |
-                                                     ((MIXED && r2x >= E.r64sq) ? 0u : kCandNear);
-                nc++;
+                const unsigned c = (unsigned)(8 * t0 + q) | (sel ? kCandSel : 0u) | (store ? kCandStore : 0u);
+                if (MIXED && r2x >= E.r64sq) {
+                    if (nf < E.cap32) cf[(nf >> 3) * 512 + (nf & 7)] = c;
+                    nf++;
+                } else {
+                    if (nn < E.cap64) cn[(nn >> 3) * 512 + (nn & 7)] = c;
+                    nn++;
+                }
             }
         }
         __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next group is staged
     }
+    E.ccnt[(size_t)list * kTile + lane] = make_int2(min(nf, E.cap32), min(nn, E.cap64));
     // ---- phase B
-    int mc = nc;
-    for (int off = 32; off > 0; off >>= 1) mc = max(mc, __shfl_xor(mc, off, 64));
-    const int wd = min((mc + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU, capd);   // dense list width
-    const int nl = min(nc, capd);
-    auto cand = [&](int n) -> unsigned {
-        const int k = min(n, capd - 1);
-        return cb[(k >> 3) * 512 + (k & 7)];
-    };
-    auto slot = [&](unsigned c) -> int {
-        const int loc = (int)(c & 0xffffu);
-        return nb[loc >> 3] * kTile + w + kEwWaves * (loc & 7);
-    };
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    int n64 = 0, s32 = 0, s64 = 0;
-    // software pipeline: the slot data of candidate n+1 and candidate n+2 itself are in flight while
-    // pair n is evaluated
-    unsigned c0 = cand(0), c1 = cand(1);
-    int j0 = 0 < nl ? slot(c0) : i;
-    double4 p0 = posa[j0];
-    double q0 = charge[j0], a0 = inva[j0];
-    for (int n = 0; n < wd; n++) {
-        const unsigned c2 = cand(n + 2);
-        const int j1 = n + 1 < nl ? slot(c1) : i;
-        const double4 p1 = posa[j1];
-        const double q1 = charge[j1], a1 = inva[j1];
-        const bool act = n < nl;
-        const unsigned c = act ? c0 : 0u;
-        double C1 = 0.0, C2 = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
-        if (act) {
-            dx = ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
-            dy = ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
-            dz = ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
-            EwR<double> R;
-            ew_radial(E, ew_r2(dx, dy, dz), pi.w, p0.w, iai, a0, true, false, R);
-            C1 = R.B1 + (R.l3 - 1.0) * R.b1;
-            C2 = R.B2 + (R.l5 - 1.0) * R.b2;
-            if (c & kCandSel) {
-                const double f = q0 * C1;
-                sx += f * dx;
-                sy += f * dy;
-                sz += f * dz;
-            }
-        }
-        // the block T_ij = C2 d d^T - C1 as (-C1, sqrt|C2| d), the sign of C2 with the index
-        const bool st = (c & kCandStore) != 0u;
-        const double g = st ? sqrt(fabs(C2)) : 0.0;
-        const bool dense = st && (MIXED ? !(c & kCandNear) : true);
-        if (MIXED) {
-            const size_t at = ((size_t)list * E.cap32 + n) * kTile + lane;
-            E.idx16[at] = dense ? (unsigned short)((c & 0xffffu) | (C2 < 0.0 ? 0x8000u : 0u)) : (unsigned short)0;
-            E.t32[at] = dense ? make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz))
-                              : make_float4(0.f, 0.f, 0.f, 0.f);
-            s32 += dense ? 1 : 0;
-            if (st && (c & kCandNear)) {
-                if (n64 < E.cap64) {
-                    const size_t at64 = ((size_t)list * E.cap64 + n64) * kTile + lane;
-                    E.idx64[at64] = j0 | (C2 < 0.0 ? (int)0x80000000u : 0);
-                    E.t64[at64] = make_double4(-C1, g * dx, g * dy, g * dz);
-                }
-                n64++;
-            }
-        } else {
-            const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
-            E.idx64[at] = dense ? (j0 | (C2 < 0.0 ? (int)0x80000000u : 0)) : i;
-            E.t64[at] = dense ? make_double4(-C1, g * dx, g * dy, g * dz) : make_double4(0.0, 0.0, 0.0, 0.0);
-            s64 += dense ? 1 : 0;
-        }
-        c0 = c1;
-        c1 = c2;
-        j0 = j1;
-        p0 = p1;
-        q0 = q1;
-        a0 = a1;
+    int mf = nf, mn = nn;
+    for (int off = 32; off > 0; off >>= 1) {
+        mf = max(mf, __shfl_xor(mf, off, 64));
+        mn = max(mn, __shfl_xor(mn, off, 64));
     }
-    // list widths and counts; the FP64 list of mixed precision padded with zero blocks
-    int m64 = n64;
-    for (int off = 32; off > 0; off >>= 1) m64 = max(m64, __shfl_xor(m64, off, 64));
-    int s32w = s32, s64w = s64 + (MIXED ? n64 : 0);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    int s32 = 0, s64 = 0;
+    // one stream: FP32 blocks (FP32 = true) or FP64 blocks; software pipeline: the slot data of
+    // candidate n+1 and candidate n+2 itself are in flight while pair n is evaluated
+    auto stream = [&](auto fp32_tag, const unsigned* cb, int nl, int wd, int cap) {
+        constexpr bool FP32 = decltype(fp32_tag)::value;
+        auto cand = [&](int n) -> unsigned {
+            const int k = min(n, cap - 1);
+            return cb[(k >> 3) * 512 + (k & 7)];
+        };
+        unsigned c0 = cand(0), c1 = cand(1);
+        int j0 = 0 < nl ? ew_cand_slot(nb, c0, w) : i;
+        double4 p0 = posa[j0];
+        double q0 = charge[j0], a0 = inva[j0];
+        for (int n = 0; n < wd; n++) {
+            const unsigned c2 = cand(n + 2);
+            const int j1 = n + 1 < nl ? ew_cand_slot(nb, c1, w) : i;
+            const double4 p1 = posa[j1];
+            const double q1 = charge[j1], a1 = inva[j1];
+            const unsigned c = n < nl ? c0 : 0u;
+            double C1 = 0.0, C2 = 0.0, dx = 0.0, dy = 0.0, dz = 0.0;
+            if (n < nl) {
+                dx = ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
+                dy = ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
+                dz = ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
+                EwR<double> R;
+                ew_radial(E, ew_r2(dx, dy, dz), pi.w, p0.w, iai, a0, true, false, R);
+                C1 = R.B1 + (R.l3 - 1.0) * R.b1;
+                C2 = R.B2 + (R.l5 - 1.0) * R.b2;
+                if (c & kCandSel) {
+                    const double f = q0 * C1;
+                    sx += f * dx;
+                    sy += f * dy;
+                    sz += f * dz;
+                }
+            }
+            // the block T_ij = C2 d d^T - C1 as (-C1, sqrt|C2| d), the sign of C2 with the index
+            const bool st = (c & kCandStore) != 0u;
+            const double g = st ? sqrt(fabs(C2)) : 0.0;
+            if (FP32) {
+                const size_t at = ((size_t)list * E.cap32 + n) * kTile + lane;
+                E.idx16[at] = st ? (unsigned short)((c & 0xffffu) | (C2 < 0.0 ? 0x8000u : 0u)) : (unsigned short)0;
+                E.t32[at] = st ? make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz))
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+                s32 += st ? 1 : 0;
+            } else {
+                const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
+                E.idx64[at] = st ? (j0 | (C2 < 0.0 ? (int)0x80000000u : 0)) : i;
+                E.t64[at] = st ? make_double4(-C1, g * dx, g * dy, g * dz) : make_double4(0.0, 0.0, 0.0, 0.0);
+                s64 += st ? 1 : 0;
+            }
+            c0 = c1;
+            c1 = c2;
+            j0 = j1;
+            p0 = p1;
+            q0 = q1;
+            a0 = a1;
+        }
+    };
+    const int wd32 = MIXED ? min((mf + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU, E.cap32) : 0;
+    const int wd64 = min(mn, E.cap64);
+    if (MIXED) stream(std::true_type(), cf, min(nf, E.cap32), wd32, E.cap32);
+    stream(std::false_type(), cn, min(nn, E.cap64), wd64, E.cap64);
+    int s32w = s32, s64w = s64;
     for (int off = 32; off > 0; off >>= 1) {
         s32w += __shfl_xor(s32w, off, 64);
         s64w += __shfl_xor(s64w, off, 64);
     }
-    int w64c = 0;
-    if (MIXED) {
-        w64c = min(m64, E.cap64);
-        for (int n = n64; n < w64c; n++) {
-            const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
-            E.idx64[at] = i;
-            E.t64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-    }
     if (lane == 0) {
-        E.w32[list] = MIXED ? wd : 0;
-        E.w64[list] = MIXED ? w64c : wd;
-        atomicMax(&E.need[0], MIXED ? mc : 0);
-        atomicMax(&E.need[1], MIXED ? m64 : mc);
+        E.w32[list] = wd32;
+        E.w64[list] = wd64;
+        atomicMax(&E.need[0], MIXED ? mf : 0);
+        atomicMax(&E.need[1], mn);
         atomicAdd(&E.need[2], s32w);
         atomicAdd(&E.need[3], s64w);
-        atomicAdd(&E.need[4], MIXED ? wd : 0);
-        atomicAdd(&E.need[5], MIXED ? w64c : wd);
+        atomicAdd(&E.need[4], wd32);
+        atomicAdd(&E.need[5], wd64);
         if (w == 0) atomicMax(&E.need[6], cnt);
     }
     // the part's field over its waves, in wave order (the epilogue adds the parts in order)
@@ -1045,6 +1039,129 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         sz += red[(u * 3 + 2) * kTile + lane];
     }
     rs[(size_t)h * E.npad + i] = make_double4(sx, sy, sz, 0.0);
+}
+
+// The force pass from the E0 pass's candidates (same positions): each lane runs through its unscaled
+// pairs (in the field sum and stored, i.e. no scale factor: the epilogue evaluates the scaled ones),
+// FP32 stream first (FP32 terms in mixed precision, summed in FP32 over at most 64 candidates, then
+// added to the FP64 sums), then the FP64 stream.  Workgroups as in k_ew_e0_build.
+template <bool MUTUAL, bool MIXED>
+__global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, const double4* __restrict__ posa,
+                                                                   const double* __restrict__ inva,
+                                                                   const double* __restrict__ charge,
+                                                                   const double4* __restrict__ vec,
+                                                                   double4* __restrict__ rs) {
+    const int I = blockIdx.x / kEwE0Parts, h = blockIdx.x % kEwE0Parts;
+    const int lane = threadIdx.x & (kTile - 1), wl = threadIdx.x / kTile, w = kEwE0Waves * h + wl;
+    __shared__ double red[kEwE0Waves * 4 * kTile];
+    const int i = I * kTile + lane;
+    const double4 pi = posa[i];
+    const double iai = inva[i];
+    const double qi = charge[i];
+    const double4 vi = vec[i];
+    const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
+    const int* nb = E.nbr + (size_t)I * E.nblk;
+    const int list = I * kEwWaves + w;
+    const int2 nc = E.ccnt[(size_t)list * kTile + lane];
+    int mf = nc.x, mn = nc.y;
+    for (int off = 32; off > 0; off >>= 1) {
+        mf = max(mf, __shfl_xor(mf, off, 64));
+        mn = max(mn, __shfl_xor(mn, off, 64));
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0;
+    constexpr unsigned kUnscaled = kCandSel | kCandStore;
+    if (MIXED) {
+        const unsigned* cb = ew_cand_base(E.cand32, list, E.cap32, lane);
+        const float alpha32 = (float)E.alpha, c1f = (float)E.c1, c2f = (float)E.c2, c3f = (float)E.c3;
+        auto cand = [&](int n) -> unsigned {
+            const int k = min(n, E.cap32 - 1);
+            return cb[(k >> 3) * 512 + (k & 7)];
+        };
+        unsigned c0 = cand(0), c1 = cand(1);
+        int j0 = 0 < nc.x ? ew_cand_slot(nb, c0, w) : i;
+        double4 p0 = posa[j0], v0 = vec[j0];
+        double q0 = charge[j0], a0 = inva[j0];
+        float fx = 0.f, fy = 0.f, fz = 0.f, fe = 0.f;
+        for (int n = 0; n < mf; n++) {
+            const unsigned c2 = cand(n + 2);
+            const int j1 = n + 1 < nc.x ? ew_cand_slot(nb, c1, w) : i;
+            const double4 p1 = posa[j1], v1 = vec[j1];
+            const double q1 = charge[j1], a1 = inva[j1];
+            if (n < nc.x && (c0 & kUnscaled) == kUnscaled) {
+                const float gx = (float)ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
+                const float gy = (float)ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
+                const float gz = (float)ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
+                EwR<float> R;
+                ew_radial32(alpha32, c1f, c2f, c3f, gx * gx + gy * gy + gz * gz, (float)pi.w, (float)p0.w, (float)iai,
+                            (float)a0, MUTUAL, R);
+                ew_pair_terms<float, kEwForce, MUTUAL>(R, gx, gy, gz, (float)qi, (float)vi.x, (float)vi.y, (float)vi.z,
+                                                       (float)q0, (float)v0.x, (float)v0.y, (float)v0.z, fx, fy, fz,
+                                                       fe);
+            }
+            if ((n & 63) == 63) {   // FP32 sums over at most 64 candidates
+                ax += (double)fx;
+                ay += (double)fy;
+                az += (double)fz;
+                ae += (double)fe;
+                fx = fy = fz = fe = 0.f;
+            }
+            c0 = c1;
+            c1 = c2;
+            p0 = p1;
+            v0 = v1;
+            q0 = q1;
+            a0 = a1;
+        }
+        ax += (double)fx;
+        ay += (double)fy;
+        az += (double)fz;
+        ae += (double)fe;
+    }
+    {
+        const unsigned* cb = ew_cand_base(E.cand64, list, E.cap64, lane);
+        auto cand = [&](int n) -> unsigned {
+            const int k = min(n, E.cap64 - 1);
+            return cb[(k >> 3) * 512 + (k & 7)];
+        };
+        unsigned c0 = cand(0), c1 = cand(1);
+        int j0 = 0 < nc.y ? ew_cand_slot(nb, c0, w) : i;
+        double4 p0 = posa[j0], v0 = vec[j0];
+        double q0 = charge[j0], a0 = inva[j0];
+        for (int n = 0; n < mn; n++) {
+            const unsigned c2 = cand(n + 2);
+            const int j1 = n + 1 < nc.y ? ew_cand_slot(nb, c1, w) : i;
+            const double4 p1 = posa[j1], v1 = vec[j1];
+            const double q1 = charge[j1], a1 = inva[j1];
+            if (n < nc.y && (c0 & kUnscaled) == kUnscaled) {
+                const double dx = ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
+                EwR<double> R;
+                ew_radial(E, ew_r2(dx, dy, dz), pi.w, p0.w, iai, a0, true, MUTUAL, R);
+                ew_pair_terms<double, kEwForce, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, q0, v0.x, v0.y, v0.z, ax, ay,
+                                                        az, ae);
+            }
+            c0 = c1;
+            c1 = c2;
+            p0 = p1;
+            v0 = v1;
+            q0 = q1;
+            a0 = a1;
+        }
+    }
+    red[(wl * 4 + 0) * kTile + lane] = ax;
+    red[(wl * 4 + 1) * kTile + lane] = ay;
+    red[(wl * 4 + 2) * kTile + lane] = az;
+    red[(wl * 4 + 3) * kTile + lane] = ae;
+    __syncthreads();
+    if (wl != 0) return;
+    for (int u = 1; u < kEwE0Waves; u++) {
+        ax += red[(u * 4 + 0) * kTile + lane];
+        ay += red[(u * 4 + 1) * kTile + lane];
+        az += red[(u * 4 + 2) * kTile + lane];
+        ae += red[(u * 4 + 3) * kTile + lane];
+    }
+    rs[(size_t)h * E.npad + i] = make_double4(ax, ay, az, ae);
 }
 
 typedef float ew_f4 __attribute__((ext_vector_type(4)));
@@ -1445,6 +1562,24 @@ void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, 
     hipLaunchKernelGGL((k_ew_epilogue<kEwE0, false>), dim3(E.nblk), dim3(64), 0, s, Ep, posa, inva, charge,
                        (const double4*)nullptr, (const double4*)nullptr, (const double*)nullptr, E.rs, out4, PcgQ(),
                        epart, (double*)nullptr, (double*)nullptr);
+}
+
+void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const double4* posa, const double* inva,
+                            const double* charge, const double4* vec, const double4* E0, double* epart, double* frc,
+                            double* eout) {
+    static_assert(kEwE0Parts == ew_parts(kEwForce), "the force epilogue adds the parts of the force pass");
+    const dim3 grid(E.nblk * kEwE0Parts), block(kEwE0Waves * 64), rows(E.nblk);
+    if (mutual) {
+        if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<true, true>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        else hipLaunchKernelGGL((k_ew_force_cand<true, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        hipLaunchKernelGGL((k_ew_epilogue<kEwForce, true>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
+                           (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
+    } else {
+        if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<false, true>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        else hipLaunchKernelGGL((k_ew_force_cand<false, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        hipLaunchKernelGGL((k_ew_epilogue<kEwForce, false>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
+                           (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
+    }
 }
 
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars) {

@@ -223,8 +223,9 @@ struct EwLaunch {
                             // (FP32, FP64), largest neighbour-tile count
     int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
     int e0_parts = 1;       // row parts whose sums the E0 epilogue adds (kEwE0Parts after the list build)
-    unsigned* cand;         // [nblk kEwWaves 64 capd] the E0 pass's pair candidates (capd: cap32 in mixed
-                            // precision, cap64 otherwise), per lane in blocks of 8
+    unsigned* cand32;       // [nblk kEwWaves 64 cap32] the E0 pass's pair candidates with FP32 blocks,
+    unsigned* cand64;       // [nblk kEwWaves 64 cap64] and with FP64 blocks, per lane in blocks of 8
+    int2* ccnt;             // [nblk kEwWaves 64] the lanes' candidate counts (FP32, FP64 stream)
     bool mixed;             // FP32 grids and FFTs (FP64 spreading and sums)
 };
 
@@ -247,6 +248,11 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
 // and the lists, then the epilogue of kEwE0 (reciprocal gather, scaled pairs) into out4 and epart.
 void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
                          const double* charge, double4* out4, double* epart);
+// The force pass from the E0 pass's candidates (lists stored), then the epilogue of kEwForce (which
+// reads the E0 epilogue's per-slot energy terms from epart).
+void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const double4* posa, const double* inva,
+                            const double* charge, const double4* vec, const double4* E0, double* epart, double* frc,
+                            double* eout);
 // T v of the solver from the stored blocks (v in FP64, and its FP32 copy v32 for the FP32 lists),
 // into the list partials E.rsw ...
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars);

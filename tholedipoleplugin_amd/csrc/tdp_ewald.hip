@@ -838,6 +838,12 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
 // counts stay for the force pass (k_ew_force_cand).
 constexpr unsigned kCandSel = 1u << 16, kCandStore = 1u << 17;
 
+// x, y, z wrapped into [0, L) in FP64, then rounded to FP32.
+__device__ __forceinline__ float4 ew_wrap32(const double4& p, const double* L, double iL0, double iL1, double iL2) {
+    return make_float4((float)(p.x - L[0] * floor(p.x * iL0)), (float)(p.y - L[1] * floor(p.y * iL1)),
+                       (float)(p.z - L[2] * floor(p.z * iL2)), 0.f);
+}
+
 __device__ __forceinline__ unsigned* ew_cand_base(unsigned* c, int list, int cap, int lane) {
     return c + (size_t)list * cap * kTile + lane * 8;   // candidate n at [(n / 8) 512 + n % 8]
 }
@@ -856,6 +862,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const int I = blockIdx.x / kEwE0Parts, h = blockIdx.x % kEwE0Parts;
     const int lane = threadIdx.x & (kTile - 1), wl = threadIdx.x / kTile, w = kEwE0Waves * h + wl;
     __shared__ double4 s_p[kEwE0Waves][kTile];
+    __shared__ float4 s_f[kEwE0Waves][kTile];   // the slots wrapped into the box, FP32
     __shared__ double red[kEwE0Waves * 3 * kTile];
     const int i = I * kTile + lane;
     const bool ivalid = lane < E.tcnt[I];
@@ -868,6 +875,18 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const int* nb = E.nbr + (size_t)I * E.nblk;
     const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
     const int list = I * kEwWaves + w;
+    // FP32 screen of the pair criterion on coordinates wrapped into the box.  Each separation
+    // component is off by at most 4 u (u = max L 2^-23: the wraps' and the box length's roundings),
+    // so |r2_FP32 - r2_FP64| < 2 sqrt(3) r 4 u + r^2 2^-21; within the band 4e-5 r_t^2 + 32 r_t u
+    // around a threshold r_t (r_c, r64), which holds that bound several times over, the FP64 rule
+    // decides
+    const float Lx = (float)E.L[0], Ly = (float)E.L[1], Lz = (float)E.L[2];
+    const float iLx = 1.f / Lx, iLy = 1.f / Ly, iLz = 1.f / Lz;
+    const float rc2f = (float)E.rc2, r64f = (float)E.r64sq;
+    const float upos = fmaxf(Lx, fmaxf(Ly, Lz)) * 0x1p-23f;
+    const float band_c2 = 4e-5f * rc2f + 32.f * sqrtf(rc2f) * upos;
+    const float band_64 = 4e-5f * r64f + 32.f * sqrtf(r64f) * upos;
+    const float4 pif = ew_wrap32(pi, E.L, iL0, iL1, iL2);
     unsigned* cf = ew_cand_base(E.cand32, list, E.cap32, lane);   // FP32 stream (mixed precision)
     unsigned* cn = ew_cand_base(E.cand64, list, E.cap64, lane);   // FP64 stream
     int nf = 0, nn = 0;
@@ -895,6 +914,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         const unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
         if (!visit) continue;
         s_p[wl][lane] = pl;
+        s_f[wl][lane] = ew_wrap32(pl, E.L, iL0, iL1, iL2);
         __builtin_amdgcn_wave_barrier();   // the slice is this wave's own: its LDS operations complete in order
         // masks of the group (bit 8 g + m): pairs scaled in the field (m0, mhalf: the epilogue
         // evaluates them) and pairs without a block of T (i0)
@@ -920,17 +940,28 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         while (v) {
             const int q = __builtin_ctzll(v);
             v &= v - 1ull;
-            const double4 pj = s_p[wl][q];
-            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
-            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
-            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
-            const double r2x = ew_r2(dx, dy, dz);
-            const bool in = r2x < E.rc2;
+            const float4 pjf = s_f[wl][q];
+            float fx = pif.x - pjf.x, fy = pif.y - pjf.y, fz = pif.z - pjf.z;
+            fx -= Lx * rintf(fx * iLx);
+            fy -= Ly * rintf(fy * iLy);
+            fz -= Lz * rintf(fz * iLz);
+            const float r2f = fx * fx + fy * fy + fz * fz;
+            bool in = r2f < rc2f;
+            bool far = r2f >= r64f;
+            if (fabsf(r2f - rc2f) <= band_c2 || (MIXED && fabsf(r2f - r64f) <= band_64)) {   // the FP64 rule (rare)
+                const double4 pj = s_p[wl][q];
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const double r2x = ew_r2(dx, dy, dz);
+                in = r2x < E.rc2;
+                far = r2x >= E.r64sq;
+            }
             const bool sel = in && !((mask >> q) & 1ull);
             const bool store = in && !((maskT >> q) & 1ull);
             if (sel || store) {
                 const unsigned c = (unsigned)(8 * t0 + q) | (sel ? kCandSel : 0u) | (store ? kCandStore : 0u);
-                if (MIXED && r2x >= E.r64sq) {
+                if (MIXED && far) {
                     if (nf < E.cap32) cf[(nf >> 3) * 512 + (nf & 7)] = c;
                     nf++;
                 } else {

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -119,6 +120,19 @@ struct tdp_context {
         hipGraphExec_t exec;
     };
     std::vector<IterGraph> graphs;
+    // Longer captured sequences (the field and the start of a solve, or the forces and energy),
+    // replayed while the tables (graph_gen), box, precision and key are unchanged.
+    struct SeqGraph {
+        int tag;
+        long long key;
+        const void* p0;
+        const void* p1;
+        unsigned gen;
+        int precision;
+        double box[3];
+        hipGraphExec_t exec;
+    };
+    std::vector<SeqGraph> seq_graphs;
     unsigned graph_gen = 0;
     bool use_graphs = true;
     std::vector<void*> allocations;
@@ -890,14 +904,15 @@ size_t stage_download_doubles(const tdp_context* c) { return (size_t)SC_COUNT * 
 size_t stage_upload_doubles(const tdp_context* c) { return (size_t)SC_COUNT * c->nseg + 2; }
 double* stage_upload(tdp_context* c) { return static_cast<double*>(c->h_stage) + stage_download_doubles(c); }
 
-// Copies the stop count and the solver scalars to the pinned staging buffer (one host round trip).
+// Copies the solver scalars to the pinned staging buffer (one copy, one host round trip); done is
+// the number of systems whose stop flag is set.
 int stage_solver_state(tdp_context* c, int& done) {
     double* h = static_cast<double*>(c->h_stage);
-    TDP_HIP(c, hipMemcpyAsync(h, c->d_done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     TDP_HIP(c, hipMemcpyAsync(h + 1, c->d_scalars, sizeof(double) * SC_COUNT * c->nseg, hipMemcpyDeviceToHost,
                               c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
-    done = *reinterpret_cast<const int*>(h);
+    done = 0;
+    for (int g = 0; g < c->nseg; g++) done += h[1 + (size_t)SC_COUNT * g + SC_DONE] != 0.0 ? 1 : 0;
     return TDP_OK;
 }
 
@@ -922,7 +937,7 @@ int read_solver_results(tdp_context* c, bool jacobi, bool staged = false) {
 
 // Starts a Mutual solve: per-system iteration limit and target, epsilon 1, and for the Jacobi
 // iteration the reference's loop condition (:829-832) as the initial stop flag.
-int start_solve(tdp_context* c, bool jacobi) {
+int start_solve_host(tdp_context* c, bool jacobi) {
     double* sc = stage_upload(c);
     const size_t nsc = (size_t)SC_COUNT * c->nseg;
     std::fill(sc, sc + nsc, 0.0);
@@ -938,12 +953,25 @@ int start_solve(tdp_context* c, bool jacobi) {
         }
     }
     if (!jacobi) done = 0;   // the CG start kernel sets every system's flag afresh
-    TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc, nsc * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int* counts = reinterpret_cast<int*>(sc + nsc);
     counts[0] = done;
     counts[1] = 0;
-    TDP_HIP(c, hipMemcpyAsync(c->d_done, counts, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
     return TDP_OK;
+}
+
+// The copies of the staged start values to the device (launches only: the values are read from
+// the pinned staging buffer when the copies run).
+int start_solve_copies(tdp_context* c) {
+    double* sc = stage_upload(c);
+    const size_t nsc = (size_t)SC_COUNT * c->nseg;
+    TDP_HIP(c, hipMemcpyAsync(c->d_scalars, sc, nsc * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    TDP_HIP(c, hipMemcpyAsync(c->d_done, sc + nsc, 2 * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    return TDP_OK;
+}
+
+int start_solve(tdp_context* c, bool jacobi) {
+    int rc = start_solve_host(c, jacobi);
+    return rc ? rc : start_solve_copies(c);
 }
 
 
@@ -951,9 +979,10 @@ int start_solve(tdp_context* c, bool jacobi) {
 // positions already in d_pos3 (caller's order): internal order, lab-frame dipoles, E0 and
 // mu = alpha E0.  host_pos3 (may be NULL) are the same positions on the host, used when the
 // internal order is recomputed.
-int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
+// The host part of prepare_field: the internal order, recomputed every kReorderInterval evaluations.
+int prepare_order(tdp_context* c, const double* box, const double* host_pos3) {
     hipStream_t s = c->stream;
-    const int n = c->n, npad = c->npad;
+    const int n = c->n;
     // (an allocation-free call keeps the current order: recomputing it needs host work and may grow
     // the exclusion tables)
     if (!c->noalloc && (!c->ordered || c->evals_since_order >= kReorderInterval)) {
@@ -969,12 +998,39 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         if (rc) return rc;
     }
     c->evals_since_order++;
-    PairLaunch L = pair_launch(c, box);
-    SegLaunch G = seg_launch(c);
+    return TDP_OK;
+}
+
+// The launches of prepare_field without the Ewald option: positions in the internal order, tile
+// boxes and pair classes, lab-frame dipoles, E0 and mu = alpha E0.
+void enqueue_field(tdp_context* c, const PairLaunch& L, const SegLaunch& G) {
+    hipStream_t s = c->stream;
+    const int npad = c->npad;
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
     launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
-    if (c->ewald) {
+    launch_tile_pairs(s, L, c->d_tp);
+    launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
+    double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
+    launch_sym_field(s, L, c->posa, c->qd, c->d_inva, pi, reinterpret_cast<double*>(pi + (size_t)c->nwork * 64), c->Emv);
+    launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
+}
+
+int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
+    hipStream_t s = c->stream;
+    const int npad = c->npad;
+    int rc0 = prepare_order(c, box, host_pos3);
+    if (rc0) return rc0;
+    PairLaunch L = pair_launch(c, box);
+    SegLaunch G = seg_launch(c);
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    if (!c->ewald) {
+        enqueue_field(c, L, G);
+        return TDP_OK;
+    }
+    launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    {
         // neighbour tiles, the charges' potential grid, E0 = real space + reciprocal + scaled pairs
         int rc = ew_update_box(c, box);
         if (rc) return rc;
@@ -1024,14 +1080,6 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
         return TDP_OK;
     }
-    launch_tile_pairs(s, L, c->d_tp);
-    launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
-    {
-        double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
-        launch_sym_field(s, L, c->posa, c->qd, c->d_inva, pi, reinterpret_cast<double*>(pi + (size_t)c->nwork * 64),
-                         c->Emv);
-    }
-    launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
     return TDP_OK;
 }
 
@@ -1039,27 +1087,68 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
 // Enqueues n solver iterations, as a replay of a captured graph of them when possible (the
 // iteration kernels then go to the device as one submission).  Events for per-launch timing are
 // not captured: with timing on, the launches are enqueued one by one.
-int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool jacobi, int n) {
+// Replays the captured launches of `enqueue` (which returns a status) for (tag, key, p0, p1) under
+// the current tables, box and precision, capturing them on first use.  Callers use it only where
+// every launch's arguments are fixed by those: no host decision inside depends on device results.
+template <class F>
+int replay_or_capture(tdp_context* c, int tag, long long key, const void* p0, const void* p1, const double* box3,
+                      F&& enqueue) {
+    hipStream_t s = c->stream;
+    hipGraphExec_t ex = nullptr;
+    for (size_t k = 0; k < c->seq_graphs.size();) {
+        auto& g = c->seq_graphs[k];
+        if (g.gen != c->graph_gen || g.box[0] != box3[0] || g.box[1] != box3[1] || g.box[2] != box3[2]) {
+            hipGraphExecDestroy(g.exec);
+            c->seq_graphs.erase(c->seq_graphs.begin() + k);
+            continue;
+        }
+        if (g.tag == tag && g.key == key && g.p0 == p0 && g.p1 == p1 && g.precision == c->precision) ex = g.exec;
+        k++;
+    }
+    if (!ex) {
+        hipGraph_t graph = nullptr;
+        TDP_HIP(c, hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue();
+        const hipError_t ee = hipStreamEndCapture(s, &graph);
+        if (rc) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        if (ee != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ee));
+        hipError_t e = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+        hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c, TDP_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        c->seq_graphs.push_back({tag, key, p0, p1, c->graph_gen, c->precision, {box3[0], box3[1], box3[2]}, ex});
+    }
+    TDP_HIP(c, hipGraphLaunch(ex, s));
+    return TDP_OK;
+}
+
+// Enqueues n solver iterations (launches only).
+void enqueue_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool jacobi, int n) {
     hipStream_t s = c->stream;
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
-    auto enqueue = [&]() {
-        for (int k = 0; k < n; k++) {
-            if (jacobi) {
-                matvec(c, L, c->mu, c->d_scalars);
-                launch_jacobi_update(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr, c->d_scalars,
-                                     c->d_red);
-            } else {
-                PcgQ pq;
-                pq.p = c->p;
-                pq.alpha = c->d_alpha;
-                pq.q = c->q;
-                pq.red = c->d_red + 2 * (size_t)c->nblk;
-                matvec(c, L, c->p, c->d_scalars, pq);
-                launch_pcg_iteration_tail(s, G, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr, c->q,
-                                          c->d_red, c->d_scalars);
-            }
+    for (int k = 0; k < n; k++) {
+        if (jacobi) {
+            matvec(c, L, c->mu, c->d_scalars);
+            launch_jacobi_update(s, G, c->Emv, c->E0, c->d_alpha, c->mu, mixed ? c->vec32 : nullptr, c->d_scalars,
+                                 c->d_red);
+        } else {
+            PcgQ pq;
+            pq.p = c->p;
+            pq.alpha = c->d_alpha;
+            pq.q = c->q;
+            pq.red = c->d_red + 2 * (size_t)c->nblk;
+            matvec(c, L, c->p, c->d_scalars, pq);
+            launch_pcg_iteration_tail(s, G, c->d_alpha, c->mu, c->r, c->z, c->p, mixed ? c->vec32 : nullptr, c->q,
+                                      c->d_red, c->d_scalars);
         }
-    };
+    }
+}
+
+int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool jacobi, int n) {
+    hipStream_t s = c->stream;
+    auto enqueue = [&]() { enqueue_iterations(c, L, G, jacobi, n); };
     if (!c->use_graphs || c->timing || c->noalloc) {
         enqueue();
         return TDP_OK;
@@ -1124,6 +1213,68 @@ int solve_mutual(tdp_context* c, const double* box) {
     return TDP_OK;
 }
 
+constexpr int kSeqFront = 1, kSeqBack = 2;
+
+// Whether an evaluation runs as captured sequences: the field, the start of the Mutual solve and its
+// first chunk of iterations as one replay (no host step between them), then the host check of the
+// stop count, then the forces and energy as one replay.  Not with the Ewald option (its list build
+// checks capacities on the host), event timing or the allocation-free entry point.
+bool sequence_graphs(const tdp_context* c) {
+    return c->use_graphs && !c->timing && !c->noalloc && !c->ewald && c->polarization == TDP_MUTUAL && c->n > 0;
+}
+
+// setup() and the Mutual solve as sequence_graphs describes.  back (may be empty) enqueues the rest
+// of the evaluation with its results guarded by the device stop count: it is launched right after
+// the first chunk, before the host check, and *back_ran tells whether that launch stands (every
+// system had stopped by then; else the caller enqueues the rest again after the solve).
+int solve_captured(tdp_context* c, const double* box, const double* host_pos3,
+                   const std::function<int()>& back = nullptr, bool* back_ran = nullptr) {
+    int rc = prepare_order(c, box, host_pos3);
+    if (rc) return rc;
+    hipStream_t s = c->stream;
+    const int npad = c->npad;
+    PairLaunch L = pair_launch(c, box);
+    SegLaunch G = seg_launch(c);
+    const bool mixed = c->precision == TDP_PRECISION_MIXED;
+    const bool jacobi = c->solver == TDP_SOLVER_JACOBI;
+    const int chunk = c->n > 20000 ? 2 : 4;
+    const int first_chunk = c->last_max_iter > 0 ? c->last_max_iter : (c->n > 20000 ? 4 : 16);
+    const bool warm = c->warm_start && c->have_prev;
+    if ((rc = start_solve_host(c, jacobi))) return rc;
+    const long long key = (long long)first_chunk | ((long long)jacobi << 32) | ((long long)warm << 33);
+    rc = replay_or_capture(c, kSeqFront, key, nullptr, nullptr, L.box, [&]() -> int {
+        enqueue_field(c, L, G);
+        if (warm)   // start from the previous solution instead of alpha E0
+            launch_gather_vec3(s, npad, c->d_perm, c->d_mu_prev, c->mu, mixed ? c->vec32 : nullptr);
+        int r = start_solve_copies(c);
+        if (r) return r;
+        if (!jacobi) {
+            // x = mu; r = E0 - x/alpha + T x; z = alpha r; p = z
+            matvec(c, L, c->mu, nullptr);
+            launch_pcg_init(s, G, c->Emv, c->E0, c->d_alpha, c->mu, c->r, c->p, mixed ? c->vec32 : nullptr, c->d_red,
+                            c->d_scalars);
+        }
+        enqueue_iterations(c, L, G, jacobi, first_chunk);
+        return TDP_OK;
+    });
+    if (rc) return rc;
+    if (back && (rc = back())) return rc;
+    int done = 0;
+    if ((rc = stage_solver_state(c, done))) return rc;
+    if (back_ran) *back_ran = back && done >= c->nseg;
+    while (done < c->nseg) {
+        if ((rc = run_iterations(c, L, G, jacobi, chunk))) return rc;
+        if ((rc = stage_solver_state(c, done))) return rc;
+    }
+    if ((rc = read_solver_results(c, jacobi, true))) return rc;
+    c->last_max_iter = *std::max_element(c->iterations.begin(), c->iterations.end());
+    if (c->warm_start) {
+        launch_scatter_vec3(s, npad, c->d_perm, c->mu, c->d_mu_prev);
+        c->have_prev = true;
+    }
+    return TDP_OK;
+}
+
 // setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764).
 int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) {
     if (c->n == 0) {
@@ -1132,6 +1283,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
         std::fill(c->epsilon.begin(), c->epsilon.end(), c->polarization == TDP_DIRECT ? 1.0e50 : 0.0);
         return TDP_OK;
     }
+    if (sequence_graphs(c)) return solve_captured(c, box, host_pos3);
     int rc = prepare_field(c, box, host_pos3);
     if (rc) return rc;
     hipStream_t s = c->stream;
@@ -1890,6 +2042,7 @@ void tdp_destroy(tdp_context* c) {
     }
     for (auto e : c->ev_pool) hipEventDestroy(e);
     for (auto& g : c->graphs) hipGraphExecDestroy(g.exec);
+    for (auto& g : c->seq_graphs) hipGraphExecDestroy(g.exec);
     for (void* p : c->allocations) hipFree(p);
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->h_need) hipHostFree(c->h_need);
@@ -1946,14 +2099,34 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
     TDP_HIP(c, hipStreamWaitEvent(c->stream, ev, 0));
     if (c->n > 0)
         TDP_HIP(c, hipMemcpyAsync(c->d_pos3, d_positions, 3 * c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
-    if ((rc = setup_and_solve(c, box, nullptr))) {
+    // forces and energy into the caller's buffers, guarded by the stop count on the captured path
+    const bool seq = sequence_graphs(c);
+    auto back = [&]() -> int {
+        forces_and_energy(c, box, d_forces != nullptr, d_energy != nullptr);
+        if (d_forces && c->n > 0)
+            launch_scatter_accumulate(c->stream, c->npad, c->d_perm, c->d_frc, d_forces, seq ? c->d_done : nullptr,
+                                      c->nseg);
+        if (d_energy) {
+            if (seq) launch_copy_guarded(c->stream, c->nseg, c->d_energy, d_energy, c->d_done, c->nseg);
+            else
+                TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToDevice,
+                                          c->stream));
+        }
+        return TDP_OK;
+    };
+    bool back_ran = false;
+    if (seq) {
+        const PairLaunch L = pair_launch(c, box);
+        auto back_graph = [&]() -> int { return replay_or_capture(c, kSeqBack, 0, d_forces, d_energy, L.box, back); };
+        if ((rc = solve_captured(c, box, nullptr, back_graph, &back_ran)) == TDP_OK && !back_ran) rc = back_graph();
+    } else {
+        rc = setup_and_solve(c, box, nullptr);
+        if (rc == TDP_OK) rc = back();
+    }
+    if (rc) {
         c->ev_pool.push_back(ev);
         return rc;
     }
-    forces_and_energy(c, box, d_forces != nullptr, d_energy != nullptr);
-    if (d_forces && c->n > 0) launch_scatter_accumulate(c->stream, c->npad, c->d_perm, c->d_frc, d_forces);
-    if (d_energy)
-        TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     TDP_HIP(c, hipEventRecord(ev, c->stream));
     TDP_HIP(c, hipStreamWaitEvent(user, ev, 0));
     c->ev_pool.push_back(ev);

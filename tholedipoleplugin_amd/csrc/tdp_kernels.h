@@ -143,7 +143,11 @@ void launch_reduce_energy(hipStream_t s, const SegLaunch& G, const double* part_
 void launch_map_torque(hipStream_t s, int n, const double4* posa, const int4* axis, const double4* torque,
                        double4* contrib);
 void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx, const double4* contrib, double* frc);
-void launch_scatter_accumulate(hipStream_t s, int n, const int* perm, const double* src, double* dst);
+// dst[perm[k]] += src[k]; with done (device stop count) only once *done >= nseg.
+void launch_scatter_accumulate(hipStream_t s, int n, const int* perm, const double* src, double* dst,
+                               const int* done = nullptr, int nseg = 0);
+// dst[k] = src[k] (k < n) once *done >= nseg.
+void launch_copy_guarded(hipStream_t s, int n, const double* src, double* dst, const int* done, int nseg);
 void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const double4* posa, const double4* qd,
                       const double4* mu, bool pbc, const double* box3, double* out);
 void launch_moments(hipStream_t s, int n, const double4* posa, const double4* qd, const double4* mu,

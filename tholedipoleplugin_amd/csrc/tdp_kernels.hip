@@ -2088,10 +2088,12 @@ __global__ void k_gather_torque(int n, const int* __restrict__ goff, const int* 
 
 // dst[perm[i]] += src[i] per component: forces in the internal order are accumulated into the
 // caller's array in its own order.
+// With a guard, only once every system's solve has stopped (*done >= nseg): the launch can then be
+// enqueued before the host has checked the solve.
 __global__ void k_scatter_accumulate(int n, const int* __restrict__ perm, const double* __restrict__ src,
-                                     double* __restrict__ dst) {
+                                     double* __restrict__ dst, const int* __restrict__ done, int nseg) {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
+    if (k >= n || (done && *done < nseg)) return;
     int o = perm[k];
     if (o < 0) return;
     dst[3 * o] += src[3 * k];
@@ -2377,8 +2379,20 @@ void launch_gather_torque(hipStream_t s, int n, const int* goff, const int* gidx
     hipLaunchKernelGGL(k_gather_torque, dim3((n + 255) / 256), dim3(256), 0, s, n, goff, gidx, contrib, frc);
 }
 
-void launch_scatter_accumulate(hipStream_t s, int n, const int* perm, const double* src, double* dst) {
-    hipLaunchKernelGGL(k_scatter_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, src, dst);
+__global__ void k_copy_guarded(int n, const double* __restrict__ src, double* __restrict__ dst,
+                               const int* __restrict__ done, int nseg) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n || *done < nseg) return;
+    dst[k] = src[k];
+}
+
+void launch_scatter_accumulate(hipStream_t s, int n, const int* perm, const double* src, double* dst, const int* done,
+                               int nseg) {
+    hipLaunchKernelGGL(k_scatter_accumulate, dim3((n + 255) / 256), dim3(256), 0, s, n, perm, src, dst, done, nseg);
+}
+
+void launch_copy_guarded(hipStream_t s, int n, const double* src, double* dst, const int* done, int nseg) {
+    hipLaunchKernelGGL(k_copy_guarded, dim3((n + 255) / 256), dim3(256), 0, s, n, src, dst, done, nseg);
 }
 
 void launch_potential(hipStream_t s, int n, int ngrid, const double* grid, const double4* posa, const double4* qd,

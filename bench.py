@@ -210,7 +210,9 @@ def cluster_bench(torch, steps, warmup):
     res = {}
     for prec in ("mixed", "double"):
         b = _lib.HipBatchHandle([c.force for c in cl], precision=prec, solver="pcg")
-        tiles_per_item = 0 if NO_AUTOTUNE else b.autotune(pos_h)
+        if not NO_AUTOTUNE:
+            b.autotune(pos_h)
+        tiles_per_item = b.work_schedule()[0]   # per size bucket: 1-2, 3-4, 5-8, 9-16, 17+ tiles
         for _ in range(warmup):
             b.execute_device(pos.data_ptr(), frc.data_ptr(), en.data_ptr(), stream)
         torch.cuda.synchronize()
@@ -224,7 +226,8 @@ def cluster_bench(torch, steps, warmup):
         mu = b.dipoles(pos_h, None, _lib.DIPOLES_INDUCED)
         b.close()
         res[prec] = dict(seconds=dt, iters=iters, energy=e, forces=f, mu=mu, tiles_per_item=tiles_per_item)
-        print("# clusters batched %s: %.2f ms per set of %d, PCG iterations %d..%d, %d tiles per work item" % (
+        print("# clusters batched %s: %.2f ms per set of %d, PCG iterations %d..%d, tiles per work item by size "
+              "bucket %s" % (
             prec, dt * 1e3, len(cl), min(iters), max(iters), tiles_per_item), file=sys.stderr, flush=True)
     # one FP64 context per cluster, evaluated in turn on device buffers
     handles = [_lib.HipContextHandle(c.force, np.zeros(9), precision="double", solver="pcg") for c in cl]
@@ -258,7 +261,9 @@ def cluster_bench(torch, steps, warmup):
                                                  np.sqrt(np.mean(d["forces"] ** 2))),
         "mixed_mu_max_err_vs_fp64_debye": float(np.max(np.abs(m["mu"] - d["mu"]))) / DEBYE,
         "pcg_iterations_max": int(max(d["iters"])),
-        "tiles_per_work_item": {"mixed": res["mixed"]["tiles_per_item"], "fp64": res["double"]["tiles_per_item"]},
+        "tiles_per_work_item_by_size_bucket": {"buckets_tiles": ["1-2", "3-4", "5-8", "9-16", "17+"],
+                                               "mixed": res["mixed"]["tiles_per_item"],
+                                               "fp64": res["double"]["tiles_per_item"]},
     }
 
 

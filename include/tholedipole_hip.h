@@ -232,11 +232,24 @@ int tdp_set_warm_start(tdp_context* ctx, int32_t enable);
 /* Schedule autotuner (no counterpart in the reference): times the T*mu pass on the given positions
  * (host [3N], caller's order; box host [9] or NULL as for tdp_execute) for each candidate length of
  * the pair kernels' work items (1, 2, 3, 4, 6, 8, 12 or 16 tiles of one tile row) and keeps the
- * fastest for the context; *tiles_per_item (may be NULL) receives it.  Without it a context uses
- * min(3, max(1, tile pairs / 16,384)) tiles.  The choice changes only the order in
+ * fastest for the context.  The systems of a context fall into size buckets by their count of 64-site
+ * tiles (1-2, 3-4, 5-8, 9-16, 17 and more), each with its own length: the tuner first sweeps one
+ * length for all buckets, then, when the systems of a batch occupy more than one bucket, sweeps each
+ * occupied bucket in turn (largest systems first) with the others held.  *tiles_per_item (may be NULL)
+ * receives the length of the bucket of the largest system.  Without it a context uses
+ * min(3, max(1, tile pairs / 16,384)) tiles in every bucket.  The choice changes only the order in
  * which partial sums are added, i.e. results at the rounding level; after it, evaluations of the
- * context are deterministic again. */
+ * context are deterministic again.  Contexts with the Ewald option have no work items: no effect. */
 int tdp_autotune(tdp_context* ctx, const double* positions, const double* box, int32_t* tiles_per_item);
+
+/* The work-item schedule of a context (no counterpart in the reference): tiles_per_item[5] receives the
+ * length of each size bucket; tiles_per_system_bound[5] (may be NULL) the largest tile count of each
+ * bucket (2, 4, 8, 16, INT32_MAX). */
+int tdp_get_work_schedule(const tdp_context* ctx, int32_t* tiles_per_item, int32_t* tiles_per_system_bound);
+
+/* Sets the work-item schedule (no counterpart in the reference): tiles_per_item[5], one length in
+ * [1, 4096] per size bucket, as tdp_autotune would choose it; the same effect on results. */
+int tdp_set_work_schedule(tdp_context* ctx, const int32_t* tiles_per_item);
 
 /* ---- Batches of independent systems.  No counterpart in the reference, whose kernel holds one
  * system: these entry points evaluate many NoCutoff systems (clusters) at once, each with the

@@ -174,6 +174,42 @@ def test_batch_of_config3_clusters_size_range():
         off += n
 
 
+def test_batch_size_bucket_schedules_keep_results():
+    """Work-item lengths set per size bucket (tdp_set_work_schedule) and chosen per bucket by
+    tdp_autotune change the results of a batch whose clusters fall into several buckets only at the
+    rounding level; evaluations after the tuner are deterministic."""
+    _say("case batch size-bucket schedules")
+    sizes = [20, 45, 90, 170, 330, 400, 25, 130]   # 1 to 19 tiles: all five buckets
+    members = []
+    for k, m in enumerate(sizes):
+        c = systems.water_cluster(m, seed=1300 + k)
+        c.force.setMutualInducedTargetEpsilon(1e-11)
+        c.force.setMutualInducedMaxIterations(1000)
+        members.append((c.force, np.asarray(c.positions, np.float64)))
+    pos = np.concatenate([p for _, p in members])
+    b = _lib.HipBatchHandle([f for f, _ in members], precision="double", solver="pcg")
+    per0, bound = b.work_schedule()
+    assert bound[:4] == [2, 4, 8, 16] and bound[4] == 2**31 - 1
+    assert len(set(per0)) == 1
+    e0, f0 = b.execute(pos)
+    b.set_work_schedule([1, 2, 3, 6, 16])
+    assert b.work_schedule()[0] == [1, 2, 3, 6, 16]
+    e1, f1 = b.execute(pos)
+    assert np.all(np.abs(e1 - e0) <= 1e-12 * np.abs(e0))
+    assert rel_rms(f1, f0) <= 1e-10
+    C = b.autotune(pos)
+    per, _ = b.work_schedule()
+    assert all(v in (1, 2, 3, 4, 6, 8, 12, 16) for v in per) and C == per[4]
+    e2, f2 = b.execute(pos)
+    e3, f3 = b.execute(pos)
+    assert np.array_equal(e2, e3) and np.array_equal(f2, f3)
+    assert np.all(np.abs(e2 - e0) <= 1e-12 * np.abs(e0))
+    assert rel_rms(f2, f0) <= 1e-10
+    with pytest.raises(_lib.TdpError, match="tiles per work item"):
+        b.set_work_schedule([0, 1, 1, 1, 1])
+    b.close()
+
+
 def test_batch_rejects_bad_input():
     members = batch_members()
     rs = ref_systems.water4()

@@ -34,7 +34,7 @@ EXPORTED_SYMBOLS = [
     "tdp_get_dipoles",
     "tdp_get_electrostatic_potential",
     "tdp_get_system_multipole_moments", "tdp_get_pme_parameters", "tdp_get_solver_stats", "tdp_set_timing",
-    "tdp_autotune", "tdp_set_warm_start",
+    "tdp_autotune", "tdp_get_work_schedule", "tdp_set_work_schedule", "tdp_set_warm_start",
     "tdp_batch_create", "tdp_batch_num_systems", "tdp_batch_execute", "tdp_batch_execute_device",
     "tdp_batch_get_solver_stats", "tdp_batch_solve_responses",
 ]
@@ -114,6 +114,8 @@ def load():
     L.tdp_get_solver_stats.argtypes = [ctx_p, C.POINTER(SolverStats)]
     L.tdp_set_timing.argtypes = [ctx_p, C.c_int32]
     L.tdp_autotune.argtypes = [ctx_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int32)]
+    L.tdp_get_work_schedule.argtypes = [ctx_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.tdp_set_work_schedule.argtypes = [ctx_p, C.POINTER(C.c_int32)]
     L.tdp_set_warm_start.argtypes = [ctx_p, C.c_int32]
     L.tdp_batch_create.argtypes = [C.POINTER(ForceDesc), C.c_int32, C.POINTER(Options), C.POINTER(ctx_p)]
     L.tdp_batch_num_systems.argtypes = [ctx_p]
@@ -302,6 +304,21 @@ class HipContextHandle:
         out = C.c_int32(0)
         self._check(self.L.tdp_autotune(self.h, _ptr(pos), None if b is None else _ptr(b), C.byref(out)))
         return out.value
+
+    def work_schedule(self):
+        """Tiles per work item of each size bucket and each bucket's largest tile count
+        (tdp_get_work_schedule)."""
+        per = (C.c_int32 * 5)()
+        bound = (C.c_int32 * 5)()
+        self._check(self.L.tdp_get_work_schedule(self.h, per, bound))
+        return list(per), list(bound)
+
+    def set_work_schedule(self, tiles_per_item):
+        """Sets the tiles per work item of the five size buckets (tdp_set_work_schedule)."""
+        if len(tiles_per_item) != 5:
+            raise ValueError("one length per size bucket (5)")
+        per = (C.c_int32 * 5)(*[int(v) for v in tiles_per_item])
+        self._check(self.L.tdp_set_work_schedule(self.h, per))
 
 
 class HipBatchHandle(HipContextHandle):

@@ -74,6 +74,11 @@ __global__ void k_zero16(uint4* __restrict__ p, size_t n) {
 // Evaluations between recomputations of the internal spatial order.
 constexpr int kReorderInterval = 100;
 
+// Size buckets of the systems of a context by tile count (1-2, 3-4, 5-8, 9-16, 17 and more): each
+// bucket has its own work-item length, which the autotuner may choose separately.
+constexpr int kSizeBuckets = 5;
+inline int size_bucket(int tiles) { return tiles <= 2 ? 0 : tiles <= 4 ? 1 : tiles <= 8 ? 2 : tiles <= 16 ? 3 : 4; }
+
 struct tdp_context {
     // ---- parameters (host copies)
     // n particles in the caller's order; nblk tiles of 64 slots (npad = 64 nblk) in the internal
@@ -102,7 +107,8 @@ struct tdp_context {
     int evals_since_order = 0;
     size_t cap_exc_J = 0, cap_exc_mask = 0, cap_gidx = 0;
     size_t cap_work = 0, cap_pbuf = 0, cap_pbuf2 = 0, cap_part_e = 0, cap_slab32 = 0;
-    int work_C = 0;   // tiles per upper-triangle work item
+    int work_C = 0;   // tiles per upper-triangle work item (of the largest system's size bucket)
+    int work_Cb[kSizeBuckets] = {};   // tiles per work item, per size bucket of the systems (size_bucket)
     std::vector<int> exc_start, exc_J;
     std::vector<unsigned long long> exc_mask;
     std::vector<int> goff, gidx;
@@ -1433,13 +1439,17 @@ int upload_positions(tdp_context* c, const double* positions) {
 // system), with the item partial buffers sized for them.  Short items keep the many one-wave
 // workgroups evenly spread over the CUs to the end of a launch; long items need fewer item
 // partials in the reduction.
-int set_work_items(tdp_context* c, int C) {
+// Work items of the pair kernels with Cb[b] tiles per item for the systems of size bucket b.
+int set_work_items_buckets(tdp_context* c, const int* Cb) {
     const int nb = c->nblk, ns = c->nseg;
     std::vector<int4> work;
     std::vector<int> row_start(nb + 1, 0);
+    int largest = 0;
     for (int g = 0; g < ns; g++) {
         c->seg_w0[g] = (int)work.size();
         const int t1 = c->seg_t1[g];
+        const int C = std::max(1, Cb[size_bucket(t1 - c->seg_t0[g])]);
+        largest = std::max(largest, t1 - c->seg_t0[g]);
         for (int I = c->seg_t0[g]; I < t1; I++) {
             row_start[I] = (int)work.size();
             for (int J0 = I; J0 < t1; J0 += C) work.push_back(make_int4(I, J0, std::min(t1, J0 + C), 0));
@@ -1448,7 +1458,8 @@ int set_work_items(tdp_context* c, int C) {
     }
     row_start[nb] = (int)work.size();
     c->nwork = (int)work.size();
-    c->work_C = C;
+    for (int b = 0; b < kSizeBuckets; b++) c->work_Cb[b] = std::max(1, Cb[b]);
+    c->work_C = c->work_Cb[size_bucket(std::max(largest, 1))];
     c->graph_gen++;
     std::vector<int2> segw(ns);
     for (int g = 0; g < ns; g++) segw[g] = make_int2(c->seg_w0[g], c->seg_w1[g]);
@@ -1470,6 +1481,12 @@ int set_work_items(tdp_context* c, int C) {
     TDP_HIP(c, hipMemcpyAsync(c->d_segw, segw.data(), ns * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     TDP_HIP(c, hipStreamSynchronize(c->stream));
     return TDP_OK;
+}
+
+int set_work_items(tdp_context* c, int C) {
+    int Cb[kSizeBuckets];
+    for (int b = 0; b < kSizeBuckets; b++) Cb[b] = C;
+    return set_work_items_buckets(c, Cb);
 }
 
 // Tiles, work items and partial-slab layout of the systems (sizes in the caller's order).
@@ -2451,44 +2468,93 @@ int tdp_autotune(tdp_context* c, const double* positions, const double* box, int
     if (c->n > 0 && !c->ewald) {   // (the Ewald option's passes have no work-item schedule)
         if ((rc = upload_positions(c, positions))) return rc;
         if ((rc = prepare_field(c, box, positions))) return rc;   // internal order, tile tables, mu = alpha E0
-        int longest = 1;
-        for (int g = 0; g < c->nseg; g++) longest = std::max(longest, c->seg_t1[g] - c->seg_t0[g]);
+        int longest[kSizeBuckets] = {};   // longest system (tiles) of each bucket, 0 when the bucket is empty
+        for (int g = 0; g < c->nseg; g++) {
+            const int t = c->seg_t1[g] - c->seg_t0[g];
+            if (t > 0) longest[size_bucket(t)] = std::max(longest[size_bucket(t)], t);
+        }
         static const int cand[] = {1, 2, 3, 4, 6, 8, 12, 16};
         constexpr int kCand = sizeof(cand) / sizeof(cand[0]);
-        float cand_ms[kCand];
-        for (int q = 0; q < kCand; q++) cand_ms[q] = 0.f;
         hipEvent_t e0 = take_event(c), e1 = take_event(c);
-        // two rounds over the candidates, each timed on three passes; a candidate's time is the smaller
-        // of its two, so that one disturbed measurement does not decide the schedule
-        for (int round = 0; round < 2; round++) {
-            for (int q = 0; q < kCand; q++) {
-                const int C = cand[q];
-                if (C > longest && C != 1) break;
-                if ((rc = set_work_items(c, C))) return rc;
-                PairLaunch L = pair_launch(c, box);
-                matvec(c, L, c->mu, nullptr);   // warm-up
-                TDP_HIP(c, hipEventRecord(e0, c->stream));
-                for (int k = 0; k < 3; k++) matvec(c, L, c->mu, nullptr);
-                TDP_HIP(c, hipEventRecord(e1, c->stream));
-                TDP_HIP(c, hipEventSynchronize(e1));
-                float ms = 0.f;
-                hipEventElapsedTime(&ms, e0, e1);
-                if (round == 0 || ms < cand_ms[q]) cand_ms[q] = ms;
-            }
-        }
+        // one schedule's time: a warm-up pass, then three timed passes
+        auto time_schedule = [&](const int* Cb, float* ms) -> int {
+            int r = set_work_items_buckets(c, Cb);
+            if (r) return r;
+            PairLaunch L = pair_launch(c, box);
+            matvec(c, L, c->mu, nullptr);
+            TDP_HIP(c, hipEventRecord(e0, c->stream));
+            for (int k = 0; k < 3; k++) matvec(c, L, c->mu, nullptr);
+            TDP_HIP(c, hipEventRecord(e1, c->stream));
+            TDP_HIP(c, hipEventSynchronize(e1));
+            *ms = 0.f;
+            hipEventElapsedTime(ms, e0, e1);
+            return TDP_OK;
+        };
+        // Sweep of the candidates for the buckets in `which` (all buckets set to the same length when
+        // which < 0), the others held at Cb: two rounds, a candidate's time the smaller of its two, so that
+        // one disturbed measurement does not decide.  Leaves the best length in Cb.
+        auto sweep = [&](int which, int* Cb) -> int {
+            int lim = 0;
+            for (int b = 0; b < kSizeBuckets; b++)
+                if (which < 0 || which == b) lim = std::max(lim, longest[b]);
+            float cand_ms[kCand];
+            for (int q = 0; q < kCand; q++) cand_ms[q] = 0.f;
+            int trial[kSizeBuckets];
+            for (int round = 0; round < 2; round++)
+                for (int q = 0; q < kCand; q++) {
+                    if (cand[q] > lim && cand[q] != 1) break;
+                    for (int b = 0; b < kSizeBuckets; b++) trial[b] = (which < 0 || which == b) ? cand[q] : Cb[b];
+                    float ms;
+                    int r = time_schedule(trial, &ms);
+                    if (r) return r;
+                    if (round == 0 || ms < cand_ms[q]) cand_ms[q] = ms;
+                }
+            int best = -1;
+            for (int q = 0; q < kCand; q++)
+                if (cand_ms[q] > 0.f && (best < 0 || cand_ms[q] < cand_ms[best])) best = q;
+            if (best >= 0)
+                for (int b = 0; b < kSizeBuckets; b++)
+                    if (which < 0 || which == b) Cb[b] = cand[best];
+            return TDP_OK;
+        };
+        int Cb[kSizeBuckets];
+        for (int b = 0; b < kSizeBuckets; b++) Cb[b] = c->work_Cb[b];
+        // one length for every system first, then, for a batch whose systems fall into several buckets,
+        // one pass of coordinate descent over the occupied buckets from the largest systems down
+        if ((rc = sweep(-1, Cb))) return rc;
+        int occupied = 0;
+        for (int b = 0; b < kSizeBuckets; b++) occupied += longest[b] > 0;
+        if (occupied > 1)
+            for (int b = kSizeBuckets - 1; b >= 0; b--)
+                if (longest[b] > 0 && (rc = sweep(b, Cb))) return rc;
         c->ev_pool.push_back(e0);
         c->ev_pool.push_back(e1);
-        int best = c->work_C;
-        float best_ms = 0.f;
-        for (int q = 0; q < kCand; q++)
-            if (cand_ms[q] > 0.f && (best_ms == 0.f || cand_ms[q] < best_ms)) {
-                best_ms = cand_ms[q];
-                best = cand[q];
-            }
-        if ((rc = set_work_items(c, best))) return rc;
+        if ((rc = set_work_items_buckets(c, Cb))) return rc;
     }
     if (tiles_per_item) *tiles_per_item = c->work_C;
     return TDP_OK;
+}
+
+int tdp_get_work_schedule(const tdp_context* c, int32_t* tiles_per_item, int32_t* tiles_per_system_bound) {
+    if (!c || !tiles_per_item) return TDP_ERR_INVALID_ARGUMENT;
+    for (int b = 0; b < kSizeBuckets; b++) {
+        tiles_per_item[b] = c->work_Cb[b];
+        if (tiles_per_system_bound) tiles_per_system_bound[b] = b == kSizeBuckets - 1 ? INT32_MAX : 2 << b;
+    }
+    return TDP_OK;
+}
+
+int tdp_set_work_schedule(tdp_context* c, const int32_t* tiles_per_item) {
+    if (!c || !tiles_per_item) return fail(c, TDP_ERR_INVALID_ARGUMENT, "null argument");
+    int Cb[kSizeBuckets];
+    for (int b = 0; b < kSizeBuckets; b++) {
+        if (tiles_per_item[b] < 1 || tiles_per_item[b] > 4096)
+            return fail(c, TDP_ERR_INVALID_ARGUMENT, "tiles per work item must lie in [1, 4096]");
+        Cb[b] = tiles_per_item[b];
+    }
+    if (c->ewald) return TDP_OK;   // (the Ewald option's passes have no work-item schedule)
+    hipSetDevice(c->device);
+    return set_work_items_buckets(c, Cb);
 }
 
 int tdp_set_warm_start(tdp_context* c, int32_t enable) {

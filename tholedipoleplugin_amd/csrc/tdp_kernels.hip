@@ -239,6 +239,12 @@ __device__ __forceinline__ void load_site(ElecSite& s, double4 p, double4 q, dou
 // j sums go to the column-major partial slab Pj (component c of slot s at Pj[c S + s]), each
 // item's i sums to Pi[w]; k_col_reduce adds them in a fixed order, so results are deterministic.
 
+// Last rotation step of a diagonal tile holding cnt sites.  Steps 1..32 cover every pair (a, b), a < b,
+// of a full tile: at step b - a from lane a when b - a <= 32, at step 64 - (b - a) from lane b otherwise.
+// With fewer sites the largest step needed is min(32, cnt - 1); the steps left out would evaluate no
+// pair, so the sums are the same.
+__device__ __forceinline__ int diag_last_step(int cnt) { return cnt > kTile / 2 ? kTile / 2 : cnt - 1; }
+
 // Tile (I, J), I <= J, of one system in column-major upper-triangle order: the tiles of column J
 // are contiguous, starting at col[J] with the system's first tile.
 __device__ __forceinline__ size_t slab_tile(const TileMap& tm, int I, int J) {
@@ -360,7 +366,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
             }
             kend = kTile;
         } else {
-            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            const int k0 = diag ? 1 : 0, k1 = diag ? diag_last_step(cntI) : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
                 const bool ok = ivalid && (jj < cntJ) && !((mi >> jj) & 1ull) &&
@@ -893,7 +899,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3))) void k_
             s_ia[lane] = inva[jb + lane];
             __syncthreads();
             jx = jy = jz = 0.0;
-            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            const int k0 = diag ? 1 : 0, k1 = diag ? diag_last_step(cntI) : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
                 const bool ok = ivalid && (jj < cntJ) && !((mi >> jj) & 1ull) &&
@@ -1026,7 +1032,7 @@ __global__ __launch_bounds__(64) void k_sym_field(TileMap tm, int nblk, const in
             }
             kend = kTile;
         } else {
-            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            const int k0 = diag ? 1 : 0, k1 = diag ? diag_last_step(cntI) : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
                 const bool ok = ivalid && (jj < cntJ) && !((m0 >> jj) & 1ull) &&
@@ -1550,7 +1556,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
             jacc.fz += s_jn[2 * kTile + lane];
             kend = kTile;
         } else {
-            const int k0 = diag ? 1 : 0, k1 = diag ? kTile / 2 : kTile - 1;
+            const int k0 = diag ? 1 : 0, k1 = diag ? diag_last_step(cntI) : kTile - 1;
             for (int k = k0; k <= k1; k++) {
                 const int jj = (lane + k) & (kTile - 1);
                 const bool z0 = (m0 >> jj) & 1ull, h = (mh >> jj) & 1ull, zi = (mi >> jj) & 1ull;

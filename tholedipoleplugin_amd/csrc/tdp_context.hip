@@ -216,7 +216,7 @@ struct tdp_context {
     unsigned short* d_idx16 = nullptr;
     int *d_idx64 = nullptr, *d_w32 = nullptr, *d_w64 = nullptr, *d_need = nullptr;
     int ew_lds_tiles = 0;   // neighbour tiles the stored T v pass stages in LDS (0: gathers from HBM)
-    float4* d_t32 = nullptr;
+    float2* d_t32 = nullptr;
     double4* d_t64 = nullptr;
     unsigned *d_cand32 = nullptr, *d_cand64 = nullptr;   // the E0 pass's pair candidates
     int2* d_ccnt = nullptr;
@@ -494,6 +494,14 @@ void compute_order(tdp_context* c, const double* pos3, const double* box) {
             const int m = r.e - r.b;
             if (m <= 64) {
                 std::sort(ids.begin() + r.b, ids.begin() + r.e);
+                if (c->ewald && m == 64) {
+                    // full tiles of the Ewald option in transposed order: the real-space passes give
+                    // wave w the slots 8w..8w+7, which then hold the sorted positions w, w+8, ...,
+                    // w+56 (an even sample of the tile, read as one 128-byte line per wave)
+                    int t[64];
+                    for (int k = 0; k < 64; k++) t[8 * (k & 7) + (k >> 3)] = ids[r.b + k];
+                    std::copy(t, t + 64, ids.begin() + r.b);
+                }
                 continue;
             }
             double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -656,14 +664,14 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
     c->d_cand32 = c->d_cand64 = nullptr;
     c->d_ccnt = nullptr;
     c->ew_stored = false;
-    // capacities in whole candidate blocks of 8 (a multiple of kEwSpmvU)
+    // capacities in whole candidate blocks of 8 (FP32 entries are stored in pairs)
     cap32 = (cap32 + 7) / 8 * 8;
     cap64 = (cap64 + 7) / 8 * 8;
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     const size_t rows = (size_t)c->nblk * kEwWaves * 64;   // 64 slots per tile
     const size_t n32 = mixed ? rows * cap32 : 1, n64 = rows * cap64;
     void* p[7] = {};
-    const size_t bytes[7] = {n32 * sizeof(unsigned short), n32 * sizeof(float4), n64 * sizeof(int),
+    const size_t bytes[7] = {n32 * sizeof(unsigned short), n32 * sizeof(float2), n64 * sizeof(int),
                              n64 * sizeof(double4), n32 * sizeof(unsigned), n64 * sizeof(unsigned), rows * sizeof(int2)};
     for (int k = 0; k < 7; k++)
         if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
@@ -673,7 +681,7 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
         }
     for (int k = 0; k < 7; k++) c->allocations.push_back(p[k]);
     c->d_idx16 = (unsigned short*)p[0];
-    c->d_t32 = (float4*)p[1];
+    c->d_t32 = (float2*)p[1];
     c->d_idx64 = (int*)p[2];
     c->d_t64 = (double4*)p[3];
     c->d_cand32 = (unsigned*)p[4];
@@ -691,7 +699,10 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.npad = c->npad;
     E.nblk = c->nblk;
     E.tcnt = c->d_tcnt;
-    for (int k = 0; k < 3; k++) E.L[k] = box9[4 * k];
+    for (int k = 0; k < 3; k++) {
+        E.L[k] = box9[4 * k];
+        E.iL[k] = 1.0 / E.L[k];
+    }
     E.rc2 = c->cutoff * c->cutoff;
     const float cm = (float)(c->cutoff * 1.0001 + 1e-4);
     E.cut2_tiles = cm * cm;
@@ -741,6 +752,8 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.w64 = c->d_w64;
     E.need = c->d_need;
     E.lds_tiles = c->ew_lds_tiles;
+    E.tb_o = c->tb_o;
+    E.posl = c->posl;
     // reciprocal space in FP64 in both precisions: FP32 grids and FFTs put 2.4e-8 e*nm into the
     // fixed point of the 36,864-site box, above the 1e-6 D (2.08e-8 e*nm) bar
     E.mixed = false;
@@ -1069,9 +1082,10 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
                 c->graph_gen++;
                 break;
             }
-            {   // LDS copy of v for the FP32 lists: the wave's slots of up to lds_tiles tiles, 64 KB at most
+            {   // LDS copy of the slot data for the FP32 lists: the wave's slots of up to lds_tiles tiles,
+                // 64 KB at most
                 int lt = (hn[6] + 7) / 8 * 8;
-                if ((size_t)lt * 8 * sizeof(float4) > 65536) lt = 0;
+                if ((size_t)lt * 8 * kEwSlotLds > 65536) lt = 0;
                 if (lt != c->ew_lds_tiles) {
                     c->ew_lds_tiles = lt;
                     c->graph_gen++;
@@ -2376,7 +2390,7 @@ int tdp_get_ewald_info(const tdp_context* cc, tdp_ewald_info* info) {
     info->entries64 = c->ew_entries[1];
     info->slots32 = c->ew_slots[0];
     info->slots64 = c->ew_slots[1];
-    info->bytes_per_entry32 = (int32_t)(sizeof(unsigned short) + sizeof(float4));
+    info->bytes_per_entry32 = (int32_t)(sizeof(unsigned short) + sizeof(float2));
     info->bytes_per_entry64 = (int32_t)(sizeof(int) + sizeof(double4));
     info->tiles = c->nblk;
     TDP_HIP(c, hipSetDevice(c->device));

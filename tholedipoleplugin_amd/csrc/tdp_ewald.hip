@@ -510,11 +510,12 @@ __device__ __forceinline__ void ew_radial32(float alpha, float c1, float c2, flo
     }
 }
 
-// Bits w, w + kEwWaves, ..., w + 7 kEwWaves of x packed into bits 0-7 (the slots of wave w in a
-// tile's 64-bit mask): the spread bits times sum_i 2^(7 i + 7) land, without carries, in the top byte.
+// The real-space passes give wave w of a tile row the slots 8w + m (m = 0..7) of every neighbour
+// tile (contiguous, so that a wave's slots of a tile are one 128-byte line of a float4 array).
+__device__ __forceinline__ int ew_wave_slot(int w, int m) { return 8 * w + m; }
+// Bits 8w..8w+7 of x (the slots of wave w in a tile's 64-bit mask) as bits 0-7.
 __device__ __forceinline__ unsigned long long ew_slots_of_wave(unsigned long long x, int w) {
-    x = (x >> w) & 0x0101010101010101ull;
-    return (x * 0x0102040810204080ull) >> 56;
+    return (x >> (8 * w)) & 0xffull;
 }
 
 // The i-side terms of one unscaled pair inside the cutoff (d = x_i - x_j): the field of charge qj
@@ -564,7 +565,7 @@ __device__ __forceinline__ void ew_pair_terms(const EwR<T>& R, T dx, T dy, T dz,
 // per row, part h taking the neighbour tiles h, h + NP, ... (the T v and force passes use two parts:
 // twice as many workgroups as rows fill the CUs more evenly, one 8-wave workgroup per CU at these
 // register counts; the E0 pass, which builds the stored lists, one, for longer lists).  Every wave
-// visits every tile of its part and takes its 8 slots k = w + kEwWaves m, eight tiles at a time
+// visits every tile of its part and takes its 8 slots k = 8w + m, eight tiles at a time
 // staged in the wave's own LDS slice, so that each wave sees an even spatial sample of the row's
 // neighbourhood and the waves' stored lists have nearly equal lengths per slot (their padding to
 // the longest is small).  Only the slots within the cutoff (plus a margin) of row I's bounding box
@@ -587,7 +588,6 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     __shared__ double4 s_p[kEwWaves][kTile];
     __shared__ double4 s_v[kEwWaves][kTile];   // charge in w
     __shared__ double s_ia[kEwWaves][kTile];
-    __shared__ int s_j[MODE == kEwE0 ? kEwWaves : 1][kTile];   // slot index of a group slot (list build)
     const int i = I * kTile + lane;
     const bool ivalid = lane < E.tcnt[I];
     const double4 pi = posa[i];
@@ -602,13 +602,8 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     const int cnt = (E.nbr_cnt[I] - h + NP - 1) / NP;
     const int* nb = E.nbr + (size_t)I * E.nblk + h;
     const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
-    const int list = I * kEwWaves + w;   // (the E0 pass, which builds the lists, has one part)
-    const bool build = MODE == kEwE0 && E.idx64 != nullptr;   // store T's blocks while forming E0
-    // (FP32 list indices address at most kEwMaxLocalTiles neighbour tiles: the host checks the
-    // largest count and does not use the lists beyond it)
-    int n32 = 0, n64 = 0;
     for (int t0 = 0; t0 < cnt; t0 += 8) {
-        // a group of up to 8 neighbour tiles: lane 8g + m holds slot k = w + kEwWaves m of tile
+        // a group of up to 8 neighbour tiles: lane 8g + m holds slot k = 8w + m of tile
         // t0 + g and tests it against row I's bounding box (cutoff plus margin)
         const int g8 = lane >> 3, m8 = lane & 7;
         int Jl = 0;
@@ -617,7 +612,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         bool near_box = false;
         if (t0 + g8 < cnt) {
             Jl = nb[NP * (t0 + g8)];
-            const int ks = w + kEwWaves * m8;
+            const int ks = ew_wave_slot(w, m8);
             if (ks < E.tcnt[Jl]) {
                 const int j = Jl * kTile + ks;
                 pl = posa[j];
@@ -639,42 +634,34 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         s_p[w][lane] = pl;
         s_v[w][lane] = vl;
         s_ia[w][lane] = ial;
-        if (MODE == kEwE0) s_j[w][lane] = Jl * kTile + w + kEwWaves * m8;
         // the slice is this wave's own: its LDS operations complete in order
         __builtin_amdgcn_wave_barrier();
-        // scaled pairs of these tile pairs are masked (the epilogue evaluates them); the E0 pass also
-        // needs the mask of T's pairs (iScale 0) for the stored blocks.  Bit 8g + m of the group's
-        // masks is slot w + kEwWaves m of tile t0 + g.
-        unsigned long long mask = 0ull, maskT = 0ull;
+        // scaled pairs of these tile pairs are masked (the epilogue evaluates them).  Bit 8g + m of
+        // the group's mask is slot 8w + m of tile t0 + g.
+        unsigned long long mask = 0ull;
         for (int g = 0; g < 8; g++) {
             if (!((visit >> (8 * g)) & 0xffull)) continue;
             const int J = __builtin_amdgcn_readlane(Jl, 8 * g);
             for (int e = eS; e < eE; e++) {
                 if (E.exc_J[e] != J) continue;
                 const unsigned long long* mm = E.exc_mask + (size_t)e * 3 * kTile;
-                unsigned long long mk, mkT = 0ull;
+                unsigned long long mk;
                 if (MODE == kEwE0) {
                     mk = mm[lane] | mm[kTile + lane];
-                    mkT = mm[2 * kTile + lane];
                 } else if (MODE == kEwTmu) {
                     mk = mm[2 * kTile + lane];
                 } else {
                     mk = mm[lane] | mm[kTile + lane] | mm[2 * kTile + lane];
                 }
                 mask |= ew_slots_of_wave(mk, w) << (8 * g);
-                maskT |= ew_slots_of_wave(mkT, w) << (8 * g);
                 break;
             }
-            if (J == I && ((lane - w) & (kEwWaves - 1)) == 0 && lane >= w) {
-                const unsigned long long self = 1ull << (8 * g + (lane - w) / kEwWaves);
-                mask |= self;
-                maskT |= self;
-            }
+            if (J == I && (lane >> 3) == w) mask |= 1ull << (8 * g + (lane & 7));
         }
-        if (!ivalid) mask = maskT = ~0ull;
+        if (!ivalid) mask = ~0ull;
         // Phase A (one step per visited slot, all lanes): the FP64 pair criterion, giving each lane
-        // the bits of the pairs it evaluates.  E0: mA = the field's pairs, mB = the stored blocks;
-        // mixed T v / force: mA = pairs beyond r64 (FP32), mB = closer pairs (FP64); FP64: mB.
+        // the bits of the pairs it evaluates.  E0: mB = the field's pairs; mixed T v / force:
+        // mA = pairs beyond r64 (FP32), mB = closer pairs (FP64); FP64: mB.
         unsigned long long mA = 0ull, mB = 0ull;
         {
             unsigned long long v = visit;
@@ -687,12 +674,8 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
                 const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
                 const double r2x = ew_r2(dx, dy, dz);
                 const unsigned long long bit = 1ull << q;
-                const bool in = r2x < E.rc2;
-                if (MODE == kEwE0) {
-                    if (in && !(mask & bit)) mA |= bit;
-                    if (build && in && !(maskT & bit)) mB |= bit;
-                } else if (in && !(mask & bit)) {
-                    if (MIXED && r2x >= E.r64sq) mA |= bit;
+                if (r2x < E.rc2 && !(mask & bit)) {
+                    if (MODE != kEwE0 && MIXED && r2x >= E.r64sq) mA |= bit;
                     else mB |= bit;
                 }
             }
@@ -724,7 +707,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             ae += (double)fe;
         }
         double sx = 0.0, sy = 0.0, sz = 0.0, se = 0.0;
-        unsigned long long v = MODE == kEwE0 ? (mA | mB) : mB;
+        unsigned long long v = mB;
         while (v) {
             const int q = __builtin_ctzll(v);
             v &= v - 1ull;
@@ -736,75 +719,14 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
             const double4 vj = s_v[w][q];
             EwR<double> R;
             ew_radial(E, r2x, pi.w, pj.w, iai, s_ia[w][q], true, MODE == kEwForce && MUTUAL, R);
-            double tx = 0.0, ty = 0.0, tz = 0.0, te = 0.0;
-            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, vj.w, vj.x, vj.y, vj.z, tx, ty, tz,
-                                                te);
-            if (MODE != kEwE0 || ((mA >> q) & 1ull)) {
-                sx += tx;
-                sy += ty;
-                sz += tz;
-                se += te;
-            }
-            if (MODE == kEwE0 && ((mB >> q) & 1ull)) {   // the block T_ij = C2 d d^T - C1 of the T v passes
-                const double C1 = R.B1 + (R.l3 - 1.0) * R.b1;
-                const double C2 = R.B2 + (R.l5 - 1.0) * R.b2;
-                const double g = sqrt(fabs(C2));
-                if (E.pairs32 && r2x >= E.r64sq) {
-                    if (n32 < E.cap32) {
-                        const size_t at = ((size_t)list * E.cap32 + n32) * kTile + lane;
-                        E.idx16[at] = (unsigned short)((8 * t0 + q) | (C2 < 0.0 ? 0x8000 : 0));
-                        E.t32[at] = make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz));
-                    }
-                    n32++;
-                } else {
-                    if (n64 < E.cap64) {
-                        const size_t at = ((size_t)list * E.cap64 + n64) * kTile + lane;
-                        E.idx64[at] = s_j[w][q] | (C2 < 0.0 ? (int)0x80000000u : 0);
-                        E.t64[at] = make_double4(-C1, g * dx, g * dy, g * dz);
-                    }
-                    n64++;
-                }
-            }
+            ew_pair_terms<double, MODE, MUTUAL>(R, dx, dy, dz, qi, vi.x, vi.y, vi.z, vj.w, vj.x, vj.y, vj.z, sx, sy, sz,
+                                                se);
         }
         ax += sx;
         ay += sy;
         az += sz;
         ae += se;
         __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next group is staged
-    }
-    if (build) {   // list widths of this row and wave; the largest needed width for the capacity check
-        int m32 = n32, m64 = n64, s32 = n32, s64 = n64;
-        for (int off = 32; off > 0; off >>= 1) {
-            m32 = max(m32, __shfl_xor(m32, off, 64));
-            m64 = max(m64, __shfl_xor(m64, off, 64));
-            s32 += __shfl_xor(s32, off, 64);
-            s64 += __shfl_xor(s64, off, 64);
-        }
-        m32 = (m32 + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU;
-        // the T v pass reads every lane's list up to the wave's width: pad the shorter ones with
-        // zero blocks (on the slot itself)
-        const int w32c = min(m32, E.cap32), w64c = min(m64, E.cap64);
-        for (int n = n32; n < w32c; n++) {
-            const size_t at = ((size_t)list * E.cap32 + n) * kTile + lane;
-            E.idx16[at] = 0;
-            E.t32[at] = make_float4(0.f, 0.f, 0.f, 0.f);
-        }
-        for (int n = n64; n < w64c; n++) {
-            const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
-            E.idx64[at] = i;
-            E.t64[at] = make_double4(0.0, 0.0, 0.0, 0.0);
-        }
-        if (lane == 0) {
-            E.w32[list] = w32c;
-            E.w64[list] = w64c;
-            atomicMax(&E.need[0], m32);
-            atomicMax(&E.need[1], m64);
-            atomicAdd(&E.need[2], s32);
-            atomicAdd(&E.need[3], s64);
-            atomicAdd(&E.need[4], w32c);
-            atomicAdd(&E.need[5], w64c);
-            if (w == 0) atomicMax(&E.need[6], cnt);
-        }
     }
     // the row's sums over the waves, in wave order (the staging slices are reused)
     __syncthreads();
@@ -827,7 +749,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
 // The E0 pass when the blocks of T are stored, in two phases per wave (list (I, w)).
 // Phase A visits, as k_ew_pairs does, the wave's slots of the row's neighbour tiles that lie near
 // row I's box and records for each lane the pairs inside the cutoff as candidates 8 t + m (slot
-// w + kEwWaves m of neighbour tile t) with two flags (in the field sum, stored as a block of T),
+// 8w + m of neighbour tile t) with two flags (in the field sum, stored as a block of T),
 // in two streams: the pairs whose blocks are FP32 (beyond r64 in mixed precision) and the others
 // (FP64).  A lane's candidates go to its own 32-byte blocks of the stream (8 per block, so that the
 // lane's stores fill whole sectors).  Phase B runs through each lane's streams densely, every lane
@@ -844,12 +766,48 @@ __device__ __forceinline__ float4 ew_wrap32(const double4& p, const double* L, d
                        (float)(p.z - L[2] * floor(p.z * iL2)), 0.f);
 }
 
+// Minimum image of the separation of two tile origins, FP64 (iL = 1/L; the same function serves the
+// E0 pass's image test and the T v pass's offsets, so that both take the same image).
+__device__ __forceinline__ double ew_tile_shift(double d, double L, double iL) { return d - L * floor(d * iL + 0.5); }
+
+// True when every pair of tiles I and J within the cutoff takes the image of the tiles' origins:
+// per axis |D| + hI + hJ < L - rc with D = o_J - o_I under its minimum image.  A pair's separation is
+// D + u_j - u_i up to whole boxes with |u_j - u_i| <= hI + hJ; its minimum image is below rc and
+// every other image of it lies beyond L - rc.
+__device__ __forceinline__ bool ew_tile_image_ok(const EwLaunch& E, const double4& oI, const float4& hI,
+                                                 const double4& oJ, const float4& hJ, double rc) {
+    const double Dx = ew_tile_shift(oJ.x - oI.x, E.L[0], E.iL[0]);
+    const double Dy = ew_tile_shift(oJ.y - oI.y, E.L[1], E.iL[1]);
+    const double Dz = ew_tile_shift(oJ.z - oI.z, E.L[2], E.iL[2]);
+    return fabs(Dx) + (double)hI.x + (double)hJ.x < E.L[0] - rc &&
+           fabs(Dy) + (double)hI.y + (double)hJ.y < E.L[1] - rc &&
+           fabs(Dz) + (double)hI.z + (double)hJ.z < E.L[2] - rc;
+}
+
+// Slot j of tile J as the FP32 T v pass of row I stages it: (v_j, y_x) and (y_y, y_z), y = u_j + D_IJ
+// formed in FP64 and rounded once (the slot's offset from tile I's origin under the tile pair's image).
+__device__ __forceinline__ void ew_slot32(const EwLaunch& E, const double4& oI, int J, int j,
+                                          const float4* __restrict__ v32, float4& a, float2& b) {
+    const double4 oJ = E.tb_o[J];
+    const float4 u = E.posl[j];
+    const float4 v = v32[j];
+    a = make_float4(v.x, v.y, v.z, (float)((double)u.x + ew_tile_shift(oJ.x - oI.x, E.L[0], E.iL[0])));
+    b = make_float2((float)((double)u.y + ew_tile_shift(oJ.y - oI.y, E.L[1], E.iL[1])),
+                    (float)((double)u.z + ew_tile_shift(oJ.z - oI.z, E.L[2], E.iL[2])));
+}
+
+// FP32 list entries are stored in pairs: entries 2p and 2p+1 of a lane are adjacent (a 4-byte index
+// pair and a 16-byte coefficient pair), pair p of list `list` at this index of the pair arrays.
+__device__ __forceinline__ size_t ew_pair32(int list, int cap32, int p, int lane) {
+    return ((size_t)list * (cap32 >> 1) + p) * kTile + lane;
+}
+
 __device__ __forceinline__ unsigned* ew_cand_base(unsigned* c, int list, int cap, int lane) {
     return c + (size_t)list * cap * kTile + lane * 8;   // candidate n at [(n / 8) 512 + n % 8]
 }
 __device__ __forceinline__ int ew_cand_slot(const int* nb, unsigned c, int w) {
     const int loc = (int)(c & 0xffffu);
-    return nb[loc >> 3] * kTile + w + kEwWaves * (loc & 7);
+    return nb[loc >> 3] * kTile + ew_wave_slot(w, loc & 7);
 }
 
 template <bool MIXED>
@@ -889,16 +847,20 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const float4 pif = ew_wrap32(pi, E.L, iL0, iL1, iL2);
     unsigned* cf = ew_cand_base(E.cand32, list, E.cap32, lane);   // FP32 stream (mixed precision)
     unsigned* cn = ew_cand_base(E.cand64, list, E.cap64, lane);   // FP64 stream
+    const double4 oI = E.tb_o[I];
+    const double rc = sqrt(E.rc2);
     int nf = 0, nn = 0;
     // ---- phase A
     for (int t0 = 0; t0 < cnt; t0 += 8) {
         const int g8 = lane >> 3, m8 = lane & 7;
         int Jl = 0;
         double4 pl = make_double4(0.0, 0.0, 0.0, 0.0);
-        bool near_box = false;
+        bool near_box = false, img_ok = false;
         if (t0 + g8 < cnt) {
             Jl = nb[t0 + g8];
-            const int ks = w + kEwWaves * m8;
+            // the FP32 blocks need the tile pair's pairs to take the image of the tile origins
+            if (MIXED) img_ok = ew_tile_image_ok(E, oI, hI, E.tb_o[Jl], E.tb_h[Jl], rc);
+            const int ks = ew_wave_slot(w, m8);
             if (ks < E.tcnt[Jl]) {
                 pl = posa[Jl * kTile + ks];
                 float dx = (float)pl.x - cI.x, dy = (float)pl.y - cI.y, dz = (float)pl.z - cI.z;
@@ -913,6 +875,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         }
         const unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
         if (!visit) continue;
+        const unsigned long long frame_ok = __builtin_amdgcn_ballot_w64(img_ok);   // bit 8 g + m: tile t0 + g
         s_p[wl][lane] = pl;
         s_f[wl][lane] = ew_wrap32(pl, E.L, iL0, iL1, iL2);
         __builtin_amdgcn_wave_barrier();   // the slice is this wave's own: its LDS operations complete in order
@@ -929,8 +892,8 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
                 maskT |= ew_slots_of_wave(mm[2 * kTile + lane], w) << (8 * g);
                 break;
             }
-            if (J == I && ((lane - w) & (kEwWaves - 1)) == 0 && lane >= w) {
-                const unsigned long long self = 1ull << (8 * g + (lane - w) / kEwWaves);
+            if (J == I && (lane >> 3) == w) {
+                const unsigned long long self = 1ull << (8 * g + (lane & 7));
                 mask |= self;
                 maskT |= self;
             }
@@ -961,7 +924,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             const bool store = in && !((maskT >> q) & 1ull);
             if (sel || store) {
                 const unsigned c = (unsigned)(8 * t0 + q) | (sel ? kCandSel : 0u) | (store ? kCandStore : 0u);
-                if (MIXED && far) {
+                if (MIXED && far && ((frame_ok >> q) & 1ull)) {
                     if (nf < E.cap32) cf[(nf >> 3) * 512 + (nf & 7)] = c;
                     nf++;
                 } else {
@@ -1015,16 +978,16 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
                     sz += f * dz;
                 }
             }
-            // the block T_ij = C2 d d^T - C1 as (-C1, sqrt|C2| d), the sign of C2 with the index
+            // the block T_ij = C2 d d^T - C1: FP32 as (C1, C2) (the T v pass forms d), FP64 as
+            // (-C1, sqrt|C2| d) with the sign of C2 in the index
             const bool st = (c & kCandStore) != 0u;
-            const double g = st ? sqrt(fabs(C2)) : 0.0;
-            if (FP32) {
-                const size_t at = ((size_t)list * E.cap32 + n) * kTile + lane;
-                E.idx16[at] = st ? (unsigned short)((c & 0xffffu) | (C2 < 0.0 ? 0x8000u : 0u)) : (unsigned short)0;
-                E.t32[at] = st ? make_float4((float)-C1, (float)(g * dx), (float)(g * dy), (float)(g * dz))
-                               : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (FP32) {   // entries in pairs per lane (ew_pair32)
+                const size_t at = ew_pair32(list, E.cap32, n >> 1, lane) * 2 + (n & 1);
+                E.idx16[at] = st ? (unsigned short)(c & 0xffffu) : (unsigned short)0;
+                E.t32[at] = st ? make_float2((float)C1, (float)C2) : make_float2(0.f, 0.f);
                 s32 += st ? 1 : 0;
             } else {
+                const double g = st ? sqrt(fabs(C2)) : 0.0;
                 const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
                 E.idx64[at] = st ? (j0 | (C2 < 0.0 ? (int)0x80000000u : 0)) : i;
                 E.t64[at] = st ? make_double4(-C1, g * dx, g * dy, g * dz) : make_double4(0.0, 0.0, 0.0, 0.0);
@@ -1038,7 +1001,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             a0 = a1;
         }
     };
-    const int wd32 = MIXED ? min((mf + kEwSpmvU - 1) / kEwSpmvU * kEwSpmvU, E.cap32) : 0;
+    const int wd32 = MIXED ? min((mf + 1) & ~1, E.cap32) : 0;   // whole entry pairs
     const int wd64 = min(mn, E.cap64);
     if (MIXED) stream(std::true_type(), cf, min(nf, E.cap32), wd32, E.cap32);
     stream(std::false_type(), cn, min(nn, E.cap64), wd64, E.cap64);
@@ -1197,114 +1160,176 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
 
 typedef float ew_f4 __attribute__((ext_vector_type(4)));
 
-// FP32 lists of one wave: the sum over its entries of T_ij v_j, v_j from the wave's LDS copy
-// (LDS) or gathered through the neighbour-tile list.  Software pipeline over steps of kEwSpmvU
-// entries, three register sets by step mod 3: while step s is summed, the indices and blocks of
-// steps s+1 and s+2 are in flight (the memory counter completes in order, so waiting for step s
-// never waits for a later step).  Entries past the end of the list are read at the last entry
-// (cached) and added with weight zero: no branch in the loop.
-template <bool LDS>
-__device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, int lane, int n32,
-                                             const float4* __restrict__ s_v, const float4* __restrict__ v32,
-                                             const int* __restrict__ nb, float& fx, float& fy, float& fz) {
-    const int ns = n32 / kEwSpmvU;
-    const unsigned short* ip = E.idx16 + (size_t)row * E.cap32 * kTile + lane;
-    const ew_f4* tp = reinterpret_cast<const ew_f4*>(E.t32 + (size_t)row * E.cap32 * kTile + lane);
-    const int last = n32 - 1;
-    unsigned short j[3][kEwSpmvU];
+// FP32 entries of one list: the sum of T_ij v_j = C2 d (d . v_j) - C1 v_j with d = u_i - y_j and the
+// slot data (v_j, y_j) from the list's LDS copy (LDS) or formed from the neighbour-tile list.
+// Software pipeline over steps of kEwSpmvU entry pairs, three register sets by step mod 3: while step
+// s is summed, the indices and coefficients of steps s+1 and s+2 are in flight (the memory counter
+// completes in order, so waiting for step s never waits for a later step); the first two steps are
+// issued by the caller before the slot data is staged.  Entries past the end of the list are read at
+// the last pair (cached); the last step adds them with weight zero.
+struct EwSpmvPipe {
+    unsigned j[3][kEwSpmvU];
     ew_f4 t[3][kEwSpmvU];
-    auto load = [&](int set, int step) {
+};
+
+__device__ __forceinline__ void ew_spmv_load(const EwLaunch& E, int row, int lane, int n32, EwSpmvPipe& P, int set,
+                                             int step) {
+    const unsigned* ip = reinterpret_cast<const unsigned*>(E.idx16) + ew_pair32(row, E.cap32, 0, lane);
+    const ew_f4* tp = reinterpret_cast<const ew_f4*>(E.t32) + ew_pair32(row, E.cap32, 0, lane);
+    const int last = ((n32 + 1) >> 1) - 1;
 #pragma unroll
-        for (int u = 0; u < kEwSpmvU; u++) {
-            const size_t at = (size_t)min(step * kEwSpmvU + u, last) * kTile;
-            j[set][u] = __builtin_nontemporal_load(ip + at);
-            t[set][u] = __builtin_nontemporal_load(tp + at);
+    for (int u = 0; u < kEwSpmvU; u++) {
+        const size_t at = (size_t)min(step * kEwSpmvU + u, last) * kTile;
+        P.j[set][u] = __builtin_nontemporal_load(ip + at);
+        P.t[set][u] = __builtin_nontemporal_load(tp + at);
+    }
+}
+
+template <bool LDS>
+__device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, int lane, int n32, EwSpmvPipe& P,
+                                             const float4* __restrict__ s_a, const float2* __restrict__ s_b,
+                                             const float4* __restrict__ v32, const int* __restrict__ nb,
+                                             const double4& oI, const float4& ui, float& fx, float& fy,
+                                             float& fz) {
+    const int ns = ((n32 + 1) / 2 + kEwSpmvU - 1) / kEwSpmvU;
+    auto term = [&](int loc, float c1, float c2) {
+        float4 a;
+        float2 b;
+        if (LDS) {
+            a = s_a[loc];
+            b = s_b[loc];
+        } else {
+            const int J = nb[loc >> 3];
+            ew_slot32(E, oI, J, J * kTile + ew_wave_slot(w, loc & 7), v32, a, b);
+        }
+        const float dx = ui.x - a.w, dy = ui.y - b.x, dz = ui.z - b.y;
+        const float p = c2 * fmaf(dz, a.z, fmaf(dy, a.y, dx * a.x));
+        fx = fmaf(-c1, a.x, fmaf(p, dx, fx));
+        fy = fmaf(-c1, a.y, fmaf(p, dy, fy));
+        fz = fmaf(-c1, a.z, fmaf(p, dz, fz));
+    };
+    auto sum = [&](int set, int step) {
+        if ((step + 1) * kEwSpmvU * 2 <= n32) {   // a whole step inside the list (uniform)
+#pragma unroll
+            for (int u = 0; u < kEwSpmvU; u++) {
+                const ew_f4 c = P.t[set][u];
+                term((int)(P.j[set][u] & 0xffffu), c.x, c.y);
+                term((int)(P.j[set][u] >> 16), c.z, c.w);
+            }
+        } else {
+#pragma unroll
+            for (int u = 0; u < kEwSpmvU; u++) {
+                const int e = 2 * (step * kEwSpmvU + u);   // first entry of the pair
+                const float on0 = e < n32 ? 1.f : 0.f, on1 = e + 1 < n32 ? 1.f : 0.f;
+                const ew_f4 c = P.t[set][u];
+                term((int)(P.j[set][u] & 0xffffu), on0 * c.x, on0 * c.y);
+                term((int)(P.j[set][u] >> 16), on1 * c.z, on1 * c.w);
+            }
         }
     };
-    auto sum = [&](int set, float on) {
-#pragma unroll
-        for (int u = 0; u < kEwSpmvU; u++) {
-            const int loc = j[set][u] & 0x7fff;
-            const float4 b = LDS ? s_v[loc] : v32[nb[loc >> 3] * kTile + w + kEwWaves * (loc & 7)];
-            const ew_f4 a = t[set][u];
-            float p = a.y * b.x + a.z * b.y + a.w * b.z;
-            p = (j[set][u] & 0x8000) ? -p : p;
-            const float c = on * a.x;
-            p *= on;
-            fx += c * b.x + p * a.y;
-            fy += c * b.y + p * a.z;
-            fz += c * b.z + p * a.w;
-        }
-    };
-    load(0, 0);
-    load(1, 1);
     for (int s0 = 0; s0 < ns; s0 += 3) {
-        load(2, s0 + 2);
+        ew_spmv_load(E, row, lane, n32, P, 2, s0 + 2);
         __builtin_amdgcn_sched_barrier(0);
-        sum(0, 1.f);
+        sum(0, s0);
         __builtin_amdgcn_sched_barrier(0);
-        load(0, s0 + 3);
+        ew_spmv_load(E, row, lane, n32, P, 0, s0 + 3);
         __builtin_amdgcn_sched_barrier(0);
-        sum(1, s0 + 1 < ns ? 1.f : 0.f);
+        if (s0 + 1 < ns) sum(1, s0 + 1);
         __builtin_amdgcn_sched_barrier(0);
-        load(1, s0 + 4);
+        ew_spmv_load(E, row, lane, n32, P, 1, s0 + 4);
         __builtin_amdgcn_sched_barrier(0);
-        sum(2, s0 + 2 < ns ? 1.f : 0.f);
+        if (s0 + 2 < ns) sum(2, s0 + 2);
         __builtin_amdgcn_sched_barrier(0);
     }
 }
 
 // T v of the real-space pairs from the stored blocks: one wave per list (tile row I, wave w), so that
-// the nblk * kEwWaves waves spread evenly over the CUs.  The FP32 lists read v_j from an LDS copy
-// of the wave's slots of the row's neighbour tiles (E.lds_tiles of them, 16 bytes each: staged
-// once, then every gather is an LDS read); the FP64 lists gather v_j from HBM (L2).  Each wave
-// writes its partial sums; the epilogue adds the kEwWaves partials of a slot in list order.  Memory
-// bound: 18 bytes per FP32 entry and 36 per FP64 entry stream from HBM once per pass.
+// the nblk * kEwWaves waves spread evenly over the CUs.  The FP32 lists read their slot data from an
+// LDS copy of the list's slots of the row's neighbour tiles (E.lds_tiles of them, kEwSlotLds bytes
+// each: v_j and y_j, staged once, then every gather is an LDS read); their first entries are
+// requested before the copy is staged.  The FP64 lists gather v_j from HBM (L2), four entries at a
+// time (their index and block loads, then the four gathers).  Each wave writes its partial sums; the
+// epilogue adds the kEwWaves partials of a slot in list order.  10 bytes per FP32 entry and 36 per
+// FP64 entry stream from HBM once per pass.
 template <bool MIXED>
 __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __restrict__ vec,
                                                 const float4* __restrict__ v32, const double* __restrict__ scalars,
                                                 double4* __restrict__ part) {
-    extern __shared__ float4 s_v[];   // [E.lds_tiles * 8]
+    extern __shared__ float4 s_a[];   // [E.lds_tiles * 8] (v, y_x), then float2 [E.lds_tiles * 8] (y_y, y_z)
     if (scalars && scalars[SC_DONE] != 0.0) return;
     const int row = blockIdx.x, lane = threadIdx.x;   // row = I kEwWaves + w
     const int I = row / kEwWaves, w = row % kEwWaves;
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    if (MIXED) {
-        const int n32 = E.w32[row];   // a multiple of kEwSpmvU
-        const int* nb = E.nbr + (size_t)I * E.nblk;
-        float fx = 0.f, fy = 0.f, fz = 0.f;
-        if (n32 > 0) {
-            if (E.lds_tiles > 0) {
-                // entry 8 t + m: slot w + kEwWaves m of neighbour tile t (the wave's own LDS: its
-                // operations complete in order)
-                const int cnt = E.nbr_cnt[I];
-                for (int e = lane; e < 8 * cnt; e += kTile)
-                    s_v[e] = v32[nb[e >> 3] * kTile + w + kEwWaves * (e & 7)];
-                __builtin_amdgcn_wave_barrier();
-                ew_spmv_fp32<true>(E, row, w, lane, n32, s_v, v32, nb, fx, fy, fz);
-            } else {
-                ew_spmv_fp32<false>(E, row, w, lane, n32, s_v, v32, nb, fx, fy, fz);
-            }
-        }
-        ax = fx;
-        ay = fy;
-        az = fz;
+    const int* nb = E.nbr + (size_t)I * E.nblk;
+    const int n32 = MIXED ? E.w32[row] : 0;
+    const bool lds = MIXED && n32 > 0 && E.lds_tiles > 0;
+    float2* s_b = reinterpret_cast<float2*>(s_a + (size_t)E.lds_tiles * 8);
+    EwSpmvPipe P;
+    double4 oI = make_double4(0.0, 0.0, 0.0, 0.0);
+    float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (MIXED && n32 > 0) {
+        ew_spmv_load(E, row, lane, n32, P, 0, 0);
+        ew_spmv_load(E, row, lane, n32, P, 1, 1);
+        oI = E.tb_o[I];
+        ui = E.posl[I * kTile + lane];
     }
+    if (lds) {
+        // entry 8 t + m: slot 8w + m of neighbour tile t (the wave's own LDS: its operations
+        // complete in order); four slots per lane at a time, their loads issued together
+        const int ne = 8 * E.nbr_cnt[I];
+        for (int e0 = lane; e0 < ne; e0 += 4 * kTile) {
+            int J[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) J[u] = nb[min(e0 + u * kTile, ne - 1) >> 3];
+            float4 a[4];
+            float2 b[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = min(e0 + u * kTile, ne - 1);
+                ew_slot32(E, oI, J[u], J[u] * kTile + ew_wave_slot(w, e & 7), v32, a[u], b[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (e0 + u * kTile < ne) {
+                    s_a[e0 + u * kTile] = a[u];
+                    s_b[e0 + u * kTile] = b[u];
+                }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    double ax = 0.0, ay = 0.0, az = 0.0;
     {
         const int n64 = E.w64[row];
         const size_t base = (size_t)row * E.cap64 * kTile + lane;
-#pragma unroll 2
-        for (int n = 0; n < n64; n++) {
-            const size_t at = base + (size_t)n * kTile;
-            const int j = E.idx64[at];
-            const double4 t = E.t64[at];
-            const double4 v = vec[j & 0x7fffffff];
-            double p = t.y * v.x + t.z * v.y + t.w * v.z;
-            p = j < 0 ? -p : p;
-            ax += t.x * v.x + p * t.y;
-            ay += t.x * v.y + p * t.z;
-            az += t.x * v.z + p * t.w;
+        for (int n0 = 0; n0 < n64; n0 += 4) {
+            int j[4];
+            double4 t[4], v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const size_t at = base + (size_t)min(n0 + u, n64 - 1) * kTile;
+                j[u] = E.idx64[at];
+                t[u] = E.t64[at];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++) v[u] = vec[j[u] & 0x7fffffff];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {   // entries past the end: the last one with weight zero
+                const double on = n0 + u < n64 ? 1.0 : 0.0;
+                double p = t[u].y * v[u].x + t[u].z * v[u].y + t[u].w * v[u].z;
+                p = j[u] < 0 ? -on * p : on * p;
+                const double c = on * t[u].x;
+                ax += c * v[u].x + p * t[u].y;
+                ay += c * v[u].y + p * t[u].z;
+                az += c * v[u].z + p * t[u].w;
+            }
         }
+    }
+    if (MIXED && n32 > 0) {
+        float fx = 0.f, fy = 0.f, fz = 0.f;
+        if (lds) ew_spmv_fp32<true>(E, row, w, lane, n32, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
+        else ew_spmv_fp32<false>(E, row, w, lane, n32, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
+        ax += fx;
+        ay += fy;
+        az += fz;
     }
     part[(size_t)row * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
@@ -1616,7 +1641,7 @@ void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars) {
     const dim3 lists(E.nblk * kEwWaves);
     if (E.pairs32)
-        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), (size_t)E.lds_tiles * 8 * sizeof(float4), s, E, vec, v32,
+        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), (size_t)E.lds_tiles * 8 * kEwSlotLds, s, E, vec, v32,
                            scalars, E.rsw);
     else
         hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);

@@ -163,8 +163,9 @@ constexpr int kEwParts = 2;   // workgroups per tile row of the T v and force pa
 constexpr int ew_parts(int mode) { return mode == 0 ? 1 : kEwParts; }   // mode: EwMode (kEwE0 is 0)
 constexpr int kEwE0Waves = 4;   // waves per workgroup of the list-building E0 pass
 constexpr int kEwE0Parts = kEwWaves / kEwE0Waves;   // its workgroups per tile row
-constexpr int kEwSpmvU = 4;   // entries per step of the stored T v pass (FP32 lists)
+constexpr int kEwSpmvU = 8;   // entry pairs per step of the stored T v pass (FP32 lists)
 constexpr int kEwMaxLocalTiles = 4096;   // neighbour tiles an FP32 list index can address
+constexpr int kEwSlotLds = 24;   // LDS bytes per staged slot of the FP32 T v pass: (v, y_x), (y_y, y_z)
 // FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
 struct EwAxis {
     int n, nst;
@@ -180,6 +181,7 @@ struct EwLaunch {
     int npad, nblk;
     const int* tcnt;
     double L[3];            // box diagonal
+    double iL[3];           // 1 / L
     double rc2;             // cutoff squared (FP64 neighbour criterion)
     double r64sq;           // mixed precision: pairs closer than this (squared) stay in FP64
     bool pairs32;           // mixed precision: real-space pairs beyond r64 in FP32 (T v and forces)
@@ -208,17 +210,21 @@ struct EwLaunch {
     double4* rsw;           // [nblk kEwWaves 64] partial T v sums of the stored lists
     // Stored real-space blocks of T (built by the E0 pass, applied by every T v pass): per tile row I
     // and wave w a list of (j, T_ij) for each of the row's 64 slots, sliced ELL: entry n of slot
-    // lane at ((I kEwWaves + w) cap + n) 64 + lane.  T_ij = C2 d d^T - C1 1 is held as
-    // (-C1, u) with u = sqrt(|C2|) d and the sign of C2 in the index's top bit, so that
-    // T_ij v = -C1 v + sign(C2) u (u . v).  FP64 lists (every pair in FP64 mode, the pairs closer
-    // than r64 in mixed precision) index j itself (int32, sign in bit 31): 4 + 32 bytes per entry.
-    // FP32 lists (the other pairs in mixed precision) index j by its place among the wave's slots
-    // of the row's neighbour tiles, 8 t + m for slot w + kEwWaves m of tile nbr[I][t] (uint16, sign
-    // in bit 15; rows with at most 4096 neighbour tiles): 2 + 16 bytes per entry; their widths are
-    // a multiple of kEwSpmvU.
+    // lane at ((I kEwWaves + w) cap + n) 64 + lane, T_ij = C2 d d^T - C1 1.
+    // FP64 lists (every pair in FP64 mode, the pairs closer than r64 in mixed precision, and in
+    // mixed precision the pairs of tile pairs whose image is not that of their tiles' centres) hold
+    // (-C1, u) with u = sqrt(|C2|) d and index j itself with the sign of C2 in bit 31, so that
+    // T_ij v = -C1 v + sign(C2) u (u . v): 4 + 32 bytes per entry.
+    // FP32 lists (the other pairs in mixed precision) hold (C1, C2) only and index j by its place
+    // among the wave's slots of the row's neighbour tiles, 8 t + m for slot 8w + m of tile
+    // nbr[I][t] (uint16; rows with at most 4096 neighbour tiles): 2 + 8 bytes per entry.  The T v
+    // pass forms d = u_i - y_j in FP32 from the slots' offsets: u_i from tile I's origin (posl),
+    // y_j = u_j + D_IJ in FP64 rounded once, D_IJ the minimum image of the tile origins, which every
+    // pair of the tile pair within the cutoff takes (ew_tile_image_ok).  FP32 entries are stored in
+    // pairs per lane (ew_pair32) and their list widths are even.
     int cap32, cap64;
     unsigned short* idx16;
-    float4* t32;
+    float2* t32;
     int* idx64;
     double4* t64;
     int* w32;               // [nblk kEwWaves] list widths
@@ -226,6 +232,8 @@ struct EwLaunch {
     int* need;              // [7] largest width (FP32, FP64), entries (FP32, FP64), summed widths
                             // (FP32, FP64), largest neighbour-tile count
     int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
+    const double4* tb_o;    // [nblk] tile origins (FP64)
+    const float4* posl;     // [npad] each slot's FP32 offset from its tile's origin
     int e0_parts = 1;       // row parts whose sums the E0 epilogue adds (kEwE0Parts after the list build)
     unsigned* cand32;       // [nblk kEwWaves 64 cap32] the E0 pass's pair candidates with FP32 blocks,
     unsigned* cand64;       // [nblk kEwWaves 64 cap64] and with FP64 blocks, per lane in blocks of 8

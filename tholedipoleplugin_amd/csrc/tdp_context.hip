@@ -203,7 +203,7 @@ struct tdp_context {
     unsigned long long* d_qfix = nullptr;
     double2* d_cgrid = nullptr;
     double *d_phiq = nullptr, *d_phiv = nullptr;
-    int *d_nbr = nullptr, *d_nbr_cnt = nullptr;
+    int *d_nbr = nullptr, *d_nbr_cnt = nullptr, *d_nbr_exc = nullptr;
     int *d_sc_off = nullptr, *d_sc_j = nullptr;
     double2* d_sc_s = nullptr;
     size_t cap_sc = 0;
@@ -717,6 +717,7 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.tb_h = c->tb_h;
     E.nbr = c->d_nbr;
     E.nbr_cnt = c->d_nbr_cnt;
+    E.nbr_exc = c->d_nbr_exc;
     E.exc_start = c->d_exc_start;
     E.exc_J = c->d_exc_J;
     E.exc_mask = c->d_exc_mask;
@@ -1679,6 +1680,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     if (c->ewald) {
         const size_t ng = (size_t)c->ew_K[0] * c->ew_K[1] * c->ew_K[2];
         if ((rc = dalloc(c, &c->d_nbr, (size_t)nb * nb)) || (rc = dalloc(c, &c->d_nbr_cnt, nb)) ||
+            (rc = dalloc(c, &c->d_nbr_exc, (size_t)nb * nb)) ||
             (rc = dalloc(c, &c->d_qfix, ng)) || (rc = dalloc(c, &c->d_cgrid, ng)) || (rc = dalloc(c, &c->d_phiq, ng)) ||
             (rc = dalloc(c, &c->d_phiv, ng)) || (rc = dalloc(c, &c->d_G64, ng)) || (rc = dalloc(c, &c->d_G32, ng)) ||
             (rc = dalloc(c, &c->d_sc_off, (size_t)npad + 1)) || (rc = dalloc(c, &c->d_epart, npad)) ||

@@ -133,7 +133,9 @@ __device__ __forceinline__ float2 twiddle<float>(const EwAxis& A, int q, bool in
 
 // ------------------------------------------------------------------ neighbour tiles
 // Tile row I lists every tile J whose bounding box (minimum image of the centres, FP32 with a
-// margin) comes within the cutoff of I's: a superset of the tiles holding a pair within the cutoff.
+// margin) comes within the cutoff of I's: a superset of the tiles holding a pair within the cutoff;
+// beside each, the tile pair's entry in the exclusion tiles (or -1), so that the pair passes need no
+// search.
 __global__ __launch_bounds__(64) void k_ew_neighbors(EwLaunch E) {
     const int I = blockIdx.x, lane = threadIdx.x;
     const float4 cI = E.tb_c[I], hI = E.tb_h[I];
@@ -155,7 +157,13 @@ __global__ __launch_bounds__(64) void k_ew_neighbors(EwLaunch E) {
         }
         const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
         const int before = __popcll(m & ((1ull << lane) - 1ull));
-        if (ok) E.nbr[(size_t)I * E.nblk + count + before] = J;
+        if (ok) {
+            int ex = -1;   // the tile pair's entry in the exclusion tiles of row I, if any
+            for (int e = E.exc_start[I]; e < E.exc_start[I + 1]; e++)
+                if (E.exc_J[e] == J) ex = e;
+            E.nbr[(size_t)I * E.nblk + count + before] = J;
+            E.nbr_exc[(size_t)I * E.nblk + count + before] = ex;
+        }
         count += __popcll(m);
     }
     if (lane == 0) E.nbr_cnt[I] = count;
@@ -601,17 +609,18 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
     // this part's neighbour tiles: t = NP l + h of the row's list, l < cnt
     const int cnt = (E.nbr_cnt[I] - h + NP - 1) / NP;
     const int* nb = E.nbr + (size_t)I * E.nblk + h;
-    const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
+    const int* nbx = E.nbr_exc + (size_t)I * E.nblk + h;
     for (int t0 = 0; t0 < cnt; t0 += 8) {
         // a group of up to 8 neighbour tiles: lane 8g + m holds slot k = 8w + m of tile
         // t0 + g and tests it against row I's bounding box (cutoff plus margin)
         const int g8 = lane >> 3, m8 = lane & 7;
-        int Jl = 0;
+        int Jl = 0, exl = -1;
         double4 pl = make_double4(0.0, 0.0, 0.0, 0.0), vl = make_double4(0.0, 0.0, 0.0, 0.0);
         double ial = 0.0;
         bool near_box = false;
         if (t0 + g8 < cnt) {
             Jl = nb[NP * (t0 + g8)];
+            exl = nbx[NP * (t0 + g8)];
             const int ks = ew_wave_slot(w, m8);
             if (ks < E.tcnt[Jl]) {
                 const int j = Jl * kTile + ks;
@@ -642,8 +651,8 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
         for (int g = 0; g < 8; g++) {
             if (!((visit >> (8 * g)) & 0xffull)) continue;
             const int J = __builtin_amdgcn_readlane(Jl, 8 * g);
-            for (int e = eS; e < eE; e++) {
-                if (E.exc_J[e] != J) continue;
+            const int e = __builtin_amdgcn_readlane(exl, 8 * g);
+            if (e >= 0) {
                 const unsigned long long* mm = E.exc_mask + (size_t)e * 3 * kTile;
                 unsigned long long mk;
                 if (MODE == kEwE0) {
@@ -654,7 +663,6 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
                     mk = mm[lane] | mm[kTile + lane] | mm[2 * kTile + lane];
                 }
                 mask |= ew_slots_of_wave(mk, w) << (8 * g);
-                break;
             }
             if (J == I && (lane >> 3) == w) mask |= 1ull << (8 * g + (lane & 7));
         }
@@ -805,6 +813,14 @@ __device__ __forceinline__ size_t ew_pair32(int list, int cap32, int p, int lane
 __device__ __forceinline__ unsigned* ew_cand_base(unsigned* c, int list, int cap, int lane) {
     return c + (size_t)list * cap * kTile + lane * 8;   // candidate n at [(n / 8) 512 + n % 8]
 }
+// Candidate n of a lane's stream of nl entries; past the end its last entry, or entry 0 (which phase A
+// writes for a lane without candidates).
+__device__ __forceinline__ unsigned ew_cand_at(const unsigned* cb, int n, int nl) {
+    const int k = min(n, max(nl, 1) - 1);
+    return cb[(k >> 3) * 512 + (k & 7)];
+}
+// Its neighbour tile (index into the row's list).
+__device__ __forceinline__ int ew_cand_tile(unsigned c) { return (int)((c & 0xffffu) >> 3); }
 __device__ __forceinline__ int ew_cand_slot(const int* nb, unsigned c, int w) {
     const int loc = (int)(c & 0xffffu);
     return nb[loc >> 3] * kTile + ew_wave_slot(w, loc & 7);
@@ -831,7 +847,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
     const int cnt = E.nbr_cnt[I];
     const int* nb = E.nbr + (size_t)I * E.nblk;
-    const int eS = E.exc_start[I], eE = E.exc_start[I + 1];
+    const int* nbx = E.nbr_exc + (size_t)I * E.nblk;
     const int list = I * kEwWaves + w;
     // FP32 screen of the pair criterion on coordinates wrapped into the box.  Each separation
     // component is off by at most 4 u (u = max L 2^-23: the wraps' and the box length's roundings),
@@ -850,19 +866,44 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const double4 oI = E.tb_o[I];
     const double rc = sqrt(E.rc2);
     int nf = 0, nn = 0;
+    const float4 ui = E.posl[i];   // offset from tile I's origin (FP32)
     // ---- phase A
+    // Group data of lane 8g + m: slot 8w + m of neighbour tile t0 + g (read unconditionally at a
+    // valid tile and slot; a lane past the row's count or the tile's particles is not visited).  The
+    // next group's neighbour tile and data are requested while the current group is screened.
+    struct Grp {
+        int J, ex, tc;
+        double4 oJ, pl;
+        float4 hJ, u;
+    };
+    const int g8 = lane >> 3, m8 = lane & 7, ks = ew_wave_slot(w, m8);
+    auto load_grp = [&](int J, int ex, Grp& G) {
+        G.J = J;
+        G.ex = ex;
+        G.oJ = E.tb_o[J];
+        G.hJ = E.tb_h[J];
+        G.tc = E.tcnt[J];
+        G.pl = posa[J * kTile + ks];
+        G.u = E.posl[J * kTile + ks];
+    };
+    Grp cur;
+    load_grp(nb[min(g8, cnt - 1)], nbx[min(g8, cnt - 1)], cur);
+    int Jnext = nb[min(8 + g8, cnt - 1)], Xnext = nbx[min(8 + g8, cnt - 1)];
     for (int t0 = 0; t0 < cnt; t0 += 8) {
-        const int g8 = lane >> 3, m8 = lane & 7;
-        int Jl = 0;
-        double4 pl = make_double4(0.0, 0.0, 0.0, 0.0);
+        Grp nxt;
+        load_grp(Jnext, Xnext, nxt);
+        Jnext = nb[min(t0 + 16 + g8, cnt - 1)];
+        Xnext = nbx[min(t0 + 16 + g8, cnt - 1)];
+        const int Jl = cur.J, exl = cur.ex;
+        const double4 pl = cur.pl, oJ = cur.oJ;
+        float4 fl = make_float4(0.f, 0.f, 0.f, 0.f);
         bool near_box = false, img_ok = false;
         if (t0 + g8 < cnt) {
-            Jl = nb[t0 + g8];
-            // the FP32 blocks need the tile pair's pairs to take the image of the tile origins
-            if (MIXED) img_ok = ew_tile_image_ok(E, oI, hI, E.tb_o[Jl], E.tb_h[Jl], rc);
-            const int ks = ew_wave_slot(w, m8);
-            if (ks < E.tcnt[Jl]) {
-                pl = posa[Jl * kTile + ks];
+            // the pairs of tile pairs that take the image of the tile origins are screened on FP32
+            // offsets from tile I's origin (and only they may have FP32 blocks); the others on
+            // coordinates wrapped into the box
+            img_ok = ew_tile_image_ok(E, oI, hI, oJ, cur.hJ, rc);
+            if (ks < cur.tc) {
                 float dx = (float)pl.x - cI.x, dy = (float)pl.y - cI.y, dz = (float)pl.z - cI.z;
                 dx -= bxf * rintf(dx / bxf);
                 dy -= byf * rintf(dy / byf);
@@ -871,13 +912,22 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
                 dy = fmaxf(0.f, fabsf(dy) - hI.y);
                 dz = fmaxf(0.f, fabsf(dz) - hI.z);
                 near_box = dx * dx + dy * dy + dz * dz < E.cut2_tiles;
+                if (img_ok) {
+                    const float4 u = cur.u;
+                    fl = make_float4((float)((double)u.x + ew_tile_shift(oJ.x - oI.x, E.L[0], E.iL[0])),
+                                     (float)((double)u.y + ew_tile_shift(oJ.y - oI.y, E.L[1], E.iL[1])),
+                                     (float)((double)u.z + ew_tile_shift(oJ.z - oI.z, E.L[2], E.iL[2])), 0.f);
+                } else {
+                    fl = ew_wrap32(pl, E.L, iL0, iL1, iL2);
+                }
             }
         }
+        cur = nxt;
         const unsigned long long visit = __builtin_amdgcn_ballot_w64(near_box);
         if (!visit) continue;
         const unsigned long long frame_ok = __builtin_amdgcn_ballot_w64(img_ok);   // bit 8 g + m: tile t0 + g
         s_p[wl][lane] = pl;
-        s_f[wl][lane] = ew_wrap32(pl, E.L, iL0, iL1, iL2);
+        s_f[wl][lane] = fl;
         __builtin_amdgcn_wave_barrier();   // the slice is this wave's own: its LDS operations complete in order
         // masks of the group (bit 8 g + m): pairs scaled in the field (m0, mhalf: the epilogue
         // evaluates them) and pairs without a block of T (i0)
@@ -885,12 +935,11 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         for (int g = 0; g < 8; g++) {
             if (!((visit >> (8 * g)) & 0xffull)) continue;
             const int J = __builtin_amdgcn_readlane(Jl, 8 * g);
-            for (int e = eS; e < eE; e++) {
-                if (E.exc_J[e] != J) continue;
+            const int e = __builtin_amdgcn_readlane(exl, 8 * g);
+            if (e >= 0) {
                 const unsigned long long* mm = E.exc_mask + (size_t)e * 3 * kTile;
                 mask |= ew_slots_of_wave(mm[lane] | mm[kTile + lane], w) << (8 * g);
                 maskT |= ew_slots_of_wave(mm[2 * kTile + lane], w) << (8 * g);
-                break;
             }
             if (J == I && (lane >> 3) == w) {
                 const unsigned long long self = 1ull << (8 * g + (lane & 7));
@@ -899,43 +948,75 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             }
         }
         if (!ivalid) mask = maskT = ~0ull;
-        unsigned long long v = visit;
-        while (v) {
-            const int q = __builtin_ctzll(v);
-            v &= v - 1ull;
-            const float4 pjf = s_f[wl][q];
-            float fx = pif.x - pjf.x, fy = pif.y - pjf.y, fz = pif.z - pjf.z;
-            fx -= Lx * rintf(fx * iLx);
-            fy -= Ly * rintf(fy * iLy);
-            fz -= Lz * rintf(fz * iLz);
-            const float r2f = fx * fx + fy * fy + fz * fz;
-            bool in = r2f < rc2f;
-            bool far = r2f >= r64f;
-            if (fabsf(r2f - rc2f) <= band_c2 || (MIXED && fabsf(r2f - r64f) <= band_64)) {   // the FP64 rule (rare)
-                const double4 pj = s_p[wl][q];
-                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
-                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
-                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
-                const double r2x = ew_r2(dx, dy, dz);
-                in = r2x < E.rc2;
-                far = r2x >= E.r64sq;
-            }
-            const bool sel = in && !((mask >> q) & 1ull);
-            const bool store = in && !((maskT >> q) & 1ull);
-            if (sel || store) {
-                const unsigned c = (unsigned)(8 * t0 + q) | (sel ? kCandSel : 0u) | (store ? kCandStore : 0u);
-                if (MIXED && far && ((frame_ok >> q) & 1ull)) {
-                    if (nf < E.cap32) cf[(nf >> 3) * 512 + (nf & 7)] = c;
-                    nf++;
+        // the pair criterion for every visited slot, as per-lane bit masks of the group (inside the
+        // cutoff, beyond r64), one 32-slot half at a time
+        unsigned long long inm = 0ull, farm = 0ull;
+#pragma unroll
+        for (int half = 0; half < 2; half++) {
+            unsigned v = (unsigned)(visit >> (32 * half));
+            unsigned in32 = 0u, far32 = 0u;
+            while (v) {
+                const int b = __builtin_ctz(v);
+                v &= v - 1u;
+                const int q = 32 * half + b;
+                const float4 pjf = s_f[wl][q];
+                float fx, fy, fz;
+                if ((frame_ok >> q) & 1ull) {   // uniform
+                    fx = ui.x - pjf.x;
+                    fy = ui.y - pjf.y;
+                    fz = ui.z - pjf.z;
                 } else {
-                    if (nn < E.cap64) cn[(nn >> 3) * 512 + (nn & 7)] = c;
-                    nn++;
+                    fx = pif.x - pjf.x;
+                    fy = pif.y - pjf.y;
+                    fz = pif.z - pjf.z;
+                    fx -= Lx * rintf(fx * iLx);
+                    fy -= Ly * rintf(fy * iLy);
+                    fz -= Lz * rintf(fz * iLz);
                 }
+                const float r2f = fx * fx + fy * fy + fz * fz;
+                bool in = r2f < rc2f;
+                bool far = r2f >= r64f;
+                if (fabsf(r2f - rc2f) <= band_c2 || (MIXED && fabsf(r2f - r64f) <= band_64)) {   // the FP64 rule (rare)
+                    const double4 pj = s_p[wl][q];
+                    const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                    const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                    const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                    const double r2x = ew_r2(dx, dy, dz);
+                    in = r2x < E.rc2;
+                    far = r2x >= E.r64sq;
+                }
+                in32 |= in ? 1u << b : 0u;
+                far32 |= far ? 1u << b : 0u;
             }
+            inm |= (unsigned long long)in32 << (32 * half);
+            farm |= (unsigned long long)far32 << (32 * half);
         }
         __builtin_amdgcn_wave_barrier();   // every lane has read the slice before the next group is staged
+        // the group's candidates (inside, in the field sum or stored) into the lane's streams in
+        // ascending slot order: FP32 blocks for pairs beyond r64 of tile pairs with the origins'
+        // image (mixed precision), FP64 blocks for the others
+        const unsigned long long cand = inm & ~(mask & maskT);
+        const unsigned long long c32 = MIXED ? cand & farm & frame_ok : 0ull;
+        auto emit = [&](unsigned long long m, unsigned* cb, int& n, int cap) {
+            while (__builtin_amdgcn_ballot_w64(m != 0ull)) {
+                if (m) {
+                    const int q = __builtin_ctzll(m);
+                    m &= m - 1ull;
+                    const unsigned c = (unsigned)(8 * t0 + q) | (((mask >> q) & 1ull) ? 0u : kCandSel) |
+                                       (((maskT >> q) & 1ull) ? 0u : kCandStore);
+                    if (n < cap) cb[(n >> 3) * 512 + (n & 7)] = c;
+                    n++;
+                }
+            }
+        };
+        if (MIXED) emit(c32, cf, nf, E.cap32);
+        emit(cand & ~c32, cn, nn, E.cap64);
     }
     E.ccnt[(size_t)list * kTile + lane] = make_int2(min(nf, E.cap32), min(nn, E.cap64));
+    // a lane without candidates gets the entry 0 = slot 8w of the first neighbour tile, no flags: every
+    // read of a stream (ew_cand_at) is then of a written entry
+    if (MIXED && nf == 0) cf[0] = 0u;
+    if (nn == 0) cn[0] = 0u;
     // ---- phase B
     int mf = nf, mn = nn;
     for (int off = 32; off > 0; off >>= 1) {
@@ -944,21 +1025,20 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     }
     double sx = 0.0, sy = 0.0, sz = 0.0;
     int s32 = 0, s64 = 0;
-    // one stream: FP32 blocks (FP32 = true) or FP64 blocks; software pipeline: the slot data of
-    // candidate n+1 and candidate n+2 itself are in flight while pair n is evaluated
-    auto stream = [&](auto fp32_tag, const unsigned* cb, int nl, int wd, int cap) {
+    // one stream: FP32 blocks (FP32 = true) or FP64 blocks.  Software pipeline: while pair n is
+    // evaluated, the slot data of candidate n+1, the neighbour tile of candidate n+2 and candidate n+3
+    // itself are in flight, so that every wait is for a load issued an iteration earlier
+    auto stream = [&](auto fp32_tag, const unsigned* cb, int nl, int wd) {
         constexpr bool FP32 = decltype(fp32_tag)::value;
-        auto cand = [&](int n) -> unsigned {
-            const int k = min(n, cap - 1);
-            return cb[(k >> 3) * 512 + (k & 7)];
-        };
-        unsigned c0 = cand(0), c1 = cand(1);
-        int j0 = 0 < nl ? ew_cand_slot(nb, c0, w) : i;
+        unsigned c0 = ew_cand_at(cb, 0, nl), c1 = ew_cand_at(cb, 1, nl), c2 = ew_cand_at(cb, 2, nl);
+        int J1 = nb[ew_cand_tile(c1)];
+        int j0 = ew_cand_slot(nb, c0, w);
         double4 p0 = posa[j0];
         double q0 = charge[j0], a0 = inva[j0];
         for (int n = 0; n < wd; n++) {
-            const unsigned c2 = cand(n + 2);
-            const int j1 = n + 1 < nl ? ew_cand_slot(nb, c1, w) : i;
+            const unsigned c3 = ew_cand_at(cb, n + 3, nl);
+            const int J2 = nb[ew_cand_tile(c2)];
+            const int j1 = J1 * kTile + ew_wave_slot(w, c1 & 7);
             const double4 p1 = posa[j1];
             const double q1 = charge[j1], a1 = inva[j1];
             const unsigned c = n < nl ? c0 : 0u;
@@ -995,6 +1075,8 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             }
             c0 = c1;
             c1 = c2;
+            c2 = c3;
+            J1 = J2;
             j0 = j1;
             p0 = p1;
             q0 = q1;
@@ -1003,8 +1085,8 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     };
     const int wd32 = MIXED ? min((mf + 1) & ~1, E.cap32) : 0;   // whole entry pairs
     const int wd64 = min(mn, E.cap64);
-    if (MIXED) stream(std::true_type(), cf, min(nf, E.cap32), wd32, E.cap32);
-    stream(std::false_type(), cn, min(nn, E.cap64), wd64, E.cap64);
+    if (MIXED) stream(std::true_type(), cf, min(nf, E.cap32), wd32);
+    stream(std::false_type(), cn, min(nn, E.cap64), wd64);
     int s32w = s32, s64w = s64;
     for (int off = 32; off > 0; off >>= 1) {
         s32w += __shfl_xor(s32w, off, 64);
@@ -1064,24 +1146,26 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
     }
     double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0;
     constexpr unsigned kUnscaled = kCandSel | kCandStore;
+    // each stream: while pair n is evaluated, the slot data of candidate n+1, the neighbour tile of
+    // candidate n+2 and candidate n+3 itself are in flight (every wait is for a load issued an
+    // iteration earlier)
     if (MIXED) {
         const unsigned* cb = ew_cand_base(E.cand32, list, E.cap32, lane);
         const float alpha32 = (float)E.alpha, c1f = (float)E.c1, c2f = (float)E.c2, c3f = (float)E.c3;
-        auto cand = [&](int n) -> unsigned {
-            const int k = min(n, E.cap32 - 1);
-            return cb[(k >> 3) * 512 + (k & 7)];
-        };
-        unsigned c0 = cand(0), c1 = cand(1);
-        int j0 = 0 < nc.x ? ew_cand_slot(nb, c0, w) : i;
+        const int nl = nc.x;
+        unsigned c0 = ew_cand_at(cb, 0, nl), c1 = ew_cand_at(cb, 1, nl), c2 = ew_cand_at(cb, 2, nl);
+        int J1 = nb[ew_cand_tile(c1)];
+        const int j0 = ew_cand_slot(nb, c0, w);
         double4 p0 = posa[j0], v0 = vec[j0];
         double q0 = charge[j0], a0 = inva[j0];
         float fx = 0.f, fy = 0.f, fz = 0.f, fe = 0.f;
         for (int n = 0; n < mf; n++) {
-            const unsigned c2 = cand(n + 2);
-            const int j1 = n + 1 < nc.x ? ew_cand_slot(nb, c1, w) : i;
+            const unsigned c3 = ew_cand_at(cb, n + 3, nl);
+            const int J2 = nb[ew_cand_tile(c2)];
+            const int j1 = J1 * kTile + ew_wave_slot(w, c1 & 7);
             const double4 p1 = posa[j1], v1 = vec[j1];
             const double q1 = charge[j1], a1 = inva[j1];
-            if (n < nc.x && (c0 & kUnscaled) == kUnscaled) {
+            if (n < nl && (c0 & kUnscaled) == kUnscaled) {
                 const float gx = (float)ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
                 const float gy = (float)ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
                 const float gz = (float)ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
@@ -1101,6 +1185,8 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
             }
             c0 = c1;
             c1 = c2;
+            c2 = c3;
+            J1 = J2;
             p0 = p1;
             v0 = v1;
             q0 = q1;
@@ -1113,20 +1199,19 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
     }
     {
         const unsigned* cb = ew_cand_base(E.cand64, list, E.cap64, lane);
-        auto cand = [&](int n) -> unsigned {
-            const int k = min(n, E.cap64 - 1);
-            return cb[(k >> 3) * 512 + (k & 7)];
-        };
-        unsigned c0 = cand(0), c1 = cand(1);
-        int j0 = 0 < nc.y ? ew_cand_slot(nb, c0, w) : i;
+        const int nl = nc.y;
+        unsigned c0 = ew_cand_at(cb, 0, nl), c1 = ew_cand_at(cb, 1, nl), c2 = ew_cand_at(cb, 2, nl);
+        int J1 = nb[ew_cand_tile(c1)];
+        const int j0 = ew_cand_slot(nb, c0, w);
         double4 p0 = posa[j0], v0 = vec[j0];
         double q0 = charge[j0], a0 = inva[j0];
         for (int n = 0; n < mn; n++) {
-            const unsigned c2 = cand(n + 2);
-            const int j1 = n + 1 < nc.y ? ew_cand_slot(nb, c1, w) : i;
+            const unsigned c3 = ew_cand_at(cb, n + 3, nl);
+            const int J2 = nb[ew_cand_tile(c2)];
+            const int j1 = J1 * kTile + ew_wave_slot(w, c1 & 7);
             const double4 p1 = posa[j1], v1 = vec[j1];
             const double q1 = charge[j1], a1 = inva[j1];
-            if (n < nc.y && (c0 & kUnscaled) == kUnscaled) {
+            if (n < nl && (c0 & kUnscaled) == kUnscaled) {
                 const double dx = ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
                 const double dy = ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
                 const double dz = ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
@@ -1137,6 +1222,8 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
             }
             c0 = c1;
             c1 = c2;
+            c2 = c3;
+            J1 = J2;
             p0 = p1;
             v0 = v1;
             q0 = q1;

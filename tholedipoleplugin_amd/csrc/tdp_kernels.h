@@ -191,6 +191,7 @@ struct EwLaunch {
     const float4* tb_h;
     int* nbr;               // [nblk][nblk] neighbour tiles of each tile row (J ascending, J == I included)
     int* nbr_cnt;           // [nblk]
+    int* nbr_exc;           // [nblk][nblk] beside nbr: the tile pair's exclusion-tile entry, or -1
     const int* exc_start;
     const int* exc_J;
     const unsigned long long* exc_mask;

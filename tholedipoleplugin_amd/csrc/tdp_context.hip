@@ -114,6 +114,10 @@ struct tdp_context {
     std::vector<int> goff, gidx;
     // ---- device state
     hipStream_t stream = nullptr;
+    // Ewald option: the reciprocal-space chain of each T v runs on stream2 beside the real-space pass
+    // (fork and join by events, also inside captured graphs); both streams at the default priority
+    hipStream_t stream2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // Captured chunks of solver iterations (hipGraph), replayed while the launch parameters hold:
     // same chunk length, solver and precision (the pair kernel and the vectors it reads), same box,
     // same device tables (graph_gen counts table uploads).
@@ -874,14 +878,21 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
     if (c->ewald) {   // reciprocal potential of v, then the real-space pass with the gather in its epilogue
         const double box9[9] = {L.box[0], 0, 0, 0, L.box[1], 0, 0, 0, L.box[2]};
         const EwLaunch E = ew_launch(c, box9);
-        launch_ew_recip(c->stream, E, c->posa, nullptr, v64, E.phi_v, scalars);
-        // timed: the stored-block T v kernel (or the matrix-free pass with its epilogue)
-        if (timed) hipEventRecord(e0, c->stream);
         if (c->ew_stored) {
+            // the reciprocal chain (spreading, FFTs) on stream2 beside the stored-block T v pass; the
+            // epilogue, which gathers from its potential grid, waits for both
+            hipEventRecord(c->ev_fork, c->stream);
+            hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+            launch_ew_recip(c->stream2, E, c->posa, nullptr, v64, E.phi_v, scalars);
+            hipEventRecord(c->ev_join, c->stream2);
+            if (timed) hipEventRecord(e0, c->stream);
             launch_ew_spmv(c->stream, E, v64, c->precision == TDP_PRECISION_MIXED ? c->vec32 : nullptr, scalars);
             if (timed) hipEventRecord(e1, c->stream);
+            hipStreamWaitEvent(c->stream, c->ev_join, 0);
             launch_ew_tmu_epilogue(c->stream, E, c->posa, c->d_inva, c->d_charge, v64, scalars, c->Emv, pq);
         } else {
+            launch_ew_recip(c->stream, E, c->posa, nullptr, v64, E.phi_v, scalars);
+            if (timed) hipEventRecord(e0, c->stream);
             launch_ew_pairs(c->stream, E, kEwTmu, false, c->posa, c->d_inva, c->d_charge, v64, nullptr, scalars, c->Emv,
                             pq, nullptr, nullptr, nullptr);
             if (timed) hipEventRecord(e1, c->stream);
@@ -1631,6 +1642,12 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         c->err = "hipStreamCreate failed";
         return bail(TDP_ERR_HIP);
     }
+    if (c->ewald && (hipStreamCreate(&c->stream2) != hipSuccess ||
+                     hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)) {
+        c->err = "hipStreamCreate / hipEventCreate failed";
+        return bail(TDP_ERR_HIP);
+    }
     {
         const char* g = std::getenv("TDP_ITERATION_GRAPHS");   // "0": enqueue iterations one launch at a time
         c->use_graphs = !(g && g[0] == '0');
@@ -2079,6 +2096,10 @@ void tdp_destroy(tdp_context* c) {
     for (void* p : c->allocations) hipFree(p);
     if (c->h_stage) hipHostFree(c->h_stage);
     if (c->h_need) hipHostFree(c->h_need);
+    if (c->stream2) hipStreamSynchronize(c->stream2);
+    if (c->ev_fork) hipEventDestroy(c->ev_fork);
+    if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -263,7 +263,11 @@ __global__ __launch_bounds__(64) void k_ew_spread(EwLaunch E, const double4* __r
 // stages so far of total span Ns: butterfly j takes x[j + r n/R] times w^(r (j mod Ns) n/(Ns R)),
 // forms the R-point DFT and writes it to (j div Ns) Ns R + (j mod Ns) + k Ns.
 // Pass kinds: 0 z forward from the fixed-point grid (which it clears), 1 y forward, 2 x forward,
-// influence function, x inverse, 3 y inverse, 4 z inverse to the real potential grid.
+// influence function, x inverse, 3 y inverse, 4 z inverse to the real potential grid.  The grids in
+// real space are real, so the complex grid holds only kz = 0 .. K2/2 (K2h values, [K0][K1][K2h]):
+// kind 0 keeps that half of each z transform, kinds 1-3 run on K2h columns, and kind 4 completes
+// each z line by X(K2 - kz) = conj(X(kz)) (the spectrum stays Hermitian: the influence function is
+// even) before its inverse transform.
 // One radix-R stage of the lines in LDS (R known at compile time, so the butterfly stays in registers).
 template <typename T, int R>
 __device__ __forceinline__ void ew_fft_stage(const typename Cplx<T>::type* __restrict__ src,
@@ -334,17 +338,17 @@ __global__ __launch_bounds__(256) void k_ew_fft(EwLaunch E, int LB, void* phi_ou
     const int axis = (KIND == 0 || KIND == 4) ? 2 : ((KIND == 1 || KIND == 3) ? 1 : 0);
     const EwAxis& A = E.ax[axis];
     const int n = A.n;
-    const int K1 = E.K[1], K2 = E.K[2];
-    const int nlines = axis == 2 ? E.K[0] * K1 : (axis == 1 ? E.K[0] * K2 : K1 * K2);
-    const size_t estride = axis == 2 ? 1 : (axis == 1 ? (size_t)K2 : (size_t)K1 * K2);
+    const int K1 = E.K[1], K2 = E.K[2], K2h = K2 / 2 + 1;
+    const int nlines = axis == 2 ? E.K[0] * K1 : (axis == 1 ? E.K[0] * K2h : K1 * K2h);
+    const size_t estride = axis == 2 ? 1 : (axis == 1 ? (size_t)K2h : (size_t)K1 * K2h);
     C* bufA = reinterpret_cast<C*>(smem);
     C* bufB = bufA + LB * n;
     C* grid = reinterpret_cast<C*>(E.cgrid);
     const int L0 = blockIdx.x * LB;
     const int lb = min(LB, nlines - L0);
-    auto line_base = [&](int Lidx) -> size_t {
-        if (axis == 2) return (size_t)Lidx * K2;
-        if (axis == 1) return (size_t)(Lidx / K2) * K1 * K2 + (Lidx % K2);
+    auto line_base = [&](int Lidx) -> size_t {   // in the complex grid (z lines: their first K2h values)
+        if (axis == 2) return (size_t)Lidx * K2h;
+        if (axis == 1) return (size_t)(Lidx / K2h) * K1 * K2h + (Lidx % K2h);
         return (size_t)Lidx;
     };
     // load (z lines: elements of a line are contiguous; y and x lines: consecutive lines are)
@@ -352,25 +356,34 @@ __global__ __launch_bounds__(256) void k_ew_fft(EwLaunch E, int LB, void* phi_ou
         int b, e;
         if (axis == 2) { b = idx / n; e = idx - b * n; }
         else { e = idx / lb; b = idx - e * lb; }
-        const size_t g = line_base(L0 + b) + (size_t)e * estride;
         C v;
         if (KIND == 0) {
+            const size_t g = (size_t)(L0 + b) * K2 + e;   // the real grid
             const long long f = (long long)E.qfix[g];
             v.x = (T)((double)f * kInvFix);
             v.y = T(0);
             E.qfix[g] = 0ull;
+        } else if (KIND == 4) {   // the z line from its kz <= K2/2 half
+            const size_t lb0 = line_base(L0 + b);
+            if (e < K2h) {
+                v = grid[lb0 + e];
+            } else {
+                v = grid[lb0 + (K2 - e)];
+                v.y = -v.y;
+            }
         } else {
-            v = grid[g];
+            v = grid[line_base(L0 + b) + (size_t)e * estride];
         }
         bufA[b * n + e] = v;
     }
     __syncthreads();
     C* res;
     ew_fft_lines<T>(bufA, bufB, lb, A, KIND >= 3, res);
-    if (KIND == 2) {   // influence function, then the inverse transform along x
+    if (KIND == 2) {   // influence function ([K0][K1][K2]), then the inverse transform along x
         for (int idx = threadIdx.x; idx < lb * n; idx += blockDim.x) {
             const int e = idx / lb, b = idx - e * lb;
-            const size_t g = line_base(L0 + b) + (size_t)e * estride;
+            const int Lb = L0 + b;   // = ky K2h + kz
+            const size_t g = ((size_t)e * K1 + Lb / K2h) * K2 + Lb % K2h;
             const T gf = E.mixed ? (T)E.G32[g] : (T)E.G64[g];
             C v = res[b * n + e];
             v.x *= gf;
@@ -385,9 +398,12 @@ __global__ __launch_bounds__(256) void k_ew_fft(EwLaunch E, int LB, void* phi_ou
         int b, e;
         if (axis == 2) { b = idx / n; e = idx - b * n; }
         else { e = idx / lb; b = idx - e * lb; }
-        const size_t g = line_base(L0 + b) + (size_t)e * estride;
-        if (KIND == 4) reinterpret_cast<T*>(phi_out)[g] = res[b * n + e].x;
-        else grid[g] = res[b * n + e];
+        if (KIND == 4) reinterpret_cast<T*>(phi_out)[(size_t)(L0 + b) * K2 + e] = res[b * n + e].x;
+        else if (KIND == 0) {
+            if (e < K2h) grid[line_base(L0 + b) + e] = res[b * n + e];
+        } else {
+            grid[line_base(L0 + b) + (size_t)e * estride] = res[b * n + e];
+        }
     }
 }
 
@@ -1637,7 +1653,8 @@ void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const doubl
     using C = typename Cplx<T>::type;
     const int axis = (kind == 0 || kind == 4) ? 2 : ((kind == 1 || kind == 3) ? 1 : 0);
     const int n = E.K[axis];
-    const int nlines = axis == 2 ? E.K[0] * E.K[1] : (axis == 1 ? E.K[0] * E.K[2] : E.K[1] * E.K[2]);
+    const int K2h = E.K[2] / 2 + 1;   // the complex grid's z extent (k_ew_fft)
+    const int nlines = axis == 2 ? E.K[0] * E.K[1] : (axis == 1 ? E.K[0] * K2h : E.K[1] * K2h);
     // lines per workgroup: at most 24 KB per LDS buffer, and enough workgroups (512) to spread over the CUs
     int LB = (int)(24576 / sizeof(C)) / n;
     LB = std::min(LB, std::max(1, nlines / 512));

@@ -1068,15 +1068,21 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         const EwLaunch E = ew_launch(c, box);
         launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
         launch_ew_neighbors(s, E);
-        launch_ew_recip(s, E, c->posa, c->d_charge, nullptr, E.phi_q, nullptr);
+        // the charges' potential grid on stream2 beside the E0 pass, joined before its epilogue
+        TDP_HIP(c, hipEventRecord(c->ev_fork, s));
+        TDP_HIP(c, hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+        launch_ew_recip(c->stream2, E, c->posa, c->d_charge, nullptr, E.phi_q, nullptr);
+        TDP_HIP(c, hipEventRecord(c->ev_join, c->stream2));
         for (int attempt = 0;; attempt++) {
             const EwLaunch Eb = ew_launch(c, box);
             if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 7 * sizeof(int), s));
-            if (c->ew_stored)
-                launch_ew_e0_stored(s, Eb, c->posa, c->d_inva, c->d_charge, c->Emv, c->d_epart);
-            else
+            if (c->ew_stored) {
+                launch_ew_e0_stored(s, Eb, c->posa, c->d_inva, c->d_charge, c->Emv, c->d_epart, c->ev_join);
+            } else {
+                TDP_HIP(c, hipStreamWaitEvent(s, c->ev_join, 0));
                 launch_ew_pairs(s, Eb, kEwE0, false, c->posa, c->d_inva, c->d_charge, nullptr, nullptr, nullptr, c->Emv,
                                 PcgQ(), c->d_epart, nullptr, nullptr);
+            }
             if (!c->ew_stored) break;
             // the lists must have held every pair: else grow them and build again
             int* hn = static_cast<int*>(c->h_need);
@@ -1373,13 +1379,19 @@ void forces_and_energy(tdp_context* c, const double* box, bool want_forces = tru
     const int npad = c->npad;
     if (c->ewald) {   // the dipoles' potential grid, then forces and energy per slot
         const EwLaunch E = ew_launch(c, box);
-        launch_ew_recip(s, E, c->posa, nullptr, c->mu, E.phi_v, nullptr);
-        if (c->ew_stored)   // the E0 pass of this evaluation left the pair candidates
+        if (c->ew_stored) {   // the E0 pass of this evaluation left the pair candidates
+            // the dipoles' potential grid on stream2 beside the force pass, joined before its epilogue
+            hipEventRecord(c->ev_fork, s);
+            hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
+            launch_ew_recip(c->stream2, E, c->posa, nullptr, c->mu, E.phi_v, nullptr);
+            hipEventRecord(c->ev_join, c->stream2);
             launch_ew_force_stored(s, E, c->polarization == TDP_MUTUAL, c->posa, c->d_inva, c->d_charge, c->mu, c->E0,
-                                   c->d_epart, c->d_frc, c->d_eout);
-        else
+                                   c->d_epart, c->d_frc, c->d_eout, c->ev_join);
+        } else {
+            launch_ew_recip(s, E, c->posa, nullptr, c->mu, E.phi_v, nullptr);
             launch_ew_pairs(s, E, kEwForce, c->polarization == TDP_MUTUAL, c->posa, c->d_inva, c->d_charge, c->mu,
                             c->E0, nullptr, nullptr, PcgQ(), c->d_epart, c->d_frc, c->d_eout);
+        }
         if (want_energy) {
             const double V = box[0] * box[4] * box[8];
             const double bg = -M_PI * c->ew_qsum * c->ew_qsum / (2.0 * V * c->ew_alpha * c->ew_alpha);

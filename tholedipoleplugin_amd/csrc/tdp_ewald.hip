@@ -1711,7 +1711,7 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
 }
 
 void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
-                         const double* charge, double4* out4, double* epart) {
+                         const double* charge, double4* out4, double* epart, hipEvent_t join) {
     EwLaunch Ep = E;
     Ep.e0_parts = kEwE0Parts;
     const dim3 grid(E.nblk * kEwE0Parts), block(kEwE0Waves * 64);
@@ -1719,6 +1719,7 @@ void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, 
         hipLaunchKernelGGL((k_ew_e0_build<true>), grid, block, 0, s, Ep, posa, inva, charge, E.rs);
     else
         hipLaunchKernelGGL((k_ew_e0_build<false>), grid, block, 0, s, Ep, posa, inva, charge, E.rs);
+    if (join) hipStreamWaitEvent(s, join, 0);
     hipLaunchKernelGGL((k_ew_epilogue<kEwE0, false>), dim3(E.nblk), dim3(64), 0, s, Ep, posa, inva, charge,
                        (const double4*)nullptr, (const double4*)nullptr, (const double*)nullptr, E.rs, out4, PcgQ(),
                        epart, (double*)nullptr, (double*)nullptr);
@@ -1726,17 +1727,19 @@ void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, 
 
 void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const double4* posa, const double* inva,
                             const double* charge, const double4* vec, const double4* E0, double* epart, double* frc,
-                            double* eout) {
+                            double* eout, hipEvent_t join) {
     static_assert(kEwE0Parts == ew_parts(kEwForce), "the force epilogue adds the parts of the force pass");
     const dim3 grid(E.nblk * kEwE0Parts), block(kEwE0Waves * 64), rows(E.nblk);
     if (mutual) {
         if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<true, true>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
         else hipLaunchKernelGGL((k_ew_force_cand<true, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        if (join) hipStreamWaitEvent(s, join, 0);
         hipLaunchKernelGGL((k_ew_epilogue<kEwForce, true>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
                            (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
     } else {
         if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<false, true>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
         else hipLaunchKernelGGL((k_ew_force_cand<false, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        if (join) hipStreamWaitEvent(s, join, 0);
         hipLaunchKernelGGL((k_ew_epilogue<kEwForce, false>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
                            (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
     }

@@ -260,12 +260,12 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
 // The E0 pass that also builds the stored blocks (when the lists are stored): the real-space field
 // and the lists, then the epilogue of kEwE0 (reciprocal gather, scaled pairs) into out4 and epart.
 void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
-                         const double* charge, double4* out4, double* epart);
+                         const double* charge, double4* out4, double* epart, hipEvent_t join = nullptr);
 // The force pass from the E0 pass's candidates (lists stored), then the epilogue of kEwForce (which
 // reads the E0 epilogue's per-slot energy terms from epart).
 void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const double4* posa, const double* inva,
                             const double* charge, const double4* vec, const double4* E0, double* epart, double* frc,
-                            double* eout);
+                            double* eout, hipEvent_t join = nullptr);
 // T v of the solver from the stored blocks (v in FP64, and its FP32 copy v32 for the FP32 lists),
 // into the list partials E.rsw ...
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars);

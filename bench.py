@@ -44,8 +44,9 @@ DEBYE = 0.020819434  # e*nm
 # Counters cannot be read inside this process.
 MATVEC_HBM_BYTES = 265.6e6
 # HBM bytes per launch of k_ew_spmv<true> on this box (64^3 grid) from the same counters and scaling:
-# 997.2 MB read (the stored lists with their padding to the wave width, 997.6 MB) + 9.3 MB written.
-SPMV_HBM_BYTES = 1006.5e6
+# 621.7 MB read (the stored lists with their padding to the wave width, 602.2 MB, and slot data of the
+# LDS copies that miss L2) + 9.4 MB written.
+SPMV_HBM_BYTES = 631.1e6
 
 
 def parse():

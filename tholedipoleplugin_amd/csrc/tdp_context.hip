@@ -117,6 +117,7 @@ struct tdp_context {
     // Ewald option: the reciprocal-space chain of each T v runs on stream2 beside the real-space pass
     // (fork and join by events, also inside captured graphs); both streams at the default priority
     hipStream_t stream2 = nullptr;
+    float4 *d_mu32 = nullptr, *d_slot32 = nullptr;   // the force pass's FP32 slot records (mixed)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     // Captured chunks of solver iterations (hipGraph), replayed while the launch parameters hold:
     // same chunk length, solver and precision (the pair kernel and the vectors it reads), same box,
@@ -759,6 +760,8 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.lds_tiles = c->ew_lds_tiles;
     E.tb_o = c->tb_o;
     E.posl = c->posl;
+    E.mu32 = c->d_mu32;
+    E.slot32 = c->d_slot32;
     // reciprocal space in FP64 in both precisions: FP32 grids and FFTs put 2.4e-8 e*nm into the
     // fixed point of the 36,864-site box, above the 1e-6 D (2.08e-8 e*nm) bar
     E.mixed = false;
@@ -1709,7 +1712,8 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     if (c->ewald) {
         const size_t ng = (size_t)c->ew_K[0] * c->ew_K[1] * c->ew_K[2];
         if ((rc = dalloc(c, &c->d_nbr, (size_t)nb * nb)) || (rc = dalloc(c, &c->d_nbr_cnt, nb)) ||
-            (rc = dalloc(c, &c->d_nbr_exc, (size_t)nb * nb)) ||
+            (rc = dalloc(c, &c->d_nbr_exc, (size_t)nb * nb)) || (rc = dalloc(c, &c->d_mu32, npad)) ||
+            (rc = dalloc(c, &c->d_slot32, npad)) ||
             (rc = dalloc(c, &c->d_qfix, ng)) || (rc = dalloc(c, &c->d_cgrid, ng)) || (rc = dalloc(c, &c->d_phiq, ng)) ||
             (rc = dalloc(c, &c->d_phiv, ng)) || (rc = dalloc(c, &c->d_G64, ng)) || (rc = dalloc(c, &c->d_G32, ng)) ||
             (rc = dalloc(c, &c->d_sc_off, (size_t)npad + 1)) || (rc = dalloc(c, &c->d_epart, npad)) ||

@@ -1166,31 +1166,56 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
     // candidate n+2 and candidate n+3 itself are in flight (every wait is for a load issued an
     // iteration earlier)
     if (MIXED) {
+        // FP32 candidates: their tile pairs take the image of the tile origins (k_ew_e0_build), so d is
+        // u_i - (u_j + D_t) with D_t the image of the origins' separation (FP64 in LDS for the row's
+        // neighbour tiles when they fit, else from the origins) and the slot data are three FP32
+        // records (offset, dipole, parameters)
+        extern __shared__ double4 s_D[];   // [E.lds_tiles] (shared by the workgroup's waves: one row)
+        const int cnt = E.nbr_cnt[I];
+        const double4 oI = E.tb_o[I];
+        const bool lds = E.lds_tiles > 0;
+        if (lds) {
+            for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
+                const double4 oJ = E.tb_o[nb[t]];
+                s_D[t] = make_double4(ew_tile_shift(oJ.x - oI.x, E.L[0], E.iL[0]),
+                                      ew_tile_shift(oJ.y - oI.y, E.L[1], E.iL[1]),
+                                      ew_tile_shift(oJ.z - oI.z, E.L[2], E.iL[2]), 0.0);
+            }
+            __syncthreads();
+        }
+        const float4 ui = E.posl[i];
         const unsigned* cb = ew_cand_base(E.cand32, list, E.cap32, lane);
         const float alpha32 = (float)E.alpha, c1f = (float)E.c1, c2f = (float)E.c2, c3f = (float)E.c3;
+        const float qif = (float)qi, aif = (float)pi.w, iaif = (float)iai;
+        const float vixf = (float)vi.x, viyf = (float)vi.y, vizf = (float)vi.z;
         const int nl = nc.x;
         unsigned c0 = ew_cand_at(cb, 0, nl), c1 = ew_cand_at(cb, 1, nl), c2 = ew_cand_at(cb, 2, nl);
         int J1 = nb[ew_cand_tile(c1)];
         const int j0 = ew_cand_slot(nb, c0, w);
-        double4 p0 = posa[j0], v0 = vec[j0];
-        double q0 = charge[j0], a0 = inva[j0];
+        float4 u0 = E.posl[j0], m0 = E.mu32[j0], r0 = E.slot32[j0];
         float fx = 0.f, fy = 0.f, fz = 0.f, fe = 0.f;
         for (int n = 0; n < mf; n++) {
             const unsigned c3 = ew_cand_at(cb, n + 3, nl);
             const int J2 = nb[ew_cand_tile(c2)];
             const int j1 = J1 * kTile + ew_wave_slot(w, c1 & 7);
-            const double4 p1 = posa[j1], v1 = vec[j1];
-            const double q1 = charge[j1], a1 = inva[j1];
+            const float4 u1 = E.posl[j1], m1 = E.mu32[j1], r1 = E.slot32[j1];
             if (n < nl && (c0 & kUnscaled) == kUnscaled) {
-                const float gx = (float)ew_image_fast(__dsub_rn(pi.x, p0.x), E.L[0], iL0);
-                const float gy = (float)ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
-                const float gz = (float)ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
+                double4 D;
+                if (lds) {
+                    D = s_D[ew_cand_tile(c0)];
+                } else {
+                    const double4 oJ = E.tb_o[nb[ew_cand_tile(c0)]];
+                    D = make_double4(ew_tile_shift(oJ.x - oI.x, E.L[0], E.iL[0]),
+                                     ew_tile_shift(oJ.y - oI.y, E.L[1], E.iL[1]),
+                                     ew_tile_shift(oJ.z - oI.z, E.L[2], E.iL[2]), 0.0);
+                }
+                const float gx = ui.x - (float)((double)u0.x + D.x);
+                const float gy = ui.y - (float)((double)u0.y + D.y);
+                const float gz = ui.z - (float)((double)u0.z + D.z);
                 EwR<float> R;
-                ew_radial32(alpha32, c1f, c2f, c3f, gx * gx + gy * gy + gz * gz, (float)pi.w, (float)p0.w, (float)iai,
-                            (float)a0, MUTUAL, R);
-                ew_pair_terms<float, kEwForce, MUTUAL>(R, gx, gy, gz, (float)qi, (float)vi.x, (float)vi.y, (float)vi.z,
-                                                       (float)q0, (float)v0.x, (float)v0.y, (float)v0.z, fx, fy, fz,
-                                                       fe);
+                ew_radial32(alpha32, c1f, c2f, c3f, gx * gx + gy * gy + gz * gz, aif, r0.y, iaif, r0.z, MUTUAL, R);
+                ew_pair_terms<float, kEwForce, MUTUAL>(R, gx, gy, gz, qif, vixf, viyf, vizf, r0.x, m0.x, m0.y, m0.z, fx,
+                                                       fy, fz, fe);
             }
             if ((n & 63) == 63) {   // FP32 sums over at most 64 candidates
                 ax += (double)fx;
@@ -1203,10 +1228,9 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
             c1 = c2;
             c2 = c3;
             J1 = J2;
-            p0 = p1;
-            v0 = v1;
-            q0 = q1;
-            a0 = a1;
+            u0 = u1;
+            m0 = m1;
+            r0 = r1;
         }
         ax += (double)fx;
         ay += (double)fy;
@@ -1259,6 +1283,19 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
         ae += red[(u * 4 + 3) * kTile + lane];
     }
     rs[(size_t)h * E.npad + i] = make_double4(ax, ay, az, ae);
+}
+
+// FP32 records of every slot for the force pass's FP32 pairs: the dipole, and (charge, Thole a, 1/a).
+__global__ __launch_bounds__(256) void k_ew_force_records(int npad, const double4* __restrict__ posa,
+                                                          const double* __restrict__ inva,
+                                                          const double* __restrict__ charge,
+                                                          const double4* __restrict__ vec, float4* __restrict__ mu32,
+                                                          float4* __restrict__ slot32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npad) return;
+    const double4 v = vec[i];
+    mu32[i] = make_float4((float)v.x, (float)v.y, (float)v.z, 0.f);
+    slot32[i] = make_float4((float)charge[i], (float)posa[i].w, (float)inva[i], 0.f);
 }
 
 typedef float ew_f4 __attribute__((ext_vector_type(4)));
@@ -1730,14 +1767,18 @@ void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const
                             double* eout, hipEvent_t join) {
     static_assert(kEwE0Parts == ew_parts(kEwForce), "the force epilogue adds the parts of the force pass");
     const dim3 grid(E.nblk * kEwE0Parts), block(kEwE0Waves * 64), rows(E.nblk);
+    const size_t shm = E.pairs32 ? (size_t)E.lds_tiles * sizeof(double4) : 0;
+    if (E.pairs32)
+        hipLaunchKernelGGL(k_ew_force_records, dim3((E.npad + 255) / 256), dim3(256), 0, s, E.npad, posa, inva, charge, vec,
+                           E.mu32, E.slot32);
     if (mutual) {
-        if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<true, true>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<true, true>), grid, block, shm, s, E, posa, inva, charge, vec, E.rs);
         else hipLaunchKernelGGL((k_ew_force_cand<true, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
         if (join) hipStreamWaitEvent(s, join, 0);
         hipLaunchKernelGGL((k_ew_epilogue<kEwForce, true>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
                            (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
     } else {
-        if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<false, true>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
+        if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<false, true>), grid, block, shm, s, E, posa, inva, charge, vec, E.rs);
         else hipLaunchKernelGGL((k_ew_force_cand<false, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
         if (join) hipStreamWaitEvent(s, join, 0);
         hipLaunchKernelGGL((k_ew_epilogue<kEwForce, false>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,

@@ -235,6 +235,8 @@ struct EwLaunch {
     int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
     const double4* tb_o;    // [nblk] tile origins (FP64)
     const float4* posl;     // [npad] each slot's FP32 offset from its tile's origin
+    float4* mu32;           // [npad] mixed: the force pass's FP32 records, dipole
+    float4* slot32;         // [npad] and (charge, Thole a, 1/a)
     int e0_parts = 1;       // row parts whose sums the E0 epilogue adds (kEwE0Parts after the list build)
     unsigned* cand32;       // [nblk kEwWaves 64 cap32] the E0 pass's pair candidates with FP32 blocks,
     unsigned* cand64;       // [nblk kEwWaves 64 cap64] and with FP64 blocks, per lane in blocks of 8

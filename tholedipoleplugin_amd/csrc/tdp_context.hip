@@ -153,6 +153,7 @@ struct tdp_context {
     bool has_perm = true;   // some particle has a nonzero molecular dipole
     float* d_slab32 = nullptr;   // mixed T*mu: FP32 tile partial slab (3 * ntri * 64), then item partials
     float4 *vec32 = nullptr, *tb_c = nullptr, *tb_h = nullptr, *posl = nullptr;
+    double4* posl64 = nullptr;   // FP64 offsets from the tile origins (the FP64 far path's frames)
     TilePair* d_tp = nullptr;   // [ntri] tile pair classification, recomputed every evaluation
     double4* tb_o = nullptr;
     // upper-triangle sweep of the pair kernels: work items, their row starts, partial sums
@@ -822,6 +823,7 @@ PairLaunch pair_launch(const tdp_context* c, const double* box9) {
     L.tb_h = c->tb_h;
     L.tb_o = c->tb_o;
     L.posl = c->posl;
+    L.posl64 = c->posl64;
     L.tp = c->d_tp;
     L.nwork = c->nwork;
     L.work = c->d_work;
@@ -1042,7 +1044,7 @@ void enqueue_field(tdp_context* c, const PairLaunch& L, const SegLaunch& G) {
     const int npad = c->npad;
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
-    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl, c->posl64);
     launch_tile_pairs(s, L, c->d_tp);
     launch_rotate(s, npad, c->posa, c->d_dmol, c->d_charge, c->d_axis, c->qd);
     double4* pi = reinterpret_cast<double4*>(c->d_pbuf);
@@ -1063,7 +1065,7 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         return TDP_OK;
     }
     launch_load_positions(s, npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
-    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl, c->posl64);
     {
         // neighbour tiles, the charges' potential grid, E0 = real space + reciprocal + scaled pairs
         int rc = ew_update_box(c, box);
@@ -1706,7 +1708,7 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
         (rc = dalloc(c, &c->d_exc_start, nb + 1)) || (rc = dalloc(c, &c->d_goff, npad + 1)) ||
         (rc = dalloc(c, &c->d_perm, npad)) || (rc = dalloc(c, &c->d_mass, npad)) ||
         (rc = dalloc(c, &c->tb_c, nb + 1)) || (rc = dalloc(c, &c->tb_h, nb + 1)) ||
-        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)) ||
+        (rc = dalloc(c, &c->tb_o, nb + 1)) || (rc = dalloc(c, &c->posl, npad)) || (rc = dalloc(c, &c->posl64, npad)) ||
         (rc = dalloc(c, &c->d_tp, c->ewald ? 1 : c->ntri)))
         return bail(rc);
     if (c->ewald) {
@@ -2288,7 +2290,7 @@ int tdp_get_neighbor_pairs(tdp_context* c, const double* positions, const double
     }
     PairLaunch L = pair_launch(c, box);
     launch_load_positions(s, c->npad, c->d_pos3, c->d_perm, c->d_thole, c->posa);
-    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl);
+    launch_tile_bounds(s, c->d_tcnt, c->nblk, c->posa, L.box, L.pbc, c->tb_c, c->tb_h, c->tb_o, c->posl, c->posl64);
     const EwLaunch E = ew_launch(c, box);
     launch_ew_neighbors(s, E);
     int* d_pairs = nullptr;

@@ -79,6 +79,7 @@ struct PairLaunch {
     const float4* tb_h;                   // [nblk] tile bounding-box half extent, smallest Thole parameter
     const double4* tb_o;                  // [nblk] tile origin (FP64) of the FP32 far-field paths
     const float4* posl;                   // [npad] particle offset from its tile origin (FP32)
+    const double4* posl64;                // [npad] the same offset in FP64 (the FP64 far path's frames)
     const TilePair* tp;                   // [ntri] tile pair classification (launch_tile_pairs)
     // upper-triangle sweep: work item w = tiles (I, J0..J1) as int4 (I, J0, J1, 0); items of tile row I
     // are row_start[I] .. row_start[I+1]
@@ -102,9 +103,10 @@ void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* 
 
 // Classification of every tile pair of the work items into L.tp (after launch_tile_bounds).
 void launch_tile_pairs(hipStream_t s, const PairLaunch& L, TilePair* tp);
-// tb_o: tile origins (FP64), posl: FP32 offsets of the particles from their tile origin
+// tb_o: tile origins (FP64), posl: FP32 offsets of the particles from their tile origin, posl64 (may be
+// null) the same offsets in FP64
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
-                        float4* tb_c, float4* tb_h, double4* tb_o, float4* posl);
+                        float4* tb_c, float4* tb_h, double4* tb_o, float4* posl, double4* posl64);
 
 // posa: FP64 positions (internal order) and Thole parameter; slots with perm < 0 are padding.
 void launch_load_positions(hipStream_t s, int npad, const double* pos3, const int* perm, const double* thole,

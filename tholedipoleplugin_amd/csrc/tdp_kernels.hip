@@ -157,7 +157,8 @@ __device__ __forceinline__ unsigned yz_key(double y, double z) {
 __global__ __launch_bounds__(64) void k_tile_bounds(const int* __restrict__ tcnt, const double4* __restrict__ posa,
                                                     double bx, double by, double bz, int pbc,
                                                     float4* __restrict__ tb_c, float4* __restrict__ tb_h,
-                                                    double4* __restrict__ tb_o, float4* __restrict__ posl) {
+                                                    double4* __restrict__ tb_o, float4* __restrict__ posl,
+                                                    double4* __restrict__ posl64) {
     const int T = blockIdx.x, lane = threadIdx.x;
     const int i = T * kTile + lane;
     const bool valid = lane < tcnt[T];
@@ -186,6 +187,7 @@ __global__ __launch_bounds__(64) void k_tile_bounds(const int* __restrict__ tcnt
     posl[i] = valid ? make_float4((float)(d[0] - c[0]), (float)(d[1] - c[1]), (float)(d[2] - c[2]),
                                   __uint_as_float(yz_key(p.y, p.z)))
                     : make_float4(1e6f, 1e6f, 1e6f, 0.f);
+    if (posl64) posl64[i] = valid ? make_double4(d[0] - c[0], d[1] - c[1], d[2] - c[2], 0.0) : make_double4(1e6, 1e6, 1e6, 0.0);
     if (lane == 0) {
         tb_c[T] = make_float4((float)(p0.x + c[0]), (float)(p0.y + c[1]), (float)(p0.z + c[2]), (float)amax);
         // half extents rounded up so the box still contains every particle after the float conversion
@@ -299,10 +301,12 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
                                                    const int* __restrict__ exc_J,
                                                    const unsigned long long* __restrict__ exc_mask,
                                                    const float4* __restrict__ tb_c, const float4* __restrict__ tb_h, const TilePair* __restrict__ tp,
-                                                   BoxT64 box, const double* __restrict__ scalars,
+                                                   const double4* __restrict__ posl64, BoxT64 box,
+                                                   const double* __restrict__ scalars,
                                                    double4* __restrict__ Pi, double* __restrict__ Pj, size_t S) {
     __shared__ double4 s_p[kTile];
     __shared__ double4 s_u[kTile];
+    __shared__ double4 s_o[kTile];   // offsets of the j tile's particles from its origin
     __shared__ double s_ia[kTile];
     const int lane = threadIdx.x, w = blockIdx.x;
     const int4 wk = work[w];
@@ -314,6 +318,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
     const double4 pi = posa[i];
     const double4 ui = mu[i];
     const double iai = inva[i];
+    const double4 oi = posl64[i];
     const float4 cI = tb_c[I], hI = tb_h[I];
     const bool ifull = cntI == kTile;
     double ax = 0.0, ay = 0.0, az = 0.0;
@@ -327,6 +332,7 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
         __syncthreads();
         s_p[lane] = posa[jb + lane];
         s_u[lane] = mu[jb + lane];
+        s_o[lane] = posl64[jb + lane];
         s_ia[lane] = inva[jb + lane];
         __syncthreads();
         const bool diag = J == I;
@@ -340,13 +346,28 @@ __global__ __launch_bounds__(64) void k_sym_matvec(TileMap tm, int nblk, const i
         int kend;
         if (!masked && !diag && ifull && cntJ == kTile &&
             (tpr.flags & kTpFar)) {
+            // a tile pair on a constant frame (every pair takes the image of the tile origins'
+            // separation D, none near half a box): d = o_j - (o_i - D) from the offsets, as in the
+            // FP32 far path, one subtraction per component; else the minimum image of each pair
+            const bool frame = (tpr.flags & (kTpStraddle | kTpGeneral)) == 0u;
+            const double six = oi.x - tpr.shx, siy = oi.y - tpr.shy, siz = oi.z - tpr.shz;
 #pragma unroll 4
             for (int k = 0; k < kTile; k++) {
                 const int jj = (lane + k) & (kTile - 1);
-                const double4 pj = s_p[jj];
                 const double4 uj = s_u[jj];
-                double dx = pj.x - pi.x, dy = pj.y - pi.y, dz = pj.z - pi.z;
-                if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                double dx, dy, dz;
+                if (frame) {
+                    const double4 oj = s_o[jj];
+                    dx = oj.x - six;
+                    dy = oj.y - siy;
+                    dz = oj.z - siz;
+                } else {
+                    const double4 pj = s_p[jj];
+                    dx = pj.x - pi.x;
+                    dy = pj.y - pi.y;
+                    dz = pj.z - pi.z;
+                    if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+                }
                 double r2 = dx * dx + dy * dy + dz * dz;
                 double rinv = rsqrt(r2);
                 double rinv2 = rinv * rinv;
@@ -1267,6 +1288,76 @@ __device__ __forceinline__ bool elec_far32_lab(const double4* __restrict__ s_p, 
     return true;
 }
 
+// FP64 far tile pair of the energy pass without permanent dipoles: the closed form of elec_far32_lab
+// in FP64, on separations by the minimum image of each pair.  Returns false, with nothing
+// accumulated, when some pair's raw y and z are equal (the reference then builds the frame from
+// e_y) or a separation lies along x (rho = 0); the caller then evaluates the explicit frame.
+template <bool PBC, bool MUTUAL>
+__device__ __forceinline__ bool elec_far64_lab(const double4* __restrict__ s_p, const double4* __restrict__ s_q,
+                                               const double4* __restrict__ s_u, const int* __restrict__ s_o, int lane,
+                                               const ElecSite& me, int my_orig, const BoxT64& box, ElecAcc& acc,
+                                               ElecAcc& jacc, double& en) {
+    const double Qm = me.q, mx = me.ux, my = me.uy, mz = me.uz;
+    double ifx = 0.0, ify = 0.0, ifz = 0.0, e64 = 0.0, jfx = 0.0, jfy = 0.0, jfz = 0.0;
+    bool bad = false;
+    for (int k = 0; k < kTile; k++) {
+        const int jj = (lane + k) & (kTile - 1);
+        const double4 pj = s_p[jj];
+        double dx = pj.x - me.x, dy = pj.y - me.y, dz = pj.z - me.z;   // x_other - x_me, minimum image
+        bad = bad || !(me.y != pj.y || me.z != pj.z);
+        if (PBC) min_image(dx, dy, dz, box.bx, box.by, box.bz, box.ibx, box.iby, box.ibz);
+        const double4 uo = s_u[jj];
+        const double qo = s_q[jj].w;
+        const double s = my_orig < s_o[jj] ? 1.0 : -1.0;
+        const double rho2 = dy * dy + dz * dz;
+        bad = bad || !(rho2 > 0.0);
+        const double r2 = dx * dx + rho2;
+        const double rinv = rsqrt(r2);
+        const double rI2 = rinv * rinv, rI3 = rI2 * rinv;
+        const double kx = rinv * rsqrt(rho2 > 0.0 ? rho2 : 1.0);
+        const double ex = -dx * kx;
+        const double Xx = rho2 * kx, Xy = ex * dy, Xz = ex * dz;
+        const double Xm = Xx * mx + Xy * my + Xz * mz;
+        const double Xo = Xx * uo.x + Xy * uo.y + Xz * uo.z;
+        const double QQ = Qm * qo;
+        const double Xw = Qm * Xo - qo * Xm;
+        double am, ao, inner;
+        if (MUTUAL) {
+            const double rI4 = rI2 * rI2;
+            const double A = s * (6.0 * rI4);
+            am = A * Xo - rI3 * qo;
+            ao = A * Xm + rI3 * Qm;
+            inner = rI2 * QQ + rI4 * (6.0 * (mx * uo.x + my * uo.y + mz * uo.z) - 30.0 * Xm * Xo);
+        } else {
+            am = -rI3 * qo;
+            ao = rI3 * Qm;
+            inner = rI2 * QQ;
+        }
+        const double lam = s * inner - 3.0 * rI3 * Xw;
+        const double Gx = mx * am + uo.x * ao + Xx * lam;
+        const double Gy = my * am + uo.y * ao + Xy * lam;
+        const double Gz = mz * am + uo.z * ao + Xz * lam;
+        e64 += rinv * QQ - 0.5 * s * rI2 * Xw;
+        // force on the lane's particle -(G_y, G_z, G_x), on the other +(G_y, G_z, G_x); j side lagging one step
+        ifx -= Gy;
+        ify -= Gz;
+        ifz -= Gx;
+        jfx = rot1(jfx) + Gy;
+        jfy = rot1(jfy) + Gz;
+        jfz = rot1(jfz) + Gx;
+    }
+    if (__builtin_amdgcn_ballot_w64(bad)) return false;
+    acc.fx += ifx;
+    acc.fy += ify;
+    acc.fz += ifz;
+    en += e64;
+    jacc.fx = jfx;
+    jacc.fy = jfy;
+    jacc.fz = jfz;
+    jacc.tx = jacc.ty = jacc.tz = 0.0;
+    return true;
+}
+
 // Near tile pair of the energy pass without permanent dipoles in mixed precision (off-diagonal
 // tiles on frame SM, full or partial), pass 1: steps whose pairs all lie beyond r2d (Thole exponent
 // a u^3 >= 4) in FP32 by the closed form of elec_far32_lab with the Thole factors of
@@ -1483,6 +1574,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PERM ? 2 : 3
                     elec_far32<PBC, MUTUAL, PERM, false>(s_p, s_l, s_qf, s_uf, s_o, lane, me, my_orig, si_x, si_y,
                                                          si_z, fr, box, acc, jacc, en);
                 kend = kTile - 1;   // the lagging sums end on slot (lane + 63)
+            } else if (!PERM && elec_far64_lab<PBC, MUTUAL>(s_p, s_q, s_u, s_o, lane, me, my_orig, box, acc, jacc, en)) {
+                kend = kTile - 1;   // closed form in the lab frame: the lagging sums end on slot (lane + 63)
             } else {
                 for (int k = 0; k < kTile; k++) {
                     const int jj = (lane + k) & (kTile - 1);
@@ -2233,9 +2326,9 @@ void launch_tile_pairs(hipStream_t s, const PairLaunch& L, TilePair* tp) {
 }
 
 void launch_tile_bounds(hipStream_t s, const int* tcnt, int nblk, const double4* posa, const double* box3, bool pbc,
-                        float4* tb_c, float4* tb_h, double4* tb_o, float4* posl) {
+                        float4* tb_c, float4* tb_h, double4* tb_o, float4* posl, double4* posl64) {
     hipLaunchKernelGGL(k_tile_bounds, dim3(nblk), dim3(kTile), 0, s, tcnt, posa, box3[0], box3[1], box3[2],
-                       pbc ? 1 : 0, tb_c, tb_h, tb_o, posl);
+                       pbc ? 1 : 0, tb_c, tb_h, tb_o, posl, posl64);
 }
 
 void launch_gather_vec3(hipStream_t s, int npad, const int* perm, const double* src3, double4* dst4, float4* dst32) {
@@ -2266,10 +2359,12 @@ void launch_sym_matvec64(hipStream_t s, const PairLaunch& L, const double4* posa
     BoxT64 b = mkbox(L.box);
     if (L.pbc)
         hipLaunchKernelGGL(k_sym_matvec<true>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, b, scalars, Pi, Pj, L.slab);
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, L.posl64, b, scalars, Pi, Pj,
+                           L.slab);
     else
         hipLaunchKernelGGL(k_sym_matvec<false>, dim3(L.nwork), dim3(kTile), 0, s, L.tm, L.nblk, L.work, posa, mu, inva,
-                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, b, scalars, Pi, Pj, L.slab);
+                           L.exc_start, L.exc_J, L.exc_mask, L.tb_c, L.tb_h, L.tp, L.posl64, b, scalars, Pi, Pj,
+                           L.slab);
 }
 
 void launch_sym_matvec_mixed(hipStream_t s, const PairLaunch& L, const double4* posa, const double4* mu,

@@ -1098,15 +1098,15 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
                 c->ew_slots[k] = (int64_t)hn[4 + k] * 64;
             }
             const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
-            if (mixed_lists && hn[6] > kEwMaxLocalTiles) {
-                // the FP32 list indices cannot address this many neighbour tiles: matrix-free T v
-                // passes from now on (this pass's E0 is complete)
+            if (hn[6] > kEwMaxLocalTiles) {
+                // the list indices (FP32 lists, and the FP64 lists of FP64 mode) cannot address this
+                // many neighbour tiles: matrix-free T v passes from now on (this pass's E0 is complete)
                 c->ew_stored = false;
                 c->graph_gen++;
                 break;
             }
-            {   // LDS copy of the slot data for the FP32 lists: the wave's slots of up to lds_tiles tiles,
-                // 64 KB at most
+            {   // LDS copy of the slot data for the FP32 lists (FP64 mode: v of the FP64 lists, also 24 B
+                // per slot): the wave's slots of up to lds_tiles tiles, 64 KB at most
                 int lt = (hn[6] + 7) / 8 * 8;
                 if ((size_t)lt * 8 * kEwSlotLds > 65536) lt = 0;
                 if (lt != c->ew_lds_tiles) {

@@ -1085,7 +1085,10 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             } else {
                 const double g = st ? sqrt(fabs(C2)) : 0.0;
                 const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
-                E.idx64[at] = st ? (j0 | (C2 < 0.0 ? (int)0x80000000u : 0)) : i;
+                // j itself in mixed precision (its FP64 pairs gather v_j from memory), its place among
+                // the wave's slots in FP64 mode (the T v pass stages them in LDS)
+                const int jx = MIXED ? j0 : (int)(c & 0xffffu);
+                E.idx64[at] = st ? (jx | (C2 < 0.0 ? (int)0x80000000u : 0)) : (MIXED ? i : 0);
                 E.t64[at] = st ? make_double4(-C1, g * dx, g * dy, g * dz) : make_double4(0.0, 0.0, 0.0, 0.0);
                 s64 += st ? 1 : 0;
             }
@@ -1414,12 +1417,19 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     }
     if (lds) {
         // entry 8 t + m: slot 8w + m of neighbour tile t (the wave's own LDS: its operations
-        // complete in order); four slots per lane at a time, their loads issued together
+        // complete in order); four slots per lane at a time, their loads issued together, and the
+        // neighbour tiles of the next four requested before them
         const int ne = 8 * E.nbr_cnt[I];
+        int Jn[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) Jn[u] = nb[min(lane + u * kTile, ne - 1) >> 3];
         for (int e0 = lane; e0 < ne; e0 += 4 * kTile) {
             int J[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) J[u] = nb[min(e0 + u * kTile, ne - 1) >> 3];
+            for (int u = 0; u < 4; u++) {
+                J[u] = Jn[u];
+                Jn[u] = nb[min(e0 + (4 + u) * kTile, ne - 1) >> 3];
+            }
             float4 a[4];
             float2 b[4];
 #pragma unroll
@@ -1432,6 +1442,29 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
                 if (e0 + u * kTile < ne) {
                     s_a[e0 + u * kTile] = a[u];
                     s_b[e0 + u * kTile] = b[u];
+                }
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // FP64 mode: the FP64 lists index the wave's slots (8 t + m) and their v_j (FP64, 24 B per slot)
+    // are staged in LDS like the FP32 lists' slot data; mixed: j itself, v_j from memory
+    const bool lds64 = !MIXED && E.lds_tiles > 0;
+    double2* s_vxy = reinterpret_cast<double2*>(s_a);
+    double* s_vz = reinterpret_cast<double*>(s_vxy + (size_t)E.lds_tiles * 8);
+    if (lds64) {
+        const int ne = 8 * E.nbr_cnt[I];
+        for (int e0 = lane; e0 < ne; e0 += 4 * kTile) {
+            double4 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; u++) {
+                const int e = min(e0 + u * kTile, ne - 1);
+                v[u] = vec[nb[e >> 3] * kTile + ew_wave_slot(w, e & 7)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; u++)
+                if (e0 + u * kTile < ne) {
+                    s_vxy[e0 + u * kTile] = make_double2(v[u].x, v[u].y);
+                    s_vz[e0 + u * kTile] = v[u].z;
                 }
         }
         __builtin_amdgcn_wave_barrier();
@@ -1450,7 +1483,17 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
                 t[u] = E.t64[at];
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) v[u] = vec[j[u] & 0x7fffffff];
+            for (int u = 0; u < 4; u++) {
+                const int loc = j[u] & 0x7fffffff;
+                if (lds64) {
+                    const double2 xy = s_vxy[loc];
+                    v[u] = make_double4(xy.x, xy.y, s_vz[loc], 0.0);
+                } else if (!MIXED) {
+                    v[u] = vec[nb[loc >> 3] * kTile + ew_wave_slot(w, loc & 7)];
+                } else {
+                    v[u] = vec[loc];
+                }
+            }
 #pragma unroll
             for (int u = 0; u < 4; u++) {   // entries past the end: the last one with weight zero
                 const double on = n0 + u < n64 ? 1.0 : 0.0;
@@ -1792,7 +1835,8 @@ void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const 
         hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), (size_t)E.lds_tiles * 8 * kEwSlotLds, s, E, vec, v32,
                            scalars, E.rsw);
     else
-        hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), 0, s, E, vec, v32, scalars, E.rsw);
+        hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), (size_t)E.lds_tiles * 8 * kEwSlotLds, s, E, vec, v32,
+                           scalars, E.rsw);
 }
 
 void launch_ew_tmu_epilogue(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,

@@ -216,8 +216,10 @@ struct EwLaunch {
     // lane at ((I kEwWaves + w) cap + n) 64 + lane, T_ij = C2 d d^T - C1 1.
     // FP64 lists (every pair in FP64 mode, the pairs closer than r64 in mixed precision, and in
     // mixed precision the pairs of tile pairs whose image is not that of their tiles' centres) hold
-    // (-C1, u) with u = sqrt(|C2|) d and index j itself with the sign of C2 in bit 31, so that
-    // T_ij v = -C1 v + sign(C2) u (u . v): 4 + 32 bytes per entry.
+    // (-C1, u) with u = sqrt(|C2|) d and an index with the sign of C2 in bit 31, so that
+    // T_ij v = -C1 v + sign(C2) u (u . v): 4 + 32 bytes per entry.  The index is j itself in mixed
+    // precision and, in FP64 mode, j's place among the wave's slots (as for the FP32 lists; the T v
+    // pass stages those v_j in LDS).
     // FP32 lists (the other pairs in mixed precision) hold (C1, C2) only and index j by its place
     // among the wave's slots of the row's neighbour tiles, 8 t + m for slot 8w + m of tile
     // nbr[I][t] (uint16; rows with at most 4096 neighbour tiles): 2 + 8 bytes per entry.  The T v

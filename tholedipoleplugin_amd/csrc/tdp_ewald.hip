@@ -472,7 +472,35 @@ struct EwR {
     T B0, B1, B2, B3, b1, b2, b3, l3, l5, l7;
 };
 
-// FP64: erfc and the exponentials of the math library.
+// erfcx(x) = e^{x^2} erfc(x) on [0, 6] as P(x) / Q(x), degree 9, coefficients in ascending powers
+// from fit_erfcx.py (relative error of the fit 2.3e-18; below 1e-15 evaluated in FP64).
+constexpr double kErfcxP[10] = {1, 1.8246330203534158, 1.6678312021636672, 0.95726832494803793,
+                                0.37200178985934518, 0.099564333415988568, 0.017898050519465095,
+                                0.0019808686982774721, 0.00010407600456896987, -3.7341872226394312e-13};
+constexpr double kErfcxQ[10] = {1, 2.9530121874489286, 3.9999486346601589, 3.2699676243660476,
+                                1.7832281212683818, 0.67512166134826335, 0.17822904124994046,
+                                0.031815678241935146, 0.0035109996444155928, 0.00018446987284089452};
+
+// erfc(x) for x >= 0 from e = e^{-x^2}, which the radial functions need anyway: e P(x) / Q(x) up to
+// x = 6 (one branch for every pair inside the cutoff: a r_c is about 3), the library's erfc beyond.
+__device__ __forceinline__ double ew_erfc(double x, double e) {
+    if (x <= 6.0) {
+        double p = kErfcxP[9], q = kErfcxQ[9];
+#pragma unroll
+        for (int k = 8; k >= 0; k--) {
+            p = fma(p, x, kErfcxP[k]);
+            q = fma(q, x, kErfcxQ[k]);
+        }
+        return e * p / q;
+    }
+    return erfc(x);
+}
+
+// FP64: erfc through ew_erfc, the exponentials of the math library; THOLE32: the Thole exponential
+// in FP32, for pairs whose exponent s is at most -4 (the mixed lists' FP32 pairs beyond r64): e^s
+// <= e^-4 enters C_n as (lambda - 1) b_n = -e^s b_n, so the FP32 rounding of s and e^s (relative
+// error below 1.5e-7 |s| + 1.2e-7) moves C_1 by less than 2e-8 b_1.
+template <bool THOLE32 = false>
 __device__ __forceinline__ void ew_radial(const EwLaunch& E, double r2, double ai, double aj, double iai, double iaj,
                                           bool inside, bool need_l7, EwR<double>& R) {
     const double ri = rsqrt(r2);
@@ -483,7 +511,7 @@ __device__ __forceinline__ void ew_radial(const EwLaunch& E, double r2, double a
     R.b3 = 5.0 * R.b2 * ri2;
     if (inside) {
         const double ex = exp(-E.alpha * E.alpha * r2);
-        R.B0 = erfc(E.alpha * r) * ri;
+        R.B0 = ew_erfc(E.alpha * r, ex) * ri;
         R.B1 = (R.B0 + E.c1 * ex) * ri2;
         R.B2 = (3.0 * R.B1 + E.c2 * ex) * ri2;
         R.B3 = (5.0 * R.B2 + E.c3 * ex) * ri2;
@@ -495,7 +523,7 @@ __device__ __forceinline__ void ew_radial(const EwLaunch& E, double r2, double a
     const double u = r * (iai * iaj);
     const double s = -g * (u * u * u);
     if (ai * aj != 0.0 && s > -50.0) {
-        const double es = exp(s);
+        const double es = THOLE32 ? (double)__expf((float)s) : exp(s);
         R.l3 = 1.0 - es;
         R.l5 = 1.0 - (1.0 - s) * es;
         if (need_l7) R.l7 = 1.0 - (1.0 - s + 0.6 * s * s) * es;
@@ -1064,7 +1092,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
                 dy = ew_image_fast(__dsub_rn(pi.y, p0.y), E.L[1], iL1);
                 dz = ew_image_fast(__dsub_rn(pi.z, p0.z), E.L[2], iL2);
                 EwR<double> R;
-                ew_radial(E, ew_r2(dx, dy, dz), pi.w, p0.w, iai, a0, true, false, R);
+                ew_radial<FP32>(E, ew_r2(dx, dy, dz), pi.w, p0.w, iai, a0, true, false, R);
                 C1 = R.B1 + (R.l3 - 1.0) * R.b1;
                 C2 = R.B2 + (R.l5 - 1.0) * R.b2;
                 if (c & kCandSel) {

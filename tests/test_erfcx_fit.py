@@ -49,3 +49,19 @@ def test_erfc_rational_is_erfcx():
     x = np.linspace(0.0, 6.0, 2001)
     r = np.polyval(P[::-1], x) / np.polyval(Q[::-1], x)
     assert np.all(np.diff(r) < 0.0)
+
+
+def test_erfc_asymptotic_branch():
+    """Beyond x = 6 the kernels take e^{-x^2} / (x sqrt(pi)) (1 - t/2 + 3t^2/4 - 15t^3/8), t = 1/x^2:
+    relative error below 105 / (16 x^8) against SciPy's erfc."""
+    scipy_special = pytest.importorskip("scipy.special")
+    x = np.linspace(6.0, 26.0, 20001)
+    t = 1.0 / (x * x)
+    got = np.exp(-x * x) * (0.56418958354775628 / x) * (1.0 - t * (0.5 - t * (0.75 - 1.875 * t)))
+    want = scipy_special.erfc(x)
+    rel = np.abs(got - want) / want
+    assert np.all(rel < 105.0 / (16.0 * x ** 8) + 1e-13), rel.max()
+    assert rel[0] > 1e-7   # the bound is near-tight at x = 6 (a dropped term would show)
+    src = open(SRC).read()
+    assert "e * (kInvSqrtPi / x) * (1.0 - t * (0.5 - t * (0.75 - 1.875 * t)))" in src
+    assert "constexpr double kInvSqrtPi = 0.56418958354775628;" in src

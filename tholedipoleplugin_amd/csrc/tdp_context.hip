@@ -1095,7 +1095,7 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
             TDP_HIP(c, hipStreamSynchronize(s));
             for (int k = 0; k < 2; k++) {
                 c->ew_entries[k] = hn[2 + k];
-                c->ew_slots[k] = (int64_t)hn[4 + k] * 64;
+                c->ew_slots[k] = hn[4 + k];
             }
             const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
             if (hn[6] > kEwMaxLocalTiles) {

@@ -474,6 +474,7 @@ struct EwR {
 
 // erfcx(x) = e^{x^2} erfc(x) on [0, 6] as P(x) / Q(x), degree 9, coefficients in ascending powers
 // from fit_erfcx.py (relative error of the fit 2.3e-18; below 1e-15 evaluated in FP64).
+constexpr double kInvSqrtPi = 0.56418958354775628;   // 1 / sqrt(pi)
 constexpr double kErfcxP[10] = {1, 1.8246330203534158, 1.6678312021636672, 0.95726832494803793,
                                 0.37200178985934518, 0.099564333415988568, 0.017898050519465095,
                                 0.0019808686982774721, 0.00010407600456896987, -3.7341872226394312e-13};
@@ -482,7 +483,11 @@ constexpr double kErfcxQ[10] = {1, 2.9530121874489286, 3.9999486346601589, 3.269
                                 0.031815678241935146, 0.0035109996444155928, 0.00018446987284089452};
 
 // erfc(x) for x >= 0 from e = e^{-x^2}, which the radial functions need anyway: e P(x) / Q(x) up to
-// x = 6 (one branch for every pair inside the cutoff: a r_c is about 3), the library's erfc beyond.
+// x = 6 (every pair inside the cutoff: a r_c is about 3, below 6 for any tolerance above 1e-15).
+// Beyond, the asymptotic series e / (x sqrt(pi)) (1 - 1/(2x^2) + 3/(4x^4) - 15/(8x^6)), relative
+// error below 105 / (16 x^8) < 4e-6 where erfc(x) < 2.2e-17: the kernel's terms there are below 1e-22
+// of the bare ones.  (The math library's erfc in that branch would hold its registers in every caller
+// and cost them a wave per SIMD.)
 __device__ __forceinline__ double ew_erfc(double x, double e) {
     if (x <= 6.0) {
         double p = kErfcxP[9], q = kErfcxQ[9];
@@ -493,7 +498,8 @@ __device__ __forceinline__ double ew_erfc(double x, double e) {
         }
         return e * p / q;
     }
-    return erfc(x);
+    const double t = 1.0 / (x * x);
+    return e * (kInvSqrtPi / x) * (1.0 - t * (0.5 - t * (0.75 - 1.875 * t)));
 }
 
 // FP64: erfc through ew_erfc, the exponentials of the math library; THOLE32: the Thole exponential
@@ -848,10 +854,21 @@ __device__ __forceinline__ void ew_slot32(const EwLaunch& E, const double4& oI, 
                     (float)((double)u.z + ew_tile_shift(oJ.z - oI.z, E.L[2], E.iL[2])));
 }
 
-// FP32 list entries are stored in pairs: entries 2p and 2p+1 of a lane are adjacent (a 4-byte index
-// pair and a 16-byte coefficient pair), pair p of list `list` at this index of the pair arrays.
-__device__ __forceinline__ size_t ew_pair32(int list, int cap32, int p, int lane) {
-    return ((size_t)list * (cap32 >> 1) + p) * kTile + lane;
+// The stored lists are jagged: row n of a list (entry n of every lane, or entry pair n of the FP32
+// lists, whose entries 2n and 2n+1 of a lane are adjacent: a 4-byte index pair and a 16-byte
+// coefficient pair) holds only the lanes whose list reaches it, packed in lane order, and the rows
+// follow one another, each rounded up to whole 128-byte lines of the coefficient array (8 FP32 entry
+// pairs, 4 FP64 entries).  The row's lanes are the set bits of ballot(n < rows of the lane); a lane's
+// place is the number of set bits below it, the row's start the sum of the earlier rows' rounded
+// counts.  A pass that walks the rows in order keeps that sum as it goes, so that no line is streamed
+// for lanes past their list's end.  A list's rows lie in its region of the arrays, (cap32 / 2) 64
+// pairs or cap64 64 entries from the list's base.
+__device__ __forceinline__ size_t ew_base32(int list, int cap32) { return (size_t)list * (cap32 >> 1) * kTile; }
+__device__ __forceinline__ size_t ew_base64(int list, int cap64) { return (size_t)list * cap64 * kTile; }
+__device__ __forceinline__ int ew_row_slots32(unsigned long long m) { return ((int)__popcll(m) + 7) & ~7; }
+__device__ __forceinline__ int ew_row_slots64(unsigned long long m) { return ((int)__popcll(m) + 3) & ~3; }
+__device__ __forceinline__ int ew_lane_rank(unsigned long long m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
 }
 
 __device__ __forceinline__ unsigned* ew_cand_base(unsigned* c, int list, int cap, int lane) {
@@ -1072,14 +1089,24 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     // one stream: FP32 blocks (FP32 = true) or FP64 blocks.  Software pipeline: while pair n is
     // evaluated, the slot data of candidate n+1, the neighbour tile of candidate n+2 and candidate n+3
     // itself are in flight, so that every wait is for a load issued an iteration earlier
-    auto stream = [&](auto fp32_tag, const unsigned* cb, int nl, int wd) {
+    auto stream = [&](auto fp32_tag, const unsigned* cb, int nl, int wd, int& slots) {
         constexpr bool FP32 = decltype(fp32_tag)::value;
+        const size_t lbase = FP32 ? ew_base32(list, E.cap32) : ew_base64(list, E.cap64);
+        const int rows = FP32 ? (nl + 1) >> 1 : nl;   // the lane's rows of the jagged list
+        int rs = 0, rank = 0;                          // start of the current row, the lane's place in it
+        unsigned long long rm = 0ull;                  // lanes of the current row
         unsigned c0 = ew_cand_at(cb, 0, nl), c1 = ew_cand_at(cb, 1, nl), c2 = ew_cand_at(cb, 2, nl);
         int J1 = nb[ew_cand_tile(c1)];
         int j0 = ew_cand_slot(nb, c0, w);
         double4 p0 = posa[j0];
         double q0 = charge[j0], a0 = inva[j0];
         for (int n = 0; n < wd; n++) {
+            const int row = FP32 ? n >> 1 : n;
+            if (!FP32 || (n & 1) == 0) {
+                rs += FP32 ? ew_row_slots32(rm) : ew_row_slots64(rm);
+                rm = __builtin_amdgcn_ballot_w64(row < rows);
+                rank = ew_lane_rank(rm);
+            }
             const unsigned c3 = ew_cand_at(cb, n + 3, nl);
             const int J2 = nb[ew_cand_tile(c2)];
             const int j1 = J1 * kTile + ew_wave_slot(w, c1 & 7);
@@ -1105,14 +1132,16 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             // the block T_ij = C2 d d^T - C1: FP32 as (C1, C2) (the T v pass forms d), FP64 as
             // (-C1, sqrt|C2| d) with the sign of C2 in the index
             const bool st = (c & kCandStore) != 0u;
-            if (FP32) {   // entries in pairs per lane (ew_pair32)
-                const size_t at = ew_pair32(list, E.cap32, n >> 1, lane) * 2 + (n & 1);
+            if (row >= rows) {
+                // past the lane's list: nothing stored
+            } else if (FP32) {   // entries in pairs per lane; the second of an odd list's last pair is zero
+                const size_t at = (lbase + rs + rank) * 2 + (n & 1);
                 E.idx16[at] = st ? (unsigned short)(c & 0xffffu) : (unsigned short)0;
                 E.t32[at] = st ? make_float2((float)C1, (float)C2) : make_float2(0.f, 0.f);
                 s32 += st ? 1 : 0;
             } else {
                 const double g = st ? sqrt(fabs(C2)) : 0.0;
-                const size_t at = ((size_t)list * E.cap64 + n) * kTile + lane;
+                const size_t at = lbase + rs + rank;
                 // j itself in mixed precision (its FP64 pairs gather v_j from memory), its place among
                 // the wave's slots in FP64 mode (the T v pass stages them in LDS)
                 const int jx = MIXED ? j0 : (int)(c & 0xffffu);
@@ -1129,11 +1158,15 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
             q0 = q1;
             a0 = a1;
         }
+        // slots of the list's rows (entries, zero halves of odd FP32 lists, candidates in the field sum
+        // only, and the rows' rounding to whole lines), FP32 entries counted singly
+        slots = FP32 ? 2 * (rs + ew_row_slots32(rm)) : rs + ew_row_slots64(rm);
     };
     const int wd32 = MIXED ? min((mf + 1) & ~1, E.cap32) : 0;   // whole entry pairs
     const int wd64 = min(mn, E.cap64);
-    if (MIXED) stream(std::true_type(), cf, min(nf, E.cap32), wd32);
-    stream(std::false_type(), cn, min(nn, E.cap64), wd64);
+    int k32 = 0, k64 = 0;
+    if (MIXED) stream(std::true_type(), cf, min(nf, E.cap32), wd32, k32);
+    stream(std::false_type(), cn, min(nn, E.cap64), wd64, k64);
     int s32w = s32, s64w = s64;
     for (int off = 32; off > 0; off >>= 1) {
         s32w += __shfl_xor(s32w, off, 64);
@@ -1146,8 +1179,8 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         atomicMax(&E.need[1], mn);
         atomicAdd(&E.need[2], s32w);
         atomicAdd(&E.need[3], s64w);
-        atomicAdd(&E.need[4], wd32);
-        atomicAdd(&E.need[5], wd64);
+        atomicAdd(&E.need[4], k32);
+        atomicAdd(&E.need[5], k64);
         if (w == 0) atomicMax(&E.need[6], cnt);
     }
     // the part's field over its waves, in wave order (the epilogue adds the parts in order)
@@ -1336,28 +1369,34 @@ typedef float ew_f4 __attribute__((ext_vector_type(4)));
 // Software pipeline over steps of kEwSpmvU entry pairs, three register sets by step mod 3: while step
 // s is summed, the indices and coefficients of steps s+1 and s+2 are in flight (the memory counter
 // completes in order, so waiting for step s never waits for a later step); the first two steps are
-// issued by the caller before the slot data is staged.  Entries past the end of the list are read at
-// the last pair (cached); the last step adds them with weight zero.
+// issued by the caller before the slot data is staged.  The rows are jagged (ew_lane_rank): steps
+// that every lane's list covers are summed without weights, the others weight each row by whether
+// the lane's list reaches it.
 struct EwSpmvPipe {
     unsigned j[3][kEwSpmvU];
     ew_f4 t[3][kEwSpmvU];
+    size_t base;   // the list's first entry pair
+    int rs;        // start of the next row to be requested (rows are requested in order)
+    int rows;      // the lane's rows (entry pairs)
 };
 
-__device__ __forceinline__ void ew_spmv_load(const EwLaunch& E, int row, int lane, int n32, EwSpmvPipe& P, int set,
-                                             int step) {
-    const unsigned* ip = reinterpret_cast<const unsigned*>(E.idx16) + ew_pair32(row, E.cap32, 0, lane);
-    const ew_f4* tp = reinterpret_cast<const ew_f4*>(E.t32) + ew_pair32(row, E.cap32, 0, lane);
-    const int last = ((n32 + 1) >> 1) - 1;
+// Requests the rows of step `step` into register set `set`; a lane past its list reads the list's
+// first pair (cached) and the sum gives it weight zero.
+__device__ __forceinline__ void ew_spmv_load(const EwLaunch& E, int lane, EwSpmvPipe& P, int set, int step) {
+    const unsigned* ip = reinterpret_cast<const unsigned*>(E.idx16);
+    const ew_f4* tp = reinterpret_cast<const ew_f4*>(E.t32);
 #pragma unroll
     for (int u = 0; u < kEwSpmvU; u++) {
-        const size_t at = (size_t)min(step * kEwSpmvU + u, last) * kTile;
-        P.j[set][u] = __builtin_nontemporal_load(ip + at);
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(step * kEwSpmvU + u < P.rows);
+        const size_t at = P.base + (((m >> lane) & 1ull) ? (size_t)(P.rs + ew_lane_rank(m)) : 0);
+        P.rs += ew_row_slots32(m);
+        P.j[set][u] = ip[at];
         P.t[set][u] = __builtin_nontemporal_load(tp + at);
     }
 }
 
 template <bool LDS>
-__device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, int lane, int n32, EwSpmvPipe& P,
+__device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int w, int lane, int n32, int minrows, EwSpmvPipe& P,
                                              const float4* __restrict__ s_a, const float2* __restrict__ s_b,
                                              const float4* __restrict__ v32, const int* __restrict__ nb,
                                              const double4& oI, const float4& ui, float& fx, float& fy,
@@ -1380,7 +1419,7 @@ __device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, 
         fz = fmaf(-c1, a.z, fmaf(p, dz, fz));
     };
     auto sum = [&](int set, int step) {
-        if ((step + 1) * kEwSpmvU * 2 <= n32) {   // a whole step inside the list (uniform)
+        if ((step + 1) * kEwSpmvU <= minrows) {   // every lane's list holds the whole step (uniform)
 #pragma unroll
             for (int u = 0; u < kEwSpmvU; u++) {
                 const ew_f4 c = P.t[set][u];
@@ -1390,24 +1429,23 @@ __device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int row, int w, 
         } else {
 #pragma unroll
             for (int u = 0; u < kEwSpmvU; u++) {
-                const int e = 2 * (step * kEwSpmvU + u);   // first entry of the pair
-                const float on0 = e < n32 ? 1.f : 0.f, on1 = e + 1 < n32 ? 1.f : 0.f;
+                const float on = step * kEwSpmvU + u < P.rows ? 1.f : 0.f;
                 const ew_f4 c = P.t[set][u];
-                term((int)(P.j[set][u] & 0xffffu), on0 * c.x, on0 * c.y);
-                term((int)(P.j[set][u] >> 16), on1 * c.z, on1 * c.w);
+                term((int)(P.j[set][u] & 0xffffu), on * c.x, on * c.y);
+                term((int)(P.j[set][u] >> 16), on * c.z, on * c.w);
             }
         }
     };
     for (int s0 = 0; s0 < ns; s0 += 3) {
-        ew_spmv_load(E, row, lane, n32, P, 2, s0 + 2);
+        ew_spmv_load(E, lane, P, 2, s0 + 2);
         __builtin_amdgcn_sched_barrier(0);
         sum(0, s0);
         __builtin_amdgcn_sched_barrier(0);
-        ew_spmv_load(E, row, lane, n32, P, 0, s0 + 3);
+        ew_spmv_load(E, lane, P, 0, s0 + 3);
         __builtin_amdgcn_sched_barrier(0);
         if (s0 + 1 < ns) sum(1, s0 + 1);
         __builtin_amdgcn_sched_barrier(0);
-        ew_spmv_load(E, row, lane, n32, P, 1, s0 + 4);
+        ew_spmv_load(E, lane, P, 1, s0 + 4);
         __builtin_amdgcn_sched_barrier(0);
         if (s0 + 2 < ns) sum(2, s0 + 2);
         __builtin_amdgcn_sched_barrier(0);
@@ -1434,14 +1472,20 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     const int n32 = MIXED ? E.w32[row] : 0;
     const bool lds = MIXED && n32 > 0 && E.lds_tiles > 0;
     float2* s_b = reinterpret_cast<float2*>(s_a + (size_t)E.lds_tiles * 8);
+    const int2 rows = E.ccnt[(size_t)row * kTile + lane];   // the lane's FP32 entries and FP64 rows
     EwSpmvPipe P;
+    P.base = ew_base32(row, E.cap32);
+    P.rs = 0;
+    P.rows = (rows.x + 1) >> 1;
+    int minrows = P.rows;
     double4 oI = make_double4(0.0, 0.0, 0.0, 0.0);
     float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
     if (MIXED && n32 > 0) {
-        ew_spmv_load(E, row, lane, n32, P, 0, 0);
-        ew_spmv_load(E, row, lane, n32, P, 1, 1);
+        ew_spmv_load(E, lane, P, 0, 0);
+        ew_spmv_load(E, lane, P, 1, 1);
         oI = E.tb_o[I];
         ui = E.posl[I * kTile + lane];
+        for (int off = 32; off > 0; off >>= 1) minrows = min(minrows, __shfl_xor(minrows, off, 64));
     }
     if (lds) {
         // entry 8 t + m: slot 8w + m of neighbour tile t (the wave's own LDS: its operations
@@ -1500,13 +1544,16 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     double ax = 0.0, ay = 0.0, az = 0.0;
     {
         const int n64 = E.w64[row];
-        const size_t base = (size_t)row * E.cap64 * kTile + lane;
+        const size_t base = ew_base64(row, E.cap64);
+        int rs = 0;   // start of the next row; a lane past its list reads the list's first entry
         for (int n0 = 0; n0 < n64; n0 += 4) {
             int j[4];
             double4 t[4], v[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                const size_t at = base + (size_t)min(n0 + u, n64 - 1) * kTile;
+                const unsigned long long m = __builtin_amdgcn_ballot_w64(n0 + u < rows.y);
+                const size_t at = base + (((m >> lane) & 1ull) ? (size_t)(rs + ew_lane_rank(m)) : 0);
+                rs += ew_row_slots64(m);
                 j[u] = E.idx64[at];
                 t[u] = E.t64[at];
             }
@@ -1523,8 +1570,8 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
                 }
             }
 #pragma unroll
-            for (int u = 0; u < 4; u++) {   // entries past the end: the last one with weight zero
-                const double on = n0 + u < n64 ? 1.0 : 0.0;
+            for (int u = 0; u < 4; u++) {   // rows past the lane's list with weight zero
+                const double on = n0 + u < rows.y ? 1.0 : 0.0;
                 double p = t[u].y * v[u].x + t[u].z * v[u].y + t[u].w * v[u].z;
                 p = j[u] < 0 ? -on * p : on * p;
                 const double c = on * t[u].x;
@@ -1536,8 +1583,8 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     }
     if (MIXED && n32 > 0) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        if (lds) ew_spmv_fp32<true>(E, row, w, lane, n32, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
-        else ew_spmv_fp32<false>(E, row, w, lane, n32, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
+        if (lds) ew_spmv_fp32<true>(E, w, lane, n32, minrows, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
+        else ew_spmv_fp32<false>(E, w, lane, n32, minrows, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
         ax += fx;
         ay += fy;
         az += fz;

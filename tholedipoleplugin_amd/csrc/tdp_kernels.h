@@ -226,7 +226,9 @@ struct EwLaunch {
     // pass forms d = u_i - y_j in FP32 from the slots' offsets: u_i from tile I's origin (posl),
     // y_j = u_j + D_IJ in FP64 rounded once, D_IJ the minimum image of the tile origins, which every
     // pair of the tile pair within the cutoff takes (ew_tile_image_ok).  FP32 entries are stored in
-    // pairs per lane (ew_pair32) and their list widths are even.
+    // pairs per lane and their list widths are even.  The lists are jagged (row n holds only the
+    // lanes whose list reaches it, packed; the per-lane lengths are ccnt): the T v pass streams no
+    // padding.
     int cap32, cap64;
     unsigned short* idx16;
     float2* t32;
@@ -234,7 +236,7 @@ struct EwLaunch {
     double4* t64;
     int* w32;               // [nblk kEwWaves] list widths
     int* w64;
-    int* need;              // [7] largest width (FP32, FP64), entries (FP32, FP64), summed widths
+    int* need;              // [7] largest width (FP32, FP64), entries (FP32, FP64), stored slots
                             // (FP32, FP64), largest neighbour-tile count
     int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
     const double4* tb_o;    // [nblk] tile origins (FP64)

@@ -296,3 +296,17 @@ def test_box36864_every_site():
     dfm = rel_rms(sm.getForces(), st.getForces())
     _say("  mixed vs double: mu max %.3e e*nm, energy rel %.3e, force rel RMS %.3e" % (dmu, dem, dfm))
     assert dmu <= MU_TOL and dem <= E_TOL and dfm <= F_TOL
+
+
+def test_device_entry_point_and_late_list_check():
+    """tdp_execute_device with the Ewald option (list counts examined at the solver's first host
+    check, forces enqueued ahead of it) against tdp_execute, and a compressed box whose stored lists
+    overflow on a context sized for the first box (the evaluation runs again with grown lists)."""
+    import subprocess
+    import sys
+    _say("case ewald device entry point, late list check")
+    script = os.path.join(os.path.dirname(os.path.abspath(__file__)), "ewald_device_check.py")
+    r = subprocess.run([sys.executable, "-u", script], capture_output=True, text=True, timeout=600)
+    print(r.stdout[-4000:])
+    print(r.stderr[-2000:])
+    assert r.returncode == 0 and "OK" in r.stdout

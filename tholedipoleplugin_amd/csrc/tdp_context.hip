@@ -119,6 +119,10 @@ struct tdp_context {
     hipStream_t stream2 = nullptr;
     float4 *d_mu32 = nullptr, *d_slot32 = nullptr;   // the force pass's FP32 slot records (mixed)
     hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    // the solver's host check waits on this event (recorded after its copy of the solver scalars), so
+    // that work enqueued behind the copy (an evaluation's forces, ahead of the check) does not hold
+    // the check up
+    hipEvent_t ev_stage = nullptr;
     // Captured chunks of solver iterations (hipGraph), replayed while the launch parameters hold:
     // same chunk length, solver and precision (the pair kernel and the vectors it reads), same box,
     // same device tables (graph_gen counts table uploads).
@@ -228,6 +232,10 @@ struct tdp_context {
     int2* d_ccnt = nullptr;
     int64_t ew_entries[2] = {0, 0}, ew_slots[2] = {0, 0};   // stored entries and padded slots of the last build
     void* h_need = nullptr;   // pinned copy of d_need
+    // the list build's counts are examined at the solver's first host check instead of right after
+    // the build (ew_defer, set by the caller for one evaluation); ew_need_pending: h_need holds the
+    // counts of a build not yet examined
+    bool ew_defer = false, ew_need_pending = false;
     int last_max_iter = 0;   // largest iteration count of the previous Mutual solve
     // warm start of the Mutual solve from the previous evaluation's induced dipoles (caller's order)
     bool warm_start = false, have_prev = false;
@@ -884,15 +892,22 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
         const double box9[9] = {L.box[0], 0, 0, 0, L.box[1], 0, 0, 0, L.box[2]};
         const EwLaunch E = ew_launch(c, box9);
         if (c->ew_stored) {
-            // the reciprocal chain (spreading, FFTs) on stream2 beside the stored-block T v pass; the
-            // epilogue, which gathers from its potential grid, waits for both
+            // the reciprocal chain (spreading, FFTs) beside the stored-block T v pass, on two streams;
+            // the epilogue, which gathers from its potential grid, waits for both, on the main
+            // stream.  In mixed precision the chain, whose passes run in the pass's leftover
+            // resources, takes the main stream and the pass the other (4.15 vs 4.21 ms per
+            // evaluation at 36,864 sites); in FP64 mode the pass (2.7x longer) ends last by far and
+            // keeps the main stream.
+            const bool mixed = c->precision == TDP_PRECISION_MIXED;
+            hipStream_t ss = mixed ? c->stream2 : c->stream;   // the T v pass
+            hipStream_t sr = mixed ? c->stream : c->stream2;   // the chain
             hipEventRecord(c->ev_fork, c->stream);
             hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
-            launch_ew_recip(c->stream2, E, c->posa, nullptr, v64, E.phi_v, scalars);
+            launch_ew_recip(sr, E, c->posa, nullptr, v64, E.phi_v, scalars);
+            if (timed) hipEventRecord(e0, ss);
+            launch_ew_spmv(ss, E, v64, mixed ? c->vec32 : nullptr, scalars);
+            if (timed) hipEventRecord(e1, ss);
             hipEventRecord(c->ev_join, c->stream2);
-            if (timed) hipEventRecord(e0, c->stream);
-            launch_ew_spmv(c->stream, E, v64, c->precision == TDP_PRECISION_MIXED ? c->vec32 : nullptr, scalars);
-            if (timed) hipEventRecord(e1, c->stream);
             hipStreamWaitEvent(c->stream, c->ev_join, 0);
             launch_ew_tmu_epilogue(c->stream, E, c->posa, c->d_inva, c->d_charge, v64, scalars, c->Emv, pq);
         } else {
@@ -941,12 +956,20 @@ size_t stage_upload_doubles(const tdp_context* c) { return (size_t)SC_COUNT * c-
 double* stage_upload(tdp_context* c) { return static_cast<double*>(c->h_stage) + stage_download_doubles(c); }
 
 // Copies the solver scalars to the pinned staging buffer (one copy, one host round trip); done is
-// the number of systems whose stop flag is set.
-int stage_solver_state(tdp_context* c, int& done) {
+// the number of systems whose stop flag is set.  ahead (may be null) enqueues work behind the copy
+// before the host waits, for the copy alone (an event).
+int stage_solver_state(tdp_context* c, int& done, const std::function<int()>* ahead = nullptr) {
     double* h = static_cast<double*>(c->h_stage);
     TDP_HIP(c, hipMemcpyAsync(h + 1, c->d_scalars, sizeof(double) * SC_COUNT * c->nseg, hipMemcpyDeviceToHost,
                               c->stream));
-    TDP_HIP(c, hipStreamSynchronize(c->stream));
+    if (ahead && c->ev_stage) {
+        TDP_HIP(c, hipEventRecord(c->ev_stage, c->stream));
+        int rc = (*ahead)();
+        if (rc) return rc;
+        TDP_HIP(c, hipEventSynchronize(c->ev_stage));
+    } else {
+        TDP_HIP(c, hipStreamSynchronize(c->stream));
+    }
     done = 0;
     for (int g = 0; g < c->nseg; g++) done += h[1 + (size_t)SC_COUNT * g + SC_DONE] != 0.0 ? 1 : 0;
     return TDP_OK;
@@ -1012,6 +1035,46 @@ int start_solve(tdp_context* c, bool jacobi) {
 
 
 // setup() and the permanent field (ReferenceTholeDipoleForce.cpp:1117-1155, :735-746) from
+// Internal status of a solve whose list build was examined late and missed pairs (never returned
+// to a caller: the evaluation runs again).
+constexpr int kEwRedo = -1000;
+
+// The counts of the last list build (h_need, copied after it): stored entries and slots, the
+// neighbour tiles the T v pass stages in LDS, and whether the lists held every pair.  Returns true
+// when they did; else either the lists' indices cannot address the rows' neighbour tiles
+// (ew_stored is now false: matrix-free T v passes from here on) or c32, c64 are the capacities a
+// new build needs.
+bool ew_take_need(tdp_context* c, int& c32, int& c64) {
+    const int* hn = static_cast<const int*>(c->h_need);
+    c->ew_need_pending = false;
+    for (int k = 0; k < 2; k++) {
+        c->ew_entries[k] = hn[2 + k];
+        c->ew_slots[k] = hn[4 + k];
+    }
+    const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
+    if (hn[6] > kEwMaxLocalTiles) {
+        // the list indices (FP32 lists, and the FP64 lists of FP64 mode) cannot address this many
+        // neighbour tiles
+        c->ew_stored = false;
+        c->graph_gen++;
+        return false;
+    }
+    {   // LDS copy of the slot data for the FP32 lists (FP64 mode: v of the FP64 lists, also 24 B per
+        // slot): the wave's slots of up to lds_tiles tiles, 64 KB at most.  A pass launched with
+        // fewer stages a row with more neighbour tiles from HBM instead.
+        int lt = (hn[6] + 7) / 8 * 8;
+        if ((size_t)lt * 8 * kEwSlotLds > 65536) lt = 0;
+        if (lt != c->ew_lds_tiles) {
+            c->ew_lds_tiles = lt;
+            c->graph_gen++;
+        }
+    }
+    if ((!mixed_lists || hn[0] <= c->ew_cap32) && hn[1] <= c->ew_cap64) return true;
+    c32 = mixed_lists ? std::max(c->ew_cap32, hn[0] + hn[0] / 4 + 8) : 0;
+    c64 = std::max(c->ew_cap64, hn[1] + hn[1] / 4 + 8);
+    return false;
+}
+
 // positions already in d_pos3 (caller's order): internal order, lab-frame dipoles, E0 and
 // mu = alpha E0.  host_pos3 (may be NULL) are the same positions on the host, used when the
 // internal order is recomputed.
@@ -1090,34 +1153,16 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
             }
             if (!c->ew_stored) break;
             // the lists must have held every pair: else grow them and build again
-            int* hn = static_cast<int*>(c->h_need);
-            TDP_HIP(c, hipMemcpyAsync(hn, c->d_need, 7 * sizeof(int), hipMemcpyDeviceToHost, s));
-            TDP_HIP(c, hipStreamSynchronize(s));
-            for (int k = 0; k < 2; k++) {
-                c->ew_entries[k] = hn[2 + k];
-                c->ew_slots[k] = hn[4 + k];
-            }
-            const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
-            if (hn[6] > kEwMaxLocalTiles) {
-                // the list indices (FP32 lists, and the FP64 lists of FP64 mode) cannot address this
-                // many neighbour tiles: matrix-free T v passes from now on (this pass's E0 is complete)
-                c->ew_stored = false;
-                c->graph_gen++;
+            TDP_HIP(c, hipMemcpyAsync(c->h_need, c->d_need, 7 * sizeof(int), hipMemcpyDeviceToHost, s));
+            if (c->ew_defer) {   // examined at the solver's first host check (ew_take_need)
+                c->ew_need_pending = true;
                 break;
             }
-            {   // LDS copy of the slot data for the FP32 lists (FP64 mode: v of the FP64 lists, also 24 B
-                // per slot): the wave's slots of up to lds_tiles tiles, 64 KB at most
-                int lt = (hn[6] + 7) / 8 * 8;
-                if ((size_t)lt * 8 * kEwSlotLds > 65536) lt = 0;
-                if (lt != c->ew_lds_tiles) {
-                    c->ew_lds_tiles = lt;
-                    c->graph_gen++;
-                }
-            }
-            if ((!mixed_lists || hn[0] <= c->ew_cap32) && hn[1] <= c->ew_cap64) break;
+            TDP_HIP(c, hipStreamSynchronize(s));
+            int c32 = 0, c64 = 0;
+            if (ew_take_need(c, c32, c64)) break;
+            if (!c->ew_stored) break;   // matrix-free T v passes from now on (this pass's E0 is complete)
             if (attempt > 0) return fail(c, TDP_ERR_HIP, "Ewald option: the stored T lists could not be sized");
-            const int c32 = mixed_lists ? std::max(c->ew_cap32, hn[0] + hn[0] / 4 + 8) : 0;
-            const int c64 = std::max(c->ew_cap64, hn[1] + hn[1] / 4 + 8);
             if ((rc = ew_alloc_lists(c, c32, c64))) return rc;
         }
         launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
@@ -1221,7 +1266,12 @@ int run_iterations(tdp_context* c, const PairLaunch& L, const SegLaunch& G, bool
     return TDP_OK;
 }
 
-int solve_mutual(tdp_context* c, const double* box) {
+// ahead (may be null): the rest of the evaluation, enqueued behind the first chunk of iterations
+// before the host check; *ahead_stands tells whether that chunk finished the solve (else the caller
+// enqueues the rest again).  With the list build's counts pending (ew_defer), the check examines
+// them too and returns kEwRedo when the lists missed pairs: the evaluation must run again.
+int solve_mutual(tdp_context* c, const double* box, const std::function<int()>* ahead = nullptr,
+                 bool* ahead_stands = nullptr) {
     hipStream_t s = c->stream;
     PairLaunch L = pair_launch(c, box);
     SegLaunch G = seg_launch(c);
@@ -1247,7 +1297,16 @@ int solve_mutual(tdp_context* c, const double* box) {
     // no host check before the first chunk: systems already stopped make its launches return at once
     for (int pass = 0;; pass++) {
         if ((rc = run_iterations(c, L, G, jacobi, pass == 0 ? first_chunk : chunk))) return rc;
-        if ((rc = stage_solver_state(c, done))) return rc;
+        if ((rc = stage_solver_state(c, done, pass == 0 ? ahead : nullptr))) return rc;
+        if (pass == 0 && ahead_stands) *ahead_stands = ahead && done >= c->nseg;
+        if (pass == 0 && c->ew_need_pending) {
+            int c32 = 0, c64 = 0;
+            if (!ew_take_need(c, c32, c64)) {
+                if (c->ew_stored && (rc = ew_alloc_lists(c, c32, c64))) return rc;
+                if (ahead_stands) *ahead_stands = false;
+                return kEwRedo;
+            }
+        }
         if (done >= c->nseg) break;
     }
     if ((rc = read_solver_results(c, jacobi, true))) return rc;
@@ -1319,7 +1378,9 @@ int solve_captured(tdp_context* c, const double* box, const double* host_pos3,
 }
 
 // setup() + calculateInducedDipoles (ReferenceTholeDipoleForce.cpp:1117-1155, :735-764).
-int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) {
+int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3,
+                    const std::function<int()>* ahead = nullptr, bool* ahead_stands = nullptr) {
+    if (ahead_stands) *ahead_stands = false;
     if (c->n == 0) {
         std::fill(c->iterations.begin(), c->iterations.end(), 0);
         std::fill(c->converged.begin(), c->converged.end(), 1);
@@ -1361,7 +1422,7 @@ int setup_and_solve(tdp_context* c, const double* box, const double* host_pos3) 
     }
     if (c->warm_start && c->have_prev)   // start from the previous solution instead of alpha E0
         launch_gather_vec3(s, npad, c->d_perm, c->d_mu_prev, c->mu, mixed ? c->vec32 : nullptr);
-    rc = solve_mutual(c, box);
+    rc = solve_mutual(c, box, ahead, ahead_stands);
     if (rc) return rc;
     if (c->warm_start) {
         launch_scatter_vec3(s, npad, c->d_perm, c->mu, c->d_mu_prev);
@@ -1661,7 +1722,8 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
     }
     if (c->ewald && (hipStreamCreate(&c->stream2) != hipSuccess ||
                      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-                     hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)) {
+                     hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreateWithFlags(&c->ev_stage, hipEventDisableTiming) != hipSuccess)) {
         c->err = "hipStreamCreate / hipEventCreate failed";
         return bail(TDP_ERR_HIP);
     }
@@ -2117,6 +2179,7 @@ void tdp_destroy(tdp_context* c) {
     if (c->stream2) hipStreamSynchronize(c->stream2);
     if (c->ev_fork) hipEventDestroy(c->ev_fork);
     if (c->ev_join) hipEventDestroy(c->ev_join);
+    if (c->ev_stage) hipEventDestroy(c->ev_stage);
     if (c->stream2) hipStreamDestroy(c->stream2);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -2173,6 +2236,10 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
         TDP_HIP(c, hipMemcpyAsync(c->d_pos3, d_positions, 3 * c->n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
     // forces and energy into the caller's buffers, guarded by the stop count on the captured path
     const bool seq = sequence_graphs(c);
+    auto compute = [&]() -> int {
+        forces_and_energy(c, box, d_forces != nullptr, d_energy != nullptr);
+        return TDP_OK;
+    };
     auto back = [&]() -> int {
         forces_and_energy(c, box, d_forces != nullptr, d_energy != nullptr);
         if (d_forces && c->n > 0)
@@ -2187,10 +2254,27 @@ int tdp_execute_device(tdp_context* c, const double* d_positions, const double* 
         return TDP_OK;
     };
     bool back_ran = false;
+    c->ew_need_pending = false;
     if (seq) {
         const PairLaunch L = pair_launch(c, box);
         auto back_graph = [&]() -> int { return replay_or_capture(c, kSeqBack, 0, d_forces, d_energy, L.box, back); };
         if ((rc = solve_captured(c, box, nullptr, back_graph, &back_ran)) == TDP_OK && !back_ran) rc = back_graph();
+    } else if (c->ewald && c->ew_stored && c->polarization == TDP_MUTUAL && !c->timing) {
+        // Ewald option with stored lists: the list build's counts are examined at the solver's first
+        // host check, and the forces and energy (into the context's buffers) are enqueued behind the
+        // first chunk of iterations before that check; they stand when that chunk finished the solve
+        // (else they run again).  The copies to the caller follow the check.
+        const std::function<int()> ahead = compute;
+        c->ew_defer = true;
+        rc = setup_and_solve(c, box, nullptr, &ahead, &back_ran);
+        c->ew_defer = false;
+        if (rc == kEwRedo) rc = setup_and_solve(c, box, nullptr);   // lists regrown: build again, synchronously
+        if (rc == TDP_OK && !back_ran) rc = compute();
+        if (rc == TDP_OK && d_forces && c->n > 0)
+            launch_scatter_accumulate(c->stream, c->npad, c->d_perm, c->d_frc, d_forces, nullptr, c->nseg);
+        if (rc == TDP_OK && d_energy)
+            TDP_HIP(c, hipMemcpyAsync(d_energy, c->d_energy, c->nseg * sizeof(double), hipMemcpyDeviceToDevice,
+                                      c->stream));
     } else {
         rc = setup_and_solve(c, box, nullptr);
         if (rc == TDP_OK) rc = back();

@@ -1237,7 +1237,9 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_force_cand(EwLaunch E, c
         extern __shared__ double4 s_D[];   // [E.lds_tiles] (shared by the workgroup's waves: one row)
         const int cnt = E.nbr_cnt[I];
         const double4 oI = E.tb_o[I];
-        const bool lds = E.lds_tiles > 0;
+        // a row with more neighbour tiles than the launch's LDS holds (the host sizes it from the
+        // previous list build) forms its shifts from the tile origins instead
+        const bool lds = E.lds_tiles > 0 && cnt <= E.lds_tiles;
         if (lds) {
             for (int t = threadIdx.x; t < cnt; t += blockDim.x) {
                 const double4 oJ = E.tb_o[nb[t]];
@@ -1470,7 +1472,9 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     const int I = row / kEwWaves, w = row % kEwWaves;
     const int* nb = E.nbr + (size_t)I * E.nblk;
     const int n32 = MIXED ? E.w32[row] : 0;
-    const bool lds = MIXED && n32 > 0 && E.lds_tiles > 0;
+    // a row with more neighbour tiles than the launch's LDS holds (sized from the previous list build)
+    // gathers its slot data from HBM
+    const bool lds = MIXED && n32 > 0 && E.lds_tiles > 0 && E.nbr_cnt[I] <= E.lds_tiles;
     float2* s_b = reinterpret_cast<float2*>(s_a + (size_t)E.lds_tiles * 8);
     const int2 rows = E.ccnt[(size_t)row * kTile + lane];   // the lane's FP32 entries and FP64 rows
     EwSpmvPipe P;
@@ -1520,7 +1524,7 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     }
     // FP64 mode: the FP64 lists index the wave's slots (8 t + m) and their v_j (FP64, 24 B per slot)
     // are staged in LDS like the FP32 lists' slot data; mixed: j itself, v_j from memory
-    const bool lds64 = !MIXED && E.lds_tiles > 0;
+    const bool lds64 = !MIXED && E.lds_tiles > 0 && E.nbr_cnt[I] <= E.lds_tiles;
     double2* s_vxy = reinterpret_cast<double2*>(s_a);
     double* s_vz = reinterpret_cast<double*>(s_vxy + (size_t)E.lds_tiles * 8);
     if (lds64) {
@@ -1810,18 +1814,19 @@ void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const doubl
     const int n = E.K[axis];
     const int K2h = E.K[2] / 2 + 1;   // the complex grid's z extent (k_ew_fft)
     const int nlines = axis == 2 ? E.K[0] * E.K[1] : (axis == 1 ? E.K[0] * K2h : E.K[1] * K2h);
-    // lines per workgroup: at most 24 KB per LDS buffer, and enough workgroups (512) to spread over the CUs
-    int LB = (int)(24576 / sizeof(C)) / n;
-    LB = std::min(LB, std::max(1, nlines / 512));
-    LB = std::max(1, std::min(LB, 32));
+    // Two lines per one-wave workgroup (4 KB of LDS for 64-point FP64 lines): the passes run beside
+    // the real-space passes, whose workgroups leave a few KB of LDS and a wave slot per CU free, and
+    // small workgroups start there as soon as they are launched (512 workgroups of eight lines, 16 KB,
+    // waited for real-space workgroups to retire: 50-70 us for a pass that takes 8 us alone)
+    int LB = std::max(1, std::min(2, (int)(4096 / sizeof(C)) / n));
     const size_t shm = 2 * (size_t)LB * n * sizeof(C);
-    const dim3 grid((nlines + LB - 1) / LB);
+    const dim3 grid((nlines + LB - 1) / LB), block(64);
     switch (kind) {
-        case 0: hipLaunchKernelGGL((k_ew_fft<T, 0>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
-        case 1: hipLaunchKernelGGL((k_ew_fft<T, 1>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
-        case 2: hipLaunchKernelGGL((k_ew_fft<T, 2>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
-        case 3: hipLaunchKernelGGL((k_ew_fft<T, 3>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
-        default: hipLaunchKernelGGL((k_ew_fft<T, 4>), grid, dim3(256), shm, s, E, LB, phi, scalars); break;
+        case 0: hipLaunchKernelGGL((k_ew_fft<T, 0>), grid, block, shm, s, E, LB, phi, scalars); break;
+        case 1: hipLaunchKernelGGL((k_ew_fft<T, 1>), grid, block, shm, s, E, LB, phi, scalars); break;
+        case 2: hipLaunchKernelGGL((k_ew_fft<T, 2>), grid, block, shm, s, E, LB, phi, scalars); break;
+        case 3: hipLaunchKernelGGL((k_ew_fft<T, 3>), grid, block, shm, s, E, LB, phi, scalars); break;
+        default: hipLaunchKernelGGL((k_ew_fft<T, 4>), grid, block, shm, s, E, LB, phi, scalars); break;
     }
 }
 }  // namespace

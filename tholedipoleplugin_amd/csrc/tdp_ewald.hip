@@ -412,7 +412,7 @@ __global__ __launch_bounds__(256) void k_ew_fft(EwLaunch E, int LB, void* phi_ou
 // derivatives in u (xx, yy, zz, xy, xz, yz), phi itself.  NEED: 0 phi and g1, 1 also g2.
 template <typename T, int NEED>
 __device__ __forceinline__ void ew_gather(const EwLaunch& E, const T* __restrict__ phi, double4 p, double& ph,
-                                          double g1[3], double g2[6]) {
+                                          double g1[3], double g2[6], int a0 = 0, int a1 = kEwOrder) {
     int base[3];
     double th[3][kEwOrder], d1[3][kEwOrder], d2[3][kEwOrder];
     const double x[3] = {p.x, p.y, p.z};
@@ -425,7 +425,7 @@ __device__ __forceinline__ void ew_gather(const EwLaunch& E, const T* __restrict
     for (int k = 0; k < 3; k++) g1[k] = 0.0;
     for (int k = 0; k < 6; k++) g2[k] = 0.0;
 #pragma unroll 1
-    for (int a = 0; a < kEwOrder; a++) {
+    for (int a = a0; a < a1; a++) {   // x planes a0 .. a1-1 of the spline's support
         const int gx = ew_wrap(base[0] + a, E.K[0]);
 #pragma unroll 1
         for (int b = 0; b < kEwOrder; b++) {
@@ -1596,120 +1596,151 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     part[(size_t)row * kTile + lane] = make_double4(ax, ay, az, 0.0);
 }
 
-// Epilogue of a pass for the 64 slots of tile I (one wave per tile): the real-space sums, the
-// scaled pairs of each slot in FP64 with their scales ([inside] B_n + (s lambda - 1) b_n), the
-// reciprocal field (E = -(K/L) grad_u phi) or force (q E + grad E . v) gathered from the potential
-// grids, and the self terms; then the outputs of the mode.
-template <int MODE, bool MUTUAL, bool PARTS = false>
-__global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* __restrict__ posa,
-                                                    const double* __restrict__ inva, const double* __restrict__ charge,
-                                                    const double4* __restrict__ vec, const double4* __restrict__ E0,
-                                                    const double* __restrict__ scalars,
-                                                    const double4* __restrict__ rs, double4* __restrict__ out4, PcgQ pq,
-                                                    double* __restrict__ epart, double* __restrict__ frc,
-                                                    double* __restrict__ eout) {
+// Epilogue of a pass for the 64 slots of tile I, on four waves so that their dependent loads
+// overlap: waves 0-2 gather the reciprocal field (E = -(K/L) grad_u phi) or force (q E + grad E . v)
+// from x planes 0-1, 2-3 and 4 of the spline support of the potential grids; wave 3 adds the row
+// parts' real-space sums (in part order), the scaled pairs of each slot in FP64 with their scales
+// ([inside] B_n + (s lambda - 1) b_n) and the self field; wave 0 adds the four parts in wave order
+// and forms the outputs of the mode.
+template <int MODE, bool MUTUAL>
+__global__ __launch_bounds__(256) void k_ew_epilogue(EwLaunch E, const double4* __restrict__ posa,
+                                                     const double* __restrict__ inva, const double* __restrict__ charge,
+                                                     const double4* __restrict__ vec, const double4* __restrict__ E0,
+                                                     const double* __restrict__ scalars,
+                                                     const double4* __restrict__ rs, double4* __restrict__ out4, PcgQ pq,
+                                                     double* __restrict__ epart, double* __restrict__ frc,
+                                                     double* __restrict__ eout) {
     if (scalars && scalars[SC_DONE] != 0.0) return;
-    const int I = blockIdx.x, lane = threadIdx.x;
+    __shared__ double s_part[3][5][kTile];   // waves 1-3: field (3), energy, potential
+    const int I = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int i = I * kTile + lane;
     const bool valid = lane < E.tcnt[I];
     const double4 pi = posa[i];
-    const double iai = inva[i];
     const double qi = charge[i];
     const double4 vi = (MODE == kEwE0) ? make_double4(0.0, 0.0, 0.0, 0.0) : vec[i];
-    double ax, ay, az, ae;
-    if (PARTS) {   // the kEwWaves list partials of the stored T v pass, in list order
-        const double4 r0 = rs[(size_t)(I * kEwWaves) * kTile + lane];
-        ax = r0.x;
-        ay = r0.y;
-        az = r0.z;
-        ae = 0.0;
-        for (int u = 1; u < kEwWaves; u++) {
-            const double4 r = rs[(size_t)(I * kEwWaves + u) * kTile + lane];
-            ax += r.x;
-            ay += r.y;
-            az += r.z;
-        }
-    } else {   // the row parts' sums, in part order
-        const double4 r0 = rs[i];
-        ax = r0.x;
-        ay = r0.y;
-        az = r0.z;
-        ae = r0.w;
-        for (int h = 1; h < (MODE == kEwE0 ? E.e0_parts : ew_parts(MODE)); h++) {
-            const double4 r = rs[(size_t)h * E.npad + i];
-            ax += r.x;
-            ay += r.y;
-            az += r.z;
-            ae += r.w;
-        }
-    }
-    const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
-    if (valid) {
-        for (int q = E.sc_off[i]; q < E.sc_off[i + 1]; q++) {
-            const int j = E.sc_j[q];
-            const double2 sc = E.sc_s[q];
-            const double m = sc.x, is = sc.y;
-            if (MODE == kEwE0 && m == 1.0) continue;
-            if (MODE == kEwTmu && is == 1.0) continue;
-            const double4 pj = posa[j];
-            const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
-            const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
-            const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
-            const double r2 = ew_r2(dx, dy, dz);
-            const bool inside = r2 < E.rc2;
-            EwR<double> R;
-            ew_radial(E, r2, pi.w, pj.w, iai, inva[j], inside, true, R);
-            const bool cm = inside || m != 1.0, ci = inside || is != 1.0;
-            const double Cqm1 = R.B1 + (cm ? (m * R.l3 - 1.0) * R.b1 : 0.0);
-            const double Cqm2 = R.B2 + (cm ? (m * R.l5 - 1.0) * R.b2 : 0.0);
-            const double Cmm1 = R.B1 + (ci ? (is * R.l3 - 1.0) * R.b1 : 0.0);
-            const double Cmm2 = R.B2 + (ci ? (is * R.l5 - 1.0) * R.b2 : 0.0);
-            const double qj = charge[j];
-            if (MODE == kEwE0) {
-                ax += qj * Cqm1 * dx;
-                ay += qj * Cqm1 * dy;
-                az += qj * Cqm1 * dz;
-            } else if (MODE == kEwTmu) {
-                const double4 vj = vec[j];
-                const double vd = vj.x * dx + vj.y * dy + vj.z * dz;
-                ax += Cmm2 * vd * dx - Cmm1 * vj.x;
-                ay += Cmm2 * vd * dy - Cmm1 * vj.y;
-                az += Cmm2 * vd * dz - Cmm1 * vj.z;
-            } else {
-                const double4 vj = vec[j];
-                const double b0 = R.b1 * r2;   // 1/r
-                const double Cqq0 = R.B0 + (cm ? (m - 1.0) * b0 : 0.0);
-                const double Cqq1 = R.B1 + (cm ? (m - 1.0) * R.b1 : 0.0);
-                double gx = -qi * qj * Cqq1 * dx, gy = -qi * qj * Cqq1 * dy, gz = -qi * qj * Cqq1 * dz;
-                const double ux = qi * vj.x - qj * vi.x, uy = qi * vj.y - qj * vi.y, uz = qi * vj.z - qj * vi.z;
-                const double ud = ux * dx + uy * dy + uz * dz;
-                gx += Cqm1 * ux - Cqm2 * ud * dx;
-                gy += Cqm1 * uy - Cqm2 * ud * dy;
-                gz += Cqm1 * uz - Cqm2 * ud * dz;
-                if (MUTUAL) {
-                    const double Cmm3 = R.B3 + (ci ? (is * R.l7 - 1.0) * R.b3 : 0.0);
-                    const double mid = vi.x * dx + vi.y * dy + vi.z * dz;
-                    const double mjd = vj.x * dx + vj.y * dy + vj.z * dz;
-                    const double mm = vi.x * vj.x + vi.y * vj.y + vi.z * vj.z;
-                    const double cd = -Cmm2 * mm + Cmm3 * mid * mjd;
-                    gx += cd * dx - Cmm2 * (vi.x * mjd + vj.x * mid);
-                    gy += cd * dy - Cmm2 * (vi.y * mjd + vj.y * mid);
-                    gz += cd * dz - Cmm2 * (vi.z * mjd + vj.z * mid);
-                }
-                ax -= gx;
-                ay -= gy;
-                az -= gz;
-                ae += 0.5 * qi * qj * Cqq0;
+    double ax = 0.0, ay = 0.0, az = 0.0, ae = 0.0, ph = 0.0;
+    if (w == 3) {
+        {   // the row parts' sums, in part order
+            const double4 r0 = rs[i];
+            ax = r0.x;
+            ay = r0.y;
+            az = r0.z;
+            ae = r0.w;
+            for (int h = 1; h < (MODE == kEwE0 ? E.e0_parts : ew_parts(MODE)); h++) {
+                const double4 r = rs[(size_t)h * E.npad + i];
+                ax += r.x;
+                ay += r.y;
+                az += r.z;
+                ae += r.w;
             }
         }
+        if (MODE == kEwTmu) {
+            ax += E.self_field * vi.x;
+            ay += E.self_field * vi.y;
+            az += E.self_field * vi.z;
+        }
+        const double iai = inva[i];
+        const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
+        if (valid) {
+            for (int q = E.sc_off[i]; q < E.sc_off[i + 1]; q++) {
+                const int j = E.sc_j[q];
+                const double2 sc = E.sc_s[q];
+                const double m = sc.x, is = sc.y;
+                if (MODE == kEwE0 && m == 1.0) continue;
+                if (MODE == kEwTmu && is == 1.0) continue;
+                const double4 pj = posa[j];
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const double r2 = ew_r2(dx, dy, dz);
+                const bool inside = r2 < E.rc2;
+                EwR<double> R;
+                ew_radial(E, r2, pi.w, pj.w, iai, inva[j], inside, true, R);
+                const bool cm = inside || m != 1.0, ci = inside || is != 1.0;
+                const double Cqm1 = R.B1 + (cm ? (m * R.l3 - 1.0) * R.b1 : 0.0);
+                const double Cqm2 = R.B2 + (cm ? (m * R.l5 - 1.0) * R.b2 : 0.0);
+                const double Cmm1 = R.B1 + (ci ? (is * R.l3 - 1.0) * R.b1 : 0.0);
+                const double Cmm2 = R.B2 + (ci ? (is * R.l5 - 1.0) * R.b2 : 0.0);
+                const double qj = charge[j];
+                if (MODE == kEwE0) {
+                    ax += qj * Cqm1 * dx;
+                    ay += qj * Cqm1 * dy;
+                    az += qj * Cqm1 * dz;
+                } else if (MODE == kEwTmu) {
+                    const double4 vj = vec[j];
+                    const double vd = vj.x * dx + vj.y * dy + vj.z * dz;
+                    ax += Cmm2 * vd * dx - Cmm1 * vj.x;
+                    ay += Cmm2 * vd * dy - Cmm1 * vj.y;
+                    az += Cmm2 * vd * dz - Cmm1 * vj.z;
+                } else {
+                    const double4 vj = vec[j];
+                    const double b0 = R.b1 * r2;   // 1/r
+                    const double Cqq0 = R.B0 + (cm ? (m - 1.0) * b0 : 0.0);
+                    const double Cqq1 = R.B1 + (cm ? (m - 1.0) * R.b1 : 0.0);
+                    double gx = -qi * qj * Cqq1 * dx, gy = -qi * qj * Cqq1 * dy, gz = -qi * qj * Cqq1 * dz;
+                    const double ux = qi * vj.x - qj * vi.x, uy = qi * vj.y - qj * vi.y, uz = qi * vj.z - qj * vi.z;
+                    const double ud = ux * dx + uy * dy + uz * dz;
+                    gx += Cqm1 * ux - Cqm2 * ud * dx;
+                    gy += Cqm1 * uy - Cqm2 * ud * dy;
+                    gz += Cqm1 * uz - Cqm2 * ud * dz;
+                    if (MUTUAL) {
+                        const double Cmm3 = R.B3 + (ci ? (is * R.l7 - 1.0) * R.b3 : 0.0);
+                        const double mid = vi.x * dx + vi.y * dy + vi.z * dz;
+                        const double mjd = vj.x * dx + vj.y * dy + vj.z * dz;
+                        const double mm = vi.x * vj.x + vi.y * vj.y + vi.z * vj.z;
+                        const double cd = -Cmm2 * mm + Cmm3 * mid * mjd;
+                        gx += cd * dx - Cmm2 * (vi.x * mjd + vj.x * mid);
+                        gy += cd * dy - Cmm2 * (vi.y * mjd + vj.y * mid);
+                        gz += cd * dz - Cmm2 * (vi.z * mjd + vj.z * mid);
+                    }
+                    ax -= gx;
+                    ay -= gy;
+                    az -= gz;
+                    ae += 0.5 * qi * qj * Cqq0;
+                }
+            }
+        }
+    } else {
+        const int a0 = 2 * w, a1 = w == 2 ? kEwOrder : 2 * w + 2;   // x planes of the support
+        const double sxk = E.K[0] / E.L[0], syk = E.K[1] / E.L[1], szk = E.K[2] / E.L[2];
+        double g1[3], g2[6];
+        if (MODE == kEwE0 || MODE == kEwTmu) {
+            ew_gather<double, 0>(E, reinterpret_cast<const double*>(MODE == kEwE0 ? E.phi_q : E.phi_v), pi, ph, g1,
+                                 g2, a0, a1);
+            ax = -g1[0] * sxk;
+            ay = -g1[1] * syk;
+            az = -g1[2] * szk;
+        } else {
+            double gv[3], hv[6], phv;
+            ew_gather<double, 1>(E, reinterpret_cast<const double*>(E.phi_q), pi, ph, g1, g2, a0, a1);
+            ew_gather<double, 1>(E, reinterpret_cast<const double*>(E.phi_v), pi, phv, gv, hv, a0, a1);
+            const double Ex = -(g1[0] + gv[0]) * sxk, Ey = -(g1[1] + gv[1]) * syk, Ez = -(g1[2] + gv[2]) * szk;
+            double h[6];
+            for (int k = 0; k < 6; k++) h[k] = g2[k] + (MUTUAL ? hv[k] : 0.0);
+            const double hxx = -h[0] * sxk * sxk, hyy = -h[1] * syk * syk, hzz = -h[2] * szk * szk;
+            const double hxy = -h[3] * sxk * syk, hxz = -h[4] * sxk * szk, hyz = -h[5] * syk * szk;
+            ax = qi * Ex + hxx * vi.x + hxy * vi.y + hxz * vi.z;
+            ay = qi * Ey + hxy * vi.x + hyy * vi.y + hyz * vi.z;
+            az = qi * Ez + hxz * vi.x + hyz * vi.y + hzz * vi.z;
+        }
     }
-    const double sxk = E.K[0] / E.L[0], syk = E.K[1] / E.L[1], szk = E.K[2] / E.L[2];
-    double ph, g1[3], g2[6];
+    if (w > 0) {
+        s_part[w - 1][0][lane] = ax;
+        s_part[w - 1][1][lane] = ay;
+        s_part[w - 1][2][lane] = az;
+        s_part[w - 1][3][lane] = ae;
+        s_part[w - 1][4][lane] = ph;
+    }
+    __syncthreads();
+    if (w != 0) return;
+    for (int u = 0; u < 3; u++) {
+        ax += s_part[u][0][lane];
+        ay += s_part[u][1][lane];
+        az += s_part[u][2][lane];
+        ae += s_part[u][3][lane];
+        ph += s_part[u][4][lane];
+    }
     if (MODE == kEwE0) {
-        ew_gather<double, 0>(E, reinterpret_cast<const double*>(E.phi_q), pi, ph, g1, g2);
-        ax -= g1[0] * sxk;
-        ay -= g1[1] * syk;
-        az -= g1[2] * szk;
         if (valid) {
             out4[i] = make_double4(ax, ay, az, 0.0);
             epart[i] = 0.5 * qi * ph + E.self_energy * qi * qi;
@@ -1718,10 +1749,6 @@ __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* _
             epart[i] = 0.0;
         }
     } else if (MODE == kEwTmu) {
-        ew_gather<double, 0>(E, reinterpret_cast<const double*>(E.phi_v), pi, ph, g1, g2);
-        ax += -g1[0] * sxk + E.self_field * vi.x;
-        ay += -g1[1] * syk + E.self_field * vi.y;
-        az += -g1[2] * szk + E.self_field * vi.z;
         if (!valid) ax = ay = az = 0.0;
         if (pq.q) {   // PCG: q = p/alpha - T p and the tile sum of p.q
             const double4 p = pq.p[i];
@@ -1735,17 +1762,6 @@ __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* _
             out4[i] = make_double4(ax, ay, az, 0.0);
         }
     } else {
-        double gv[3], hv[6], phv;
-        ew_gather<double, 1>(E, reinterpret_cast<const double*>(E.phi_q), pi, ph, g1, g2);
-        ew_gather<double, 1>(E, reinterpret_cast<const double*>(E.phi_v), pi, phv, gv, hv);
-        const double Ex = -(g1[0] + gv[0]) * sxk, Ey = -(g1[1] + gv[1]) * syk, Ez = -(g1[2] + gv[2]) * szk;
-        double h[6];
-        for (int k = 0; k < 6; k++) h[k] = g2[k] + (MUTUAL ? hv[k] : 0.0);
-        const double hxx = -h[0] * sxk * sxk, hyy = -h[1] * syk * syk, hzz = -h[2] * szk * szk;
-        const double hxy = -h[3] * sxk * syk, hxz = -h[4] * sxk * szk, hyz = -h[5] * syk * szk;
-        ax += qi * Ex + hxx * vi.x + hxy * vi.y + hxz * vi.z;
-        ay += qi * Ey + hxy * vi.x + hyy * vi.y + hyz * vi.z;
-        az += qi * Ez + hxz * vi.x + hyz * vi.y + hzz * vi.z;
         if (valid) {
             const double4 e0 = E0[i];
             frc[3 * i] = ax;
@@ -1759,16 +1775,30 @@ __global__ __launch_bounds__(64) void k_ew_epilogue(EwLaunch E, const double4* _
     }
 }
 
-// Energy: slots summed in a fixed order by one workgroup, plus the background term.
-__global__ __launch_bounds__(256) void k_ew_energy(int npad, const double* __restrict__ eout, double background,
-                                                   double* __restrict__ energy) {
-    __shared__ double s[4];
+// Energy: slots summed in a fixed order by one workgroup of 1024 threads (each thread its slots,
+// eight loads in flight at a time, then the waves' sums in wave order), plus the background term.
+__global__ __launch_bounds__(1024) void k_ew_energy(int npad, const double* __restrict__ eout, double background,
+                                                    double* __restrict__ energy) {
+    __shared__ double s[16];
     double acc = 0.0;
-    for (int i = threadIdx.x; i < npad; i += blockDim.x) acc += eout[i];
+    for (int i0 = threadIdx.x; i0 < npad; i0 += 8 * blockDim.x) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int i = i0 + u * blockDim.x;
+            v[u] = i < npad ? eout[i] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; u++) acc += v[u];
+    }
     acc = wave_sum(acc);
     if ((threadIdx.x & 63) == 0) s[threadIdx.x / 64] = acc;
     __syncthreads();
-    if (threadIdx.x == 0) energy[0] = ((s[0] + s[1]) + (s[2] + s[3])) + background;
+    if (threadIdx.x == 0) {
+        double e = 0.0;
+        for (int k = 0; k < (int)(blockDim.x / 64); k++) e += s[k];
+        energy[0] = e + background;
+    }
 }
 
 // Every pair within the cutoff, from the side of its lower caller index.
@@ -1798,6 +1828,101 @@ __global__ __launch_bounds__(64) void k_ew_dump_pairs(EwLaunch E, const double4*
                 pairs[2 * at + 1] = oj;
             }
         }
+    }
+}
+
+// Epilogue of the stored-block T v pass for the 64 slots of tile I, on four waves so that its
+// dependent loads overlap: waves 0 and 1 gather the reciprocal field from x planes 0-1 and 2-3 of
+// the spline support, wave 2 from plane 4 and adds the kEwWaves list partials (in list order),
+// wave 3 the scaled pairs (FP64, [inside] B_n + (s lambda - 1) b_n) and the self field; wave 0 adds
+// the four parts in wave order and forms the outputs (q = p/alpha - T p and the tile sum of p.q for
+// PCG, else T p).
+__global__ __launch_bounds__(256) void k_ew_tmu_epilogue(EwLaunch E, const double4* __restrict__ posa,
+                                                         const double* __restrict__ inva,
+                                                         const double4* __restrict__ vec,
+                                                         const double* __restrict__ scalars,
+                                                         const double4* __restrict__ rs, double4* __restrict__ out4,
+                                                         PcgQ pq) {
+    if (scalars && scalars[SC_DONE] != 0.0) return;
+    __shared__ double s_part[3][3][kTile];   // waves 1-3: their parts of the field
+    const int I = blockIdx.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = I * kTile + lane;
+    const bool valid = lane < E.tcnt[I];
+    const double4 pi = posa[i];
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    if (w <= 2) {
+        if (w == 2) {   // the list partials, in list order
+            const double4 r0 = rs[(size_t)(I * kEwWaves) * kTile + lane];
+            ax = r0.x;
+            ay = r0.y;
+            az = r0.z;
+            for (int u = 1; u < kEwWaves; u++) {
+                const double4 r = rs[(size_t)(I * kEwWaves + u) * kTile + lane];
+                ax += r.x;
+                ay += r.y;
+                az += r.z;
+            }
+        }
+        double ph, g1[3], g2[6];
+        const int a0 = 2 * w, a1 = w == 2 ? kEwOrder : 2 * w + 2;
+        ew_gather<double, 0>(E, reinterpret_cast<const double*>(E.phi_v), pi, ph, g1, g2, a0, a1);
+        ax -= g1[0] * (E.K[0] / E.L[0]);
+        ay -= g1[1] * (E.K[1] / E.L[1]);
+        az -= g1[2] * (E.K[2] / E.L[2]);
+    } else {
+        const double4 vi = vec[i];
+        ax = E.self_field * vi.x;
+        ay = E.self_field * vi.y;
+        az = E.self_field * vi.z;
+        if (valid) {
+            const double iai = inva[i];
+            const double iL0 = 1.0 / E.L[0], iL1 = 1.0 / E.L[1], iL2 = 1.0 / E.L[2];
+            for (int q = E.sc_off[i]; q < E.sc_off[i + 1]; q++) {
+                const double is = E.sc_s[q].y;
+                if (is == 1.0) continue;
+                const int j = E.sc_j[q];
+                const double4 pj = posa[j];
+                const double dx = ew_image_fast(__dsub_rn(pi.x, pj.x), E.L[0], iL0);
+                const double dy = ew_image_fast(__dsub_rn(pi.y, pj.y), E.L[1], iL1);
+                const double dz = ew_image_fast(__dsub_rn(pi.z, pj.z), E.L[2], iL2);
+                const double r2 = ew_r2(dx, dy, dz);
+                const bool inside = r2 < E.rc2;
+                EwR<double> R;
+                ew_radial(E, r2, pi.w, pj.w, iai, inva[j], inside, true, R);
+                const bool ci = inside || is != 1.0;
+                const double Cmm1 = R.B1 + (ci ? (is * R.l3 - 1.0) * R.b1 : 0.0);
+                const double Cmm2 = R.B2 + (ci ? (is * R.l5 - 1.0) * R.b2 : 0.0);
+                const double4 vj = vec[j];
+                const double vd = vj.x * dx + vj.y * dy + vj.z * dz;
+                ax += Cmm2 * vd * dx - Cmm1 * vj.x;
+                ay += Cmm2 * vd * dy - Cmm1 * vj.y;
+                az += Cmm2 * vd * dz - Cmm1 * vj.z;
+            }
+        }
+    }
+    if (w > 0) {
+        s_part[w - 1][0][lane] = ax;
+        s_part[w - 1][1][lane] = ay;
+        s_part[w - 1][2][lane] = az;
+    }
+    __syncthreads();
+    if (w != 0) return;
+    for (int u = 0; u < 3; u++) {
+        ax += s_part[u][0][lane];
+        ay += s_part[u][1][lane];
+        az += s_part[u][2][lane];
+    }
+    if (!valid) ax = ay = az = 0.0;
+    if (pq.q) {   // PCG: q = p/alpha - T p and the tile sum of p.q
+        const double4 p = pq.p[i];
+        const double a = pq.alpha[i];
+        const double ia = a > 0.0 ? 1.0 / a : 0.0;
+        const double4 qv = make_double4(p.x * ia - ax, p.y * ia - ay, p.z * ia - az, 0.0);
+        pq.q[i] = qv;
+        const double d = wave_sum(p.x * qv.x + p.y * qv.y + p.z * qv.z);
+        if (lane == 0) pq.red[I] = d;
+    } else {
+        out4[i] = make_double4(ax, ay, az, 0.0);
     }
 }
 
@@ -1847,7 +1972,7 @@ void launch_ew_pairs(hipStream_t s, const EwLaunch& E, int mode, bool mutual, co
 #define TDP_EW_PAIRS(M, MU, MX) \
     hipLaunchKernelGGL((k_ew_pairs<M, MU, MX>), grid, block, 0, s, E, posa, inva, charge, vec, scalars, E.rs)
 #define TDP_EW_EPI(M, MU)                                                                                      \
-    hipLaunchKernelGGL((k_ew_epilogue<M, MU>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0, scalars, E.rs, \
+    hipLaunchKernelGGL((k_ew_epilogue<M, MU>), rows, dim3(256), 0, s, E, posa, inva, charge, vec, E0, scalars, E.rs, \
                        out4, pq, epart, frc, eout)
     // the permanent field stays in FP64 in both precisions (cancellation of the molecules' charge fields)
     if (mode == kEwE0) {
@@ -1880,7 +2005,7 @@ void launch_ew_e0_stored(hipStream_t s, const EwLaunch& E, const double4* posa, 
     else
         hipLaunchKernelGGL((k_ew_e0_build<false>), grid, block, 0, s, Ep, posa, inva, charge, E.rs);
     if (join) hipStreamWaitEvent(s, join, 0);
-    hipLaunchKernelGGL((k_ew_epilogue<kEwE0, false>), dim3(E.nblk), dim3(64), 0, s, Ep, posa, inva, charge,
+    hipLaunchKernelGGL((k_ew_epilogue<kEwE0, false>), dim3(E.nblk), dim3(256), 0, s, Ep, posa, inva, charge,
                        (const double4*)nullptr, (const double4*)nullptr, (const double*)nullptr, E.rs, out4, PcgQ(),
                        epart, (double*)nullptr, (double*)nullptr);
 }
@@ -1898,13 +2023,13 @@ void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const
         if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<true, true>), grid, block, shm, s, E, posa, inva, charge, vec, E.rs);
         else hipLaunchKernelGGL((k_ew_force_cand<true, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
         if (join) hipStreamWaitEvent(s, join, 0);
-        hipLaunchKernelGGL((k_ew_epilogue<kEwForce, true>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
+        hipLaunchKernelGGL((k_ew_epilogue<kEwForce, true>), rows, dim3(256), 0, s, E, posa, inva, charge, vec, E0,
                            (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
     } else {
         if (E.pairs32) hipLaunchKernelGGL((k_ew_force_cand<false, true>), grid, block, shm, s, E, posa, inva, charge, vec, E.rs);
         else hipLaunchKernelGGL((k_ew_force_cand<false, false>), grid, block, 0, s, E, posa, inva, charge, vec, E.rs);
         if (join) hipStreamWaitEvent(s, join, 0);
-        hipLaunchKernelGGL((k_ew_epilogue<kEwForce, false>), rows, dim3(64), 0, s, E, posa, inva, charge, vec, E0,
+        hipLaunchKernelGGL((k_ew_epilogue<kEwForce, false>), rows, dim3(256), 0, s, E, posa, inva, charge, vec, E0,
                            (const double*)nullptr, E.rs, (double4*)nullptr, PcgQ(), epart, frc, eout);
     }
 }
@@ -1922,13 +2047,12 @@ void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const 
 void launch_ew_tmu_epilogue(hipStream_t s, const EwLaunch& E, const double4* posa, const double* inva,
                             const double* charge, const double4* vec, const double* scalars, double4* out4,
                             const PcgQ& pq) {
-    hipLaunchKernelGGL((k_ew_epilogue<kEwTmu, false, true>), dim3(E.nblk), dim3(64), 0, s, E, posa, inva, charge, vec,
-                       (const double4*)nullptr, scalars, E.rsw, out4, pq, (double*)nullptr, (double*)nullptr,
-                       (double*)nullptr);
+    (void)charge;
+    hipLaunchKernelGGL(k_ew_tmu_epilogue, dim3(E.nblk), dim3(256), 0, s, E, posa, inva, vec, scalars, E.rsw, out4, pq);
 }
 
 void launch_ew_energy(hipStream_t s, int npad, const double* eout, double background, double* energy) {
-    hipLaunchKernelGGL(k_ew_energy, dim3(1), dim3(256), 0, s, npad, eout, background, energy);
+    hipLaunchKernelGGL(k_ew_energy, dim3(1), dim3(1024), 0, s, npad, eout, background, energy);
 }
 
 void launch_ew_dump_pairs(hipStream_t s, const EwLaunch& E, const double4* posa, const int* perm, int* pairs,

@@ -895,7 +895,7 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
             // the reciprocal chain (spreading, FFTs) beside the stored-block T v pass, on two streams;
             // the epilogue, which gathers from its potential grid, waits for both, on the main
             // stream.  In mixed precision the chain, whose passes run in the pass's leftover
-            // resources, takes the main stream and the pass the other (4.15 vs 4.21 ms per
+            // resources, takes the main stream and the pass the other (4.00 vs 4.05 ms per
             // evaluation at 36,864 sites); in FP64 mode the pass (2.7x longer) ends last by far and
             // keeps the main stream.
             const bool mixed = c->precision == TDP_PRECISION_MIXED;
@@ -903,7 +903,7 @@ void matvec(tdp_context* c, const PairLaunch& L, const double4* v64, const doubl
             hipStream_t sr = mixed ? c->stream : c->stream2;   // the chain
             hipEventRecord(c->ev_fork, c->stream);
             hipStreamWaitEvent(c->stream2, c->ev_fork, 0);
-            launch_ew_recip(sr, E, c->posa, nullptr, v64, E.phi_v, scalars);
+            launch_ew_recip(sr, E, c->posa, nullptr, v64, E.phi_v, scalars, true);
             if (timed) hipEventRecord(e0, ss);
             launch_ew_spmv(ss, E, v64, mixed ? c->vec32 : nullptr, scalars);
             if (timed) hipEventRecord(e1, ss);

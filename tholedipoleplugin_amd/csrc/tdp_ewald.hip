@@ -1933,19 +1933,29 @@ void launch_ew_neighbors(hipStream_t s, const EwLaunch& E) {
 
 namespace {
 template <typename T>
-void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const double* scalars) {
+void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const double* scalars, bool small) {
     using C = typename Cplx<T>::type;
     const int axis = (kind == 0 || kind == 4) ? 2 : ((kind == 1 || kind == 3) ? 1 : 0);
     const int n = E.K[axis];
     const int K2h = E.K[2] / 2 + 1;   // the complex grid's z extent (k_ew_fft)
     const int nlines = axis == 2 ? E.K[0] * E.K[1] : (axis == 1 ? E.K[0] * K2h : E.K[1] * K2h);
-    // Two lines per one-wave workgroup (4 KB of LDS for 64-point FP64 lines): the passes run beside
-    // the real-space passes, whose workgroups leave a few KB of LDS and a wave slot per CU free, and
-    // small workgroups start there as soon as they are launched (512 workgroups of eight lines, 16 KB,
-    // waited for real-space workgroups to retire: 50-70 us for a pass that takes 8 us alone)
-    int LB = std::max(1, std::min(2, (int)(4096 / sizeof(C)) / n));
+    // small: two lines per one-wave workgroup (4 KB of LDS for 64-point FP64 lines).  The T v pass's
+    // workgroups leave a few KB of LDS and a wave slot per CU free, and small workgroups start there
+    // as soon as they are launched (512 workgroups of eight lines, 16 KB, waited for the pass's
+    // workgroups to retire: 50-70 us for a pass that takes 8 us alone).  Beside the force pass,
+    // which is bound by its VALU work, the chain does better waiting (it still ends first).
+    int LB, threads;
+    if (small) {
+        LB = std::max(1, std::min(2, (int)(4096 / sizeof(C)) / n));
+        threads = 64;
+    } else {   // at most 24 KB per LDS buffer, and enough workgroups (512) to spread over the CUs
+        LB = (int)(24576 / sizeof(C)) / n;
+        LB = std::min(LB, std::max(1, nlines / 512));
+        LB = std::max(1, std::min(LB, 32));
+        threads = 256;
+    }
     const size_t shm = 2 * (size_t)LB * n * sizeof(C);
-    const dim3 grid((nlines + LB - 1) / LB), block(64);
+    const dim3 grid((nlines + LB - 1) / LB), block(threads);
     switch (kind) {
         case 0: hipLaunchKernelGGL((k_ew_fft<T, 0>), grid, block, shm, s, E, LB, phi, scalars); break;
         case 1: hipLaunchKernelGGL((k_ew_fft<T, 1>), grid, block, shm, s, E, LB, phi, scalars); break;
@@ -1957,11 +1967,11 @@ void fft_pass(hipStream_t s, const EwLaunch& E, int kind, void* phi, const doubl
 }  // namespace
 
 void launch_ew_recip(hipStream_t s, const EwLaunch& E, const double4* posa, const double* charge, const double4* vec,
-                     void* phi, const double* scalars) {
+                     void* phi, const double* scalars, bool small_fft) {
     hipLaunchKernelGGL(k_ew_spread, dim3(E.nblk), dim3(64), 0, s, E, posa, charge, vec, scalars);
     for (int kind = 0; kind < 5; kind++) {
-        if (E.mixed) fft_pass<float>(s, E, kind, phi, scalars);
-        else fft_pass<double>(s, E, kind, phi, scalars);
+        if (E.mixed) fft_pass<float>(s, E, kind, phi, scalars, small_fft);
+        else fft_pass<double>(s, E, kind, phi, scalars, small_fft);
     }
 }
 

@@ -254,9 +254,11 @@ struct EwLaunch {
 void launch_ew_neighbors(hipStream_t s, const EwLaunch& E);
 // Reciprocal-space potential grid of the charges (vec == null) or of the dipoles v (charge == null)
 // into phi (E.phi_q or E.phi_v): spreading, forward FFT, influence function, inverse FFT.
-// Skipped when scalars says the solve has stopped.
+// Skipped when scalars says the solve has stopped.  small_fft: the FFT passes as one-wave
+// workgroups of two lines (4 KB of LDS), which start beside a running real-space pass; else
+// workgroups of eight lines, which wait for its workgroups to retire.
 void launch_ew_recip(hipStream_t s, const EwLaunch& E, const double4* posa, const double* charge,
-                     const double4* vec, void* phi, const double* scalars);
+                     const double4* vec, void* phi, const double* scalars, bool small_fft = false);
 // Real-space pass + reciprocal gather + self and scaled-pair terms for the 64 slots of every tile.
 // kEwE0: out4 = E0, epart = 1/2 q phi_q - a/sqrt(pi) q^2.  kEwTmu: out4 = T v, or with pq the PCG
 // q and p.q tile sums.  kEwForce (v = mu): frc (3 doubles per slot, overwritten) and eout (energy

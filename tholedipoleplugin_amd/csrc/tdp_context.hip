@@ -226,6 +226,10 @@ struct tdp_context {
     unsigned short* d_idx16 = nullptr;
     int *d_idx64 = nullptr, *d_w32 = nullptr, *d_w64 = nullptr, *d_need = nullptr;
     int ew_lds_tiles = 0;   // neighbour tiles the stored T v pass stages in LDS (0: gathers from HBM)
+    int ew_lds_slots = 0;   // mixed: slot-table entries the T v pass stages in LDS
+    int* d_slot_tab = nullptr;   // mixed: per FP32 list, the slots its entries name
+    int* d_slot_cnt = nullptr;
+    int ew_tcap = 0;
     float2* d_t32 = nullptr;
     double4* d_t64 = nullptr;
     unsigned *d_cand32 = nullptr, *d_cand64 = nullptr;   // the E0 pass's pair candidates
@@ -661,7 +665,7 @@ void ew_particle_terms(tdp_context* c) {
 
 // (Re)allocates the stored T lists for the given per-wave capacities.  Returns TDP_OK with
 // ew_stored false when the device cannot hold them (the T v passes then stay matrix-free).
-int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
+int ew_alloc_lists(tdp_context* c, int cap32, int cap64, int tcap) {
     auto release = [c](void* p) {
         if (!p) return;
         auto it = std::find(c->allocations.begin(), c->allocations.end(), p);
@@ -670,7 +674,8 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
     };
     release(c->d_idx16); release(c->d_t32);
     release(c->d_idx64); release(c->d_t64);
-    release(c->d_cand32); release(c->d_cand64); release(c->d_ccnt);
+    release(c->d_cand32); release(c->d_cand64); release(c->d_ccnt); release(c->d_slot_tab);
+    c->d_slot_tab = nullptr;
     c->d_idx16 = nullptr;
     c->d_idx64 = nullptr;
     c->d_t32 = nullptr;
@@ -684,16 +689,21 @@ int ew_alloc_lists(tdp_context* c, int cap32, int cap64) {
     const bool mixed = c->precision == TDP_PRECISION_MIXED;
     const size_t rows = (size_t)c->nblk * kEwWaves * 64;   // 64 slots per tile
     const size_t n32 = mixed ? rows * cap32 : 1, n64 = rows * cap64;
-    void* p[7] = {};
-    const size_t bytes[7] = {n32 * sizeof(unsigned short), n32 * sizeof(float2), n64 * sizeof(int),
-                             n64 * sizeof(double4), n32 * sizeof(unsigned), n64 * sizeof(unsigned), rows * sizeof(int2)};
-    for (int k = 0; k < 7; k++)
+    tcap = mixed ? (tcap + 63) / 64 * 64 : 0;
+    const size_t ntab = mixed ? (size_t)c->nblk * kEwWaves * tcap : 1;
+    void* p[8] = {};
+    const size_t bytes[8] = {n32 * sizeof(unsigned short), n32 * sizeof(float2), n64 * sizeof(int),
+                             n64 * sizeof(double4), n32 * sizeof(unsigned), n64 * sizeof(unsigned), rows * sizeof(int2),
+                             ntab * sizeof(int)};
+    for (int k = 0; k < 8; k++)
         if (hipMalloc(&p[k], bytes[k]) != hipSuccess) {
             for (int q = 0; q < k; q++) hipFree(p[q]);
             (void)hipGetLastError();
             return TDP_OK;   // matrix-free fallback
         }
-    for (int k = 0; k < 7; k++) c->allocations.push_back(p[k]);
+    for (int k = 0; k < 8; k++) c->allocations.push_back(p[k]);
+    c->d_slot_tab = (int*)p[7];
+    c->ew_tcap = tcap;
     c->d_idx16 = (unsigned short*)p[0];
     c->d_t32 = (float2*)p[1];
     c->d_idx64 = (int*)p[2];
@@ -766,6 +776,10 @@ EwLaunch ew_launch(const tdp_context* c, const double* box9) {
     E.w32 = c->d_w32;
     E.w64 = c->d_w64;
     E.need = c->d_need;
+    E.slot_tab = c->d_slot_tab;
+    E.slot_cnt = c->d_slot_cnt;
+    E.tcap = c->ew_tcap;
+    E.lds_slots = c->ew_lds_slots;
     E.lds_tiles = c->ew_lds_tiles;
     E.tb_o = c->tb_o;
     E.posl = c->posl;
@@ -1044,7 +1058,7 @@ constexpr int kEwRedo = -1000;
 // when they did; else either the lists' indices cannot address the rows' neighbour tiles
 // (ew_stored is now false: matrix-free T v passes from here on) or c32, c64 are the capacities a
 // new build needs.
-bool ew_take_need(tdp_context* c, int& c32, int& c64) {
+bool ew_take_need(tdp_context* c, int& c32, int& c64, int& tcap) {
     const int* hn = static_cast<const int*>(c->h_need);
     c->ew_need_pending = false;
     for (int k = 0; k < 2; k++) {
@@ -1052,26 +1066,30 @@ bool ew_take_need(tdp_context* c, int& c32, int& c64) {
         c->ew_slots[k] = hn[4 + k];
     }
     const bool mixed_lists = c->precision == TDP_PRECISION_MIXED;
-    if (hn[6] > kEwMaxLocalTiles) {
+    if (hn[6] > kEwMaxLocalTiles || (mixed_lists && hn[7] > kEwMaxSlots)) {
         // the list indices (FP32 lists, and the FP64 lists of FP64 mode) cannot address this many
-        // neighbour tiles
+        // neighbour tiles or slots
         c->ew_stored = false;
         c->graph_gen++;
         return false;
     }
-    {   // LDS copy of the slot data for the FP32 lists (FP64 mode: v of the FP64 lists, also 24 B per
-        // slot): the wave's slots of up to lds_tiles tiles, 64 KB at most.  A pass launched with
-        // fewer stages a row with more neighbour tiles from HBM instead.
+    {   // LDS copies of the T v pass (24 B per slot, 64 KB at most): the FP32 lists' slot tables
+        // (mixed), the wave's slots of up to lds_tiles tiles (FP64 mode; and the force pass's tile
+        // shifts).  A pass launched with fewer stages a longer row from HBM instead.
         int lt = (hn[6] + 7) / 8 * 8;
         if ((size_t)lt * 8 * kEwSlotLds > 65536) lt = 0;
-        if (lt != c->ew_lds_tiles) {
+        int ls = mixed_lists ? (hn[7] + 7) / 8 * 8 : 0;
+        if ((size_t)ls * kEwSlotLds > 65536) ls = 0;
+        if (lt != c->ew_lds_tiles || ls != c->ew_lds_slots) {
             c->ew_lds_tiles = lt;
+            c->ew_lds_slots = ls;
             c->graph_gen++;
         }
     }
-    if ((!mixed_lists || hn[0] <= c->ew_cap32) && hn[1] <= c->ew_cap64) return true;
+    if ((!mixed_lists || (hn[0] <= c->ew_cap32 && hn[7] <= c->ew_tcap)) && hn[1] <= c->ew_cap64) return true;
     c32 = mixed_lists ? std::max(c->ew_cap32, hn[0] + hn[0] / 4 + 8) : 0;
     c64 = std::max(c->ew_cap64, hn[1] + hn[1] / 4 + 8);
+    tcap = mixed_lists ? std::max(c->ew_tcap, hn[7] + hn[7] / 4 + 64) : 0;
     return false;
 }
 
@@ -1143,7 +1161,7 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
         TDP_HIP(c, hipEventRecord(c->ev_join, c->stream2));
         for (int attempt = 0;; attempt++) {
             const EwLaunch Eb = ew_launch(c, box);
-            if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 7 * sizeof(int), s));
+            if (c->ew_stored) TDP_HIP(c, hipMemsetAsync(c->d_need, 0, 8 * sizeof(int), s));
             if (c->ew_stored) {
                 launch_ew_e0_stored(s, Eb, c->posa, c->d_inva, c->d_charge, c->Emv, c->d_epart, c->ev_join);
             } else {
@@ -1153,17 +1171,17 @@ int prepare_field(tdp_context* c, const double* box, const double* host_pos3) {
             }
             if (!c->ew_stored) break;
             // the lists must have held every pair: else grow them and build again
-            TDP_HIP(c, hipMemcpyAsync(c->h_need, c->d_need, 7 * sizeof(int), hipMemcpyDeviceToHost, s));
+            TDP_HIP(c, hipMemcpyAsync(c->h_need, c->d_need, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
             if (c->ew_defer) {   // examined at the solver's first host check (ew_take_need)
                 c->ew_need_pending = true;
                 break;
             }
             TDP_HIP(c, hipStreamSynchronize(s));
-            int c32 = 0, c64 = 0;
-            if (ew_take_need(c, c32, c64)) break;
+            int c32 = 0, c64 = 0, tc = 0;
+            if (ew_take_need(c, c32, c64, tc)) break;
             if (!c->ew_stored) break;   // matrix-free T v passes from now on (this pass's E0 is complete)
             if (attempt > 0) return fail(c, TDP_ERR_HIP, "Ewald option: the stored T lists could not be sized");
-            if ((rc = ew_alloc_lists(c, c32, c64))) return rc;
+            if ((rc = ew_alloc_lists(c, c32, c64, tc))) return rc;
         }
         launch_field_init(s, G, c->Emv, c->d_alpha, c->E0, c->mu, mixed ? c->vec32 : nullptr);
         return TDP_OK;
@@ -1300,9 +1318,9 @@ int solve_mutual(tdp_context* c, const double* box, const std::function<int()>* 
         if ((rc = stage_solver_state(c, done, pass == 0 ? ahead : nullptr))) return rc;
         if (pass == 0 && ahead_stands) *ahead_stands = ahead && done >= c->nseg;
         if (pass == 0 && c->ew_need_pending) {
-            int c32 = 0, c64 = 0;
-            if (!ew_take_need(c, c32, c64)) {
-                if (c->ew_stored && (rc = ew_alloc_lists(c, c32, c64))) return rc;
+            int c32 = 0, c64 = 0, tc = 0;
+            if (!ew_take_need(c, c32, c64, tc)) {
+                if (c->ew_stored && (rc = ew_alloc_lists(c, c32, c64, tc))) return rc;
                 if (ahead_stands) *ahead_stands = false;
                 return kEwRedo;
             }
@@ -1784,9 +1802,9 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             (rc = dalloc(c, &c->d_eout, npad)) || (rc = dalloc(c, &c->d_rs, (size_t)npad * std::max(kEwParts, kEwE0Parts))) ||
             (rc = dalloc(c, &c->d_rsw, (size_t)npad * kEwWaves)) ||
             (rc = dalloc(c, &c->d_w32, (size_t)nb * kEwWaves)) || (rc = dalloc(c, &c->d_w64, (size_t)nb * kEwWaves)) ||
-            (rc = dalloc(c, &c->d_need, 7)))
+            (rc = dalloc(c, &c->d_need, 8)) || (rc = dalloc(c, &c->d_slot_cnt, (size_t)nb * kEwWaves)))
             return bail(rc);
-        if (hipHostMalloc(&c->h_need, 7 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc(&c->h_need, 8 * sizeof(int), hipHostMallocDefault) != hipSuccess) {
             c->err = "hipHostMalloc failed";
             return bail(TDP_ERR_HIP);
         }
@@ -1800,7 +1818,12 @@ int create_context(const tdp_force_desc* d, const std::vector<int>& sizes, const
             const int c32 = mixed_lists ? (int)std::ceil(1.5 * est / kEwWaves) + 32 : 0;
             const int c64 = mixed_lists ? (int)std::ceil(1.5 * est64 / kEwWaves) + 16
                                         : (int)std::ceil(1.5 * est / kEwWaves) + 32;
-            if ((rc = ew_alloc_lists(c, c32, c64))) return bail(rc);
+            // slot tables: the wave's share of the slots within the cutoff of a tile (a ball of radius
+            // r_c plus twice the radius of a tile's volume), 1.5x
+            const double rt = rho > 0.0 ? std::cbrt(3.0 * 64 / (4.0 * M_PI * rho)) : 0.0;
+            const double rr = c->cutoff + 2.0 * rt;
+            const int tc = mixed_lists ? (int)std::min(8.0 * c->nblk, std::ceil(1.5 * rho * ball * rr * rr * rr / kEwWaves) + 64) : 0;
+            if ((rc = ew_alloc_lists(c, c32, c64, tc))) return bail(rc);
         }
         for (int k = 0; k < 3; k++) {
             const int K = c->ew_K[k];

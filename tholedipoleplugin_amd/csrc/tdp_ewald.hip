@@ -817,6 +817,7 @@ __global__ __launch_bounds__(kEwWaves * 64) void k_ew_pairs(EwLaunch E, const do
 // blocks where candidate n is not stored or lies past the lane's end).  The candidates and their
 // counts stay for the force pass (k_ew_force_cand).
 constexpr unsigned kCandSel = 1u << 16, kCandStore = 1u << 17;
+constexpr int kCandTabShift = 18;   // FP32 candidates: bits 18-31 their slot's place in the list's slot table
 
 // x, y, z wrapped into [0, L) in FP64, then rounded to FP32.
 __device__ __forceinline__ float4 ew_wrap32(const double4& p, const double* L, double iL0, double iL1, double iL2) {
@@ -927,6 +928,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     const double4 oI = E.tb_o[I];
     const double rc = sqrt(E.rc2);
     int nf = 0, nn = 0;
+    int ntab = 0;   // mixed: entries of the list's slot table so far
     const float4 ui = E.posl[i];   // offset from tile I's origin (FP32)
     // ---- phase A
     // Group data of lane 8g + m: slot 8w + m of neighbour tile t0 + g (read unconditionally at a
@@ -1058,20 +1060,39 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
         // image (mixed precision), FP64 blocks for the others
         const unsigned long long cand = inm & ~(mask & maskT);
         const unsigned long long c32 = MIXED ? cand & farm & frame_ok : 0ull;
-        auto emit = [&](unsigned long long m, unsigned* cb, int& n, int cap) {
+        // mixed: the group's slots that some lane's FP32 blocks name (stored candidates) join the
+        // list's slot table in ascending order; lane 8 g + m enters its own slot
+        unsigned long long used = 0ull;
+        if (MIXED) {
+            unsigned lo = (unsigned)(c32 & ~maskT), hi = (unsigned)((c32 & ~maskT) >> 32);
+            for (int off = 32; off > 0; off >>= 1) {
+                lo |= __shfl_xor(lo, off, 64);
+                hi |= __shfl_xor(hi, off, 64);
+            }
+            used = ((unsigned long long)hi << 32) | lo;
+            if ((used >> lane) & 1ull) {
+                const int k = ntab + __popcll(used & ((1ull << lane) - 1ull));
+                if (k < E.tcap) E.slot_tab[(size_t)list * E.tcap + k] = Jl * kTile + ks;
+            }
+        }
+        auto emit = [&](unsigned long long m, unsigned* cb, int& n, int cap, bool tab) {
             while (__builtin_amdgcn_ballot_w64(m != 0ull)) {
                 if (m) {
                     const int q = __builtin_ctzll(m);
                     m &= m - 1ull;
-                    const unsigned c = (unsigned)(8 * t0 + q) | (((mask >> q) & 1ull) ? 0u : kCandSel) |
-                                       (((maskT >> q) & 1ull) ? 0u : kCandStore);
+                    const bool st = !((maskT >> q) & 1ull);
+                    unsigned c = (unsigned)(8 * t0 + q) | (((mask >> q) & 1ull) ? 0u : kCandSel) | (st ? kCandStore : 0u);
+                    if (tab && st)   // (a table that overflows keeps every index inside it; the host regrows it)
+                        c |= (unsigned)min(ntab + __popcll(used & ((1ull << q) - 1ull)), min(E.tcap - 1, kEwMaxSlots))
+                             << kCandTabShift;
                     if (n < cap) cb[(n >> 3) * 512 + (n & 7)] = c;
                     n++;
                 }
             }
         };
-        if (MIXED) emit(c32, cf, nf, E.cap32);
-        emit(cand & ~c32, cn, nn, E.cap64);
+        if (MIXED) emit(c32, cf, nf, E.cap32, true);
+        emit(cand & ~c32, cn, nn, E.cap64, false);
+        ntab += __popcll(used);
     }
     E.ccnt[(size_t)list * kTile + lane] = make_int2(min(nf, E.cap32), min(nn, E.cap64));
     // a lane without candidates gets the entry 0 = slot 8w of the first neighbour tile, no flags: every
@@ -1136,7 +1157,7 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
                 // past the lane's list: nothing stored
             } else if (FP32) {   // entries in pairs per lane; the second of an odd list's last pair is zero
                 const size_t at = (lbase + rs + rank) * 2 + (n & 1);
-                E.idx16[at] = st ? (unsigned short)(c & 0xffffu) : (unsigned short)0;
+                E.idx16[at] = st ? (unsigned short)(c >> kCandTabShift) : (unsigned short)0;
                 E.t32[at] = st ? make_float2((float)C1, (float)C2) : make_float2(0.f, 0.f);
                 s32 += st ? 1 : 0;
             } else {
@@ -1175,6 +1196,10 @@ __global__ __launch_bounds__(kEwE0Waves * 64) void k_ew_e0_build(EwLaunch E, con
     if (lane == 0) {
         E.w32[list] = wd32;
         E.w64[list] = wd64;
+        if (MIXED) {
+            E.slot_cnt[list] = min(ntab, E.tcap);
+            atomicMax(&E.need[7], ntab);
+        }
         atomicMax(&E.need[0], MIXED ? mf : 0);
         atomicMax(&E.need[1], mn);
         atomicAdd(&E.need[2], s32w);
@@ -1400,7 +1425,7 @@ __device__ __forceinline__ void ew_spmv_load(const EwLaunch& E, int lane, EwSpmv
 template <bool LDS>
 __device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int w, int lane, int n32, int minrows, EwSpmvPipe& P,
                                              const float4* __restrict__ s_a, const float2* __restrict__ s_b,
-                                             const float4* __restrict__ v32, const int* __restrict__ nb,
+                                             const float4* __restrict__ v32, const int* __restrict__ tab,
                                              const double4& oI, const float4& ui, float& fx, float& fy,
                                              float& fz) {
     const int ns = ((n32 + 1) / 2 + kEwSpmvU - 1) / kEwSpmvU;
@@ -1411,8 +1436,8 @@ __device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int w, int lane,
             a = s_a[loc];
             b = s_b[loc];
         } else {
-            const int J = nb[loc >> 3];
-            ew_slot32(E, oI, J, J * kTile + ew_wave_slot(w, loc & 7), v32, a, b);
+            const int j = tab[loc];
+            ew_slot32(E, oI, j / kTile, j, v32, a, b);
         }
         const float dx = ui.x - a.w, dy = ui.y - b.x, dz = ui.z - b.y;
         const float p = c2 * fmaf(dz, a.z, fmaf(dy, a.y, dx * a.x));
@@ -1456,9 +1481,9 @@ __device__ __forceinline__ void ew_spmv_fp32(const EwLaunch& E, int w, int lane,
 
 // T v of the real-space pairs from the stored blocks: one wave per list (tile row I, wave w), so that
 // the nblk * kEwWaves waves spread evenly over the CUs.  The FP32 lists read their slot data from an
-// LDS copy of the list's slots of the row's neighbour tiles (E.lds_tiles of them, kEwSlotLds bytes
-// each: v_j and y_j, staged once, then every gather is an LDS read); their first entries are
-// requested before the copy is staged.  The FP64 lists gather v_j from HBM (L2), four entries at a
+// LDS copy of the slots their entries name (the list's slot table, up to E.lds_slots entries of
+// kEwSlotLds bytes: v_j and y_j, staged once, then every gather is an LDS read); their first entries
+// are requested before the copy is staged.  The FP64 lists gather v_j from HBM (L2), four entries at a
 // time (their index and block loads, then the four gathers).  Each wave writes its partial sums; the
 // epilogue adds the kEwWaves partials of a slot in list order.  10 bytes per FP32 entry and 36 per
 // FP64 entry stream from HBM once per pass.
@@ -1466,16 +1491,21 @@ template <bool MIXED>
 __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __restrict__ vec,
                                                 const float4* __restrict__ v32, const double* __restrict__ scalars,
                                                 double4* __restrict__ part) {
-    extern __shared__ float4 s_a[];   // [E.lds_tiles * 8] (v, y_x), then float2 [E.lds_tiles * 8] (y_y, y_z)
+    // mixed: [E.lds_slots] (v, y_x), then float2 [E.lds_slots] (y_y, y_z); FP64 mode: see below
+    extern __shared__ float4 s_a[];
     if (scalars && scalars[SC_DONE] != 0.0) return;
     const int row = blockIdx.x, lane = threadIdx.x;   // row = I kEwWaves + w
     const int I = row / kEwWaves, w = row % kEwWaves;
     const int* nb = E.nbr + (size_t)I * E.nblk;
     const int n32 = MIXED ? E.w32[row] : 0;
-    // a row with more neighbour tiles than the launch's LDS holds (sized from the previous list build)
-    // gathers its slot data from HBM
-    const bool lds = MIXED && n32 > 0 && E.lds_tiles > 0 && E.nbr_cnt[I] <= E.lds_tiles;
-    float2* s_b = reinterpret_cast<float2*>(s_a + (size_t)E.lds_tiles * 8);
+    const int ntab = MIXED ? E.slot_cnt[row] : 0;
+    const int* tab = MIXED ? E.slot_tab + (size_t)row * E.tcap : nullptr;
+    // a list whose slot table is longer than the launch's LDS holds (sized from the previous list
+    // build) gathers its slot data from HBM
+    // (a list whose FP32 blocks are all zero, of pairs without a T block, names no slot: skipped)
+    const bool has32 = MIXED && n32 > 0 && ntab > 0;
+    const bool lds = has32 && E.lds_slots > 0 && ntab <= E.lds_slots;
+    float2* s_b = reinterpret_cast<float2*>(s_a + (size_t)E.lds_slots);
     const int2 rows = E.ccnt[(size_t)row * kTile + lane];   // the lane's FP32 entries and FP64 rows
     EwSpmvPipe P;
     P.base = ew_base32(row, E.cap32);
@@ -1484,7 +1514,7 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
     int minrows = P.rows;
     double4 oI = make_double4(0.0, 0.0, 0.0, 0.0);
     float4 ui = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (MIXED && n32 > 0) {
+    if (has32) {
         ew_spmv_load(E, lane, P, 0, 0);
         ew_spmv_load(E, lane, P, 1, 1);
         oI = E.tb_o[I];
@@ -1492,30 +1522,26 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
         for (int off = 32; off > 0; off >>= 1) minrows = min(minrows, __shfl_xor(minrows, off, 64));
     }
     if (lds) {
-        // entry 8 t + m: slot 8w + m of neighbour tile t (the wave's own LDS: its operations
-        // complete in order); four slots per lane at a time, their loads issued together, and the
-        // neighbour tiles of the next four requested before them
-        const int ne = 8 * E.nbr_cnt[I];
-        int Jn[4];
+        // entry k: slot tab[k] (the wave's own LDS: its operations complete in order); four slots per
+        // lane at a time, their loads issued together, and the table entries of the next four
+        // requested before them
+        int jn[4];
 #pragma unroll
-        for (int u = 0; u < 4; u++) Jn[u] = nb[min(lane + u * kTile, ne - 1) >> 3];
-        for (int e0 = lane; e0 < ne; e0 += 4 * kTile) {
-            int J[4];
+        for (int u = 0; u < 4; u++) jn[u] = tab[min(lane + u * kTile, ntab - 1)];
+        for (int e0 = lane; e0 < ntab; e0 += 4 * kTile) {
+            int j[4];
 #pragma unroll
             for (int u = 0; u < 4; u++) {
-                J[u] = Jn[u];
-                Jn[u] = nb[min(e0 + (4 + u) * kTile, ne - 1) >> 3];
+                j[u] = jn[u];
+                jn[u] = tab[min(e0 + (4 + u) * kTile, ntab - 1)];
             }
             float4 a[4];
             float2 b[4];
 #pragma unroll
-            for (int u = 0; u < 4; u++) {
-                const int e = min(e0 + u * kTile, ne - 1);
-                ew_slot32(E, oI, J[u], J[u] * kTile + ew_wave_slot(w, e & 7), v32, a[u], b[u]);
-            }
+            for (int u = 0; u < 4; u++) ew_slot32(E, oI, j[u] / kTile, j[u], v32, a[u], b[u]);
 #pragma unroll
             for (int u = 0; u < 4; u++)
-                if (e0 + u * kTile < ne) {
+                if (e0 + u * kTile < ntab) {
                     s_a[e0 + u * kTile] = a[u];
                     s_b[e0 + u * kTile] = b[u];
                 }
@@ -1585,10 +1611,10 @@ __global__ __launch_bounds__(64) void k_ew_spmv(EwLaunch E, const double4* __res
             }
         }
     }
-    if (MIXED && n32 > 0) {
+    if (has32) {
         float fx = 0.f, fy = 0.f, fz = 0.f;
-        if (lds) ew_spmv_fp32<true>(E, w, lane, n32, minrows, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
-        else ew_spmv_fp32<false>(E, w, lane, n32, minrows, P, s_a, s_b, v32, nb, oI, ui, fx, fy, fz);
+        if (lds) ew_spmv_fp32<true>(E, w, lane, n32, minrows, P, s_a, s_b, v32, tab, oI, ui, fx, fy, fz);
+        else ew_spmv_fp32<false>(E, w, lane, n32, minrows, P, s_a, s_b, v32, tab, oI, ui, fx, fy, fz);
         ax += fx;
         ay += fy;
         az += fz;
@@ -2044,10 +2070,17 @@ void launch_ew_force_stored(hipStream_t s, const EwLaunch& E, bool mutual, const
     }
 }
 
+constexpr size_t kEwSpmvMinLds = 22 * 1024 + 512;   // 7 x 23,040 B <= 160 KB < 8 x 23,040 B
+
 void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const float4* v32, const double* scalars) {
     const dim3 lists(E.nblk * kEwWaves);
     if (E.pairs32)
-        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), (size_t)E.lds_tiles * 8 * kEwSlotLds, s, E, vec, v32,
+        // at least 22.5 KB of LDS per workgroup: at most 7 of the pass's one-wave workgroups per CU (160 KB),
+        // which leaves a SIMD with room for the reciprocal chain's FFT workgroups beside it (with 8, its
+        // register limit, the pass took 115 us instead of 121 but the chain waited and the evaluation
+        // took 4.02 instead of 3.93 ms at 36,864 sites)
+        hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), std::max((size_t)E.lds_slots * kEwSlotLds, kEwSpmvMinLds), s,
+                           E, vec, v32,
                            scalars, E.rsw);
     else
         hipLaunchKernelGGL((k_ew_spmv<false>), lists, dim3(64), (size_t)E.lds_tiles * 8 * kEwSlotLds, s, E, vec, v32,

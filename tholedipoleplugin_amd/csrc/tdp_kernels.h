@@ -167,6 +167,7 @@ constexpr int kEwE0Waves = 4;   // waves per workgroup of the list-building E0 p
 constexpr int kEwE0Parts = kEwWaves / kEwE0Waves;   // its workgroups per tile row
 constexpr int kEwSpmvU = 8;   // entry pairs per step of the stored T v pass (FP32 lists)
 constexpr int kEwMaxLocalTiles = 4096;   // neighbour tiles an FP32 list index can address
+constexpr int kEwMaxSlots = 16383;        // slots an FP32 list's table can hold (14-bit candidate field)
 constexpr int kEwSlotLds = 24;   // LDS bytes per staged slot of the FP32 T v pass: (v, y_x), (y_y, y_z)
 // FFT plan of one axis: n = prod r[0..nst), twiddles tw[q] = exp(-2 pi i q / n) (FP64 and FP32).
 struct EwAxis {
@@ -221,8 +222,8 @@ struct EwLaunch {
     // precision and, in FP64 mode, j's place among the wave's slots (as for the FP32 lists; the T v
     // pass stages those v_j in LDS).
     // FP32 lists (the other pairs in mixed precision) hold (C1, C2) only and index j by its place
-    // among the wave's slots of the row's neighbour tiles, 8 t + m for slot 8w + m of tile
-    // nbr[I][t] (uint16; rows with at most 4096 neighbour tiles): 2 + 8 bytes per entry.  The T v
+    // in the list's slot table (slot_tab: the slots the list's entries name, in ascending order of
+    // 8 t + m for slot 8w + m of tile nbr[I][t]; uint16): 2 + 8 bytes per entry.  The T v
     // pass forms d = u_i - y_j in FP32 from the slots' offsets: u_i from tile I's origin (posl),
     // y_j = u_j + D_IJ in FP64 rounded once, D_IJ the minimum image of the tile origins, which every
     // pair of the tile pair within the cutoff takes (ew_tile_image_ok).  FP32 entries are stored in
@@ -236,9 +237,17 @@ struct EwLaunch {
     double4* t64;
     int* w32;               // [nblk kEwWaves] list widths
     int* w64;
-    int* need;              // [7] largest width (FP32, FP64), entries (FP32, FP64), stored slots
-                            // (FP32, FP64), largest neighbour-tile count
-    int lds_tiles;          // neighbour tiles the T v pass stages in LDS (0: FP32 lists gather from HBM)
+    int* need;              // [8] largest width (FP32, FP64), entries (FP32, FP64), stored slots
+                            // (FP32, FP64), largest neighbour-tile count, most slots an FP32 list names
+    int lds_tiles;          // neighbour tiles the FP64-mode T v pass stages in LDS, and the force pass's
+                            // tile shifts (0: from HBM)
+    // Mixed precision: the slots an FP32 list's entries name (of the wave's slots of the row's
+    // neighbour tiles), in ascending order, as global slot indices; the entries index this table,
+    // which the T v pass copies into LDS (about half the wave's slots of the neighbour tiles)
+    int* slot_tab;          // [nblk kEwWaves tcap]
+    int* slot_cnt;          // [nblk kEwWaves] table length
+    int tcap;
+    int lds_slots;          // table entries the T v pass stages in LDS (0: FP32 lists gather from HBM)
     const double4* tb_o;    // [nblk] tile origins (FP64)
     const float4* posl;     // [npad] each slot's FP32 offset from its tile's origin
     float4* mu32;           // [npad] mixed: the force pass's FP32 records, dipole

@@ -44,9 +44,9 @@ DEBYE = 0.020819434  # e*nm
 # Counters cannot be read inside this process.
 MATVEC_HBM_BYTES = 265.6e6
 # HBM bytes per launch of k_ew_spmv<true> on this box (64^3 grid) from the same counters and scaling:
-# 541.5 MB read (the stored lists' jagged rows rounded to whole lines, 520.0 MB, and slot data of the
-# LDS copies that miss L2) + 9.4 MB written.
-SPMV_HBM_BYTES = 550.9e6
+# 546.7 MB read (the stored lists' jagged rows rounded to whole lines, 520.0 MB, the lists' slot tables
+# and slot data of the LDS copies that miss L2) + 9.4 MB written.
+SPMV_HBM_BYTES = 556.1e6
 
 
 def parse():

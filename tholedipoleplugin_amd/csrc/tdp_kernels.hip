@@ -1910,6 +1910,10 @@ __global__ __launch_bounds__(64) void k_pcg_update(const int* __restrict__ tseg,
                                                    double4* __restrict__ r, double4* __restrict__ z,
                                                    double* __restrict__ scalars, const double* __restrict__ red_pq,
                                                    double* __restrict__ red, int ntile) {
+    // the site's vectors are loaded first: their latency overlaps that of the scalars and the sums
+    const int i = blockIdx.x * kTile + threadIdx.x;
+    double4 pv = p[i], qv = q[i], xv = x[i], rv = r[i];
+    const double a = alpha[i];
     const int g = tseg[blockIdx.x];
     double* sc = scalars + SC_COUNT * g;
     if (solver_done(sc)) return;
@@ -1917,9 +1921,6 @@ __global__ __launch_bounds__(64) void k_pcg_update(const int* __restrict__ tseg,
     const double rz = sc[SC_RZ];
     const double gam = rz / segment_sum(sg, red_pq);
     if ((int)blockIdx.x == sg.x && threadIdx.x == 0) sc[SC_RZOLD] = rz;   // for k_pcg_direction
-    const int i = blockIdx.x * kTile + threadIdx.x;
-    double4 pv = p[i], qv = q[i], xv = x[i], rv = r[i];
-    double a = alpha[i];
     xv.x += gam * pv.x; xv.y += gam * pv.y; xv.z += gam * pv.z;
     rv.x -= gam * qv.x; rv.y -= gam * qv.y; rv.z -= gam * qv.z;
     double4 zv = make_double4(rv.x * a, rv.y * a, rv.z * a, 0.0);
@@ -1938,6 +1939,8 @@ __global__ __launch_bounds__(64) void k_pcg_direction(const int* __restrict__ ts
                                                       float4* __restrict__ p32, double* __restrict__ scalars,
                                                       const double* __restrict__ red, int ntile,
                                                       int* __restrict__ done) {
+    const int i = blockIdx.x * kTile + threadIdx.x;   // (loaded first, as in k_pcg_update)
+    const double4 zv = z[i], pv = p[i];
     const int g = tseg[blockIdx.x];
     double* sc = scalars + SC_COUNT * g;
     if (solver_done(sc)) return;
@@ -1945,8 +1948,6 @@ __global__ __launch_bounds__(64) void k_pcg_direction(const int* __restrict__ ts
     double rz, zz;
     segment_sums(sg, red, ntile, rz, zz);
     const double b = rz / sc[SC_RZOLD];
-    const int i = blockIdx.x * kTile + threadIdx.x;
-    double4 zv = z[i], pv = p[i];
     double4 np = make_double4(zv.x + b * pv.x, zv.y + b * pv.y, zv.z + b * pv.z, 0.0);
     p[i] = np;
     if (p32) p32[i] = make_float4((float)np.x, (float)np.y, (float)np.z, 0.f);

@@ -2078,7 +2078,7 @@ void launch_ew_spmv(hipStream_t s, const EwLaunch& E, const double4* vec, const 
         // at least 22.5 KB of LDS per workgroup: at most 7 of the pass's one-wave workgroups per CU (160 KB),
         // which leaves a SIMD with room for the reciprocal chain's FFT workgroups beside it (with 8, its
         // register limit, the pass took 115 us instead of 121 but the chain waited and the evaluation
-        // took 4.02 instead of 3.93 ms at 36,864 sites)
+        // took 4.02 instead of 3.93 ms at 36,864 sites; with 6, 26 KB each, 3.93 instead of 3.85 ms)
         hipLaunchKernelGGL((k_ew_spmv<true>), lists, dim3(64), std::max((size_t)E.lds_slots * kEwSlotLds, kEwSpmvMinLds), s,
                            E, vec, v32,
                            scalars, E.rsw);
